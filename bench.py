@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the waveform + likelihood path on one MI355X.
+
+Workload: the batched PT ensemble at Tobs = 16 s, fs = 4096 Hz, 3 detectors
+(H, L, Virgo-like with 2x PSD), band 20-2048 Hz (32448 bins), a low-mass
+injection (1.5 + 1.3 Msun at 40 Mpc) in seeded Gaussian noise, and 32
+temperatures x 64 chains = 2048 parameter points per step.  A step is one PT
+step resident on the GPU: propose 2048 points, set up IMRPhenomD for each,
+evaluate amplitude, phase, projection and the PSD-weighted inner products over
+all bins and detectors (one fused launch), Metropolis accept, temperature
+swaps.  The metric is logL evaluations per second over the timed steps.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line on stdout; diagnostics go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_VALU_PEAK_TF = 78.6   # MI355X FP64 vector
+FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def cpu_baseline(an, D, SN, P, seconds):
+    """The CPU oracle (the reference's per-call structure, OpenMP over points
+    as QuickCBC.c:2063) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    O.lib()
+    n = 16
+    t0 = time.perf_counter()
+    O.loglike_batch(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, P[:n])
+    dt = time.perf_counter() - t0
+    n2 = int(min(P.shape[0] * 8, max(n, seconds / max(dt / n, 1e-6))))
+    idx = np.arange(n2) % P.shape[0]
+    t0 = time.perf_counter()
+    O.loglike_batch(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, P[idx])
+    dt = time.perf_counter() - t0
+    return n2 / dt, n2, dt, O.lib().qo_num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--plain-points", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="autotune the launch shape first")
+    ap.add_argument("--only-main", action="store_true", help="time the FP64 PT steps only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how the "
+              "likelihood would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from quickcbc_amd import qcbc as Q
+    from quickcbc_amd import workloads as W
+
+    name, cus, mem = Q.device_info()
+    log("device:", name, cus, "CUs", "%.0f GB" % mem)
+
+    an = W.config1()
+    inj = W.injection1()
+    ctx = Q.Context(an)
+    h, st = ctx.templates(inj[None, :])
+    assert st[0] == 0
+    D = np.ascontiguousarray(h[0] + W.noise(an, 2024))
+    SN = an.SN()
+    ctx.bind(D, SN)
+    n_temps, n_chains = 32, 64
+    B = n_temps * n_chains
+    init = W.ensemble_around(inj, B, seed=99)
+    sigma = np.array([2e-5, 1e-4, 2e-3, 2e-3, 0.05, 2e-5, 0.01, 0.01, 0.01, 0.02, 0.01])
+    nb = an.imax - an.imin
+    bytes_per_logl = an.nifo * nb * 24          # (Re d/S, Im d/S, 1/S) per bin and detector, FP64
+    flops_per_logl = (60 + 19 * an.nifo) * nb    # waveform + projection + inner products
+
+    if args.autotune:
+        for prec in (Q.FP64, Q.MIXED):
+            log("autotune", prec, ctx.autotune(prec, B))
+
+    # ---- headline: FP64 fused path, PT steps, kernel timed with HIP events
+    pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64)
+    pt.run(args.warmup)
+    ctx.timing(1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    pt.run(args.steps)
+    ctx.sync()
+    T = time.perf_counter() - t0
+    kms, nl = ctx.timing(0)
+    value = B * args.steps / T
+    kernel_ms = kms / max(nl, 1)
+    ms_per_step = 1000.0 * T / args.steps
+    log("FP64 fused: %.1f logL/s, %.3f ms/step, kernel %.3f ms/launch (%d launches)" %
+        (value, ms_per_step, kernel_ms, nl))
+    achieved = bytes_per_logl * B / (kernel_ms * 1e-3) / 1e9
+    valu_tf = flops_per_logl * B / (kernel_ms * 1e-3) / 1e12
+    extra = {}
+    if not args.only_main:
+        # graph-replayed steps
+        pt.run(2, graph=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        pt.run(args.steps, graph=True)
+        Tg = time.perf_counter() - t0
+        extra["fp64_graph_logl_per_s"] = B * args.steps / Tg
+        # step latency
+        pt.step_timing(True)
+        pt.run(args.latency_steps)
+        lat = pt.latencies()
+        pt.step_timing(False)
+        extra["fp64_step_ms_p50"] = percentile(lat, 50)
+        extra["fp64_step_ms_p99"] = percentile(lat, 99)
+        x, Lx, acc, sw = pt.get()
+        extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + args.latency_steps))
+        pt.close()
+        # mixed precision on the same shape
+        ptm = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED)
+        ptm.run(args.warmup)
+        ctx.timing(1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ptm.run(args.steps)
+        Tm = time.perf_counter() - t0
+        mms, mnl = ctx.timing(0)
+        extra["mixed_logl_per_s"] = B * args.steps / Tm
+        extra["mixed_kernel_ms"] = mms / max(mnl, 1)
+        ptm.step_timing(True)
+        ptm.run(args.latency_steps)
+        lat = ptm.latencies()
+        extra["mixed_step_ms_p50"] = percentile(lat, 50)
+        extra["mixed_step_ms_p99"] = percentile(lat, 99)
+        ptm.close()
+        L64, _, _ = ctx.loglike(init, Q.FP64)
+        L32, _, _ = ctx.loglike(init, Q.MIXED)
+        err = np.abs(L32 - L64)
+        extra["mixed_abs_dlogl_vs_fp64"] = {"max": float(err.max()), "p99": percentile(err, 99),
+                                            "median": percentile(err, 50)}
+        # plain FP64 path: the reference's three loops, one point after another
+        npl = args.plain_points
+        ctx.loglike_plain(init[:8])
+        t0 = time.perf_counter()
+        Lp, _, _ = ctx.loglike_plain(init[:npl])
+        Tp = time.perf_counter() - t0
+        extra["plain_fp64_logl_per_s"] = npl / Tp
+        extra["plain_vs_fused_max_abs_dlogl"] = float(np.abs(Lp - L64[:npl]).max())
+        extra["mixed_over_plain"] = extra["mixed_logl_per_s"] / extra["plain_fp64_logl_per_s"]
+        extra["fp64_fused_over_plain"] = value / extra["plain_fp64_logl_per_s"]
+        extra["launch_shape_fp64"] = ctx.get_shape(Q.FP64)
+        extra["launch_shape_mixed"] = ctx.get_shape(Q.MIXED)
+        log(json.dumps(extra))
+
+    cpu = None
+    if not args.no_cpu and not args.only_main:
+        rate, n, dt, threads = cpu_baseline(an, D, SN, init, args.cpu_seconds)
+        cpu = {"value": rate, "unit": "logL evals/s", "cores": threads, "kind": "port",
+               "sample": "%d points of the same 2048-point ensemble (Tobs=16 s, 3 det, 32448 bins), "
+                         "%.1f s, gcc -O2 -fopenmp, OpenMP over points" % (n, dt)}
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": "logL evals/sec (waveform+logL) at Tobs=16s 3-det; p99 PT-step latency",
+        "value": value,
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Gaussian noise coloured by the analytic PSD plus an IMRPhenomD injection",
+        "config": {"workload": "batched PT ensemble, Tobs=16 s, fs=4096 Hz, 3 det (H/L/V), band 20-2048 Hz "
+                               "(32448 bins), 32 temps x 64 chains = 2048 points/step, FP64 fused launch",
+                   "points_per_step": B, "bins": nb, "detectors": an.nifo},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_ll_f64<3>", "kernel_ms": kernel_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_logl * B},
+        "valu": {"achieved_tflops": valu_tf, "peak_tflops": FP64_VALU_PEAK_TF, "frac": valu_tf / FP64_VALU_PEAK_TF,
+                 "flops_per_logl": flops_per_logl},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

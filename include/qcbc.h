@@ -1,0 +1,242 @@
+/*
+ * libqcbc: MI355X (gfx950) implementation of QuickCBC's waveform + likelihood
+ * path.  Plain C ABI: plain pointers and sizes, no C++ or device types.
+ *
+ * Part 1 re-exports QuickCBC's own entry points with identical names and
+ * signatures, so a QuickCBC build links against this library instead of the
+ * reference objects.  Part 2 is the batch API the drop-ins are built on: one
+ * analysis (detector set, data, PSD) is made resident on the GPU once and
+ * then evaluated for batches of parameter points.  Parts 3-5 add the PT
+ * ensemble step, the multi-analysis persistent batch and the serving loop.
+ *
+ * Every computation runs on the GPU; if no GPU (or no gfx950 code object) is
+ * usable every entry point reports QCBC_ENODEV (the drop-ins print the error
+ * and exit, the reference's error convention).
+ */
+#ifndef QCBC_H
+#define QCBC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ *
+ * ABI types, layout-identical to QuickCBC's                           *
+ * ------------------------------------------------------------------ */
+#ifndef QCBC_NO_REFERENCE_TYPES
+/* QuickCBC.h:13-25 */
+struct Net {
+  int Nifo;
+  double Tobs;
+  double tmax;
+  double tmin;
+  double GMST;
+  int *labels;
+  double *tds;
+  double **delays;
+  int *lstart;
+  int *lstop;
+};
+/* IMRPhenomD_internals.h:662-665 */
+typedef struct tagRealVector {
+  double *data;
+  size_t length;
+} RealVector;
+/* IMRPhenomD_internals.h:649-654 */
+typedef struct tagAmpPhaseFDWaveform {
+  double *freq;
+  double *amp;
+  double *phase;
+  size_t length;
+} AmpPhaseFDWaveform;
+#endif
+
+/* ------------------------------------------------------------------ *
+ * Part 1: drop-in replacements of QuickCBC entry points               *
+ * ------------------------------------------------------------------ */
+
+/* Replaces log_likelihood_full, QuickCBC.c:4236-4273 (prototype QuickCBC.h:110).
+ * logL = sum_d [(d|h) - (h|h)/2], rho[d] = (d|h)/sqrt((h|h)), band
+ * [(int)(Tobs*fmin), min((int)(Tobs*fmax), N/2)).  fmin/fmax default to the
+ * reference's compiled 16/1024 Hz (ConstCBC.h:33-34); see qcbc_set_band.
+ * D and SN are cached on the GPU keyed by their row pointers; call
+ * qcbc_invalidate() after changing their contents in place. */
+double log_likelihood_full(struct Net *net, double **D, double *params, RealVector *freq,
+                           double **SN, double *rho, int N, double Tobs);
+
+/* Replaces fulltemplates, QuickCBC.c:3683-3777 (prototype QuickCBC.h:97).
+ * hwave: net->Nifo rows of N doubles, half-complex. */
+void fulltemplates(struct Net *net, double **hwave, RealVector *freq, double *params, int N);
+
+/* Replaces IMRPhenomDGenerateh22FDAmpPhase, IMRPhenomD.c:186-248 (prototype
+ * IMRPhenomD.h:20-30).  *h22 must be NULL; it is allocated here and freed by
+ * the caller with DestroyAmpPhaseFDWaveform.  Returns 0 (PD_SUCCESS) or, in
+ * non-exiting mode, the reference's error code. */
+int IMRPhenomDGenerateh22FDAmpPhase(AmpPhaseFDWaveform **h22, RealVector *freq, double phi0,
+                                    double fRef_in, double m1_SI, double m2_SI, double chi1,
+                                    double chi2, double distance);
+
+/* Replaces fourier_nwip, Utilities.c:716-736 (prototypes Utilities.h:38, QuickCBC.h:84). */
+double fourier_nwip(double *a, double *b, double *Sn, int imin, int imax, int N);
+
+/* Replace F_ant (Utilities.c:995-1000, prototype Utilities.h:46), Times
+ * (Utilities.c:1078-1109, Utilities.h:50) and fbegin (Utilities.c:587-629,
+ * Utilities.h:34). */
+void F_ant(double psi, double alpha, double sindelta, double GMST, double *Fplus, double *Fcross,
+           int id);
+void Times(double alpha, double sindelta, double GMST, double *dtimes);
+double fbegin(double *param);
+
+/* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
+AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
+void DestroyAmpPhaseFDWaveform(AmpPhaseFDWaveform *wf);
+RealVector *CreateRealVector(size_t length);
+void DestroyRealVector(RealVector *v);
+
+/* Drop-in configuration (no counterpart in the reference, whose values are
+ * compile-time constants): band edges used by log_likelihood_full, the
+ * reference frequency (ConstCBC.h:31), and whether domain errors exit(-1) as
+ * the reference's ERROR() does (default 1) or return silently with logL = 0. */
+void qcbc_set_band(double fmin, double fmax);
+void qcbc_set_fref(double fref);
+void qcbc_set_exit_on_error(int on);
+/* Forget cached GPU copies of D/SN (their contents changed in place). */
+void qcbc_invalidate(void);
+
+/* ------------------------------------------------------------------ *
+ * Part 2: resident-analysis batch API                                 *
+ * ------------------------------------------------------------------ */
+enum {
+  QCBC_OK = 0,
+  QCBC_EINVAL = 1,
+  QCBC_ENOMEM = 12,   /* as PD_ENOMEM */
+  QCBC_EFAULT = 14,   /* as PD_EFAULT */
+  QCBC_EDOM = 33,     /* as PD_EDOM: a parameter point outside the model */
+  QCBC_EHIP = 100,    /* a HIP runtime call failed */
+  QCBC_ENODEV = 101,  /* no usable gfx950 device */
+  QCBC_ENOTBOUND = 102,
+  QCBC_EBUSY = 103,   /* admission refused: memory budget or slots */
+  QCBC_ECANCELED = 104
+};
+
+enum { QCBC_FP64 = 0, QCBC_MIXED = 1 };
+
+typedef struct {
+  int n_det;        /* 1..3 */
+  int labels[3];    /* 0 = H, 1 = L, 2 = V (F_ant / Times labels) */
+  double gmst;
+  int N;            /* samples per segment; N/2 frequency bins */
+  double Tobs;      /* segment length (s) */
+  double fmin, fmax;/* likelihood band (Hz) */
+  double fref;      /* IMRPhenomD reference frequency (Hz) */
+} qcbc_desc;
+
+typedef struct qcbc_ctx qcbc_ctx;
+
+int qcbc_ctx_create(const qcbc_desc *desc, qcbc_ctx **out);
+int qcbc_ctx_destroy(qcbc_ctx *ctx);
+/* Device bytes a context holds for this shape (data, PSD, tables), excluding
+ * per-batch scratch. */
+size_t qcbc_ctx_footprint(const qcbc_desc *desc);
+
+/* Bind data and PSD.  D: n_det rows of N doubles (half-complex, QuickCBC's
+ * D layout); SN: n_det rows of N/2 doubles (one-sided PSD).  Row-major flat
+ * arrays; the _rows variant takes QuickCBC's double** row pointers. */
+int qcbc_bind_data(qcbc_ctx *ctx, const double *D, const double *SN);
+int qcbc_bind_rows(qcbc_ctx *ctx, double *const *D, double *const *SN);
+
+/* logL and rho for B points (params: B x 11, host memory).  status[b] is 0
+ * or QCBC_EDOM for a point outside the model (its logL is then 0).
+ * precision: QCBC_FP64 (parity path) or QCBC_MIXED. */
+int qcbc_loglike(qcbc_ctx *ctx, const double *params, int B, int precision, double *logL,
+                 double *rho, int *status);
+/* Same with device pointers, asynchronous on the context's stream. */
+int qcbc_loglike_device(qcbc_ctx *ctx, const double *d_params, int B, int precision,
+                        double *d_logL, double *d_rho, int *d_status);
+/* The plain FP64 GPU path: per point, the reference's three loops as three
+ * launches (waveform -> amp/phase in HBM, projection -> templates in HBM,
+ * inner products), one point after another.  Host pointers. */
+int qcbc_loglike_plain(qcbc_ctx *ctx, const double *params, int B, double *logL, double *rho,
+                       int *status);
+
+/* IMRPhenomD h22 amplitude and phase on the context's N/2 grid (freq[0] =
+ * 1/Tobs, freq[i] = i/Tobs) for B points: amp, phase are B x N/2. */
+int qcbc_waveform(qcbc_ctx *ctx, const double *params, int B, double *amp, double *phase,
+                  int *status);
+/* Projected half-complex templates (fulltemplates) for B points: hwave is
+ * B x n_det x N. */
+int qcbc_templates(qcbc_ctx *ctx, const double *params, int B, double *hwave, int *status);
+
+/* Kernel launch shape of the fused likelihood for this context:
+ * chains per block, bins per LDS tile, bin splits (0 = choose). */
+int qcbc_ctx_set_shape(qcbc_ctx *ctx, int precision, int chains_per_block, int tile, int splits);
+int qcbc_ctx_get_shape(qcbc_ctx *ctx, int precision, int *chains_per_block, int *tile,
+                       int *splits);
+/* Time candidate shapes on synthetic points and keep the fastest. */
+int qcbc_autotune(qcbc_ctx *ctx, int precision, int B);
+/* Cumulative device time (ms) and launch count of the fused likelihood
+ * kernel, measured with HIP events on the context's stream. */
+int qcbc_ctx_timing(qcbc_ctx *ctx, int enable, double *kernel_ms, long long *launches);
+int qcbc_ctx_sync(qcbc_ctx *ctx);
+
+const char *qcbc_strerror(int code);
+int qcbc_device_info(char *name, size_t len, int *cu_count, double *mem_gb);
+
+/* ------------------------------------------------------------------ *
+ * Part 3: parallel-tempered ensemble step (resident state)            *
+ * ------------------------------------------------------------------ */
+typedef struct qcbc_pt qcbc_pt;
+/* n_temps x n_chains points; temperatures ladder^t (t = 0..n_temps-1).
+ * init: (n_temps*n_chains) x 11.  sigma: 11 per-parameter jump scales at
+ * temperature 1 (scaled by sqrt(T)). */
+int qcbc_pt_create(qcbc_ctx *ctx, int n_temps, int n_chains, double ladder, const double *init,
+                   const double *sigma, uint64_t seed, int precision, qcbc_pt **out);
+/* nsteps PT steps: propose, evaluate, Metropolis accept, adjacent-temperature
+ * swaps, all on the GPU.  use_graph: replay one captured step as a hipGraph. */
+int qcbc_pt_run(qcbc_pt *pt, int nsteps, int use_graph);
+int qcbc_pt_get(qcbc_pt *pt, double *params, double *logL, long long *accepted,
+                long long *swaps);
+/* Per-step wall latencies (ms) of the last qcbc_pt_run with timing on. */
+int qcbc_pt_latencies(qcbc_pt *pt, double *ms, int max, int *n);
+int qcbc_pt_set_step_timing(qcbc_pt *pt, int on);
+int qcbc_pt_destroy(qcbc_pt *pt);
+
+/* ------------------------------------------------------------------ *
+ * Part 4: one persistent launch over many analyses                    *
+ * ------------------------------------------------------------------ */
+/* Evaluate n batches (ctxs[i], d_params[i], B[i]) in one persistent kernel
+ * draining an (analysis, chain group, bin range) work queue.  Device
+ * pointers; synchronous. */
+int qcbc_multi_loglike(qcbc_ctx *const *ctxs, const double *const *d_params, const int *B,
+                       int n, int precision, double *const *d_logL, double *const *d_rho,
+                       int *const *d_status);
+
+/* ------------------------------------------------------------------ *
+ * Part 5: serving loop                                                *
+ * ------------------------------------------------------------------ */
+typedef struct qcbc_server qcbc_server;
+/* budget_bytes: device memory the resident analyses may hold. */
+int qcbc_server_create(size_t budget_bytes, int precision, qcbc_server **out);
+/* Admit an analysis (copies D/SN to the GPU).  QCBC_EBUSY if it does not fit
+ * the remaining budget. */
+int qcbc_server_admit(qcbc_server *s, const qcbc_desc *desc, const double *D, const double *SN,
+                      int *analysis_id);
+int qcbc_server_evict(qcbc_server *s, int analysis_id);
+/* Submit B points (host memory, copied) for an analysis; returns a request id. */
+int qcbc_server_submit(qcbc_server *s, int analysis_id, const double *params, int B,
+                       int64_t *request_id);
+/* Wait for a request; logL, rho (B x n_det) and status are written to host
+ * memory; latency_ms is submit-to-completion.  timeout_ms < 0 waits forever. */
+int qcbc_server_wait(qcbc_server *s, int64_t request_id, double *logL, double *rho, int *status,
+                     double *latency_ms, int timeout_ms);
+int qcbc_server_stats(qcbc_server *s, size_t *bytes_in_use, int *n_analyses, long long *evals,
+                      long long *batches);
+int qcbc_server_destroy(qcbc_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

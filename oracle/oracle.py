@@ -1,0 +1,193 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: imported by tests/, the smoke check and the bench's
+CPU-baseline leg only.  The product (quickcbc_amd) never imports this.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.qo_fbegin.restype = ctypes.c_double
+        L.qo_fbegin.argtypes = [_dp]
+        L.qo_times.argtypes = [ctypes.c_double] * 3 + [_dp]
+        L.qo_f_ant.argtypes = [ctypes.c_double] * 4 + [_dp, _dp, ctypes.c_int]
+        L.qo_fourier_nwip.restype = ctypes.c_double
+        L.qo_fourier_nwip.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qo_loglike_flat.restype = ctypes.c_double
+        L.qo_loglike_flat.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int,
+                                      ctypes.c_double, _dp, _dp, _ip]
+        L.qo_loglike_batch_flat.restype = None
+        L.qo_loglike_batch_flat.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int,
+                                            ctypes.c_double, _dp, ctypes.c_int, _dp, _dp, _ip,
+                                            ctypes.c_int]
+        L.qo_fulltemplates_flat.restype = ctypes.c_int
+        L.qo_fulltemplates_flat.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double,
+                                            _dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.qo_h22_flat.restype = ctypes.c_int
+        L.qo_h22_flat.argtypes = [_dp, ctypes.c_size_t] + [ctypes.c_double] * 7 + [_dp, _dp]
+        L.qo_setup_flat.restype = ctypes.c_int
+        L.qo_setup_flat.argtypes = [ctypes.c_double] * 7 + [_dp, ctypes.c_int]
+        L.qo_spline_eval_flat.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp]
+        L.qo_final_spin.restype = ctypes.c_double
+        L.qo_final_spin.argtypes = [ctypes.c_double] * 3
+        L.qo_erad.restype = ctypes.c_double
+        L.qo_erad.argtypes = [ctypes.c_double] * 3
+        L.qo_fit.restype = ctypes.c_double
+        L.qo_fit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.qo_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+# Field order of qo_phenomd (qcbc_oracle.h); all doubles.
+SETUP_FIELDS = (
+    "m1 m2 chi1 chi2 M eta Msec amp0 fCut finspin chi fRD fDM "
+    "gamma1 gamma2 gamma3 fmaxCalc rho1 rho2 rho3 f1 f2 f3 v1 v2 v3 d1 d2 "
+    "delta0 delta1 delta2 delta3 delta4 "
+    "ai_amp0 ai_23 ai_1 ai_43 ai_53 ai_2 ai_73 ai_83 ai_3 "
+    "sigma1 sigma2 sigma3 sigma4 beta1 beta2 beta3 alpha1 alpha2 alpha3 alpha4 alpha5 "
+    "C1Int C2Int C1MRD C2MRD fPhiIns fPhiMRD").split()
+SETUP_FIELDS += ["pn_v%d" % i for i in range(13)] + ["pn_vlogv%d" % i for i in range(13)]
+SETUP_FIELDS += ("pi_init pi_23 pi_13 pi_13L pi_L pi_m13 pi_m23 pi_m1 pi_m53 "
+                 "pi_1 pi_43 pi_53 pi_2 t0 MfRef phifRef phi_precalc").split()
+
+
+def setup(m1, m2, chi1, chi2, distance=1.0, phi0=0.0, fref=100.0):
+    out = np.zeros(len(SETUP_FIELDS))
+    st = lib().qo_setup_flat(m1, m2, chi1, chi2, distance, phi0, fref, _d(out), len(out))
+    return st, dict(zip(SETUP_FIELDS, out))
+
+
+def fbegin(params):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    return lib().qo_fbegin(_d(p))
+
+
+def times(alpha, sindelta, gmst):
+    out = np.zeros(3)
+    lib().qo_times(alpha, sindelta, gmst, _d(out))
+    return out
+
+
+def f_ant(psi, alpha, sindelta, gmst, det):
+    fp = ctypes.c_double(0.0)
+    fc = ctypes.c_double(0.0)
+    lib().qo_f_ant(psi, alpha, sindelta, gmst, ctypes.byref(fp), ctypes.byref(fc), det)
+    return fp.value, fc.value
+
+
+def fourier_nwip(a, b, Sn, imin, imax, N):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    Sn = np.ascontiguousarray(Sn, np.float64)
+    return lib().qo_fourier_nwip(_d(a), _d(b), _d(Sn), imin, imax, N)
+
+
+def h22(freq, phi0, fref, m1_SI, m2_SI, chi1, chi2, distance):
+    freq = np.ascontiguousarray(freq, np.float64)
+    amp = np.zeros_like(freq)
+    ph = np.zeros_like(freq)
+    st = lib().qo_h22_flat(_d(freq), freq.size, phi0, fref, m1_SI, m2_SI, chi1, chi2, distance,
+                           _d(amp), _d(ph))
+    return st, amp, ph
+
+
+def fulltemplates(labels, gmst, fref, N, Tobs, params):
+    labels = np.ascontiguousarray(labels, np.int32)
+    p = np.ascontiguousarray(params, np.float64)
+    h = np.zeros((labels.size, N))
+    st = lib().qo_fulltemplates_flat(labels.size, _i(labels), gmst, fref, _d(h), N, Tobs, _d(p))
+    return st, h
+
+
+def loglike(labels, gmst, fref, fmin, fmax, D, SN, N, Tobs, params):
+    labels = np.ascontiguousarray(labels, np.int32)
+    D = np.ascontiguousarray(D, np.float64)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    rho = np.zeros(labels.size)
+    st = ctypes.c_int(0)
+    v = lib().qo_loglike_flat(labels.size, _i(labels), gmst, fref, fmin, fmax, _d(D), _d(SN), N,
+                              Tobs, _d(p), _d(rho), ctypes.byref(st))
+    return v, rho, st.value
+
+
+def loglike_batch(labels, gmst, fref, fmin, fmax, D, SN, N, Tobs, params, nthreads=0):
+    labels = np.ascontiguousarray(labels, np.int32)
+    D = np.ascontiguousarray(D, np.float64)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+    B = p.shape[0]
+    logL = np.zeros(B)
+    rho = np.zeros((B, labels.size))
+    st = np.zeros(B, np.int32)
+    lib().qo_loglike_batch_flat(labels.size, _i(labels), gmst, fref, fmin, fmax, _d(D), _d(SN), N,
+                                Tobs, _d(p), B, _d(logL), _d(rho), _i(st), nthreads)
+    return logL, rho, st
+
+
+def spline_eval(which, x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    lib().qo_spline_eval_flat(which, _d(x), x.size, _d(y))
+    return y
+
+
+def final_spin(eta, chi1, chi2):
+    return lib().qo_final_spin(eta, chi1, chi2)
+
+
+def erad(eta, chi1, chi2):
+    return lib().qo_erad(eta, chi1, chi2)
+
+
+def fit(which, eta, chi):
+    return lib().qo_fit(which, eta, chi)
+
+
+def waveform_params(params, N, Tobs, fref=100.0):
+    L = lib()
+    L.qo_waveform_params.restype = ctypes.c_int
+    L.qo_waveform_params.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    p = np.ascontiguousarray(params, np.float64)
+    amp = np.zeros(N // 2)
+    ph = np.zeros(N // 2)
+    st = L.qo_waveform_params(_d(p), N, Tobs, fref, _d(amp), _d(ph))
+    return st, amp, ph
+
+
+def probe(which, m1, m2, chi1, chi2, Mf):
+    L = lib()
+    L.qo_probe.restype = ctypes.c_double
+    L.qo_probe.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5
+    return L.qo_probe(which, m1, m2, chi1, chi2, Mf)

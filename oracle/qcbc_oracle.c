@@ -1,0 +1,995 @@
+/*
+ * CPU oracle for the QuickCBC waveform + likelihood path (TEST
+ * INFRASTRUCTURE: only tests/, the smoke check and the bench's CPU-baseline
+ * leg load it; the product never does).
+ *
+ * A plain C99 FP64 restatement of the reference's algorithm.  It follows the
+ * reference's operation order wherever that order is visible (per-bin
+ * evaluators, projection, inner product), including the long-double PN
+ * coefficients (x87 80-bit on x86-64, as in the reference build).  Two pieces
+ * are computed by a different but equivalent method, because the reference's
+ * form of them is a Mathematica expansion rather than an algorithm:
+ *   - the intermediate-amplitude collocation coefficients delta0..4
+ *     (reference: closed forms IMRPhenomD_internals.c:848-999) are obtained
+ *     here by solving the same five collocation conditions (arXiv:1508.07253
+ *     Eqs. 22-26) with long-double Gaussian elimination;
+ *   - DAmpInsAnsatz (reference: explicit formula :671-702) is the term-wise
+ *     derivative of the inspiral ansatz written from its prefactors.
+ * The GSL natural cubic spline of the QNM table (:479-513) is replaced by a
+ * natural cubic spline solved here (tridiagonal Thomas algorithm) and
+ * evaluated in the same power form GSL's cspline uses.
+ *
+ * What pins each part (tests/test_oracle_*.py):
+ *   PN phasing         closed-form values at eta=1/4, chi=0; pfaN scaling
+ *   QNM spline         exact at knots; natural end conditions; scipy
+ *                      CubicSpline(bc_type='natural') at random spins
+ *   fit tables         parsed from the reference text (gen_tables.py), layout
+ *                      checked there; values re-derived at a few points
+ *   collocation        A_int(f1,f2,f3)=v1,v2,v3 and A_int'(f1,f3)=d1,d2
+ *   DAmpIns / DAmpMRD  central finite differences of AmpIns / AmpMRD
+ *   phase connection   C0 and C1 continuity at 0.018 and 0.5 fRD
+ *   reference phase    phase(fRef) = -2 phi0
+ *   fCut / fbegin mask zero outputs exactly where the reference zeroes
+ *   antenna patterns   least-squares fit of a detector tensor: residual at
+ *                      the 1e-9 level of the 10-digit coefficients,
+ *                      eigenvalues (+1/2, -1/2, 0); F+^2+Fx^2 invariant in psi
+ *   time delays        |dt_HL| <= 10.6 ms etc.; dt=0 for equal labels
+ *   fourier_nwip       brute force in numpy
+ *   logL               noise-free data: logL = (h|h)/2, rho = sqrt((h|h));
+ *                      invariance under phase+2pi and psi+pi
+ * There is no reference build to compare bit patterns with (GSL is absent),
+ * so bit-level parity with the reference binary is unpinned.
+ */
+#include "qcbc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define QCBC_TABLE_STORAGE static const
+#include "../quickcbc_amd/csrc/qcbc_tables.inc"
+
+/* IMRPhenomD_internals.h:102-129 */
+#define QO_PI 3.141592653589793238462643383279502884
+#define QO_PI_4 0.785398163397448309615660845819875721
+#define QO_GAMMA 0.577215664901532860606512090082402431
+#define QO_MSUN_SI 1.988546954961461467461011951140572744e30
+#define QO_MTSUN_SI 4.925491025543575903411922162094833998e-6
+#define QO_MRSUN_SI 1.476625061404649406193430731479084713e3
+#define QO_F_CUT 0.2
+#define QO_AMP_JOIN 0.014
+#define QO_PHI_JOIN 0.018
+/* ConstCBC.h:1,4,11,12 */
+#define QO_TSUN 4.92569043916e-6
+#define QO_TPI 6.2831853071795862319959269370884
+#define QO_CLIGHT 299792458.0
+#define QO_H22FAC 0.31539156525252
+
+enum { QO_OK = 0, QO_EDOM = 33, QO_EFAULT = 14, QO_ENOMEM = 12 };
+
+/* ---------------------------------------------------------------- powers */
+/* UsefulPowers and init_useful_powers, IMRPhenomD_internals.c:515-533 */
+typedef struct {
+  double sixth, third, two_thirds, four_thirds, five_thirds, two, seven_thirds, eight_thirds;
+} qo_pow;
+
+static void qo_powers(qo_pow *p, double x) {
+  if (x < 0.0) x = 1.0e-10;
+  p->sixth = pow(x, 1 / 6.0);
+  p->third = p->sixth * p->sixth;
+  p->two_thirds = x / p->third;
+  p->four_thirds = x * p->third;
+  p->five_thirds = p->four_thirds * p->third;
+  p->two = x * x;
+  p->seven_thirds = p->third * p->two;
+  p->eight_thirds = p->two_thirds * p->two;
+}
+
+static qo_pow qo_pi_powers(void) {
+  qo_pow p;
+  qo_powers(&p, QO_PI);
+  return p;
+}
+
+/* ------------------------------------------------------- fits and spins */
+/* Table-5 fit layout (IMRPhenomD_internals.c:552-592 and siblings). */
+double qo_fit(int which, double eta, double chi) {
+  const double *c = qcbc_fit[which];
+  double xi = -1 + chi;
+  double xi2 = xi * xi;
+  double xi3 = xi2 * xi;
+  double eta2 = eta * eta;
+  return c[0] + c[1] * eta + (c[2] + c[3] * eta + c[4] * eta2) * xi +
+         (c[5] + c[6] * eta + c[7] * eta2) * xi2 + (c[8] + c[9] * eta + c[10] * eta2) * xi3;
+}
+
+/* chiPN, IMRPhenomD_internals.c:373-379 */
+static double qo_chipn(double eta, double chi1, double chi2) {
+  double delta = sqrt(1.0 - 4.0 * eta);
+  double chi_s = (chi1 + chi2) / 2.0;
+  double chi_a = (chi1 - chi2) / 2.0;
+  return chi_s * (1.0 - eta * 76.0 / 113.0) + delta * chi_a;
+}
+
+/* FinalSpin0815, IMRPhenomD_internals.c:415-444 */
+double qo_final_spin(double eta, double chi1, double chi2) {
+  const double *c = qcbc_finalspin;
+  double Seta = sqrt(1.0 - 4.0 * eta);
+  double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+  double s = m1 * m1 * chi1 + m2 * m2 * chi2;
+  double eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+  double s2 = s * s, s3 = s2 * s, s4 = s3 * s;
+  return c[0] * eta + c[1] * eta2 + c[2] * eta3 + c[3] * eta4 +
+         (c[4] + c[5] * eta + c[6] * eta2) * s + (c[7] * eta + c[8] * eta2) * s2 +
+         (c[9] * eta + c[10] * eta2) * s3 + (c[11] * eta + c[12] * eta2) * s4;
+}
+
+/* EradRational0815, IMRPhenomD_internals.c:450-473 */
+double qo_erad(double eta, double chi1, double chi2) {
+  const double *c = qcbc_erad;
+  double Seta = sqrt(1.0 - 4.0 * eta);
+  double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+  double m1s = m1 * m1, m2s = m2 * m2;
+  double s = (m1s * chi1 + m2s * chi2) / (m1s + m2s);
+  double eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+  return ((c[0] * eta + c[1] * eta2 + c[2] * eta3 + c[3] * eta4) *
+          (c[4] + (c[5] + c[6] * eta + c[7] * eta2) * s)) /
+         (c[8] + (c[9] + c[10] * eta + c[11] * eta2) * s);
+}
+
+/* ------------------------------------------------------ QNM cubic spline */
+/* Natural cubic spline through the QNM table; stands in for GSL's
+ * gsl_interp_cspline used by fring/fdamp (IMRPhenomD_internals.c:484-511).
+ * c[i] holds half the second derivative at knot i, the quantity GSL's
+ * cspline state stores, and evaluation uses the same power form. */
+static double qo_sp_c[2][QCBC_QNM_N];
+static int qo_sp_ready = 0;
+
+static void qo_spline_solve(const double *x, const double *y, int n, double *c) {
+  double *diag = malloc(sizeof(double) * n), *rhs = malloc(sizeof(double) * n);
+  double *off = malloc(sizeof(double) * n);
+  int i;
+  /* interior equations for M_i = 2 c_i:
+     h_{i-1} M_{i-1} + 2 (h_{i-1} + h_i) M_i + h_i M_{i+1} = 6 (s_i - s_{i-1}) */
+  for (i = 1; i < n - 1; i++) {
+    double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+    diag[i] = 2.0 * (h0 + h1);
+    off[i] = h1;
+    rhs[i] = 6.0 * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+  }
+  /* Thomas algorithm on the symmetric tridiagonal system */
+  for (i = 2; i < n - 1; i++) {
+    double h0 = x[i] - x[i - 1];
+    double m = h0 / diag[i - 1];
+    diag[i] -= m * off[i - 1];
+    rhs[i] -= m * rhs[i - 1];
+  }
+  c[0] = 0.0;
+  c[n - 1] = 0.0;
+  {
+    double Mnext = 0.0;
+    for (i = n - 2; i >= 1; i--) {
+      double M = (rhs[i] - off[i] * Mnext) / diag[i];
+      c[i] = 0.5 * M;
+      Mnext = M;
+    }
+  }
+  free(diag);
+  free(rhs);
+  free(off);
+}
+
+static void qo_spline_init(void) {
+  if (qo_sp_ready) return;
+#ifdef _OPENMP
+#pragma omp critical(qo_spline)
+#endif
+  {
+    if (!qo_sp_ready) {
+      qo_spline_solve(qcbc_qnm_a, qcbc_qnm_fring, QCBC_QNM_N, qo_sp_c[0]);
+      qo_spline_solve(qcbc_qnm_a, qcbc_qnm_fdamp, QCBC_QNM_N, qo_sp_c[1]);
+      qo_sp_ready = 1;
+    }
+  }
+}
+
+static double qo_spline_eval(int which, double x) {
+  const double *xa = qcbc_qnm_a;
+  const double *ya = which ? qcbc_qnm_fdamp : qcbc_qnm_fring;
+  const double *c = qo_sp_c[which];
+  int lo = 0, hi = QCBC_QNM_N - 1;
+  /* largest lo with xa[lo] <= x, restricted to [0, n-2] */
+  while (hi > lo + 1) {
+    int mid = (lo + hi) / 2;
+    if (xa[mid] > x)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  {
+    double h = xa[lo + 1] - xa[lo];
+    double dy = ya[lo + 1] - ya[lo];
+    double dx = x - xa[lo];
+    double b = dy / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
+    double d = (c[lo + 1] - c[lo]) / (3.0 * h);
+    return ya[lo] + dx * (b + dx * (c[lo] + dx * d));
+  }
+}
+
+int qo_qnm(double finspin, double *fr, double *fd) {
+  qo_spline_init();
+  if (!(finspin <= 1.0) || !(finspin >= -1.0)) return QO_EDOM;
+  *fr = qo_spline_eval(0, finspin);
+  *fd = qo_spline_eval(1, finspin);
+  return QO_OK;
+}
+
+int qo_spline_eval_flat(int which, const double *x, int n, double *y) {
+  int i;
+  qo_spline_init();
+  for (i = 0; i < n; i++) y[i] = qo_spline_eval(which, x[i]);
+  return 0;
+}
+
+/* --------------------------------------------------------- PN phasing */
+/* PNPhasing_F2 (IMRPhenomD_internals.c:271-365; arXiv:0907.0700 Eq. 3.18,
+ * arXiv:1303.7412 Eqs. 3.15-16) with the 3PN spin-spin term removed again as
+ * IMRPhenomD.c:493 does.  Long-double literals as in the reference; each
+ * coefficient is rounded to double on assignment. */
+static double qo_pn_ss3(double eta, double m1M, double m2M, double chi1, double chi2) {
+  double c1sq = chi1 * chi1, c2sq = chi2 * chi2;
+  double ss = (326.75L / 1.12L + 557.5L / 1.8L * eta) * eta * chi1 * chi2;
+  ss += ((4703.5L / 8.4L + 2935.L / 6.L * m1M - 120.L * m1M * m1M) +
+         (-4108.25L / 6.72L - 108.5L / 1.2L * m1M + 125.5L / 3.6L * m1M * m1M)) *
+        m1M * m1M * c1sq;
+  ss += ((4703.5L / 8.4L + 2935.L / 6.L * m2M - 120.L * m2M * m2M) +
+         (-4108.25L / 6.72L - 108.5L / 1.2L * m2M + 125.5L / 3.6L * m2M * m2M)) *
+        m2M * m2M * c2sq;
+  return ss;
+}
+
+static void qo_pn_phasing(qo_phenomd *p) {
+  const double m1 = p->m1, m2 = p->m2, chi1 = p->chi1, chi2 = p->chi2;
+  const double mtot = m1 + m2;
+  const double d = (m1 - m2) / (m1 + m2);
+  const double eta = m1 * m2 / mtot / mtot;
+  const double m1M = m1 / mtot, m2M = m2 / mtot;
+  const double SL = m1M * m1M * chi1 + m2M * m2M * chi2;
+  const double dSigmaL = d * (m2M * chi2 - m1M * chi1);
+  const double pfaN = 3.L / (128.L * eta);
+  double *v = p->pn_v, *vl = p->pn_vlogv;
+  double sigma, ss3, gam;
+  int k;
+  memset(p->pn_v, 0, sizeof(p->pn_v));
+  memset(p->pn_vlogv, 0, sizeof(p->pn_vlogv));
+
+  /* non-spinning terms */
+  v[0] = 1.L;
+  v[2] = 5.L * (743.L / 84.L + 11.L * eta) / 9.L;
+  v[3] = -16.L * QO_PI;
+  v[4] = 5.L * (3058.673L / 7.056L + 5429.L / 7.L * eta + 617.L * eta * eta) / 72.L;
+  v[5] = 5.L / 9.L * (7729.L / 84.L - 13.L * eta) * QO_PI;
+  vl[5] = 5.L / 3.L * (7729.L / 84.L - 13.L * eta) * QO_PI;
+  v[6] = (11583.231236531L / 4.694215680L - 640.L / 3.L * QO_PI * QO_PI -
+          6848.L / 21.L * QO_GAMMA) +
+         eta * (-15737.765635L / 3.048192L + 2255. / 12. * QO_PI * QO_PI) +
+         eta * eta * 76055.L / 1728.L - eta * eta * eta * 127825.L / 1296.L;
+  v[6] += (-6848.L / 21.L) * log(4.);
+  vl[6] = -6848.L / 21.L;
+  v[7] = QO_PI * (77096675.L / 254016.L + 378515.L / 1512.L * eta - 74045.L / 756.L * eta * eta);
+
+  /* 2PN spin-spin, quadrupole-monopole and self-spin (qm_def = 1) */
+  sigma = eta * (721.L / 48.L * chi1 * chi2 - 247.L / 48.L * (chi1 * chi2));
+  sigma += (720.L - 1.L) / 96.0L * m1M * m1M * chi1 * chi1;
+  sigma += (720.L - 1.L) / 96.0L * m2M * m2M * chi2 * chi2;
+  sigma -= (240.L - 7.L) / 96.0L * m1M * m1M * (chi1 * chi1);
+  sigma -= (240.L - 7.L) / 96.0L * m2M * m2M * (chi2 * chi2);
+  ss3 = qo_pn_ss3(eta, m1M, m2M, chi1, chi2);
+  gam = (554345.L / 1134.L + 110.L * eta / 9.L) * SL + (13915.L / 84.L - 10.L * eta / 3.L) * dSigmaL;
+
+  /* spin-orbit and spin-spin up to 3.5PN (the reference's fall-through) */
+  v[7] += (-8980424995.L / 762048.L + 6586595.L * eta / 756.L - 305.L * eta * eta / 36.L) * SL -
+          (170978035.L / 48384.L - 2876425.L * eta / 672.L - 4735.L * eta * eta / 144.L) * dSigmaL;
+  v[6] += QO_PI * (3760.L * SL + 1490.L * dSigmaL) / 3.L + ss3;
+  v[5] += -1.L * gam;
+  vl[5] += -3.L * gam;
+  v[4] += -10.L * sigma;
+  v[3] += 188.L * SL / 3.L + 25.L * dSigmaL;
+
+  for (k = 0; k <= 12; k++) {
+    v[k] *= pfaN;
+    vl[k] *= pfaN;
+  }
+  /* Subtract3PNSS, IMRPhenomD_internals.c:1619-1629, applied at IMRPhenomD.c:493 */
+  v[6] -= qo_pn_ss3(m1 * m2 / (mtot * mtot), m1 / mtot, m2 / mtot, chi1, chi2) * v[0];
+}
+
+/* ------------------------------------------------------------ amplitude */
+/* init_amp_ins_prefactors, IMRPhenomD_internals.c:612-665 (arXiv:1508.07253
+ * Eqs. 29-30 and App. B). */
+static void qo_amp_prefactors(qo_phenomd *p) {
+  const qo_pow pp = qo_pi_powers();
+  const double eta = p->eta, chi1 = p->chi1, chi2 = p->chi2;
+  const double chi12 = chi1 * chi1, chi22 = chi2 * chi2;
+  const double eta2 = eta * eta, eta3 = eta * eta2;
+  const double Pi = QO_PI, Pi2 = pp.two;
+  const double Seta = sqrt(1.0 - 4.0 * eta);
+  double n43, n53, n2;
+
+  p->ai_amp0 = (sqrt(2.0 / 3.0) * sqrt(eta)) / pp.sixth; /* amp0Func :540-542 */
+  p->ai_23 = ((-969 + 1804 * eta) * pp.two_thirds) / 672.;
+  p->ai_1 = ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48.;
+  n43 = -27312085.0 - 10287648 * chi22 - 10287648 * chi12 * (1 + Seta) + 10287648 * chi22 * Seta +
+        24 * (-1975055 + 857304 * chi12 - 994896 * chi1 * chi2 + 857304 * chi22) * eta +
+        35371056 * eta2;
+  p->ai_43 = (n43 * pp.four_thirds) / 8.128512e6;
+  n53 = chi2 * (-285197 * (-1 + Seta) + 4 * (-91902 + 1579 * Seta) * eta - 35632 * eta2) +
+        chi1 * (285197 * (1 + Seta) - 4 * (91902 + 1579 * Seta) * eta - 35632 * eta2) +
+        42840 * (-1.0 + 4 * eta) * Pi;
+  p->ai_53 = (pp.five_thirds * n53) / 32256.;
+  n2 = -336 * (-3248849057.0 + 2943675504 * chi12 - 3339284256 * chi1 * chi2 + 2943675504 * chi22) * eta2 -
+       324322727232 * eta3 -
+       7 * (-177520268561 + 107414046432 * chi22 + 107414046432 * chi12 * (1 + Seta) -
+            107414046432 * chi22 * Seta + 11087290368 * (chi1 + chi2 + chi1 * Seta - chi2 * Seta) * Pi) +
+       12 * eta *
+           (-545384828789 - 176491177632 * chi1 * chi2 + 202603761360 * chi22 +
+            77616 * chi12 * (2610335 + 995766 * Seta) - 77287373856 * chi22 * Seta +
+            5841690624 * (chi1 + chi2) * Pi + 21384760320 * Pi2);
+  p->ai_2 = -(Pi2 * n2) / 6.0085960704e10;
+  p->ai_73 = p->rho1;
+  p->ai_83 = p->rho2;
+  p->ai_3 = p->rho3;
+}
+
+/* AmpInsAnsatz, IMRPhenomD_internals.c:601-610 */
+static double qo_amp_ins_pw(const qo_phenomd *p, double Mf, const qo_pow *w) {
+  double Mf2 = w->two;
+  double Mf3 = Mf * Mf2;
+  return 1.0 + w->two_thirds * p->ai_23 + Mf * p->ai_1 + w->four_thirds * p->ai_43 +
+         w->five_thirds * p->ai_53 + Mf2 * p->ai_2 + w->seven_thirds * p->ai_73 +
+         w->eight_thirds * p->ai_83 + Mf3 * p->ai_3;
+}
+
+double qo_amp_ins_ansatz(const qo_phenomd *p, double Mf) {
+  qo_pow w;
+  qo_powers(&w, Mf);
+  return qo_amp_ins_pw(p, Mf, &w);
+}
+
+/* d/dMf of AmpInsAnsatz (the quantity of DAmpInsAnsatz, :671-702), written
+ * term by term from the prefactors. */
+double qo_damp_ins_ansatz(const qo_phenomd *p, double Mf) {
+  double t = cbrt(Mf); /* Mf^(1/3) */
+  return (2.0 / 3.0) * p->ai_23 / t + p->ai_1 + (4.0 / 3.0) * p->ai_43 * t +
+         (5.0 / 3.0) * p->ai_53 * t * t + 2.0 * p->ai_2 * Mf +
+         (7.0 / 3.0) * p->ai_73 * Mf * t + (8.0 / 3.0) * p->ai_83 * Mf * t * t +
+         3.0 * p->ai_3 * Mf * Mf;
+}
+
+/* AmpMRDAnsatz, :757-767 (arXiv:1508.07253 Eq. 19) */
+double qo_amp_mrd_ansatz(const qo_phenomd *p, double f) {
+  double fDMg3 = p->fDM * p->gamma3;
+  double df = f - p->fRD;
+  return exp(-(df)*p->gamma2 / (fDMg3)) * (fDMg3 * p->gamma1) / (df * df + fDMg3 * fDMg3);
+}
+
+/* DAmpMRDAnsatz, :772-787 */
+double qo_damp_mrd_ansatz(const qo_phenomd *p, double f) {
+  double fDMg3 = p->fDM * p->gamma3;
+  double df = f - p->fRD;
+  double ex = exp((df * p->gamma2) / fDMg3);
+  double s = df * df + fDMg3 * fDMg3;
+  return (-2 * p->fDM * df * p->gamma3 * p->gamma1) / (ex * (s * s)) -
+         (p->gamma2 * p->gamma1) / (ex * s);
+}
+
+/* fmaxCalc, :793-805 (Eq. 20) */
+static double qo_fmax(const qo_phenomd *p) {
+  if (p->gamma2 <= 1)
+    return fabs(p->fRD + (p->fDM * (-1 + sqrt(1 - p->gamma2 * p->gamma2)) * p->gamma3) / p->gamma2);
+  return fabs(p->fRD + (p->fDM * (-1) * p->gamma3) / p->gamma2);
+}
+
+/* AmpIntAnsatz, :816-821 (Eq. 21) */
+double qo_amp_int_ansatz(const qo_phenomd *p, double Mf) {
+  double Mf2 = Mf * Mf, Mf3 = Mf * Mf2, Mf4 = Mf * Mf3;
+  return p->delta[0] + p->delta[1] * Mf + p->delta[2] * Mf2 + p->delta[3] * Mf3 + p->delta[4] * Mf4;
+}
+
+/* ComputeDeltasFromCollocation, :1005-1063: the quartic through (f1,v1),
+ * (f2,v2), (f3,v3) with slopes d1 at f1 and d2 at f3.  Solved in long double
+ * by Gaussian elimination with partial pivoting on the variable t = Mf/f3,
+ * then rescaled. */
+static void qo_collocation(qo_phenomd *p) {
+  long double A[5][6];
+  long double s = p->f3;
+  long double t[3] = {p->f1 / s, p->f2 / s, 1.0L};
+  long double vals[3] = {p->v1, p->v2, p->v3};
+  int r, c, k;
+  for (r = 0; r < 3; r++) {
+    long double tp = 1.0L;
+    for (c = 0; c < 5; c++) {
+      A[r][c] = tp;
+      tp *= t[r];
+    }
+    A[r][5] = vals[r];
+  }
+  for (k = 0; k < 2; k++) {
+    long double tt = k == 0 ? t[0] : t[2];
+    long double dv = (k == 0 ? p->d1 : p->d2) * s; /* dP/dt = s dP/dMf */
+    long double tp = 1.0L;
+    A[3 + k][0] = 0.0L;
+    for (c = 1; c < 5; c++) {
+      A[3 + k][c] = c * tp;
+      tp *= tt;
+    }
+    A[3 + k][5] = dv;
+  }
+  for (c = 0; c < 5; c++) {
+    int piv = c;
+    for (r = c + 1; r < 5; r++)
+      if (fabsl(A[r][c]) > fabsl(A[piv][c])) piv = r;
+    if (piv != c)
+      for (k = 0; k < 6; k++) {
+        long double tmp = A[c][k];
+        A[c][k] = A[piv][k];
+        A[piv][k] = tmp;
+      }
+    for (r = c + 1; r < 5; r++) {
+      long double m = A[r][c] / A[c][c];
+      for (k = c; k < 6; k++) A[r][k] -= m * A[c][k];
+    }
+  }
+  {
+    long double x[5], sp = 1.0L;
+    for (r = 4; r >= 0; r--) {
+      long double acc = A[r][5];
+      for (k = r + 1; k < 5; k++) acc -= A[r][k] * x[k];
+      x[r] = acc / A[r][r];
+    }
+    for (k = 0; k < 5; k++) {
+      p->delta[k] = (double)(x[k] / sp);
+      sp *= s;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- phase */
+/* PhiInsAnsatzInt, :1399-1425 */
+static double qo_phi_ins(const qo_phenomd *p, double Mf, const qo_pow *w) {
+  static qo_pow pp;
+  static int ready = 0;
+  double v, logv, ph;
+  if (!ready) {
+    pp = qo_pi_powers();
+    ready = 1;
+  }
+  v = w->third * pp.third;
+  logv = log(v);
+  ph = p->pi_init;
+  ph += p->pi_23 * w->two_thirds;
+  ph += p->pi_13 * w->third;
+  ph += p->pi_13L * logv * w->third;
+  ph += p->pi_L * logv;
+  ph += p->pi_m13 / w->third;
+  ph += p->pi_m23 / w->two_thirds;
+  ph += p->pi_m1 / Mf;
+  ph += p->pi_m53 / w->five_thirds;
+  ph += (p->pi_1 * Mf + p->pi_43 * w->four_thirds + p->pi_53 * w->five_thirds + p->pi_2 * w->two) / p->eta;
+  return ph;
+}
+
+/* DPhiInsAnsatzInt, :1461-1501 */
+static double qo_dphi_ins(const qo_phenomd *p, double Mf) {
+  const qo_pow pp = qo_pi_powers();
+  const double *pv = p->pn_v, *pl = p->pn_vlogv;
+  const double v = cbrt(QO_PI * Mf);
+  const double logv = log(v);
+  const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6, v8 = v * v7;
+  double D = 0.0;
+  D += +2.0 * pv[7] * v7;
+  D += (pv[6] + pl[6] * (1.0 + logv)) * v6;
+  D += pl[5] * v5;
+  D += -1.0 * pv[4] * v4;
+  D += -2.0 * pv[3] * v3;
+  D += -3.0 * pv[2] * v2;
+  D += -4.0 * pv[1] * v;
+  D += -5.0 * pv[0];
+  D /= v8 * 3.0 / QO_PI;
+  D += (p->sigma1 + p->sigma2 * v / pp.third + p->sigma3 * v2 / pp.two_thirds + (p->sigma4 / QO_PI) * v3) / p->eta;
+  return D;
+}
+
+/* PhiIntAnsatz, :1298-1303 and DPhiIntAnsatz, :1309-1311 */
+static double qo_phi_int(const qo_phenomd *p, double Mf) {
+  return p->beta1 * Mf - p->beta3 / (3. * (Mf * Mf * Mf)) + p->beta2 * log(Mf);
+}
+static double qo_dphi_int(const qo_phenomd *p, double Mf) {
+  double m2 = Mf * Mf;
+  return (p->beta1 + p->beta3 / (m2 * m2) + p->beta2 / Mf) / p->eta;
+}
+
+/* PhiMRDAnsatzInt, :1222-1233 and DPhiMRD, :1238-1240 */
+static double qo_phi_mrd(const qo_phenomd *p, double f) {
+  double sq = sqrt(f);
+  double f15 = f * sq;
+  double f075 = sqrt(f15);
+  return -(p->alpha2 / f) + (4.0 / 3.0) * (p->alpha3 * f075) + p->alpha1 * f +
+         p->alpha4 * atan((f - p->alpha5 * p->fRD) / p->fDM);
+}
+static double qo_dphi_mrd(const qo_phenomd *p, double f) {
+  double x = f - p->alpha5 * p->fRD;
+  return (p->alpha1 + p->alpha2 / (f * f) + p->alpha3 / pow(f, 0.25) +
+          p->alpha4 / (p->fDM * (1 + (x * x) / (p->fDM * p->fDM)))) /
+         p->eta;
+}
+
+/* IMRPhenDPhase, :1590-1612 (with StepFunc_boolean, :403-405) */
+static double qo_phase_pw(const qo_phenomd *p, double f, const qo_pow *w) {
+  if (!(f >= p->fPhiIns)) return qo_phi_ins(p, f, w);
+  if (f >= p->fPhiMRD) return 1.0 / p->eta * qo_phi_mrd(p, f) + p->C1MRD + p->C2MRD * f;
+  return 1.0 / p->eta * qo_phi_int(p, f) + p->C1Int + p->C2Int * f;
+}
+
+double qo_phase_region(const qo_phenomd *p, double Mf, int region) {
+  qo_pow w;
+  qo_powers(&w, Mf);
+  if (region == 0) return qo_phi_ins(p, Mf, &w);
+  if (region == 2) return 1.0 / p->eta * qo_phi_mrd(p, Mf) + p->C1MRD + p->C2MRD * Mf;
+  return 1.0 / p->eta * qo_phi_int(p, Mf) + p->C1Int + p->C2Int * Mf;
+}
+
+/* IMRPhenDAmplitude, :1107-1135 */
+static double qo_amp_pw(const qo_phenomd *p, double f, const qo_pow *w) {
+  double pre = p->ai_amp0 / (f * w->sixth);
+  if (!(f >= QO_AMP_JOIN)) return pre * qo_amp_ins_pw(p, f, w);
+  if (f >= p->fmaxCalc) return pre * qo_amp_mrd_ansatz(p, f);
+  return pre * qo_amp_int_ansatz(p, f);
+}
+
+/* ----------------------------------------------------------- setup */
+/* IMRPhenomDGenerateh22FDAmpPhase_internal, IMRPhenomD.c:413-523, plus
+ * ComputeIMRPhenomDAmplitudeCoefficients (:1072-1100),
+ * ComputeIMRPhenomDPhaseCoefficients (:1509-1540) and
+ * ComputeIMRPhenDPhaseConnectionCoefficients (:1547-1584). */
+int qo_phenomd_setup(qo_phenomd *p, double m1_in, double m2_in, double chi1_in, double chi2_in,
+                     double distance, double phi0, double fref) {
+  double fr, fd, erad;
+  qo_pow w;
+  int st;
+  memset(p, 0, sizeof(*p));
+  if (m1_in > m2_in) {
+    p->m1 = m1_in; p->m2 = m2_in; p->chi1 = chi1_in; p->chi2 = chi2_in;
+  } else {
+    p->m1 = m2_in; p->m2 = m1_in; p->chi1 = chi2_in; p->chi2 = chi1_in;
+  }
+  p->M = p->m1 + p->m2;
+  p->eta = p->m1 * p->m2 / (p->M * p->M);
+  if (p->eta > 0.25 || p->eta < 0.0) return QO_EDOM;
+  p->Msec = p->M * QO_MTSUN_SI;
+  p->amp0 = p->M * QO_MRSUN_SI * p->M * QO_MTSUN_SI / distance;
+  p->fCut = QO_F_CUT / p->Msec;
+  p->finspin = qo_final_spin(p->eta, p->chi1, p->chi2);
+
+  /* amplitude coefficients */
+  p->chi = qo_chipn(p->eta, p->chi1, p->chi2);
+  st = qo_qnm(p->finspin, &fr, &fd);
+  if (st) return st;
+  erad = qo_erad(p->eta, p->chi1, p->chi2);
+  p->fRD = fr / (1.0 - erad);
+  p->fDM = fd / (1.0 - erad);
+  p->gamma1 = qo_fit(QCBC_FIT_GAMMA1, p->eta, p->chi);
+  p->gamma2 = qo_fit(QCBC_FIT_GAMMA2, p->eta, p->chi);
+  p->gamma3 = qo_fit(QCBC_FIT_GAMMA3, p->eta, p->chi);
+  p->fmaxCalc = qo_fmax(p);
+  p->rho1 = qo_fit(QCBC_FIT_RHO1, p->eta, p->chi);
+  p->rho2 = qo_fit(QCBC_FIT_RHO2, p->eta, p->chi);
+  p->rho3 = qo_fit(QCBC_FIT_RHO3, p->eta, p->chi);
+  qo_amp_prefactors(p);
+  p->f1 = QO_AMP_JOIN;
+  p->f3 = p->fmaxCalc;
+  p->f2 = p->f1 + (p->f3 - p->f1) / 2.0;
+  qo_powers(&w, p->f1);
+  p->v1 = qo_amp_ins_pw(p, p->f1, &w);
+  p->d1 = qo_damp_ins_ansatz(p, p->f1);
+  p->v3 = qo_amp_mrd_ansatz(p, p->f3);
+  p->d2 = qo_damp_mrd_ansatz(p, p->f3);
+  p->v2 = qo_fit(QCBC_FIT_V2, p->eta, p->chi);
+  qo_collocation(p);
+
+  /* phase coefficients */
+  p->sigma1 = qo_fit(QCBC_FIT_SIGMA1, p->eta, p->chi);
+  p->sigma2 = qo_fit(QCBC_FIT_SIGMA2, p->eta, p->chi);
+  p->sigma3 = qo_fit(QCBC_FIT_SIGMA3, p->eta, p->chi);
+  p->sigma4 = qo_fit(QCBC_FIT_SIGMA4, p->eta, p->chi);
+  p->beta1 = qo_fit(QCBC_FIT_BETA1, p->eta, p->chi);
+  p->beta2 = qo_fit(QCBC_FIT_BETA2, p->eta, p->chi);
+  p->beta3 = qo_fit(QCBC_FIT_BETA3, p->eta, p->chi);
+  p->alpha1 = qo_fit(QCBC_FIT_ALPHA1, p->eta, p->chi);
+  p->alpha2 = qo_fit(QCBC_FIT_ALPHA2, p->eta, p->chi);
+  p->alpha3 = qo_fit(QCBC_FIT_ALPHA3, p->eta, p->chi);
+  p->alpha4 = qo_fit(QCBC_FIT_ALPHA4, p->eta, p->chi);
+  p->alpha5 = qo_fit(QCBC_FIT_ALPHA5, p->eta, p->chi);
+  qo_pn_phasing(p);
+
+  /* init_phi_ins_prefactors, :1427-1456 */
+  {
+    const qo_pow pp = qo_pi_powers();
+    p->pi_init = p->pn_v[5] - QO_PI_4;
+    p->pi_23 = p->pn_v[7] * pp.two_thirds;
+    p->pi_13 = p->pn_v[6] * pp.third;
+    p->pi_13L = p->pn_vlogv[6] * pp.third;
+    p->pi_L = p->pn_vlogv[5];
+    p->pi_m13 = p->pn_v[4] / pp.third;
+    p->pi_m23 = p->pn_v[3] / pp.two_thirds;
+    p->pi_m1 = p->pn_v[2] / QO_PI;
+    p->pi_m53 = p->pn_v[0] / pp.five_thirds;
+    p->pi_1 = p->sigma1;
+    p->pi_43 = p->sigma2 * 3.0 / 4.0;
+    p->pi_53 = p->sigma3 * 3.0 / 5.0;
+    p->pi_2 = p->sigma4 / 2.0;
+  }
+
+  /* C1 connection, :1547-1584 */
+  p->fPhiIns = QO_PHI_JOIN;
+  p->fPhiMRD = 0.5 * p->fRD;
+  p->C2Int = qo_dphi_ins(p, p->fPhiIns) - qo_dphi_int(p, p->fPhiIns);
+  qo_powers(&w, p->fPhiIns);
+  p->C1Int = qo_phi_ins(p, p->fPhiIns, &w) - 1.0 / p->eta * qo_phi_int(p, p->fPhiIns) - p->C2Int * p->fPhiIns;
+  {
+    double phiT = 1.0 / p->eta * qo_phi_int(p, p->fPhiMRD) + p->C1Int + p->C2Int * p->fPhiMRD;
+    double dphiT = p->C2Int + qo_dphi_int(p, p->fPhiMRD); /* DPhiIntTemp, :1317-1325 */
+    double dphiM = qo_dphi_mrd(p, p->fPhiMRD);
+    p->C2MRD = dphiT - dphiM;
+    p->C1MRD = phiT - 1.0 / p->eta * qo_phi_mrd(p, p->fPhiMRD) - p->C2MRD * p->fPhiMRD;
+  }
+
+  /* time shift and reference phase, IMRPhenomD.c:504-521 */
+  p->t0 = qo_dphi_mrd(p, p->fmaxCalc);
+  p->MfRef = p->Msec * fref;
+  qo_powers(&w, p->MfRef);
+  p->phifRef = qo_phase_pw(p, p->MfRef, &w);
+  p->phi_precalc = 2. * phi0 + p->phifRef;
+  return QO_OK;
+}
+
+/* Per-bin body of IMRPhenomD.c:527-556 */
+void qo_phenomd_eval(const qo_phenomd *p, double f, double *amp, double *phase) {
+  if (f > p->fCut) {
+    *amp = 0.;
+    *phase = 0.;
+  } else {
+    double Mf = p->Msec * f;
+    qo_pow w;
+    double a, phi;
+    qo_powers(&w, Mf);
+    a = qo_amp_pw(p, Mf, &w);
+    phi = qo_phase_pw(p, Mf, &w);
+    phi -= p->t0 * (Mf - p->MfRef) + p->phi_precalc;
+    *amp = p->amp0 * a;
+    *phase = phi;
+  }
+}
+
+/* IMRPhenomDGenerateh22FDAmpPhase, IMRPhenomD.c:186-248 */
+int qo_h22_ampphase(const double *freq, size_t n, double phi0, double fref, double m1_SI,
+                    double m2_SI, double chi1, double chi2, double distance, double *amp,
+                    double *phase) {
+  const double m1 = m1_SI / QO_MSUN_SI, m2 = m2_SI / QO_MSUN_SI;
+  qo_phenomd p;
+  size_t i;
+  int st;
+  if (!freq || !amp || !phase || n == 0) return QO_EFAULT;
+  if (fref < 0 || m1 <= 0 || m2 <= 0 || freq[0] <= 0 || freq[n - 1] < 0 || distance <= 0)
+    return QO_EDOM;
+  if (chi1 > 1.0 || chi1 < -1.0 || chi2 > 1.0 || chi2 < -1.0) return QO_EDOM;
+  if (QO_F_CUT / ((m1 + m2) * QO_MTSUN_SI) <= freq[0]) return QO_EDOM;
+  st = qo_phenomd_setup(&p, m1, m2, chi1, chi2, distance, phi0, fref);
+  if (st) return st;
+  for (i = 0; i < n; i++) qo_phenomd_eval(&p, freq[i], &amp[i], &phase[i]);
+  return QO_OK;
+}
+
+int qo_setup_flat(double m1, double m2, double chi1, double chi2, double distance, double phi0,
+                  double fref, double *out, int nout) {
+  qo_phenomd p;
+  int st = qo_phenomd_setup(&p, m1, m2, chi1, chi2, distance, phi0, fref);
+  int n = (int)(sizeof(p) / sizeof(double));
+  if (nout > n) nout = n;
+  memcpy(out, &p, sizeof(double) * nout);
+  return st;
+}
+
+/* ------------------------------------------------ detector utilities */
+/* fbegin, Utilities.c:587-629: 3PN frequency at the segment start */
+double qo_fbegin(const double *param) {
+  const double PI = 3.1415926535897931159979634685442; /* Constants.h:2 */
+  double gamma_E = 0.5772156649;
+  double Mchirp = exp(param[0]) / QO_MSUN_SI * QO_TSUN;
+  double M = exp(param[1]) / QO_MSUN_SI * QO_TSUN;
+  double eta = pow((Mchirp / M), (5.0 / 3.0));
+  double dm, m1, m2, chi1, chi2, chi, tc, th, th2, th3, th4, th5, th6;
+  double PN1, PN15, PN2, PN25, PN3;
+  if (eta > 0.25) eta = 0.25;
+  dm = sqrt(1.0 - 4.0 * eta);
+  m1 = M * (1.0 + dm) / 2.0;
+  m2 = M * (1.0 - dm) / 2.0;
+  chi1 = param[2];
+  chi2 = param[3];
+  chi = (m1 * chi1 + m2 * chi2) / M;
+  tc = param[5];
+  th = pow(eta * tc / (5.0 * M), -1.0 / 8.0);
+  th2 = th * th;
+  th3 = th2 * th;
+  th4 = th2 * th2;
+  th5 = th2 * th3;
+  th6 = th3 * th3;
+  PN1 = (11.0 / 32.0 * eta + 743.0 / 2688.0) * th2;
+  PN15 = -3.0 * PI / 10.0 * th3 + (1.0 / 160.0) * (113.0 * chi - 38.0 * eta * (chi1 + chi2)) * th3;
+  PN2 = (1855099.0 / 14450688.0 + 56975.0 / 258048.0 * eta + 371.0 / 2048.0 * eta * eta) * th4 +
+        (1.0 / 14450688.0) * (-3386880.0 * chi * chi + 1512.0 * chi1 * chi2) * th4;
+  PN25 = -(7729.0 / 21504.0 - 13.0 / 256.0 * eta) * PI * th5;
+  PN3 = (-720817631400877.0 / 288412611379200.0 + 53.0 / 200.0 * PI * PI + 107.0 / 280.0 * gamma_E +
+         (25302017977.0 / 4161798144.0 - 451.0 / 2048.0 * PI * PI) * eta - 30913.0 / 1835008.0 * eta * eta +
+         235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * log(2.0 * th)) *
+        th6;
+  return th3 / (8.0 * M * PI) * (1.0 + PN1 + PN15 + PN2 + PN25 + PN3);
+}
+
+/* Times, Utilities.c:1078-1109 */
+void qo_times(double alpha, double sindelta, double gmst, double *dtimes) {
+  double cosd = sqrt(1.0 - sindelta * sindelta);
+  double ca = cos(gmst - alpha), sa = sin(gmst - alpha);
+  int d;
+  for (d = 0; d < 3; d++) {
+    const double *v = qcbc_vertex[d];
+    double r = v[0] * cosd * ca + v[1] * cosd * sa + v[2] * sindelta;
+    dtimes[d] = -r / QO_CLIGHT;
+  }
+}
+
+/* F_ant / F_LHO / F_LLO / F_VIRGO, Utilities.c:995-1076: the detectors'
+ * expanded polynomials, evaluated term by term in source order. */
+static double qo_ant_poly(int k, const double *x) {
+  double s = 0.0;
+  int t, j;
+  for (t = 0; t < qcbc_ant_nterms[k]; t++) {
+    double term = qcbc_ant_coef[k][t];
+    for (j = 0; j < QCBC_ANT_MAXFAC && qcbc_ant_fac[k][t][j] >= 0; j++) term *= x[(int)qcbc_ant_fac[k][t][j]];
+    s = t == 0 ? term : s + term;
+  }
+  return s;
+}
+
+void qo_f_ant(double psi, double alpha, double sindelta, double gmst, double *fp, double *fc, int id) {
+  double x[9];
+  if (id < 0 || id > 2) return; /* the reference leaves the outputs untouched */
+  x[0] = cos(psi);
+  x[1] = sin(psi);
+  x[2] = sqrt(1.0 - sindelta * sindelta);
+  x[3] = sindelta;
+  x[4] = cos(alpha - gmst);
+  x[5] = sin(alpha - gmst);
+  x[6] = x[0] * x[0];
+  x[7] = x[3] * x[3];
+  x[8] = x[4] * x[4];
+  *fp = qo_ant_poly(2 * id, x);
+  *fc = qo_ant_poly(2 * id + 1, x);
+}
+
+/* fourier_nwip, Utilities.c:716-736 */
+double qo_fourier_nwip(const double *a, const double *b, const double *Sn, int imin, int imax, int N) {
+  double arg = 0.0;
+  int i;
+  for (i = imin; i < imax; i++) {
+    double product = a[i] * b[i] + a[N - i] * b[N - i];
+    arg += product / (Sn[i]);
+  }
+  return 4.0 * arg;
+}
+
+/* ------------------------------------------------ templates and logL */
+/* fulltemplates, QuickCBC.c:3683-3777 */
+int qo_fulltemplates(int nifo, const int *labels, double gmst, double fref, double **hwave,
+                     const double *freq, const double *params, int N) {
+  double dtimes[3];
+  double fs = qo_fbegin(params);
+  double Tobs = 1.0 / freq[1];
+  double sqT = sqrt(Tobs);
+  double mc = exp(params[0]), mt = exp(params[1]);
+  double eta = pow((mc / mt), (5.0 / 3.0));
+  double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  double phi0 = 0.5 * params[4];
+  double ts = Tobs - params[5];
+  double distance = exp(params[6]);
+  double alpha = params[7], sindelta = params[8], psi = params[9], ciota = params[10];
+  double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
+  double *amp = malloc(sizeof(double) * (N / 2)), *phase = malloc(sizeof(double) * (N / 2));
+  int st, id, i;
+  if (!amp || !phase) {
+    free(amp);
+    free(phase);
+    return QO_ENOMEM;
+  }
+  st = qo_h22_ampphase(freq, N / 2, phi0, fref, m1_SI, m2_SI, params[2], params[3], distance, amp, phase);
+  if (st) {
+    free(amp);
+    free(phase);
+    return st;
+  }
+  qo_times(alpha, sindelta, gmst, dtimes);
+  for (id = 0; id < nifo; id++) {
+    double Fp = 0.0, Fc = 0.0, Fs, lambda, td;
+    qo_f_ant(psi, alpha, sindelta, gmst, &Fp, &Fc, labels[id]);
+    Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
+    lambda = atan2(Ac * Fc, Ap * Fp);
+    if (lambda < 0.0) lambda += QO_TPI;
+    td = dtimes[labels[id]];
+    hwave[id][0] = 0.0;
+    hwave[id][N / 2] = 0.0;
+    for (i = 1; i < N / 2; i++) {
+      double f = freq[i];
+      if (f > fs) {
+        double A = Fs * QO_H22FAC * amp[i] / sqT;
+        double x = QO_TPI * f * (ts - td) + lambda - phase[i];
+        hwave[id][i] = A * cos(x);
+        hwave[id][N - i] = A * sin(x);
+      } else {
+        hwave[id][i] = 0.0;
+        hwave[id][N - i] = 0.0;
+      }
+    }
+  }
+  free(amp);
+  free(phase);
+  return QO_OK;
+}
+
+/* log_likelihood_full, QuickCBC.c:4236-4273 (fmin/fmax at run time) */
+double qo_log_likelihood_full(int nifo, const int *labels, double gmst, double fref, double fmin,
+                              double fmax, double **D, const double *params, const double *freq,
+                              double **SN, double *rho, int N, double Tobs, int *status) {
+  int imin = (int)(Tobs * fmin), imax = (int)(Tobs * fmax);
+  double logL = 0.0;
+  double **tw;
+  int i, st;
+  if (imax > N / 2) imax = N / 2;
+  tw = malloc(sizeof(double *) * (nifo + 1));
+  for (i = 0; i < nifo + 1; i++) tw[i] = malloc(sizeof(double) * (N + 1));
+  st = qo_fulltemplates(nifo, labels, gmst, fref, tw, freq, params, N);
+  if (status) *status = st;
+  if (st == 0) {
+    for (i = 0; i < nifo; i++) {
+      double HH = qo_fourier_nwip(tw[i], tw[i], SN[i], imin, imax, N);
+      double HD = qo_fourier_nwip(D[i], tw[i], SN[i], imin, imax, N);
+      rho[i] = HD / sqrt(HH);
+      logL += HD - 0.5 * HH;
+    }
+  }
+  for (i = 0; i < nifo + 1; i++) free(tw[i]);
+  free(tw);
+  return logL;
+}
+
+static double *qo_make_freq(int N, double Tobs) {
+  double *f = malloc(sizeof(double) * (N / 2));
+  int i;
+  f[0] = 1.0 / Tobs; /* QuickCBC.c:418-419 */
+  for (i = 1; i < N / 2; i++) f[i] = (double)(i) / Tobs;
+  return f;
+}
+
+void qo_loglike_batch(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax,
+                      double **D, double **SN, const double *freq, int N, double Tobs,
+                      const double *params, int B, double *logL, double *rho, int *status,
+                      int nthreads) {
+  int b;
+  (void)nthreads;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (b = 0; b < B; b++) {
+    int st = 0;
+    logL[b] = qo_log_likelihood_full(nifo, labels, gmst, fref, fmin, fmax, D, params + 11 * b, freq, SN,
+                                     rho + nifo * b, N, Tobs, &st);
+    if (status) status[b] = st;
+  }
+}
+
+double qo_loglike_flat(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax,
+                       const double *D, const double *SN, int N, double Tobs, const double *params,
+                       double *rho, int *status) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs), r;
+  int i;
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+  r = qo_log_likelihood_full(nifo, labels, gmst, fref, fmin, fmax, Dr, params, freq, Sr, rho, N, Tobs, status);
+  free(freq);
+  return r;
+}
+
+void qo_loglike_batch_flat(int nifo, const int *labels, double gmst, double fref, double fmin,
+                           double fmax, const double *D, const double *SN, int N, double Tobs,
+                           const double *params, int B, double *logL, double *rho, int *status,
+                           int nthreads) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs);
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+  qo_loglike_batch(nifo, labels, gmst, fref, fmin, fmax, Dr, Sr, freq, N, Tobs, params, B, logL, rho,
+                   status, nthreads);
+  free(freq);
+}
+
+int qo_fulltemplates_flat(int nifo, const int *labels, double gmst, double fref, double *hwave, int N,
+                          double Tobs, const double *params) {
+  double *rows[3];
+  double *freq = qo_make_freq(N, Tobs);
+  int i, st;
+  for (i = 0; i < nifo; i++) rows[i] = hwave + (size_t)i * N;
+  st = qo_fulltemplates(nifo, labels, gmst, fref, rows, freq, params, N);
+  free(freq);
+  return st;
+}
+
+int qo_h22_flat(const double *freq, size_t n, double phi0, double fref, double m1_SI, double m2_SI,
+                double chi1, double chi2, double distance, double *amp, double *phase) {
+  return qo_h22_ampphase(freq, n, phi0, fref, m1_SI, m2_SI, chi1, chi2, distance, amp, phase);
+}
+
+int qo_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* amplitude and phase of the h22 mode for one 11-entry parameter vector on
+ * the N/2 grid, with fulltemplates' parameter handling (QuickCBC.c:3700-3731) */
+int qo_waveform_params(const double *params, int N, double Tobs, double fref, double *amp, double *phase) {
+  double *freq = qo_make_freq(N, Tobs);
+  double mc = exp(params[0]), mt = exp(params[1]);
+  double eta = pow((mc / mt), (5.0 / 3.0));
+  double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  int st = qo_h22_ampphase(freq, N / 2, 0.5 * params[4], fref, mt * (1.0 + dm) / 2.0, mt * (1.0 - dm) / 2.0,
+                           params[2], params[3], exp(params[6]), amp, phase);
+  free(freq);
+  return st;
+}
+
+/* Individual pieces of the model for one system (tests pin them by finite
+ * differences and continuity): which = 0 AmpInsAnsatz, 1 its derivative,
+ * 2 AmpMRDAnsatz, 3 its derivative, 4 AmpIntAnsatz, 5/6/7 phase of the
+ * inspiral / intermediate / merger-ringdown branch (with connection
+ * constants), 8 DPhiInsAnsatzInt, 9 DPhiIntAnsatz, 10 DPhiMRD. */
+double qo_probe(int which, double m1, double m2, double chi1, double chi2, double Mf) {
+  qo_phenomd p;
+  if (qo_phenomd_setup(&p, m1, m2, chi1, chi2, 1.0, 0.0, 100.0)) return NAN;
+  switch (which) {
+    case 0: return qo_amp_ins_ansatz(&p, Mf);
+    case 1: return qo_damp_ins_ansatz(&p, Mf);
+    case 2: return qo_amp_mrd_ansatz(&p, Mf);
+    case 3: return qo_damp_mrd_ansatz(&p, Mf);
+    case 4: return qo_amp_int_ansatz(&p, Mf);
+    case 5: return qo_phase_region(&p, Mf, 0);
+    case 6: return qo_phase_region(&p, Mf, 1);
+    case 7: return qo_phase_region(&p, Mf, 2);
+    case 8: return qo_dphi_ins(&p, Mf);
+    case 9: return qo_dphi_int(&p, Mf);
+    case 10: return qo_dphi_mrd(&p, Mf);
+    default: return NAN;
+  }
+}

@@ -1,0 +1,112 @@
+/*
+ * CPU oracle for the QuickCBC waveform + likelihood path.  TEST
+ * INFRASTRUCTURE ONLY: nothing in the product links or calls this; it is the
+ * checker that the HIP path is compared against (and, timed, the CPU
+ * baseline).  Plain C99, FP64, GSL-free; the QuickCBC function each routine
+ * restates is cited beside it.
+ *
+ * Parity status of each part (what pins it) is listed in qcbc_oracle.c.
+ */
+#ifndef QCBC_ORACLE_H
+#define QCBC_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-point IMRPhenomD state: every frequency-independent quantity that the
+ * reference computes before its per-bin loop (IMRPhenomD.c:413-523). */
+typedef struct {
+  double m1, m2, chi1, chi2;       /* after the m1 > m2 swap, solar masses */
+  double M, eta, Msec, amp0, fCut, finspin;
+  double chi;                      /* chiPN */
+  double fRD, fDM;
+  /* amplitude (IMRPhenomD_internals.h:739-776) */
+  double gamma1, gamma2, gamma3, fmaxCalc, rho1, rho2, rho3;
+  double f1, f2, f3, v1, v2, v3, d1, d2;
+  double delta[5];
+  /* inspiral amplitude prefactors (IMRPhenomD_internals.h:891-906) */
+  double ai_amp0, ai_23, ai_1, ai_43, ai_53, ai_2, ai_73, ai_83, ai_3;
+  /* phase (IMRPhenomD_internals.h:781-818) */
+  double sigma1, sigma2, sigma3, sigma4, beta1, beta2, beta3;
+  double alpha1, alpha2, alpha3, alpha4, alpha5;
+  double C1Int, C2Int, C1MRD, C2MRD, fPhiIns, fPhiMRD;
+  double pn_v[13], pn_vlogv[13];
+  /* inspiral phase prefactors (IMRPhenomD_internals.h:914-929) */
+  double pi_init, pi_23, pi_13, pi_13L, pi_L, pi_m13, pi_m23, pi_m1, pi_m53;
+  double pi_1, pi_43, pi_53, pi_2;
+  double t0, MfRef, phifRef, phi_precalc;
+} qo_phenomd;
+
+/* status codes: 0 ok, 33 domain (IMRPhenomD_internals.h:675-682) */
+int qo_phenomd_setup(qo_phenomd *p, double m1_msun, double m2_msun, double chi1,
+                     double chi2, double distance, double phi0, double fref);
+void qo_phenomd_eval(const qo_phenomd *p, double f, double *amp, double *phase);
+double qo_amp_ins_ansatz(const qo_phenomd *p, double Mf);
+double qo_amp_int_ansatz(const qo_phenomd *p, double Mf);
+double qo_amp_mrd_ansatz(const qo_phenomd *p, double Mf);
+double qo_damp_ins_ansatz(const qo_phenomd *p, double Mf);
+double qo_damp_mrd_ansatz(const qo_phenomd *p, double Mf);
+double qo_phase_region(const qo_phenomd *p, double Mf, int region); /* 0 ins, 1 int, 2 mrd */
+double qo_final_spin(double eta, double chi1, double chi2);
+double qo_erad(double eta, double chi1, double chi2);
+int qo_qnm(double finspin, double *fring_raw, double *fdamp_raw);
+double qo_fit(int which, double eta, double chi);
+
+/* IMRPhenomDGenerateh22FDAmpPhase restated (IMRPhenomD.c:186-248, :413-563).
+ * m1_SI, m2_SI in kg; distance in m.  Returns 0 or an error code. */
+int qo_h22_ampphase(const double *freq, size_t n, double phi0, double fref,
+                    double m1_SI, double m2_SI, double chi1, double chi2,
+                    double distance, double *amp, double *phase);
+
+double qo_fbegin(const double *params);                          /* Utilities.c:587-629 */
+void qo_times(double alpha, double sindelta, double gmst, double *dtimes); /* Utilities.c:1078-1109 */
+void qo_f_ant(double psi, double alpha, double sindelta, double gmst,
+              double *fplus, double *fcross, int id);             /* Utilities.c:995-1076 */
+double qo_fourier_nwip(const double *a, const double *b, const double *Sn,
+                       int imin, int imax, int N);                 /* Utilities.c:716-736 */
+
+/* fulltemplates (QuickCBC.c:3683-3777): hwave is nifo rows of N doubles,
+ * halfcomplex.  freq has N/2 entries with freq[0]=1/Tobs. */
+int qo_fulltemplates(int nifo, const int *labels, double gmst, double fref,
+                     double **hwave, const double *freq, const double *params, int N);
+
+/* log_likelihood_full (QuickCBC.c:4236-4273) with run-time band edges. */
+double qo_log_likelihood_full(int nifo, const int *labels, double gmst, double fref,
+                              double fmin, double fmax, double **D, const double *params,
+                              const double *freq, double **SN, double *rho, int N, double Tobs,
+                              int *status);
+
+/* OpenMP batch over points, as QuickCBC.c:2063 parallelises over chains.
+ * D and SN are nifo rows; params is B x 11; rho is B x nifo. */
+void qo_loglike_batch(int nifo, const int *labels, double gmst, double fref, double fmin,
+                      double fmax, double **D, double **SN, const double *freq, int N,
+                      double Tobs, const double *params, int B, double *logL, double *rho,
+                      int *status, int nthreads);
+
+/* Flat-array conveniences for the ctypes wrapper. */
+double qo_loglike_flat(int nifo, const int *labels, double gmst, double fref, double fmin,
+                       double fmax, const double *D, const double *SN, int N, double Tobs,
+                       const double *params, double *rho, int *status);
+void qo_loglike_batch_flat(int nifo, const int *labels, double gmst, double fref, double fmin,
+                           double fmax, const double *D, const double *SN, int N, double Tobs,
+                           const double *params, int B, double *logL, double *rho, int *status,
+                           int nthreads);
+int qo_fulltemplates_flat(int nifo, const int *labels, double gmst, double fref,
+                          double *hwave, int N, double Tobs, const double *params);
+int qo_h22_flat(const double *freq, size_t n, double phi0, double fref, double m1_SI,
+                double m2_SI, double chi1, double chi2, double distance, double *amp,
+                double *phase);
+int qo_setup_flat(double m1_msun, double m2_msun, double chi1, double chi2, double distance,
+                  double phi0, double fref, double *out, int nout);
+int qo_spline_eval_flat(int which, const double *x, int n, double *y);
+int qo_waveform_params(const double *params, int N, double Tobs, double fref, double *amp, double *phase);
+double qo_probe(int which, double m1, double m2, double chi1, double chi2, double Mf);
+int qo_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

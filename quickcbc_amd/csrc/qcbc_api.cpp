@@ -1,0 +1,512 @@
+// Host side of libqcbc: resident analyses, the batch C ABI and the QuickCBC
+// drop-in entry points.  Every numerical step runs in the HIP kernels; this
+// file only moves data, sizes launches and maps errors.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/qcbc.h"
+#include "qcbc_api_internal.h"
+#include "qcbc_internal.h"
+
+using namespace qcbc;
+
+namespace qcbc {
+
+thread_local hipError_t t_last_hip = hipSuccess;
+
+// ------------------------------------------------------------ device init
+static std::mutex g_init_mu;
+static int g_ready_dev[64];
+
+int ensure_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return QCBC_ENODEV;
+  if (dev < 0 || dev >= 64) return QCBC_ENODEV;
+  if (g_ready_dev[dev] == 1) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_ready_dev[dev] == 1) return QCBC_OK;
+  if (g_ready_dev[dev] == -1) return QCBC_ENODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strstr(prop.gcnArchName, "gfx950") == nullptr) {
+    g_ready_dev[dev] = -1;
+    return QCBC_ENODEV;
+  }
+  if (launch_qnm_init(nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    g_ready_dev[dev] = -1;
+    return QCBC_ENODEV;
+  }
+  g_ready_dev[dev] = 1;
+  return QCBC_OK;
+}
+
+}  // namespace qcbc
+
+#define HIPCHK(x)                     \
+  do {                                \
+    hipError_t e_ = (x);              \
+    if (e_ != hipSuccess) {           \
+      qcbc::t_last_hip = e_;          \
+      return QCBC_EHIP;               \
+    }                                 \
+  } while (0)
+
+// ------------------------------------------------------------------ ctx
+static int ctx_reserve(qcbc_ctx* c, int B) {
+  if (B <= c->cap_B) return QCBC_OK;
+  int nb = std::max(B, 2 * c->cap_B);
+  hipFree(c->d_params);
+  hipFree(c->d_pts);
+  hipFree(c->d_status);
+  hipFree(c->d_logL);
+  hipFree(c->d_rho);
+  hipFree(c->d_partial);
+  c->d_params = nullptr; c->d_pts = nullptr; c->d_status = nullptr;
+  c->d_logL = nullptr; c->d_rho = nullptr; c->d_partial = nullptr;
+  c->cap_B = 0;
+  HIPCHK(hipMalloc(&c->d_params, sizeof(double) * 11 * nb));
+  HIPCHK(hipMalloc(&c->d_pts, sizeof(Point) * nb));
+  HIPCHK(hipMalloc(&c->d_status, sizeof(int) * nb));
+  HIPCHK(hipMalloc(&c->d_logL, sizeof(double) * nb));
+  HIPCHK(hipMalloc(&c->d_rho, sizeof(double) * 3 * nb));
+  HIPCHK(hipMalloc(&c->d_partial, sizeof(double) * 6 * (size_t)QCBC_MAX_SPLITS * nb));
+  c->cap_B = nb;
+  return QCBC_OK;
+}
+
+static int make_analysis(const qcbc_desc* d, AnalysisConst* an, int* nhalf, int* nb, int* nbp) {
+  if (!d || d->n_det < 1 || d->n_det > 3 || d->N < 4 || (d->N & 1) || !(d->Tobs > 0)) return QCBC_EINVAL;
+  for (int i = 0; i < d->n_det; i++)
+    if (d->labels[i] < 0 || d->labels[i] > 2) return QCBC_EINVAL;
+  std::memset(an, 0, sizeof(*an));
+  an->Tobs = d->Tobs;
+  an->inv_T = 1.0 / d->Tobs;
+  an->sqT = std::sqrt(d->Tobs);
+  an->gmst = d->gmst;
+  an->fref = d->fref;
+  an->fmin0 = 1.0 / d->Tobs;  // freq[0] = 1/Tobs (QuickCBC.c:418)
+  an->n_det = d->n_det;
+  for (int i = 0; i < 3; i++) an->labels[i] = i < d->n_det ? d->labels[i] : 0;
+  an->N = d->N;
+  *nhalf = d->N / 2;
+  int imin = (int)(d->Tobs * d->fmin);  // QuickCBC.c:4250-4252
+  int imax = (int)(d->Tobs * d->fmax);
+  if (imax > d->N / 2) imax = d->N / 2;
+  if (imin < 1) imin = 1;
+  if (imax < imin) imax = imin;
+  an->imin = imin;
+  an->imax = imax;
+  *nb = imax - imin;
+  *nbp = ((*nb + 1023) / 1024) * 1024;
+  if (*nbp == 0) *nbp = 1024;
+  return QCBC_OK;
+}
+
+size_t qcbc_ctx_footprint(const qcbc_desc* d) {
+  AnalysisConst an;
+  int nhalf, nb, nbp;
+  if (make_analysis(d, &an, &nhalf, &nb, &nbp) != QCBC_OK) return 0;
+  size_t nd = d->n_det;
+  size_t bytes = sizeof(double) * (4 + 3 * nd) * nbp + sizeof(float) * 3 * nd * nbp;  // band planes
+  bytes += sizeof(double) * nd * ((size_t)d->N + nhalf);                               // raw D, SN
+  bytes += sizeof(double) * 2 * nhalf + sizeof(double) * nd * d->N;                    // plain-path scratch
+  return bytes;
+}
+
+int qcbc_ctx_create(const qcbc_desc* desc, qcbc_ctx** out) {
+  if (!out) return QCBC_EINVAL;
+  *out = nullptr;
+  int st = ensure_device();
+  if (st) return st;
+  qcbc_ctx* c = new qcbc_ctx();
+  st = make_analysis(desc, &c->an, &c->nhalf, &c->nb, &c->nbp);
+  if (st) {
+    delete c;
+    return st;
+  }
+  c->desc = *desc;
+  hipGetDevice(&c->device);
+  size_t nd = desc->n_det;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->d_p64, sizeof(double) * (4 + 3 * nd) * c->nbp) != hipSuccess ||
+      hipMalloc(&c->d_p32, sizeof(float) * 3 * nd * c->nbp) != hipSuccess ||
+      hipMalloc(&c->d_D, sizeof(double) * nd * desc->N) != hipSuccess ||
+      hipMalloc(&c->d_SN, sizeof(double) * nd * c->nhalf) != hipSuccess ||
+      hipMalloc(&c->d_wref, sizeof(double) * 3) != hipSuccess) {
+    qcbc_ctx_destroy(c);
+    return QCBC_ENOMEM;
+  }
+  for (int p = 0; p < 2; p++) {
+    c->cpb[p] = 4;
+    c->tile[p] = 256;
+    c->splits[p] = 0;
+  }
+  *out = c;
+  return QCBC_OK;
+}
+
+int qcbc_ctx_destroy(qcbc_ctx* c) {
+  if (!c) return QCBC_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev_pairs) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  void* bufs[] = {c->d_p64, c->d_p32, c->d_D, c->d_SN, c->d_wref, c->d_params, c->d_pts, c->d_status,
+                  c->d_logL, c->d_rho, c->d_partial, c->d_amp, c->d_phase, c->d_hw, c->d_plain_partial};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return QCBC_OK;
+}
+
+static int bind_impl(qcbc_ctx* c, const double* const* D, const double* const* SN) {
+  if (!c || !D || !SN) return QCBC_EINVAL;
+  hipSetDevice(c->device);
+  const int nd = c->desc.n_det, N = c->desc.N, nh = c->nhalf;
+  for (int d = 0; d < nd; d++) {
+    if (!D[d] || !SN[d]) return QCBC_EINVAL;
+    HIPCHK(hipMemcpyAsync(c->d_D + (size_t)d * N, D[d], sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_SN + (size_t)d * nh, SN[d], sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
+  }
+  // per-detector FP32 scale: 1/S at the middle of the band
+  double wref[3] = {1.0, 1.0, 1.0};
+  int kmid = (c->an.imin + c->an.imax) / 2;
+  for (int d = 0; d < nd; d++) {
+    double s = SN[d][kmid];
+    if (s > 0 && std::isfinite(s)) wref[d] = 1.0 / s;
+    c->wref[d] = wref[d];
+  }
+  HIPCHK(hipMemcpyAsync(c->d_wref, wref, sizeof(wref), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(launch_bind(c->d_D, c->d_SN, nd, N, c->desc.Tobs, c->an.imin, c->nb, c->nbp, c->d_wref, c->d_p64, c->d_p32,
+                     c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->bound = true;
+  return QCBC_OK;
+}
+
+int qcbc_bind_rows(qcbc_ctx* c, double* const* D, double* const* SN) {
+  if (!c) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return bind_impl(c, (const double* const*)D, (const double* const*)SN);
+}
+
+int qcbc_bind_data(qcbc_ctx* c, const double* D, const double* SN) {
+  if (!c || !D || !SN) return QCBC_EINVAL;
+  const double* rd[3];
+  const double* rs[3];
+  for (int d = 0; d < c->desc.n_det; d++) {
+    rd[d] = D + (size_t)d * c->desc.N;
+    rs[d] = SN + (size_t)d * c->nhalf;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  return bind_impl(c, rd, rs);
+}
+
+// Launch shape: default splits give >= ~2048 blocks.
+int qcbc::effective_splits(const qcbc_ctx* c, int prec, int B) {
+  int s = c->splits[prec];
+  if (s <= 0) {
+    int blocks = (B + c->cpb[prec] - 1) / c->cpb[prec];
+    s = (2048 + blocks - 1) / std::max(blocks, 1);
+  }
+  int maxs = std::max(1, c->nb / 256);
+  s = std::min(s, std::min(maxs, QCBC_MAX_SPLITS));
+  return std::max(s, 1);
+}
+
+static void harvest_events(qcbc_ctx* c) {
+  for (auto& e : c->ev_pairs) {
+    float ms = 0.f;
+    if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
+      c->kernel_ms += ms;
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  c->ev_pairs.clear();
+}
+
+int qcbc::loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
+                       int* d_status) {
+  if (!c->bound) return QCBC_ENOTBOUND;
+  if (prec != QCBC_FP64 && prec != QCBC_MIXED) return QCBC_EINVAL;
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  HIPCHK(launch_setup(d_params, B, c->an, c->d_pts, d_status, c->stream));
+  const int splits = effective_splits(c, prec, B);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+  }
+  HIPCHK(launch_ll(prec, c->desc.n_det, c->d_p64, c->d_p32, c->nbp, c->an.imin, c->nb, c->d_pts, B, c->cpb[prec],
+                   c->tile[prec], splits, c->d_partial, c->stream));
+  if (c->timing) {
+    HIPCHK(hipEventRecord(e1, c->stream));
+    c->ev_pairs.emplace_back(e0, e1);
+    c->launches++;
+    if (c->ev_pairs.size() >= 512) harvest_events(c);
+  }
+  HIPCHK(launch_finalize(c->d_partial, splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED, c->d_wref, d_logL,
+                         d_rho, c->stream));
+  return QCBC_OK;
+}
+
+int qcbc_loglike_device(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
+                        int* d_status) {
+  if (!c || B < 0) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return loglike_core(c, d_params, B, prec, d_logL, d_rho, d_status);
+}
+
+int qcbc_loglike(qcbc_ctx* c, const double* params, int B, int prec, double* logL, double* rho, int* status) {
+  if (!c || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  const int nd = c->desc.n_det;
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, c->stream));
+  st = loglike_core(c, c->d_params, B, prec, c->d_logL, c->d_rho, c->d_status);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(logL, c->d_logL, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
+  if (rho) HIPCHK(hipMemcpyAsync(rho, c->d_rho, sizeof(double) * nd * B, hipMemcpyDeviceToHost, c->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, c->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QCBC_OK;
+}
+
+// ---------------------------------------------------------- plain path
+static int plain_reserve(qcbc_ctx* c) {
+  if (c->d_amp) return QCBC_OK;
+  HIPCHK(hipMalloc(&c->d_amp, sizeof(double) * c->nhalf));
+  HIPCHK(hipMalloc(&c->d_phase, sizeof(double) * c->nhalf));
+  HIPCHK(hipMalloc(&c->d_hw, sizeof(double) * c->desc.n_det * c->desc.N));
+  HIPCHK(hipMalloc(&c->d_plain_partial, sizeof(double) * 6 * QCBC_PLAIN_BLOCKS));
+  return QCBC_OK;
+}
+
+int qcbc_loglike_plain(qcbc_ctx* c, const double* params, int B, double* logL, double* rho, int* status) {
+  if (!c || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  if (!c->bound) return QCBC_ENOTBOUND;
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  st = plain_reserve(c);
+  if (st) return st;
+  const int nd = c->desc.n_det, N = c->desc.N;
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, c->stream));
+  for (int b = 0; b < B; b++) {
+    // one point at a time, as each OpenMP thread of the reference runs one
+    // log_likelihood_full: set-up, waveform loop, projection loop, inner products
+    HIPCHK(launch_setup(c->d_params + 11 * b, 1, c->an, c->d_pts + b, c->d_status + b, c->stream));
+    HIPCHK(launch_waveform(c->d_pts + b, 1, c->nhalf, c->desc.Tobs, c->d_amp, c->d_phase, c->stream));
+    HIPCHK(launch_project(c->d_pts + b, 1, c->d_amp, c->d_phase, nd, N, c->desc.Tobs, c->an.sqT, c->d_hw, c->stream));
+    HIPCHK(launch_nwip(c->d_hw, c->d_D, c->d_SN, 1, nd, N, c->an.imin, c->an.imax, QCBC_PLAIN_BLOCKS,
+                       c->d_plain_partial, c->stream));
+    HIPCHK(launch_finalize(c->d_plain_partial, QCBC_PLAIN_BLOCKS, nd, c->d_pts + b, 1, 0, c->d_wref, c->d_logL + b,
+                           c->d_rho + (size_t)nd * b, c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(logL, c->d_logL, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
+  if (rho) HIPCHK(hipMemcpyAsync(rho, c->d_rho, sizeof(double) * nd * B, hipMemcpyDeviceToHost, c->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, c->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QCBC_OK;
+}
+
+// ------------------------------------------------- waveform / templates
+static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, double* phase, double* hwave,
+                      int* status) {
+  if (!c || B < 0 || (B > 0 && !params)) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  const int nd = c->desc.n_det, N = c->desc.N, nh = c->nhalf;
+  const int chunk = std::max(1, std::min(B, (int)((size_t)1 << 28) / std::max(1, (nd + 2) * N)));
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  double *d_amp = nullptr, *d_ph = nullptr, *d_hw = nullptr;
+  HIPCHK(hipMalloc(&d_amp, sizeof(double) * nh * chunk));
+  HIPCHK(hipMalloc(&d_ph, sizeof(double) * nh * chunk));
+  if (hwave) HIPCHK(hipMalloc(&d_hw, sizeof(double) * nd * N * chunk));
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_setup(c->d_params, B, c->an, c->d_pts, c->d_status, c->stream));
+  for (int b0 = 0; b0 < B; b0 += chunk) {
+    int n = std::min(chunk, B - b0);
+    HIPCHK(launch_waveform(c->d_pts + b0, n, nh, c->desc.Tobs, d_amp, d_ph, c->stream));
+    if (amp) HIPCHK(hipMemcpyAsync(amp + (size_t)b0 * nh, d_amp, sizeof(double) * nh * n, hipMemcpyDeviceToHost, c->stream));
+    if (phase) HIPCHK(hipMemcpyAsync(phase + (size_t)b0 * nh, d_ph, sizeof(double) * nh * n, hipMemcpyDeviceToHost, c->stream));
+    if (hwave) {
+      HIPCHK(launch_project(c->d_pts + b0, n, d_amp, d_ph, nd, N, c->desc.Tobs, c->an.sqT, d_hw, c->stream));
+      HIPCHK(hipMemcpyAsync(hwave + (size_t)b0 * nd * N, d_hw, sizeof(double) * nd * N * n, hipMemcpyDeviceToHost,
+                            c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (status) HIPCHK(hipMemcpyAsync(status, c->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(d_amp);
+  hipFree(d_ph);
+  if (d_hw) hipFree(d_hw);
+  return QCBC_OK;
+}
+
+int qcbc_waveform(qcbc_ctx* c, const double* params, int B, double* amp, double* phase, int* status) {
+  return wave_batch(c, params, B, amp, phase, nullptr, status);
+}
+
+int qcbc_templates(qcbc_ctx* c, const double* params, int B, double* hwave, int* status) {
+  if (!hwave) return QCBC_EINVAL;
+  return wave_batch(c, params, B, nullptr, nullptr, hwave, status);
+}
+
+// ------------------------------------------------------ shapes / timing
+int qcbc_ctx_set_shape(qcbc_ctx* c, int prec, int cpb, int tile, int splits) {
+  if (!c || (prec != 0 && prec != 1)) return QCBC_EINVAL;
+  if (cpb < 1 || cpb > 8 || tile < 64 || (tile & 63) || splits < 0 || splits > QCBC_MAX_SPLITS) return QCBC_EINVAL;
+  if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->cpb[prec] = cpb;
+  c->tile[prec] = tile;
+  c->splits[prec] = splits;
+  return QCBC_OK;
+}
+
+int qcbc_ctx_get_shape(qcbc_ctx* c, int prec, int* cpb, int* tile, int* splits) {
+  if (!c || (prec != 0 && prec != 1)) return QCBC_EINVAL;
+  if (cpb) *cpb = c->cpb[prec];
+  if (tile) *tile = c->tile[prec];
+  if (splits) *splits = c->splits[prec];
+  return QCBC_OK;
+}
+
+int qcbc_ctx_timing(qcbc_ctx* c, int enable, double* kernel_ms, long long* launches) {
+  if (!c) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  harvest_events(c);
+  if (kernel_ms) *kernel_ms = c->kernel_ms;
+  if (launches) *launches = c->launches;
+  if (enable >= 0) {
+    c->timing = enable != 0;
+    c->kernel_ms = 0.0;
+    c->launches = 0;
+  }
+  return QCBC_OK;
+}
+
+int qcbc_ctx_sync(qcbc_ctx* c) {
+  if (!c) return QCBC_EINVAL;
+  hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QCBC_OK;
+}
+
+// Autotuner: times the fused kernel for candidate (chains per block, tile,
+// splits) on a synthetic batch and keeps the fastest.  The batch repeats the
+// centre of a typical prior so every candidate does the same work.
+int qcbc_autotune(qcbc_ctx* c, int prec, int B) {
+  if (!c || (prec != 0 && prec != 1) || B <= 0) return QCBC_EINVAL;
+  if (!c->bound) return QCBC_ENOTBOUND;
+  std::vector<double> prm((size_t)11 * B);
+  // a spread of masses from 1.2+1.2 to 60+50 Msun at tc = Tobs - 2 s
+  for (int b = 0; b < B; b++) {
+    double u = (b + 0.5) / B;
+    double m1 = 1.3 * std::pow(60.0 / 1.3, u), m2 = 0.85 * m1;
+    double mc = std::pow(m1 * m2, 0.6) / std::pow(m1 + m2, 0.2);
+    const double MS = 1.988546954961461467461011951140572744e30, MPC = 3.085677581491367278913937957796471611e22;
+    double* p = &prm[(size_t)11 * b];
+    p[0] = std::log(mc * MS); p[1] = std::log((m1 + m2) * MS); p[2] = 0.1; p[3] = -0.1; p[4] = 1.0;
+    p[5] = c->desc.Tobs - 2.0; p[6] = std::log(400.0 * MPC); p[7] = 1.0; p[8] = 0.2; p[9] = 0.5; p[10] = 0.3;
+  }
+  std::vector<double> logL(B);
+  int cpbs[] = {1, 2, 4, 8};
+  int tiles[] = {128, 256, 512};
+  int best_cpb = c->cpb[prec], best_tile = c->tile[prec], best_split = c->splits[prec];
+  double best = 1e30;
+  int old_timing;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    old_timing = c->timing;
+  }
+  for (int cpb : cpbs)
+    for (int tile : tiles) {
+      if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) continue;
+      int blocks = (B + cpb - 1) / cpb;
+      int base = std::max(1, (2048 + blocks - 1) / blocks);
+      int cands[] = {std::max(1, base / 2), base, base * 2};
+      for (int sp : cands) {
+        if (sp > QCBC_MAX_SPLITS) continue;
+        if (qcbc_ctx_set_shape(c, prec, cpb, tile, sp) != QCBC_OK) continue;
+        qcbc_ctx_timing(c, 1, nullptr, nullptr);
+        int st = QCBC_OK;
+        for (int r = 0; r < 4 && st == QCBC_OK; r++) st = qcbc_loglike(c, prm.data(), B, prec, logL.data(), nullptr, nullptr);
+        double ms = 0;
+        long long n = 0;
+        qcbc_ctx_timing(c, 0, &ms, &n);
+        if (st != QCBC_OK) return st;
+        if (n > 1) {
+          // first launch warms caches; rate of the rest
+          double per = ms / n;
+          if (per < best) {
+            best = per;
+            best_cpb = cpb; best_tile = tile; best_split = sp;
+          }
+        }
+      }
+    }
+  qcbc_ctx_set_shape(c, prec, best_cpb, best_tile, best_split);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->timing = old_timing;
+  return QCBC_OK;
+}
+
+// ------------------------------------------------------------ misc
+const char* qcbc_strerror(int code) {
+  switch (code) {
+    case QCBC_OK: return "success";
+    case QCBC_EINVAL: return "invalid argument";
+    case QCBC_ENOMEM: return "out of device memory";
+    case QCBC_EFAULT: return "invalid pointer";
+    case QCBC_EDOM: return "parameter point outside the IMRPhenomD domain";
+    case QCBC_EHIP: return hipGetErrorString(qcbc::t_last_hip);
+    case QCBC_ENODEV: return "no usable gfx950 device";
+    case QCBC_ENOTBOUND: return "no data bound to this analysis";
+    case QCBC_EBUSY: return "admission refused: memory budget or slots exhausted";
+    case QCBC_ECANCELED: return "request canceled";
+    default: return "unknown error";
+  }
+}
+
+int qcbc_device_info(char* name, size_t len, int* cu_count, double* mem_gb) {
+  int st = ensure_device();
+  if (st) return st;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (name && len) {
+    std::snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (mem_gb) *mem_gb = prop.totalGlobalMem / 1e9;
+  return QCBC_OK;
+}

@@ -1,0 +1,61 @@
+// Host-side state of libqcbc shared by its C++ files.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/qcbc.h"
+#include "qcbc_types.h"
+
+#define QCBC_MAX_SPLITS 64
+#define QCBC_PLAIN_BLOCKS 64
+
+// One resident analysis: band planes (FP64 and scaled FP32), the raw
+// half-complex data and PSD, per-batch scratch, launch shape and timing.
+struct qcbc_ctx {
+  qcbc_desc desc{};
+  qcbc::AnalysisConst an{};
+  int nhalf = 0, nb = 0, nbp = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* d_p64 = nullptr;
+  float* d_p32 = nullptr;
+  double* d_D = nullptr;
+  double* d_SN = nullptr;
+  double* d_wref = nullptr;
+  double wref[3] = {1, 1, 1};
+  bool bound = false;
+  // batch scratch
+  int cap_B = 0;
+  double* d_params = nullptr;
+  qcbc::Point* d_pts = nullptr;
+  int* d_status = nullptr;
+  double* d_logL = nullptr;
+  double* d_rho = nullptr;
+  double* d_partial = nullptr;
+  // plain-path scratch (one point)
+  double* d_amp = nullptr;
+  double* d_phase = nullptr;
+  double* d_hw = nullptr;
+  double* d_plain_partial = nullptr;
+  // fused-kernel launch shape per precision
+  int cpb[2] = {4, 4}, tile[2] = {256, 256}, splits[2] = {0, 0};
+  // kernel timing
+  bool timing = false;
+  double kernel_ms = 0.0;
+  long long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+  std::mutex mu;
+};
+
+namespace qcbc {
+extern thread_local hipError_t t_last_hip;
+int ensure_device();
+int effective_splits(const qcbc_ctx* c, int prec, int B);
+// setup + fused likelihood + finalize on the context stream (caller holds mu)
+int loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
+                 int* d_status);
+}  // namespace qcbc

@@ -1,0 +1,581 @@
+// Device-side physics of the waveform + likelihood path (gfx950).
+//
+// One thread builds the frequency-independent state of one parameter point
+// (qcbc_point_setup); the per-bin evaluators then run with bins on lanes.
+// The per-point record qcbc_point is read wave-uniformly by the likelihood
+// kernels (every lane of a wave works on the same point), so it is an
+// array-of-structs: the compiler turns its loads into scalar loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qcbc_types.h"
+
+#define QCBC_TABLE_STORAGE static __constant__
+#include "qcbc_tables.inc"
+
+namespace qcbc {
+
+// IMRPhenomD_internals.h:102-161
+constexpr double PI = 3.141592653589793238462643383279502884;
+constexpr double PI_4 = 0.785398163397448309615660845819875721;
+constexpr double EULER_GAMMA = 0.577215664901532860606512090082402431;
+constexpr double MSUN_SI = 1.988546954961461467461011951140572744e30;
+constexpr double MTSUN_SI = 4.925491025543575903411922162094833998e-6;
+constexpr double MRSUN_SI = 1.476625061404649406193430731479084713e3;
+constexpr double F_CUT = 0.2;
+constexpr double AMP_JOIN = 0.014;
+constexpr double PHI_JOIN = 0.018;
+// ConstCBC.h:1-12
+constexpr double TSUN = 4.92569043916e-6;
+constexpr double TPI = 6.2831853071795862319959269370884;
+constexpr double CLIGHT = 299792458.0;
+constexpr double H22FAC = 0.31539156525252;
+constexpr double LOG_PI = 1.1447298858494001741434273513530587;
+constexpr double INV_TPI = 0.15915494309189533576888376337251436;
+
+// QNM spline slopes, filled once per device by qcbc_qnm_init_kernel.
+extern __device__ double g_qnm_slope[2][QCBC_QNM_N];
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ double fit11(int w, double eta, double chi) {
+  const double* c = qcbc_fit[w];
+  double xi = -1 + chi;
+  double xi2 = xi * xi;
+  double xi3 = xi2 * xi;
+  double eta2 = eta * eta;
+  return c[0] + c[1] * eta + (c[2] + c[3] * eta + c[4] * eta2) * xi +
+         (c[5] + c[6] * eta + c[7] * eta2) * xi2 + (c[8] + c[9] * eta + c[10] * eta2) * xi3;
+}
+
+// Natural cubic spline through the QNM table in Hermite (slope) form.
+__device__ inline double qnm_spline(int which, double x) {
+  const double* ya = which ? qcbc_qnm_fdamp : qcbc_qnm_fring;
+  int lo = 0, hi = QCBC_QNM_N - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (qcbc_qnm_a[mid] > x) hi = mid; else lo = mid;
+  }
+  double h = qcbc_qnm_a[lo + 1] - qcbc_qnm_a[lo];
+  double t = (x - qcbc_qnm_a[lo]) / h;
+  double m0 = g_qnm_slope[which][lo] * h, m1 = g_qnm_slope[which][lo + 1] * h;
+  double y0 = ya[lo], y1 = ya[lo + 1];
+  // cubic Hermite basis in nested form
+  double dy = y1 - y0;
+  double a = m0;
+  double b = 3.0 * dy - 2.0 * m0 - m1;
+  double c = m0 + m1 - 2.0 * dy;
+  return y0 + t * (a + t * (b + t * c));
+}
+
+// Final spin and radiated energy fits, arXiv:1508.07250 Eqs. 3.6-3.8
+// (IMRPhenomD_internals.c:415-473).
+__device__ inline double final_spin(double eta, double chi1, double chi2) {
+  const double* c = qcbc_finalspin;
+  double Seta = sqrt(1.0 - 4.0 * eta);
+  double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+  double s = m1 * m1 * chi1 + m2 * m2 * chi2;
+  double e2 = eta * eta, e3 = e2 * eta, e4 = e3 * eta;
+  double s2 = s * s, s3 = s2 * s, s4 = s3 * s;
+  return c[0] * eta + c[1] * e2 + c[2] * e3 + c[3] * e4 + (c[4] + c[5] * eta + c[6] * e2) * s +
+         (c[7] * eta + c[8] * e2) * s2 + (c[9] * eta + c[10] * e2) * s3 + (c[11] * eta + c[12] * e2) * s4;
+}
+
+__device__ inline double erad(double eta, double chi1, double chi2) {
+  const double* c = qcbc_erad;
+  double Seta = sqrt(1.0 - 4.0 * eta);
+  double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+  double m1s = m1 * m1, m2s = m2 * m2;
+  double s = (m1s * chi1 + m2s * chi2) / (m1s + m2s);
+  double e2 = eta * eta, e3 = e2 * eta, e4 = e3 * eta;
+  return ((c[0] * eta + c[1] * e2 + c[2] * e3 + c[3] * e4) * (c[4] + (c[5] + c[6] * eta + c[7] * e2) * s)) /
+         (c[8] + (c[9] + c[10] * eta + c[11] * e2) * s);
+}
+
+// ----------------------------------------------------- frequency powers
+// Powers of Mf shared by amplitude and phase (UsefulPowers,
+// IMRPhenomD_internals.c:515-533) plus the inverse powers the phase needs.
+struct Pow {
+  double Mf, s6, third, two_thirds, four_thirds, five_thirds, two, seven_thirds, eight_thirds;
+  double i_s6, i_third, i_two_thirds, i_Mf, i_five_thirds;
+  double logMf, logv;
+};
+
+// Direct form: one pow() per frequency, as the reference.
+__device__ __forceinline__ Pow pow_direct(double Mf) {
+  Pow p;
+  double x = Mf < 0.0 ? 1.0e-10 : Mf;
+  p.Mf = Mf;
+  p.s6 = pow(x, 1 / 6.0);
+  p.third = p.s6 * p.s6;
+  p.two_thirds = x / p.third;
+  p.four_thirds = x * p.third;
+  p.five_thirds = p.four_thirds * p.third;
+  p.two = x * x;
+  p.seven_thirds = p.third * p.two;
+  p.eight_thirds = p.two_thirds * p.two;
+  p.i_s6 = 1.0 / p.s6;
+  p.i_third = 1.0 / p.third;
+  p.i_two_thirds = 1.0 / p.two_thirds;
+  p.i_Mf = 1.0 / Mf;
+  p.i_five_thirds = 1.0 / p.five_thirds;
+  p.logMf = log(Mf);
+  p.logv = log(p.third * 0x1.76ef7e73104b8p+0);  // v = (pi Mf)^(1/3); pow(pi,1/6)^2
+  return p;
+}
+
+// Table form: Mf = Msec f_k with f_k^(1/6), f_k^(-1/6) and log f_k
+// precomputed once per analysis, so no transcendental per bin.
+__device__ __forceinline__ Pow pow_table(const Point& c, double f, double s6f, double is6f,
+                                         double logf) {
+  Pow p;
+  p.Mf = c.Msec * f;
+  p.s6 = c.s6_scale * s6f;
+  p.i_s6 = c.is6_scale * is6f;
+  p.third = p.s6 * p.s6;
+  p.two_thirds = p.third * p.third;
+  p.four_thirds = p.Mf * p.third;
+  p.five_thirds = p.four_thirds * p.third;
+  p.two = p.Mf * p.Mf;
+  p.seven_thirds = p.third * p.two;
+  p.eight_thirds = p.two_thirds * p.two;
+  p.i_third = p.i_s6 * p.i_s6;
+  p.i_two_thirds = p.i_third * p.i_third;
+  p.i_Mf = p.i_third * p.i_two_thirds;
+  p.i_five_thirds = p.i_Mf * p.i_two_thirds;
+  p.logMf = c.logMsec + logf;
+  p.logv = (LOG_PI + p.logMf) * (1.0 / 3.0);
+  return p;
+}
+
+// ------------------------------------------------------ amplitude pieces
+// AmpInsAnsatz (IMRPhenomD_internals.c:601-610, arXiv:1508.07253 Eqs. 29-30)
+__device__ __forceinline__ double amp_ins(const Point& c, const Pow& p) {
+  return 1.0 + p.two_thirds * c.ai_23 + p.Mf * c.ai_1 + p.four_thirds * c.ai_43 + p.five_thirds * c.ai_53 +
+         p.two * c.ai_2 + p.seven_thirds * c.ai_73 + p.eight_thirds * c.ai_83 + p.Mf * p.two * c.ai_3;
+}
+// AmpMRDAnsatz (:757-767, Eq. 19)
+__device__ __forceinline__ double amp_mrd(const Point& c, double f) {
+  double df = f - c.fRD;
+  return exp(-df * c.mrd_rate) * c.mrd_num / (df * df + c.fDMg3 * c.fDMg3);
+}
+// AmpIntAnsatz (:816-821, Eq. 21), evaluated in Newton form.
+__device__ __forceinline__ double amp_int(const Point& c, double f) {
+  double x1 = f - c.col_f1;
+  return c.col_c0 + x1 * (c.col_c1 + x1 * (c.col_c2 + (f - c.col_f2) * (c.col_c3 + (f - c.col_f3) * c.col_c4)));
+}
+// IMRPhenDAmplitude (:1107-1135) without the amp0 scale.
+__device__ __forceinline__ double amplitude(const Point& c, const Pow& p) {
+  double pre = c.amp_pre * p.i_Mf * p.i_s6;
+  double f = p.Mf;
+  if (!(f >= AMP_JOIN)) return pre * amp_ins(c, p);
+  if (f >= c.fmaxCalc) return pre * amp_mrd(c, f);
+  return pre * amp_int(c, f);
+}
+
+// ----------------------------------------------------------- phase pieces
+// PhiInsAnsatzInt (:1399-1425, Eqs. 27-28)
+__device__ __forceinline__ double phi_ins(const Point& c, const Pow& p) {
+  double ph = c.pi_init;
+  ph += c.pi_23 * p.two_thirds;
+  ph += c.pi_13 * p.third;
+  ph += c.pi_13L * p.logv * p.third;
+  ph += c.pi_L * p.logv;
+  ph += c.pi_m13 * p.i_third;
+  ph += c.pi_m23 * p.i_two_thirds;
+  ph += c.pi_m1 * p.i_Mf;
+  ph += c.pi_m53 * p.i_five_thirds;
+  ph += (c.pi_1 * p.Mf + c.pi_43 * p.four_thirds + c.pi_53 * p.five_thirds + c.pi_2 * p.two) * c.inv_eta;
+  return ph;
+}
+// PhiIntAnsatz (:1298-1303, Eq. 16)
+__device__ __forceinline__ double phi_int(const Point& c, double f, double logf) {
+  return c.beta1 * f - c.beta3 / (3. * (f * f * f)) + c.beta2 * logf;
+}
+// PhiMRDAnsatzInt (:1222-1233, Eq. 14)
+__device__ __forceinline__ double phi_mrd(const Point& c, double f) {
+  double sq = sqrt(f);
+  double f075 = sqrt(f * sq);
+  return -(c.alpha2 / f) + (4.0 / 3.0) * (c.alpha3 * f075) + c.alpha1 * f + c.alpha4 * atan((f - c.a5fRD) / c.fDM);
+}
+// IMRPhenDPhase (:1590-1612), then the time/phase shift of IMRPhenomD.c:550.
+__device__ __forceinline__ double phase(const Point& c, const Pow& p) {
+  double f = p.Mf, ph;
+  if (!(f >= PHI_JOIN))
+    ph = phi_ins(c, p);
+  else if (f >= c.fPhiMRD)
+    ph = c.inv_eta * phi_mrd(c, f) + c.C1MRD + c.C2MRD * f;
+  else
+    ph = c.inv_eta * phi_int(c, f, p.logMf) + c.C1Int + c.C2Int * f;
+  return ph - (c.t0 * (f - c.MfRef) + c.phi_precalc);
+}
+
+// ------------------------------------------------ detector geometry
+// F_ant (Utilities.c:995-1076): expanded polynomials, table-driven.
+__device__ inline void antenna(double psi, double alpha, double sindelta, double gmst, int id, double* fp,
+                               double* fc) {
+  double x[9];
+  x[0] = cos(psi);
+  x[1] = sin(psi);
+  x[2] = sqrt(1.0 - sindelta * sindelta);
+  x[3] = sindelta;
+  x[4] = cos(alpha - gmst);
+  x[5] = sin(alpha - gmst);
+  x[6] = x[0] * x[0];
+  x[7] = x[3] * x[3];
+  x[8] = x[4] * x[4];
+  for (int comp = 0; comp < 2; comp++) {
+    int k = 2 * id + comp;
+    double s = 0.0;
+    for (int t = 0; t < qcbc_ant_nterms[k]; t++) {
+      double term = qcbc_ant_coef[k][t];
+      for (int j = 0; j < QCBC_ANT_MAXFAC; j++) {
+        int v = qcbc_ant_fac[k][t][j];
+        if (v < 0) break;
+        term *= x[v];
+      }
+      s = (t == 0) ? term : s + term;
+    }
+    if (comp == 0) *fp = s; else *fc = s;
+  }
+}
+
+// Times (Utilities.c:1078-1109): geocentre-to-detector delay per label.
+__device__ inline double delay(double alpha, double sindelta, double gmst, int id) {
+  double cosd = sqrt(1.0 - sindelta * sindelta);
+  double ca = cos(gmst - alpha), sa = sin(gmst - alpha);
+  const double* v = qcbc_vertex[id];
+  return -(v[0] * cosd * ca + v[1] * cosd * sa + v[2] * sindelta) / CLIGHT;
+}
+
+// fbegin (Utilities.c:587-629): 3PN frequency at the segment start.
+__device__ inline double fbegin(const double* prm) {
+  const double gamma_E = 0.5772156649;
+  double Mc = exp(prm[0]) / MSUN_SI * TSUN;
+  double M = exp(prm[1]) / MSUN_SI * TSUN;
+  double eta = pow(Mc / M, 5.0 / 3.0);
+  if (eta > 0.25) eta = 0.25;
+  double dm = sqrt(1.0 - 4.0 * eta);
+  double m1 = M * (1.0 + dm) / 2.0, m2 = M * (1.0 - dm) / 2.0;
+  double chi1 = prm[2], chi2 = prm[3];
+  double chi = (m1 * chi1 + m2 * chi2) / M;
+  double th = pow(eta * prm[5] / (5.0 * M), -1.0 / 8.0);
+  double th2 = th * th, th3 = th2 * th, th4 = th2 * th2, th5 = th2 * th3, th6 = th3 * th3;
+  double PN1 = (11.0 / 32.0 * eta + 743.0 / 2688.0) * th2;
+  double PN15 = -3.0 * PI / 10.0 * th3 + (1.0 / 160.0) * (113.0 * chi - 38.0 * eta * (chi1 + chi2)) * th3;
+  double PN2 = (1855099.0 / 14450688.0 + 56975.0 / 258048.0 * eta + 371.0 / 2048.0 * eta * eta) * th4 +
+               (1.0 / 14450688.0) * (-3386880.0 * chi * chi + 1512.0 * chi1 * chi2) * th4;
+  double PN25 = -(7729.0 / 21504.0 - 13.0 / 256.0 * eta) * PI * th5;
+  double PN3 = (-720817631400877.0 / 288412611379200.0 + 53.0 / 200.0 * PI * PI + 107.0 / 280.0 * gamma_E +
+                (25302017977.0 / 4161798144.0 - 451.0 / 2048.0 * PI * PI) * eta - 30913.0 / 1835008.0 * eta * eta +
+                235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * log(2.0 * th)) *
+               th6;
+  return th3 / (8.0 * M * PI) * (1.0 + PN1 + PN15 + PN2 + PN25 + PN3);
+}
+
+// --------------------------------------------------------- PN phasing
+// TaylorF2 coefficients (PNPhasing_F2, IMRPhenomD_internals.c:271-365;
+// arXiv:0907.0700 Eq. 3.18, arXiv:1303.7412 Eqs. 3.15-16), already scaled by
+// pfaN, with the 3PN spin-spin term subtracted again (IMRPhenomD.c:493).
+struct PN {
+  double v[8], vl5, vl6;
+};
+
+__device__ inline double pn_ss3(double eta, double m1M, double m2M, double chi1, double chi2) {
+  double ss = (326.75 / 1.12 + 557.5 / 1.8 * eta) * eta * chi1 * chi2;
+  ss += ((4703.5 / 8.4 + 2935. / 6. * m1M - 120. * m1M * m1M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m1M + 125.5 / 3.6 * m1M * m1M)) *
+        m1M * m1M * (chi1 * chi1);
+  ss += ((4703.5 / 8.4 + 2935. / 6. * m2M - 120. * m2M * m2M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m2M + 125.5 / 3.6 * m2M * m2M)) *
+        m2M * m2M * (chi2 * chi2);
+  return ss;
+}
+
+__device__ inline PN pn_phasing(double m1, double m2, double chi1, double chi2) {
+  PN pn;
+  const double mtot = m1 + m2;
+  const double d = (m1 - m2) / mtot;
+  const double eta = m1 * m2 / mtot / mtot;
+  const double m1M = m1 / mtot, m2M = m2 / mtot;
+  const double SL = m1M * m1M * chi1 + m2M * m2M * chi2;
+  const double dSL = d * (m2M * chi2 - m1M * chi1);
+  const double pfaN = 3.0 / (128.0 * eta);
+  const double e2 = eta * eta;
+  double v0 = 1.0, v2, v3, v4, v5, v6, v7, vl5, vl6;
+  v2 = 5.0 * (743.0 / 84.0 + 11.0 * eta) / 9.0;
+  v3 = -16.0 * PI;
+  v4 = 5.0 * (3058.673 / 7.056 + 5429.0 / 7.0 * eta + 617.0 * e2) / 72.0;
+  v5 = 5.0 / 9.0 * (7729.0 / 84.0 - 13.0 * eta) * PI;
+  vl5 = 5.0 / 3.0 * (7729.0 / 84.0 - 13.0 * eta) * PI;
+  v6 = (11583.231236531 / 4.694215680 - 640.0 / 3.0 * PI * PI - 6848.0 / 21.0 * EULER_GAMMA) +
+       eta * (-15737.765635 / 3.048192 + 2255. / 12. * PI * PI) + e2 * 76055.0 / 1728.0 - e2 * eta * 127825.0 / 1296.0;
+  v6 += (-6848.0 / 21.0) * 1.3862943611198906;  // log(4)
+  vl6 = -6848.0 / 21.0;
+  v7 = PI * (77096675.0 / 254016.0 + 378515.0 / 1512.0 * eta - 74045.0 / 756.0 * e2);
+  double sigma = eta * (721.0 / 48.0 * chi1 * chi2 - 247.0 / 48.0 * (chi1 * chi2));
+  sigma += 719.0 / 96.0 * m1M * m1M * chi1 * chi1 + 719.0 / 96.0 * m2M * m2M * chi2 * chi2;
+  sigma -= 233.0 / 96.0 * m1M * m1M * (chi1 * chi1) + 233.0 / 96.0 * m2M * m2M * (chi2 * chi2);
+  double ss3 = pn_ss3(eta, m1M, m2M, chi1, chi2);
+  double gam = (554345.0 / 1134.0 + 110.0 * eta / 9.0) * SL + (13915.0 / 84.0 - 10.0 * eta / 3.0) * dSL;
+  v7 += (-8980424995.0 / 762048.0 + 6586595.0 * eta / 756.0 - 305.0 * e2 / 36.0) * SL -
+        (170978035.0 / 48384.0 - 2876425.0 * eta / 672.0 - 4735.0 * e2 / 144.0) * dSL;
+  v6 += PI * (3760.0 * SL + 1490.0 * dSL) / 3.0 + ss3;
+  v5 += -gam;
+  vl5 += -3.0 * gam;
+  v4 += -10.0 * sigma;
+  v3 += 188.0 * SL / 3.0 + 25.0 * dSL;
+  pn.v[0] = v0 * pfaN;
+  pn.v[1] = 0.0;
+  pn.v[2] = v2 * pfaN;
+  pn.v[3] = v3 * pfaN;
+  pn.v[4] = v4 * pfaN;
+  pn.v[5] = v5 * pfaN;
+  pn.v[6] = v6 * pfaN;
+  pn.v[7] = v7 * pfaN;
+  pn.vl5 = vl5 * pfaN;
+  pn.vl6 = vl6 * pfaN;
+  pn.v[6] -= pn_ss3(m1 * m2 / (mtot * mtot), m1M, m2M, chi1, chi2) * pn.v[0];
+  return pn;
+}
+
+// DPhiInsAnsatzInt (:1461-1501): dPhi_ins/dMf.
+__device__ inline double dphi_ins(const Point& c, const PN& pn, double Mf, double pi_third, double pi_two_thirds,
+                                  double s1, double s2, double s3, double s4) {
+  const double v = cbrt(PI * Mf);
+  const double logv = log(v);
+  const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6, v8 = v * v7;
+  double D = 2.0 * pn.v[7] * v7 + (pn.v[6] + pn.vl6 * (1.0 + logv)) * v6 + pn.vl5 * v5 - pn.v[4] * v4 -
+             2.0 * pn.v[3] * v3 - 3.0 * pn.v[2] * v2 - 5.0 * pn.v[0];
+  D /= v8 * 3.0 / PI;
+  D += (s1 + s2 * v / pi_third + s3 * v2 / pi_two_thirds + (s4 / PI) * v3) * c.inv_eta;
+  return D;
+}
+
+__device__ __forceinline__ double dphi_int(const Point& c, double f) {
+  double f2 = f * f;
+  return (c.beta1 + c.beta3 / (f2 * f2) + c.beta2 / f) * c.inv_eta;
+}
+
+__device__ __forceinline__ double dphi_mrd(const Point& c, double f) {
+  double x = f - c.a5fRD;
+  return (c.alpha1 + c.alpha2 / (f * f) + c.alpha3 / pow(f, 0.25) + c.alpha4 / (c.fDM * (1 + (x * x) / (c.fDM * c.fDM)))) *
+         c.inv_eta;
+}
+
+// d/dMf of AmpInsAnsatz, from its prefactors (the quantity of DAmpInsAnsatz,
+// IMRPhenomD_internals.c:671-702).
+__device__ inline double damp_ins(const Point& c, double Mf) {
+  double t = cbrt(Mf);
+  return (2.0 / 3.0) * c.ai_23 / t + c.ai_1 + (4.0 / 3.0) * c.ai_43 * t + (5.0 / 3.0) * c.ai_53 * t * t +
+         2.0 * c.ai_2 * Mf + (7.0 / 3.0) * c.ai_73 * Mf * t + (8.0 / 3.0) * c.ai_83 * Mf * t * t + 3.0 * c.ai_3 * Mf * Mf;
+}
+
+// DAmpMRDAnsatz (:772-787)
+__device__ inline double damp_mrd(const Point& c, double f, double gamma1, double gamma2, double gamma3) {
+  double df = f - c.fRD;
+  double ex = exp((df * gamma2) / c.fDMg3);
+  double s = df * df + c.fDMg3 * c.fDMg3;
+  return (-2 * c.fDM * df * gamma3 * gamma1) / (ex * (s * s)) - (gamma2 * gamma1) / (ex * s);
+}
+
+// ------------------------------------------------------ per-point set-up
+// IMRPhenomDGenerateh22FDAmpPhase's checks and set-up (IMRPhenomD.c:186-523)
+// for masses in kg and distance in m; fmin0 is the first output frequency.
+__device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, double chi2_in, double distance,
+                                    double phi0, double fref, double fmin0, Point& c) {
+  // IMRPhenomD.c:197-242
+  const double m1_in = m1_SI / MSUN_SI, m2_in = m2_SI / MSUN_SI;
+  if (!(m1_in > 0) || !(m2_in > 0) || !(distance > 0) || !(fref >= 0) || !(fmin0 > 0)) return ST_EDOM;
+  if (!(chi1_in <= 1.0 && chi1_in >= -1.0 && chi2_in <= 1.0 && chi2_in >= -1.0)) return ST_EDOM;
+  if (F_CUT / ((m1_in + m2_in) * MTSUN_SI) <= fmin0) return ST_EDOM;
+  // IMRPhenomD.c:429-439: m1 > m2
+  double m1, m2, chi1, chi2;
+  if (m1_in > m2_in) { m1 = m1_in; m2 = m2_in; chi1 = chi1_in; chi2 = chi2_in; }
+  else { m1 = m2_in; m2 = m1_in; chi1 = chi2_in; chi2 = chi1_in; }
+  const double M = m1 + m2;
+  const double eta = m1 * m2 / (M * M);
+  if (eta > 0.25 || eta < 0.0) return ST_EDOM;
+  c.eta = eta;
+  c.inv_eta = 1.0 / eta;
+  c.Msec = M * MTSUN_SI;
+  c.amp0 = M * MRSUN_SI * M * MTSUN_SI / distance;
+  c.fCut = F_CUT / c.Msec;
+  c.s6_scale = pow(c.Msec, 1.0 / 6.0);
+  c.is6_scale = 1.0 / c.s6_scale;
+  c.logMsec = log(c.Msec);
+  const double pi_s6 = pow(PI, 1.0 / 6.0);
+  const double pi_third = pi_s6 * pi_s6;
+  const double pi_two_thirds = PI / pi_third;
+  const double pi_four_thirds = PI * pi_third;
+  const double pi_five_thirds = pi_four_thirds * pi_third;
+  const double pi2 = PI * PI;
+
+  // final state (IMRPhenomD_internals.c:415-513)
+  const double finspin = final_spin(eta, chi1, chi2);
+  c.finspin = finspin;
+  if (!(finspin <= 1.0 && finspin >= -1.0)) return ST_EDOM;
+  const double er = 1.0 - erad(eta, chi1, chi2);
+  c.fRD = qnm_spline(0, finspin) / er;
+  c.fDM = qnm_spline(1, finspin) / er;
+
+  // amplitude coefficients (:1072-1100)
+  const double delta_m = sqrt(1.0 - 4.0 * eta);
+  const double chi = ((chi1 + chi2) / 2.0) * (1.0 - eta * 76.0 / 113.0) + delta_m * ((chi1 - chi2) / 2.0);  // chiPN
+  const double g1 = fit11(QCBC_FIT_GAMMA1, eta, chi), g2 = fit11(QCBC_FIT_GAMMA2, eta, chi),
+               g3 = fit11(QCBC_FIT_GAMMA3, eta, chi);
+  c.fDMg3 = c.fDM * g3;
+  c.mrd_num = c.fDMg3 * g1;
+  c.mrd_rate = g2 / c.fDMg3;
+  c.fmaxCalc = (g2 <= 1) ? fabs(c.fRD + (c.fDM * (-1 + sqrt(1 - g2 * g2)) * g3) / g2)
+                         : fabs(c.fRD + (c.fDM * (-1) * g3) / g2);
+  const double rho1 = fit11(QCBC_FIT_RHO1, eta, chi), rho2 = fit11(QCBC_FIT_RHO2, eta, chi),
+               rho3 = fit11(QCBC_FIT_RHO3, eta, chi);
+  // init_amp_ins_prefactors (:612-665)
+  {
+    const double c12 = chi1 * chi1, c22 = chi2 * chi2, e2 = eta * eta, e3 = eta * e2, S = delta_m;
+    c.amp_pre = (sqrt(2.0 / 3.0) * sqrt(eta)) / pi_s6;
+    c.ai_23 = ((-969 + 1804 * eta) * pi_two_thirds) / 672.;
+    c.ai_1 = ((chi1 * (81 * (1 + S) - 44 * eta) + chi2 * (81 - 81 * S - 44 * eta)) * PI) / 48.;
+    c.ai_43 = ((-27312085.0 - 10287648 * c22 - 10287648 * c12 * (1 + S) + 10287648 * c22 * S +
+                24 * (-1975055 + 857304 * c12 - 994896 * chi1 * chi2 + 857304 * c22) * eta + 35371056 * e2) *
+               pi_four_thirds) / 8.128512e6;
+    c.ai_53 = (pi_five_thirds * (chi2 * (-285197 * (-1 + S) + 4 * (-91902 + 1579 * S) * eta - 35632 * e2) +
+                                 chi1 * (285197 * (1 + S) - 4 * (91902 + 1579 * S) * eta - 35632 * e2) +
+                                 42840 * (-1.0 + 4 * eta) * PI)) / 32256.;
+    c.ai_2 = -(pi2 * (-336 * (-3248849057.0 + 2943675504 * c12 - 3339284256 * chi1 * chi2 + 2943675504 * c22) * e2 -
+                      324322727232 * e3 -
+                      7 * (-177520268561 + 107414046432 * c22 + 107414046432 * c12 * (1 + S) - 107414046432 * c22 * S +
+                           11087290368 * (chi1 + chi2 + chi1 * S - chi2 * S) * PI) +
+                      12 * eta * (-545384828789 - 176491177632 * chi1 * chi2 + 202603761360 * c22 +
+                                  77616 * c12 * (2610335 + 995766 * S) - 77287373856 * c22 * S +
+                                  5841690624 * (chi1 + chi2) * PI + 21384760320 * pi2))) / 6.0085960704e10;
+    c.ai_73 = rho1;
+    c.ai_83 = rho2;
+    c.ai_3 = rho3;
+  }
+  // collocation of the intermediate amplitude (:1005-1063): the quartic with
+  // values v1, v2, v3 at f1, f2, f3 and slopes d1 at f1, d2 at f3, built by
+  // Hermite divided differences on the nodes (f1, f1, f2, f3, f3).
+  {
+    const double f1 = AMP_JOIN, f3 = c.fmaxCalc, f2 = f1 + (f3 - f1) / 2.0;
+    c.col_f1 = f1; c.col_f2 = f2; c.col_f3 = f3;
+    Pow p1 = pow_direct(f1);
+    const double v1 = amp_ins(c, p1);
+    const double d1 = damp_ins(c, f1);
+    const double v3 = amp_mrd(c, f3);
+    const double d2 = damp_mrd(c, f3, g1, g2, g3);
+    const double v2 = fit11(QCBC_FIT_V2, eta, chi);
+    if (f3 > f1) {
+      double a01 = d1, a12 = (v2 - v1) / (f2 - f1), a23 = (v3 - v2) / (f3 - f2), a34 = d2;
+      double b02 = (a12 - a01) / (f2 - f1), b13 = (a23 - a12) / (f3 - f1), b24 = (a34 - a23) / (f3 - f2);
+      double c03 = (b13 - b02) / (f3 - f1), c14 = (b24 - b13) / (f3 - f1);
+      double d04 = (c14 - c03) / (f3 - f1);
+      c.col_c0 = v1; c.col_c1 = a01; c.col_c2 = b02; c.col_c3 = c03; c.col_c4 = d04;
+    } else {  // the intermediate region is empty
+      c.col_c0 = c.col_c1 = c.col_c2 = c.col_c3 = c.col_c4 = 0.0;
+    }
+  }
+  // phase coefficients (:1509-1540)
+  const double sg1 = fit11(QCBC_FIT_SIGMA1, eta, chi), sg2 = fit11(QCBC_FIT_SIGMA2, eta, chi),
+               sg3 = fit11(QCBC_FIT_SIGMA3, eta, chi), sg4 = fit11(QCBC_FIT_SIGMA4, eta, chi);
+  c.beta1 = fit11(QCBC_FIT_BETA1, eta, chi);
+  c.beta2 = fit11(QCBC_FIT_BETA2, eta, chi);
+  c.beta3 = fit11(QCBC_FIT_BETA3, eta, chi);
+  c.alpha1 = fit11(QCBC_FIT_ALPHA1, eta, chi);
+  c.alpha2 = fit11(QCBC_FIT_ALPHA2, eta, chi);
+  c.alpha3 = fit11(QCBC_FIT_ALPHA3, eta, chi);
+  c.alpha4 = fit11(QCBC_FIT_ALPHA4, eta, chi);
+  c.a5fRD = fit11(QCBC_FIT_ALPHA5, eta, chi) * c.fRD;
+  const PN pn = pn_phasing(m1, m2, chi1, chi2);
+  // init_phi_ins_prefactors (:1427-1456)
+  c.pi_init = pn.v[5] - PI_4;
+  c.pi_23 = pn.v[7] * pi_two_thirds;
+  c.pi_13 = pn.v[6] * pi_third;
+  c.pi_13L = pn.vl6 * pi_third;
+  c.pi_L = pn.vl5;
+  c.pi_m13 = pn.v[4] / pi_third;
+  c.pi_m23 = pn.v[3] / pi_two_thirds;
+  c.pi_m1 = pn.v[2] / PI;
+  c.pi_m53 = pn.v[0] / pi_five_thirds;
+  c.pi_1 = sg1;
+  c.pi_43 = sg2 * 3.0 / 4.0;
+  c.pi_53 = sg3 * 3.0 / 5.0;
+  c.pi_2 = sg4 / 2.0;
+  // C1 connection (:1547-1584)
+  {
+    const double fj = PHI_JOIN;
+    c.C2Int = dphi_ins(c, pn, fj, pi_third, pi_two_thirds, sg1, sg2, sg3, sg4) - dphi_int(c, fj);
+    Pow pj = pow_direct(fj);
+    c.C1Int = phi_ins(c, pj) - c.inv_eta * phi_int(c, fj, log(fj)) - c.C2Int * fj;
+    c.fPhiMRD = 0.5 * c.fRD;
+    const double fm = c.fPhiMRD;
+    const double phiT = c.inv_eta * phi_int(c, fm, log(fm)) + c.C1Int + c.C2Int * fm;
+    const double dphiT = c.C2Int + dphi_int(c, fm);
+    c.C2MRD = dphiT - dphi_mrd(c, fm);
+    c.C1MRD = phiT - c.inv_eta * phi_mrd(c, fm) - c.C2MRD * fm;
+  }
+  // time shift and reference phase (IMRPhenomD.c:504-521)
+  c.t0 = dphi_mrd(c, c.fmaxCalc);
+  c.MfRef = c.Msec * fref;
+  c.phi_precalc = 0.0;
+  {
+    Pow pr = pow_direct(c.MfRef);
+    c.phi_precalc = 2. * phi0 + phase(c, pr);  // phase() subtracts t0*0 + 0 here
+  }
+  return ST_OK;
+}
+
+// fulltemplates' parameter handling and projection constants
+// (QuickCBC.c:3700-3750) for one 11-entry parameter vector.
+__device__ inline int point_setup(const double* prm, const AnalysisConst& an, Point& c) {
+  c.status = ST_OK;
+  c.kstart = c.kend = an.imin;
+  c.fs = fbegin(prm);
+  const double mc = exp(prm[0]), mt = exp(prm[1]);
+  const double eta_in = pow(mc / mt, 5.0 / 3.0);
+  const double dm = eta_in > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta_in);
+  const double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  const double ts = an.Tobs - prm[5];
+  int st = phenomd_setup(m1_SI, m2_SI, prm[2], prm[3], exp(prm[6]), 0.5 * prm[4], an.fref, an.fmin0, c);
+  if (st != ST_OK) {
+    c.status = st;
+    return st;
+  }
+  // projection (QuickCBC.c:3734-3750)
+  {
+    const double alpha = prm[7], sindelta = prm[8], psi = prm[9], ciota = prm[10];
+    const double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
+    for (int d = 0; d < 3; d++) {
+      c.Fs[d] = c.tshift[d] = c.lambda[d] = c.Ascale[d] = 0.0;
+      if (d >= an.n_det) continue;
+      int id = an.labels[d];
+      double Fp, Fc;
+      antenna(psi, alpha, sindelta, an.gmst, id, &Fp, &Fc);
+      double Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
+      double lam = atan2(Ac * Fc, Ap * Fp);
+      if (lam < 0.0) lam += TPI;
+      c.Fs[d] = Fs;
+      c.lambda[d] = lam;
+      c.tshift[d] = ts - delay(alpha, sindelta, an.gmst, id);
+      c.Ascale[d] = Fs * H22FAC * c.amp0 / an.sqT;
+    }
+  }
+  // bins that contribute: f_k > fs (QuickCBC.c:3758) and f_k <= fCut
+  // (IMRPhenomD.c:530), inside the band; f_k = k / Tobs as QuickCBC.c:419.
+  {
+    int lo = an.imin, hi = an.imax;
+    double kf = floor(c.fs * an.Tobs) - 1.0;
+    if (kf != kf) kf = (double)hi;  // fs is NaN: no bin passes f > fs
+    int k0 = kf > (double)hi ? hi : (kf > (double)lo ? (int)kf : lo);
+    while (k0 < hi && !((double)k0 / an.Tobs > c.fs)) k0++;
+    double kc = floor(c.fCut * an.Tobs) + 2.0;
+    int k1 = kc < (double)hi ? (int)kc : hi;
+    if (k1 < k0) k1 = k0;
+    while (k1 > k0 && (double)(k1 - 1) / an.Tobs > c.fCut) k1--;
+    c.kstart = k0;
+    c.kend = k1;
+  }
+  return ST_OK;
+}
+
+}  // namespace qcbc

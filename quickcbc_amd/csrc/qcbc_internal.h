@@ -1,0 +1,47 @@
+// Launch wrappers shared by the kernels (qcbc_kernels.hip) and the host side
+// of libqcbc (qcbc_api.cpp).  Host-safe: no device code in here.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "qcbc_types.h"
+
+namespace qcbc {
+
+hipError_t launch_qnm_init(hipStream_t s);
+hipError_t launch_setup(const double* d_params, int B, const AnalysisConst& an, Point* d_pts, int* d_status,
+                        hipStream_t s);
+hipError_t launch_setup_h22(double m1, double m2, double c1, double c2, double dist, double phi0, double fref,
+                            double fmin0, Point* d_pts, int* d_status, hipStream_t s);
+hipError_t launch_bind(const double* D, const double* SN, int nd, int N, double Tobs, int imin, int nb, int nbp,
+                       const double* wref, double* p64, float* p32, hipStream_t s);
+size_t ll_lds_bytes(int precision, int nd, int tile);
+hipError_t launch_ll(int precision, int nd, const double* p64, const float* p32, int nbp, int imin, int nb,
+                     const Point* pts, int B, int cpb, int tile, int splits, double* partial, hipStream_t s);
+hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mixed,
+                           const double* wref, double* logL, double* rho, hipStream_t s);
+hipError_t launch_waveform(const Point* pts, int B, int nhalf, double Tobs, double* amp, double* phs, hipStream_t s);
+hipError_t launch_waveform_freqs(const Point* pts, const double* freq, int n, double* amp, double* phs,
+                                 hipStream_t s);
+hipError_t launch_project(const Point* pts, int B, const double* amp, const double* phs, int nd, int N, double Tobs,
+                          double sqT, double* hw, hipStream_t s);
+hipError_t launch_nwip(const double* hw, const double* D, const double* SN, int B, int nd, int N, int imin, int imax,
+                       int nblk, double* partial, hipStream_t s);
+hipError_t launch_nwip_pair(const double* a, const double* b, const double* Sn, int imin, int imax, int N, int nblk,
+                            double* partial, double* out, hipStream_t s);
+hipError_t launch_geometry(int what, const double* in, int n, double* out, hipStream_t s);
+
+}  // namespace qcbc
+
+namespace qcbc {
+hipError_t launch_pt_propose(const double* x, double* y, const double* sigma, int n_chains, int P, double ladder,
+                             uint64_t seed, const unsigned long long* step, hipStream_t s);
+hipError_t launch_pt_accept(double* x, double* L, const double* y, const double* Ly, const int* st_y, int n_chains,
+                            int P, double ladder, uint64_t seed, const unsigned long long* step,
+                            unsigned long long* acc, hipStream_t s);
+hipError_t launch_pt_swap(double* x, double* L, int n_temps, int n_chains, double ladder, uint64_t seed,
+                          unsigned long long* step, unsigned long long* swaps, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
+}  // namespace qcbc

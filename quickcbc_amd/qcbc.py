@@ -1,0 +1,385 @@
+"""ctypes binding of libqcbc (include/qcbc.h).
+
+The library is the product: every call below goes through its C ABI and runs
+on the GPU.  There is no CPU fallback; if libqcbc.so is missing or no gfx950
+device is usable, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqcbc.so")
+
+FP64, MIXED = 0, 1
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+ERRORS = {0: "QCBC_OK", 1: "QCBC_EINVAL", 12: "QCBC_ENOMEM", 14: "QCBC_EFAULT", 33: "QCBC_EDOM",
+          100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY", 104: "QCBC_ECANCELED"}
+
+
+class QcbcError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__("%s: %s (%s)" % (what, ERRORS.get(code, code), strerror(code)))
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("n_det", ctypes.c_int), ("labels", ctypes.c_int * 3), ("gmst", ctypes.c_double),
+                ("N", ctypes.c_int), ("Tobs", ctypes.c_double), ("fmin", ctypes.c_double),
+                ("fmax", ctypes.c_double), ("fref", ctypes.c_double)]
+
+
+class Net(ctypes.Structure):
+    """struct Net of QuickCBC.h:13-25."""
+    _fields_ = [("Nifo", ctypes.c_int), ("Tobs", ctypes.c_double), ("tmax", ctypes.c_double),
+                ("tmin", ctypes.c_double), ("GMST", ctypes.c_double), ("labels", _ip),
+                ("tds", _dp), ("delays", ctypes.POINTER(_dp)), ("lstart", _ip), ("lstop", _ip)]
+
+
+class RealVector(ctypes.Structure):
+    _fields_ = [("data", _dp), ("length", ctypes.c_size_t)]
+
+
+class AmpPhaseFDWaveform(ctypes.Structure):
+    _fields_ = [("freq", _dp), ("amp", _dp), ("phase", _dp), ("length", ctypes.c_size_t)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqcbc.so is not built (run python -m quickcbc_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.qcbc_strerror.restype = ctypes.c_char_p
+        L.qcbc_strerror.argtypes = [ctypes.c_int]
+        L.qcbc_ctx_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(_vp)]
+        L.qcbc_ctx_destroy.argtypes = [_vp]
+        L.qcbc_ctx_footprint.restype = ctypes.c_size_t
+        L.qcbc_ctx_footprint.argtypes = [ctypes.POINTER(Desc)]
+        L.qcbc_bind_data.argtypes = [_vp, _dp, _dp]
+        L.qcbc_loglike.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+        L.qcbc_loglike_device.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+        L.qcbc_loglike_plain.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
+        L.qcbc_waveform.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
+        L.qcbc_templates.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.qcbc_ctx_set_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qcbc_ctx_get_shape.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
+        L.qcbc_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.qcbc_ctx_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_ctx_sync.argtypes = [_vp]
+        L.qcbc_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, _ip, _dp]
+        # drop-ins
+        L.log_likelihood_full.restype = ctypes.c_double
+        L.log_likelihood_full.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(_dp), _dp, ctypes.POINTER(RealVector),
+                                          ctypes.POINTER(_dp), _dp, ctypes.c_int, ctypes.c_double]
+        L.fulltemplates.restype = None
+        L.fulltemplates.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(_dp), ctypes.POINTER(RealVector), _dp,
+                                    ctypes.c_int]
+        L.IMRPhenomDGenerateh22FDAmpPhase.restype = ctypes.c_int
+        L.IMRPhenomDGenerateh22FDAmpPhase.argtypes = [ctypes.POINTER(ctypes.POINTER(AmpPhaseFDWaveform)),
+                                                      ctypes.POINTER(RealVector)] + [ctypes.c_double] * 7
+        L.DestroyAmpPhaseFDWaveform.argtypes = [ctypes.POINTER(AmpPhaseFDWaveform)]
+        L.fourier_nwip.restype = ctypes.c_double
+        L.fourier_nwip.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.F_ant.restype = None
+        L.F_ant.argtypes = [ctypes.c_double] * 4 + [_dp, _dp, ctypes.c_int]
+        L.Times.restype = None
+        L.Times.argtypes = [ctypes.c_double] * 3 + [_dp]
+        L.fbegin.restype = ctypes.c_double
+        L.fbegin.argtypes = [_dp]
+        L.qcbc_set_band.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.qcbc_set_fref.argtypes = [ctypes.c_double]
+        L.qcbc_set_exit_on_error.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def strerror(code):
+    try:
+        return lib().qcbc_strerror(code).decode()
+    except Exception:  # noqa: BLE001 - error text only
+        return "?"
+
+
+def _check(code, what):
+    if code != 0:
+        raise QcbcError(code, what)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int(0)
+    mem = ctypes.c_double(0)
+    _check(lib().qcbc_device_info(name, 256, ctypes.byref(cu), ctypes.byref(mem)), "qcbc_device_info")
+    return name.value.decode(), cu.value, mem.value
+
+
+def make_desc(an):
+    d = Desc()
+    d.n_det = an.nifo
+    for i, l in enumerate(an.labels):
+        d.labels[i] = l
+    d.gmst = an.gmst
+    d.N = an.N
+    d.Tobs = an.Tobs
+    d.fmin = an.fmin
+    d.fmax = an.fmax
+    d.fref = an.fref
+    return d
+
+
+class Context:
+    """One resident analysis on the GPU (qcbc_ctx)."""
+
+    def __init__(self, an, D=None, SN=None):
+        self.an = an
+        self.desc = make_desc(an)
+        h = _vp()
+        _check(lib().qcbc_ctx_create(ctypes.byref(self.desc), ctypes.byref(h)), "qcbc_ctx_create")
+        self.h = h
+        if D is not None:
+            self.bind(D, SN if SN is not None else an.SN())
+
+    def close(self):
+        if self.h:
+            lib().qcbc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def bind(self, D, SN):
+        D = np.ascontiguousarray(D, np.float64)
+        SN = np.ascontiguousarray(SN, np.float64)
+        assert D.shape == (self.an.nifo, self.an.N) and SN.shape == (self.an.nifo, self.an.N // 2)
+        self._D, self._SN = D, SN
+        _check(lib().qcbc_bind_data(self.h, _d(D), _d(SN)), "qcbc_bind_data")
+
+    def loglike(self, params, precision=FP64):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        logL = np.zeros(B)
+        rho = np.zeros((B, self.an.nifo))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_loglike(self.h, _d(p), B, precision, _d(logL), _d(rho), _i(st)), "qcbc_loglike")
+        return logL, rho, st
+
+    def loglike_plain(self, params):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        logL = np.zeros(B)
+        rho = np.zeros((B, self.an.nifo))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_loglike_plain(self.h, _d(p), B, _d(logL), _d(rho), _i(st)), "qcbc_loglike_plain")
+        return logL, rho, st
+
+    def loglike_device(self, d_params, B, precision, d_logL, d_rho, d_status=0):
+        _check(lib().qcbc_loglike_device(self.h, d_params, B, precision, d_logL, d_rho, d_status),
+               "qcbc_loglike_device")
+
+    def waveform(self, params):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        nh = self.an.N // 2
+        amp = np.zeros((B, nh))
+        ph = np.zeros((B, nh))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_waveform(self.h, _d(p), B, _d(amp), _d(ph), _i(st)), "qcbc_waveform")
+        return amp, ph, st
+
+    def templates(self, params):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        hw = np.zeros((B, self.an.nifo, self.an.N))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_templates(self.h, _d(p), B, _d(hw), _i(st)), "qcbc_templates")
+        return hw, st
+
+    def set_shape(self, precision, cpb, tile, splits):
+        _check(lib().qcbc_ctx_set_shape(self.h, precision, cpb, tile, splits), "qcbc_ctx_set_shape")
+
+    def get_shape(self, precision):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().qcbc_ctx_get_shape(self.h, precision, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+               "qcbc_ctx_get_shape")
+        return a.value, b.value, c.value
+
+    def autotune(self, precision, B):
+        _check(lib().qcbc_autotune(self.h, precision, B), "qcbc_autotune")
+        return self.get_shape(precision)
+
+    def timing(self, enable):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_longlong(0)
+        _check(lib().qcbc_ctx_timing(self.h, enable, ctypes.byref(ms), ctypes.byref(n)), "qcbc_ctx_timing")
+        return ms.value, n.value
+
+    def sync(self):
+        _check(lib().qcbc_ctx_sync(self.h), "qcbc_ctx_sync")
+
+
+# ------------------------------------------------------------ drop-ins
+class RowMatrix:
+    """double** rows as QuickCBC's double_matrix builds them (Utilities.c:2981-2992)."""
+
+    def __init__(self, arr):
+        self.arr = np.ascontiguousarray(arr, np.float64)
+        n = self.arr.shape[0]
+        self.ptrs = (_dp * n)(*[self.arr[i].ctypes.data_as(_dp) for i in range(n)])
+
+    @property
+    def p(self):
+        return ctypes.cast(self.ptrs, ctypes.POINTER(_dp))
+
+
+def make_net(an):
+    labels = (ctypes.c_int * 3)(*an.labels, *([0] * (3 - an.nifo)))
+    net = Net()
+    net.Nifo = an.nifo
+    net.Tobs = an.Tobs
+    net.GMST = an.gmst
+    net.labels = ctypes.cast(labels, _ip)
+    net._keep = labels
+    return net
+
+
+def make_freq(an):
+    f = np.ascontiguousarray(an.freq())
+    rv = RealVector()
+    rv.data = _d(f)
+    rv.length = f.size
+    rv._keep = f
+    return rv
+
+
+def dropin_log_likelihood_full(an, Drows, SNrows, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    rho = np.zeros(an.nifo)
+    p = np.ascontiguousarray(params, np.float64)
+    v = lib().log_likelihood_full(ctypes.byref(net), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, _d(rho),
+                                  an.N, an.Tobs)
+    return v, rho
+
+
+def dropin_fulltemplates(an, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    out = RowMatrix(np.zeros((an.nifo, an.N)))
+    p = np.ascontiguousarray(params, np.float64)
+    lib().fulltemplates(ctypes.byref(net), out.p, ctypes.byref(fr), _d(p), an.N)
+    return out.arr
+
+
+def dropin_h22(freq, phi0, fref, m1_SI, m2_SI, chi1, chi2, distance):
+    f = np.ascontiguousarray(freq, np.float64)
+    rv = RealVector()
+    rv.data = _d(f)
+    rv.length = f.size
+    h = ctypes.POINTER(AmpPhaseFDWaveform)()
+    st = lib().IMRPhenomDGenerateh22FDAmpPhase(ctypes.byref(h), ctypes.byref(rv), phi0, fref, m1_SI, m2_SI,
+                                                chi1, chi2, distance)
+    if st != 0:
+        return st, None, None
+    n = h.contents.length
+    amp = np.ctypeslib.as_array(h.contents.amp, shape=(n,)).copy()
+    ph = np.ctypeslib.as_array(h.contents.phase, shape=(n,)).copy()
+    lib().DestroyAmpPhaseFDWaveform(h)
+    return st, amp, ph
+
+
+def dropin_fourier_nwip(a, b, Sn, imin, imax, N):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    Sn = np.ascontiguousarray(Sn, np.float64)
+    return lib().fourier_nwip(_d(a), _d(b), _d(Sn), imin, imax, N)
+
+
+def dropin_f_ant(psi, alpha, sindelta, gmst, det):
+    fp = ctypes.c_double(0)
+    fc = ctypes.c_double(0)
+    lib().F_ant(psi, alpha, sindelta, gmst, ctypes.byref(fp), ctypes.byref(fc), det)
+    return fp.value, fc.value
+
+
+def dropin_times(alpha, sindelta, gmst):
+    out = np.zeros(3)
+    lib().Times(alpha, sindelta, gmst, _d(out))
+    return out
+
+
+def dropin_fbegin(params):
+    p = np.ascontiguousarray(params, np.float64)
+    return lib().fbegin(_d(p))
+
+
+class PT:
+    """Resident parallel-tempered ensemble (qcbc_pt)."""
+
+    def __init__(self, ctx, n_temps, n_chains, ladder, init, sigma, seed=1, precision=FP64):
+        L = lib()
+        L.qcbc_pt_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_uint64,
+                                     ctypes.c_int, ctypes.POINTER(_vp)]
+        L.qcbc_pt_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.qcbc_pt_get.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_pt_latencies.argtypes = [_vp, _dp, ctypes.c_int, _ip]
+        L.qcbc_pt_set_step_timing.argtypes = [_vp, ctypes.c_int]
+        L.qcbc_pt_destroy.argtypes = [_vp]
+        self.ctx = ctx
+        self.P = n_temps * n_chains
+        init = np.ascontiguousarray(init, np.float64).reshape(self.P, 11)
+        sigma = np.ascontiguousarray(sigma, np.float64).reshape(11)
+        h = _vp()
+        _check(L.qcbc_pt_create(ctx.h, n_temps, n_chains, ladder, _d(init), _d(sigma), seed, precision,
+                                ctypes.byref(h)), "qcbc_pt_create")
+        self.h = h
+
+    def run(self, nsteps, graph=False):
+        _check(lib().qcbc_pt_run(self.h, nsteps, 1 if graph else 0), "qcbc_pt_run")
+
+    def step_timing(self, on):
+        _check(lib().qcbc_pt_set_step_timing(self.h, 1 if on else 0), "qcbc_pt_set_step_timing")
+
+    def latencies(self):
+        n = ctypes.c_int(0)
+        _check(lib().qcbc_pt_latencies(self.h, None, 0, ctypes.byref(n)), "qcbc_pt_latencies")
+        out = np.zeros(max(n.value, 1))
+        _check(lib().qcbc_pt_latencies(self.h, _d(out), n.value, ctypes.byref(n)), "qcbc_pt_latencies")
+        return out[:n.value]
+
+    def get(self):
+        x = np.zeros((self.P, 11))
+        L = np.zeros(self.P)
+        acc = (ctypes.c_longlong * self.P)()
+        sw = ctypes.c_longlong(0)
+        _check(lib().qcbc_pt_get(self.h, _d(x), _d(L), acc, ctypes.byref(sw)), "qcbc_pt_get")
+        return x, L, np.array(acc[:]), sw.value
+
+    def close(self):
+        if self.h:
+            lib().qcbc_pt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass

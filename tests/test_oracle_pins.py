@@ -1,0 +1,306 @@
+"""Pins of the CPU oracle against what the mathematics and physics fix.
+
+The reference publishes no test vectors and cannot be built here (its
+IMRPhenomD needs GSL), so each part of the oracle is checked against an
+independent statement: closed forms, a second implementation in scipy,
+continuity and derivative identities, symmetry invariants and physical
+bounds.  Each check is chosen so that a plausible mistake (a dropped term, a
+wrong sign or index, a transposed operand) fails it.
+"""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+import scipy.interpolate
+
+from tests import _data
+from quickcbc_amd import workloads as W
+
+# ----------------------------------------------------------------- PN phasing
+
+
+def test_pn_nonspinning_closed_forms(oracle):
+    """TaylorF2 coefficients at eta = 1/4, chi = 0 against the exact rational
+    values of arXiv:0907.0700 Eq. 3.18 (pfaN = 3/(128 eta) = 3/32)."""
+    st, s = oracle.setup(10.0, 10.0, 0.0, 0.0)
+    assert st == 0
+    pfaN = 3.0 / 32.0
+    eta = Fr(1, 4)
+    assert s["pn_v0"] == pytest.approx(pfaN, rel=1e-15)
+    assert s["pn_v1"] == 0.0
+    assert s["pn_v2"] == pytest.approx(pfaN * float(Fr(5, 9) * (Fr(743, 84) + 11 * eta)), rel=1e-15)
+    assert s["pn_v3"] == pytest.approx(pfaN * (-16 * np.pi), rel=1e-15)
+    v4 = Fr(5, 72) * (Fr(3058673, 7056) + Fr(5429, 7) * eta + 617 * eta * eta)
+    assert s["pn_v4"] == pytest.approx(pfaN * float(v4), rel=1e-15)
+    assert s["pn_v5"] == pytest.approx(pfaN * float(Fr(5, 9) * (Fr(7729, 84) - 13 * eta)) * np.pi, rel=1e-15)
+    assert s["pn_vlogv5"] == pytest.approx(3 * s["pn_v5"], rel=1e-15)
+    g = 0.577215664901532860606512090082402431
+    v6 = (11583231236531 / 4694215680 - 640 / 3 * np.pi ** 2 - 6848 / 21 * g
+          + 0.25 * (-15737765635 / 3048192 + 2255 / 12 * np.pi ** 2) + 76055 / 1728 / 16 - 127825 / 1296 / 64
+          - 6848 / 21 * np.log(4.0))
+    assert s["pn_v6"] == pytest.approx(pfaN * v6, rel=1e-13)
+    assert s["pn_vlogv6"] == pytest.approx(pfaN * (-6848 / 21), rel=1e-15)
+    v7 = np.pi * (77096675 / 254016 + 378515 / 1512 / 4 - 74045 / 756 / 16)
+    assert s["pn_v7"] == pytest.approx(pfaN * v7, rel=1e-14)
+
+
+@pytest.mark.parametrize("m1,m2,c1,c2", [(30.0, 10.0, 0.7, -0.4), (5.0, 4.0, -0.3, 0.9), (1.5, 1.3, 0.05, -0.05)])
+def test_pn_spin_orbit_in_symmetric_antisymmetric_form(oracle, m1, m2, c1, c2):
+    """The 1.5PN and 2.5PN spin-orbit terms, written in the reference with the
+    SL / dSigmaL variables of arXiv:1303.7412, against the chi_s / chi_a form
+    of the literature (e.g. arXiv:0810.5336 Eq. 6.22, arXiv:0909.2867 Eq. 6)."""
+    st, s = oracle.setup(m1, m2, c1, c2)
+    M = m1 + m2
+    eta = m1 * m2 / M ** 2
+    delta = (m1 - m2) / M
+    xs, xa = 0.5 * (c1 + c2), 0.5 * (c1 - c2)
+    pfaN = 3.0 / (128.0 * eta)
+    beta = 113.0 / 3.0 * delta * xa + (113.0 / 3.0 - 76.0 * eta / 3.0) * xs
+    assert s["pn_v3"] == pytest.approx(pfaN * (-16 * np.pi + beta), rel=1e-13)
+    gam = (732985 / 2268 - 24260 * eta / 81 - 340 * eta ** 2 / 9) * xs + (732985 / 2268 + 140 * eta / 9) * delta * xa
+    assert s["pn_v5"] == pytest.approx(pfaN * ((38645 / 756 - 65 * eta / 9) * np.pi - gam), rel=1e-12)
+    assert s["pn_vlogv5"] == pytest.approx(pfaN * 3 * ((38645 / 756 - 65 * eta / 9) * np.pi - gam), rel=1e-12)
+
+
+# ------------------------------------------------------------ QNM spline
+
+
+def test_qnm_spline_matches_scipy_natural_spline(oracle):
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "quickcbc_amd", "csrc",
+                        "qcbc_tables.inc")
+    txt = open(path).read()
+
+    def arr(name):
+        body = re.search(name + r"\[QCBC_QNM_N\] = \{(.*?)\};", txt, re.S).group(1)
+        return np.array([float.fromhex(t) for t in re.findall(r"-?0x[0-9a-fA-F.]+p[+-]?\d+", body)])
+
+    a, fr, fd = arr("qcbc_qnm_a"), arr("qcbc_qnm_fring"), arr("qcbc_qnm_fdamp")
+    assert a.size == fr.size == fd.size == 1003
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-1, 1, 2000), a[::37]])
+    for which, y in ((0, fr), (1, fd)):
+        ref = scipy.interpolate.CubicSpline(a, y, bc_type="natural")(x)
+        got = oracle.spline_eval(which, x)
+        assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-12
+        # exact at the knots
+        assert np.max(np.abs(oracle.spline_eval(which, a) - y) / y) < 1e-15
+
+
+def test_final_state_physics(oracle):
+    """Equal-mass non-spinning remnant: final spin ~0.686 and radiated energy
+    ~4.8% (numerical relativity; arXiv:1508.07250), and the spin fit's
+    extreme-mass-ratio limit a_f -> chi1."""
+    assert oracle.final_spin(0.25, 0.0, 0.0) == pytest.approx(0.686, abs=2e-3)
+    assert oracle.erad(0.25, 0.0, 0.0) == pytest.approx(0.048, abs=2e-3)
+    assert oracle.final_spin(1e-6, 0.5, 0.0) == pytest.approx(0.5, abs=1e-4)
+    # aligned spins raise the final spin
+    assert oracle.final_spin(0.25, 0.5, 0.5) > oracle.final_spin(0.25, 0.0, 0.0) > oracle.final_spin(0.25, -0.5, -0.5)
+    # ringdown frequency of the equal-mass remnant: Kerr l=m=2 QNM at a=0.686,
+    # M_f omega = 0.5265 (Berti et al. tables), f = omega/2pi, M = M_f/(1-E_rad)
+    st, s = oracle.setup(20.0, 20.0, 0.0, 0.0)
+    assert s["fRD"] == pytest.approx(0.5265 / (2 * np.pi) / (1 - 0.0484), abs=1e-3)
+
+
+# -------------------------------------------------------- amplitude model
+
+
+SYSTEMS = [(36.0, 29.0, 0.3, -0.2), (10.0, 1.5, 0.8, 0.0), (1.5, 1.3, 0.05, -0.05), (60.0, 50.0, -0.9, -0.9)]
+
+
+@pytest.mark.parametrize("sysp", SYSTEMS)
+def test_collocation_conditions(oracle, sysp):
+    """The intermediate quartic meets the collocation conditions of
+    arXiv:1508.07253 Eqs. 22-26: values at f1, f2, f3, slopes at f1, f3."""
+    st, s = oracle.setup(*sysp)
+    d = [s["delta%d" % i] for i in range(5)]
+    P = np.polynomial.Polynomial(d)
+    dP = P.deriv()
+    f1, f2, f3 = s["f1"], s["f2"], s["f3"]
+    assert f1 == 0.014 and f2 == pytest.approx(0.5 * (f1 + f3), rel=1e-15)
+    for f, v in ((f1, s["v1"]), (f2, s["v2"]), (f3, s["v3"])):
+        assert P(f) == pytest.approx(v, rel=1e-11)
+    assert dP(f1) == pytest.approx(s["d1"], rel=1e-9)
+    assert dP(f3) == pytest.approx(s["d2"], rel=1e-9, abs=1e-9 * abs(s["d1"]))
+    # v1 is the inspiral ansatz and v3 the merger-ringdown ansatz at the ends
+    assert oracle.probe(0, *sysp, f1) == pytest.approx(s["v1"], rel=1e-15)
+    assert oracle.probe(2, *sysp, f3) == pytest.approx(s["v3"], rel=1e-15)
+
+
+@pytest.mark.parametrize("sysp", SYSTEMS)
+@pytest.mark.parametrize("Mf", [0.004, 0.011, 0.05, 0.09])
+def test_amplitude_derivatives_by_finite_differences(oracle, sysp, Mf):
+    h = 1e-5 * Mf
+    for f_, df_ in ((0, 1), (2, 3)):
+        fd = (oracle.probe(f_, *sysp, Mf + h) - oracle.probe(f_, *sysp, Mf - h)) / (2 * h)
+        an = oracle.probe(df_, *sysp, Mf)
+        assert an == pytest.approx(fd, rel=1e-7, abs=1e-7 * abs(oracle.probe(f_, *sysp, Mf)) / Mf)
+
+
+@pytest.mark.parametrize("sysp", SYSTEMS)
+def test_phase_c1_continuity_and_derivatives(oracle, sysp):
+    """Phase and its slope are continuous at Mf = 0.018 and 0.5 f_RD
+    (ComputeIMRPhenDPhaseConnectionCoefficients), and DPhi* are the
+    derivatives of the branches they belong to."""
+    st, s = oracle.setup(*sysp)
+    fj, fm = 0.018, 0.5 * s["fRD"]
+    p_ins, p_int, p_mrd = (lambda f, w=w: oracle.probe(w, *sysp, f) for w in (5, 6, 7))
+    assert p_int(fj) == pytest.approx(p_ins(fj), rel=1e-12, abs=1e-10)
+    assert p_mrd(fm) == pytest.approx(p_int(fm), rel=1e-12, abs=1e-10)
+    for f0 in (fj, fm):
+        h = 1e-6 * f0
+        slopes = [(p(f0 + h) - p(f0 - h)) / (2 * h) for p in (p_ins, p_int, p_mrd)]
+        if f0 == fj:
+            assert slopes[1] == pytest.approx(slopes[0], rel=1e-6)
+        else:
+            assert slopes[2] == pytest.approx(slopes[1], rel=1e-6)
+    for Mf in (0.006, 0.03, 0.08):
+        h = 1e-6 * Mf
+        fd_ins = (p_ins(Mf + h) - p_ins(Mf - h)) / (2 * h)
+        assert oracle.probe(8, *sysp, Mf) == pytest.approx(fd_ins, rel=1e-6)
+        fd_mrd = (p_mrd(Mf + h) - p_mrd(Mf - h)) / (2 * h)
+        assert oracle.probe(10, *sysp, Mf) + s["C2MRD"] == pytest.approx(fd_mrd, rel=1e-6)
+        fd_int = (p_int(Mf + h) - p_int(Mf - h)) / (2 * h)
+        assert oracle.probe(9, *sysp, Mf) + s["C2Int"] == pytest.approx(fd_int, rel=1e-6)
+
+
+def test_reference_phase_and_fcut(oracle):
+    """phase(fRef) = -2 phi0 (IMRPhenomD.c:518, :550) and zero amplitude and
+    phase above fCut = 0.2/M (IMRPhenomD.c:530-533)."""
+    m1, m2 = 36.0, 29.0
+    M_sec = (m1 + m2) * 4.925491025543575903e-6
+    fcut = 0.2 / M_sec
+    f = np.array([20.0, 100.0, 0.999 * fcut, fcut, 1.001 * fcut, 3 * fcut])
+    for phi0 in (0.0, 0.7, 2.9):
+        st, amp, ph = oracle.h22(f, phi0, 100.0, m1 * W.MSUN_SI, m2 * W.MSUN_SI, 0.3, -0.2, 400 * W.MPC_SI)
+        assert st == 0
+        assert ph[1] == pytest.approx(-2 * phi0, abs=1e-12)
+        assert amp[2] > 0 and amp[3] > 0
+        assert amp[4] == 0 and ph[4] == 0 and amp[5] == 0 and ph[5] == 0
+    # domain errors: |chi| > 1, fCut below the first frequency
+    assert oracle.h22(f, 0.0, 100.0, m1 * W.MSUN_SI, m2 * W.MSUN_SI, 1.2, 0.0, 1e25)[0] == 33
+    assert oracle.h22(np.array([2 * fcut]), 0.0, 100.0, m1 * W.MSUN_SI, m2 * W.MSUN_SI, 0.0, 0.0, 1e25)[0] == 33
+
+
+def test_amplitude_low_frequency_pn_limit(oracle):
+    """At low frequency the amplitude tends to the leading order
+    A = (M^2/D) sqrt(2 eta/3) pi^(-1/6) (Mf)^(-7/6) of the 22 mode without the
+    Y22 factor (arXiv:1508.07253 Eq. 17), with the 1PN correction
+    -(323/224 - 451 eta/168) (pi Mf)^(2/3) of the SPA amplitude."""
+    m1, m2 = 10.0, 10.0
+    M = m1 + m2
+    eta = 0.25
+    Msec = M * 4.925491025543575903e-6
+    dist = 1e25
+    f = np.array([0.5, 1.0])
+    st, amp, ph = oracle.h22(f, 0.0, 100.0, m1 * W.MSUN_SI, m2 * W.MSUN_SI, 0.0, 0.0, dist)
+    amp0 = M * 1476.6250614046494 * M * 4.925491025543575903e-6 / dist
+    lead = amp0 * np.sqrt(2 * eta / 3) * np.pi ** (-1 / 6) * (Msec * f) ** (-7 / 6)
+    one_pn = -(323 / 224 - 451 * eta / 168) * (np.pi * Msec * f) ** (2 / 3)
+    assert np.all(np.abs((amp / lead - 1) / one_pn - 1) < 0.03)
+
+
+# ------------------------------------------------------ detector utilities
+
+
+def test_antenna_patterns_are_spin2_tensor_contractions(oracle):
+    rng = np.random.default_rng(3)
+    gmst = 1.1
+    for det in (0, 1, 2):
+        for _ in range(50):
+            psi, alpha, sd = rng.uniform(0, np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(-1, 1)
+            fp, fc = oracle.f_ant(psi, alpha, sd, gmst, det)
+            fp2, fc2 = oracle.f_ant(psi + np.pi / 2, alpha, sd, gmst, det)
+            fp4, fc4 = oracle.f_ant(psi + np.pi / 4, alpha, sd, gmst, det)
+            fp7, fc7 = oracle.f_ant(psi + 0.7, alpha, sd, gmst, det)
+            # spin-2: psi -> psi + pi/2 flips both; F+^2 + Fx^2 is psi-invariant
+            assert fp2 == pytest.approx(-fp, abs=2e-8) and fc2 == pytest.approx(-fc, abs=2e-8)
+            assert fp7 ** 2 + fc7 ** 2 == pytest.approx(fp ** 2 + fc ** 2, abs=2e-8)
+            # rotation by pi/4 exchanges the polarisations
+            assert abs(fp4) == pytest.approx(abs(fc), abs=2e-8)
+            assert fp ** 2 + fc ** 2 <= 1.0 + 1e-8
+        # isotropic sky average of F+^2 + Fx^2 is 2/5 for a 90-degree interferometer
+        n = 200000
+        psi = rng.uniform(0, np.pi, n)
+        al = rng.uniform(0, 2 * np.pi, n)
+        sd = rng.uniform(-1, 1, n)
+        acc = 0.0
+        for i in range(0, n, 97):
+            fp, fc = oracle.f_ant(psi[i], al[i], sd[i], gmst, det)
+            acc += fp * fp + fc * fc
+        mean = acc / len(range(0, n, 97))
+        assert mean == pytest.approx(0.4, abs=0.02)
+
+
+def test_light_travel_delays(oracle):
+    """Largest |dt| between sites equals the published light travel times
+    (H-L 10.0 ms, H-V 27.3 ms, L-V 26.4 ms), delays are antisymmetric under
+    n -> -n and bounded by the Earth radius.  (ConstCBC.h:19-21 caps L-V at
+    25.7 ms, below the geometric 26.4 ms of its own vertex vectors.)"""
+    rng = np.random.default_rng(4)
+    gmst = 0.3
+    mx = np.zeros(3)
+    for _ in range(3000):
+        a, sd = rng.uniform(0, 2 * np.pi), rng.uniform(-1, 1)
+        t = oracle.times(a, sd, gmst)
+        ta = oracle.times(a + np.pi, -sd, gmst)
+        assert np.allclose(t, -ta, atol=1e-15)
+        assert np.all(np.abs(t) < 6.4e6 / 299792458.0)
+        mx = np.maximum(mx, [abs(t[0] - t[1]), abs(t[0] - t[2]), abs(t[1] - t[2])])
+    assert mx == pytest.approx([0.0100, 0.0273, 0.0264], rel=0.01)
+
+
+def test_fbegin_tracks_newtonian_chirp(oracle):
+    """fbegin's 3PN start frequency agrees with the Newtonian
+    f = (1/8 pi M) (eta tc / 5M)^(-3/8) to PN accuracy and falls with tc."""
+    p = W.make_params(1.5, 1.3, 0.0, 0.0, 0.0, 14.0, 40.0, 0.0, 0.0, 0.0, 0.0)
+    f14 = oracle.fbegin(p)
+    M = 2.8 * 4.92569043916e-6
+    eta = 1.5 * 1.3 / 2.8 ** 2
+    fN = (eta * 14.0 / (5 * M)) ** (-3.0 / 8.0) / (8 * np.pi * M)
+    assert f14 == pytest.approx(fN, rel=0.05)
+    p2 = p.copy()
+    p2[5] = 10.0
+    assert oracle.fbegin(p2) > f14
+
+
+# ------------------------------------------------------ inner product, logL
+
+
+def test_fourier_nwip_brute_force(oracle):
+    rng = np.random.default_rng(6)
+    N = 512
+    a, b = rng.standard_normal(N), rng.standard_normal(N)
+    S = rng.uniform(0.5, 2.0, N // 2)
+    ref = 4.0 * sum((a[i] * b[i] + a[N - i] * b[N - i]) / S[i] for i in range(10, 200))
+    assert oracle.fourier_nwip(a, b, S, 10, 200, N) == pytest.approx(ref, rel=1e-14)
+    assert oracle.fourier_nwip(a, b, S, 50, 50, N) == 0.0
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_loglike_identities(oracle, which):
+    an, inj, D0, SN = _data.analysis(which, noise=False)
+    L, rho, st = oracle.loglike(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D0, SN, an.N, an.Tobs, inj)
+    assert st == 0
+    # noise-free: (d|h) = (h|h), so logL = (h|h)/2 and rho_d = sqrt((h_d|h_d))
+    st2, h = oracle.fulltemplates(an.labels, an.gmst, an.fref, an.N, an.Tobs, inj)
+    HH = [oracle.fourier_nwip(h[d], h[d], SN[d], an.imin, an.imax, an.N) for d in range(an.nifo)]
+    assert L == pytest.approx(0.5 * sum(HH), rel=1e-14)
+    assert rho == pytest.approx(np.sqrt(HH), rel=1e-14)
+    # invariance under GW phase + 2 pi and psi + pi
+    for k, dv in ((4, 2 * np.pi), (9, np.pi)):
+        q = inj.copy()
+        q[k] += dv
+        L2, _, _ = oracle.loglike(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D0, SN, an.N, an.Tobs, q)
+        assert L2 == pytest.approx(L, rel=1e-9)
+
+
+def test_noise_normalisation(oracle):
+    """E[(n|n)] = 2 per in-band bin for Re, Im ~ N(0, S/4)."""
+    an = W.config0()
+    n = W.noise(an, 77)
+    SN = an.SN()
+    tot = sum(oracle.fourier_nwip(n[d], n[d], SN[d], an.imin, an.imax, an.N) for d in range(an.nifo))
+    nb = an.nifo * (an.imax - an.imin)
+    assert tot / nb == pytest.approx(2.0, abs=4 * 2.0 / np.sqrt(nb))
