@@ -119,7 +119,8 @@ enum {
   QCBC_ENODEV = 101,  /* no usable gfx950 device */
   QCBC_ENOTBOUND = 102,
   QCBC_EBUSY = 103,   /* admission refused: memory budget or slots */
-  QCBC_ECANCELED = 104
+  QCBC_ECANCELED = 104,
+  QCBC_ETIMEDOUT = 105
 };
 
 enum { QCBC_FP64 = 0, QCBC_MIXED = 1 };
@@ -169,6 +170,12 @@ int qcbc_waveform(qcbc_ctx *ctx, const double *params, int B, double *amp, doubl
 /* Projected half-complex templates (fulltemplates) for B points: hwave is
  * B x n_det x N. */
 int qcbc_templates(qcbc_ctx *ctx, const double *params, int B, double *hwave, int *status);
+/* The per-point set-up records (IMRPhenomD coefficients, projection
+ * constants, contributing bin range) the kernels use, copied to host memory:
+ * B records of qcbc_point_record_size() bytes each, for inspection and
+ * set-up parity tests. */
+size_t qcbc_point_record_size(void);
+int qcbc_point_records(qcbc_ctx *ctx, const double *params, int B, void *records);
 
 /* Kernel launch shape of the fused likelihood for this context:
  * chains per block, bins per LDS tile, bin splits (0 = choose). */
@@ -209,10 +216,39 @@ int qcbc_pt_destroy(qcbc_pt *pt);
  * ------------------------------------------------------------------ */
 /* Evaluate n batches (ctxs[i], d_params[i], B[i]) in one persistent kernel
  * draining an (analysis, chain group, bin range) work queue.  Device
- * pointers; synchronous. */
+ * pointers; synchronous.  One-shot form of the plan below. */
 int qcbc_multi_loglike(qcbc_ctx *const *ctxs, const double *const *d_params, const int *B,
                        int n, int precision, double *const *d_logL, double *const *d_rho,
                        int *const *d_status);
+
+/* A reusable plan for a fixed set of analyses and batch sizes: the work
+ * queue is built once; each run launches set-up per analysis, the persistent
+ * queue kernel and finalize per analysis on the plan's stream, synchronously.
+ * use_graph: capture that sequence once as a hipGraph and replay it (captured
+ * again if the buffer pointers change).  The contexts must not be used by
+ * other calls while a plan built on them runs. */
+typedef struct qcbc_multi qcbc_multi;
+int qcbc_multi_create(qcbc_ctx *const *ctxs, const int *B, int n, int precision,
+                      qcbc_multi **out);
+/* points per block (1..8), bins per LDS tile (multiple of 64), bins per split */
+int qcbc_multi_set_shape(qcbc_multi *m, int chains_per_block, int tile, int split_bins);
+int qcbc_multi_get_shape(qcbc_multi *m, int *chains_per_block, int *tile, int *split_bins,
+                         int *n_items);
+int qcbc_multi_run(qcbc_multi *m, const double *const *d_params, double *const *d_logL,
+                   double *const *d_rho, int *const *d_status, int use_graph);
+/* Time candidate shapes (reps runs each) and keep the fastest. */
+int qcbc_multi_autotune(qcbc_multi *m, const double *const *d_params, double *const *d_logL,
+                        double *const *d_rho, int reps, int *best_cpb, int *best_tile,
+                        int *best_split, double *best_ms);
+/* Host wall time (ms) of the last run, launch to completion. */
+int qcbc_multi_last_ms(qcbc_multi *m, double *ms);
+int qcbc_multi_destroy(qcbc_multi *m);
+
+/* Device memory helpers for callers without their own allocator. */
+int qcbc_dev_alloc(size_t bytes, void **ptr);
+int qcbc_dev_free(void *ptr);
+int qcbc_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int qcbc_memcpy_d2h(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------ *
  * Part 5: serving loop                                                *
@@ -229,7 +265,9 @@ int qcbc_server_evict(qcbc_server *s, int analysis_id);
 int qcbc_server_submit(qcbc_server *s, int analysis_id, const double *params, int B,
                        int64_t *request_id);
 /* Wait for a request; logL, rho (B x n_det) and status are written to host
- * memory; latency_ms is submit-to-completion.  timeout_ms < 0 waits forever. */
+ * memory; latency_ms is submit-to-completion.  timeout_ms < 0 waits forever;
+ * QCBC_ETIMEDOUT leaves the request pending (wait again), QCBC_ECANCELED
+ * reports a request whose analysis was evicted. */
 int qcbc_server_wait(qcbc_server *s, int64_t request_id, double *logL, double *rho, int *status,
                      double *latency_ms, int timeout_ms);
 int qcbc_server_stats(qcbc_server *s, size_t *bytes_in_use, int *n_analyses, long long *evals,

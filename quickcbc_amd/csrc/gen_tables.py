@@ -121,7 +121,12 @@ def main():
         verts.append(terms)
 
     L = []
-    L.append("/* Generated by gen_tables.py from the QuickCBC sources; do not edit. */")
+    L.append("/* Generated by gen_tables.py from the QuickCBC sources; do not edit.")
+    L.append("   QCBC_TABLE_STORAGE: tables indexed at run time (the QNM spline);")
+    L.append("   QCBC_CONST_STORAGE: tables whose indices are compile-time constants. */")
+    L.append("#ifndef QCBC_CONST_STORAGE")
+    L.append("#define QCBC_CONST_STORAGE QCBC_TABLE_STORAGE")
+    L.append("#endif")
     L.append("/* Kerr QNM table, IMRPhenomD_internals.h:176-617 (1003 spins). */")
     L.append("#define QCBC_QNM_N 1003")
     for nm, arr in (("qcbc_qnm_a", qa), ("qcbc_qnm_fring", qr), ("qcbc_qnm_fdamp", qd)):
@@ -138,13 +143,13 @@ def main():
     nt_max = max(len(t) for _, _, t in polys)
     L.append("#define QCBC_ANT_MAXTERMS %d" % nt_max)
     L.append("#define QCBC_ANT_MAXFAC 6")
-    L.append("QCBC_TABLE_STORAGE int qcbc_ant_nterms[6] = {%s};" % ", ".join(str(len(t)) for _, _, t in polys))
-    L.append("QCBC_TABLE_STORAGE double qcbc_ant_coef[6][QCBC_ANT_MAXTERMS] = {")
+    L.append("QCBC_CONST_STORAGE int qcbc_ant_nterms[6] = {%s};" % ", ".join(str(len(t)) for _, _, t in polys))
+    L.append("QCBC_CONST_STORAGE double qcbc_ant_coef[6][QCBC_ANT_MAXTERMS] = {")
     for det, comp, terms in polys:
         row = [float.hex(c) for c, _, _ in terms] + ["0x0p+0"] * (nt_max - len(terms))
         L.append("  {" + ", ".join(row) + "},")
     L.append("};")
-    L.append("QCBC_TABLE_STORAGE signed char qcbc_ant_fac[6][QCBC_ANT_MAXTERMS][QCBC_ANT_MAXFAC] = {")
+    L.append("QCBC_CONST_STORAGE signed char qcbc_ant_fac[6][QCBC_ANT_MAXTERMS][QCBC_ANT_MAXFAC] = {")
     for det, comp, terms in polys:
         rows = []
         for _, idx, _ in terms:
@@ -160,7 +165,7 @@ def main():
     L.append("   :1147-1217 (alpha), :1253-1293 (beta), :1335-1390 (sigma). */")
     fitpat = ["eta", "eta", "eta2", "xi", "eta", "eta2", "xi2", "eta", "eta2", "xi3"]
     L.append("enum { " + ", ".join("QCBC_FIT_%s" % ("V2" if f == "AmpIntColFitCoeff" else f.replace("_fun", "").replace("Fit", "").upper()) for f in FITS) + ", QCBC_NFIT };")
-    L.append("QCBC_TABLE_STORAGE double qcbc_fit[QCBC_NFIT][11] = {")
+    L.append("QCBC_CONST_STORAGE double qcbc_fit[QCBC_NFIT][11] = {")
     for f in FITS:
         v = fit_numbers(src, f, fitpat)
         assert len(v) == 11, (f, v)
@@ -171,16 +176,16 @@ def main():
     L.append("/* Final spin fit, arXiv:1508.07250 Eq. 3.6; IMRPhenomD_internals.c:415-430:")
     L.append("   c0 eta + c1 eta^2 + c2 eta^3 + c3 eta^4 + (c4 + c5 eta + c6 eta^2) s")
     L.append("   + (c7 eta + c8 eta^2) s^2 + (c9 eta + c10 eta^2) s^3 + (c11 eta + c12 eta^2) s^4 */")
-    L.append("QCBC_TABLE_STORAGE double qcbc_finalspin[13] = {" + ", ".join(float.hex(x) for x in fs) + "};")
+    L.append("QCBC_CONST_STORAGE double qcbc_finalspin[13] = {" + ", ".join(float.hex(x) for x in fs) + "};")
     er = fit_numbers(src, "EradRational0815_s", ["eta", "eta2", "eta3", "eta4", "eta", "eta2", "s", "eta", "eta2", "s"])
     assert len(er) == 12
     L.append("/* Radiated energy fit, arXiv:1508.07250 Eqs. 3.7-3.8; IMRPhenomD_internals.c:450-459:")
     L.append("   (c0 eta + c1 eta^2 + c2 eta^3 + c3 eta^4)(c4 + (c5 + c6 eta + c7 eta^2) s)")
     L.append("   / (c8 + (c9 + c10 eta + c11 eta^2) s) */")
-    L.append("QCBC_TABLE_STORAGE double qcbc_erad[12] = {" + ", ".join(float.hex(x) for x in er) + "};")
+    L.append("QCBC_CONST_STORAGE double qcbc_erad[12] = {" + ", ".join(float.hex(x) for x in er) + "};")
     L.append("/* Detector vertex projections, Utilities.c:1078-1109: r_d . n =")
     L.append("   c0*cosd*cos(GMST-alpha) + c1*cosd*sin(GMST-alpha) + c2*sind (metres). */")
-    L.append("QCBC_TABLE_STORAGE double qcbc_vertex[3][3] = {")
+    L.append("QCBC_CONST_STORAGE double qcbc_vertex[3][3] = {")
     for terms in verts:
         assert [t[1] for t in terms] == [[0, 1], [0, 2], [3]], terms
         L.append("  {" + ", ".join(float.hex(t[0]) for t in terms) + "},")

@@ -227,6 +227,27 @@ int qcbc::effective_splits(const qcbc_ctx* c, int prec, int B) {
   return std::max(s, 1);
 }
 
+// Kernel view of a context for B points: split length a multiple of the tile.
+LLInst qcbc::make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial) {
+  LLInst in{};
+  in.p64 = c->d_p64;
+  in.p32 = c->d_p32;
+  in.pts = pts;
+  in.partial = partial;
+  in.inv_T = c->an.inv_T;
+  in.nbp = c->nbp;
+  in.imin = c->an.imin;
+  in.nb = c->nb;
+  in.nd = c->desc.n_det;
+  in.B = B;
+  int len = (c->nb + splits - 1) / std::max(splits, 1);
+  len = ((len + tile - 1) / tile) * tile;
+  in.split_len = std::max(len, tile);
+  in.splits = (c->nb + in.split_len - 1) / in.split_len;
+  if (in.splits < 1) in.splits = 1;
+  return in;
+}
+
 static void harvest_events(qcbc_ctx* c) {
   for (auto& e : c->ev_pairs) {
     float ms = 0.f;
@@ -245,22 +266,21 @@ int qcbc::loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, dou
   int st = ctx_reserve(c, B);
   if (st) return st;
   HIPCHK(launch_setup(d_params, B, c->an, c->d_pts, d_status, c->stream));
-  const int splits = effective_splits(c, prec, B);
+  const LLInst in = make_inst(c, c->tile[prec], B, effective_splits(c, prec, B), c->d_pts, c->d_partial);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c->stream));
   }
-  HIPCHK(launch_ll(prec, c->desc.n_det, c->d_p64, c->d_p32, c->nbp, c->an.imin, c->nb, c->d_pts, B, c->cpb[prec],
-                   c->tile[prec], splits, c->d_partial, c->stream));
+  HIPCHK(launch_ll(prec, in, c->cpb[prec], c->tile[prec], c->stream));
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, c->stream));
     c->ev_pairs.emplace_back(e0, e1);
     c->launches++;
     if (c->ev_pairs.size() >= 512) harvest_events(c);
   }
-  HIPCHK(launch_finalize(c->d_partial, splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED, c->d_wref, d_logL,
+  HIPCHK(launch_finalize(c->d_partial, in.splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED, c->d_wref, d_logL,
                          d_rho, c->stream));
   return QCBC_OK;
 }
@@ -492,6 +512,7 @@ const char* qcbc_strerror(int code) {
     case QCBC_ENOTBOUND: return "no data bound to this analysis";
     case QCBC_EBUSY: return "admission refused: memory budget or slots exhausted";
     case QCBC_ECANCELED: return "request canceled";
+    case QCBC_ETIMEDOUT: return "timed out waiting for a request";
     default: return "unknown error";
   }
 }
@@ -508,5 +529,46 @@ int qcbc_device_info(char* name, size_t len, int* cu_count, double* mem_gb) {
   }
   if (cu_count) *cu_count = prop.multiProcessorCount;
   if (mem_gb) *mem_gb = prop.totalGlobalMem / 1e9;
+  return QCBC_OK;
+}
+
+// ------------------------------------------------------ device memory
+int qcbc_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return QCBC_EINVAL;
+  int st = ensure_device();
+  if (st) return st;
+  if (hipMalloc(ptr, bytes) != hipSuccess) return QCBC_ENOMEM;
+  return QCBC_OK;
+}
+
+int qcbc_dev_free(void* ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return QCBC_OK;
+}
+
+int qcbc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return QCBC_OK;
+}
+
+int qcbc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return QCBC_OK;
+}
+
+// ------------------------------------------------ set-up records
+size_t qcbc_point_record_size(void) { return sizeof(Point); }
+
+int qcbc_point_records(qcbc_ctx* c, const double* params, int B, void* records) {
+  if (!c || B < 0 || (B > 0 && (!params || !records))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_setup(c->d_params, B, c->an, c->d_pts, c->d_status, c->stream));
+  HIPCHK(hipMemcpyAsync(records, c->d_pts, sizeof(Point) * B, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return QCBC_OK;
 }

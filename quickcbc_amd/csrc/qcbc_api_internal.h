@@ -55,6 +55,7 @@ namespace qcbc {
 extern thread_local hipError_t t_last_hip;
 int ensure_device();
 int effective_splits(const qcbc_ctx* c, int prec, int B);
+LLInst make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial);
 // setup + fused likelihood + finalize on the context stream (caller holds mu)
 int loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
                  int* d_status);

@@ -13,6 +13,7 @@
 #include "qcbc_types.h"
 
 #define QCBC_TABLE_STORAGE static __constant__
+#define QCBC_CONST_STORAGE static constexpr
 #include "qcbc_tables.inc"
 
 namespace qcbc {
@@ -34,6 +35,14 @@ constexpr double CLIGHT = 299792458.0;
 constexpr double H22FAC = 0.31539156525252;
 constexpr double LOG_PI = 1.1447298858494001741434273513530587;
 constexpr double INV_TPI = 0.15915494309189533576888376337251436;
+// powers of pi as init_useful_powers(&powers_of_pi, PI) forms them
+// (IMRPhenomD_internals.c:515-533): pow(pi, 1/6) rounded to double, then
+// products and quotients in double
+constexpr double PI_S6 = 0x1.35cfe132fe3abp+0;
+constexpr double PI_THIRD = 0x1.76ef7e73104b8p+0;
+constexpr double PI_TWO_THIRDS = 0x1.129052c2c6753p+1;
+constexpr double PI_FOUR_THIRDS = 0x1.2679428626b38p+2;
+constexpr double PI_FIVE_THIRDS = 0x1.af48a3d921bdbp+2;
 
 // QNM spline slopes, filled once per device by qcbc_qnm_init_kernel.
 extern __device__ double g_qnm_slope[2][QCBC_QNM_N];
@@ -49,14 +58,17 @@ __device__ __forceinline__ double fit11(int w, double eta, double chi) {
          (c[5] + c[6] * eta + c[7] * eta2) * xi2 + (c[8] + c[9] * eta + c[10] * eta2) * xi3;
 }
 
-// Natural cubic spline through the QNM table in Hermite (slope) form.
+// Natural cubic spline through the QNM table in Hermite (slope) form.  The
+// knots are spaced 0.002 apart except the two outermost intervals at each
+// end (0.001), so the interval follows from the abscissa directly.
 __device__ inline double qnm_spline(int which, double x) {
   const double* ya = which ? qcbc_qnm_fdamp : qcbc_qnm_fring;
-  int lo = 0, hi = QCBC_QNM_N - 1;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (qcbc_qnm_a[mid] > x) hi = mid; else lo = mid;
-  }
+  int lo = 2 + (int)floor((x + 0.996) * 500.0);
+  lo = lo < 0 ? 0 : (lo > QCBC_QNM_N - 2 ? QCBC_QNM_N - 2 : lo);
+  if (lo > 0 && qcbc_qnm_a[lo] > x) lo--;
+  if (lo > 0 && qcbc_qnm_a[lo] > x) lo--;
+  if (lo < QCBC_QNM_N - 2 && qcbc_qnm_a[lo + 1] <= x) lo++;
+  if (lo < QCBC_QNM_N - 2 && qcbc_qnm_a[lo + 1] <= x) lo++;
   double h = qcbc_qnm_a[lo + 1] - qcbc_qnm_a[lo];
   double t = (x - qcbc_qnm_a[lo]) / h;
   double m0 = g_qnm_slope[which][lo] * h, m1 = g_qnm_slope[which][lo + 1] * h;
@@ -121,7 +133,7 @@ __device__ __forceinline__ Pow pow_direct(double Mf) {
   p.i_Mf = 1.0 / Mf;
   p.i_five_thirds = 1.0 / p.five_thirds;
   p.logMf = log(Mf);
-  p.logv = log(p.third * 0x1.76ef7e73104b8p+0);  // v = (pi Mf)^(1/3); pow(pi,1/6)^2
+  p.logv = log(p.third * PI_THIRD);  // v = (pi Mf)^(1/3)
   return p;
 }
 
@@ -212,41 +224,68 @@ __device__ __forceinline__ double phase(const Point& c, const Pow& p) {
 }
 
 // ------------------------------------------------ detector geometry
-// F_ant (Utilities.c:995-1076): expanded polynomials, table-driven.
-__device__ inline void antenna(double psi, double alpha, double sindelta, double gmst, int id, double* fp,
-                               double* fc) {
-  double x[9];
-  x[0] = cos(psi);
-  x[1] = sin(psi);
+// F_ant (Utilities.c:995-1076): the detectors' expanded polynomials,
+// table-driven.  x holds the nine trigonometric factors of one sky position
+// and polarisation (shared by all detectors).
+__device__ inline void antenna_factors(double psi, double alpha, double sindelta, double gmst, double* x) {
+  sincos(psi, &x[1], &x[0]);
   x[2] = sqrt(1.0 - sindelta * sindelta);
   x[3] = sindelta;
-  x[4] = cos(alpha - gmst);
-  x[5] = sin(alpha - gmst);
+  sincos(alpha - gmst, &x[5], &x[4]);
   x[6] = x[0] * x[0];
   x[7] = x[3] * x[3];
   x[8] = x[4] * x[4];
-  for (int comp = 0; comp < 2; comp++) {
-    int k = 2 * id + comp;
-    double s = 0.0;
-    for (int t = 0; t < qcbc_ant_nterms[k]; t++) {
-      double term = qcbc_ant_coef[k][t];
-      for (int j = 0; j < QCBC_ANT_MAXFAC; j++) {
-        int v = qcbc_ant_fac[k][t][j];
-        if (v < 0) break;
-        term *= x[v];
-      }
+}
+
+// Polynomial K evaluated term by term in source order.  The tables are
+// compile-time constants and every loop is unrolled, so each factor index
+// folds to a register operand (a run-time index into x[] would put x in
+// scratch memory).
+template <int K>
+__device__ __forceinline__ double antenna_poly(const double* x) {
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < QCBC_ANT_MAXTERMS; t++) {
+    if (t < qcbc_ant_nterms[K]) {
+      double term = qcbc_ant_coef[K][t];
+#pragma unroll
+      for (int j = 0; j < QCBC_ANT_MAXFAC; j++)
+        if (qcbc_ant_fac[K][t][j] >= 0) term *= x[qcbc_ant_fac[K][t][j]];
       s = (t == 0) ? term : s + term;
     }
-    if (comp == 0) *fp = s; else *fc = s;
+  }
+  return s;
+}
+
+__device__ __forceinline__ void antenna_id(int id, const double* x, double* fp, double* fc) {
+  switch (id) {
+    case 0: *fp = antenna_poly<0>(x); *fc = antenna_poly<1>(x); break;
+    case 1: *fp = antenna_poly<2>(x); *fc = antenna_poly<3>(x); break;
+    default: *fp = antenna_poly<4>(x); *fc = antenna_poly<5>(x); break;
   }
 }
 
+__device__ inline void antenna(double psi, double alpha, double sindelta, double gmst, int id, double* fp,
+                               double* fc) {
+  double x[9];
+  antenna_factors(psi, alpha, sindelta, gmst, x);
+  antenna_id(id, x, fp, fc);
+}
+
 // Times (Utilities.c:1078-1109): geocentre-to-detector delay per label.
+__device__ inline double delay_cs(double cosd, double sindelta, double ca, double sa, int id) {
+  double v0, v1, v2;
+  switch (id) {
+    case 0: v0 = qcbc_vertex[0][0]; v1 = qcbc_vertex[0][1]; v2 = qcbc_vertex[0][2]; break;
+    case 1: v0 = qcbc_vertex[1][0]; v1 = qcbc_vertex[1][1]; v2 = qcbc_vertex[1][2]; break;
+    default: v0 = qcbc_vertex[2][0]; v1 = qcbc_vertex[2][1]; v2 = qcbc_vertex[2][2]; break;
+  }
+  return -(v0 * cosd * ca + v1 * cosd * sa + v2 * sindelta) / CLIGHT;
+}
 __device__ inline double delay(double alpha, double sindelta, double gmst, int id) {
-  double cosd = sqrt(1.0 - sindelta * sindelta);
-  double ca = cos(gmst - alpha), sa = sin(gmst - alpha);
-  const double* v = qcbc_vertex[id];
-  return -(v[0] * cosd * ca + v[1] * cosd * sa + v[2] * sindelta) / CLIGHT;
+  double sa, ca;
+  sincos(gmst - alpha, &sa, &ca);
+  return delay_cs(sqrt(1.0 - sindelta * sindelta), sindelta, ca, sa, id);
 }
 
 // fbegin (Utilities.c:587-629): 3PN frequency at the segment start.
@@ -403,11 +442,8 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
   c.s6_scale = pow(c.Msec, 1.0 / 6.0);
   c.is6_scale = 1.0 / c.s6_scale;
   c.logMsec = log(c.Msec);
-  const double pi_s6 = pow(PI, 1.0 / 6.0);
-  const double pi_third = pi_s6 * pi_s6;
-  const double pi_two_thirds = PI / pi_third;
-  const double pi_four_thirds = PI * pi_third;
-  const double pi_five_thirds = pi_four_thirds * pi_third;
+  const double pi_s6 = PI_S6, pi_third = PI_THIRD, pi_two_thirds = PI_TWO_THIRDS;
+  const double pi_four_thirds = PI_FOUR_THIRDS, pi_five_thirds = PI_FIVE_THIRDS;
   const double pi2 = PI * PI;
 
   // final state (IMRPhenomD_internals.c:415-513)
@@ -545,18 +581,23 @@ __device__ inline int point_setup(const double* prm, const AnalysisConst& an, Po
   {
     const double alpha = prm[7], sindelta = prm[8], psi = prm[9], ciota = prm[10];
     const double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
+    double x[9];
+    antenna_factors(psi, alpha, sindelta, an.gmst, x);
+    // Times uses cos/sin(gmst - alpha) (Utilities.c:1090-1091): the same
+    // angle with the opposite sign as the antenna patterns' alpha - gmst
+    const double sa = -x[5], ca = x[4];
     for (int d = 0; d < 3; d++) {
       c.Fs[d] = c.tshift[d] = c.lambda[d] = c.Ascale[d] = 0.0;
       if (d >= an.n_det) continue;
       int id = an.labels[d];
       double Fp, Fc;
-      antenna(psi, alpha, sindelta, an.gmst, id, &Fp, &Fc);
+      antenna_id(id, x, &Fp, &Fc);
       double Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
       double lam = atan2(Ac * Fc, Ap * Fp);
       if (lam < 0.0) lam += TPI;
       c.Fs[d] = Fs;
       c.lambda[d] = lam;
-      c.tshift[d] = ts - delay(alpha, sindelta, an.gmst, id);
+      c.tshift[d] = ts - delay_cs(x[2], sindelta, ca, sa, id);
       c.Ascale[d] = Fs * H22FAC * c.amp0 / an.sqT;
     }
   }

@@ -18,8 +18,10 @@ hipError_t launch_setup_h22(double m1, double m2, double c1, double c2, double d
 hipError_t launch_bind(const double* D, const double* SN, int nd, int N, double Tobs, int imin, int nb, int nbp,
                        const double* wref, double* p64, float* p32, hipStream_t s);
 size_t ll_lds_bytes(int precision, int nd, int tile);
-hipError_t launch_ll(int precision, int nd, const double* p64, const float* p32, int nbp, int imin, int nb,
-                     const Point* pts, int B, int cpb, int tile, int splits, double* partial, hipStream_t s);
+hipError_t launch_ll(int precision, const LLInst& in, int cpb, int tile, hipStream_t s);
+hipError_t launch_ll_queue(int precision, const LLInst* d_insts, const LLItem* d_items, int n_items, int* d_counter,
+                           int max_nd, int cpb, int tile, int blocks, hipStream_t s);
+hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s);
 hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mixed,
                            const double* wref, double* logL, double* rho, hipStream_t s);
 hipError_t launch_waveform(const Point* pts, int B, int nhalf, double Tobs, double* amp, double* phs, hipStream_t s);

@@ -60,29 +60,30 @@ __global__ void k_qnm_init() {
 }
 
 // ------------------------------------------------------------- set-up
-__global__ void k_setup(const double* __restrict__ params, int B, AnalysisConst an, Point* __restrict__ pts,
-                        int* __restrict__ status) {
+// One thread per point; the record is built in place in global memory (a
+// ~0.8 KB local copy would not fit in registers).
+__global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int B, AnalysisConst an,
+                                              Point* __restrict__ pts, int* __restrict__ status) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  Point c;
   double prm[11];
 #pragma unroll
   for (int i = 0; i < 11; i++) prm[i] = params[11 * b + i];
+  Point& c = pts[b];
   int st = point_setup(prm, an, c);
   c.status = st;
   if (st != ST_OK) c.kstart = c.kend = an.imin;
-  pts[b] = c;
   if (status) status[b] = st;
 }
 
 // One point from explicit IMRPhenomD arguments (the h22 drop-in).
-__global__ void k_setup_h22(double m1_SI, double m2_SI, double chi1, double chi2, double distance, double phi0,
-                            double fref, double fmin0, Point* pts, int* status) {
+__global__ __launch_bounds__(64) void k_setup_h22(double m1_SI, double m2_SI, double chi1, double chi2,
+                                                  double distance, double phi0, double fref, double fmin0,
+                                                  Point* pts, int* status) {
   if (threadIdx.x != 0) return;
-  Point c;
+  Point& c = pts[0];
   int st = phenomd_setup(m1_SI, m2_SI, chi1, chi2, distance, phi0, fref, fmin0, c);
   c.status = st;
-  pts[0] = c;
   status[0] = st;
 }
 

@@ -35,4 +35,22 @@ struct alignas(16) Point {
   double eta, finspin;
 };
 
+// One analysis as the fused likelihood kernels see it.  Bin splits are
+// split_len band bins long (a multiple of the LDS tile) so that tile and
+// split boundaries depend only on the analysis, never on the batch.
+struct LLInst {
+  const double* p64;
+  const float* p32;
+  const Point* pts;
+  double* partial;   // (B x splits x 2 nd) partial (h|h), (d|h)
+  double inv_T;
+  int nbp, imin, nb, nd;
+  int B, split_len, splits, pad;
+};
+
+// One work item of the persistent kernel: (analysis, group of points, split).
+struct LLItem {
+  int inst, group, split, pad;
+};
+
 }  // namespace qcbc

@@ -383,3 +383,217 @@ class PT:
             self.close()
         except Exception:  # noqa: BLE001
             pass
+
+
+# ------------------------------------------------------- device buffers
+class DeviceBuffer:
+    """Device memory from qcbc_dev_alloc (for the device-pointer entry points)."""
+
+    def __init__(self, nbytes):
+        L = lib()
+        L.qcbc_dev_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.qcbc_dev_free.argtypes = [_vp]
+        L.qcbc_memcpy_h2d.argtypes = [_vp, _vp, ctypes.c_size_t]
+        L.qcbc_memcpy_d2h.argtypes = [_vp, _vp, ctypes.c_size_t]
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(L.qcbc_dev_alloc(max(self.nbytes, 8), ctypes.byref(p)), "qcbc_dev_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().qcbc_memcpy_h2d(self.ptr, a.ctypes.data_as(_vp), a.nbytes), "qcbc_memcpy_h2d")
+
+    def download(self, shape, dtype=np.float64):
+        out = np.empty(shape, dtype)
+        _check(lib().qcbc_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes), "qcbc_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().qcbc_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def _ptr_array(ptrs):
+    arr = (_vp * len(ptrs))(*[p.ptr if isinstance(p, DeviceBuffer) else p for p in ptrs])
+    return arr
+
+
+class Multi:
+    """Persistent multi-analysis plan (qcbc_multi)."""
+
+    def __init__(self, ctxs, B, precision=FP64):
+        L = lib()
+        L.qcbc_multi_create.argtypes = [ctypes.POINTER(_vp), _ip, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.qcbc_multi_set_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qcbc_multi_get_shape.argtypes = [_vp, _ip, _ip, _ip, _ip]
+        L.qcbc_multi_run.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                     ctypes.POINTER(_vp), ctypes.c_int]
+        L.qcbc_multi_autotune.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.c_int, _ip, _ip, _ip, _dp]
+        L.qcbc_multi_last_ms.argtypes = [_vp, _dp]
+        L.qcbc_multi_destroy.argtypes = [_vp]
+        self.ctxs = list(ctxs)
+        self.B = [int(b) for b in B]
+        hs = (_vp * len(ctxs))(*[c.h for c in ctxs])
+        bb = (ctypes.c_int * len(B))(*self.B)
+        h = _vp()
+        _check(L.qcbc_multi_create(hs, bb, len(ctxs), precision, ctypes.byref(h)), "qcbc_multi_create")
+        self.h = h
+        self.params = [DeviceBuffer(8 * 11 * b) for b in self.B]
+        self.logL = [DeviceBuffer(8 * b) for b in self.B]
+        self.rho = [DeviceBuffer(8 * 3 * b) for b in self.B]
+        self.status = [DeviceBuffer(4 * b) for b in self.B]
+
+    def set_params(self, params_list):
+        for buf, p in zip(self.params, params_list):
+            buf.upload(np.ascontiguousarray(p, np.float64))
+
+    def set_shape(self, cpb, tile, split_bins):
+        _check(lib().qcbc_multi_set_shape(self.h, cpb, tile, split_bins), "qcbc_multi_set_shape")
+
+    def get_shape(self):
+        a, b, c, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().qcbc_multi_get_shape(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                          ctypes.byref(n)), "qcbc_multi_get_shape")
+        return a.value, b.value, c.value, n.value
+
+    def run(self, graph=False):
+        _check(lib().qcbc_multi_run(self.h, _ptr_array(self.params), _ptr_array(self.logL), _ptr_array(self.rho),
+                                    _ptr_array(self.status), 1 if graph else 0), "qcbc_multi_run")
+        ms = ctypes.c_double()
+        lib().qcbc_multi_last_ms(self.h, ctypes.byref(ms))
+        return ms.value
+
+    def autotune(self, reps=3):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(lib().qcbc_multi_autotune(self.h, _ptr_array(self.params), _ptr_array(self.logL),
+                                         _ptr_array(self.rho), reps, ctypes.byref(a), ctypes.byref(b),
+                                         ctypes.byref(c), ctypes.byref(ms)), "qcbc_multi_autotune")
+        return (a.value, b.value, c.value), ms.value
+
+    def results(self):
+        out = []
+        for c, b, l, r, s in zip(self.ctxs, self.B, self.logL, self.rho, self.status):
+            out.append((l.download((b,)), r.download((b, c.an.nifo)), s.download((b,), np.int32)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().qcbc_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class Server:
+    """Continuous-batching likelihood server (qcbc_server)."""
+
+    def __init__(self, budget_bytes, precision=FP64):
+        L = lib()
+        L.qcbc_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.qcbc_server_admit.argtypes = [_vp, ctypes.POINTER(Desc), _dp, _dp, _ip]
+        L.qcbc_server_evict.argtypes = [_vp, ctypes.c_int]
+        L.qcbc_server_submit.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.qcbc_server_wait.argtypes = [_vp, ctypes.c_int64, _dp, _dp, _ip, _dp, ctypes.c_int]
+        L.qcbc_server_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t), _ip,
+                                        ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_server_destroy.argtypes = [_vp]
+        h = _vp()
+        _check(L.qcbc_server_create(budget_bytes, precision, ctypes.byref(h)), "qcbc_server_create")
+        self.h = h
+        self.nd = {}
+
+    def admit(self, an, D, SN):
+        d = make_desc(an)
+        D = np.ascontiguousarray(D, np.float64)
+        SN = np.ascontiguousarray(SN, np.float64)
+        aid = ctypes.c_int(0)
+        code = lib().qcbc_server_admit(self.h, ctypes.byref(d), _d(D), _d(SN), ctypes.byref(aid))
+        if code != 0:
+            return code, None
+        self.nd[aid.value] = an.nifo
+        return 0, aid.value
+
+    def evict(self, aid):
+        return lib().qcbc_server_evict(self.h, aid)
+
+    def submit(self, aid, params):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        rid = ctypes.c_int64(0)
+        _check(lib().qcbc_server_submit(self.h, aid, _d(p), p.shape[0], ctypes.byref(rid)), "qcbc_server_submit")
+        return rid.value, p.shape[0], aid
+
+    def wait(self, ticket, timeout_ms=-1):
+        rid, B, aid = ticket
+        nd = self.nd.get(aid, 3)
+        logL = np.zeros(B)
+        rho = np.zeros((B, nd))
+        st = np.zeros(B, np.int32)
+        lat = ctypes.c_double(0)
+        code = lib().qcbc_server_wait(self.h, rid, _d(logL), _d(rho), _i(st), ctypes.byref(lat), timeout_ms)
+        return code, logL, rho, st, lat.value
+
+    def stats(self):
+        b = ctypes.c_size_t(0)
+        n = ctypes.c_int(0)
+        e = ctypes.c_longlong(0)
+        k = ctypes.c_longlong(0)
+        _check(lib().qcbc_server_stats(self.h, ctypes.byref(b), ctypes.byref(n), ctypes.byref(e), ctypes.byref(k)),
+               "qcbc_server_stats")
+        return {"bytes_in_use": b.value, "analyses": n.value, "evals": e.value, "batches": k.value}
+
+    def close(self):
+        if self.h:
+            lib().qcbc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class PointRecord(ctypes.Structure):
+    """Mirror of qcbc::Point (quickcbc_amd/csrc/qcbc_types.h)."""
+    _fields_ = ([(n, ctypes.c_int) for n in ("status", "kstart", "kend", "region_flags")] +
+                [(n, ctypes.c_double) for n in (
+                    "Msec fs fCut amp0 s6_scale is6_scale logMsec inv_eta amp_pre ai_23 ai_1 ai_43 ai_53 ai_2 "
+                    "ai_73 ai_83 ai_3 fmaxCalc col_f1 col_f2 col_f3 col_c0 col_c1 col_c2 col_c3 col_c4 fRD fDM "
+                    "fDMg3 mrd_num mrd_rate pi_init pi_23 pi_13 pi_13L pi_L pi_m13 pi_m23 pi_m1 pi_m53 pi_1 pi_43 "
+                    "pi_53 pi_2 fPhiMRD beta1 beta2 beta3 C1Int C2Int alpha1 alpha2 alpha3 alpha4 a5fRD C1MRD "
+                    "C2MRD t0 MfRef phi_precalc").split()] +
+                [(n, ctypes.c_double * 3) for n in ("Fs", "tshift", "lambda_", "Ascale")] +
+                [("eta", ctypes.c_double), ("finspin", ctypes.c_double)])
+
+
+def point_records(ctx, params):
+    L = lib()
+    L.qcbc_point_record_size.restype = ctypes.c_size_t
+    L.qcbc_point_records.argtypes = [_vp, _dp, ctypes.c_int, _vp]
+    assert L.qcbc_point_record_size() == ctypes.sizeof(PointRecord), "PointRecord layout out of date"
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+    recs = (PointRecord * p.shape[0])()
+    _check(L.qcbc_point_records(ctx.h, _d(p), p.shape[0], recs), "qcbc_point_records")
+    return list(recs)
