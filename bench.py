@@ -66,6 +66,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="autotune the launch shape first")
     ap.add_argument("--only-main", action="store_true", help="time the FP64 PT steps only (for profiling)")
+    ap.add_argument("--config", type=int, default=1,
+                    help="1: the PT-ensemble headline (default); 2: uneven persistent batch; "
+                         "3: mixed-precision accuracy; 4: serving stream")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--draws", type=int, default=100000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how the "
@@ -77,6 +82,19 @@ def main():
 
     name, cus, mem = Q.device_info()
     log("device:", name, cus, "CUs", "%.0f GB" % mem)
+    if args.config != 1:
+        from quickcbc_amd import bench_configs as C
+        if args.config == 2:
+            out = {"config": "uneven persistent batch", "fp64": C.uneven(precision=Q.FP64, log=log),
+                   "mixed": C.uneven(precision=Q.MIXED, log=log)}
+        elif args.config == 3:
+            out = {"config": "mixed precision vs FP64", **C.precision(n_draws=args.draws, log=log)}
+        elif args.config == 4:
+            out = {"config": "serving stream", **C.serving(seconds=args.serve_seconds, log=log)}
+        else:
+            raise SystemExit("unknown --config %d" % args.config)
+        print(json.dumps(out), flush=True)
+        return
 
     an = W.config1()
     inj = W.injection1()
@@ -89,7 +107,7 @@ def main():
     n_temps, n_chains = 32, 64
     B = n_temps * n_chains
     init = W.ensemble_around(inj, B, seed=99)
-    sigma = np.array([2e-5, 1e-4, 2e-3, 2e-3, 0.05, 2e-5, 0.01, 0.01, 0.01, 0.02, 0.01])
+    sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
     bytes_per_logl = an.nifo * nb * 24          # (Re d/S, Im d/S, 1/S) per bin and detector, FP64
     flops_per_logl = (60 + 19 * an.nifo) * nb    # waveform + projection + inner products

@@ -191,3 +191,17 @@ def probe(which, m1, m2, chi1, chi2, Mf):
     L.qo_probe.restype = ctypes.c_double
     L.qo_probe.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5
     return L.qo_probe(which, m1, m2, chi1, chi2, Mf)
+
+
+def exp_cr(x):
+    L = lib()
+    L.qo_exp_cr.restype = ctypes.c_double
+    L.qo_exp_cr.argtypes = [ctypes.c_double]
+    return L.qo_exp_cr(x)
+
+
+def pow_cr(x, y):
+    L = lib()
+    L.qo_pow_cr.restype = ctypes.c_double
+    L.qo_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
+    return L.qo_pow_cr(x, y)

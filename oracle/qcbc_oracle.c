@@ -43,6 +43,7 @@
 #include "qcbc_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -69,6 +70,17 @@
 #define QO_H22FAC 0.31539156525252
 
 enum { QO_OK = 0, QO_EDOM = 33, QO_EFAULT = 14, QO_ENOMEM = 12 };
+
+/* Correctly rounded exp, pow and log (quad precision rounded once) for the
+ * per-point mass and distance conversion (QuickCBC.c:3705-3727) and fbegin
+ * (Utilities.c:597-623).  glibc's exp and pow return a value one ulp away
+ * from the correctly rounded one for ~0.09% of arguments (measured against
+ * 60-digit decimal arithmetic); the Table-V fits amplify a one-ulp change of
+ * eta a hundredfold, so the oracle pins these few calls to the unique
+ * correctly rounded value. */
+double qo_exp_cr(double x) { return (double)expq((__float128)x); }
+double qo_pow_cr(double x, double y) { return (double)powq((__float128)x, (__float128)y); }
+double qo_log_cr(double x) { return (double)logq((__float128)x); }
 
 /* ---------------------------------------------------------------- powers */
 /* UsefulPowers and init_useful_powers, IMRPhenomD_internals.c:515-533 */
@@ -708,9 +720,9 @@ int qo_setup_flat(double m1, double m2, double chi1, double chi2, double distanc
 double qo_fbegin(const double *param) {
   const double PI = 3.1415926535897931159979634685442; /* Constants.h:2 */
   double gamma_E = 0.5772156649;
-  double Mchirp = exp(param[0]) / QO_MSUN_SI * QO_TSUN;
-  double M = exp(param[1]) / QO_MSUN_SI * QO_TSUN;
-  double eta = pow((Mchirp / M), (5.0 / 3.0));
+  double Mchirp = qo_exp_cr(param[0]) / QO_MSUN_SI * QO_TSUN;
+  double M = qo_exp_cr(param[1]) / QO_MSUN_SI * QO_TSUN;
+  double eta = qo_pow_cr((Mchirp / M), (5.0 / 3.0));
   double dm, m1, m2, chi1, chi2, chi, tc, th, th2, th3, th4, th5, th6;
   double PN1, PN15, PN2, PN25, PN3;
   if (eta > 0.25) eta = 0.25;
@@ -721,7 +733,7 @@ double qo_fbegin(const double *param) {
   chi2 = param[3];
   chi = (m1 * chi1 + m2 * chi2) / M;
   tc = param[5];
-  th = pow(eta * tc / (5.0 * M), -1.0 / 8.0);
+  th = qo_pow_cr(eta * tc / (5.0 * M), -1.0 / 8.0);
   th2 = th * th;
   th3 = th2 * th;
   th4 = th2 * th2;
@@ -734,7 +746,7 @@ double qo_fbegin(const double *param) {
   PN25 = -(7729.0 / 21504.0 - 13.0 / 256.0 * eta) * PI * th5;
   PN3 = (-720817631400877.0 / 288412611379200.0 + 53.0 / 200.0 * PI * PI + 107.0 / 280.0 * gamma_E +
          (25302017977.0 / 4161798144.0 - 451.0 / 2048.0 * PI * PI) * eta - 30913.0 / 1835008.0 * eta * eta +
-         235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * log(2.0 * th)) *
+         235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * qo_log_cr(2.0 * th)) *
         th6;
   return th3 / (8.0 * M * PI) * (1.0 + PN1 + PN15 + PN2 + PN25 + PN3);
 }
@@ -799,13 +811,13 @@ int qo_fulltemplates(int nifo, const int *labels, double gmst, double fref, doub
   double fs = qo_fbegin(params);
   double Tobs = 1.0 / freq[1];
   double sqT = sqrt(Tobs);
-  double mc = exp(params[0]), mt = exp(params[1]);
-  double eta = pow((mc / mt), (5.0 / 3.0));
+  double mc = qo_exp_cr(params[0]), mt = qo_exp_cr(params[1]);
+  double eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
   double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
   double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
   double phi0 = 0.5 * params[4];
   double ts = Tobs - params[5];
-  double distance = exp(params[6]);
+  double distance = qo_exp_cr(params[6]);
   double alpha = params[7], sindelta = params[8], psi = params[9], ciota = params[10];
   double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
   double *amp = malloc(sizeof(double) * (N / 2)), *phase = malloc(sizeof(double) * (N / 2));
@@ -961,11 +973,11 @@ int qo_num_threads(void) {
  * the N/2 grid, with fulltemplates' parameter handling (QuickCBC.c:3700-3731) */
 int qo_waveform_params(const double *params, int N, double Tobs, double fref, double *amp, double *phase) {
   double *freq = qo_make_freq(N, Tobs);
-  double mc = exp(params[0]), mt = exp(params[1]);
-  double eta = pow((mc / mt), (5.0 / 3.0));
+  double mc = qo_exp_cr(params[0]), mt = qo_exp_cr(params[1]);
+  double eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
   double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
   int st = qo_h22_ampphase(freq, N / 2, 0.5 * params[4], fref, mt * (1.0 + dm) / 2.0, mt * (1.0 - dm) / 2.0,
-                           params[2], params[3], exp(params[6]), amp, phase);
+                           params[2], params[3], qo_exp_cr(params[6]), amp, phase);
   free(freq);
   return st;
 }

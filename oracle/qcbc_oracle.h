@@ -104,6 +104,9 @@ int qo_setup_flat(double m1_msun, double m2_msun, double chi1, double chi2, doub
 int qo_spline_eval_flat(int which, const double *x, int n, double *y);
 int qo_waveform_params(const double *params, int N, double Tobs, double fref, double *amp, double *phase);
 double qo_probe(int which, double m1, double m2, double chi1, double chi2, double Mf);
+double qo_exp_cr(double x);
+double qo_pow_cr(double x, double y);
+double qo_log_cr(double x);
 int qo_num_threads(void);
 
 #ifdef __cplusplus

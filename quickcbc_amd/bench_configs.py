@@ -1,0 +1,223 @@
+"""Benchmarks of the batched, mixed-precision and serving forms.
+
+Each function builds its synthetic workload on the GPU (templates from the
+library, seeded Gaussian noise coloured by the analytic PSD), runs it, and
+returns a dict of measurements.  bench.py calls them with --config.
+
+  uneven(...)     12 analyses with Tobs in {4..64} s, fs in {2048, 4096} Hz,
+                  1-3 detectors and 64/256/1024 points each, in one persistent
+                  queue launch: fixed launch shape vs autotuned vs hipGraph
+                  replay vs one launch per analysis.
+  precision(...)  mixed precision against FP64 at Tobs = 16 s, 3 detectors:
+                  |dlogL| over many draws and over injections at network SNR
+                  10, 25 and 80, and the throughput ratio.
+  serving(...)    Poisson arrivals of analyses (sizes as in uneven) issuing
+                  PT-step requests of 512-4096 points, admitted against a
+                  device-memory budget, continuously batched; p50/p99 request
+                  latency and aggregate logL evaluations per second.
+"""
+import threading
+import time
+
+import numpy as np
+
+from . import qcbc as Q
+from . import workloads as W
+
+SIGMA = np.array([2e-5, 1e-4, 2e-3, 2e-3, 0.05, 2e-5, 0.01, 0.01, 0.01, 0.02, 0.01])
+
+
+def _analysis(Tobs, fs, labels):
+    scales = [2.0 if l == 2 else 1.0 for l in labels]
+    fmax = min(2048.0, fs / 2)
+    return W.Analysis(labels=list(labels), Tobs=float(Tobs), fs=float(fs), fmin=20.0, fmax=fmax, psd_scale=scales)
+
+
+def uneven_specs(seed=2026, n=12):
+    """n analyses of mixed shape: (an, injection, n_points)."""
+    r = np.random.default_rng(seed)
+    det_sets = [[0], [1], [0, 1], [1, 2], [0, 1, 2]]
+    out = []
+    for i in range(n):
+        Tobs = float(r.choice([4, 8, 16, 32, 64]))
+        fs = float(r.choice([2048, 4096]))
+        labels = det_sets[r.integers(len(det_sets))]
+        an = _analysis(Tobs, fs, labels)
+        # masses from 1.2+1.2 to 60+50 Msun; heavier systems need shorter segments to merge inside
+        u = r.uniform()
+        m1 = 1.2 * (60.0 / 1.2) ** u
+        m2 = m1 * r.uniform(0.5, 1.0)
+        m2 = max(m2, 1.2)
+        tc = Tobs - 2.0
+        dl = 40.0 * (m1 / 1.2) ** 0.8
+        inj = W.make_params(m1, m2, r.uniform(-0.3, 0.3), r.uniform(-0.3, 0.3), r.uniform(0, 2 * np.pi), tc, dl,
+                            r.uniform(0, 2 * np.pi), r.uniform(-1, 1), r.uniform(0, np.pi), r.uniform(-1, 1))
+        npts = int(r.choice([64, 256, 1024]))
+        out.append((an, inj, npts))
+    return out
+
+
+def build_context(an, inj, noise_seed):
+    ctx = Q.Context(an)
+    h, st = ctx.templates(inj[None, :])
+    D = np.ascontiguousarray(h[0] + W.noise(an, noise_seed))
+    ctx.bind(D, an.SN())
+    return ctx, D
+
+
+def uneven(reps=20, precision=Q.FP64, log=print):
+    specs = uneven_specs()
+    ctxs, Ps, bins = [], [], []
+    for i, (an, inj, npts) in enumerate(specs):
+        ctx, D = build_context(an, inj, 500 + i)
+        ctxs.append(ctx)
+        Ps.append(W.ensemble_around(inj, npts, seed=600 + i, scale=3.0))
+        bins.append(an.imax - an.imin)
+    total = sum(p.shape[0] for p in Ps)
+    res = {"analyses": len(specs), "points": total,
+           "shapes": [{"Tobs": a.Tobs, "fs": a.fs, "det": a.nifo, "points": n, "bins": b}
+                      for (a, _, n), b in zip(specs, bins)]}
+    m = Q.Multi(ctxs, [p.shape[0] for p in Ps], precision)
+    m.set_params(Ps)
+
+    def timeit(graph=False):
+        m.run(graph=graph)
+        ts = [m.run(graph=graph) for _ in range(reps)]
+        return float(np.median(ts))
+
+    m.set_shape(4, 256, 4096)
+    t_fixed = timeit()
+    res["fixed_shape"] = {"shape": m.get_shape()[:3], "ms": t_fixed, "logl_per_s": total / t_fixed * 1e3}
+    best, _ = m.autotune(reps=3)
+    t_tuned = timeit()
+    res["autotuned"] = {"shape": best, "ms": t_tuned, "logl_per_s": total / t_tuned * 1e3}
+    t_graph = timeit(graph=True)
+    res["autotuned_graph"] = {"ms": t_graph, "logl_per_s": total / t_graph * 1e3}
+    ref = [r[0].copy() for r in m.results()]
+    # one launch per analysis, same points, through the single-analysis API
+    for c, p in zip(ctxs, Ps):
+        c.loglike(p, precision)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs = [c.loglike(p, precision)[0] for c, p in zip(ctxs, Ps)]
+    t_sep = (time.perf_counter() - t0) / reps * 1e3
+    res["per_analysis_launches"] = {"ms": t_sep, "logl_per_s": total / t_sep * 1e3}
+    dev = max(float(np.max(np.abs(a - b))) for a, b in zip(ref, outs))
+    res["max_abs_dlogl_queue_vs_per_analysis"] = dev
+    m.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
+def precision(n_draws=100000, log=print):
+    an = W.config1()
+    res = {"draws": n_draws}
+    ctx = Q.Context(an)
+    out = {}
+    for snr_target in (None, 10.0, 25.0, 80.0):
+        inj = W.injection1()
+        if snr_target is not None:
+            # rescale the distance so the injection has the requested network SNR
+            h, _ = ctx.templates(inj[None, :])
+            SN = an.SN()
+            hh = 0.0
+            for d in range(an.nifo):
+                k = np.arange(an.imin, an.imax)
+                hh += 4 * np.sum((h[0, d, k] ** 2 + h[0, d, an.N - k] ** 2) / SN[d, k])
+            inj = inj.copy()
+            inj[6] += np.log(np.sqrt(hh) / snr_target)
+        ctx2, D = build_context(an, inj, 4242)
+        r = np.random.default_rng(17)
+        n = n_draws if snr_target is None else max(4096, n_draws // 10)
+        near = W.ensemble_around(inj, n // 2, seed=18, scale=5.0)
+        far = W.ensemble_around(inj, n - n // 2, seed=19, scale=1.0)
+        far[:, 0] += r.uniform(-0.05, 0.05, far.shape[0])
+        far[:, 1] = np.maximum(far[:, 1], far[:, 0] - np.log(0.25 ** 0.6) + 1e-9)
+        P = np.vstack([near, far])
+        L64 = np.concatenate([ctx2.loglike(P[i:i + 8192], Q.FP64)[0] for i in range(0, n, 8192)])
+        L32 = np.concatenate([ctx2.loglike(P[i:i + 8192], Q.MIXED)[0] for i in range(0, n, 8192)])
+        err = np.abs(L32 - L64)
+        key = "prior_and_posterior_draws" if snr_target is None else "snr_%g" % snr_target
+        out[key] = {"n": int(n), "max": float(err.max()), "p99": float(np.percentile(err, 99)),
+                    "median": float(np.median(err)), "logl_range": [float(L64.min()), float(L64.max())]}
+        log(key, out[key])
+        ctx2.close()
+    res["abs_dlogl_mixed_vs_fp64"] = out
+    ctx.close()
+    return res
+
+
+def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precision=Q.MIXED, seed=7, log=print):
+    """Poisson arrivals of analyses; each issues lifetime_steps PT-step
+    requests of 512-4096 points and leaves.  Returns latency percentiles and
+    the aggregate evaluation rate."""
+    specs = uneven_specs(seed=seed, n=16)
+    pool = []
+    for i, (an, inj, _) in enumerate(specs):
+        ctx, D = build_context(an, inj, 900 + i)
+        ctx.close()
+        pool.append((an, inj, D, an.SN()))
+    srv = Q.Server(int(budget_gb * 1e9), precision)
+    rng = np.random.default_rng(seed)
+    lat, lock = [], threading.Lock()
+    counters = {"admitted": 0, "refused": 0, "requests": 0, "points": 0, "errors": 0}
+    stop = threading.Event()
+    threads = []
+
+    def client(aid, an, inj, cseed):
+        r = np.random.default_rng(cseed)
+        B = int(r.choice([512, 1024, 2048, 4096]))
+        x = W.ensemble_around(inj, B, seed=cseed)
+        for _ in range(lifetime_steps):
+            if stop.is_set():
+                break
+            y = x + r.standard_normal(x.shape) * SIGMA * 0.3
+            y[:, 2:4] = np.clip(y[:, 2:4], -0.95, 0.95)
+            y[:, 8] = np.clip(y[:, 8], -0.999, 0.999)
+            y[:, 10] = np.clip(y[:, 10], -0.999, 0.999)
+            y[:, 0] = np.minimum(y[:, 0], y[:, 1] + np.log(0.25 ** 0.6) - 1e-9)
+            t = srv.submit(aid, y)
+            code, L, R, S, ms = srv.wait(t)
+            with lock:
+                if code == 0:
+                    lat.append(ms)
+                    counters["requests"] += 1
+                    counters["points"] += B
+                else:
+                    counters["errors"] += 1
+            x = y
+        srv.evict(aid)
+
+    t_start = time.perf_counter()
+    next_arrival = t_start
+    k = 0
+    while time.perf_counter() - t_start < seconds:
+        now = time.perf_counter()
+        if now < next_arrival:
+            time.sleep(min(0.005, next_arrival - now))
+            continue
+        next_arrival += rng.exponential(1.0 / rate)
+        an, inj, D, SN = pool[k % len(pool)]
+        code, aid = srv.admit(an, D, SN)
+        if code != 0:
+            counters["refused"] += 1
+        else:
+            counters["admitted"] += 1
+            th = threading.Thread(target=client, args=(aid, an, inj, 1000 + k), daemon=True)
+            th.start()
+            threads.append(th)
+        k += 1
+    t_end = time.perf_counter()
+    n_done = counters["points"]
+    stop.set()
+    for th in threads:
+        th.join(timeout=30)
+    st = srv.stats()
+    srv.close()
+    lat = np.asarray(lat)
+    return {"seconds": t_end - t_start, "arrival_rate_per_s": rate, "budget_gb": budget_gb,
+            "precision": "mixed" if precision == Q.MIXED else "fp64",
+            "latency_ms_p50": float(np.percentile(lat, 50)) if lat.size else None,
+            "latency_ms_p99": float(np.percentile(lat, 99)) if lat.size else None,
+            "logl_per_s": n_done / (t_end - t_start), "batches": st["batches"], **counters}

@@ -44,6 +44,69 @@ constexpr double PI_TWO_THIRDS = 0x1.129052c2c6753p+1;
 constexpr double PI_FOUR_THIRDS = 0x1.2679428626b38p+2;
 constexpr double PI_FIVE_THIRDS = 0x1.af48a3d921bdbp+2;
 
+// --------------------------------------------- correctly rounded exp / pow
+// The per-point masses come from exp(ln Mc), exp(ln Mt) and
+// pow(Mc/Mt, 5/3) (QuickCBC.c:3705-3718); the Table-V fits in eta and chi
+// amplify a one-ulp change of eta a hundredfold.  These calls are therefore
+// evaluated in double-double and rounded once, which gives the correctly
+// rounded result (as glibc's < 0.52-ulp exp and pow return it for all but
+// vanishingly rare arguments).  Per point only; nothing per bin.
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ DD dd_two_sum(double a, double b) {
+  double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_fast_two_sum(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+  DD s = dd_two_sum(a.hi, b.hi);
+  double t = s.lo + (a.lo + b.lo);
+  return dd_fast_two_sum(s.hi, t);
+}
+__device__ __forceinline__ DD dd_mul(DD a, DD b) {
+  double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
+  return dd_fast_two_sum(p, e);
+}
+__device__ __forceinline__ DD dd_mul_d(DD a, double b) {
+  double p = a.hi * b;
+  double e = fma(a.hi, b, -p) + a.lo * b;
+  return dd_fast_two_sum(p, e);
+}
+__device__ __forceinline__ DD dd_div_d(DD a, double b) {
+  double q = a.hi / b;
+  double p = q * b, pe = fma(q, b, -p);
+  double r = ((a.hi - p) - pe + a.lo) / b;
+  return dd_fast_two_sum(q, r);
+}
+// exp of a double-double: x = k ln2 + r, exp(r/512) by its Taylor series
+// (|r/512| < 7e-4, 9 terms), squared nine times, scaled by 2^k.
+__device__ inline DD dd_exp(DD x) {
+  const double LN2_HI = 0x1.62e42fefa39efp-1, LN2_LO = 0x1.abc9e3b39803fp-56;
+  const double k = rint(x.hi * 1.4426950408889634);
+  DD r = dd_add(x, {-k * LN2_HI, -fma(k, LN2_HI, -k * LN2_HI)});
+  r = dd_add(r, {-k * LN2_LO, 0.0});
+  r = {r.hi * (1.0 / 512.0), r.lo * (1.0 / 512.0)};
+  DD e = {1.0, 0.0};  // Horner: 1 + r/1 (1 + r/2 (1 + ... (1 + r/9)))
+  for (int n = 9; n >= 1; n--) e = dd_add({1.0, 0.0}, dd_div_d(dd_mul(r, e), (double)n));
+  DD em1 = dd_add(e, {-1.0, 0.0});  // exp(r/512) - 1, squared as (1+m)^2 = 1 + (2m + m^2)
+  for (int i = 0; i < 9; i++) em1 = dd_add(dd_mul_d(em1, 2.0), dd_mul(em1, em1));
+  DD v = dd_add({1.0, 0.0}, em1);
+  return {ldexp(v.hi, (int)k), ldexp(v.lo, (int)k)};
+}
+// log by one Newton step on dd_exp from the double log: y + (x e^-y - 1).
+__device__ inline DD dd_log(double x) {
+  const double y0 = log(x);
+  DD t = dd_add(dd_mul_d(dd_exp({-y0, 0.0}), x), {-1.0, 0.0});
+  return dd_add({y0, 0.0}, t);
+}
+__device__ inline double exp_cr(double x) { return dd_exp({x, 0.0}).hi; }
+__device__ inline double pow_cr(double x, double y) { return dd_exp(dd_mul_d(dd_log(x), y)).hi; }
+
 // QNM spline slopes, filled once per device by qcbc_qnm_init_kernel.
 extern __device__ double g_qnm_slope[2][QCBC_QNM_N];
 
@@ -291,15 +354,15 @@ __device__ inline double delay(double alpha, double sindelta, double gmst, int i
 // fbegin (Utilities.c:587-629): 3PN frequency at the segment start.
 __device__ inline double fbegin(const double* prm) {
   const double gamma_E = 0.5772156649;
-  double Mc = exp(prm[0]) / MSUN_SI * TSUN;
-  double M = exp(prm[1]) / MSUN_SI * TSUN;
-  double eta = pow(Mc / M, 5.0 / 3.0);
+  double Mc = exp_cr(prm[0]) / MSUN_SI * TSUN;
+  double M = exp_cr(prm[1]) / MSUN_SI * TSUN;
+  double eta = pow_cr(Mc / M, 5.0 / 3.0);
   if (eta > 0.25) eta = 0.25;
   double dm = sqrt(1.0 - 4.0 * eta);
   double m1 = M * (1.0 + dm) / 2.0, m2 = M * (1.0 - dm) / 2.0;
   double chi1 = prm[2], chi2 = prm[3];
   double chi = (m1 * chi1 + m2 * chi2) / M;
-  double th = pow(eta * prm[5] / (5.0 * M), -1.0 / 8.0);
+  double th = pow_cr(eta * prm[5] / (5.0 * M), -1.0 / 8.0);
   double th2 = th * th, th3 = th2 * th, th4 = th2 * th2, th5 = th2 * th3, th6 = th3 * th3;
   double PN1 = (11.0 / 32.0 * eta + 743.0 / 2688.0) * th2;
   double PN15 = -3.0 * PI / 10.0 * th3 + (1.0 / 160.0) * (113.0 * chi - 38.0 * eta * (chi1 + chi2)) * th3;
@@ -308,7 +371,7 @@ __device__ inline double fbegin(const double* prm) {
   double PN25 = -(7729.0 / 21504.0 - 13.0 / 256.0 * eta) * PI * th5;
   double PN3 = (-720817631400877.0 / 288412611379200.0 + 53.0 / 200.0 * PI * PI + 107.0 / 280.0 * gamma_E +
                 (25302017977.0 / 4161798144.0 - 451.0 / 2048.0 * PI * PI) * eta - 30913.0 / 1835008.0 * eta * eta +
-                235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * log(2.0 * th)) *
+                235925.0 / 1769472.0 * eta * eta * eta + 107.0 / 280.0 * dd_log(2.0 * th).hi) *
                th6;
   return th3 / (8.0 * M * PI) * (1.0 + PN1 + PN15 + PN2 + PN25 + PN3);
 }
@@ -567,12 +630,12 @@ __device__ inline int point_setup(const double* prm, const AnalysisConst& an, Po
   c.status = ST_OK;
   c.kstart = c.kend = an.imin;
   c.fs = fbegin(prm);
-  const double mc = exp(prm[0]), mt = exp(prm[1]);
-  const double eta_in = pow(mc / mt, 5.0 / 3.0);
+  const double mc = exp_cr(prm[0]), mt = exp_cr(prm[1]);
+  const double eta_in = pow_cr(mc / mt, 5.0 / 3.0);
   const double dm = eta_in > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta_in);
   const double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
   const double ts = an.Tobs - prm[5];
-  int st = phenomd_setup(m1_SI, m2_SI, prm[2], prm[3], exp(prm[6]), 0.5 * prm[4], an.fref, an.fmin0, c);
+  int st = phenomd_setup(m1_SI, m2_SI, prm[2], prm[3], exp_cr(prm[6]), 0.5 * prm[4], an.fref, an.fmin0, c);
   if (st != ST_OK) {
     c.status = st;
     return st;

@@ -127,11 +127,13 @@ def test_setup_records_vs_oracle(ctx_cfg, oracle):
     worst = (0.0, None)
     for b, r in enumerate(recs):
         p = P[b]
-        mc, mt = np.exp(p[0]), np.exp(p[1])
-        eta = (mc / mt) ** (5 / 3)
+        # the mass conversion of fulltemplates (QuickCBC.c:3705-3718), correctly rounded as in
+        # the oracle and the GPU set-up
+        mc, mt = oracle.exp_cr(p[0]), oracle.exp_cr(p[1])
+        eta = oracle.pow_cr(mc / mt, 5.0 / 3.0)
         dm = 0.0 if eta > 0.25 else np.sqrt(1 - 4 * eta)
         m1, m2 = mt * (1 + dm) / 2 / W.MSUN_SI, mt * (1 - dm) / 2 / W.MSUN_SI
-        st, ref = oracle.setup(m1, m2, p[2], p[3], np.exp(p[6]), 0.5 * p[4], an.fref)
+        st, ref = oracle.setup(m1, m2, p[2], p[3], oracle.exp_cr(p[6]), 0.5 * p[4], an.fref)
         assert st == 0 and r.status == 0
         for name in ORACLE_FIELDS:
             g = getattr(r, name)

@@ -304,3 +304,16 @@ def test_noise_normalisation(oracle):
     tot = sum(oracle.fourier_nwip(n[d], n[d], SN[d], an.imin, an.imax, an.N) for d in range(an.nifo))
     nb = an.nifo * (an.imax - an.imin)
     assert tot / nb == pytest.approx(2.0, abs=4 * 2.0 / np.sqrt(nb))
+
+
+def test_mass_conversion_is_correctly_rounded(oracle):
+    """The oracle's exp/pow for the mass and distance conversion return the
+    correctly rounded double (checked against 60-digit decimal arithmetic)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    rng = np.random.default_rng(8)
+    for x in np.concatenate([rng.uniform(60.0, 75.0, 300), rng.uniform(-3.0, 3.0, 300)]):
+        assert oracle.exp_cr(x) == float(Decimal(float(x)).exp())
+    for y in rng.uniform(0.05, 0.999, 300):
+        exact = float((Decimal(float(y)).ln() * Decimal(5.0 / 3.0)).exp())
+        assert oracle.pow_cr(y, 5.0 / 3.0) == exact
