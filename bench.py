@@ -70,6 +70,8 @@ def main():
                     help="1: the PT-ensemble headline (default); 2: uneven persistent batch; "
                          "3: mixed-precision accuracy; 4: serving stream")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-steps", type=int, default=200, help="PT-step requests per analysis")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="analysis arrivals per second")
     ap.add_argument("--draws", type=int, default=100000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -90,7 +92,8 @@ def main():
         elif args.config == 3:
             out = {"config": "mixed precision vs FP64", **C.precision(n_draws=args.draws, log=log)}
         elif args.config == 4:
-            out = {"config": "serving stream", **C.serving(seconds=args.serve_seconds, log=log)}
+            out = {"config": "serving stream", **C.serving(seconds=args.serve_seconds, rate=args.serve_rate,
+                                                           lifetime_steps=args.serve_steps, log=log)}
         else:
             raise SystemExit("unknown --config %d" % args.config)
         print(json.dumps(out), flush=True)
@@ -115,6 +118,16 @@ def main():
     if args.autotune:
         for prec in (Q.FP64, Q.MIXED):
             log("autotune", prec, ctx.autotune(prec, B))
+
+    if args.only_main:
+        # a copy of known size beside the timed kernel, for scaling byte counters
+        import ctypes
+        nbytes = 1 << 30
+        a, b = Q.DeviceBuffer(nbytes), Q.DeviceBuffer(nbytes)
+        Q.lib().qcbc_dev_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        Q._check(Q.lib().qcbc_dev_copy(b.ptr, a.ptr, nbytes), "qcbc_dev_copy")
+        a.free()
+        b.free()
 
     # ---- headline: FP64 fused path, PT steps, kernel timed with HIP events
     pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64)

@@ -249,6 +249,9 @@ int qcbc_dev_alloc(size_t bytes, void **ptr);
 int qcbc_dev_free(void *ptr);
 int qcbc_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int qcbc_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain kernel (bytes a multiple of 16): the
+ * reference of known size for hardware byte counters. */
+int qcbc_dev_copy(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------ *
  * Part 5: serving loop                                                *

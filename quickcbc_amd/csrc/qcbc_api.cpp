@@ -572,3 +572,10 @@ int qcbc_point_records(qcbc_ctx* c, const double* params, int B, void* records) 
   HIPCHK(hipStreamSynchronize(c->stream));
   return QCBC_OK;
 }
+
+int qcbc_dev_copy(void* dst, const void* src, size_t bytes) {
+  if (!dst || !src || (bytes & 15)) return QCBC_EINVAL;
+  HIPCHK(launch_copy(src, dst, bytes, nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  return QCBC_OK;
+}

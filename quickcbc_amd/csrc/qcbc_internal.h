@@ -46,4 +46,5 @@ hipError_t launch_pt_accept(double* x, double* L, const double* y, const double*
 hipError_t launch_pt_swap(double* x, double* L, int n_temps, int n_chains, double ladder, uint64_t seed,
                           unsigned long long* step, unsigned long long* swaps, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 }  // namespace qcbc

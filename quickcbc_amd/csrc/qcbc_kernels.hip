@@ -499,3 +499,18 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
 }
 
 }  // namespace qcbc
+
+namespace qcbc {
+// Plain device copy, 16 bytes per lane: the calibration reference for the HBM
+// byte counters (a copy of n bytes reads n and writes n).
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  size_t n16 = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+  return hipGetLastError();
+}
+}  // namespace qcbc
