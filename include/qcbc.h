@@ -178,7 +178,7 @@ size_t qcbc_point_record_size(void);
 int qcbc_point_records(qcbc_ctx *ctx, const double *params, int B, void *records);
 
 /* Kernel launch shape of the fused likelihood for this context:
- * chains per block, bins per LDS tile, bin splits (0 = choose). */
+ * chains per block (1..8), bins per LDS tile (128, 256 or 512), bin splits (0 = choose). */
 int qcbc_ctx_set_shape(qcbc_ctx *ctx, int precision, int chains_per_block, int tile, int splits);
 int qcbc_ctx_get_shape(qcbc_ctx *ctx, int precision, int *chains_per_block, int *tile,
                        int *splits);
@@ -230,7 +230,7 @@ int qcbc_multi_loglike(qcbc_ctx *const *ctxs, const double *const *d_params, con
 typedef struct qcbc_multi qcbc_multi;
 int qcbc_multi_create(qcbc_ctx *const *ctxs, const int *B, int n, int precision,
                       qcbc_multi **out);
-/* points per block (1..8), bins per LDS tile (multiple of 64), bins per split */
+/* points per block (1..8), bins per LDS tile (128, 256 or 512), bins per split */
 int qcbc_multi_set_shape(qcbc_multi *m, int chains_per_block, int tile, int split_bins);
 int qcbc_multi_get_shape(qcbc_multi *m, int *chains_per_block, int *tile, int *split_bins,
                          int *n_items);

@@ -280,7 +280,7 @@ int qcbc::loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, dou
     c->launches++;
     if (c->ev_pairs.size() >= 512) harvest_events(c);
   }
-  HIPCHK(launch_finalize(c->d_partial, in.splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED, c->d_wref, d_logL,
+  HIPCHK(launch_finalize(c->d_partial, in.splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED ? FIN_MIXED : FIN_FP64, c->d_wref, d_logL,
                          d_rho, c->stream));
   return QCBC_OK;
 }
@@ -342,7 +342,7 @@ int qcbc_loglike_plain(qcbc_ctx* c, const double* params, int B, double* logL, d
     HIPCHK(launch_project(c->d_pts + b, 1, c->d_amp, c->d_phase, nd, N, c->desc.Tobs, c->an.sqT, c->d_hw, c->stream));
     HIPCHK(launch_nwip(c->d_hw, c->d_D, c->d_SN, 1, nd, N, c->an.imin, c->an.imax, QCBC_PLAIN_BLOCKS,
                        c->d_plain_partial, c->stream));
-    HIPCHK(launch_finalize(c->d_plain_partial, QCBC_PLAIN_BLOCKS, nd, c->d_pts + b, 1, 0, c->d_wref, c->d_logL + b,
+    HIPCHK(launch_finalize(c->d_plain_partial, QCBC_PLAIN_BLOCKS, nd, c->d_pts + b, 1, FIN_RAW, c->d_wref, c->d_logL + b,
                            c->d_rho + (size_t)nd * b, c->stream));
   }
   HIPCHK(hipMemcpyAsync(logL, c->d_logL, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
@@ -401,7 +401,7 @@ int qcbc_templates(qcbc_ctx* c, const double* params, int B, double* hwave, int*
 // ------------------------------------------------------ shapes / timing
 int qcbc_ctx_set_shape(qcbc_ctx* c, int prec, int cpb, int tile, int splits) {
   if (!c || (prec != 0 && prec != 1)) return QCBC_EINVAL;
-  if (cpb < 1 || cpb > 8 || tile < 64 || (tile & 63) || splits < 0 || splits > QCBC_MAX_SPLITS) return QCBC_EINVAL;
+  if (cpb < 1 || cpb > 8 || !ll_tile_ok(tile) || splits < 0 || splits > QCBC_MAX_SPLITS) return QCBC_EINVAL;
   if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) return QCBC_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   c->cpb[prec] = cpb;

@@ -624,6 +624,83 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
   return ST_OK;
 }
 
+// Smallest k in [lo, hi) with Msec (k / Tobs) >= thr, else hi: the region
+// tests of amplitude() and phase() on a bin's own Mf = Msec f_k, with f_k
+// as the band tables hold it; Mf does not decrease with k.
+__device__ inline int first_bin_at(double thr, double Msec, double Tobs, int lo, int hi) {
+  const double e = ceil(thr / Msec * Tobs);
+  int k = !(e == e) ? hi : (e < (double)lo ? lo : (e > (double)hi ? hi : (int)e));
+  while (k > lo && Msec * ((double)(k - 1) / Tobs) >= thr) k--;
+  while (k < hi && !(Msec * ((double)k / Tobs) >= thr)) k++;
+  return k;
+}
+
+// Constants of the fused likelihood kernels: region boundaries as bin
+// indices, and the amplitude and the projected argument with every
+// per-point constant folded in (the projected detector-0 argument
+// f tshift_0 + (lambda_0 - Psi)/2 pi in turns, Psi = phi(Mf) - t0 (Mf - MfRef)
+// - phi_precalc as phase() evaluates it).
+__device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
+  c.seg_a1 = first_bin_at(AMP_JOIN, c.Msec, an.Tobs, c.kstart, c.kend);
+  c.seg_a2 = first_bin_at(c.fmaxCalc, c.Msec, an.Tobs, c.kstart, c.kend);
+  c.seg_p1 = first_bin_at(PHI_JOIN, c.Msec, an.Tobs, c.kstart, c.kend);
+  c.seg_p2 = first_bin_at(c.fPhiMRD, c.Msec, an.Tobs, c.kstart, c.kend);
+  const double K = -INV_TPI;
+  const double G = (LOG_PI + c.logMsec) / 3.0;  // log v = G + log(f) / 3
+  const double lin = c.tshift[0] / c.Msec;      // f tshift_0 = Mf tshift_0 / Msec
+  const double base = (c.lambda[0] + c.phi_precalc - c.t0 * c.MfRef) * INV_TPI;
+  // inspiral (PhiInsAnsatzInt)
+  c.ins_q0 = base - (c.pi_init + c.pi_L * G) * INV_TPI;
+  c.ins_q1 = lin + (c.t0 - c.pi_1 * c.inv_eta) * INV_TPI;
+  c.ins_q13 = K * (c.pi_13 + c.pi_13L * G);
+  c.ins_q13L = K * c.pi_13L / 3.0;
+  c.ins_qL = K * c.pi_L / 3.0;
+  c.ins_q23 = K * c.pi_23;
+  c.ins_q43 = K * c.inv_eta * c.pi_43;
+  c.ins_q53 = K * c.inv_eta * c.pi_53;
+  c.ins_q2 = K * c.inv_eta * c.pi_2;
+  c.ins_qm13 = K * c.pi_m13;
+  c.ins_qm23 = K * c.pi_m23;
+  c.ins_qm1 = K * c.pi_m1;
+  c.ins_qm53 = K * c.pi_m53;
+  // intermediate: inv_eta (beta1 Mf - beta3 / (3 Mf^3) + beta2 log Mf) + C1Int + C2Int Mf
+  c.int_qL = K * c.inv_eta * c.beta2;
+  c.int_q0 = base - c.C1Int * INV_TPI + c.int_qL * c.logMsec;
+  c.int_q1 = lin + (c.t0 - c.C2Int - c.inv_eta * c.beta1) * INV_TPI;
+  c.int_qm3 = c.inv_eta * c.beta3 * INV_TPI / 3.0;
+  // merger-ringdown: inv_eta (-alpha2 / Mf + 4/3 alpha3 Mf^(3/4) + alpha1 Mf
+  //                  + alpha4 atan((Mf - a5fRD) / fDM)) + C1MRD + C2MRD Mf
+  c.mrd_q0 = base - c.C1MRD * INV_TPI;
+  c.mrd_q1 = lin + (c.t0 - c.C2MRD - c.inv_eta * c.alpha1) * INV_TPI;
+  c.mrd_qm1 = c.inv_eta * c.alpha2 * INV_TPI;
+  c.mrd_q34 = K * c.inv_eta * (4.0 / 3.0) * c.alpha3;
+  c.mrd_qat = K * c.inv_eta * c.alpha4;
+  c.mrd_a5 = c.a5fRD;
+  c.mrd_ifDM = 1.0 / c.fDM;
+  // amplitude, amp_pre folded in
+  c.ia0 = c.amp_pre;
+  c.ia23 = c.amp_pre * c.ai_23;
+  c.ia1 = c.amp_pre * c.ai_1;
+  c.ia43 = c.amp_pre * c.ai_43;
+  c.ia53 = c.amp_pre * c.ai_53;
+  c.ia2 = c.amp_pre * c.ai_2;
+  c.ia73 = c.amp_pre * c.ai_73;
+  c.ia83 = c.amp_pre * c.ai_83;
+  c.ia3 = c.amp_pre * c.ai_3;
+  c.ib0 = c.amp_pre * c.col_c0;
+  c.ib1 = c.amp_pre * c.col_c1;
+  c.ib2 = c.amp_pre * c.col_c2;
+  c.ib3 = c.amp_pre * c.col_c3;
+  c.ib4 = c.amp_pre * c.col_c4;
+  c.mnum = c.amp_pre * c.mrd_num;
+  c.mg2 = c.fDMg3 * c.fDMg3;
+  for (int d = 0; d < 3; d++) {
+    const bool on = d < an.n_det;
+    c.dt[d] = on ? c.tshift[d] - c.tshift[0] : 0.0;
+    c.dl[d] = on ? (c.lambda[d] - c.lambda[0]) * INV_TPI : 0.0;
+  }
+}
+
 // fulltemplates' parameter handling and projection constants
 // (QuickCBC.c:3700-3750) for one 11-entry parameter vector.
 __device__ inline int point_setup(const double* prm, const AnalysisConst& an, Point& c) {
@@ -679,6 +756,7 @@ __device__ inline int point_setup(const double* prm, const AnalysisConst& an, Po
     c.kstart = k0;
     c.kend = k1;
   }
+  fused_setup(an, c);
   return ST_OK;
 }
 

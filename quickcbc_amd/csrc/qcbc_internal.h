@@ -18,11 +18,15 @@ hipError_t launch_setup_h22(double m1, double m2, double c1, double c2, double d
 hipError_t launch_bind(const double* D, const double* SN, int nd, int N, double Tobs, int imin, int nb, int nbp,
                        const double* wref, double* p64, float* p32, hipStream_t s);
 size_t ll_lds_bytes(int precision, int nd, int tile);
+// LDS tile lengths the fused kernels are built for: 128, 256, 512 bins
+bool ll_tile_ok(int tile);
 hipError_t launch_ll(int precision, const LLInst& in, int cpb, int tile, hipStream_t s);
 hipError_t launch_ll_queue(int precision, const LLInst* d_insts, const LLItem* d_items, int n_items, int* d_counter,
                            int max_nd, int cpb, int tile, int blocks, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s);
-hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mixed,
+// partial sums as written by: the plain path (raw), k_ll / k_ll_queue FP64, mixed
+enum { FIN_RAW = 0, FIN_FP64 = 1, FIN_MIXED = 2 };
+hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mode,
                            const double* wref, double* logL, double* rho, hipStream_t s);
 hipError_t launch_waveform(const Point* pts, int B, int nhalf, double Tobs, double* amp, double* phs, hipStream_t s);
 hipError_t launch_waveform_freqs(const Point* pts, const double* freq, int n, double* amp, double* phs,

@@ -21,38 +21,45 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// sin(2 pi r) and cos(2 pi r) for a phase r given in turns.  v_fract_f64
-// drops the whole turns (exactly), rounding to the nearest quarter turn leaves
-// |t| <= 1/8 (also exact), and the Taylor series of sin and cos through
-// x^15 and x^16 on |x| = |2 pi t| <= pi/4 are below 5e-17 truncation error.
-// This replaces the general sincos, whose large-argument reduction the
-// ~1e5 rad projected phase would otherwise pull in.
+// Hides a constant from the optimiser in a scalar register, so that an FMA
+// takes it as its (negated) third operand instead of copying it into a vector
+// register first.  The asm body is empty.
+__device__ __forceinline__ double sreg(double v) {
+  asm("" : "+s"(v));
+  return v;
+}
+
+// sin(2 pi r) and cos(2 pi r) for a phase r given in turns.  t = r - rint(r)
+// (exact, |t| <= 1/2); degree-6 polynomials in t^2 give the quarter angle,
+// sin(pi t / 2) = t P(t^2) and cos(pi t / 2) = Q(t^2), to below 2 ulp
+// (Chebyshev fits in t^2 at 40 digits, generated by gen_sincos.py beside this
+// file), and two complex squarings give the full angle, to a few ulp.  This
+// replaces the general sincos, whose large-argument reduction the ~1e4-turn
+// projected phase would otherwise pull in.
 __device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
-  const double u = __builtin_amdgcn_fract(r);
-  const double k = rint(4.0 * u);
-  const double x = TPI * (u - 0.25 * k);
-  const double x2 = x * x;
-  const double ps =
-      x2 * (-1.0 / 6.0 +
-            x2 * (1.0 / 120.0 +
-                  x2 * (-1.0 / 5040.0 +
-                        x2 * (1.0 / 362880.0 +
-                              x2 * (-1.0 / 39916800.0 + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0)))))));
-  const double sn = x + x * ps;
-  const double cs =
-      1.0 + x2 * (-0.5 +
-                  x2 * (1.0 / 24.0 +
-                        x2 * (-1.0 / 720.0 +
-                              x2 * (1.0 / 40320.0 +
-                                    x2 * (-1.0 / 3628800.0 +
-                                          x2 * (1.0 / 479001600.0 +
-                                                x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0))))))));
-  const int q = ((int)k) & 3;
-  // rotate (cos, sin) by q quarter turns
-  const double s1 = (q & 1) ? cs : sn;
-  const double c1 = (q & 1) ? sn : cs;
-  *s = (q & 2) ? -s1 : s1;
-  *c = ((q + 1) & 2) ? -c1 : c1;
+  const double t = r - rint(r);
+  const double z = t * t;
+  double p = z * sreg(0x1.e3f38399551bfp-25) + sreg(-0x1.e30071afc3e59p-19);
+  p = fma(p, z, -sreg(-0x1.50782fda12d96p-13));
+  p = fma(p, z, -sreg(0x1.32d2cce2e5b19p-8));
+  p = fma(p, z, -sreg(-0x1.466bc677587f8p-4));
+  p = fma(p, z, -sreg(0x1.4abbce625be41p-1));
+  p = fma(p, z, -sreg(-0x1.921fb54442d18p+0));
+  double q = z * sreg(0x1.f3dbcea61b1a4p-22) + sreg(-0x1.a6c9c1be9eb49p-16);
+  q = fma(q, z, -sreg(-0x1.e1f4fb60281f6p-11));
+  q = fma(q, z, -sreg(0x1.55d3c7dbfd139p-6));
+  q = fma(q, z, -sreg(-0x1.03c1f081b0780p-2));
+  q = fma(q, z, -sreg(0x1.3bd3cc9be458bp+0));
+  q = fma(q, z, 1.0);
+  double sn = t * p, cs = q;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {  // (cs + i sn)^2, twice
+    const double sc = sn * cs;
+    cs = fma(cs, cs, -(sn * sn));
+    sn = sc + sc;
+  }
+  *s = sn;
+  *c = cs;
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
@@ -67,15 +74,21 @@ __device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
 // point, so every per-point constant is wave-uniform) and one split of the
 // band.  It walks the LDS tiles of the split that any of its points needs:
 // each tile (frequency tables and whitened data of every detector) is read
-// from HBM once and used by all cpb waves.  Tiles sit on a fixed grid of the
-// band and a lane's bins within a tile start at the point's own first bin,
-// so a point's sums do not depend on the other points of its block.
-// Per-lane sums, one wave reduction, one partial per (point, split).
+// from HBM once and used by all cpb waves.  Lane l of a point's wave takes
+// the band bins j = ks + l + 64 n of the point's range [ks, ke) whatever the
+// tiles, so a point's sums do not depend on the tile grid or on the other
+// points of its block.  Within a tile a wave walks the point's segments:
+// runs of bins over which neither the amplitude nor the phase changes region
+// (the bin indices come from the set-up, on the same Mf = Msec f_k tests as
+// the piecewise waveform), each with a loop of its own that holds only that
+// region pair's constants.  Per-lane sums, one wave reduction, one partial
+// per (point, split).
 
 // Bins of the split that the block's points need: [lo, hi), with lo rounded
 // down to the tile grid.
-__device__ __forceinline__ void block_range(const LLInst& in, int b, bool active, int j0, int j1, int tile,
-                                            int* s_lohi, int* ks, int* ke, int* lo, int* hi) {
+template <int TILE>
+__device__ __forceinline__ void block_range(const LLInst& in, int b, bool active, int j0, int j1, int* s_lohi,
+                                            int* ks, int* ke, int* lo, int* hi) {
   const int lane = threadIdx.x & 63;
   *ks = 0;
   *ke = 0;
@@ -91,12 +104,156 @@ __device__ __forceinline__ void block_range(const LLInst& in, int b, bool active
     atomicMax(&s_lohi[1], *ke);
   }
   __syncthreads();
-  *lo = j0 + ((s_lohi[0] - j0) / tile) * tile;
+  *lo = j0 + ((s_lohi[0] - j0) / TILE) * TILE;
   *hi = s_lohi[1];
 }
 
-template <int ND>
-__device__ void ll_f64_block(const LLInst& in, int group, int split, int tile, double* lds, int* s_lohi) {
+// Amplitude without the detector scale, and the projected detector-0
+// argument in turns, of one bin in amplitude region AR and phase region PR
+// (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
+// constants of the point and the bin's f, f^(1/6), f^(-1/6), log f.
+template <int AR, int PR>
+__device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, double is6f, double logf,
+                                          double* a, double* r) {
+  const double Mf = c.Msec * f;
+  const double i_s6 = c.is6_scale * is6f;
+  const double i_third = i_s6 * i_s6;
+  const double i_two_thirds = i_third * i_third;
+  const double i_Mf = i_third * i_two_thirds;
+  double third = 0.0, two_thirds = 0.0, four_thirds = 0.0, five_thirds = 0.0, two = 0.0;
+  if (AR == 0 || PR == 0) {
+    const double s6 = c.s6_scale * s6f;
+    third = s6 * s6;
+    two_thirds = third * third;
+    four_thirds = Mf * third;
+    five_thirds = four_thirds * third;
+    two = Mf * Mf;
+  }
+  double A;
+  if (AR == 0) {
+    const double seven_thirds = third * two, eight_thirds = two_thirds * two, three = Mf * two;
+    const double lo = fma(c.ia53, five_thirds, fma(c.ia43, four_thirds, fma(c.ia1, Mf, c.ia23 * two_thirds)));
+    const double hi = fma(c.ia3, three, fma(c.ia83, eight_thirds, fma(c.ia73, seven_thirds, c.ia2 * two)));
+    A = (lo + c.ia0) + hi;
+  } else if (AR == 1) {
+    const double x1 = Mf - c.col_f1;
+    A = fma(x1, fma(x1, (Mf - c.col_f2) * ((Mf - c.col_f3) * c.ib4 + c.ib3) + c.ib2, c.ib1), c.ib0);
+  } else {
+    const double df = Mf - c.fRD;
+    A = c.mnum * exp(-df * c.mrd_rate) / fma(df, df, c.mg2);
+  }
+  *a = (i_Mf * i_s6) * A;  // Mf^(-7/6)
+  if (PR == 0) {
+    const double i_five_thirds = i_Mf * i_two_thirds;
+    const double neg = fma(c.ins_qm53, i_five_thirds, fma(c.ins_qm1, i_Mf, fma(c.ins_qm23, i_two_thirds,
+                       c.ins_qm13 * i_third)));
+    const double pos = fma(c.ins_q2, two, fma(c.ins_q53, five_thirds, fma(c.ins_q43, four_thirds,
+                       fma(c.ins_q23, two_thirds, fma(c.ins_qL, logf, fma(third, c.ins_q13L * logf + c.ins_q13,
+                       c.ins_q1 * Mf))))));
+    *r = (neg + c.ins_q0) + pos;
+  } else if (PR == 1) {
+    const double i_Mf3 = i_Mf * i_Mf * i_Mf;
+    *r = fma(c.int_qm3, i_Mf3, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
+  } else {
+    const double sq = sqrt(Mf);
+    const double f34 = sqrt(Mf * sq);
+    *r = fma(c.mrd_qat, atan((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, i_Mf, c.mrd_q1 * Mf))) +
+         c.mrd_q0;
+  }
+}
+
+// Walks the segments of [pos, hi): SEG<AR, PR>::run(st, pos, end) for each
+// run [pos, end) of one region pair.  Band-index region boundaries a1..p2.
+template <template <int, int> class SEG, class ST>
+__device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int a2, int p1, int p2) {
+  while (pos < hi) {
+    const int ar = pos < a1 ? 0 : (pos >= a2 ? 2 : 1);
+    const int pr = pos < p1 ? 0 : (pos >= p2 ? 2 : 1);
+    int end = hi;
+    if (a1 > pos && a1 < end) end = a1;
+    if (a2 > pos && a2 < end) end = a2;
+    if (p1 > pos && p1 < end) end = p1;
+    if (p2 > pos && p2 < end) end = p2;
+    switch (ar * 3 + pr) {
+      case 0: SEG<0, 0>::run(st, pos, end); break;
+      case 1: SEG<0, 1>::run(st, pos, end); break;
+      case 2: SEG<0, 2>::run(st, pos, end); break;
+      case 3: SEG<1, 0>::run(st, pos, end); break;
+      case 4: SEG<1, 1>::run(st, pos, end); break;
+      case 5: SEG<1, 2>::run(st, pos, end); break;
+      case 6: SEG<2, 0>::run(st, pos, end); break;
+      case 7: SEG<2, 1>::run(st, pos, end); break;
+      default: SEG<2, 2>::run(st, pos, end); break;
+    }
+    pos = end;
+  }
+  return pos;
+}
+
+// ---- FP64
+template <int ND, int TILE>
+struct F64State {
+  const Point* c;
+  const double* lds;
+  int t0, j, age;
+  double acc[2 * ND];
+  // Detector d > 0 differs from detector 0 by the phase f dt_d + dl_d turns,
+  // linear in f.  A lane steps 64 bins at a time, so exp(i theta_d) advances
+  // by the constant rotation w_d = exp(i 2 pi 64 dt_d / Tobs); the recurrence
+  // is re-seeded directly every 32 wave iterations (drift below ~1e-14).
+  double rc[ND], rs[ND], wc[ND], ws[ND];
+};
+
+template <int ND, int TILE>
+__device__ __forceinline__ void f64_reseed(F64State<ND, TILE>& st, double f) {
+#pragma unroll
+  for (int d = 1; d < ND; d++) sincos_turns(fma(f, st.c->dt[d], st.c->dl[d]), &st.rs[d], &st.rc[d]);
+}
+
+template <int ND, int TILE>
+struct F64Seg {
+  template <int AR, int PR>
+  struct S {
+    static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
+      const Point& c = *st.c;
+      const int nit = (end - pos + 63) >> 6;
+      for (int n = 0; n < nit; n++) {
+        if (st.age >= 32) {
+          if (st.j < end) f64_reseed(st, st.lds[st.j - st.t0]);
+          st.age = 0;
+        }
+        if (st.j < end) {
+          const double* L = st.lds + (st.j - st.t0);
+          double a, r;
+          amp_turns<AR, PR>(c, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
+          double s0, c0;
+          sincos_turns(r, &s0, &c0);
+          // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the per-bin
+          // sums carry a^2 / S and a Re(D e^{-i x_d}) / S, and k_finalize applies
+          // the per-point detector scale once.
+          const double a2 = a * a;
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            const double co = (d == 0) ? c0 : c0 * st.rc[d] - s0 * st.rs[d];
+            const double sn = (d == 0) ? s0 : s0 * st.rc[d] + c0 * st.rs[d];
+            st.acc[2 * d] = fma(a2, L[(6 + 3 * d) * TILE], st.acc[2 * d]);
+            st.acc[2 * d + 1] = fma(a, fma(L[(4 + 3 * d) * TILE], co, L[(5 + 3 * d) * TILE] * sn), st.acc[2 * d + 1]);
+            if (d > 0) {
+              const double t = st.rc[d] * st.wc[d] - st.rs[d] * st.ws[d];
+              st.rs[d] = st.rs[d] * st.wc[d] + st.rc[d] * st.ws[d];
+              st.rc[d] = t;
+            }
+          }
+          st.j += 64;
+        }
+        st.age++;
+      }
+    }
+  };
+};
+
+template <int ND, int TILE>
+__device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds, int* s_lohi) {
   constexpr int NP = 4 + 3 * ND;
   const int cpb = blockDim.x >> 6;
   const int wave = uniform(threadIdx.x >> 6);
@@ -105,90 +262,101 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, int tile, d
   const int j0 = split * in.split_len, j1 = min(in.nb, j0 + in.split_len);
   const bool active = b < in.B && in.pts[b].status == ST_OK;
   int ks, ke, blo, bhi;
-  block_range(in, b, active, j0, j1, tile, s_lohi, &ks, &ke, &blo, &bhi);
-  double acc[2 * ND];
+  block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
+  F64State<ND, TILE> st;
+  st.c = in.pts + uniform(active ? b : 0);
+  st.lds = lds;
+  st.j = ks + lane;
+  st.age = 0;
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = 0.0;
-  // Detector d > 0 differs from detector 0 by the phase
-  // theta_d(f) = 2 pi f (tshift_d - tshift_0) + lambda_d - lambda_0, linear in
-  // f.  A lane steps 64 bins at a time, so exp(i theta_d) advances by the
-  // constant rotation w_d = exp(i 2 pi 64 (tshift_d - tshift_0) / Tobs); the
-  // recurrence is re-seeded directly at a lane's first bin of a run and
-  // every 32 steps (drift below ~1e-14 relative).
-  double rc[ND], rs[ND], wc[ND], ws[ND];
+  for (int q = 0; q < 2 * ND; q++) st.acc[q] = 0.0;
 #pragma unroll
-  for (int d = 0; d < ND; d++) { rc[d] = 1.0; rs[d] = 0.0; wc[d] = 1.0; ws[d] = 0.0; }
+  for (int d = 0; d < ND; d++) { st.rc[d] = 1.0; st.rs[d] = 0.0; st.wc[d] = 1.0; st.ws[d] = 0.0; }
+  int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
   if (active) {
-    const Point& c = in.pts[b];
+    const Point& c = *st.c;
 #pragma unroll
-    for (int d = 1; d < ND; d++) sincos_turns(64.0 * in.inv_T * (c.tshift[d] - c.tshift[0]), &ws[d], &wc[d]);
+    for (int d = 1; d < ND; d++) sincos_turns(64.0 * in.inv_T * c.dt[d], &st.ws[d], &st.wc[d]);
+    if (st.j < ke) f64_reseed(st, in.p64[st.j]);
+    a1 = c.seg_a1 - in.imin;
+    a2 = c.seg_a2 - in.imin;
+    p1 = c.seg_p1 - in.imin;
+    p2 = c.seg_p2 - in.imin;
   }
-  int jnext = -1, age = 0;
-
-  for (int t0 = blo; t0 < bhi; t0 += tile) {
-    const int tn = min(tile, bhi - t0);
+  for (int t0 = blo; t0 < bhi; t0 += TILE) {
+    const int tn = min(TILE, bhi - t0);
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
 #pragma unroll
-      for (int p = 0; p < NP; p++) lds[p * tile + i] = in.p64[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < NP; p++) lds[p * TILE + i] = in.p64[(size_t)p * in.nbp + t0 + i];
     }
     __syncthreads();
-    const int lo = max(ks, t0), hi = min(ke, t0 + tn);
-    if (active && lo < hi) {
-      const Point& c = in.pts[b];
-      for (int j = lo + lane; j < hi; j += 64) {
-        const int i = j - t0;
-        const double f = lds[i];
-        if (j != jnext || age >= 32) {
-#pragma unroll
-          for (int d = 1; d < ND; d++)
-            sincos_turns(f * (c.tshift[d] - c.tshift[0]) + (c.lambda[d] - c.lambda[0]) * INV_TPI, &rs[d], &rc[d]);
-          age = 0;
-        }
-        Pow pw = pow_table(c, f, lds[tile + i], lds[2 * tile + i], lds[3 * tile + i]);
-        const double a = amplitude(c, pw);
-        const double ph = phase(c, pw);
-        double s0, c0;
-        // projected phase in turns: f (ts - td_0) + (lambda_0 - Psi) / 2 pi
-        sincos_turns(fma(f, c.tshift[0], (c.lambda[0] - ph) * INV_TPI), &s0, &c0);
-#pragma unroll
-        for (int d = 0; d < ND; d++) {
-          const double co = (d == 0) ? c0 : c0 * rc[d] - s0 * rs[d];
-          const double sn = (d == 0) ? s0 : s0 * rc[d] + c0 * rs[d];
-          const double A = c.Ascale[d] * a;
-          const double hr = A * co, hi2 = A * sn;
-          const double w = lds[(6 + 3 * d) * tile + i];
-          acc[2 * d] += (hr * hr + hi2 * hi2) * w;
-          acc[2 * d + 1] += lds[(4 + 3 * d) * tile + i] * hr + lds[(5 + 3 * d) * tile + i] * hi2;
-          if (d > 0) {
-            const double t = rc[d] * wc[d] - rs[d] * ws[d];
-            rs[d] = rs[d] * wc[d] + rc[d] * ws[d];
-            rc[d] = t;
-          }
-        }
-        jnext = j + 64;
-        age++;
-      }
+    if (active) {
+      st.t0 = t0;
+      pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }
+  double acc[2 * ND];
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum(acc[q]);
+  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum(st.acc[q]);
   if (lane == 0 && b < in.B) {
 #pragma unroll
     for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
   }
 }
 
-// Mixed precision.  FP64: powers, amplitude and phase (gfx950 issues FP64
-// FMAs at the FP32 rate, and the phase needs FP64), and the full argument
-// 2 pi f (ts - td) + lambda - Psi reduced to a fraction of a turn
+// ---- mixed precision.  FP64: powers, amplitude and the projected argument
+// f (ts - td) + (lambda - Psi)/2 pi in turns (gfx950 issues FP64 FMAs at the
+// FP32 rate, and the phase needs FP64), reduced to a fraction of a turn
 // (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to ~1e-11.
 // FP32: sin/cos of the reduced turn (hardware v_sin/v_cos take turns), the
 // products with the scaled data, and per-lane Kahan-compensated sums; the
 // wave and split combine is FP64.
-template <int ND>
-__device__ void ll_mixed_block(const LLInst& in, int group, int split, int tile, double* lds, int* s_lohi) {
-  float* lds32 = reinterpret_cast<float*>(lds + 4 * tile);
+template <int ND, int TILE>
+struct MixState {
+  const Point* c;
+  const double* lds;
+  const float* lds32;
+  int t0, j;
+  float sum[2 * ND], comp[2 * ND];
+};
+
+template <int ND, int TILE>
+struct MixSeg {
+  template <int AR, int PR>
+  struct S {
+    static __device__ __forceinline__ void run(MixState<ND, TILE>& st, int pos, int end) {
+      const Point& c = *st.c;
+      const int nit = (end - pos + 63) >> 6;
+      for (int n = 0; n < nit; n++) {
+        if (st.j < end) {
+          const int i = st.j - st.t0;
+          const double* L = st.lds + i;
+          const float* M = st.lds32 + i;
+          const double f = L[0];
+          double ad, r;
+          amp_turns<AR, PR>(c, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
+          const float a = (float)ad;
+          const float a2 = a * a;
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            const double q = (d == 0) ? r : r + fma(f, c.dt[d], c.dl[d]);
+            const float u = (float)__builtin_amdgcn_fract(q);
+            const float sn = __builtin_amdgcn_sinf(u), cs = __builtin_amdgcn_cosf(u);
+            kahan_add(st.sum[2 * d], st.comp[2 * d], a2 * M[(3 * d + 2) * TILE]);
+            kahan_add(st.sum[2 * d + 1], st.comp[2 * d + 1],
+                      a * fmaf(M[(3 * d) * TILE], cs, M[(3 * d + 1) * TILE] * sn));
+          }
+          st.j += 64;
+        }
+      }
+    }
+  };
+};
+
+template <int ND, int TILE>
+__device__ void ll_mixed_block(const LLInst& in, int group, int split, double* lds, int* s_lohi) {
+  float* lds32 = reinterpret_cast<float*>(lds + 4 * TILE);
   const int cpb = blockDim.x >> 6;
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -196,84 +364,76 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, int tile,
   const int j0 = split * in.split_len, j1 = min(in.nb, j0 + in.split_len);
   const bool active = b < in.B && in.pts[b].status == ST_OK;
   int ks, ke, blo, bhi;
-  block_range(in, b, active, j0, j1, tile, s_lohi, &ks, &ke, &blo, &bhi);
-  float sum[2 * ND], comp[2 * ND];
+  block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
+  MixState<ND, TILE> st;
+  st.c = in.pts + uniform(active ? b : 0);
+  st.lds = lds;
+  st.lds32 = lds32;
+  st.j = ks + lane;
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) sum[q] = comp[q] = 0.0f;
-
-  for (int t0 = blo; t0 < bhi; t0 += tile) {
-    const int tn = min(tile, bhi - t0);
+  for (int q = 0; q < 2 * ND; q++) st.sum[q] = st.comp[q] = 0.0f;
+  int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
+  if (active) {
+    const Point& c = *st.c;
+    a1 = c.seg_a1 - in.imin;
+    a2 = c.seg_a2 - in.imin;
+    p1 = c.seg_p1 - in.imin;
+    p2 = c.seg_p2 - in.imin;
+  }
+  for (int t0 = blo; t0 < bhi; t0 += TILE) {
+    const int tn = min(TILE, bhi - t0);
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
 #pragma unroll
-      for (int p = 0; p < 4; p++) lds[p * tile + i] = in.p64[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < 4; p++) lds[p * TILE + i] = in.p64[(size_t)p * in.nbp + t0 + i];
 #pragma unroll
-      for (int p = 0; p < 3 * ND; p++) lds32[p * tile + i] = in.p32[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < 3 * ND; p++) lds32[p * TILE + i] = in.p32[(size_t)p * in.nbp + t0 + i];
     }
     __syncthreads();
-    const int lo = max(ks, t0), hi = min(ke, t0 + tn);
-    if (active && lo < hi) {
-      const Point& c = in.pts[b];
-      for (int j = lo + lane; j < hi; j += 64) {
-        const int i = j - t0;
-        const double f = lds[i];
-        Pow pw = pow_table(c, f, lds[tile + i], lds[2 * tile + i], lds[3 * tile + i]);
-        const float a = (float)amplitude(c, pw);
-        const double phr = phase(c, pw) * INV_TPI;  // phase in turns
-        const float a2 = a * a;
-#pragma unroll
-        for (int d = 0; d < ND; d++) {
-          // turns: f (ts - td) + lambda/2pi - Psi/2pi, reduced to [0, 1)
-          double q = fma(f, c.tshift[d], c.lambda[d] * INV_TPI - phr);
-          float r = (float)__builtin_amdgcn_fract(q);
-          float sn = __builtin_amdgcn_sinf(r), cs = __builtin_amdgcn_cosf(r);
-          kahan_add(sum[2 * d], comp[2 * d], a2 * lds32[(3 * d + 2) * tile + i]);
-          kahan_add(sum[2 * d + 1], comp[2 * d + 1],
-                    a * (lds32[(3 * d) * tile + i] * cs + lds32[(3 * d + 1) * tile + i] * sn));
-        }
-      }
+    if (active) {
+      st.t0 = t0;
+      pos = run_segments<MixSeg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }
   double acc[2 * ND];
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum((double)sum[q] - (double)comp[q]);
+  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum((double)st.sum[q] - (double)st.comp[q]);
   if (lane == 0 && b < in.B) {
 #pragma unroll
     for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
   }
 }
 
-template <int PREC>
-__device__ __forceinline__ void ll_block(const LLInst& in, int group, int split, int tile, double* lds,
-                                         int* s_lohi) {
+template <int PREC, int TILE>
+__device__ __forceinline__ void ll_block(const LLInst& in, int group, int split, double* lds, int* s_lohi) {
   if (PREC == 0) {
-    if (in.nd == 1) ll_f64_block<1>(in, group, split, tile, lds, s_lohi);
-    else if (in.nd == 2) ll_f64_block<2>(in, group, split, tile, lds, s_lohi);
-    else ll_f64_block<3>(in, group, split, tile, lds, s_lohi);
+    if (in.nd == 1) ll_f64_block<1, TILE>(in, group, split, lds, s_lohi);
+    else if (in.nd == 2) ll_f64_block<2, TILE>(in, group, split, lds, s_lohi);
+    else ll_f64_block<3, TILE>(in, group, split, lds, s_lohi);
   } else {
-    if (in.nd == 1) ll_mixed_block<1>(in, group, split, tile, lds, s_lohi);
-    else if (in.nd == 2) ll_mixed_block<2>(in, group, split, tile, lds, s_lohi);
-    else ll_mixed_block<3>(in, group, split, tile, lds, s_lohi);
+    if (in.nd == 1) ll_mixed_block<1, TILE>(in, group, split, lds, s_lohi);
+    else if (in.nd == 2) ll_mixed_block<2, TILE>(in, group, split, lds, s_lohi);
+    else ll_mixed_block<3, TILE>(in, group, split, lds, s_lohi);
   }
 }
 
 // One analysis: grid x = point groups, y = splits.
-template <int ND, int PREC>
-__global__ __launch_bounds__(512) void k_ll(LLInst in, int tile) {
+template <int ND, int PREC, int TILE>
+__global__ __launch_bounds__(512) void k_ll(LLInst in) {
   extern __shared__ double lds[];
   __shared__ int s_lohi[2];
   if (PREC == 0)
-    ll_f64_block<ND>(in, blockIdx.x, blockIdx.y, tile, lds, s_lohi);
+    ll_f64_block<ND, TILE>(in, blockIdx.x, blockIdx.y, lds, s_lohi);
   else
-    ll_mixed_block<ND>(in, blockIdx.x, blockIdx.y, tile, lds, s_lohi);
+    ll_mixed_block<ND, TILE>(in, blockIdx.x, blockIdx.y, lds, s_lohi);
 }
 
 // Many analyses in one persistent launch: every block takes (analysis, point
 // group, split) items from a queue until it is empty.  The exit test reads a
 // block-uniform item index, so every wave of every block reaches it.
-template <int PREC>
+template <int PREC, int TILE>
 __global__ __launch_bounds__(512) void k_ll_queue(const LLInst* __restrict__ insts, const LLItem* __restrict__ items,
-                                                  int n_items, int* __restrict__ counter, int tile) {
+                                                  int n_items, int* __restrict__ counter) {
   extern __shared__ double lds[];
   __shared__ int s_lohi[2];
   __shared__ int s_item;
@@ -285,16 +445,18 @@ __global__ __launch_bounds__(512) void k_ll_queue(const LLInst* __restrict__ ins
     if (it >= n_items) break;
     const LLItem w = items[it];
     const LLInst in = insts[w.inst];
-    ll_block<PREC>(in, w.group, w.split, tile, lds, s_lohi);
+    ll_block<PREC, TILE>(in, w.group, w.split, lds, s_lohi);
   }
 }
 
 // ------------------------------------------------------------ finalize
 // logL = sum_d [HD_d - HH_d/2], rho_d = HD_d / sqrt(HH_d) with HH = 4 sum |h|^2/S,
 // HD = 4 sum Re(d h*)/S (QuickCBC.c:4259-4266, Utilities.c:716-736).  Splits are summed in index order, so a point's
-// result does not depend on what else shares the launch.
+// result does not depend on what else shares the launch.  mode FIN_RAW: the
+// partials are the sums themselves; FIN_FP64: they lack the factors Ascale_d^2
+// and Ascale_d; FIN_MIXED: also the plane scales w_ref and sqrt(w_ref).
 __global__ void k_finalize(const double* __restrict__ partial, int splits, int nd, const Point* __restrict__ pts,
-                           int B, int mixed, const double* __restrict__ wref, double* __restrict__ logL,
+                           int B, int mode, const double* __restrict__ wref, double* __restrict__ logL,
                            double* __restrict__ rho) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -311,7 +473,11 @@ __global__ void k_finalize(const double* __restrict__ partial, int splits, int n
       HH += partial[((size_t)b * splits + s) * (2 * nd) + 2 * d];
       HD += partial[((size_t)b * splits + s) * (2 * nd) + 2 * d + 1];
     }
-    if (mixed) {
+    if (mode == FIN_FP64) {
+      const double A = c.Ascale[d];
+      HH *= A * A;
+      HD *= A;
+    } else if (mode == FIN_MIXED) {
       const double A = c.Ascale[d];
       HH *= (A * A) * wref[d];
       HD *= A * sqrt(wref[d]);
@@ -330,40 +496,60 @@ size_t ll_lds_bytes(int precision, int nd, int tile) {
   return (size_t)4 * tile * sizeof(double) + (size_t)3 * nd * tile * sizeof(float);
 }
 
-hipError_t launch_ll(int precision, const LLInst& in, int cpb, int tile, hipStream_t s) {
-  if (in.B <= 0) return hipSuccess;
-  dim3 grid((in.B + cpb - 1) / cpb, in.splits), block(64 * cpb);
-  size_t lds = ll_lds_bytes(precision, in.nd, tile);
+bool ll_tile_ok(int tile) { return tile == 128 || tile == 256 || tile == 512; }
+
+template <int TILE>
+static void launch_ll_t(int precision, const LLInst& in, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
   if (precision == 0) {
     switch (in.nd) {
-      case 1: hipLaunchKernelGGL((k_ll<1, 0>), grid, block, lds, s, in, tile); break;
-      case 2: hipLaunchKernelGGL((k_ll<2, 0>), grid, block, lds, s, in, tile); break;
-      default: hipLaunchKernelGGL((k_ll<3, 0>), grid, block, lds, s, in, tile); break;
+      case 1: hipLaunchKernelGGL((k_ll<1, 0, TILE>), grid, block, lds, s, in); break;
+      case 2: hipLaunchKernelGGL((k_ll<2, 0, TILE>), grid, block, lds, s, in); break;
+      default: hipLaunchKernelGGL((k_ll<3, 0, TILE>), grid, block, lds, s, in); break;
     }
   } else {
     switch (in.nd) {
-      case 1: hipLaunchKernelGGL((k_ll<1, 1>), grid, block, lds, s, in, tile); break;
-      case 2: hipLaunchKernelGGL((k_ll<2, 1>), grid, block, lds, s, in, tile); break;
-      default: hipLaunchKernelGGL((k_ll<3, 1>), grid, block, lds, s, in, tile); break;
+      case 1: hipLaunchKernelGGL((k_ll<1, 1, TILE>), grid, block, lds, s, in); break;
+      case 2: hipLaunchKernelGGL((k_ll<2, 1, TILE>), grid, block, lds, s, in); break;
+      default: hipLaunchKernelGGL((k_ll<3, 1, TILE>), grid, block, lds, s, in); break;
     }
   }
+}
+
+hipError_t launch_ll(int precision, const LLInst& in, int cpb, int tile, hipStream_t s) {
+  if (in.B <= 0) return hipSuccess;
+  if (!ll_tile_ok(tile) || in.nd < 1 || in.nd > 3) return hipErrorInvalidValue;
+  dim3 grid((in.B + cpb - 1) / cpb, in.splits), block(64 * cpb);
+  size_t lds = ll_lds_bytes(precision, in.nd, tile);
+  if (tile == 128) launch_ll_t<128>(precision, in, grid, block, lds, s);
+  else if (tile == 256) launch_ll_t<256>(precision, in, grid, block, lds, s);
+  else launch_ll_t<512>(precision, in, grid, block, lds, s);
   return hipGetLastError();
+}
+
+template <int TILE>
+static void launch_queue_t(int precision, const LLInst* d_insts, const LLItem* d_items, int n_items, int* d_counter,
+                           dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+  if (precision == 0)
+    hipLaunchKernelGGL((k_ll_queue<0, TILE>), grid, block, lds, s, d_insts, d_items, n_items, d_counter);
+  else
+    hipLaunchKernelGGL((k_ll_queue<1, TILE>), grid, block, lds, s, d_insts, d_items, n_items, d_counter);
 }
 
 hipError_t launch_ll_queue(int precision, const LLInst* d_insts, const LLItem* d_items, int n_items, int* d_counter,
                            int max_nd, int cpb, int tile, int blocks, hipStream_t s) {
+  if (!ll_tile_ok(tile)) return hipErrorInvalidValue;
   size_t lds = ll_lds_bytes(precision, max_nd, tile);
-  if (precision == 0)
-    hipLaunchKernelGGL((k_ll_queue<0>), dim3(blocks), dim3(64 * cpb), lds, s, d_insts, d_items, n_items, d_counter, tile);
-  else
-    hipLaunchKernelGGL((k_ll_queue<1>), dim3(blocks), dim3(64 * cpb), lds, s, d_insts, d_items, n_items, d_counter, tile);
+  dim3 grid(blocks), block(64 * cpb);
+  if (tile == 128) launch_queue_t<128>(precision, d_insts, d_items, n_items, d_counter, grid, block, lds, s);
+  else if (tile == 256) launch_queue_t<256>(precision, d_insts, d_items, n_items, d_counter, grid, block, lds, s);
+  else launch_queue_t<512>(precision, d_insts, d_items, n_items, d_counter, grid, block, lds, s);
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mixed,
+hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mode,
                            const double* wref, double* logL, double* rho, hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3((B + 127) / 128), dim3(128), 0, s, partial, splits, nd, pts, B, mixed, wref, logL, rho);
+  hipLaunchKernelGGL(k_finalize, dim3((B + 127) / 128), dim3(128), 0, s, partial, splits, nd, pts, B, mode, wref, logL, rho);
   return hipGetLastError();
 }
 

@@ -109,7 +109,7 @@ static int multi_enqueue(qcbc_multi* m, const double* const* d_params, double* c
                          m->blocks, s));
   for (int i = 0; i < n; i++)
     HIPCHK(launch_finalize(m->partial[i], m->insts[i].splits, m->ctxs[i]->desc.n_det, m->pts[i], m->B[i],
-                           m->precision == QCBC_MIXED, m->ctxs[i]->d_wref, d_logL[i], d_rho[i], s));
+                           m->precision == QCBC_MIXED ? FIN_MIXED : FIN_FP64, m->ctxs[i]->d_wref, d_logL[i], d_rho[i], s));
   return QCBC_OK;
 }
 
@@ -162,7 +162,7 @@ int qcbc_multi_create(qcbc_ctx* const* ctxs, const int* B, int n, int precision,
 }
 
 int qcbc_multi_set_shape(qcbc_multi* m, int cpb, int tile, int split_bins) {
-  if (!m || cpb < 1 || cpb > 8 || tile < 64 || (tile & 63) || split_bins < tile) return QCBC_EINVAL;
+  if (!m || cpb < 1 || cpb > 8 || !ll_tile_ok(tile) || split_bins < tile) return QCBC_EINVAL;
   int maxnb = 0, maxnd = 1;
   for (auto* c : m->ctxs) {
     maxnb = std::max(maxnb, c->nb);

@@ -170,7 +170,7 @@ static int run_batch(qcbc_server* s, std::vector<std::shared_ptr<Request>>& batc
                          std::min(s->blocks, std::max(1, (int)flat.size())), q));
   for (size_t i = 0; i < insts.size(); i++)
     HIPCHK(launch_finalize(insts[i].partial, insts[i].splits, insts[i].nd, s->d_pts + offs[i], insts[i].B,
-                           s->precision == QCBC_MIXED, ctxs[i]->d_wref, s->d_logL + offs[i],
+                           s->precision == QCBC_MIXED ? FIN_MIXED : FIN_FP64, ctxs[i]->d_wref, s->d_logL + offs[i],
                            s->d_rho + (size_t)3 * offs[i], q));
   // results: logL (P), rho packed 3 per point slot, status (P)
   HIPCHK(hipMemcpyAsync(s->h_logL, s->d_logL, sizeof(double) * P, hipMemcpyDeviceToHost, q));

@@ -33,6 +33,32 @@ struct alignas(16) Point {
   // projection per detector slot
   double Fs[3], tshift[3], lambda[3], Ascale[3];
   double eta, finspin;
+  // ---- fused likelihood kernels only: the same waveform with its constants
+  // folded.  First bins (absolute index k) at which the amplitude leaves the
+  // inspiral (Mf >= 0.014) and enters the merger-ringdown (Mf >= fmaxCalc),
+  // and the phase leaves the inspiral (Mf >= 0.018) and enters the
+  // merger-ringdown (Mf >= fPhiMRD); clipped to [kstart, kend].
+  int seg_a1, seg_a2, seg_p1, seg_p2;
+  // Projected detector-0 argument in turns, f tshift_0 + (lambda_0 - Psi)/2 pi,
+  // as a function of Mf per phase region.
+  //   inspiral:      q0 + q1 Mf + Mf^(1/3) (q13 + q13L log f) + qL log f + q23 Mf^(2/3)
+  //                  + q43 Mf^(4/3) + q53 Mf^(5/3) + q2 Mf^2 + qm13 Mf^(-1/3)
+  //                  + qm23 Mf^(-2/3) + qm1 / Mf + qm53 Mf^(-5/3)
+  //   intermediate:  q0 + q1 Mf + qL log f + qm3 / Mf^3
+  //   merger:        q0 + q1 Mf + qm1 / Mf + q34 Mf^(3/4) + qat atan((Mf - a5) ifDM)
+  double ins_q0, ins_q1, ins_q13, ins_q13L, ins_qL, ins_q23, ins_q43, ins_q53, ins_q2;
+  double ins_qm13, ins_qm23, ins_qm1, ins_qm53;
+  double int_q0, int_q1, int_qL, int_qm3;
+  double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
+  // amplitude without Ascale: Mf^(-7/6) times
+  //   inspiral:      amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3)   (coefficients ia*)
+  //   intermediate:  amp_pre x the Newton form (coefficients ib*, nodes col_f*)
+  //   merger:        mnum exp(-(Mf - fRD) mrd_rate) / ((Mf - fRD)^2 + fDMg3^2)
+  double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia83, ia3;
+  double ib0, ib1, ib2, ib3, ib4;
+  double mnum, mg2;
+  // detector d relative to detector 0: f dt_d + dl_d turns
+  double dt[3], dl[3];
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are

@@ -119,6 +119,9 @@ ORACLE_FIELDS = ["Msec", "amp0", "fCut", "eta", "finspin", "fRD", "fDM", "fmaxCa
                  "phi_precalc"]
 
 
+PHASE_CONSTANTS = {"C1Int", "C2Int", "C1MRD", "C2MRD", "phi_precalc", "pi_init"}
+
+
 def test_setup_records_vs_oracle(ctx_cfg, oracle):
     """Per-point IMRPhenomD set-up on the GPU against the oracle's, field by field."""
     which, an, inj, D, SN, c = ctx_cfg
@@ -139,7 +142,11 @@ def test_setup_records_vs_oracle(ctx_cfg, oracle):
             g = getattr(r, name)
             o = ref["rho1" if name == "ai_73" else "rho2" if name == "ai_83" else "rho3" if name == "ai_3"
                     else name]
-            err = abs(g - o) / max(abs(o), 1e-300)
+            # the phase connection constants and slopes (radians, and radians per unit Mf with
+            # Mf < 0.2) come out of cancelling differences of connection values: they are judged
+            # as the phase is, absolute error over 2 pi
+            scale = 2 * np.pi if name in PHASE_CONSTANTS else max(abs(o), 1e-300)
+            err = abs(g - o) / scale
             if err > worst[0]:
                 worst = (err, (b, name, g, o))
     print("worst set-up field:", worst)
