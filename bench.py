@@ -26,8 +26,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-FP64_VALU_PEAK_TF = 78.6   # MI355X FP64 vector
+FP64_PEAK_TF = 78.6        # MI355X FP64, vector ALU and matrix cores alike
 FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
+
+# HBM bytes per launch of the fused FP64 kernel on this workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/k_ll_fp64_pt_step_hbm_traffic.md),
+# each scaled by what the counter reads for a 1 GiB device copy.
+K_LL_FP64_TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -47,7 +52,7 @@ def cpu_baseline(an, D, SN, P, seconds):
     t0 = time.perf_counter()
     O.loglike_batch(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, P[:n])
     dt = time.perf_counter() - t0
-    n2 = int(min(P.shape[0] * 8, max(n, seconds / max(dt / n, 1e-6))))
+    n2 = int(min(P.shape[0] * 64, max(n, seconds / max(dt / n, 1e-6))))
     idx = np.arange(n2) % P.shape[0]
     t0 = time.perf_counter()
     O.loglike_batch(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, P[idx])
@@ -58,14 +63,16 @@ def cpu_baseline(an, D, SN, P, seconds):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--kernel-steps", type=int, default=500, help="PT steps of the kernel-alone timing")
     ap.add_argument("--plain-points", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="autotune the launch shape first")
     ap.add_argument("--only-main", action="store_true", help="time the FP64 PT steps only (for profiling)")
+    ap.add_argument("--pt-groups", type=int, default=0, help="PT column groups (0: the library's choice)")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the PT-ensemble headline (default); 2: uneven persistent batch; "
                          "3: mixed-precision accuracy; 4: serving stream")
@@ -112,8 +119,7 @@ def main():
     init = W.ensemble_around(inj, B, seed=99)
     sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
-    bytes_per_logl = an.nifo * nb * 24          # (Re d/S, Im d/S, 1/S) per bin and detector, FP64
-    flops_per_logl = (60 + 19 * an.nifo) * nb    # waveform + projection + inner products
+    plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/6), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
 
     if args.autotune:
         for prec in (Q.FP64, Q.MIXED):
@@ -129,23 +135,37 @@ def main():
         a.free()
         b.free()
 
-    # ---- headline: FP64 fused path, PT steps, kernel timed with HIP events
-    pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64)
+    # ---- headline: FP64 fused path, PT steps (the library's column groups,
+    # overlapping on their own streams), wall clock over the timed steps
+    pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=args.pt_groups)
     pt.run(args.warmup)
-    ctx.timing(1)
     ctx.sync()
     t0 = time.perf_counter()
     pt.run(args.steps)
     ctx.sync()
     T = time.perf_counter() - t0
-    kms, nl = ctx.timing(0)
     value = B * args.steps / T
-    kernel_ms = kms / max(nl, 1)
     ms_per_step = 1000.0 * T / args.steps
-    log("FP64 fused: %.1f logL/s, %.3f ms/step, kernel %.3f ms/launch (%d launches)" %
-        (value, ms_per_step, kernel_ms, nl))
-    achieved = bytes_per_logl * B / (kernel_ms * 1e-3) / 1e9
-    valu_tf = flops_per_logl * B / (kernel_ms * 1e-3) / 1e12
+    log("FP64 fused: %.1f logL/s, %.3f ms/step (%d column groups)" % (value, ms_per_step, pt.groups))
+
+    # ---- the likelihood kernel alone: one group, each launch over all 2048
+    # points, timed with HIP events on the stream it runs on
+    pt1 = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=1)
+    pt1.run(args.warmup)
+    ctx.timing(1)
+    ctx.sync()
+    pt1.run(args.kernel_steps)
+    ctx.sync()
+    kms, nl = ctx.timing(0)
+    kernel_ms = kms / max(nl, 1)
+    cpb, tile, _ = ctx.get_shape(Q.FP64)
+    log("likelihood kernel alone: %.4f ms/launch over %d launches (%d points per block, %d-bin tiles)" %
+        (kernel_ms, nl, cpb, tile))
+    # algorithmic FP64 work of one launch, on the ensemble as the timed steps left it
+    from quickcbc_amd import bench_configs as C
+    bin_points, flops = C.fused_work(ctx, an, pt1.get()[0])
+    pt1.close()
+    achieved_tf = flops / (kernel_ms * 1e-3) / 1e12
     extra = {}
     if not args.only_main:
         # graph-replayed steps
@@ -164,17 +184,23 @@ def main():
         extra["fp64_step_ms_p99"] = percentile(lat, 99)
         x, Lx, acc, sw = pt.get()
         extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + args.latency_steps))
+        extra["pt_groups"] = pt.groups
         pt.close()
         # mixed precision on the same shape
-        ptm = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED)
+        ptm = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED, groups=args.pt_groups)
         ptm.run(args.warmup)
-        ctx.timing(1)
         ctx.sync()
         t0 = time.perf_counter()
         ptm.run(args.steps)
         Tm = time.perf_counter() - t0
-        mms, mnl = ctx.timing(0)
         extra["mixed_logl_per_s"] = B * args.steps / Tm
+        pm1 = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED, groups=1)
+        pm1.run(args.warmup)
+        ctx.timing(1)
+        ctx.sync()
+        pm1.run(args.kernel_steps)
+        mms, mnl = ctx.timing(0)
+        pm1.close()
         extra["mixed_kernel_ms"] = mms / max(mnl, 1)
         ptm.step_timing(True)
         ptm.run(args.latency_steps)
@@ -225,12 +251,18 @@ def main():
         "config": {"workload": "batched PT ensemble, Tobs=16 s, fs=4096 Hz, 3 det (H/L/V), band 20-2048 Hz "
                                "(32448 bins), 32 temps x 64 chains = 2048 points/step, FP64 fused launch",
                    "points_per_step": B, "bins": nb, "detectors": an.nifo},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_ll_f64<3>", "kernel_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_logl * B},
-        "valu": {"achieved_tflops": valu_tf, "peak_tflops": FP64_VALU_PEAK_TF, "frac": valu_tf / FP64_VALU_PEAK_TF,
-                 "flops_per_logl": flops_per_logl},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP64_PEAK_TF, "traffic": K_LL_FP64_TRAFFIC_BYTES,
+                     "ceiling": "FP64 arithmetic: the kernel runs on the FP64 vector ALU, no MFMA; MI355X's "
+                                "FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
+                     "kernel": "k_ll<3,FP64,%d>" % tile, "kernel_ms": kernel_ms, "points_per_block": cpb,
+                     "flops_per_launch": flops, "bin_points_per_launch": bin_points,
+                     "flops_per_bin_point": flops / max(bin_points, 1)},
+        "hbm": {"data_stream_bytes_per_launch": plane_bytes,
+                "traffic_bytes_per_launch": K_LL_FP64_TRAFFIC_BYTES,
+                "achieved_gbs": (K_LL_FP64_TRAFFIC_BYTES / (kernel_ms * 1e-3) / 1e9
+                                 if K_LL_FP64_TRAFFIC_BYTES else None),
+                "peak_gbs": HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }
     out.update(extra)

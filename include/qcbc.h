@@ -201,6 +201,15 @@ typedef struct qcbc_pt qcbc_pt;
  * temperature 1 (scaled by sqrt(T)). */
 int qcbc_pt_create(qcbc_ctx *ctx, int n_temps, int n_chains, double ladder, const double *init,
                    const double *sigma, uint64_t seed, int precision, qcbc_pt **out);
+/* As qcbc_pt_create, with the chain columns split into `groups` (1..4, at
+ * most n_chains; 0 = 3 when n_chains >= 3 and n_temps*n_chains >= 1536, else
+ * 1) groups that step on streams of their own, so that one group's
+ * latency-bound kernels and likelihood tail overlap another group's
+ * likelihood kernel.  The chains do not depend on the number of groups. */
+int qcbc_pt_create_groups(qcbc_ctx *ctx, int n_temps, int n_chains, double ladder, const double *init,
+                          const double *sigma, uint64_t seed, int precision, int groups,
+                          qcbc_pt **out);
+int qcbc_pt_groups(qcbc_pt *pt, int *groups);
 /* nsteps PT steps: propose, evaluate, Metropolis accept, adjacent-temperature
  * swaps, all on the GPU.  use_graph: replay one captured step as a hipGraph. */
 int qcbc_pt_run(qcbc_pt *pt, int nsteps, int use_graph);

@@ -221,3 +221,38 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
             "latency_ms_p50": float(np.percentile(lat, 50)) if lat.size else None,
             "latency_ms_p99": float(np.percentile(lat, 99)) if lat.size else None,
             "logl_per_s": n_done / (t_end - t_start), "batches": st["batches"], **counters}
+
+
+# FP64 operations of one (point, bin) in the fused FP64 kernel, counted from
+# its source (an FMA counts 2; exp, atan, sqrt and a division count 1 each):
+# the waveform part by (amplitude region, phase region), 0 inspiral,
+# 1 intermediate, 2 merger-ringdown; the quarter-angle sincos of the
+# projected argument; and per detector the inner-product terms, plus the
+# rotation and its recurrence for every detector after the first.
+FUSED_WAVEFORM_FLOPS = {(0, 0): 57, (0, 1): 40, (0, 2): 46, (1, 0): 49, (1, 1): 26, (1, 2): 32,
+                        (2, 0): 45, (2, 1): 22, (2, 2): 28}
+FUSED_SINCOS_FLOPS = 37
+
+
+def fused_detector_flops(nd):
+    return 1 + 7 + 19 * (nd - 1)
+
+
+def fused_work(ctx, an, params):
+    """(bin-points, FP64 flops) of one fused FP64 launch over params: each
+    point's bins [max(kstart, imin), min(kend, imax)) split by region pair."""
+    recs = Q.point_records(ctx, params)
+    per_det = fused_detector_flops(an.nifo) + FUSED_SINCOS_FLOPS
+    bins = 0
+    flops = 0
+    for r in recs:
+        if r.status != 0:
+            continue
+        k = np.arange(max(r.kstart, an.imin), min(r.kend, an.imax))
+        ar = np.where(k < r.seg_a1, 0, np.where(k >= r.seg_a2, 2, 1))
+        pr = np.where(k < r.seg_p1, 0, np.where(k >= r.seg_p2, 2, 1))
+        cnt = np.bincount(ar * 3 + pr, minlength=9)
+        bins += int(cnt.sum())
+        flops += sum(int(cnt[a * 3 + p]) * (FUSED_WAVEFORM_FLOPS[(a, p)] + per_det)
+                     for a in range(3) for p in range(3))
+    return bins, flops

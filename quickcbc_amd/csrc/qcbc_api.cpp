@@ -146,8 +146,8 @@ int qcbc_ctx_create(const qcbc_desc* desc, qcbc_ctx** out) {
     return QCBC_ENOMEM;
   }
   for (int p = 0; p < 2; p++) {
-    c->cpb[p] = 4;
-    c->tile[p] = 256;
+    c->cpb[p] = 8;
+    c->tile[p] = 512;
     c->splits[p] = 0;
   }
   *out = c;
@@ -215,12 +215,12 @@ int qcbc_bind_data(qcbc_ctx* c, const double* D, const double* SN) {
   return bind_impl(c, rd, rs);
 }
 
-// Launch shape: default splits give >= ~2048 blocks.
+// Launch shape: default splits give >= ~4096 blocks (16 per CU).
 int qcbc::effective_splits(const qcbc_ctx* c, int prec, int B) {
   int s = c->splits[prec];
   if (s <= 0) {
     int blocks = (B + c->cpb[prec] - 1) / c->cpb[prec];
-    s = (2048 + blocks - 1) / std::max(blocks, 1);
+    s = (4096 + blocks - 1) / std::max(blocks, 1);
   }
   int maxs = std::max(1, c->nb / 256);
   s = std::min(s, std::min(maxs, QCBC_MAX_SPLITS));
@@ -259,29 +259,41 @@ static void harvest_events(qcbc_ctx* c) {
   c->ev_pairs.clear();
 }
 
-int qcbc::loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
-                       int* d_status) {
+int qcbc::loglike_partials(qcbc_ctx* c, const double* d_params, int B, int prec, int* d_status, LLInst* out) {
   if (!c->bound) return QCBC_ENOTBOUND;
   if (prec != QCBC_FP64 && prec != QCBC_MIXED) return QCBC_EINVAL;
   int st = ctx_reserve(c, B);
   if (st) return st;
   HIPCHK(launch_setup(d_params, B, c->an, c->d_pts, d_status, c->stream));
   const LLInst in = make_inst(c, c->tile[prec], B, effective_splits(c, prec, B), c->d_pts, c->d_partial);
+  *out = in;
+  return ll_launch_timed(c, prec, in, c->stream);
+}
+
+int qcbc::ll_launch_timed(qcbc_ctx* c, int prec, const LLInst& in, hipStream_t s) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(hipEventRecord(e0, s));
   }
-  HIPCHK(launch_ll(prec, in, c->cpb[prec], c->tile[prec], c->stream));
+  HIPCHK(launch_ll(prec, in, c->cpb[prec], c->tile[prec], s));
   if (c->timing) {
-    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventRecord(e1, s));
     c->ev_pairs.emplace_back(e0, e1);
     c->launches++;
     if (c->ev_pairs.size() >= 512) harvest_events(c);
   }
-  HIPCHK(launch_finalize(c->d_partial, in.splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED ? FIN_MIXED : FIN_FP64, c->d_wref, d_logL,
-                         d_rho, c->stream));
+  return QCBC_OK;
+}
+
+int qcbc::loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
+                       int* d_status) {
+  LLInst in;
+  int st = loglike_partials(c, d_params, B, prec, d_status, &in);
+  if (st) return st;
+  HIPCHK(launch_finalize(c->d_partial, in.splits, c->desc.n_det, c->d_pts, B, prec == QCBC_MIXED ? FIN_MIXED : FIN_FP64,
+                         c->d_wref, d_logL, d_rho, c->stream));
   return QCBC_OK;
 }
 
@@ -471,7 +483,7 @@ int qcbc_autotune(qcbc_ctx* c, int prec, int B) {
     for (int tile : tiles) {
       if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) continue;
       int blocks = (B + cpb - 1) / cpb;
-      int base = std::max(1, (2048 + blocks - 1) / blocks);
+      int base = std::max(1, (4096 + blocks - 1) / blocks);
       int cands[] = {std::max(1, base / 2), base, base * 2};
       for (int sp : cands) {
         if (sp > QCBC_MAX_SPLITS) continue;

@@ -12,6 +12,7 @@
 
 #define QCBC_MAX_SPLITS 64
 #define QCBC_PLAIN_BLOCKS 64
+#define QCBC_PT_MAX_GROUPS 4
 
 // One resident analysis: band planes (FP64 and scaled FP32), the raw
 // half-complex data and PSD, per-batch scratch, launch shape and timing.
@@ -42,7 +43,7 @@ struct qcbc_ctx {
   double* d_hw = nullptr;
   double* d_plain_partial = nullptr;
   // fused-kernel launch shape per precision
-  int cpb[2] = {4, 4}, tile[2] = {256, 256}, splits[2] = {0, 0};
+  int cpb[2] = {8, 8}, tile[2] = {512, 512}, splits[2] = {0, 0};
   // kernel timing
   bool timing = false;
   double kernel_ms = 0.0;
@@ -56,6 +57,12 @@ extern thread_local hipError_t t_last_hip;
 int ensure_device();
 int effective_splits(const qcbc_ctx* c, int prec, int B);
 LLInst make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial);
+// the fused likelihood launch on stream s, bracketed by timing events when
+// the context's kernel timing is on (caller holds mu)
+int ll_launch_timed(qcbc_ctx* c, int prec, const LLInst& in, hipStream_t s);
+// setup + fused likelihood partial sums on the context stream (caller holds
+// mu); *out describes the launch (splits, partial buffer) for the finalize
+int loglike_partials(qcbc_ctx* c, const double* d_params, int B, int prec, int* d_status, LLInst* out);
 // setup + fused likelihood + finalize on the context stream (caller holds mu)
 int loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
                  int* d_status);

@@ -42,13 +42,18 @@ hipError_t launch_geometry(int what, const double* in, int n, double* out, hipSt
 }  // namespace qcbc
 
 namespace qcbc {
-hipError_t launch_pt_propose(const double* x, double* y, const double* sigma, int n_chains, int P, double ladder,
-                             uint64_t seed, const unsigned long long* step, hipStream_t s);
-hipError_t launch_pt_accept(double* x, double* L, const double* y, const double* Ly, const int* st_y, int n_chains,
-                            int P, double ladder, uint64_t seed, const unsigned long long* step,
-                            unsigned long long* acc, hipStream_t s);
-hipError_t launch_pt_swap(double* x, double* L, int n_temps, int n_chains, double ladder, uint64_t seed,
-                          unsigned long long* step, unsigned long long* swaps, hipStream_t s);
+// per-temperature constants tc[3 n_temps]: sqrt(T), 1/T, ladder^-t
+hipError_t launch_pt_temps(double* tc, int n_temps, double ladder, hipStream_t s);
+// proposals of the group of chain columns [c0, c0 + ncols) at step *step
+hipError_t launch_pt_propose(const double* x, double* y, const double* sigma, const double* tc, int n_temps,
+                             int n_chains, int c0, int ncols, uint64_t seed, const unsigned long long* step,
+                             hipStream_t s);
+// finalize + accept + swap + next proposals + step counter of one group, one
+// launch (block per chain column)
+hipError_t launch_pt_post(const LLInst& in, int mode, const double* wref, double* x, double* L, double* y, double* Ly,
+                          double* rho, const int* st_y, const double* sigma, const double* tc, int n_temps,
+                          int n_chains, int c0, int ncols, uint64_t seed, unsigned long long* step,
+                          unsigned long long* acc, unsigned long long* swaps, unsigned int* done, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 }  // namespace qcbc

@@ -455,12 +455,9 @@ __global__ __launch_bounds__(512) void k_ll_queue(const LLInst* __restrict__ ins
 // result does not depend on what else shares the launch.  mode FIN_RAW: the
 // partials are the sums themselves; FIN_FP64: they lack the factors Ascale_d^2
 // and Ascale_d; FIN_MIXED: also the plane scales w_ref and sqrt(w_ref).
-__global__ void k_finalize(const double* __restrict__ partial, int splits, int nd, const Point* __restrict__ pts,
-                           int B, int mode, const double* __restrict__ wref, double* __restrict__ logL,
-                           double* __restrict__ rho) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const Point& c = pts[b];
+__device__ __forceinline__ void finalize_point(const double* __restrict__ partial, int splits, int nd,
+                                               const Point& c, int b, int mode, const double* __restrict__ wref,
+                                               double* __restrict__ logL, double* __restrict__ rho) {
   if (c.status != ST_OK) {
     logL[b] = 0.0;
     for (int d = 0; d < nd; d++) rho[(size_t)b * nd + d] = 0.0;
@@ -488,6 +485,14 @@ __global__ void k_finalize(const double* __restrict__ partial, int splits, int n
     L += HD - 0.5 * HH;
   }
   logL[b] = L;
+}
+
+__global__ void k_finalize(const double* __restrict__ partial, int splits, int nd, const Point* __restrict__ pts,
+                           int B, int mode, const double* __restrict__ wref, double* __restrict__ logL,
+                           double* __restrict__ rho) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  finalize_point(partial, splits, nd, pts[b], b, mode, wref, logL, rho);
 }
 
 // ------------------------------------------------------------ launchers
@@ -566,7 +571,11 @@ hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s) {
 // ------------------------------------------------------ PT ensemble step
 // Counter-based random numbers: SplitMix64 finaliser of (seed, step, point,
 // draw), so a step's draws depend only on its counters (graph replays and
-// reruns give the same chain).
+// reruns give the same chain).  Points are numbered p = t n_chains + c
+// (temperature t, chain column c) for the random counters whatever the
+// storage order: a group of chain columns [c0, c0 + ncols) stores its points
+// as t ncols + (c - c0), and every group sees the draws the whole ensemble
+// would.
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9e3779b97f4a7c15ULL;
@@ -579,22 +588,42 @@ __device__ __forceinline__ double u01(uint64_t seed, uint64_t a, uint64_t b) {
   return ((r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-// Gaussian jiggle proposal (QuickCBC.c:2290-2291 shape): x' = x + sqrt(T) s z.
+// Per-temperature constants, tc[3 t .. 3 t + 2] = sqrt(T_t), 1 / T_t,
+// ladder^-t with T_t = ladder^t.
+__global__ void k_pt_temps(double* __restrict__ tc, int n_temps, double ladder) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_temps) return;
+  const double T = pow(ladder, (double)t);
+  tc[3 * t] = sqrt(T);
+  tc[3 * t + 1] = 1.0 / T;
+  tc[3 * t + 2] = pow(ladder, -(double)t);
+}
+
+// Gaussian jiggle proposal (QuickCBC.c:2290-2291 shape): x' = x + sqrt(T) s z,
+// z by Box-Muller from the counters (seed, step, point); parameters i and
+// i + 1 (i even) share one pair of uniforms.  lp: storage index, gp: point
+// number of the random counters.
+__device__ __forceinline__ void propose_pair(const double* __restrict__ x, double* __restrict__ y,
+                                             const double* __restrict__ sigma, double sT, int lp, uint64_t gp, int i,
+                                             uint64_t P, uint64_t seed, uint64_t st) {
+  double u1 = u01(seed, st * P + gp, 2 * i), u2 = u01(seed, st * P + gp, 2 * i + 1);
+  double r = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  y[11 * lp + i] = x[11 * lp + i] + sT * sigma[i] * r * c;
+  if (i + 1 < 11) y[11 * lp + i + 1] = x[11 * lp + i + 1] + sT * sigma[i + 1] * r * s;
+}
+
+// Proposals of one group at step *step (used once, at creation; afterwards
+// k_pt_post makes them).
 __global__ void k_pt_propose(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ sigma,
-                             int n_chains, int P, double ladder, uint64_t seed, const unsigned long long* step) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const uint64_t st = *step;
-  const double T = pow(ladder, (double)(p / n_chains));
-  const double sT = sqrt(T);
-  for (int i = 0; i < 11; i += 2) {
-    double u1 = u01(seed, st * (uint64_t)P + p, 2 * i), u2 = u01(seed, st * (uint64_t)P + p, 2 * i + 1);
-    double r = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincospi(2.0 * u2, &s, &c);
-    y[11 * p + i] = x[11 * p + i] + sT * sigma[i] * r * c;
-    if (i + 1 < 11) y[11 * p + i + 1] = x[11 * p + i + 1] + sT * sigma[i + 1] * r * s;
-  }
+                             const double* __restrict__ tc, int n_temps, int n_chains, int c0, int ncols,
+                             uint64_t seed, const unsigned long long* step) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 6 * n_temps * ncols) return;
+  const int lp = q / 6, t = lp / ncols, c = c0 + lp % ncols;
+  propose_pair(x, y, sigma, tc[3 * t], lp, (uint64_t)t * n_chains + c, 2 * (q % 6), (uint64_t)n_temps * n_chains,
+               seed, *step);
 }
 
 __device__ __forceinline__ bool in_prior(const double* q) {
@@ -602,75 +631,90 @@ __device__ __forceinline__ bool in_prior(const double* q) {
          q[10] < 1.0 && q[0] < q[1] + (-0.8317766166719343);  // eta <= 1/4: ln Mc <= ln Mt + 0.6 ln(1/4)
 }
 
-// Metropolis-Hastings at inverse temperature 1/T; flat priors inside bounds.
-__global__ void k_pt_accept(double* __restrict__ x, double* __restrict__ L, const double* __restrict__ y,
-                            const double* __restrict__ Ly, const int* __restrict__ st_y, int n_chains, int P,
-                            double ladder, uint64_t seed, const unsigned long long* step,
-                            unsigned long long* __restrict__ acc) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const double beta = 1.0 / pow(ladder, (double)(p / n_chains));
-  const double u = u01(seed ^ 0x5bd1e995ULL, *step * (uint64_t)P + p, 99);
-  const double* q = y + 11 * p;
-  if (st_y[p] == 0 && in_prior(q) && log(u) < beta * (Ly[p] - L[p])) {
-    for (int i = 0; i < 11; i++) x[11 * p + i] = q[i];
-    L[p] = Ly[p];
-    acc[p] += 1;
-  }
-}
-
-// Adjacent-temperature swaps: one thread per (chain column, temperature
-// pair); even pairs on even steps, odd pairs on odd steps, so no two threads
-// touch the same point.
-__global__ void k_pt_swap(double* __restrict__ x, double* __restrict__ L, int n_temps, int n_chains, double ladder,
-                          uint64_t seed, const unsigned long long* step, unsigned long long* __restrict__ swaps) {
-  const unsigned long long s = *step;
-  const int npairs = (n_temps - (int)(s & 1)) / 2;
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npairs * n_chains) return;
-  const int c = q % n_chains;
-  const int t = (int)(s & 1) + 2 * (q / n_chains);
-  const int a = t * n_chains + c, b = (t + 1) * n_chains + c;
-  const double ba = pow(ladder, -(double)t), bb = pow(ladder, -(double)(t + 1));
-  const double u = u01(seed ^ 0x27d4eb2fULL, s * (uint64_t)(n_temps * n_chains) + a, 7);
-  if (log(u) < (ba - bb) * (L[b] - L[a])) {
-    for (int i = 0; i < 11; i++) {
-      double tmp = x[11 * a + i];
-      x[11 * a + i] = x[11 * b + i];
-      x[11 * b + i] = tmp;
+// Everything of a PT step after the likelihood partial sums, in one launch,
+// for the group of chain columns [c0, c0 + ncols): block b owns column
+// c0 + b (its n_temps points).  logL of the proposals (as k_finalize),
+// Metropolis acceptance at inverse temperature 1/T with flat priors inside
+// the bounds, the adjacent-temperature swaps of the step (even pairs on even
+// steps, odd pairs on odd steps), then the proposals of the next step.  The
+// last block to finish advances the group's step counter and re-arms the
+// completion count, so the kernel replays in a graph with no fill or memset
+// node.
+__global__ __launch_bounds__(1024) void k_pt_post(
+    const double* __restrict__ partial, int splits, int nd, const Point* __restrict__ pts, int mode,
+    const double* __restrict__ wref, double* __restrict__ x, double* __restrict__ L, double* __restrict__ y,
+    double* __restrict__ Ly, double* __restrict__ rho, const int* __restrict__ st_y, const double* __restrict__ sigma,
+    const double* __restrict__ tc, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
+    unsigned long long* __restrict__ step, unsigned long long* __restrict__ acc,
+    unsigned long long* __restrict__ swaps, unsigned int* __restrict__ done) {
+  const int b = blockIdx.x, c = c0 + b;
+  const uint64_t P = (uint64_t)n_temps * n_chains;
+  const uint64_t s = *step;
+  for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
+    const int lp = t * ncols + b;
+    const uint64_t gp = (uint64_t)t * n_chains + c;
+    finalize_point(partial, splits, nd, pts[lp], lp, mode, wref, Ly, rho);
+    const double u = u01(seed ^ 0x5bd1e995ULL, s * P + gp, 99);
+    const double* q = y + 11 * lp;
+    if (st_y[lp] == 0 && in_prior(q) && log(u) < tc[3 * t + 1] * (Ly[lp] - L[lp])) {
+      for (int i = 0; i < 11; i++) x[11 * lp + i] = q[i];
+      L[lp] = Ly[lp];
+      acc[lp] += 1;
     }
-    double tl = L[a];
-    L[a] = L[b];
-    L[b] = tl;
-    atomicAdd(&swaps[c], 1ULL);
+  }
+  __syncthreads();
+  for (int t = (int)(s & 1) + 2 * (int)threadIdx.x; t + 1 < n_temps; t += 2 * (int)blockDim.x) {
+    const int la = t * ncols + b, lb = (t + 1) * ncols + b;
+    const double u = u01(seed ^ 0x27d4eb2fULL, s * P + (uint64_t)t * n_chains + c, 7);
+    if (log(u) < (tc[3 * t + 2] - tc[3 * t + 5]) * (L[lb] - L[la])) {
+      for (int i = 0; i < 11; i++) {
+        double tmp = x[11 * la + i];
+        x[11 * la + i] = x[11 * lb + i];
+        x[11 * lb + i] = tmp;
+      }
+      double tl = L[la];
+      L[la] = L[lb];
+      L[lb] = tl;
+      atomicAdd(&swaps[c], 1ULL);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 6 * n_temps; q += blockDim.x) {
+    const int t = q / 6;
+    propose_pair(x, y, sigma, tc[3 * t], t * ncols + b, (uint64_t)t * n_chains + c, 2 * (q % 6), P, seed, s + 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1u) == gridDim.x - 1) {
+      *step = s + 1;
+      *done = 0u;
+      __threadfence();
+    }
   }
 }
 
-// Advance the step counter once every kernel of the step has read it.
-__global__ void k_pt_tick(unsigned long long* step) {
-  if (threadIdx.x == 0) *step += 1;
-}
-
-hipError_t launch_pt_propose(const double* x, double* y, const double* sigma, int n_chains, int P, double ladder,
-                             uint64_t seed, const unsigned long long* step, hipStream_t s) {
-  hipLaunchKernelGGL(k_pt_propose, dim3((P + 127) / 128), dim3(128), 0, s, x, y, sigma, n_chains, P, ladder, seed, step);
+hipError_t launch_pt_temps(double* tc, int n_temps, double ladder, hipStream_t s) {
+  hipLaunchKernelGGL(k_pt_temps, dim3((n_temps + 63) / 64), dim3(64), 0, s, tc, n_temps, ladder);
   return hipGetLastError();
 }
 
-hipError_t launch_pt_accept(double* x, double* L, const double* y, const double* Ly, const int* st_y, int n_chains,
-                            int P, double ladder, uint64_t seed, const unsigned long long* step,
-                            unsigned long long* acc, hipStream_t s) {
-  hipLaunchKernelGGL(k_pt_accept, dim3((P + 127) / 128), dim3(128), 0, s, x, L, y, Ly, st_y, n_chains, P, ladder, seed,
-                     step, acc);
+hipError_t launch_pt_propose(const double* x, double* y, const double* sigma, const double* tc, int n_temps,
+                             int n_chains, int c0, int ncols, uint64_t seed, const unsigned long long* step,
+                             hipStream_t s) {
+  const int n = 6 * n_temps * ncols;
+  hipLaunchKernelGGL(k_pt_propose, dim3((n + 127) / 128), dim3(128), 0, s, x, y, sigma, tc, n_temps, n_chains, c0,
+                     ncols, seed, step);
   return hipGetLastError();
 }
 
-hipError_t launch_pt_swap(double* x, double* L, int n_temps, int n_chains, double ladder, uint64_t seed,
-                          unsigned long long* step, unsigned long long* swaps, hipStream_t s) {
-  int n = ((n_temps + 1) / 2) * n_chains;
-  hipLaunchKernelGGL(k_pt_swap, dim3((n + 255) / 256), dim3(256), 0, s, x, L, n_temps, n_chains, ladder, seed, step,
-                     swaps);
-  hipLaunchKernelGGL(k_pt_tick, dim3(1), dim3(64), 0, s, step);
+hipError_t launch_pt_post(const LLInst& in, int mode, const double* wref, double* x, double* L, double* y, double* Ly,
+                          double* rho, const int* st_y, const double* sigma, const double* tc, int n_temps,
+                          int n_chains, int c0, int ncols, uint64_t seed, unsigned long long* step,
+                          unsigned long long* acc, unsigned long long* swaps, unsigned int* done, hipStream_t s) {
+  const int threads = 6 * n_temps >= 1024 ? 1024 : ((6 * n_temps + 63) / 64) * 64;
+  hipLaunchKernelGGL(k_pt_post, dim3(ncols), dim3(threads), 0, s, in.partial, in.splits, in.nd, in.pts, mode, wref,
+                     x, L, y, Ly, rho, st_y, sigma, tc, n_temps, n_chains, c0, ncols, seed, step, acc, swaps, done);
   return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ struct qcbc_multi {
   std::vector<qcbc_ctx*> ctxs;
   std::vector<int> B;
   int precision = 0;
-  int cpb = 4, tile = 256, split_bins = 4096, blocks = 0;
+  int cpb = 8, tile = 512, split_bins = 4096, blocks = 0;
   int max_nd = 1;
   hipStream_t stream = nullptr;
   int device = 0;

@@ -3,6 +3,15 @@
 // proposals, Metropolis acceptance per temperature, adjacent-temperature
 // swaps).  State, proposals and random counters never leave the device, so a
 // step is a fixed sequence of launches that can be captured as a hipGraph.
+//
+// Chain columns only meet through swaps between temperatures of the same
+// column, so the ensemble splits into groups of columns that step
+// independently, each on a stream of its own with nothing ordering one group
+// against another.  The GPU then overlaps them freely: one group's
+// latency-bound kernels (set-up, and the post-likelihood kernel: acceptance,
+// swaps, next proposals) and the tail of its likelihood kernel run beside
+// another group's likelihood kernel.  Random counters use the whole
+// ensemble's point numbers, so every number of groups gives the same chains.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -16,14 +25,31 @@
 
 using namespace qcbc;
 
+namespace {
+struct Group {
+  int c0 = 0, ncols = 0, P = 0, off = 0;  // columns [c0, c0 + ncols), points [off, off + P) of the storage
+  hipStream_t s = nullptr;
+  bool own_stream = false;
+  Point* d_pts = nullptr;
+  double* d_partial = nullptr;
+  unsigned long long* d_step = nullptr;
+  unsigned int* d_done = nullptr;  // blocks of k_pt_post finished this step
+};
+}  // namespace
+
 struct qcbc_pt {
   qcbc_ctx* ctx = nullptr;
-  int n_temps = 0, n_chains = 0, P = 0, precision = 0;
+  int n_temps = 0, n_chains = 0, P = 0, precision = 0, G = 1;
   double ladder = 1.0;
   uint64_t seed = 0;
+  Group g[QCBC_PT_MAX_GROUPS];
+  // storage order: group by group, t * ncols + (c - c0) within a group
   double *d_x = nullptr, *d_L = nullptr, *d_y = nullptr, *d_Ly = nullptr, *d_rho = nullptr, *d_sigma = nullptr;
+  double* d_tc = nullptr;  // per-temperature constants
   int* d_st = nullptr;
-  unsigned long long *d_step = nullptr, *d_acc = nullptr, *d_swaps = nullptr;
+  unsigned long long *d_acc = nullptr, *d_swaps = nullptr;
+  std::vector<int> perm;  // storage index -> point number t * n_chains + c
+  hipEvent_t ev_fork = nullptr, ev_join[QCBC_PT_MAX_GROUPS] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool step_timing = false;
@@ -39,25 +65,61 @@ struct qcbc_pt {
     }                            \
   } while (0)
 
-static int pt_enqueue_step(qcbc_pt* pt) {
+// One PT step of group k: the proposals y are already on the device (made by
+// the previous step, or at creation).  Set-up and likelihood partial sums of
+// y, then k_pt_post: logL of y, acceptance, swaps, the next proposals and the
+// group's step counter.
+static int group_step(qcbc_pt* pt, int k) {
   qcbc_ctx* c = pt->ctx;
-  hipStream_t s = c->stream;
-  HIPCHK(launch_pt_propose(pt->d_x, pt->d_y, pt->d_sigma, pt->n_chains, pt->P, pt->ladder, pt->seed, pt->d_step, s));
-  int st = loglike_core(c, pt->d_y, pt->P, pt->precision, pt->d_Ly, pt->d_rho, pt->d_st);
+  Group& gr = pt->g[k];
+  const int prec = pt->precision;
+  const size_t o = gr.off;
+  HIPCHK(launch_setup(pt->d_y + 11 * o, gr.P, c->an, gr.d_pts, pt->d_st + o, gr.s));
+  const LLInst in = make_inst(c, c->tile[prec], gr.P, effective_splits(c, prec, gr.P), gr.d_pts, gr.d_partial);
+  int st = ll_launch_timed(c, prec, in, gr.s);
   if (st) return st;
-  HIPCHK(launch_pt_accept(pt->d_x, pt->d_L, pt->d_y, pt->d_Ly, pt->d_st, pt->n_chains, pt->P, pt->ladder, pt->seed,
-                          pt->d_step, pt->d_acc, s));
-  HIPCHK(launch_pt_swap(pt->d_x, pt->d_L, pt->n_temps, pt->n_chains, pt->ladder, pt->seed, pt->d_step, pt->d_swaps,
-                        s));
+  HIPCHK(launch_pt_post(in, prec == QCBC_MIXED ? FIN_MIXED : FIN_FP64, c->d_wref, pt->d_x + 11 * o, pt->d_L + o,
+                        pt->d_y + 11 * o, pt->d_Ly + o, pt->d_rho + 3 * o, pt->d_st + o, pt->d_sigma, pt->d_tc,
+                        pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_acc + o, pt->d_swaps,
+                        gr.d_done, gr.s));
   return QCBC_OK;
+}
+
+// One step of the whole ensemble.
+static int pt_enqueue_step(qcbc_pt* pt) {
+  for (int k = 0; k < pt->G; k++) {
+    int st = group_step(pt, k);
+    if (st) return st;
+  }
+  return QCBC_OK;
+}
+
+static void pt_free(qcbc_pt* pt) {
+  for (Group& gr : pt->g) {
+    if (gr.own_stream && gr.s) (void)hipStreamDestroy(gr.s);
+    void* gb[] = {gr.d_pts, gr.d_partial, gr.d_step, gr.d_done};
+    for (void* b : gb)
+      if (b) (void)hipFree(b);
+  }
+  if (pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
+  for (hipEvent_t e : pt->ev_join)
+    if (e) (void)hipEventDestroy(e);
+  if (pt->exec) (void)hipGraphExecDestroy(pt->exec);
+  if (pt->graph) (void)hipGraphDestroy(pt->graph);
+  void* bufs[] = {pt->d_x, pt->d_L, pt->d_y, pt->d_Ly, pt->d_rho, pt->d_sigma, pt->d_tc, pt->d_st, pt->d_acc,
+                  pt->d_swaps};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete pt;
 }
 
 extern "C" {
 
-int qcbc_pt_create(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init, const double* sigma,
-                   uint64_t seed, int precision, qcbc_pt** out) {
+int qcbc_pt_create_groups(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init,
+                          const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out) {
   if (!ctx || !out || !init || !sigma || n_temps < 1 || n_chains < 1 || n_chains > 1024 || !(ladder >= 1.0) ||
-      (precision != QCBC_FP64 && precision != QCBC_MIXED))
+      (precision != QCBC_FP64 && precision != QCBC_MIXED) || groups < 0 || groups > QCBC_PT_MAX_GROUPS ||
+      groups > n_chains)
     return QCBC_EINVAL;
   if (!ctx->bound) return QCBC_ENOTBOUND;
   *out = nullptr;
@@ -71,32 +133,104 @@ int qcbc_pt_create(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, cons
   pt->ladder = ladder;
   pt->seed = seed;
   pt->precision = precision;
+  pt->G = groups ? groups : (n_chains >= 3 && pt->P >= 1536 ? 3 : 1);
   const size_t P = pt->P;
-  if (hipMalloc(&pt->d_x, sizeof(double) * 11 * P) || hipMalloc(&pt->d_y, sizeof(double) * 11 * P) ||
-      hipMalloc(&pt->d_L, sizeof(double) * P) || hipMalloc(&pt->d_Ly, sizeof(double) * P) ||
-      hipMalloc(&pt->d_rho, sizeof(double) * 3 * P) || hipMalloc(&pt->d_sigma, sizeof(double) * 11) ||
-      hipMalloc(&pt->d_st, sizeof(int) * P) || hipMalloc(&pt->d_step, sizeof(unsigned long long)) ||
-      hipMalloc(&pt->d_acc, sizeof(unsigned long long) * P) ||
-      hipMalloc(&pt->d_swaps, sizeof(unsigned long long) * n_chains)) {
-    qcbc_pt_destroy(pt);
+  int c0 = 0, off = 0;
+  for (int k = 0; k < pt->G; k++) {
+    Group& gr = pt->g[k];
+    gr.c0 = c0;
+    gr.ncols = n_chains / pt->G + (k < n_chains % pt->G ? 1 : 0);
+    gr.P = gr.ncols * n_temps;
+    gr.off = off;
+    c0 += gr.ncols;
+    off += gr.P;
+  }
+  pt->perm.resize(P);
+  for (int k = 0; k < pt->G; k++) {
+    const Group& gr = pt->g[k];
+    for (int t = 0; t < n_temps; t++)
+      for (int j = 0; j < gr.ncols; j++) pt->perm[gr.off + t * gr.ncols + j] = t * n_chains + gr.c0 + j;
+  }
+  bool ok = hipMalloc(&pt->d_x, sizeof(double) * 11 * P) == hipSuccess &&
+            hipMalloc(&pt->d_y, sizeof(double) * 11 * P) == hipSuccess &&
+            hipMalloc(&pt->d_L, sizeof(double) * P) == hipSuccess &&
+            hipMalloc(&pt->d_Ly, sizeof(double) * P) == hipSuccess &&
+            hipMalloc(&pt->d_rho, sizeof(double) * 3 * P) == hipSuccess &&
+            hipMalloc(&pt->d_sigma, sizeof(double) * 11) == hipSuccess &&
+            hipMalloc(&pt->d_tc, sizeof(double) * 3 * n_temps) == hipSuccess &&
+            hipMalloc(&pt->d_st, sizeof(int) * P) == hipSuccess &&
+            hipMalloc(&pt->d_acc, sizeof(unsigned long long) * P) == hipSuccess &&
+            hipMalloc(&pt->d_swaps, sizeof(unsigned long long) * n_chains) == hipSuccess;
+  for (int k = 0; k < pt->G && ok; k++) {
+    Group& gr = pt->g[k];
+    ok = hipMalloc(&gr.d_pts, sizeof(Point) * gr.P) == hipSuccess &&
+         hipMalloc(&gr.d_partial, sizeof(double) * 6 * (size_t)QCBC_MAX_SPLITS * gr.P) == hipSuccess &&
+         hipMalloc(&gr.d_step, sizeof(unsigned long long)) == hipSuccess &&
+         hipMalloc(&gr.d_done, sizeof(unsigned int)) == hipSuccess &&
+         hipEventCreateWithFlags(&pt->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (ok && k == 0) {
+      gr.s = ctx->stream;
+    } else if (ok) {
+      ok = hipStreamCreateWithFlags(&gr.s, hipStreamNonBlocking) == hipSuccess;
+      gr.own_stream = ok;
+    }
+  }
+  ok = ok && hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    pt_free(pt);
     return QCBC_ENOMEM;
   }
+  std::vector<double> x0(11 * P);
+  for (size_t i = 0; i < P; i++) std::memcpy(&x0[11 * i], init + 11 * (size_t)pt->perm[i], sizeof(double) * 11);
   hipStream_t s = ctx->stream;
-  HIPCHK(hipMemcpyAsync(pt->d_x, init, sizeof(double) * 11 * P, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(pt->d_sigma, sigma, sizeof(double) * 11, hipMemcpyHostToDevice, s));
-  HIPCHK(launch_fill_u64(pt->d_step, 1, 0ULL, s));
-  HIPCHK(launch_fill_u64(pt->d_acc, (int)P, 0ULL, s));
-  HIPCHK(launch_fill_u64(pt->d_swaps, n_chains, 0ULL, s));
+  auto fail = [&](int code) {
+    hipStreamSynchronize(s);
+    pt_free(pt);
+    return code;
+  };
+  if (hipMemcpyAsync(pt->d_x, x0.data(), sizeof(double) * 11 * P, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(pt->d_sigma, sigma, sizeof(double) * 11, hipMemcpyHostToDevice, s) != hipSuccess ||
+      launch_pt_temps(pt->d_tc, n_temps, ladder, s) != hipSuccess ||
+      launch_fill_u64(pt->d_acc, (int)P, 0ULL, s) != hipSuccess ||
+      launch_fill_u64(pt->d_swaps, n_chains, 0ULL, s) != hipSuccess)
+    return fail(QCBC_EHIP);
+  for (int k = 0; k < pt->G; k++)
+    if (launch_fill_u64(pt->g[k].d_step, 1, 0ULL, s) != hipSuccess ||
+        launch_fill_i32(reinterpret_cast<int*>(pt->g[k].d_done), 1, 0, s) != hipSuccess)
+      return fail(QCBC_EHIP);
   int st = loglike_core(ctx, pt->d_x, pt->P, precision, pt->d_L, pt->d_rho, pt->d_st);
-  if (st) return st;
-  HIPCHK(hipStreamSynchronize(s));
+  if (st) return fail(st);
+  // proposals of step 0
+  for (int k = 0; k < pt->G; k++) {
+    const Group& gr = pt->g[k];
+    if (launch_pt_propose(pt->d_x + 11 * (size_t)gr.off, pt->d_y + 11 * (size_t)gr.off, pt->d_sigma, pt->d_tc,
+                          n_temps, n_chains, gr.c0, gr.ncols, seed, gr.d_step, s) != hipSuccess)
+      return fail(QCBC_EHIP);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
   *out = pt;
+  return QCBC_OK;
+}
+
+int qcbc_pt_create(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init, const double* sigma,
+                   uint64_t seed, int precision, qcbc_pt** out) {
+  return qcbc_pt_create_groups(ctx, n_temps, n_chains, ladder, init, sigma, seed, precision, 0, out);
+}
+
+int qcbc_pt_groups(qcbc_pt* pt, int* groups) {
+  if (!pt || !groups) return QCBC_EINVAL;
+  *groups = pt->G;
   return QCBC_OK;
 }
 
 int qcbc_pt_set_step_timing(qcbc_pt* pt, int on) {
   if (!pt) return QCBC_EINVAL;
   pt->step_timing = on != 0;
+  return QCBC_OK;
+}
+
+static int pt_sync(qcbc_pt* pt) {
+  for (int k = 0; k < pt->G; k++) HIPCHK(hipStreamSynchronize(pt->g[k].s));
   return QCBC_OK;
 }
 
@@ -112,11 +246,22 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
     bool timing = c->timing;
     c->timing = false;  // event records stay out of the graph
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = pt_enqueue_step(pt);
+    // the other groups' streams fork from the context stream and join it again
+    int st = QCBC_OK;
+    if (pt->G > 1 && hipEventRecord(pt->ev_fork, s) != hipSuccess) st = QCBC_EHIP;
+    for (int k = 1; k < pt->G && !st; k++)
+      if (hipStreamWaitEvent(pt->g[k].s, pt->ev_fork, 0) != hipSuccess) st = QCBC_EHIP;
+    if (!st) st = pt_enqueue_step(pt);
+    for (int k = 1; k < pt->G && !st; k++)
+      if (hipEventRecord(pt->ev_join[k], pt->g[k].s) != hipSuccess || hipStreamWaitEvent(s, pt->ev_join[k], 0) != hipSuccess)
+        st = QCBC_EHIP;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     c->timing = timing;
-    if (st) return st;
+    if (st) {
+      if (g) hipGraphDestroy(g);
+      return st;
+    }
     HIPCHK(e);
     pt->graph = g;
     HIPCHK(hipGraphInstantiate(&pt->exec, g, nullptr, nullptr, 0));
@@ -130,13 +275,13 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
       if (st) return st;
     }
     if (pt->step_timing) {
-      HIPCHK(hipStreamSynchronize(s));
+      int st = pt_sync(pt);
+      if (st) return st;
       auto t1 = std::chrono::steady_clock::now();
       pt->lat_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
     }
   }
-  HIPCHK(hipStreamSynchronize(s));
-  return QCBC_OK;
+  return pt_sync(pt);
 }
 
 int qcbc_pt_get(qcbc_pt* pt, double* params, double* logL, long long* accepted, long long* swaps) {
@@ -144,16 +289,23 @@ int qcbc_pt_get(qcbc_pt* pt, double* params, double* logL, long long* accepted, 
   qcbc_ctx* c = pt->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
+  int st = pt_sync(pt);
+  if (st) return st;
   hipStream_t s = c->stream;
   const size_t P = pt->P;
-  if (params) HIPCHK(hipMemcpyAsync(params, pt->d_x, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
-  if (logL) HIPCHK(hipMemcpyAsync(logL, pt->d_L, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  std::vector<double> x(11 * P), L(P);
   std::vector<unsigned long long> acc(P), sw(pt->n_chains);
+  HIPCHK(hipMemcpyAsync(x.data(), pt->d_x, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(L.data(), pt->d_L, sizeof(double) * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(acc.data(), pt->d_acc, sizeof(unsigned long long) * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(sw.data(), pt->d_swaps, sizeof(unsigned long long) * pt->n_chains, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  if (accepted)
-    for (size_t i = 0; i < P; i++) accepted[i] = (long long)acc[i];
+  for (size_t i = 0; i < P; i++) {
+    const size_t p = pt->perm[i];
+    if (params) std::memcpy(params + 11 * p, &x[11 * i], sizeof(double) * 11);
+    if (logL) logL[p] = L[i];
+    if (accepted) accepted[p] = (long long)acc[i];
+  }
   if (swaps) {
     long long t = 0;
     for (auto v : sw) t += (long long)v;
@@ -174,14 +326,8 @@ int qcbc_pt_latencies(qcbc_pt* pt, double* ms, int max, int* n) {
 int qcbc_pt_destroy(qcbc_pt* pt) {
   if (!pt) return QCBC_OK;
   hipSetDevice(pt->ctx->device);
-  hipStreamSynchronize(pt->ctx->stream);
-  if (pt->exec) hipGraphExecDestroy(pt->exec);
-  if (pt->graph) hipGraphDestroy(pt->graph);
-  void* bufs[] = {pt->d_x, pt->d_L, pt->d_y, pt->d_Ly, pt->d_rho, pt->d_sigma, pt->d_st, pt->d_step, pt->d_acc,
-                  pt->d_swaps};
-  for (void* b : bufs)
-    if (b) hipFree(b);
-  delete pt;
+  pt_sync(pt);
+  pt_free(pt);
   return QCBC_OK;
 }
 

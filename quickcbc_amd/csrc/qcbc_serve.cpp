@@ -29,8 +29,8 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-constexpr int kCpb = 4;
-constexpr int kTile = 256;
+constexpr int kCpb = 8;
+constexpr int kTile = 512;
 constexpr int kSplitBins = 4096;
 
 struct Request {

@@ -204,6 +204,37 @@ def test_pt_graph_equals_eager():
     c.close()
 
 
+def test_pt_groups_give_the_same_chains():
+    """Two column groups pipelined on two streams step exactly as one group:
+    eager, with per-step synchronisation, and replayed as a graph."""
+    an, inj, D, SN = _data.analysis(1)
+    c = Q.Context(an, D, SN)
+    nt, nc = 8, 64
+    init = W.ensemble_around(inj, nt * nc, seed=5)
+    sigma = np.array([2e-5, 1e-4, 2e-3, 2e-3, 0.05, 2e-5, 0.01, 0.01, 0.01, 0.02, 0.01])
+    one = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=1)
+    two = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=2)
+    auto = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=3)
+    assert (one.groups, two.groups, auto.groups) == (1, 2, 3)
+    one.run(7)
+    two.run(3)
+    two.step_timing(True)
+    two.run(2)
+    two.step_timing(False)
+    two.run(2)
+    auto.run(7, graph=True)
+    r1, r2, r3 = one.get(), two.get(), auto.get()
+    for r in (r2, r3):
+        assert np.array_equal(r1[0], r[0]) and np.array_equal(r1[1], r[1])
+        assert np.array_equal(r1[2], r[2]) and r1[3] == r[3]
+    assert r1[2].sum() > 0 and r1[3] > 0
+    Lc, _, _ = c.loglike(r2[0], Q.FP64)
+    assert np.allclose(Lc, r2[1], rtol=1e-12, atol=1e-9)
+    for pt in (one, two, auto):
+        pt.close()
+    c.close()
+
+
 def test_dropins_against_oracle():
     an, inj, D, SN = _data.analysis(0)
     Q.lib().qcbc_set_band(an.fmin, an.fmax)
