@@ -229,8 +229,8 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
 # 1 intermediate, 2 merger-ringdown; the quarter-angle sincos of the
 # projected argument; and per detector the inner-product terms, plus the
 # rotation and its recurrence for every detector after the first.
-FUSED_WAVEFORM_FLOPS = {(0, 0): 57, (0, 1): 40, (0, 2): 46, (1, 0): 49, (1, 1): 26, (1, 2): 32,
-                        (2, 0): 45, (2, 1): 22, (2, 2): 28}
+FUSED_WAVEFORM_FLOPS = {(0, 0): 54, (0, 1): 36, (0, 2): 42, (1, 0): 47, (1, 1): 27, (1, 2): 33,
+                        (2, 0): 44, (2, 1): 24, (2, 2): 30}
 FUSED_SINCOS_FLOPS = 37
 
 

@@ -781,7 +781,8 @@ __device__ inline int first_bin_at(double thr, double Msec, double Tobs, int lo,
 // indices, and the amplitude and the projected argument with every
 // per-point constant folded in (the projected detector-0 argument
 // f tshift_0 + (lambda_0 - Psi)/2 pi in turns, Psi = phi(Mf) - t0 (Mf - MfRef)
-// - phi_precalc as phase() evaluates it).
+// - phi_precalc as phase() evaluates it; the kernels add the f tshift_0 term,
+// up to ~3e4 turns, with one rounding).
 __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   c.seg_a1 = first_bin_at(AMP_JOIN, c.Msec, an.Tobs, c.kstart, c.kend);
   c.seg_a2 = first_bin_at(c.fmaxCalc, c.Msec, an.Tobs, c.kstart, c.kend);
@@ -789,11 +790,10 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   c.seg_p2 = first_bin_at(c.fPhiMRD, c.Msec, an.Tobs, c.kstart, c.kend);
   const double K = -INV_TPI;
   const double G = (LOG_PI + c.logMsec) / 3.0;  // log v = G + log(f) / 3
-  const double lin = c.tshift[0] / c.Msec;      // f tshift_0 = Mf tshift_0 / Msec
   const double base = (c.lambda[0] + c.phi_precalc - c.t0 * c.MfRef) * INV_TPI;
   // inspiral (PhiInsAnsatzInt)
   c.ins_q0 = base - (c.pi_init + c.pi_L * G) * INV_TPI;
-  c.ins_q1 = lin + (c.t0 - c.pi_1 * c.inv_eta) * INV_TPI;
+  c.ins_q1 = (c.t0 - c.pi_1 * c.inv_eta) * INV_TPI;
   c.ins_q13 = K * (c.pi_13 + c.pi_13L * G);
   c.ins_q13L = K * c.pi_13L / 3.0;
   c.ins_qL = K * c.pi_L / 3.0;
@@ -808,12 +808,12 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   // intermediate: inv_eta (beta1 Mf - beta3 / (3 Mf^3) + beta2 log Mf) + C1Int + C2Int Mf
   c.int_qL = K * c.inv_eta * c.beta2;
   c.int_q0 = base - c.C1Int * INV_TPI + c.int_qL * c.logMsec;
-  c.int_q1 = lin + (c.t0 - c.C2Int - c.inv_eta * c.beta1) * INV_TPI;
+  c.int_q1 = (c.t0 - c.C2Int - c.inv_eta * c.beta1) * INV_TPI;
   c.int_qm3 = c.inv_eta * c.beta3 * INV_TPI / 3.0;
   // merger-ringdown: inv_eta (-alpha2 / Mf + 4/3 alpha3 Mf^(3/4) + alpha1 Mf
   //                  + alpha4 atan((Mf - a5fRD) / fDM)) + C1MRD + C2MRD Mf
   c.mrd_q0 = base - c.C1MRD * INV_TPI;
-  c.mrd_q1 = lin + (c.t0 - c.C2MRD - c.inv_eta * c.alpha1) * INV_TPI;
+  c.mrd_q1 = (c.t0 - c.C2MRD - c.inv_eta * c.alpha1) * INV_TPI;
   c.mrd_qm1 = c.inv_eta * c.alpha2 * INV_TPI;
   c.mrd_q34 = K * c.inv_eta * (4.0 / 3.0) * c.alpha3;
   c.mrd_qat = K * c.inv_eta * c.alpha4;

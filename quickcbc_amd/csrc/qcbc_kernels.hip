@@ -108,33 +108,41 @@ __device__ __forceinline__ void block_range(const LLInst& in, int b, bool active
   *hi = s_lohi[1];
 }
 
+// One Horner step on negated partial sums: -(c + x p) from np = -p.  The
+// negated third operand keeps the FMA in its three-address form (the
+// two-address form would first copy c into the destination register).
+__device__ __forceinline__ double hstep(double np, double x, double c) { return fma(np, x, -c); }
+
 // Amplitude without the detector scale, and the projected detector-0
 // argument in turns, of one bin in amplitude region AR and phase region PR
 // (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
-// constants of the point and the bin's f, f^(1/6), f^(-1/6), log f.
+// constants of the point and the bin's f, f^(1/6), f^(-1/6), log f.  The
+// inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
+// they need no other power of Mf.
 template <int AR, int PR>
 __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, double is6f, double logf,
                                           double* a, double* r) {
-  const double Mf = c.Msec * f;
-  const double i_s6 = c.is6_scale * is6f;
-  const double i_third = i_s6 * i_s6;
-  const double i_two_thirds = i_third * i_third;
-  const double i_Mf = i_third * i_two_thirds;
-  double third = 0.0, two_thirds = 0.0, four_thirds = 0.0, five_thirds = 0.0, two = 0.0;
+  const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
+  const double w = i_s6 * i_s6;            // Mf^(-1/3)
+  const double w3 = w * w * w;             // 1 / Mf
+  double Mf = 0.0, v = 0.0;
+  if (AR > 0 || PR > 0) Mf = c.Msec * f;
   if (AR == 0 || PR == 0) {
     const double s6 = c.s6_scale * s6f;
-    third = s6 * s6;
-    two_thirds = third * third;
-    four_thirds = Mf * third;
-    five_thirds = four_thirds * third;
-    two = Mf * Mf;
+    v = s6 * s6;  // Mf^(1/3)
   }
   double A;
   if (AR == 0) {
-    const double seven_thirds = third * two, eight_thirds = two_thirds * two, three = Mf * two;
-    const double lo = fma(c.ia53, five_thirds, fma(c.ia43, four_thirds, fma(c.ia1, Mf, c.ia23 * two_thirds)));
-    const double hi = fma(c.ia3, three, fma(c.ia83, eight_thirds, fma(c.ia73, seven_thirds, c.ia2 * two)));
-    A = (lo + c.ia0) + hi;
+    // amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ai_43 v^4 + ... + ai_3 v^9), Horner
+    // on the negated partial sums
+    double np = -(v * c.ia3) - c.ia83;
+    np = hstep(np, v, c.ia73);
+    np = hstep(np, v, c.ia2);
+    np = hstep(np, v, c.ia53);
+    np = hstep(np, v, c.ia43);
+    np = hstep(np, v, c.ia1);
+    np = hstep(np, v, c.ia23);
+    A = -hstep(np, v * v, c.ia0);
   } else if (AR == 1) {
     const double x1 = Mf - c.col_f1;
     A = fma(x1, fma(x1, (Mf - c.col_f2) * ((Mf - c.col_f3) * c.ib4 + c.ib3) + c.ib2, c.ib1), c.ib0);
@@ -142,24 +150,34 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
     const double df = Mf - c.fRD;
     A = c.mnum * exp(-df * c.mrd_rate) / fma(df, df, c.mg2);
   }
-  *a = (i_Mf * i_s6) * A;  // Mf^(-7/6)
+  *a = (w3 * i_s6) * A;  // Mf^(-7/6)
+  // the projected argument: f tshift_0, exact product with one rounding, plus
+  // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
+  double g;
   if (PR == 0) {
-    const double i_five_thirds = i_Mf * i_two_thirds;
-    const double neg = fma(c.ins_qm53, i_five_thirds, fma(c.ins_qm1, i_Mf, fma(c.ins_qm23, i_two_thirds,
-                       c.ins_qm13 * i_third)));
-    const double pos = fma(c.ins_q2, two, fma(c.ins_q53, five_thirds, fma(c.ins_q43, four_thirds,
-                       fma(c.ins_q23, two_thirds, fma(c.ins_qL, logf, fma(third, c.ins_q13L * logf + c.ins_q13,
-                       c.ins_q1 * Mf))))));
-    *r = (neg + c.ins_q0) + pos;
+    // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
+    // + (qL + q13L v) log f
+    double np = -(v * c.ins_q2) - c.ins_q53;
+    np = hstep(np, v, c.ins_q43);
+    np = np * v;  // no v^3 term: q1 Mf is added below
+    np = hstep(np, v, c.ins_q23);
+    np = hstep(np, v, c.ins_q13);  // -(q13 + q23 v + q43 v^3 + q53 v^4 + q2 v^5)
+    double nn = -(w * c.ins_qm53);
+    nn = hstep(nn, w, c.ins_qm1);
+    nn = hstep(nn, w, c.ins_qm23);
+    nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
+    const double lg = (v * c.ins_q13L + c.ins_qL) * logf;
+    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, fma(c.ins_q1, c.Msec * f, lg));
   } else if (PR == 1) {
-    const double i_Mf3 = i_Mf * i_Mf * i_Mf;
-    *r = fma(c.int_qm3, i_Mf3, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
+    const double w9 = w3 * w3 * w3;  // 1 / Mf^3
+    g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
   } else {
     const double sq = sqrt(Mf);
     const double f34 = sqrt(Mf * sq);
-    *r = fma(c.mrd_qat, atan((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, i_Mf, c.mrd_q1 * Mf))) +
-         c.mrd_q0;
+    g = fma(c.mrd_qat, atan((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
+        c.mrd_q0;
   }
+  *r = fma(f, c.tshift[0], g);
 }
 
 // Walks the segments of [pos, hi): SEG<AR, PR>::run(st, pos, end) for each

@@ -40,7 +40,7 @@ struct alignas(16) Point {
   // merger-ringdown (Mf >= fPhiMRD); clipped to [kstart, kend].
   int seg_a1, seg_a2, seg_p1, seg_p2;
   // Projected detector-0 argument in turns, f tshift_0 + (lambda_0 - Psi)/2 pi,
-  // as a function of Mf per phase region.
+  // is f tshift_0 plus a function of Mf per phase region:
   //   inspiral:      q0 + q1 Mf + Mf^(1/3) (q13 + q13L log f) + qL log f + q23 Mf^(2/3)
   //                  + q43 Mf^(4/3) + q53 Mf^(5/3) + q2 Mf^2 + qm13 Mf^(-1/3)
   //                  + qm23 Mf^(-2/3) + qm1 / Mf + qm53 Mf^(-5/3)

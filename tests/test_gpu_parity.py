@@ -68,6 +68,8 @@ def test_loglike_fp64(ctx_cfg):
     L, R, S = c.loglike(P, Q.FP64)
     Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
     assert (S == Sr).all()
+    print("fused FP64 |dlogL| max %.3e (|logL| up to %.0f), |drho| max %.3e" %
+          (np.abs(L - Lr).max(), np.abs(Lr).max(), np.abs(R - Rr).max()))
     assert np.abs(L - Lr).max() <= 1e-8, np.abs(L - Lr).max()
     assert np.abs(R - Rr).max() <= 1e-8
 
