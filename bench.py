@@ -29,10 +29,11 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TF = 78.6        # MI355X FP64, vector ALU and matrix cores alike
 FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
 
-# HBM bytes per launch of the fused FP64 kernel on this workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/k_ll_fp64_pt_step_hbm_traffic.md),
-# each scaled by what the counter reads for a 1 GiB device copy.
-K_LL_FP64_TRAFFIC_BYTES = None
+# HBM bytes per launch of the fused FP64 kernel on this workload (one launch
+# over the 2048 points), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (profiles/k_ll_fp64_pt_step_hbm_traffic.md), each scaled by what the counter
+# reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+K_LL_FP64_TRAFFIC_BYTES = 31_268_000
 
 
 def log(*a):
