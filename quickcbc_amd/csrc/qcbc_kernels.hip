@@ -108,6 +108,11 @@ __device__ __forceinline__ void block_range(const LLInst& in, int b, bool active
   *hi = s_lohi[1];
 }
 
+// Makes v opaque to the optimiser in place (empty asm): later subtractions of
+// scalar constants from it stay single subtractions instead of FMAs that need
+// the constant copied into a vector register.
+__device__ __forceinline__ void vreg(double& v) { asm("" : "+v"(v)); }
+
 // One Horner step on negated partial sums: -(c + x p) from np = -p.  The
 // negated third operand keeps the FMA in its three-address form (the
 // two-address form would first copy c into the destination register).
@@ -126,7 +131,10 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
   const double w = i_s6 * i_s6;            // Mf^(-1/3)
   const double w3 = w * w * w;             // 1 / Mf
   double Mf = 0.0, v = 0.0;
-  if (AR > 0 || PR > 0) Mf = c.Msec * f;
+  if (AR > 0 || PR > 0) {
+    Mf = c.Msec * f;
+    vreg(Mf);  // Mf - node stays one subtraction with the node in a scalar register
+  }
   if (AR == 0 || PR == 0) {
     const double s6 = c.s6_scale * s6f;
     v = s6 * s6;  // Mf^(1/3)
@@ -144,8 +152,12 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
     np = hstep(np, v, c.ia23);
     A = -hstep(np, v * v, c.ia0);
   } else if (AR == 1) {
+    // Newton form on the nodes (f1, f1, f2, f3), Horner on negated partials
     const double x1 = Mf - c.col_f1;
-    A = fma(x1, fma(x1, (Mf - c.col_f2) * ((Mf - c.col_f3) * c.ib4 + c.ib3) + c.ib2, c.ib1), c.ib0);
+    double np = -((Mf - c.col_f3) * c.ib4) - c.ib3;
+    np = hstep(np, Mf - c.col_f2, c.ib2);
+    np = hstep(np, x1, c.ib1);
+    A = -hstep(np, x1, c.ib0);
   } else {
     const double df = Mf - c.fRD;
     A = c.mnum * exp(-df * c.mrd_rate) / fma(df, df, c.mg2);
