@@ -2,10 +2,14 @@
 // all of them batched continuously into persistent queue launches.
 //
 // Requests that arrive while a batch is on the GPU wait in the queue and go
-// into the next batch together, whatever analysis they belong to.  Each
-// analysis keeps a fixed bin-split length and the kernels' tile grid is fixed,
-// so a point's result is bit-for-bit the same alone or in any mix.
-// Admission charges each analysis its resident footprint against the budget.
+// into the next batch together, whatever analysis they belong to.  The worker
+// keeps two batch slots (device and pinned staging, a stream each): it builds
+// and enqueues the next batch while the previous one runs, then completes the
+// previous one, so the GPU does not idle while the host gathers, packs and
+// unpacks.  Each analysis keeps a fixed bin-split length and the kernels' tile
+// grid is fixed, so a point's result is bit-for-bit the same alone or in any
+// mix.  Admission charges each analysis its resident footprint against the
+// budget.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -53,6 +57,25 @@ struct Served {
   int in_flight = 0;
 };
 
+// Device and pinned staging of one batch, and the batch's layout.
+struct Slot {
+  hipStream_t stream = nullptr;
+  double *d_params = nullptr, *d_logL = nullptr, *d_rho = nullptr, *d_partial = nullptr;
+  int* d_status = nullptr;
+  Point* d_pts = nullptr;
+  LLInst* d_insts = nullptr;
+  LLItem* d_items = nullptr;
+  int* d_counter = nullptr;
+  int cap_items = 0, cap_insts = 0;
+  double *h_params = nullptr, *h_logL = nullptr, *h_rho = nullptr;  // pinned
+  int* h_status = nullptr;
+  // the batch on this slot: its requests by analysis, point offsets, detectors
+  std::vector<std::shared_ptr<Request>> batch;
+  std::vector<std::vector<Request*>> groups;
+  std::vector<int> offs, nds;
+  int P = 0;
+};
+
 }  // namespace
 
 struct qcbc_server {
@@ -69,17 +92,7 @@ struct qcbc_server {
   int64_t next_rid = 1;
   bool stop = false;
   std::thread worker;
-  // device staging for one batch
-  hipStream_t stream = nullptr;
-  double *d_params = nullptr, *d_logL = nullptr, *d_rho = nullptr, *d_partial = nullptr;
-  int* d_status = nullptr;
-  Point* d_pts = nullptr;
-  LLInst* d_insts = nullptr;
-  LLItem* d_items = nullptr;
-  int* d_counter = nullptr;
-  int cap_items = 0, cap_insts = 0;
-  double *h_params = nullptr, *h_logL = nullptr, *h_rho = nullptr;  // pinned
-  int* h_status = nullptr;
+  Slot slots[2];
   size_t scratch_bytes = 0;
   long long evals = 0, batches = 0;
   int blocks = 1024;
@@ -94,114 +107,151 @@ struct qcbc_server {
     }                            \
   } while (0)
 
-static int grow_items(qcbc_server* s, int n_items, int n_insts) {
-  if (n_items > s->cap_items) {
-    if (s->d_items) hipFree(s->d_items);
-    s->d_items = nullptr;
-    int cap = std::max(n_items, 2 * s->cap_items);
-    HIPCHK(hipMalloc(&s->d_items, sizeof(LLItem) * cap));
-    s->cap_items = cap;
+static int grow_items(Slot* sl, int n_items, int n_insts) {
+  if (n_items > sl->cap_items) {
+    if (sl->d_items) (void)hipFree(sl->d_items);
+    sl->d_items = nullptr;
+    int cap = std::max(n_items, 2 * sl->cap_items);
+    HIPCHK(hipMalloc(&sl->d_items, sizeof(LLItem) * cap));
+    sl->cap_items = cap;
   }
-  if (n_insts > s->cap_insts) {
-    if (s->d_insts) hipFree(s->d_insts);
-    s->d_insts = nullptr;
-    int cap = std::max(n_insts, 2 * s->cap_insts);
-    HIPCHK(hipMalloc(&s->d_insts, sizeof(LLInst) * cap));
-    s->cap_insts = cap;
+  if (n_insts > sl->cap_insts) {
+    if (sl->d_insts) (void)hipFree(sl->d_insts);
+    sl->d_insts = nullptr;
+    int cap = std::max(n_insts, 2 * sl->cap_insts);
+    HIPCHK(hipMalloc(&sl->d_insts, sizeof(LLInst) * cap));
+    sl->cap_insts = cap;
   }
   return QCBC_OK;
 }
 
-// Evaluate one batch: requests grouped by analysis, one set-up launch per
-// analysis, one persistent queue launch for everything, finalize per analysis.
-static int run_batch(qcbc_server* s, std::vector<std::shared_ptr<Request>>& batch) {
+// Pack one batch into slot sl and enqueue it on the slot's stream: requests
+// grouped by analysis, one set-up launch per analysis, one persistent queue
+// launch for everything (items longest first), finalize per analysis, and the
+// copies of the results back to pinned memory.  Returns without waiting.
+static int enqueue_batch(qcbc_server* s, Slot& sl) {
   std::map<int, std::vector<Request*>> by_a;
-  for (auto& r : batch) by_a[r->aid].push_back(r.get());
+  for (auto& r : sl.batch) by_a[r->aid].push_back(r.get());
   std::vector<LLInst> insts;
-  std::vector<std::pair<double, LLItem>> items;
-  std::vector<int> offs;  // point offset of each analysis in the staging
   std::vector<qcbc_ctx*> ctxs;
-  int P = 0;
+  sl.groups.clear();
+  sl.offs.clear();
+  sl.nds.clear();
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    for (auto& kv : by_a) {
-      Served& a = s->analyses.at(kv.first);
-      ctxs.push_back(a.ctx);
-    }
+    for (auto& kv : by_a) ctxs.push_back(s->analyses.at(kv.first).ctx);
   }
-  int ai = 0;
+  // per analysis: the item cost (bins x points x detectors) and its item count
+  std::vector<std::pair<double, int>> order;
+  int P = 0, ai = 0;
   for (auto& kv : by_a) {
     qcbc_ctx* c = ctxs[ai];
     int Ba = 0;
     for (Request* r : kv.second) {
-      std::memcpy(s->h_params + (size_t)11 * (P + Ba), r->params.data(), sizeof(double) * 11 * r->B);
+      std::memcpy(sl.h_params + (size_t)11 * (P + Ba), r->params.data(), sizeof(double) * 11 * r->B);
       Ba += r->B;
     }
     int splits = std::min(QCBC_MAX_SPLITS, std::max(1, (c->nb + kSplitBins - 1) / kSplitBins));
-    LLInst in = make_inst(c, kTile, Ba, splits, s->d_pts + P, s->d_partial + (size_t)6 * QCBC_MAX_SPLITS * P);
-    const int groups = (Ba + kCpb - 1) / kCpb;
-    for (int g = 0; g < groups; g++)
-      for (int sp = 0; sp < in.splits; sp++) {
-        int bins = std::min(in.split_len, c->nb - sp * in.split_len);
-        int npts = std::min(kCpb, Ba - g * kCpb);
-        items.push_back({(double)bins * npts * (1.0 + 0.5 * in.nd), LLItem{ai, g, sp, 0}});
-      }
+    LLInst in = make_inst(c, kTile, Ba, splits, sl.d_pts + P, sl.d_partial + (size_t)6 * QCBC_MAX_SPLITS * P);
+    order.push_back({(double)in.split_len * kCpb * (1.0 + 0.5 * in.nd), ai});
     insts.push_back(in);
-    offs.push_back(P);
+    sl.groups.push_back(kv.second);
+    sl.offs.push_back(P);
+    sl.nds.push_back(in.nd);
     P += Ba;
     ai++;
   }
-  std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  std::vector<LLItem> flat(items.size());
-  for (size_t k = 0; k < items.size(); k++) flat[k] = items[k].second;
-  int st = grow_items(s, (int)flat.size(), (int)insts.size());
+  sl.P = P;
+  // longest items first: analyses by per-item cost, then their (group, split) items
+  std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<LLItem> flat;
+  for (auto& o : order) {
+    const LLInst& in = insts[o.second];
+    const int groups = (in.B + kCpb - 1) / kCpb;
+    for (int g = 0; g < groups; g++)
+      for (int sp = 0; sp < in.splits; sp++) flat.push_back(LLItem{o.second, g, sp, 0});
+  }
+  int st = grow_items(&sl, (int)flat.size(), (int)insts.size());
   if (st) return st;
-  hipStream_t q = s->stream;
+  hipStream_t q = sl.stream;
   int max_nd = 1;
   for (auto& in : insts) max_nd = std::max(max_nd, in.nd);
-  HIPCHK(hipMemcpyAsync(s->d_params, s->h_params, sizeof(double) * 11 * P, hipMemcpyHostToDevice, q));
-  HIPCHK(hipMemcpyAsync(s->d_insts, insts.data(), sizeof(LLInst) * insts.size(), hipMemcpyHostToDevice, q));
-  HIPCHK(hipMemcpyAsync(s->d_items, flat.data(), sizeof(LLItem) * flat.size(), hipMemcpyHostToDevice, q));
+  HIPCHK(hipMemcpyAsync(sl.d_params, sl.h_params, sizeof(double) * 11 * P, hipMemcpyHostToDevice, q));
+  HIPCHK(hipMemcpyAsync(sl.d_insts, insts.data(), sizeof(LLInst) * insts.size(), hipMemcpyHostToDevice, q));
+  HIPCHK(hipMemcpyAsync(sl.d_items, flat.data(), sizeof(LLItem) * flat.size(), hipMemcpyHostToDevice, q));
   for (size_t i = 0; i < insts.size(); i++)
-    HIPCHK(launch_setup(s->d_params + (size_t)11 * offs[i], insts[i].B, ctxs[i]->an, s->d_pts + offs[i],
-                        s->d_status + offs[i], q));
-  HIPCHK(launch_fill_i32(s->d_counter, 1, 0, q));
-  HIPCHK(launch_ll_queue(s->precision, s->d_insts, s->d_items, (int)flat.size(), s->d_counter, max_nd, kCpb, kTile,
+    HIPCHK(launch_setup(sl.d_params + (size_t)11 * sl.offs[i], insts[i].B, ctxs[i]->an, sl.d_pts + sl.offs[i],
+                        sl.d_status + sl.offs[i], q));
+  HIPCHK(launch_fill_i32(sl.d_counter, 1, 0, q));
+  HIPCHK(launch_ll_queue(s->precision, sl.d_insts, sl.d_items, (int)flat.size(), sl.d_counter, max_nd, kCpb, kTile,
                          std::min(s->blocks, std::max(1, (int)flat.size())), q));
   for (size_t i = 0; i < insts.size(); i++)
-    HIPCHK(launch_finalize(insts[i].partial, insts[i].splits, insts[i].nd, s->d_pts + offs[i], insts[i].B,
-                           s->precision == QCBC_MIXED ? FIN_MIXED : FIN_FP64, ctxs[i]->d_wref, s->d_logL + offs[i],
-                           s->d_rho + (size_t)3 * offs[i], q));
+    HIPCHK(launch_finalize(insts[i].partial, insts[i].splits, insts[i].nd, sl.d_pts + sl.offs[i], insts[i].B,
+                           s->precision == QCBC_MIXED ? FIN_MIXED : FIN_FP64, ctxs[i]->d_wref, sl.d_logL + sl.offs[i],
+                           sl.d_rho + (size_t)3 * sl.offs[i], q));
   // results: logL (P), rho packed 3 per point slot, status (P)
-  HIPCHK(hipMemcpyAsync(s->h_logL, s->d_logL, sizeof(double) * P, hipMemcpyDeviceToHost, q));
-  HIPCHK(hipMemcpyAsync(s->h_rho, s->d_rho, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, q));
-  HIPCHK(hipMemcpyAsync(s->h_status, s->d_status, sizeof(int) * P, hipMemcpyDeviceToHost, q));
-  HIPCHK(hipStreamSynchronize(q));
-  // scatter back (finalize wrote rho with stride nd from each analysis offset*3)
-  ai = 0;
-  for (auto& kv : by_a) {
-    int off = offs[ai], nd = insts[ai].nd, k = 0;
-    for (Request* r : kv.second) {
-      r->logL.assign(s->h_logL + off + k, s->h_logL + off + k + r->B);
-      r->rho.assign(s->h_rho + (size_t)3 * off + (size_t)nd * k, s->h_rho + (size_t)3 * off + (size_t)nd * (k + r->B));
-      r->status.assign(s->h_status + off + k, s->h_status + off + k + r->B);
-      k += r->B;
-    }
-    ai++;
-  }
-  s->evals += P;
-  s->batches++;
+  HIPCHK(hipMemcpyAsync(sl.h_logL, sl.d_logL, sizeof(double) * P, hipMemcpyDeviceToHost, q));
+  HIPCHK(hipMemcpyAsync(sl.h_rho, sl.d_rho, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, q));
+  HIPCHK(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int) * P, hipMemcpyDeviceToHost, q));
   return QCBC_OK;
 }
 
+// Wait for slot sl's batch and hand each request its results (st: the
+// enqueue's status).
+static void complete_batch(qcbc_server* s, Slot& sl, int st) {
+  if (!st && hipStreamSynchronize(sl.stream) != hipSuccess) st = QCBC_EHIP;
+  if (!st) {
+    // finalize wrote rho with stride nd from each analysis offset * 3
+    for (size_t ai = 0; ai < sl.groups.size(); ai++) {
+      int off = sl.offs[ai], nd = sl.nds[ai], k = 0;
+      for (Request* r : sl.groups[ai]) {
+        r->logL.assign(sl.h_logL + off + k, sl.h_logL + off + k + r->B);
+        r->rho.assign(sl.h_rho + (size_t)3 * off + (size_t)nd * k, sl.h_rho + (size_t)3 * off + (size_t)nd * (k + r->B));
+        r->status.assign(sl.h_status + off + k, sl.h_status + off + k + r->B);
+        k += r->B;
+      }
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto now = Clock::now();
+    for (auto& r : sl.batch) {
+      r->result = st;
+      r->state = 2;
+      r->t_done = now;
+      s->analyses[r->aid].in_flight--;
+    }
+    if (!st) {
+      s->evals += sl.P;
+      s->batches++;
+    }
+    // analyses evicted while their requests were on the GPU
+    for (auto it = s->analyses.begin(); it != s->analyses.end();) {
+      if (it->second.evicting && it->second.in_flight == 0) {
+        qcbc_ctx_destroy(it->second.ctx);
+        s->in_use -= it->second.bytes;
+        it = s->analyses.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  sl.batch.clear();
+  s->cv_done.notify_all();
+}
+
+// Two-deep pipeline: gather a batch (blocking only when nothing is on the
+// GPU), enqueue it on the free slot, then complete the batch that was already
+// running.
 static void worker_main(qcbc_server* s) {
   hipSetDevice(s->device);
+  int inflight = -1, inflight_st = QCBC_OK;  // slot of the running batch
   for (;;) {
     std::vector<std::shared_ptr<Request>> batch;
+    bool stopping;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-      if (s->stop && s->queue.empty()) break;
+      if (inflight < 0) s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
       int pts = 0;
       while (!s->queue.empty()) {
         auto r = s->queue.front();
@@ -213,31 +263,18 @@ static void worker_main(qcbc_server* s) {
         s->analyses[r->aid].in_flight++;
         batch.push_back(r);
       }
+      stopping = s->stop && s->queue.empty();
     }
-    if (batch.empty()) continue;
-    int st = run_batch(s, batch);
-    std::vector<int> done_aids;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      auto now = Clock::now();
-      for (auto& r : batch) {
-        r->result = st;
-        r->state = 2;
-        r->t_done = now;
-        s->analyses[r->aid].in_flight--;
-      }
-      // analyses evicted while their requests were on the GPU
-      for (auto it = s->analyses.begin(); it != s->analyses.end();) {
-        if (it->second.evicting && it->second.in_flight == 0) {
-          qcbc_ctx_destroy(it->second.ctx);
-          s->in_use -= it->second.bytes;
-          it = s->analyses.erase(it);
-        } else {
-          ++it;
-        }
-      }
+    int next = -1, next_st = QCBC_OK;
+    if (!batch.empty()) {
+      next = inflight == 0 ? 1 : 0;
+      s->slots[next].batch = std::move(batch);
+      next_st = enqueue_batch(s, s->slots[next]);
     }
-    s->cv_done.notify_all();
+    if (inflight >= 0) complete_batch(s, s->slots[inflight], inflight_st);
+    inflight = next;
+    inflight_st = next_st;
+    if (stopping && inflight < 0) break;
   }
 }
 
@@ -253,21 +290,23 @@ int qcbc_server_create(size_t budget_bytes, int precision, qcbc_server** out) {
   s->precision = precision;
   hipGetDevice(&s->device);
   const size_t P = s->max_points + 4096;  // a batch may exceed max_points by one request
-  s->scratch_bytes = P * (sizeof(double) * (11 + 1 + 3) + sizeof(int) + sizeof(Point) +
-                          sizeof(double) * 6 * QCBC_MAX_SPLITS);
-  if (s->scratch_bytes > budget_bytes ||
-      hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&s->d_params, sizeof(double) * 11 * P) != hipSuccess ||
-      hipMalloc(&s->d_logL, sizeof(double) * P) != hipSuccess ||
-      hipMalloc(&s->d_rho, sizeof(double) * 3 * P) != hipSuccess ||
-      hipMalloc(&s->d_status, sizeof(int) * P) != hipSuccess ||
-      hipMalloc(&s->d_pts, sizeof(Point) * P) != hipSuccess ||
-      hipMalloc(&s->d_partial, sizeof(double) * 6 * QCBC_MAX_SPLITS * P) != hipSuccess ||
-      hipMalloc(&s->d_counter, sizeof(int)) != hipSuccess ||
-      hipHostMalloc(&s->h_params, sizeof(double) * 11 * P, 0) != hipSuccess ||
-      hipHostMalloc(&s->h_logL, sizeof(double) * P, 0) != hipSuccess ||
-      hipHostMalloc(&s->h_rho, sizeof(double) * 3 * P, 0) != hipSuccess ||
-      hipHostMalloc(&s->h_status, sizeof(int) * P, 0) != hipSuccess) {
+  s->scratch_bytes = 2 * P * (sizeof(double) * (11 + 1 + 3) + sizeof(int) + sizeof(Point) +
+                              sizeof(double) * 6 * QCBC_MAX_SPLITS);
+  bool ok = s->scratch_bytes <= budget_bytes;
+  for (Slot& sl : s->slots)
+    ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+         hipMalloc(&sl.d_params, sizeof(double) * 11 * P) == hipSuccess &&
+         hipMalloc(&sl.d_logL, sizeof(double) * P) == hipSuccess &&
+         hipMalloc(&sl.d_rho, sizeof(double) * 3 * P) == hipSuccess &&
+         hipMalloc(&sl.d_status, sizeof(int) * P) == hipSuccess &&
+         hipMalloc(&sl.d_pts, sizeof(Point) * P) == hipSuccess &&
+         hipMalloc(&sl.d_partial, sizeof(double) * 6 * QCBC_MAX_SPLITS * P) == hipSuccess &&
+         hipMalloc(&sl.d_counter, sizeof(int)) == hipSuccess &&
+         hipHostMalloc(&sl.h_params, sizeof(double) * 11 * P, 0) == hipSuccess &&
+         hipHostMalloc(&sl.h_logL, sizeof(double) * P, 0) == hipSuccess &&
+         hipHostMalloc(&sl.h_rho, sizeof(double) * 3 * P, 0) == hipSuccess &&
+         hipHostMalloc(&sl.h_status, sizeof(int) * P, 0) == hipSuccess;
+  if (!ok) {
     bool over = s->scratch_bytes > budget_bytes;
     qcbc_server_destroy(s);
     return over ? QCBC_EBUSY : QCBC_ENOMEM;
@@ -401,14 +440,16 @@ int qcbc_server_destroy(qcbc_server* s) {
   s->cv_done.notify_all();
   hipSetDevice(s->device);
   for (auto& kv : s->analyses) qcbc_ctx_destroy(kv.second.ctx);
-  void* dbufs[] = {s->d_params, s->d_logL, s->d_rho, s->d_partial, s->d_status, s->d_pts, s->d_insts, s->d_items,
-                   s->d_counter};
-  for (void* b : dbufs)
-    if (b) hipFree(b);
-  void* hbufs[] = {s->h_params, s->h_logL, s->h_rho, s->h_status};
-  for (void* b : hbufs)
-    if (b) hipHostFree(b);
-  if (s->stream) hipStreamDestroy(s->stream);
+  for (Slot& sl : s->slots) {
+    void* dbufs[] = {sl.d_params, sl.d_logL, sl.d_rho, sl.d_partial, sl.d_status, sl.d_pts, sl.d_insts, sl.d_items,
+                     sl.d_counter};
+    for (void* b : dbufs)
+      if (b) (void)hipFree(b);
+    void* hbufs[] = {sl.h_params, sl.h_logL, sl.h_rho, sl.h_status};
+    for (void* b : hbufs)
+      if (b) (void)hipHostFree(b);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
   delete s;
   return QCBC_OK;
 }
