@@ -843,46 +843,51 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   }
 }
 
-// fulltemplates' parameter handling and projection constants
-// (QuickCBC.c:3700-3750) for one 11-entry parameter vector.
-__device__ inline int point_setup(const double* prm, const AnalysisConst& an, Point& c) {
+// fulltemplates' parameter handling (QuickCBC.c:3700-3718) and IMRPhenomD's
+// set-up for one 11-entry parameter vector.
+__device__ inline int point_setup_model(const double* prm, const AnalysisConst& an, Point& c) {
   c.status = ST_OK;
   c.kstart = c.kend = an.imin;
   const double mc = exp_cr(prm[0]), mt = exp_cr(prm[1]);
-  c.fs = fbegin_m(prm, mc, mt);
   const double eta_in = pow_cr(mc / mt, 5.0 / 3.0);
   const double dm = eta_in > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta_in);
   const double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
-  const double ts = an.Tobs - prm[5];
   int st = phenomd_setup(m1_SI, m2_SI, prm[2], prm[3], exp_cr(prm[6]), 0.5 * prm[4], an.fref, an.fmin0, c);
-  if (st != ST_OK) {
-    c.status = st;
-    return st;
+  if (st != ST_OK) c.status = st;
+  return st;
+}
+
+// The projection constants (QuickCBC.c:3734-3750); Ascale without amp0,
+// which point_setup_finish applies.
+__device__ inline void point_setup_projection(const double* prm, const AnalysisConst& an, Point& c) {
+  const double ts = an.Tobs - prm[5];
+  const double alpha = prm[7], sindelta = prm[8], psi = prm[9], ciota = prm[10];
+  const double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
+  double x[9];
+  antenna_factors(psi, alpha, sindelta, an.gmst, x);
+  // Times uses cos/sin(gmst - alpha) (Utilities.c:1090-1091): the same
+  // angle with the opposite sign as the antenna patterns' alpha - gmst
+  const double sa = -x[5], ca = x[4];
+  for (int d = 0; d < 3; d++) {
+    c.Fs[d] = c.tshift[d] = c.lambda[d] = c.Ascale[d] = 0.0;
+    if (d >= an.n_det) continue;
+    int id = an.labels[d];
+    double Fp, Fc;
+    antenna_id(id, x, &Fp, &Fc);
+    double Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
+    double lam = atan2(Ac * Fc, Ap * Fp);
+    if (lam < 0.0) lam += TPI;
+    c.Fs[d] = Fs;
+    c.lambda[d] = lam;
+    c.tshift[d] = ts - delay_cs(x[2], sindelta, ca, sa, id);
+    c.Ascale[d] = Fs * H22FAC;
   }
-  // projection (QuickCBC.c:3734-3750)
-  {
-    const double alpha = prm[7], sindelta = prm[8], psi = prm[9], ciota = prm[10];
-    const double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
-    double x[9];
-    antenna_factors(psi, alpha, sindelta, an.gmst, x);
-    // Times uses cos/sin(gmst - alpha) (Utilities.c:1090-1091): the same
-    // angle with the opposite sign as the antenna patterns' alpha - gmst
-    const double sa = -x[5], ca = x[4];
-    for (int d = 0; d < 3; d++) {
-      c.Fs[d] = c.tshift[d] = c.lambda[d] = c.Ascale[d] = 0.0;
-      if (d >= an.n_det) continue;
-      int id = an.labels[d];
-      double Fp, Fc;
-      antenna_id(id, x, &Fp, &Fc);
-      double Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
-      double lam = atan2(Ac * Fc, Ap * Fp);
-      if (lam < 0.0) lam += TPI;
-      c.Fs[d] = Fs;
-      c.lambda[d] = lam;
-      c.tshift[d] = ts - delay_cs(x[2], sindelta, ca, sa, id);
-      c.Ascale[d] = Fs * H22FAC * c.amp0 / an.sqT;
-    }
-  }
+}
+
+// After the model, fbegin (c.fs) and the projection: the detector scales,
+// the contributing bins and the fused kernels' constants.
+__device__ inline void point_setup_finish(const AnalysisConst& an, Point& c) {
+  for (int d = 0; d < an.n_det && d < 3; d++) c.Ascale[d] = c.Ascale[d] * c.amp0 / an.sqT;
   // bins that contribute: f_k > fs (QuickCBC.c:3758) and f_k <= fCut
   // (IMRPhenomD.c:530), inside the band; f_k = k / Tobs as QuickCBC.c:419.
   {
@@ -899,6 +904,17 @@ __device__ inline int point_setup(const double* prm, const AnalysisConst& an, Po
     c.kend = k1;
   }
   fused_setup(an, c);
+}
+
+// fulltemplates' parameter handling and projection constants
+// (QuickCBC.c:3700-3750) for one 11-entry parameter vector, in one thread.
+// k_setup runs the model, fbegin and the projection in three waves instead.
+__device__ inline int point_setup(const double* prm, const AnalysisConst& an, Point& c) {
+  int st = point_setup_model(prm, an, c);
+  c.fs = fbegin(prm);
+  if (st != ST_OK) return st;
+  point_setup_projection(prm, an, c);
+  point_setup_finish(an, c);
   return ST_OK;
 }
 

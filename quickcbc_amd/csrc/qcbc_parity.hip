@@ -60,20 +60,38 @@ __global__ void k_qnm_init() {
 }
 
 // ------------------------------------------------------------- set-up
-// One thread per point; the record is built in place in global memory (a
-// ~0.8 KB local copy would not fit in registers).
-__global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int B, AnalysisConst an,
-                                              Point* __restrict__ pts, int* __restrict__ status) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double prm[11];
+// 64 points per block, three waves: wave 0 the model (masses, IMRPhenomD
+// set-up), wave 1 fbegin, wave 2 the projection, each writing its own fields
+// of the record in place in global memory (a ~1 KB local copy would not fit in
+// registers); after the barrier wave 0 finishes the record.  The three parts
+// are independent, so a point's set-up latency is the longest part's instead
+// of their sum.
+__global__ __launch_bounds__(192) void k_setup(const double* __restrict__ params, int B, AnalysisConst an,
+                                               Point* __restrict__ pts, int* __restrict__ status) {
+  const int lane = threadIdx.x & 63, role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * 64 + lane;
+  const bool on = b < B;
+  int st = ST_OK;
+  if (on) {
+    double prm[11];
 #pragma unroll
-  for (int i = 0; i < 11; i++) prm[i] = params[11 * b + i];
-  Point& c = pts[b];
-  int st = point_setup(prm, an, c);
-  c.status = st;
-  if (st != ST_OK) c.kstart = c.kend = an.imin;
-  if (status) status[b] = st;
+    for (int i = 0; i < 11; i++) prm[i] = params[11 * b + i];
+    Point& c = pts[b];
+    if (role == 0)
+      st = point_setup_model(prm, an, c);
+    else if (role == 1)
+      c.fs = fbegin(prm);
+    else
+      point_setup_projection(prm, an, c);
+  }
+  __syncthreads();
+  if (on && role == 0) {
+    Point& c = pts[b];
+    if (st == ST_OK) point_setup_finish(an, c);
+    else c.kstart = c.kend = an.imin;
+    c.status = st;
+    if (status) status[b] = st;
+  }
 }
 
 // One point from explicit IMRPhenomD arguments (the h22 drop-in).
@@ -270,7 +288,7 @@ hipError_t launch_qnm_init(hipStream_t s) {
 hipError_t launch_setup(const double* d_params, int B, const AnalysisConst& an, Point* d_pts, int* d_status,
                         hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3((B + 63) / 64), dim3(64), 0, s, d_params, B, an, d_pts, d_status);
+  hipLaunchKernelGGL(k_setup, dim3((B + 63) / 64), dim3(192), 0, s, d_params, B, an, d_pts, d_status);
   return hipGetLastError();
 }
 
