@@ -49,7 +49,7 @@ def cpu_baseline(an, D, SN, P, seconds):
     as QuickCBC.c:2063) on a bounded sample of the same workload."""
     from oracle import oracle as O
     O.lib()
-    n = 16
+    n = 256
     t0 = time.perf_counter()
     O.loglike_batch(an.labels, an.gmst, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, P[:n])
     dt = time.perf_counter() - t0
