@@ -178,7 +178,7 @@ size_t qcbc_point_record_size(void);
 int qcbc_point_records(qcbc_ctx *ctx, const double *params, int B, void *records);
 
 /* Kernel launch shape of the fused likelihood for this context:
- * chains per block (1..8), bins per LDS tile (128, 256 or 512), bin splits (0 = choose). */
+ * chains per block (1..16), bins per LDS tile (128, 256 or 512), bin splits (0 = choose). */
 int qcbc_ctx_set_shape(qcbc_ctx *ctx, int precision, int chains_per_block, int tile, int splits);
 int qcbc_ctx_get_shape(qcbc_ctx *ctx, int precision, int *chains_per_block, int *tile,
                        int *splits);

@@ -413,7 +413,7 @@ int qcbc_templates(qcbc_ctx* c, const double* params, int B, double* hwave, int*
 // ------------------------------------------------------ shapes / timing
 int qcbc_ctx_set_shape(qcbc_ctx* c, int prec, int cpb, int tile, int splits) {
   if (!c || (prec != 0 && prec != 1)) return QCBC_EINVAL;
-  if (cpb < 1 || cpb > 8 || !ll_tile_ok(tile) || splits < 0 || splits > QCBC_MAX_SPLITS) return QCBC_EINVAL;
+  if (cpb < 1 || cpb > 16 || !ll_tile_ok(tile) || splits < 0 || splits > QCBC_MAX_SPLITS) return QCBC_EINVAL;
   if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) return QCBC_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   c->cpb[prec] = cpb;
@@ -470,7 +470,7 @@ int qcbc_autotune(qcbc_ctx* c, int prec, int B) {
     p[5] = c->desc.Tobs - 2.0; p[6] = std::log(400.0 * MPC); p[7] = 1.0; p[8] = 0.2; p[9] = 0.5; p[10] = 0.3;
   }
   std::vector<double> logL(B);
-  int cpbs[] = {1, 2, 4, 8};
+  int cpbs[] = {1, 2, 4, 8, 16};
   int tiles[] = {128, 256, 512};
   int best_cpb = c->cpb[prec], best_tile = c->tile[prec], best_split = c->splits[prec];
   double best = 1e30;

@@ -840,6 +840,7 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
     const bool on = d < an.n_det;
     c.dt[d] = on ? c.tshift[d] - c.tshift[0] : 0.0;
     c.dl[d] = on ? (c.lambda[d] - c.lambda[0]) * INV_TPI : 0.0;
+    sincospi(128.0 * an.inv_T * c.dt[d], &c.ws[d], &c.wc[d]);
   }
 }
 

@@ -29,6 +29,16 @@ __device__ __forceinline__ double sreg(double v) {
   return v;
 }
 
+// In-place FMAs for loop-carried sums: the result is written over acc's own
+// registers, so a sum updated in a conditional block needs no copy back at
+// its end.  y is wave-uniform (a scalar register).
+__device__ __forceinline__ void fma_acc_x(double& acc, double y, double z) {  // acc = acc y + z
+  asm("v_fma_f64 %0, %0, %1, %2" : "+v"(acc) : "s"(y), "v"(z));
+}
+__device__ __forceinline__ void fma_acc_z(double& acc, double x, double y) {  // acc = x y + acc
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(y));
+}
+
 // sin(2 pi r) and cos(2 pi r) for a phase r given in turns.  t = r - rint(r)
 // (exact, |t| <= 1/2); degree-6 polynomials in t^2 give the quarter angle,
 // sin(pi t / 2) = t P(t^2) and cos(pi t / 2) = Q(t^2), to below 2 ulp
@@ -221,62 +231,60 @@ __device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int
 }
 
 // ---- FP64
+//
+// Detector d > 0 sees the phase of detector 0 plus theta_d(f) = f dt_d + dl_d
+// turns, linear in f.  A lane steps 64 bins at a time, so along its bins
+// theta_d advances by the constant phi_d = 64 dt_d / Tobs, and the lane's sum
+//   sum_m y_m e^{-i theta_d,m},  y_m = a_m D_m e^{-i x0_m} / S_m
+// is a polynomial in e^{-i phi_d} evaluated by Horner's rule,
+//   H <- H e^{i phi_d} + y_m,  sum = e^{-i theta_d,last} H,
+// which takes one complex multiply-add per bin and detector and one sincos
+// per lane and detector at the end.
 template <int ND, int TILE>
 struct F64State {
   const Point* c;
-  const double* lds;
-  int t0, j, age;
-  double acc[2 * ND];
-  // Detector d > 0 differs from detector 0 by the phase f dt_d + dl_d turns,
-  // linear in f.  A lane steps 64 bins at a time, so exp(i theta_d) advances
-  // by the constant rotation w_d = exp(i 2 pi 64 dt_d / Tobs); the recurrence
-  // is re-seeded directly every 32 wave iterations (drift below ~1e-14).
-  double rc[ND], rs[ND], wc[ND], ws[ND];
+  const double* lb;  // LDS tile base minus the tile's first band bin
+  int j;             // the lane's next band bin
+  double hh[ND];     // sum a^2 / S_d
+  double dh0;        // sum a Re(D_0 e^{-i x0}) / S_0
+  double hr[ND], hi[ND];  // Horner sums of detectors d > 0
 };
-
-template <int ND, int TILE>
-__device__ __forceinline__ void f64_reseed(F64State<ND, TILE>& st, double f) {
-#pragma unroll
-  for (int d = 1; d < ND; d++) sincos_turns(fma(f, st.c->dt[d], st.c->dl[d]), &st.rs[d], &st.rc[d]);
-}
 
 template <int ND, int TILE>
 struct F64Seg {
   template <int AR, int PR>
   struct S {
+    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const Point& c) {
+      const double* L = st.lb + st.j;
+      double a, r;
+      amp_turns<AR, PR>(c, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
+      double s0, c0;
+      sincos_turns(r, &s0, &c0);
+      // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the sums carry
+      // a^2 / S and a D e^{-i x_d} / S, and k_finalize applies the per-point
+      // detector scale once.
+      const double a2 = a * a, uc = a * c0, us = a * s0;
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        const double Dre = L[(4 + 3 * d) * TILE], Dim = L[(5 + 3 * d) * TILE];
+        st.hh[d] = fma(a2, L[(6 + 3 * d) * TILE], st.hh[d]);
+        if (d == 0) {
+          st.dh0 = fma(Dre, uc, fma(Dim, us, st.dh0));
+        } else {
+          const double yr = fma(Dre, uc, Dim * us), yi = fma(Dim, uc, -(Dre * us));
+          const double tr = fma(-st.hi[d], c.ws[d], yr);
+          fma_acc_x(st.hi[d], c.wc[d], yi);       // hi wc + yi
+          fma_acc_z(st.hi[d], st.hr[d], c.ws[d]);  // + hr ws
+          fma_acc_x(st.hr[d], c.wc[d], tr);       // hr wc - hi ws + yr
+        }
+      }
+      st.j += 64;
+    }
     static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
       const Point& c = *st.c;
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
-        if (st.age >= 32) {
-          if (st.j < end) f64_reseed(st, st.lds[st.j - st.t0]);
-          st.age = 0;
-        }
-        if (st.j < end) {
-          const double* L = st.lds + (st.j - st.t0);
-          double a, r;
-          amp_turns<AR, PR>(c, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
-          double s0, c0;
-          sincos_turns(r, &s0, &c0);
-          // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the per-bin
-          // sums carry a^2 / S and a Re(D e^{-i x_d}) / S, and k_finalize applies
-          // the per-point detector scale once.
-          const double a2 = a * a;
-#pragma unroll
-          for (int d = 0; d < ND; d++) {
-            const double co = (d == 0) ? c0 : c0 * st.rc[d] - s0 * st.rs[d];
-            const double sn = (d == 0) ? s0 : s0 * st.rc[d] + c0 * st.rs[d];
-            st.acc[2 * d] = fma(a2, L[(6 + 3 * d) * TILE], st.acc[2 * d]);
-            st.acc[2 * d + 1] = fma(a, fma(L[(4 + 3 * d) * TILE], co, L[(5 + 3 * d) * TILE] * sn), st.acc[2 * d + 1]);
-            if (d > 0) {
-              const double t = st.rc[d] * st.wc[d] - st.rs[d] * st.ws[d];
-              st.rs[d] = st.rs[d] * st.wc[d] + st.rc[d] * st.ws[d];
-              st.rc[d] = t;
-            }
-          }
-          st.j += 64;
-        }
-        st.age++;
+        if (st.j < end) bin(st, c);
       }
     }
   };
@@ -295,19 +303,13 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
   F64State<ND, TILE> st;
   st.c = in.pts + uniform(active ? b : 0);
-  st.lds = lds;
   st.j = ks + lane;
-  st.age = 0;
+  st.dh0 = 0.0;
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) st.acc[q] = 0.0;
-#pragma unroll
-  for (int d = 0; d < ND; d++) { st.rc[d] = 1.0; st.rs[d] = 0.0; st.wc[d] = 1.0; st.ws[d] = 0.0; }
+  for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
   int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
   if (active) {
     const Point& c = *st.c;
-#pragma unroll
-    for (int d = 1; d < ND; d++) sincos_turns(64.0 * in.inv_T * c.dt[d], &st.ws[d], &st.wc[d]);
-    if (st.j < ke) f64_reseed(st, in.p64[st.j]);
     a1 = c.seg_a1 - in.imin;
     a2 = c.seg_a2 - in.imin;
     p1 = c.seg_p1 - in.imin;
@@ -322,13 +324,27 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
     }
     __syncthreads();
     if (active) {
-      st.t0 = t0;
+      st.lb = lds - t0;
       pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }
   double acc[2 * ND];
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum(st.acc[q]);
+  for (int d = 0; d < ND; d++) {
+    double dh = st.dh0;
+    if (d > 0) {
+      // back from the lane's last bin: e^{-i theta_d,last} H (zero if the lane took no bin)
+      dh = 0.0;
+      if (active && st.j - 64 >= ks) {
+        const Point& c = *st.c;
+        double s, co;
+        sincos_turns(fma(in.p64[st.j - 64], c.dt[d], c.dl[d]), &s, &co);
+        dh = fma(st.hr[d], co, st.hi[d] * s);
+      }
+    }
+    acc[2 * d] = wave_sum(st.hh[d]);
+    acc[2 * d + 1] = wave_sum(dh);
+  }
   if (lane == 0 && b < in.B) {
 #pragma unroll
     for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
@@ -447,9 +463,10 @@ __device__ __forceinline__ void ll_block(const LLInst& in, int group, int split,
   }
 }
 
-// One analysis: grid x = point groups, y = splits.
+// One analysis: grid x = point groups, y = splits; up to 16 points (waves)
+// per block.
 template <int ND, int PREC, int TILE>
-__global__ __launch_bounds__(512) void k_ll(LLInst in) {
+__global__ __launch_bounds__(1024) void k_ll(LLInst in) {
   extern __shared__ double lds[];
   __shared__ int s_lohi[2];
   if (PREC == 0)

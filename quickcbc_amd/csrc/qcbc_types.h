@@ -57,8 +57,10 @@ struct alignas(16) Point {
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia83, ia3;
   double ib0, ib1, ib2, ib3, ib4;
   double mnum, mg2;
-  // detector d relative to detector 0: f dt_d + dl_d turns
+  // detector d relative to detector 0: f dt_d + dl_d turns, and the rotation
+  // of that phase over 64 bins, exp(i 2 pi 64 dt_d / Tobs) = wc_d + i ws_d
   double dt[3], dl[3];
+  double wc[3], ws[3];
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are

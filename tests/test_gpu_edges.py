@@ -1,0 +1,105 @@
+"""Edge cases of the batched likelihood on the GPU, against the oracle.
+
+Ragged batch sizes (not multiples of the points a block takes), the empty
+batch, points at the edges of the waveform's domain (equal masses, a template
+that lies wholly below the band, a long low-mass inspiral) and the
+invariances of logL under GW phase + 2 pi and psi + pi
+(QuickCBC.c:3705-3777: phi0 enters as params[4]/2, the antenna patterns have
+period pi in psi).  Tolerances as in the parity tests: FP64 logL and rho 1e-8
+absolute, mixed |dlogL| <= 0.01.
+"""
+import numpy as np
+import pytest
+
+from tests import _data
+from quickcbc_amd import qcbc as Q
+from quickcbc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[0, 1])
+def ctx_cfg(request):
+    an, inj, D, SN = _data.analysis(request.param)
+    c = Q.Context(an, D, SN)
+    yield request.param, an, inj, D, SN, c
+    c.close()
+
+
+def _masses(mt_msun, eta):
+    mt = mt_msun * W.MSUN_SI
+    return np.log(mt * eta ** 0.6), np.log(mt)
+
+
+def domain_points(inj):
+    """Points at the edges of the waveform's domain, all with status 0."""
+    rows = []
+    for mt, eta in ((3000.0, 0.2),      # fCut = 0.2/Msec ~ 13 Hz: no in-band bin, h = 0
+                    (2.0, 0.24),        # long inspiral, fbegin above the band start
+                    (60.0, 0.25),       # equal masses (dm = 0 branch)
+                    (400.0, 0.05)):     # heavy, unequal, merger in band
+        p = inj.copy()
+        p[0], p[1] = _masses(mt, eta)
+        rows.append(p)
+    for chi in (0.99, -0.99):
+        p = inj.copy()
+        p[2] = p[3] = chi
+        rows.append(p)
+    return np.vstack(rows)
+
+
+@pytest.mark.parametrize("B", [1, 5, 9, 63, 65, 257])
+def test_ragged_batches(ctx_cfg, B):
+    which, an, inj, D, SN, c = ctx_cfg
+    P = np.vstack([inj, _data.points(which, B - 1, seed=30 + B)])[:B]
+    Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
+    L, R, S = c.loglike(P, Q.FP64)
+    assert (S == Sr).all()
+    assert np.abs(L - Lr).max() <= 1e-8, np.abs(L - Lr).max()
+    assert np.abs(R - Rr).max() <= 1e-8
+    Lm, _, Sm = c.loglike(P, Q.MIXED)
+    assert (Sm == Sr).all()
+    assert np.abs(Lm - Lr).max() <= 0.01
+
+
+def test_empty_batch(ctx_cfg):
+    which, an, inj, D, SN, c = ctx_cfg
+    for prec in (Q.FP64, Q.MIXED):
+        L, R, S = c.loglike(np.zeros((0, 11)), prec)
+        assert L.shape == (0,) and S.shape == (0,)
+
+
+def test_domain_edges(ctx_cfg):
+    which, an, inj, D, SN, c = ctx_cfg
+    P = domain_points(inj)
+    Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
+    assert (Sr == 0).all()
+    L, R, S = c.loglike(P, Q.FP64)
+    assert (S == 0).all()
+    # a template with no in-band bin: logL = 0 and rho = 0/sqrt(0), as the reference computes it
+    assert Lr[0] == 0.0 and L[0] == 0.0
+    assert np.isnan(Rr[0]).all() and np.isnan(R[0]).all()
+    assert np.abs(L - Lr).max() <= 1e-8, (np.abs(L - Lr), list(Lr))
+    assert np.abs(R[1:] - Rr[1:]).max() <= 1e-8
+    Lm, _, _ = c.loglike(P, Q.MIXED)
+    assert np.abs(Lm - Lr).max() <= 0.01
+    Lp, _, Sp = c.loglike_plain(P)
+    assert (Sp == 0).all()
+    assert np.abs(Lp - Lr).max() <= 1e-8
+
+
+def test_phase_and_polarisation_invariance(ctx_cfg):
+    """logL(params[4] + 2 pi) = logL(psi + pi) = logL on the GPU path, noise-free and noisy."""
+    which, an, inj, D, SN, c = ctx_cfg
+    P = np.vstack([inj, _data.points(which, 15, seed=40)])
+    base, _, _ = c.loglike(P, Q.FP64)
+    base_m, _, _ = c.loglike(P, Q.MIXED)
+    for k, dv in ((4, 2 * np.pi), (9, np.pi)):
+        Q2 = P.copy()
+        Q2[:, k] += dv
+        L, _, S = c.loglike(Q2, Q.FP64)
+        assert (S == 0).all()
+        # each evaluation lies within 1e-8 of the oracle's, which is invariant to ~1e-10
+        assert np.abs(L - base).max() <= 2e-8, (k, np.abs(L - base).max())
+        Lm, _, _ = c.loglike(Q2, Q.MIXED)
+        assert np.abs(Lm - base_m).max() <= 0.02
