@@ -135,6 +135,11 @@ int qcbc_ctx_create(const qcbc_desc* desc, qcbc_ctx** out) {
   }
   c->desc = *desc;
   hipGetDevice(&c->device);
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->n_cu = prop.multiProcessorCount;
+  }
   size_t nd = desc->n_det;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_p64, sizeof(double) * (4 + 3 * nd) * c->nbp) != hipSuccess ||
@@ -215,12 +220,17 @@ int qcbc_bind_data(qcbc_ctx* c, const double* D, const double* SN) {
   return bind_impl(c, rd, rs);
 }
 
-// Launch shape: default splits give >= ~4096 blocks (16 per CU).
+// Launch shape: default splits give >= ~12 blocks per CU for B points (B
+// counts every point in flight at once: a PT ensemble's column groups run
+// concurrently, so their launches share that target).  Measured on the
+// 16 s / 3-detector PT step: 12 per CU beats 8 (tail) and 16 or more
+// (per-block set-up and reductions).
 int qcbc::effective_splits(const qcbc_ctx* c, int prec, int B) {
   int s = c->splits[prec];
   if (s <= 0) {
+    const int target = 12 * c->n_cu;
     int blocks = (B + c->cpb[prec] - 1) / c->cpb[prec];
-    s = (4096 + blocks - 1) / std::max(blocks, 1);
+    s = (target + blocks - 1) / std::max(blocks, 1);
   }
   int maxs = std::max(1, c->nb / 256);
   s = std::min(s, std::min(maxs, QCBC_MAX_SPLITS));
@@ -483,7 +493,7 @@ int qcbc_autotune(qcbc_ctx* c, int prec, int B) {
     for (int tile : tiles) {
       if (ll_lds_bytes(prec, c->desc.n_det, tile) > 64 * 1024) continue;
       int blocks = (B + cpb - 1) / cpb;
-      int base = std::max(1, (4096 + blocks - 1) / blocks);
+      int base = std::max(1, (12 * c->n_cu + blocks - 1) / blocks);
       int cands[] = {std::max(1, base / 2), base, base * 2};
       for (int sp : cands) {
         if (sp > QCBC_MAX_SPLITS) continue;

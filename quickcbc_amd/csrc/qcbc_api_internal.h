@@ -20,7 +20,7 @@ struct qcbc_ctx {
   qcbc_desc desc{};
   qcbc::AnalysisConst an{};
   int nhalf = 0, nb = 0, nbp = 0;
-  int device = 0;
+  int device = 0, n_cu = 256;
   hipStream_t stream = nullptr;
   double* d_p64 = nullptr;
   float* d_p32 = nullptr;

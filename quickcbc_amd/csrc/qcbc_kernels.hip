@@ -46,16 +46,34 @@ __device__ __forceinline__ void fma_acc_z(double& acc, double x, double y) {  //
 // file), and two complex squarings give the full angle, to a few ulp.  This
 // replaces the general sincos, whose large-argument reduction the ~1e4-turn
 // projected phase would otherwise pull in.
-__device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
+//
+// The first Horner step of each polynomial adds two constants; a VALU
+// operation reads at most one scalar register, so a bin loop keeps the second
+// coefficients in vector registers (SinCosK, loaded once per block) instead
+// of copying one of them every bin.
+struct SinCosK {
+  double p1, q1;
+};
+
+__device__ __forceinline__ double vreg_v(double v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
+__device__ __forceinline__ SinCosK sincos_consts() {
+  return SinCosK{vreg_v(-0x1.e30071afc3e59p-19), vreg_v(-0x1.a6c9c1be9eb49p-16)};
+}
+
+__device__ __forceinline__ void sincos_turns(double r, double* s, double* c, const SinCosK& k) {
   const double t = r - rint(r);
   const double z = t * t;
-  double p = z * sreg(0x1.e3f38399551bfp-25) + sreg(-0x1.e30071afc3e59p-19);
+  double p = fma(z, sreg(0x1.e3f38399551bfp-25), k.p1);
   p = fma(p, z, -sreg(-0x1.50782fda12d96p-13));
   p = fma(p, z, -sreg(0x1.32d2cce2e5b19p-8));
   p = fma(p, z, -sreg(-0x1.466bc677587f8p-4));
   p = fma(p, z, -sreg(0x1.4abbce625be41p-1));
   p = fma(p, z, -sreg(-0x1.921fb54442d18p+0));
-  double q = z * sreg(0x1.f3dbcea61b1a4p-22) + sreg(-0x1.a6c9c1be9eb49p-16);
+  double q = fma(z, sreg(0x1.f3dbcea61b1a4p-22), k.q1);
   q = fma(q, z, -sreg(-0x1.e1f4fb60281f6p-11));
   q = fma(q, z, -sreg(0x1.55d3c7dbfd139p-6));
   q = fma(q, z, -sreg(-0x1.03c1f081b0780p-2));
@@ -63,13 +81,17 @@ __device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
   q = fma(q, z, 1.0);
   double sn = t * p, cs = q;
 #pragma unroll
-  for (int k = 0; k < 2; k++) {  // (cs + i sn)^2, twice
+  for (int k2 = 0; k2 < 2; k2++) {  // (cs + i sn)^2, twice
     const double sc = sn * cs;
     cs = fma(cs, cs, -(sn * sn));
     sn = sc + sc;
   }
   *s = sn;
   *c = cs;
+}
+
+__device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
+  sincos_turns(r, s, c, SinCosK{-0x1.e30071afc3e59p-19, -0x1.a6c9c1be9eb49p-16});
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
@@ -128,6 +150,23 @@ __device__ __forceinline__ void vreg(double& v) { asm("" : "+v"(v)); }
 // two-address form would first copy c into the destination register).
 __device__ __forceinline__ double hstep(double np, double x, double c) { return fma(np, x, -c); }
 
+// Point constants a segment's bin loop keeps in vector registers: the first
+// Horner step of the inspiral amplitude, the intermediate amplitude and the
+// inspiral phase each add two point constants, one of which would otherwise
+// be copied into a vector register every bin.
+template <int AR, int PR>
+struct VConst {
+  double a0 = 0.0, p0 = 0.0, pL = 0.0;
+  __device__ __forceinline__ explicit VConst(const Point& c) {
+    if (AR == 0) a0 = vreg_v(c.ia83);
+    if (AR == 1) a0 = vreg_v(c.ib3);
+    if (PR == 0) {
+      p0 = vreg_v(c.ins_q53);
+      pL = vreg_v(c.ins_qL);
+    }
+  }
+};
+
 // Amplitude without the detector scale, and the projected detector-0
 // argument in turns, of one bin in amplitude region AR and phase region PR
 // (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
@@ -135,8 +174,8 @@ __device__ __forceinline__ double hstep(double np, double x, double c) { return 
 // inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
 // they need no other power of Mf.
 template <int AR, int PR>
-__device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, double is6f, double logf,
-                                          double* a, double* r) {
+__device__ __forceinline__ void amp_turns(const Point& c, const VConst<AR, PR>& k, double f, double s6f,
+                                          double is6f, double logf, double* a, double* r) {
   const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
   const double w = i_s6 * i_s6;            // Mf^(-1/3)
   const double w3 = w * w * w;             // 1 / Mf
@@ -153,7 +192,7 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
   if (AR == 0) {
     // amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ai_43 v^4 + ... + ai_3 v^9), Horner
     // on the negated partial sums
-    double np = -(v * c.ia3) - c.ia83;
+    double np = fma(v, -c.ia3, -k.a0);
     np = hstep(np, v, c.ia73);
     np = hstep(np, v, c.ia2);
     np = hstep(np, v, c.ia53);
@@ -164,7 +203,7 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
   } else if (AR == 1) {
     // Newton form on the nodes (f1, f1, f2, f3), Horner on negated partials
     const double x1 = Mf - c.col_f1;
-    double np = -((Mf - c.col_f3) * c.ib4) - c.ib3;
+    double np = fma(Mf - c.col_f3, -c.ib4, -k.a0);
     np = hstep(np, Mf - c.col_f2, c.ib2);
     np = hstep(np, x1, c.ib1);
     A = -hstep(np, x1, c.ib0);
@@ -179,7 +218,7 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
   if (PR == 0) {
     // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
     // + (qL + q13L v) log f
-    double np = -(v * c.ins_q2) - c.ins_q53;
+    double np = fma(v, -c.ins_q2, -k.p0);
     np = hstep(np, v, c.ins_q43);
     np = np * v;  // no v^3 term: q1 Mf is added below
     np = hstep(np, v, c.ins_q23);
@@ -188,7 +227,7 @@ __device__ __forceinline__ void amp_turns(const Point& c, double f, double s6f, 
     nn = hstep(nn, w, c.ins_qm1);
     nn = hstep(nn, w, c.ins_qm23);
     nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
-    const double lg = (v * c.ins_q13L + c.ins_qL) * logf;
+    const double lg = fma(v, c.ins_q13L, k.pL) * logf;
     g = -hstep(nn, w, c.ins_q0) + fma(-np, v, fma(c.ins_q1, c.Msec * f, lg));
   } else if (PR == 1) {
     const double w9 = w3 * w3 * w3;  // 1 / Mf^3
@@ -248,18 +287,19 @@ struct F64State {
   double hh[ND];     // sum a^2 / S_d
   double dh0;        // sum a Re(D_0 e^{-i x0}) / S_0
   double hr[ND], hi[ND];  // Horner sums of detectors d > 0
+  SinCosK sk;
 };
 
 template <int ND, int TILE>
 struct F64Seg {
   template <int AR, int PR>
   struct S {
-    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const Point& c) {
+    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const Point& c, const VConst<AR, PR>& k) {
       const double* L = st.lb + st.j;
       double a, r;
-      amp_turns<AR, PR>(c, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
+      amp_turns<AR, PR>(c, k, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
       double s0, c0;
-      sincos_turns(r, &s0, &c0);
+      sincos_turns(r, &s0, &c0, st.sk);
       // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the sums carry
       // a^2 / S and a D e^{-i x_d} / S, and k_finalize applies the per-point
       // detector scale once.
@@ -282,9 +322,10 @@ struct F64Seg {
     }
     static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
       const Point& c = *st.c;
+      const VConst<AR, PR> k(c);
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
-        if (st.j < end) bin(st, c);
+        if (st.j < end) bin(st, c, k);
       }
     }
   };
@@ -304,6 +345,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   F64State<ND, TILE> st;
   st.c = in.pts + uniform(active ? b : 0);
   st.j = ks + lane;
+  st.sk = sincos_consts();
   st.dh0 = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
@@ -373,6 +415,7 @@ struct MixSeg {
   struct S {
     static __device__ __forceinline__ void run(MixState<ND, TILE>& st, int pos, int end) {
       const Point& c = *st.c;
+      const VConst<AR, PR> k(c);
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
         if (st.j < end) {
@@ -381,7 +424,7 @@ struct MixSeg {
           const float* M = st.lds32 + i;
           const double f = L[0];
           double ad, r;
-          amp_turns<AR, PR>(c, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
+          amp_turns<AR, PR>(c, k, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
           const float a = (float)ad;
           const float a2 = a * a;
 #pragma unroll

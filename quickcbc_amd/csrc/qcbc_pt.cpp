@@ -75,7 +75,7 @@ static int group_step(qcbc_pt* pt, int k) {
   const int prec = pt->precision;
   const size_t o = gr.off;
   HIPCHK(launch_setup(pt->d_y + 11 * o, gr.P, c->an, gr.d_pts, pt->d_st + o, gr.s));
-  const LLInst in = make_inst(c, c->tile[prec], gr.P, effective_splits(c, prec, gr.P), gr.d_pts, gr.d_partial);
+  const LLInst in = make_inst(c, c->tile[prec], gr.P, effective_splits(c, prec, pt->P), gr.d_pts, gr.d_partial);
   int st = ll_launch_timed(c, prec, in, gr.s);
   if (st) return st;
   HIPCHK(launch_pt_post(in, prec == QCBC_MIXED ? FIN_MIXED : FIN_FP64, c->d_wref, pt->d_x + 11 * o, pt->d_L + o,
