@@ -140,6 +140,19 @@ __device__ __forceinline__ void block_range(const LLInst& in, int b, bool active
   *hi = s_lohi[1];
 }
 
+// The band planes from bin t0 on as a raw buffer (np planes of nbp doubles):
+// a tile's loads then take the plane offset in a scalar register and the
+// lane's byte offset in a vector register, with no per-load address
+// arithmetic, and a read past the planes returns 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const LLInst& in, int t0, int np) {
+  const int bytes = (np * in.nbp - t0) * 8;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(in.p64 + t0), (short)0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rs, unsigned voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+}
+
 // Makes v opaque to the optimiser in place (empty asm): later subtractions of
 // scalar constants from it stay single subtractions instead of FMAs that need
 // the constant copied into a vector register.
@@ -167,14 +180,70 @@ struct VConst {
   }
 };
 
+// A wave-uniform double pinned in scalar registers (readfirstlane first, so
+// that a value the compiler placed in vector registers can still go there).
+__device__ __forceinline__ double spin(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u), hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return sreg(__longlong_as_double((long long)(((unsigned long long)hi << 32) | lo)));
+}
+
+// The point constants of one region pair, read once per segment into scalar
+// registers (the empty asm of sreg keeps each one there for the whole bin
+// loop instead of re-loading it from the point record every bin: the records
+// of the ~24 points resident on a CU would not stay in the scalar cache).
+// Field names follow Point, so amp_turns reads either.
+template <int AR, int PR>
+struct SConst {
+  double Msec, s6_scale, is6_scale, tshift[1];
+  double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia3;
+  double col_f1, col_f2, col_f3, ib0, ib1, ib2, ib4;
+  double fRD, mrd_rate, mnum, mg2;
+  double ins_q0, ins_q1, ins_q13, ins_q13L, ins_q23, ins_q43, ins_q2, ins_qm13, ins_qm23, ins_qm1, ins_qm53;
+  double int_q0, int_q1, int_qL, int_qm3;
+  double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
+  double wc[3], ws[3];
+  template <int ND>
+  __device__ __forceinline__ void load(const Point& c) {
+    Msec = spin(c.Msec);
+    tshift[0] = spin(c.tshift[0]);
+    if (AR == 0 || PR == 0) s6_scale = spin(c.s6_scale);
+    is6_scale = spin(c.is6_scale);
+    if (AR == 0) {
+      ia0 = spin(c.ia0); ia23 = spin(c.ia23); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
+      ia53 = spin(c.ia53); ia2 = spin(c.ia2); ia73 = spin(c.ia73); ia3 = spin(c.ia3);
+    } else if (AR == 1) {
+      col_f1 = spin(c.col_f1); col_f2 = spin(c.col_f2); col_f3 = spin(c.col_f3);
+      ib0 = spin(c.ib0); ib1 = spin(c.ib1); ib2 = spin(c.ib2); ib4 = spin(c.ib4);
+    } else {
+      fRD = spin(c.fRD); mrd_rate = spin(c.mrd_rate); mnum = spin(c.mnum); mg2 = spin(c.mg2);
+    }
+    if (PR == 0) {
+      ins_q0 = spin(c.ins_q0); ins_q1 = spin(c.ins_q1); ins_q13 = spin(c.ins_q13); ins_q13L = spin(c.ins_q13L);
+      ins_q23 = spin(c.ins_q23); ins_q43 = spin(c.ins_q43); ins_q2 = spin(c.ins_q2); ins_qm13 = spin(c.ins_qm13);
+      ins_qm23 = spin(c.ins_qm23); ins_qm1 = spin(c.ins_qm1); ins_qm53 = spin(c.ins_qm53);
+    } else if (PR == 1) {
+      int_q0 = spin(c.int_q0); int_q1 = spin(c.int_q1); int_qL = spin(c.int_qL); int_qm3 = spin(c.int_qm3);
+    } else {
+      mrd_q0 = spin(c.mrd_q0); mrd_q1 = spin(c.mrd_q1); mrd_qm1 = spin(c.mrd_qm1); mrd_q34 = spin(c.mrd_q34);
+      mrd_qat = spin(c.mrd_qat); mrd_a5 = spin(c.mrd_a5); mrd_ifDM = spin(c.mrd_ifDM);
+    }
+#pragma unroll
+    for (int d = 1; d < ND; d++) {
+      wc[d] = spin(c.wc[d]);
+      ws[d] = spin(c.ws[d]);
+    }
+  }
+};
+
 // Amplitude without the detector scale, and the projected detector-0
 // argument in turns, of one bin in amplitude region AR and phase region PR
 // (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
 // constants of the point and the bin's f, f^(1/6), f^(-1/6), log f.  The
 // inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
 // they need no other power of Mf.
-template <int AR, int PR>
-__device__ __forceinline__ void amp_turns(const Point& c, const VConst<AR, PR>& k, double f, double s6f,
+template <int AR, int PR, class K>
+__device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double s6f,
                                           double is6f, double logf, double* a, double* r) {
   const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
   const double w = i_s6 * i_s6;            // Mf^(-1/3)
@@ -294,7 +363,8 @@ template <int ND, int TILE>
 struct F64Seg {
   template <int AR, int PR>
   struct S {
-    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const Point& c, const VConst<AR, PR>& k) {
+    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const SConst<AR, PR>& c,
+                                               const VConst<AR, PR>& k) {
       const double* L = st.lb + st.j;
       double a, r;
       amp_turns<AR, PR>(c, k, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
@@ -321,8 +391,9 @@ struct F64Seg {
       st.j += 64;
     }
     static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
-      const Point& c = *st.c;
-      const VConst<AR, PR> k(c);
+      const VConst<AR, PR> k(*st.c);
+      SConst<AR, PR> c;
+      c.template load<ND>(*st.c);
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
         if (st.j < end) bin(st, c, k);
@@ -360,9 +431,11 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   for (int t0 = blo; t0 < bhi; t0 += TILE) {
     const int tn = min(TILE, bhi - t0);
     __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+      const unsigned off = 8u * i;
 #pragma unroll
-      for (int p = 0; p < NP; p++) lds[p * TILE + i] = in.p64[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < NP; p++) lds[p * TILE + i] = buffer_load_f64(rs, off, p * in.nbp * 8);
     }
     __syncthreads();
     if (active) {
@@ -531,7 +604,7 @@ __global__ __launch_bounds__(512) void k_ll_queue(const LLInst* __restrict__ ins
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = s_item;
+    const int it = uniform(s_item);
     if (it >= n_items) break;
     const LLItem w = items[it];
     const LLInst in = insts[w.inst];
