@@ -12,33 +12,24 @@ struct AnalysisConst {
   int imin, imax, N;
 };
 
-// Frequency-independent state of one parameter point (~90 doubles).
+// Frequency-independent state of one parameter point (~130 doubles).  The
+// fields the fused likelihood kernels read in their bin loops come first,
+// grouped by region so that a segment's constants sit on few 64-byte lines
+// (the scalar cache holds the records of every point resident on a CU).
 struct alignas(16) Point {
   int status, kstart, kend, region_flags;
-  double Msec, fs, fCut, amp0;
-  double s6_scale, is6_scale, logMsec, inv_eta;
-  // amplitude: amp_pre = sqrt(2/3) sqrt(eta) / pi^(1/6) (amp0Func)
-  double amp_pre, ai_23, ai_1, ai_43, ai_53, ai_2, ai_73, ai_83, ai_3;
-  double fmaxCalc;
-  // intermediate amplitude in Newton form on nodes (f1, f1, f2, f3)
-  double col_f1, col_f2, col_f3, col_c0, col_c1, col_c2, col_c3, col_c4;
-  double fRD, fDM, fDMg3, mrd_num, mrd_rate;  // mrd_num = fDM g3 g1, mrd_rate = g2/(fDM g3)
-  // phase
-  double pi_init, pi_23, pi_13, pi_13L, pi_L, pi_m13, pi_m23, pi_m1, pi_m53;
-  double pi_1, pi_43, pi_53, pi_2;
-  double fPhiMRD;
-  double beta1, beta2, beta3, C1Int, C2Int;
-  double alpha1, alpha2, alpha3, alpha4, a5fRD, C1MRD, C2MRD;
-  double t0, MfRef, phi_precalc;
-  // projection per detector slot
-  double Fs[3], tshift[3], lambda[3], Ascale[3];
-  double eta, finspin;
-  // ---- fused likelihood kernels only: the same waveform with its constants
+  // ---- fused likelihood kernels: the same waveform with its constants
   // folded.  First bins (absolute index k) at which the amplitude leaves the
   // inspiral (Mf >= 0.014) and enters the merger-ringdown (Mf >= fmaxCalc),
   // and the phase leaves the inspiral (Mf >= 0.018) and enters the
   // merger-ringdown (Mf >= fPhiMRD); clipped to [kstart, kend].
   int seg_a1, seg_a2, seg_p1, seg_p2;
+  double Msec, s6_scale, is6_scale;
+  // projection: detector time shifts (s), and for detector d the rotation of
+  // its phase relative to detector 0 over 64 bins, exp(i 2 pi 64 dt_d / Tobs)
+  // = wc_d + i ws_d
+  double tshift[3];
+  double wc[3], ws[3];
   // Projected detector-0 argument in turns, f tshift_0 + (lambda_0 - Psi)/2 pi,
   // is f tshift_0 plus a function of Mf per phase region:
   //   inspiral:      q0 + q1 Mf + Mf^(1/3) (q13 + q13L log f) + qL log f + q23 Mf^(2/3)
@@ -48,19 +39,37 @@ struct alignas(16) Point {
   //   merger:        q0 + q1 Mf + qm1 / Mf + q34 Mf^(3/4) + qat atan((Mf - a5) ifDM)
   double ins_q0, ins_q1, ins_q13, ins_q13L, ins_qL, ins_q23, ins_q43, ins_q53, ins_q2;
   double ins_qm13, ins_qm23, ins_qm1, ins_qm53;
-  double int_q0, int_q1, int_qL, int_qm3;
-  double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
   // amplitude without Ascale: Mf^(-7/6) times
   //   inspiral:      amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3)   (coefficients ia*)
   //   intermediate:  amp_pre x the Newton form (coefficients ib*, nodes col_f*)
   //   merger:        mnum exp(-(Mf - fRD) mrd_rate) / ((Mf - fRD)^2 + fDMg3^2)
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia83, ia3;
+  double int_q0, int_q1, int_qL, int_qm3;
+  // intermediate amplitude in Newton form on nodes (f1, f1, f2, f3)
+  double col_f1, col_f2, col_f3;
   double ib0, ib1, ib2, ib3, ib4;
-  double mnum, mg2;
-  // detector d relative to detector 0: f dt_d + dl_d turns, and the rotation
-  // of that phase over 64 bins, exp(i 2 pi 64 dt_d / Tobs) = wc_d + i ws_d
+  double fRD, mrd_rate, mnum, mg2;  // mrd_rate = g2/(fDM g3)
+  double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
+  // detector d relative to detector 0: f dt_d + dl_d turns
   double dt[3], dl[3];
-  double wc[3], ws[3];
+  // ---- the reference's set-up (IMRPhenomD coefficients, projection)
+  double fs, fCut, amp0;
+  double logMsec, inv_eta;
+  // amplitude: amp_pre = sqrt(2/3) sqrt(eta) / pi^(1/6) (amp0Func)
+  double amp_pre, ai_23, ai_1, ai_43, ai_53, ai_2, ai_73, ai_83, ai_3;
+  double fmaxCalc;
+  double col_c0, col_c1, col_c2, col_c3, col_c4;
+  double fDM, fDMg3, mrd_num;  // mrd_num = fDM g3 g1
+  // phase
+  double pi_init, pi_23, pi_13, pi_13L, pi_L, pi_m13, pi_m23, pi_m1, pi_m53;
+  double pi_1, pi_43, pi_53, pi_2;
+  double fPhiMRD;
+  double beta1, beta2, beta3, C1Int, C2Int;
+  double alpha1, alpha2, alpha3, alpha4, a5fRD, C1MRD, C2MRD;
+  double t0, MfRef, phi_precalc;
+  // projection per detector slot
+  double Fs[3], lambda[3], Ascale[3];
+  double eta, finspin;
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are

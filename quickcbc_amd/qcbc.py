@@ -581,21 +581,23 @@ class Server:
 
 class PointRecord(ctypes.Structure):
     """Mirror of qcbc::Point (quickcbc_amd/csrc/qcbc_types.h)."""
-    _fields_ = ([(n, ctypes.c_int) for n in ("status", "kstart", "kend", "region_flags")] +
-                [(n, ctypes.c_double) for n in (
-                    "Msec fs fCut amp0 s6_scale is6_scale logMsec inv_eta amp_pre ai_23 ai_1 ai_43 ai_53 ai_2 "
-                    "ai_73 ai_83 ai_3 fmaxCalc col_f1 col_f2 col_f3 col_c0 col_c1 col_c2 col_c3 col_c4 fRD fDM "
-                    "fDMg3 mrd_num mrd_rate pi_init pi_23 pi_13 pi_13L pi_L pi_m13 pi_m23 pi_m1 pi_m53 pi_1 pi_43 "
-                    "pi_53 pi_2 fPhiMRD beta1 beta2 beta3 C1Int C2Int alpha1 alpha2 alpha3 alpha4 a5fRD C1MRD "
-                    "C2MRD t0 MfRef phi_precalc").split()] +
-                [(n, ctypes.c_double * 3) for n in ("Fs", "tshift", "lambda_", "Ascale")] +
-                [("eta", ctypes.c_double), ("finspin", ctypes.c_double)] +
-                [(n, ctypes.c_int) for n in ("seg_a1", "seg_a2", "seg_p1", "seg_p2")] +
+    _fields_ = ([(n, ctypes.c_int) for n in ("status", "kstart", "kend", "region_flags",
+                                             "seg_a1", "seg_a2", "seg_p1", "seg_p2")] +
+                [(n, ctypes.c_double) for n in ("Msec", "s6_scale", "is6_scale")] +
+                [(n, ctypes.c_double * 3) for n in ("tshift", "wc", "ws")] +
                 [(n, ctypes.c_double) for n in (
                     "ins_q0 ins_q1 ins_q13 ins_q13L ins_qL ins_q23 ins_q43 ins_q53 ins_q2 ins_qm13 ins_qm23 "
-                    "ins_qm1 ins_qm53 int_q0 int_q1 int_qL int_qm3 mrd_q0 mrd_q1 mrd_qm1 mrd_q34 mrd_qat mrd_a5 "
-                    "mrd_ifDM ia0 ia23 ia1 ia43 ia53 ia2 ia73 ia83 ia3 ib0 ib1 ib2 ib3 ib4 mnum mg2").split()] +
-                [(n, ctypes.c_double * 3) for n in ("dt", "dl", "wc", "ws")])
+                    "ins_qm1 ins_qm53 ia0 ia23 ia1 ia43 ia53 ia2 ia73 ia83 ia3 int_q0 int_q1 int_qL int_qm3 "
+                    "col_f1 col_f2 col_f3 ib0 ib1 ib2 ib3 ib4 fRD mrd_rate mnum mg2 "
+                    "mrd_q0 mrd_q1 mrd_qm1 mrd_q34 mrd_qat mrd_a5 mrd_ifDM").split()] +
+                [(n, ctypes.c_double * 3) for n in ("dt", "dl")] +
+                [(n, ctypes.c_double) for n in (
+                    "fs fCut amp0 logMsec inv_eta amp_pre ai_23 ai_1 ai_43 ai_53 ai_2 ai_73 ai_83 ai_3 fmaxCalc "
+                    "col_c0 col_c1 col_c2 col_c3 col_c4 fDM fDMg3 mrd_num pi_init pi_23 pi_13 pi_13L pi_L "
+                    "pi_m13 pi_m23 pi_m1 pi_m53 pi_1 pi_43 pi_53 pi_2 fPhiMRD beta1 beta2 beta3 C1Int C2Int "
+                    "alpha1 alpha2 alpha3 alpha4 a5fRD C1MRD C2MRD t0 MfRef phi_precalc").split()] +
+                [(n, ctypes.c_double * 3) for n in ("Fs", "lambda_", "Ascale")] +
+                [("eta", ctypes.c_double), ("finspin", ctypes.c_double)])
 
 
 def point_records(ctx, params):

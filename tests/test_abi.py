@@ -73,3 +73,24 @@ def test_oracle_is_not_linked_into_the_product():
     assert "qo_" not in out
     deps = subprocess.run(["ldd", LIB], capture_output=True, text=True).stdout
     assert "oracle" not in deps
+
+
+def test_point_record_mirror_matches_the_struct(tmp_path):
+    """Every field of the Python mirror of qcbc::Point sits at the offset g++
+    gives it, and the sizes agree."""
+    from quickcbc_amd.qcbc import PointRecord
+    names = [f[0] for f in PointRecord._fields_]
+    src = ['#include <cstdio>', '#include <cstddef>', '#include "qcbc_types.h"', "int main() {"]
+    for n in names:
+        src.append('  std::printf("%%s %%zu\\n", "%s", offsetof(qcbc::Point, %s));' % (n, "lambda" if n == "lambda_" else n))
+    src.append('  std::printf("sizeof %zu\\n", sizeof(qcbc::Point));')
+    src.append("}")
+    cpp, exe = tmp_path / "off.cpp", tmp_path / "off"
+    cpp.write_text("\n".join(src) + "\n")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "quickcbc_amd", "csrc"), "-o", str(exe), str(cpp)],
+                   check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+               .stdout.splitlines())
+    assert int(got.pop("sizeof")) == ctypes.sizeof(PointRecord)
+    for n in names:
+        assert getattr(PointRecord, n).offset == int(got[n]), n
