@@ -224,18 +224,27 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
 
 
 # FP64 operations of one (point, bin) in the fused FP64 kernel, counted from
-# its source (an FMA counts 2; exp, atan, sqrt and a division count 1 each):
-# the waveform part by (amplitude region, phase region), 0 inspiral,
-# 1 intermediate, 2 merger-ringdown; the quarter-angle sincos of the
-# projected argument; and per detector the inner-product terms, plus the
-# rotation and its recurrence for every detector after the first.
-FUSED_WAVEFORM_FLOPS = {(0, 0): 54, (0, 1): 36, (0, 2): 42, (1, 0): 47, (1, 1): 27, (1, 2): 33,
-                        (2, 0): 44, (2, 1): 24, (2, 2): 30}
-FUSED_SINCOS_FLOPS = 37
+# its source (qcbc_kernels.hip: amp_turns, sincos_turns, F64Seg; an FMA
+# counts 2, exp, atan, sqrt and a division 1 each).  The waveform part by
+# (amplitude region, phase region), 0 inspiral, 1 intermediate, 2
+# merger-ringdown, is the shared powers and products (Mf^(-1/6), Mf^(-1/3),
+# 1/Mf, the Mf^(-7/6) scale and the f tshift_0 term: 8), Mf = Msec f (1, in
+# the intermediate and merger regions), Mf^(1/3) (2, inspiral), and each
+# region's amplitude and phase:
+_FUSED_AMP = (17, 11, 7)     # Horner in Mf^(1/3); Newton form; exp / Lorentzian
+_FUSED_PHASE = (27, 8, 14)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan
+FUSED_WAVEFORM_FLOPS = {(a, p): 8 + (1 if a or p else 0) + (2 if a == 0 or p == 0 else 0)
+                        + _FUSED_AMP[a] + _FUSED_PHASE[p] for a in range(3) for p in range(3)}
+# reduction to t in [-1/2, 1/2] (2), t^2 (1), the two degree-6 polynomials
+# (12 + 12), t P (1), two complex squarings (5 each)
+FUSED_SINCOS_FLOPS = 38
 
 
 def fused_detector_flops(nd):
-    return 1 + 7 + 19 * (nd - 1)
+    """a^2, a cos x0, a sin x0 (3); per detector the a^2 / S sum (2); detector
+    0's Re(D e^{-i x0}) sum (4); every further detector's Horner step
+    H e^{i phi} + a D e^{-i x0} (14)."""
+    return 3 + 2 * nd + 4 + 14 * (nd - 1)
 
 
 def fused_work(ctx, an, params):

@@ -153,6 +153,16 @@ __device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rs, uns
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
 }
 
+// The mixed kernel's FP32 planes (np planes of nbp floats) from bin t0 on.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc32(const LLInst& in, int t0, int np) {
+  const int bytes = (np * in.nbp - t0) * 4;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in.p32 + t0), (short)0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t rs, unsigned voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
 // Makes v opaque to the optimiser in place (empty asm): later subtractions of
 // scalar constants from it stay single subtractions instead of FMAs that need
 // the constant copied into a vector register.
@@ -202,8 +212,10 @@ struct SConst {
   double ins_q0, ins_q1, ins_q13, ins_q13L, ins_q23, ins_q43, ins_q2, ins_qm13, ins_qm23, ins_qm1, ins_qm53;
   double int_q0, int_q1, int_qL, int_qm3;
   double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
-  double wc[3], ws[3];
-  template <int ND>
+  double wc[3], ws[3], dt[3], dl[3];
+  // ND detectors; the Horner rotation (wc, ws) for the FP64 kernel, the
+  // detector phase offsets (dt, dl) for the mixed one
+  template <int ND, bool ROT = true>
   __device__ __forceinline__ void load(const Point& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
@@ -230,8 +242,13 @@ struct SConst {
     }
 #pragma unroll
     for (int d = 1; d < ND; d++) {
-      wc[d] = spin(c.wc[d]);
-      ws[d] = spin(c.ws[d]);
+      if (ROT) {
+        wc[d] = spin(c.wc[d]);
+        ws[d] = spin(c.ws[d]);
+      } else {
+        dt[d] = spin(c.dt[d]);
+        dl[d] = spin(c.dl[d]);
+      }
     }
   }
 };
@@ -476,9 +493,9 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
 template <int ND, int TILE>
 struct MixState {
   const Point* c;
-  const double* lds;
-  const float* lds32;
-  int t0, j;
+  const double* lb;  // LDS tile bases minus the tile's first band bin
+  const float* lb32;
+  int j;
   float sum[2 * ND], comp[2 * ND];
 };
 
@@ -487,14 +504,14 @@ struct MixSeg {
   template <int AR, int PR>
   struct S {
     static __device__ __forceinline__ void run(MixState<ND, TILE>& st, int pos, int end) {
-      const Point& c = *st.c;
-      const VConst<AR, PR> k(c);
+      const VConst<AR, PR> k(*st.c);
+      SConst<AR, PR> c;
+      c.template load<ND, false>(*st.c);
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
         if (st.j < end) {
-          const int i = st.j - st.t0;
-          const double* L = st.lds + i;
-          const float* M = st.lds32 + i;
+          const double* L = st.lb + st.j;
+          const float* M = st.lb32 + st.j;
           const double f = L[0];
           double ad, r;
           amp_turns<AR, PR>(c, k, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
@@ -529,8 +546,6 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
   MixState<ND, TILE> st;
   st.c = in.pts + uniform(active ? b : 0);
-  st.lds = lds;
-  st.lds32 = lds32;
   st.j = ks + lane;
 #pragma unroll
   for (int q = 0; q < 2 * ND; q++) st.sum[q] = st.comp[q] = 0.0f;
@@ -545,15 +560,18 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   for (int t0 = blo; t0 < bhi; t0 += TILE) {
     const int tn = min(TILE, bhi - t0);
     __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, 4);
+    const __amdgpu_buffer_rsrc_t rs32 = plane_rsrc32(in, t0, 3 * ND);
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
 #pragma unroll
-      for (int p = 0; p < 4; p++) lds[p * TILE + i] = in.p64[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < 4; p++) lds[p * TILE + i] = buffer_load_f64(rs, 8u * i, p * in.nbp * 8);
 #pragma unroll
-      for (int p = 0; p < 3 * ND; p++) lds32[p * TILE + i] = in.p32[(size_t)p * in.nbp + t0 + i];
+      for (int p = 0; p < 3 * ND; p++) lds32[p * TILE + i] = buffer_load_f32(rs32, 4u * i, p * in.nbp * 4);
     }
     __syncthreads();
     if (active) {
-      st.t0 = t0;
+      st.lb = lds - t0;
+      st.lb32 = lds32 - t0;
       pos = run_segments<MixSeg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }
