@@ -173,6 +173,70 @@ __device__ __forceinline__ void vreg(double& v) { asm("" : "+v"(v)); }
 // two-address form would first copy c into the destination register).
 __device__ __forceinline__ double hstep(double np, double x, double c) { return fma(np, x, -c); }
 
+// e^x for the merger-ringdown amplitude's exponent (|x| below ~50 there, so
+// no overflow or underflow handling): Cody and Waite's reduction x = k ln2 + r
+// with k = rint(x / ln2) and ln2 split into a 32-bit head (k ln2_hi exact)
+// and a tail, then e^r by its Taylor series through r^13 (|r| <= ln2/2, so
+// the truncation is below 5e-18 relative) in Horner form, and 2^k by ldexp.
+__device__ __forceinline__ double exp_mrd(double x) {
+  const double k = rint(x * sreg(0x1.71547652b82fep+0));  // 1 / ln2
+  double r = fma(k, -sreg(0x1.62e42ff000000p-1), x);      // ln2 head
+  r = fma(k, sreg(0x1.718432a1b0e26p-35), r);             // minus the (negative) tail
+  double p = r * sreg(0x1.6124613a86d09p-33) + sreg(0x1.1eed8eff8d898p-29);  // 1/13!, 1/12!
+  p = fma(p, r, -sreg(-0x1.ae64567f544e4p-26));  // 1/11!
+  p = fma(p, r, -sreg(-0x1.27e4fb7789f5cp-22));  // 1/10!
+  p = fma(p, r, -sreg(-0x1.71de3a556c734p-19));  // 1/9!
+  p = fma(p, r, -sreg(-0x1.a01a01a01a01ap-16));  // 1/8!
+  p = fma(p, r, -sreg(-0x1.a01a01a01a01ap-13));  // 1/7!
+  p = fma(p, r, -sreg(-0x1.6c16c16c16c17p-10));  // 1/6!
+  p = fma(p, r, -sreg(-0x1.1111111111111p-7));   // 1/5!
+  p = fma(p, r, -sreg(-0x1.5555555555555p-5));   // 1/4!
+  p = fma(p, r, -sreg(-0x1.5555555555555p-3));   // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+// 1/d for a positive normal d: the hardware reciprocal estimate and two
+// Newton steps (each squares the relative error), to about an ulp.
+__device__ __forceinline__ double rcp_pos(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
+// atan(x) for the merger-ringdown phase: |x| is reduced to [0, 1] by
+// atan a = pi/2 - atan(1/a), then to [0, tan(pi/12)] by atan t = pi/6 +
+// atan((sqrt3 t - 1)/(t + sqrt3)), where atan t = t R(t^2) with R of degree 9
+// (Chebyshev fit in t^2 at 40 digits, generated by gen_atan.py beside this
+// file; about 2 ulp overall).  Both reductions are selects, so the lanes of a
+// wave take one path.
+__device__ __forceinline__ double atan_mrd(double x) {
+  const double a = fabs(x);
+  const bool inv = a > 1.0;
+  const double t1 = inv ? rcp_pos(a) : a;
+  const double s3 = sreg(0x1.bb67ae8584caap+0);  // sqrt 3
+  const bool big = t1 > sreg(0x1.126145e9ecd56p-2);  // tan(pi/12)
+  const double t2 = big ? fma(t1, s3, -1.0) * rcp_pos(t1 + s3) : t1;
+  const double z = t2 * t2;
+  double p = z * sreg(-0x1.3a23485e6c654p-5) + sreg(0x1.d11bfd68cc957p-5);
+  p = fma(p, z, -sreg(0x1.106138ecc7d04p-4));
+  p = fma(p, z, -sreg(-0x1.3b0ae034ec65bp-4));
+  p = fma(p, z, -sreg(0x1.745cd0c6674e0p-4));
+  p = fma(p, z, -sreg(-0x1.c71c706e9f431p-4));
+  p = fma(p, z, -sreg(0x1.24924922ada3cp-3));
+  p = fma(p, z, -sreg(-0x1.9999999996e3ep-3));
+  p = fma(p, z, -sreg(0x1.5555555555549p-2));
+  p = fma(p, z, 1.0);
+  double r = t2 * p;
+  if (big) r += sreg(0x1.0c152382d7366p-1);        // pi/6
+  if (inv) r = sreg(0x1.921fb54442d18p+0) - r;     // pi/2
+  return copysign(r, x);
+}
+
 // Point constants a segment's bin loop keeps in vector registers: the first
 // Horner step of the inspiral amplitude, the intermediate amplitude and the
 // inspiral phase each add two point constants, one of which would otherwise
@@ -219,7 +283,7 @@ struct SConst {
   __device__ __forceinline__ void load(const Point& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
-    if (AR == 0 || PR == 0) s6_scale = spin(c.s6_scale);
+    s6_scale = spin(c.s6_scale);
     is6_scale = spin(c.is6_scale);
     if (AR == 0) {
       ia0 = spin(c.ia0); ia23 = spin(c.ia23); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
@@ -295,7 +359,7 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     A = -hstep(np, x1, c.ib0);
   } else {
     const double df = Mf - c.fRD;
-    A = c.mnum * exp(-df * c.mrd_rate) / fma(df, df, c.mg2);
+    A = (c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2));
   }
   *a = (w3 * i_s6) * A;  // Mf^(-7/6)
   // the projected argument: f tshift_0, exact product with one rounding, plus
@@ -319,9 +383,10 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     const double w9 = w3 * w3 * w3;  // 1 / Mf^3
     g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
   } else {
-    const double sq = sqrt(Mf);
+    const double u = c.s6_scale * s6f;  // Mf^(1/6)
+    const double sq = (u * u) * u;      // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
-    g = fma(c.mrd_qat, atan((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
+    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
         c.mrd_q0;
   }
   *r = fma(f, c.tshift[0], g);
@@ -612,8 +677,12 @@ __global__ __launch_bounds__(1024) void k_ll(LLInst in) {
 // Many analyses in one persistent launch: every block takes (analysis, point
 // group, split) items from a queue until it is empty.  The exit test reads a
 // block-uniform item index, so every wave of every block reaches it.
+// The queue kernel holds its arguments and loop state in scalar registers
+// beside the segment constants, and the compiler places the vector register
+// that takes the scalar spills above the others: the occupancy hint keeps it
+// within the 80 VGPRs of 6 waves per SIMD, the count the LDS tile allows.
 template <int PREC, int TILE>
-__global__ __launch_bounds__(512) void k_ll_queue(const LLInst* __restrict__ insts, const LLItem* __restrict__ items,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_ll_queue(const LLInst* __restrict__ insts, const LLItem* __restrict__ items,
                                                   int n_items, int* __restrict__ counter) {
   extern __shared__ double lds[];
   __shared__ int s_lohi[2];
