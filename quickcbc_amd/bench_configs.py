@@ -227,14 +227,19 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
 # its source (qcbc_kernels.hip: amp_turns, sincos_turns, F64Seg; an FMA
 # counts 2, exp, atan, sqrt and a division 1 each).  The waveform part by
 # (amplitude region, phase region), 0 inspiral, 1 intermediate, 2
-# merger-ringdown, is the shared powers and products (Mf^(-1/6), Mf^(-1/3),
-# 1/Mf, the Mf^(-7/6) scale and the f tshift_0 term: 8), Mf = Msec f (1, in
-# the intermediate and merger regions), Mf^(1/3) (2, inspiral), and each
-# region's amplitude and phase:
-_FUSED_AMP = (17, 11, 7)     # Horner in Mf^(1/3); Newton form; exp / Lorentzian
-_FUSED_PHASE = (27, 8, 14)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan
-FUSED_WAVEFORM_FLOPS = {(a, p): 8 + (1 if a or p else 0) + (2 if a == 0 or p == 0 else 0)
-                        + _FUSED_AMP[a] + _FUSED_PHASE[p] for a in range(3) for p in range(3)}
+# merger-ringdown: Mf^(-1/6), Mf^(-1/3), Mf^(-2/3) and the f tshift_0 term
+# (5), Mf^(1/3) (2, inspiral), Mf = Msec f and 1/Mf (1 each, intermediate and
+# merger), and each region's amplitude with its Mf^(-7/6) scale and phase:
+_FUSED_AMP = (17, 13, 9)     # Horner in Mf^(1/3) and w^2; Newton form; exp / Lorentzian
+_FUSED_PHASE = (27, 8, 17)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan
+
+
+def _fused_waveform_flops(a, p):
+    return (5 + (2 if a == 0 or p == 0 else 0) + (2 if a or p else 0)
+            + _FUSED_AMP[a] + _FUSED_PHASE[p])
+
+
+FUSED_WAVEFORM_FLOPS = {(a, p): _fused_waveform_flops(a, p) for a in range(3) for p in range(3)}
 # reduction to t in [-1/2, 1/2] (2), t^2 (1), the two degree-6 polynomials
 # (12 + 12), t P (1), two complex squarings (5 each)
 FUSED_SINCOS_FLOPS = 38

@@ -239,13 +239,17 @@ __device__ __forceinline__ double atan_mrd(double x) {
 
 // Point constants a segment's bin loop keeps in vector registers: the first
 // Horner step of the inspiral amplitude, the intermediate amplitude and the
-// inspiral phase each add two point constants, one of which would otherwise
-// be copied into a vector register every bin.
+// inspiral phase, and the inspiral amplitude's w^2 term, each add two point
+// constants, one of which would otherwise be copied into a vector register
+// every bin.
 template <int AR, int PR>
 struct VConst {
-  double a0 = 0.0, p0 = 0.0, pL = 0.0;
+  double a0 = 0.0, a1 = 0.0, p0 = 0.0, pL = 0.0;
   __device__ __forceinline__ explicit VConst(const Point& c) {
-    if (AR == 0) a0 = vreg_v(c.ia83);
+    if (AR == 0) {
+      a0 = vreg_v(c.ia83);
+      a1 = vreg_v(c.ia23);
+    }
     if (AR == 1) a0 = vreg_v(c.ib3);
     if (PR == 0) {
       p0 = vreg_v(c.ins_q53);
@@ -283,10 +287,10 @@ struct SConst {
   __device__ __forceinline__ void load(const Point& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
-    s6_scale = spin(c.s6_scale);
+    if (AR == 0 || PR != 1) s6_scale = spin(c.s6_scale);  // Mf^(1/3), or Mf^(1/2) in the merger phase
     is6_scale = spin(c.is6_scale);
     if (AR == 0) {
-      ia0 = spin(c.ia0); ia23 = spin(c.ia23); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
+      ia0 = spin(c.ia0); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
       ia53 = spin(c.ia53); ia2 = spin(c.ia2); ia73 = spin(c.ia73); ia3 = spin(c.ia3);
     } else if (AR == 1) {
       col_f1 = spin(c.col_f1); col_f2 = spin(c.col_f2); col_f3 = spin(c.col_f3);
@@ -328,7 +332,8 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
                                           double is6f, double logf, double* a, double* r) {
   const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
   const double w = i_s6 * i_s6;            // Mf^(-1/3)
-  const double w3 = w * w * w;             // 1 / Mf
+  const double w2 = w * w;
+  const double w3 = w2 * w;                // 1 / Mf (unused by the inspiral-inspiral pair)
   double Mf = 0.0, v = 0.0;
   if (AR > 0 || PR > 0) {
     Mf = c.Msec * f;
@@ -340,16 +345,16 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
   }
   double A;
   if (AR == 0) {
-    // amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ai_43 v^4 + ... + ai_3 v^9), Horner
-    // on the negated partial sums
+    // Mf^(-7/6) amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ... + ai_3 v^9) with
+    // Mf^(-7/6) = Mf^(-1/6) w^3: Mf^(-1/6) [w (ia0 w^2 + ia23) + ia1 + ia43 v
+    // + ... + ia3 v^6], the v part by Horner on the negated partial sums
     double np = fma(v, -c.ia3, -k.a0);
     np = hstep(np, v, c.ia73);
     np = hstep(np, v, c.ia2);
     np = hstep(np, v, c.ia53);
     np = hstep(np, v, c.ia43);
     np = hstep(np, v, c.ia1);
-    np = hstep(np, v, c.ia23);
-    A = -hstep(np, v * v, c.ia0);
+    *a = i_s6 * fma(w, fma(w2, c.ia0, k.a1), -np);
   } else if (AR == 1) {
     // Newton form on the nodes (f1, f1, f2, f3), Horner on negated partials
     const double x1 = Mf - c.col_f1;
@@ -361,7 +366,7 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     const double df = Mf - c.fRD;
     A = (c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2));
   }
-  *a = (w3 * i_s6) * A;  // Mf^(-7/6)
+  if (AR > 0) *a = (w3 * i_s6) * A;  // Mf^(-7/6)
   // the projected argument: f tshift_0, exact product with one rounding, plus
   // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
   double g;
