@@ -50,8 +50,10 @@ struct qcbc_pt {
   unsigned long long *d_acc = nullptr, *d_swaps = nullptr;
   std::vector<int> perm;  // storage index -> point number t * n_chains + c
   hipEvent_t ev_fork = nullptr, ev_join[QCBC_PT_MAX_GROUPS] = {};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured steps: [0] one step, [1] kGraphSteps steps (the column groups
+  // run through them without meeting, as the free-running streams do)
+  hipGraph_t graph[2] = {};
+  hipGraphExec_t exec[2] = {};
   bool step_timing = false;
   std::vector<double> lat_ms;
 };
@@ -104,8 +106,10 @@ static void pt_free(qcbc_pt* pt) {
   if (pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
   for (hipEvent_t e : pt->ev_join)
     if (e) (void)hipEventDestroy(e);
-  if (pt->exec) (void)hipGraphExecDestroy(pt->exec);
-  if (pt->graph) (void)hipGraphDestroy(pt->graph);
+  for (int i = 0; i < 2; i++) {
+    if (pt->exec[i]) (void)hipGraphExecDestroy(pt->exec[i]);
+    if (pt->graph[i]) (void)hipGraphDestroy(pt->graph[i]);
+  }
   void* bufs[] = {pt->d_x, pt->d_L, pt->d_y, pt->d_Ly, pt->d_rho, pt->d_sigma, pt->d_tc, pt->d_st, pt->d_acc,
                   pt->d_swaps};
   for (void* b : bufs)
@@ -234,6 +238,41 @@ static int pt_sync(qcbc_pt* pt) {
   return QCBC_OK;
 }
 
+// Steps per launch of the multi-step graph.  Within one graph each column
+// group's steps form a chain of its own and the groups join only at the end,
+// so one group's set-up and post kernels overlap another's likelihood as they
+// do on the free-running streams; a one-step graph would join every step.
+constexpr int kGraphSteps = 32;
+
+// Captures n consecutive steps of the whole ensemble into pt->graph/exec[slot].
+static int pt_capture(qcbc_pt* pt, int slot, int n) {
+  qcbc_ctx* c = pt->ctx;
+  hipStream_t s = c->stream;
+  bool timing = c->timing;
+  c->timing = false;  // event records stay out of the graph
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  // the other groups' streams fork from the context stream and join it again
+  int st = QCBC_OK;
+  if (pt->G > 1 && hipEventRecord(pt->ev_fork, s) != hipSuccess) st = QCBC_EHIP;
+  for (int k = 1; k < pt->G && !st; k++)
+    if (hipStreamWaitEvent(pt->g[k].s, pt->ev_fork, 0) != hipSuccess) st = QCBC_EHIP;
+  for (int i = 0; i < n && !st; i++) st = pt_enqueue_step(pt);
+  for (int k = 1; k < pt->G && !st; k++)
+    if (hipEventRecord(pt->ev_join[k], pt->g[k].s) != hipSuccess || hipStreamWaitEvent(s, pt->ev_join[k], 0) != hipSuccess)
+      st = QCBC_EHIP;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  c->timing = timing;
+  if (st) {
+    if (g) hipGraphDestroy(g);
+    return st;
+  }
+  HIPCHK(e);
+  pt->graph[slot] = g;
+  HIPCHK(hipGraphInstantiate(&pt->exec[slot], g, nullptr, nullptr, 0));
+  return QCBC_OK;
+}
+
 int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
   if (!pt || nsteps < 0) return QCBC_EINVAL;
   qcbc_ctx* c = pt->ctx;
@@ -241,35 +280,23 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
   hipSetDevice(c->device);
   hipStream_t s = c->stream;
   pt->lat_ms.clear();
-  if (use_graph && !pt->exec) {
-    // capture one steady-state step; no memset nodes (fills are kernels)
-    bool timing = c->timing;
-    c->timing = false;  // event records stay out of the graph
-    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    // the other groups' streams fork from the context stream and join it again
-    int st = QCBC_OK;
-    if (pt->G > 1 && hipEventRecord(pt->ev_fork, s) != hipSuccess) st = QCBC_EHIP;
-    for (int k = 1; k < pt->G && !st; k++)
-      if (hipStreamWaitEvent(pt->g[k].s, pt->ev_fork, 0) != hipSuccess) st = QCBC_EHIP;
-    if (!st) st = pt_enqueue_step(pt);
-    for (int k = 1; k < pt->G && !st; k++)
-      if (hipEventRecord(pt->ev_join[k], pt->g[k].s) != hipSuccess || hipStreamWaitEvent(s, pt->ev_join[k], 0) != hipSuccess)
-        st = QCBC_EHIP;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s, &g);
-    c->timing = timing;
-    if (st) {
-      if (g) hipGraphDestroy(g);
-      return st;
-    }
-    HIPCHK(e);
-    pt->graph = g;
-    HIPCHK(hipGraphInstantiate(&pt->exec, g, nullptr, nullptr, 0));
+  // no memset nodes in either graph: every fill is a kernel
+  const bool multi = use_graph && !pt->step_timing && nsteps >= kGraphSteps;
+  if (use_graph && !pt->exec[0]) {
+    int st = pt_capture(pt, 0, 1);
+    if (st) return st;
   }
-  for (int i = 0; i < nsteps; i++) {
+  if (multi && !pt->exec[1]) {
+    int st = pt_capture(pt, 1, kGraphSteps);
+    if (st) return st;
+  }
+  int i = 0;
+  if (multi)
+    for (; i + kGraphSteps <= nsteps; i += kGraphSteps) HIPCHK(hipGraphLaunch(pt->exec[1], s));
+  for (; i < nsteps; i++) {
     auto t0 = std::chrono::steady_clock::now();
     if (use_graph) {
-      HIPCHK(hipGraphLaunch(pt->exec, s));
+      HIPCHK(hipGraphLaunch(pt->exec[0], s));
     } else {
       int st = pt_enqueue_step(pt);
       if (st) return st;

@@ -227,6 +227,19 @@ def test_pt_groups_give_the_same_chains():
     for r in (r2, r3):
         assert np.array_equal(r1[0], r[0]) and np.array_equal(r1[1], r[1])
         assert np.array_equal(r1[2], r[2]) and r1[3] == r[3]
+    # 67 more steps: the graph path replays the 32-step graph twice and the
+    # one-step graph three times, the groups running through each 32 steps
+    # without meeting
+    one.run(67)
+    two.run(67)
+    auto.run(67, graph=True)
+    auto.run(1, graph=True)
+    one.run(1)
+    two.run(1)
+    r1, r2, r3 = one.get(), two.get(), auto.get()
+    for r in (r2, r3):
+        assert np.array_equal(r1[0], r[0]) and np.array_equal(r1[1], r[1])
+        assert np.array_equal(r1[2], r[2]) and r1[3] == r[3]
     assert r1[2].sum() > 0 and r1[3] > 0
     Lc, _, _ = c.loglike(r2[0], Q.FP64)
     assert np.allclose(Lc, r2[1], rtol=1e-12, atol=1e-9)
