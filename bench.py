@@ -180,11 +180,16 @@ def main():
         pt.step_timing(True)
         pt.run(args.latency_steps)
         lat = pt.latencies()
-        pt.step_timing(False)
         extra["fp64_step_ms_p50"] = percentile(lat, 50)
         extra["fp64_step_ms_p99"] = percentile(lat, 99)
+        # the same, each step one replay of the captured one-step graph
+        pt.run(args.latency_steps, graph=True)
+        lat = pt.latencies()
+        pt.step_timing(False)
+        extra["fp64_graph_step_ms_p50"] = percentile(lat, 50)
+        extra["fp64_graph_step_ms_p99"] = percentile(lat, 99)
         x, Lx, acc, sw = pt.get()
-        extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + args.latency_steps))
+        extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + 2 * args.latency_steps))
         extra["pt_groups"] = pt.groups
         pt.close()
         # mixed precision on the same shape
@@ -208,6 +213,10 @@ def main():
         lat = ptm.latencies()
         extra["mixed_step_ms_p50"] = percentile(lat, 50)
         extra["mixed_step_ms_p99"] = percentile(lat, 99)
+        ptm.run(args.latency_steps, graph=True)
+        lat = ptm.latencies()
+        extra["mixed_graph_step_ms_p50"] = percentile(lat, 50)
+        extra["mixed_graph_step_ms_p99"] = percentile(lat, 99)
         ptm.close()
         L64, _, _ = ctx.loglike(init, Q.FP64)
         L32, _, _ = ctx.loglike(init, Q.MIXED)
