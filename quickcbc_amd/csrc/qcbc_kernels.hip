@@ -13,11 +13,38 @@
 namespace qcbc {
 
 // ------------------------------------------------------------ utilities
+// One DPP step of a wave reduction on a double (two 32-bit DPP moves; lanes
+// the control leaves without a source, or rows outside ROWMASK, add 0).  With
+// every row written, the bound control alone supplies the zeros.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  int lo, hi;
+  if (ROWMASK == 0xf) {
+    lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, true);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWMASK, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWMASK, 0xf, true);
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes of a wave, in lane 63 (the other lanes hold partial
+// sums): shifts by 1, 2, 4, 8 within each row of 16 lanes leave each row's sum
+// in its lane 15, row_bcast:15 adds row 0's into row 1 and row 2's into row 3,
+// row_bcast:31 adds lane 31 into rows 2 and 3.  Register-to-register, with no
+// LDS round trips.  Every lane must be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31
   return v;
 }
+constexpr int SUM_LANE = 63;
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -547,7 +574,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
     acc[2 * d] = wave_sum(st.hh[d]);
     acc[2 * d + 1] = wave_sum(dh);
   }
-  if (lane == 0 && b < in.B) {
+  if (lane == SUM_LANE && b < in.B) {
 #pragma unroll
     for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
   }
@@ -648,7 +675,7 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   double acc[2 * ND];
 #pragma unroll
   for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum((double)st.sum[q] - (double)st.comp[q]);
-  if (lane == 0 && b < in.B) {
+  if (lane == SUM_LANE && b < in.B) {
 #pragma unroll
     for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
   }
