@@ -120,7 +120,7 @@ def main():
     init = W.ensemble_around(inj, B, seed=99)
     sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
-    plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/6), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
+    plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/3), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
 
     if args.autotune:
         for prec in (Q.FP64, Q.MIXED):

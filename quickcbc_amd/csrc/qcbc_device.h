@@ -340,30 +340,6 @@ __device__ __forceinline__ Pow pow_direct(double Mf) {
   return p;
 }
 
-// Table form: Mf = Msec f_k with f_k^(1/6), f_k^(-1/6) and log f_k
-// precomputed once per analysis, so no transcendental per bin.
-__device__ __forceinline__ Pow pow_table(const Point& c, double f, double s6f, double is6f,
-                                         double logf) {
-  Pow p;
-  p.Mf = c.Msec * f;
-  p.s6 = c.s6_scale * s6f;
-  p.i_s6 = c.is6_scale * is6f;
-  p.third = p.s6 * p.s6;
-  p.two_thirds = p.third * p.third;
-  p.four_thirds = p.Mf * p.third;
-  p.five_thirds = p.four_thirds * p.third;
-  p.two = p.Mf * p.Mf;
-  p.seven_thirds = p.third * p.two;
-  p.eight_thirds = p.two_thirds * p.two;
-  p.i_third = p.i_s6 * p.i_s6;
-  p.i_two_thirds = p.i_third * p.i_third;
-  p.i_Mf = p.i_third * p.i_two_thirds;
-  p.i_five_thirds = p.i_Mf * p.i_two_thirds;
-  p.logMf = c.logMsec + logf;
-  p.logv = (LOG_PI + p.logMf) * (1.0 / 3.0);
-  return p;
-}
-
 // ------------------------------------------------------ amplitude pieces
 // AmpInsAnsatz (IMRPhenomD_internals.c:601-610, arXiv:1508.07253 Eqs. 29-30)
 __device__ __forceinline__ double amp_ins(const Point& c, const Pow& p) {
@@ -646,6 +622,7 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
   c.fCut = F_CUT / c.Msec;
   c.s6_scale = pow(c.Msec, 1.0 / 6.0);
   c.is6_scale = 1.0 / c.s6_scale;
+  c.v_scale = cbrt(c.Msec);
   c.logMsec = log(c.Msec);
   const double pi_s6 = PI_S6, pi_third = PI_THIRD, pi_two_thirds = PI_TWO_THIRDS;
   const double pi_four_thirds = PI_FOUR_THIRDS, pi_five_thirds = PI_FIVE_THIRDS;

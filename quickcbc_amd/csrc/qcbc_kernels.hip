@@ -2,7 +2,7 @@
 //
 // Data layout of one analysis in HBM ("band planes"): band bins
 // j = k - imin, k in [imin, imax), padded to nbp; FP64 planes
-//   0: f_k = k/Tobs   1: f_k^(1/6)   2: f_k^(-1/6)   3: log f_k
+//   0: f_k = k/Tobs   1: f_k^(1/3)   2: f_k^(-1/6)   3: log f_k
 //   4+3d: D_re/S   5+3d: D_im/S   6+3d: 1/S          (detector slot d)
 // and, for the mixed-precision kernel, FP32 planes scaled by w_ref (per
 // detector, so 1/S fits in FP32):
@@ -300,7 +300,7 @@ __device__ __forceinline__ double spin(double v) {
 // Field names follow Point, so amp_turns reads either.
 template <int AR, int PR>
 struct SConst {
-  double Msec, s6_scale, is6_scale, tshift[1];
+  double Msec, v_scale, is6_scale, tshift[1];
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia3;
   double col_f1, col_f2, col_f3, ib0, ib1, ib2, ib4;
   double fRD, mrd_rate, mnum, mg2;
@@ -314,7 +314,7 @@ struct SConst {
   __device__ __forceinline__ void load(const Point& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
-    if (AR == 0 || PR != 1) s6_scale = spin(c.s6_scale);  // Mf^(1/3), or Mf^(1/2) in the merger phase
+    if (AR == 0 || PR == 0) v_scale = spin(c.v_scale);
     is6_scale = spin(c.is6_scale);
     if (AR == 0) {
       ia0 = spin(c.ia0); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
@@ -351,11 +351,11 @@ struct SConst {
 // Amplitude without the detector scale, and the projected detector-0
 // argument in turns, of one bin in amplitude region AR and phase region PR
 // (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
-// constants of the point and the bin's f, f^(1/6), f^(-1/6), log f.  The
+// constants of the point and the bin's f, f^(1/3), f^(-1/6), log f.  The
 // inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
 // they need no other power of Mf.
 template <int AR, int PR, class K>
-__device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double s6f,
+__device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double v3f,
                                           double is6f, double logf, double* a, double* r) {
   const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
   const double w = i_s6 * i_s6;            // Mf^(-1/3)
@@ -366,10 +366,7 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     Mf = c.Msec * f;
     vreg(Mf);  // Mf - node stays one subtraction with the node in a scalar register
   }
-  if (AR == 0 || PR == 0) {
-    const double s6 = c.s6_scale * s6f;
-    v = s6 * s6;  // Mf^(1/3)
-  }
+  if (AR == 0 || PR == 0) v = c.v_scale * v3f;  // Mf^(1/3)
   double A;
   if (AR == 0) {
     // Mf^(-7/6) amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ... + ai_3 v^9) with
@@ -415,8 +412,7 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     const double w9 = w3 * w3 * w3;  // 1 / Mf^3
     g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
   } else {
-    const double u = c.s6_scale * s6f;  // Mf^(1/6)
-    const double sq = (u * u) * u;      // Mf^(1/2)
+    const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
     g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
         c.mrd_q0;
@@ -465,8 +461,9 @@ __device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int
 template <int ND, int TILE>
 struct F64State {
   const Point* c;
-  const double* lb;  // LDS tile base minus the tile's first band bin
-  int j;             // the lane's next band bin
+  const double* lds;  // the LDS tile
+  int t0;             // its first band bin
+  int j;              // the lane's next band bin
   double hh[ND];     // sum a^2 / S_d
   double dh0;        // sum a Re(D_0 e^{-i x0}) / S_0
   double hr[ND], hi[ND];  // Horner sums of detectors d > 0
@@ -477,9 +474,8 @@ template <int ND, int TILE>
 struct F64Seg {
   template <int AR, int PR>
   struct S {
-    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const SConst<AR, PR>& c,
+    static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const double* L, const SConst<AR, PR>& c,
                                                const VConst<AR, PR>& k) {
-      const double* L = st.lb + st.j;
       double a, r;
       amp_turns<AR, PR>(c, k, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
       double s0, c0;
@@ -502,16 +498,24 @@ struct F64Seg {
           fma_acc_x(st.hr[d], c.wc[d], tr);       // hr wc - hi ws + yr
         }
       }
-      st.j += 64;
     }
+    // The lane walks its bins by their byte offset in the tile (one add and
+    // one compare per bin); its band index advances by the bins it took.
     static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
       c.template load<ND>(*st.c);
       const int nit = (end - pos + 63) >> 6;
+      const int j0 = st.j;
+      unsigned o = 8u * (unsigned)(j0 - st.t0);
+      const unsigned oend = 8u * (unsigned)(end - st.t0);
       for (int n = 0; n < nit; n++) {
-        if (st.j < end) bin(st, c, k);
+        if (o < oend) {
+          bin(st, reinterpret_cast<const double*>(reinterpret_cast<const char*>(st.lds) + o), c, k);
+          o += 8u * 64u;
+        }
       }
+      if (j0 < end) st.j = j0 + 64 * ((end - j0 + 63) >> 6);
     }
   };
 };
@@ -553,7 +557,8 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
     }
     __syncthreads();
     if (active) {
-      st.lb = lds - t0;
+      st.lds = lds;
+      st.t0 = t0;
       pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }
@@ -590,8 +595,9 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
 template <int ND, int TILE>
 struct MixState {
   const Point* c;
-  const double* lb;  // LDS tile bases minus the tile's first band bin
-  const float* lb32;
+  const double* lds;  // the LDS tiles and the first band bin
+  const float* lds32;
+  int t0;
   int j;
   float sum[2 * ND], comp[2 * ND];
 };
@@ -607,8 +613,8 @@ struct MixSeg {
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
         if (st.j < end) {
-          const double* L = st.lb + st.j;
-          const float* M = st.lb32 + st.j;
+          const double* L = st.lds + (st.j - st.t0);
+          const float* M = st.lds32 + (st.j - st.t0);
           const double f = L[0];
           double ad, r;
           amp_turns<AR, PR>(c, k, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
@@ -667,8 +673,9 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
     }
     __syncthreads();
     if (active) {
-      st.lb = lds - t0;
-      st.lb32 = lds32 - t0;
+      st.lds = lds;
+      st.lds32 = lds32;
+      st.t0 = t0;
       pos = run_segments<MixSeg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
     }
   }

@@ -119,7 +119,7 @@ __global__ void k_bind(const double* __restrict__ D, const double* __restrict__ 
   int k = imin + j;
   double f = (k == 0) ? 1.0 / Tobs : (double)k / Tobs;
   p64[0 * (size_t)nbp + j] = f;
-  p64[1 * (size_t)nbp + j] = pow(f, 1.0 / 6.0);
+  p64[1 * (size_t)nbp + j] = cbrt(f);
   p64[2 * (size_t)nbp + j] = pow(f, -1.0 / 6.0);
   p64[3 * (size_t)nbp + j] = log(f);
   for (int d = 0; d < nd; d++) {

@@ -24,7 +24,7 @@ struct alignas(16) Point {
   // and the phase leaves the inspiral (Mf >= 0.018) and enters the
   // merger-ringdown (Mf >= fPhiMRD); clipped to [kstart, kend].
   int seg_a1, seg_a2, seg_p1, seg_p2;
-  double Msec, s6_scale, is6_scale;
+  double Msec, s6_scale, is6_scale, v_scale;  // Msec^(1/6), Msec^(-1/6), Msec^(1/3)
   // projection: detector time shifts (s), and for detector d the rotation of
   // its phase relative to detector 0 over 64 bins, exp(i 2 pi 64 dt_d / Tobs)
   // = wc_d + i ws_d
@@ -70,6 +70,7 @@ struct alignas(16) Point {
   // projection per detector slot
   double Fs[3], lambda[3], Ascale[3];
   double eta, finspin;
+  double pad;  // sizeof a multiple of the 16-byte alignment without hidden padding
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are

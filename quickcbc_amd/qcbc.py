@@ -583,7 +583,7 @@ class PointRecord(ctypes.Structure):
     """Mirror of qcbc::Point (quickcbc_amd/csrc/qcbc_types.h)."""
     _fields_ = ([(n, ctypes.c_int) for n in ("status", "kstart", "kend", "region_flags",
                                              "seg_a1", "seg_a2", "seg_p1", "seg_p2")] +
-                [(n, ctypes.c_double) for n in ("Msec", "s6_scale", "is6_scale")] +
+                [(n, ctypes.c_double) for n in ("Msec", "s6_scale", "is6_scale", "v_scale")] +
                 [(n, ctypes.c_double * 3) for n in ("tshift", "wc", "ws")] +
                 [(n, ctypes.c_double) for n in (
                     "ins_q0 ins_q1 ins_q13 ins_q13L ins_qL ins_q23 ins_q43 ins_q53 ins_q2 ins_qm13 ins_qm23 "
@@ -597,7 +597,7 @@ class PointRecord(ctypes.Structure):
                     "pi_m13 pi_m23 pi_m1 pi_m53 pi_1 pi_43 pi_53 pi_2 fPhiMRD beta1 beta2 beta3 C1Int C2Int "
                     "alpha1 alpha2 alpha3 alpha4 a5fRD C1MRD C2MRD t0 MfRef phi_precalc").split()] +
                 [(n, ctypes.c_double * 3) for n in ("Fs", "lambda_", "Ascale")] +
-                [("eta", ctypes.c_double), ("finspin", ctypes.c_double)])
+                [("eta", ctypes.c_double), ("finspin", ctypes.c_double), ("pad", ctypes.c_double)])
 
 
 def point_records(ctx, params):
