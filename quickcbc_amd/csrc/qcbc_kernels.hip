@@ -264,6 +264,12 @@ __device__ __forceinline__ double atan_mrd(double x) {
   return copysign(r, x);
 }
 
+// The point records as the fused FP64 kernel reads them: through the
+// constant address space (read-only while a kernel runs).  Their wave-uniform
+// fields then stay scalar loads even after the compiler-opaque waits of the
+// tile staging, which would otherwise send them down the vector memory path.
+typedef const __attribute__((address_space(4))) Point CPoint;
+
 // Point constants a segment's bin loop keeps in vector registers: the first
 // Horner step of the inspiral amplitude, the intermediate amplitude and the
 // inspiral phase, and the inspiral amplitude's w^2 term, each add two point
@@ -272,7 +278,8 @@ __device__ __forceinline__ double atan_mrd(double x) {
 template <int AR, int PR>
 struct VConst {
   double a0 = 0.0, a1 = 0.0, p0 = 0.0, pL = 0.0;
-  __device__ __forceinline__ explicit VConst(const Point& c) {
+  template <class PT>
+  __device__ __forceinline__ explicit VConst(const PT& c) {
     if (AR == 0) {
       a0 = vreg_v(c.ia83);
       a1 = vreg_v(c.ia23);
@@ -310,8 +317,8 @@ struct SConst {
   double wc[3], ws[3], dt[3], dl[3];
   // ND detectors; the Horner rotation (wc, ws) for the FP64 kernel, the
   // detector phase offsets (dt, dl) for the mixed one
-  template <int ND, bool ROT = true>
-  __device__ __forceinline__ void load(const Point& c) {
+  template <int ND, bool ROT = true, class PT>
+  __device__ __forceinline__ void load(const PT& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
     if (AR == 0 || PR == 0) v_scale = spin(c.v_scale);
@@ -460,7 +467,7 @@ __device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int
 // per lane and detector at the end.
 template <int ND, int TILE>
 struct F64State {
-  const Point* c;
+  CPoint* c;
   const double* lds;  // the LDS tile
   int t0;             // its first band bin
   int j;              // the lane's next band bin
@@ -532,7 +539,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   int ks, ke, blo, bhi;
   block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
   F64State<ND, TILE> st;
-  st.c = in.pts + uniform(active ? b : 0);
+  st.c = (CPoint*)(in.pts + uniform(active ? b : 0));
   st.j = ks + lane;
   st.sk = sincos_consts();
   st.dh0 = 0.0;
@@ -540,26 +547,59 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
   int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
   if (active) {
-    const Point& c = *st.c;
+    CPoint& c = *st.c;
     a1 = c.seg_a1 - in.imin;
     a2 = c.seg_a2 - in.imin;
     p1 = c.seg_p1 - in.imin;
     p2 = c.seg_p2 - in.imin;
   }
-  for (int t0 = blo; t0 < bhi; t0 += TILE) {
-    const int tn = min(TILE, bhi - t0);
-    __syncthreads();
-    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
-    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-      const unsigned off = 8u * i;
+  if (blockDim.x >= TILE) {
+    // One bin per thread: a thread loads its bin of the next tile as soon as
+    // its wave is done with this one, so the loads are in flight while the
+    // wave waits at the barrier for the others.  Loads and stores of a tile
+    // sit in one iteration, so their registers are never live across the
+    // bin loops.
+    auto stage = [&](int t0) {
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
+      double pre[NP];
+      const bool mine = (int)threadIdx.x < min(TILE, bhi - t0);
+      if (mine) {
 #pragma unroll
-      for (int p = 0; p < NP; p++) lds[p * TILE + i] = buffer_load_f64(rs, off, p * in.nbp * 8);
+        for (int p = 0; p < NP; p++) pre[p] = buffer_load_f64(rs, 8u * threadIdx.x, p * in.nbp * 8);
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's reads of the tile before are done
+      __builtin_amdgcn_s_barrier();                        // and every wave's
+      if (mine) {
+#pragma unroll
+        for (int p = 0; p < NP; p++) lds[p * TILE + threadIdx.x] = pre[p];
+      }
+      __syncthreads();
+    };
+    if (blo < bhi) stage(blo);
+    for (int t0 = blo; t0 < bhi; t0 += TILE) {
+      if (active) {
+        st.lds = lds;
+        st.t0 = t0;
+        pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + TILE), a1, a2, p1, p2);
+      }
+      if (t0 + TILE < bhi) stage(t0 + TILE);
     }
-    __syncthreads();
-    if (active) {
-      st.lds = lds;
-      st.t0 = t0;
-      pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
+  } else {
+    for (int t0 = blo; t0 < bhi; t0 += TILE) {
+      const int tn = min(TILE, bhi - t0);
+      __syncthreads();
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
+      for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+        const unsigned off = 8u * i;
+#pragma unroll
+        for (int p = 0; p < NP; p++) lds[p * TILE + i] = buffer_load_f64(rs, off, p * in.nbp * 8);
+      }
+      __syncthreads();
+      if (active) {
+        st.lds = lds;
+        st.t0 = t0;
+        pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
+      }
     }
   }
   double acc[2 * ND];
@@ -570,7 +610,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
       // back from the lane's last bin: e^{-i theta_d,last} H (zero if the lane took no bin)
       dh = 0.0;
       if (active && st.j - 64 >= ks) {
-        const Point& c = *st.c;
+        CPoint& c = *st.c;
         double s, co;
         sincos_turns(fma(in.p64[st.j - 64], c.dt[d], c.dl[d]), &s, &co);
         dh = fma(st.hr[d], co, st.hi[d] * s);
