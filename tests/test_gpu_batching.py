@@ -139,6 +139,26 @@ def test_multi_autotune_and_shapes(ctxs):
     m.close()
 
 
+@pytest.mark.parametrize("prec", [Q.FP64, Q.MIXED])
+def test_ctx_launch_shapes(prec):
+    """Every launch shape gives the default shape's logL: one bin per thread
+    (the staging that prefetches the next tile) and fewer threads than tile
+    bins (the staging loop), 1 to 16 points per block, ragged point counts."""
+    an, inj, D, SN = _data.analysis(1)
+    c = Q.Context(an, D, SN)
+    P = np.vstack([inj, _data.points(1, 100, seed=71)])
+    ref, _, st = c.loglike(P, prec)
+    assert (st == 0).all()
+    tol = (1e-9, 1e-9) if prec == Q.FP64 else (1e-6, 1e-4)
+    for cpb, tile, splits in [(1, 128, 0), (2, 512, 0), (4, 512, 3), (8, 256, 0), (12, 512, 0), (16, 512, 7),
+                              (16, 128, 0), (3, 256, 1)]:
+        c.set_shape(prec, cpb, tile, splits)
+        L, _, st = c.loglike(P, prec)
+        assert (st == 0).all()
+        assert np.allclose(L, ref, rtol=tol[0], atol=tol[1]), (cpb, tile, splits, np.abs(L - ref).max())
+    c.close()
+
+
 def test_server_alone_and_mixed_and_admission():
     budget = 2 * 1024 ** 3
     s = Q.Server(budget, Q.FP64)
