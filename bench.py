@@ -192,6 +192,17 @@ def main():
         extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + 2 * args.latency_steps))
         extra["pt_groups"] = pt.groups
         pt.close()
+        # latency-first configuration: one column group (three launches per
+        # step instead of nine, one stream to wait for)
+        ptl = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=1)
+        ptl.run(args.warmup)
+        ptl.step_timing(True)
+        ptl.run(args.latency_steps)
+        lat = ptl.latencies()
+        ptl.step_timing(False)
+        ptl.close()
+        extra["fp64_one_group_step_ms_p50"] = percentile(lat, 50)
+        extra["fp64_one_group_step_ms_p99"] = percentile(lat, 99)
         # mixed precision on the same shape
         ptm = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED, groups=args.pt_groups)
         ptm.run(args.warmup)

@@ -598,11 +598,19 @@ __device__ inline double damp_mrd(const Point& c, double f, double gamma1, doubl
   return (-2 * c.fDM * df * gamma3 * gamma1) / (ex * (s * s)) - (gamma2 * gamma1) / (ex * s);
 }
 
-// ------------------------------------------------------ per-point set-up
-// IMRPhenomDGenerateh22FDAmpPhase's checks and set-up (IMRPhenomD.c:186-523)
-// for masses in kg and distance in m; fmin0 is the first output frequency.
-__device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, double chi2_in, double distance,
-                                    double phi0, double fref, double fmin0, Point& c) {
+// The prefix both halves of IMRPhenomD's set-up need: the ordered masses and
+// spins, eta, chi_PN and delta = sqrt(1 - 4 eta).
+struct PDPre {
+  double m1, m2, chi1, chi2, eta, chi, delta_m;
+};
+
+// IMRPhenomD's set-up in four parts: the prefix (domain checks, masses,
+// scales, final spin, ringdown and damping frequencies), the amplitude
+// coefficients, the phase coefficients with their C1 connection, and the
+// time shift and reference phase, which need both halves.  phenomd_setup runs
+// them in order in one thread; k_setup runs the two middle parts in two waves.
+__device__ inline int phenomd_prefix(double m1_SI, double m2_SI, double chi1_in, double chi2_in, double distance,
+                                     double fref, double fmin0, Point& c, PDPre& o) {
   // IMRPhenomD.c:197-242
   const double m1_in = m1_SI / MSUN_SI, m2_in = m2_SI / MSUN_SI;
   if (!(m1_in > 0) || !(m2_in > 0) || !(distance > 0) || !(fref >= 0) || !(fmin0 > 0)) return ST_EDOM;
@@ -624,10 +632,6 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
   c.is6_scale = 1.0 / c.s6_scale;
   c.v_scale = cbrt(c.Msec);
   c.logMsec = log(c.Msec);
-  const double pi_s6 = PI_S6, pi_third = PI_THIRD, pi_two_thirds = PI_TWO_THIRDS;
-  const double pi_four_thirds = PI_FOUR_THIRDS, pi_five_thirds = PI_FIVE_THIRDS;
-  const double pi2 = PI * PI;
-
   // final state (IMRPhenomD_internals.c:415-513)
   const double finspin = final_spin(eta, chi1, chi2);
   c.finspin = finspin;
@@ -635,10 +639,18 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
   const double er = 1.0 - erad(eta, chi1, chi2);
   c.fRD = qnm_spline(0, finspin) / er;
   c.fDM = qnm_spline(1, finspin) / er;
-
-  // amplitude coefficients (:1072-1100)
   const double delta_m = sqrt(1.0 - 4.0 * eta);
   const double chi = ((chi1 + chi2) / 2.0) * (1.0 - eta * 76.0 / 113.0) + delta_m * ((chi1 - chi2) / 2.0);  // chiPN
+  o = PDPre{m1, m2, chi1, chi2, eta, chi, delta_m};
+  return ST_OK;
+}
+
+__device__ inline void phenomd_amp(const PDPre& o, Point& c) {
+  const double chi1 = o.chi1, chi2 = o.chi2, eta = o.eta, chi = o.chi, delta_m = o.delta_m;
+  const double pi_s6 = PI_S6, pi_two_thirds = PI_TWO_THIRDS;
+  const double pi_four_thirds = PI_FOUR_THIRDS, pi_five_thirds = PI_FIVE_THIRDS;
+  const double pi2 = PI * PI;
+  // amplitude coefficients (:1072-1100)
   const double g1 = fit11(QCBC_FIT_GAMMA1, eta, chi), g2 = fit11(QCBC_FIT_GAMMA2, eta, chi),
                g3 = fit11(QCBC_FIT_GAMMA3, eta, chi);
   c.fDMg3 = c.fDM * g3;
@@ -693,6 +705,11 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
       c.col_c0 = c.col_c1 = c.col_c2 = c.col_c3 = c.col_c4 = 0.0;
     }
   }
+}
+
+__device__ inline void phenomd_phase(const PDPre& o, Point& c) {
+  const double m1 = o.m1, m2 = o.m2, chi1 = o.chi1, chi2 = o.chi2, eta = o.eta, chi = o.chi;
+  const double pi_third = PI_THIRD, pi_two_thirds = PI_TWO_THIRDS, pi_five_thirds = PI_FIVE_THIRDS;
   // phase coefficients (:1509-1540)
   const double sg1 = fit11(QCBC_FIT_SIGMA1, eta, chi), sg2 = fit11(QCBC_FIT_SIGMA2, eta, chi),
                sg3 = fit11(QCBC_FIT_SIGMA3, eta, chi), sg4 = fit11(QCBC_FIT_SIGMA4, eta, chi);
@@ -732,6 +749,9 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
     c.C2MRD = dphiT - dphi_mrd(c, fm);
     c.C1MRD = phiT - c.inv_eta * phi_mrd(c, fm) - c.C2MRD * fm;
   }
+}
+
+__device__ inline void phenomd_refphase(double phi0, double fref, Point& c) {
   // time shift and reference phase (IMRPhenomD.c:504-521)
   c.t0 = dphi_mrd(c, c.fmaxCalc);
   c.MfRef = c.Msec * fref;
@@ -740,6 +760,16 @@ __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, 
     Pow pr = pow_direct(c.MfRef);
     c.phi_precalc = 2. * phi0 + phase(c, pr);  // phase() subtracts t0*0 + 0 here
   }
+}
+
+__device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, double chi2_in, double distance,
+                                    double phi0, double fref, double fmin0, Point& c) {
+  PDPre o;
+  const int st = phenomd_prefix(m1_SI, m2_SI, chi1_in, chi2_in, distance, fref, fmin0, c, o);
+  if (st != ST_OK) return st;
+  phenomd_amp(o, c);
+  phenomd_phase(o, c);
+  phenomd_refphase(phi0, fref, c);
   return ST_OK;
 }
 
@@ -833,6 +863,32 @@ __device__ inline int point_setup_model(const double* prm, const AnalysisConst& 
   int st = phenomd_setup(m1_SI, m2_SI, prm[2], prm[3], exp_cr(prm[6]), 0.5 * prm[4], an.fref, an.fmin0, c);
   if (st != ST_OK) c.status = st;
   return st;
+}
+
+// The same in halves for two threads: each converts the masses and runs the
+// prefix (both write the prefix fields, with the same values), then half 0
+// the amplitude coefficients and half 1 the phase coefficients.  After both,
+// point_setup_refphase completes the model.
+__device__ inline int point_setup_model_half(const double* prm, const AnalysisConst& an, Point& c, int half) {
+  c.status = ST_OK;
+  c.kstart = c.kend = an.imin;
+  const double mc = exp_cr(prm[0]), mt = exp_cr(prm[1]);
+  const double eta_in = pow_cr(mc / mt, 5.0 / 3.0);
+  const double dm = eta_in > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta_in);
+  const double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  PDPre o;
+  const int st = phenomd_prefix(m1_SI, m2_SI, prm[2], prm[3], exp_cr(prm[6]), an.fref, an.fmin0, c, o);
+  if (st != ST_OK) {
+    c.status = st;
+    return st;
+  }
+  if (half == 0) phenomd_amp(o, c);
+  else phenomd_phase(o, c);
+  return ST_OK;
+}
+
+__device__ inline void point_setup_refphase(const double* prm, const AnalysisConst& an, Point& c) {
+  phenomd_refphase(0.5 * prm[4], an.fref, c);
 }
 
 // The projection constants (QuickCBC.c:3734-3750); Ascale without amp0,

@@ -60,25 +60,29 @@ __global__ void k_qnm_init() {
 }
 
 // ------------------------------------------------------------- set-up
-// 64 points per block, three waves: wave 0 the model (masses, IMRPhenomD
-// set-up), wave 1 fbegin, wave 2 the projection, each writing its own fields
-// of the record in place in global memory (a ~1 KB local copy would not fit in
-// registers); after the barrier wave 0 finishes the record.  The three parts
-// are independent, so a point's set-up latency is the longest part's instead
-// of their sum.
-__global__ __launch_bounds__(192) void k_setup(const double* __restrict__ params, int B, AnalysisConst an,
+// 64 points per block, four waves: waves 0 and 3 the model (both the masses
+// and IMRPhenomD's prefix, then wave 0 the amplitude and wave 3 the phase
+// coefficients), wave 1 fbegin, wave 2 the projection, each writing its own
+// fields of the record in place in global memory (a ~1 KB local copy would not
+// fit in registers; the prefix fields both model waves write hold the same
+// values); after the barrier wave 0 adds the time shift and reference phase
+// and finishes the record.  The parts are independent, so a point's set-up
+// latency is the longest part's instead of their sum.
+__global__ __launch_bounds__(256) void k_setup(const double* __restrict__ params, int B, AnalysisConst an,
                                                Point* __restrict__ pts, int* __restrict__ status) {
   const int lane = threadIdx.x & 63, role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.x * 64 + lane;
   const bool on = b < B;
   int st = ST_OK;
+  double prm[11];
   if (on) {
-    double prm[11];
 #pragma unroll
     for (int i = 0; i < 11; i++) prm[i] = params[11 * b + i];
     Point& c = pts[b];
     if (role == 0)
-      st = point_setup_model(prm, an, c);
+      st = point_setup_model_half(prm, an, c, 0);
+    else if (role == 3)
+      point_setup_model_half(prm, an, c, 1);
     else if (role == 1)
       c.fs = fbegin(prm);
     else
@@ -87,8 +91,12 @@ __global__ __launch_bounds__(192) void k_setup(const double* __restrict__ params
   __syncthreads();
   if (on && role == 0) {
     Point& c = pts[b];
-    if (st == ST_OK) point_setup_finish(an, c);
-    else c.kstart = c.kend = an.imin;
+    if (st == ST_OK) {
+      point_setup_refphase(prm, an, c);
+      point_setup_finish(an, c);
+    } else {
+      c.kstart = c.kend = an.imin;
+    }
     c.status = st;
     if (status) status[b] = st;
   }
@@ -288,7 +296,7 @@ hipError_t launch_qnm_init(hipStream_t s) {
 hipError_t launch_setup(const double* d_params, int B, const AnalysisConst& an, Point* d_pts, int* d_status,
                         hipStream_t s) {
   if (B <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3((B + 63) / 64), dim3(192), 0, s, d_params, B, an, d_pts, d_status);
+  hipLaunchKernelGGL(k_setup, dim3((B + 63) / 64), dim3(256), 0, s, d_params, B, an, d_pts, d_status);
   return hipGetLastError();
 }
 
