@@ -33,7 +33,7 @@ FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
 # over the 2048 points), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (profiles/k_ll_fp64_pt_step_hbm_traffic.md), each scaled by what the counter
 # reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-K_LL_FP64_TRAFFIC_BYTES = 30_650_000
+K_LL_FP64_TRAFFIC_BYTES = 30_670_000
 
 
 def log(*a):
