@@ -205,3 +205,87 @@ def pow_cr(x, y):
     L.qo_pow_cr.restype = ctypes.c_double
     L.qo_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
     return L.qo_pow_cr(x, y)
+
+
+# ------------------------------------------------- heterodyned likelihood
+def legendre(J, U):
+    """legendre_maker (QuickCBC.c:5055): P[j][k], j <= J, k <= U."""
+    L = lib()
+    L.qo_legendre.restype = None
+    L.qo_legendre.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    P = np.zeros((J + 1, U + 1))
+    L.qo_legendre(J, U, _d(P))
+    return P
+
+
+def inverse(M):
+    """Inverse (Utilities.c:1862): the transpose of inv(M)."""
+    L = lib()
+    L.qo_inverse.restype = None
+    L.qo_inverse.argtypes = [_dp, _dp, ctypes.c_int]
+    M = np.ascontiguousarray(M, np.float64)
+    IM = np.zeros_like(M)
+    L.qo_inverse(_d(M), _d(IM), M.shape[0])
+    return IM
+
+
+def fullphaseamp(labels, gmst, fref, Tobs, freq, params):
+    """fullphaseamp (QuickCBC.c:3780) on a frequency list: (status, amp, phase), nifo x n."""
+    L = lib()
+    L.qo_fullphaseamp_flat.restype = ctypes.c_int
+    L.qo_fullphaseamp_flat.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       _dp, ctypes.c_int, _dp, _dp, _dp]
+    labels = np.ascontiguousarray(labels, np.int32)
+    freq = np.ascontiguousarray(freq, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    amp = np.zeros((labels.size, freq.size))
+    ph = np.zeros((labels.size, freq.size))
+    st = L.qo_fullphaseamp_flat(labels.size, _i(labels), gmst, fref, Tobs, _d(freq), freq.size, _d(p),
+                                _d(amp), _d(ph))
+    return st, amp, ph
+
+
+def heterodyne(labels, gmst, fref, D, SN, N, Tobs, params, fgrid, J, MN, MM):
+    """heterodyne (QuickCBC.c:5112) on a given grid: (status, het dict)."""
+    L = lib()
+    L.qo_heterodyne_flat.restype = ctypes.c_int
+    L.qo_heterodyne_flat.argtypes = ([ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                                      ctypes.c_int, ctypes.c_double, _ip] + [ctypes.c_int] * 4 + [_dp] * 10)
+    labels = np.ascontiguousarray(labels, np.int32)
+    nd = labels.size
+    fgrid = np.ascontiguousarray(fgrid, np.int32)
+    M = fgrid.size
+    NR = (M - 1) // J
+    D = np.ascontiguousarray(D, np.float64)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    h = dict(M=M, J=J, NR=NR, MN=MN, MM=MM, fgrid=fgrid, freq=fgrid / Tobs,
+             amp=np.zeros((nd, M)), phase=np.zeros((nd, M)), IP=np.zeros((NR, J + 1, J + 1)),
+             SL=np.zeros((NR, nd, J + 1)), lc=np.zeros((NR, nd, J + 1)), ls=np.zeros((NR, nd, J + 1)),
+             rc=np.zeros((nd, NR)), rs=np.zeros((nd, NR)), aa=np.zeros((nd, NR)))
+    lb = np.zeros(1)
+    st = L.qo_heterodyne_flat(nd, _i(labels), gmst, fref, _d(D), _d(SN), _d(p), N, Tobs, _i(fgrid), M, J, MN, MM,
+                              *[_d(h[k]) for k in ("amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")],
+                              _d(lb))
+    h["logLb"] = float(lb[0])
+    h["pref"] = p.copy()
+    return st, h
+
+
+def loglike_het(labels, gmst, fref, Tobs, het, params, nthreads=0):
+    """log_likelihood_het (QuickCBC.c:5303) for a batch: (logL, status)."""
+    L = lib()
+    L.qo_loglike_het_batch_flat.restype = None
+    L.qo_loglike_het_batch_flat.argtypes = ([ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_dp] * 10 +
+                                            [ctypes.c_double, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int])
+    labels = np.ascontiguousarray(labels, np.int32)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+    B = p.shape[0]
+    logL = np.zeros(B)
+    st = np.zeros(B, np.int32)
+    arr = [np.ascontiguousarray(het[k], np.float64)
+           for k in ("freq", "amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")]
+    L.qo_loglike_het_batch_flat(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"],
+                                *[_d(a) for a in arr], het["logLb"], _d(p), B, _d(logL), _i(st), nthreads)
+    return logL, st

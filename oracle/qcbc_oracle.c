@@ -1005,3 +1005,430 @@ double qo_probe(int which, double m1, double m2, double chi1, double chi2, doubl
     default: return NAN;
   }
 }
+
+/* ------------------------------------------------ heterodyned likelihood */
+/* fullphaseamp, QuickCBC.c:3780-3869: per detector, the projected amplitude
+ * Fs h22fac A(f)/sqrt(Tobs) with the taper exp(10 (f - fs)/fs) below fbegin,
+ * and the phase 2 pi f (ts - td) + lambda - Psi(f), on a list of N/2
+ * frequencies.  amp and phase are nifo rows of N/2.  The mass conversion is
+ * the correctly rounded one of the other entry points.  Returns the
+ * IMRPhenomD status (on a domain error the reference dereferences a null
+ * waveform; here the rows are left untouched and the status returned). */
+int qo_fullphaseamp(int nifo, const int *labels, double gmst, double fref, double Tobs, double **amp,
+                    double **phase, const double *freq, const double *params, int N) {
+  double dtimes[3];
+  double fs = qo_fbegin(params);
+  double sqT = sqrt(Tobs);
+  double mc = qo_exp_cr(params[0]), mt = qo_exp_cr(params[1]);
+  double eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
+  double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  double phi0 = 0.5 * params[4];
+  double ts = Tobs - params[5];
+  double distance = qo_exp_cr(params[6]);
+  double alpha = params[7], sindelta = params[8], psi = params[9], ciota = params[10];
+  double Ap = (1.0 + ciota * ciota) / 2.0, Ac = -ciota;
+  int n = N / 2, id, i, st;
+  double *wa, *wp;
+  if (n < 1) return QO_EFAULT;
+  wa = malloc(sizeof(double) * n);
+  wp = malloc(sizeof(double) * n);
+  if (!wa || !wp) {
+    free(wa);
+    free(wp);
+    return QO_ENOMEM;
+  }
+  st = qo_h22_ampphase(freq, n, phi0, fref, m1_SI, m2_SI, params[2], params[3], distance, wa, wp);
+  if (st == 0) {
+    qo_times(alpha, sindelta, gmst, dtimes);
+    for (id = 0; id < nifo; id++) {
+      double Fp = 0.0, Fc = 0.0, Fs, lambda, td;
+      qo_f_ant(psi, alpha, sindelta, gmst, &Fp, &Fc, labels[id]);
+      Fs = sqrt(Ap * Ap * Fp * Fp + Ac * Ac * Fc * Fc);
+      lambda = atan2(Ac * Fc, Ap * Fp);
+      if (lambda < 0.0) lambda += QO_TPI;
+      td = dtimes[labels[id]];
+      for (i = 0; i < n; i++) {
+        double f = freq[i];
+        double A = Fs * QO_H22FAC * wa[i] / sqT;
+        double x = QO_TPI * f * (ts - td) + lambda - wp[i];
+        double y = 1.0;
+        if (f < fs) y = exp(10.0 * (f - fs) / fs);
+        amp[id][i] = A * y;
+        phase[id][i] = x;
+      }
+    }
+  }
+  free(wa);
+  free(wp);
+  return st;
+}
+
+/* legendre_maker, QuickCBC.c:5055-5094: P[j][k], j = 0..J, k = 0..U, the
+ * polynomials orthogonal on the U + 1 points k (discrete Chebyshev or Gram
+ * polynomials, P_j(0) = 1), by the three-term recurrence
+ *   (i+1)(U-i) P_{i+1}(k) = (2i+1)(U-2k) P_i(k) - i(U+i+1) P_{i-1}(k)
+ * with P_0 = 1, P_1 = (U-2k)/U; the coefficients are built by running sums
+ * as the reference builds them.  P is row-major (J+1) x (U+1), J >= 1. */
+void qo_legendre(int J, int U, double *P) {
+  int i, k;
+  const int W = U + 1;
+  for (k = 0; k <= U; ++k) {
+    double c0 = 0.0, c1 = (double)(U - 2 * k), c2 = (double)(U);
+    double d0 = c2, d1 = c1 + c1, d2 = d0;
+    P[0 * W + k] = 1.0;
+    P[1 * W + k] = c1 / (double)(U);
+    for (i = 1; i < J; ++i) {
+      d0 += 2.0;
+      d2 -= 2.0;
+      c0 += d0;
+      c1 += d1;
+      c2 += d2;
+      P[(i + 1) * W + k] = (c1 * P[i * W + k] - c0 * P[(i - 1) * W + k]) / c2;
+    }
+  }
+}
+
+/* Inverse, Utilities.c:1862-1903: IM = transpose(inverse(M)), d x d
+ * row-major, d <= 9.  The reference inverts with GSL (gsl_linalg_LU_decomp,
+ * gsl_linalg_LU_invert); GSL is absent here, so this is the same method
+ * written out: Doolittle elimination with row pivoting on the largest
+ * |a_ij| of the column (first one on ties), then the inverse column by
+ * column by forward and back substitution of the permuted unit vectors. */
+void qo_inverse(const double *M, double *IM, int d) {
+  double a[81], x[9], inv[81];
+  int perm[9];
+  int i, j, k, c;
+  for (i = 0; i < d * d; i++) a[i] = M[i];
+  for (i = 0; i < d; i++) perm[i] = i;
+  for (j = 0; j < d - 1; j++) {
+    int ip = j;
+    double amax = fabs(a[j * d + j]);
+    for (i = j + 1; i < d; i++)
+      if (fabs(a[i * d + j]) > amax) {
+        amax = fabs(a[i * d + j]);
+        ip = i;
+      }
+    if (ip != j) {
+      for (k = 0; k < d; k++) {
+        double t = a[j * d + k];
+        a[j * d + k] = a[ip * d + k];
+        a[ip * d + k] = t;
+      }
+      k = perm[j];
+      perm[j] = perm[ip];
+      perm[ip] = k;
+    }
+    if (a[j * d + j] != 0.0)
+      for (i = j + 1; i < d; i++) {
+        double l = a[i * d + j] / a[j * d + j];
+        a[i * d + j] = l;
+        for (k = j + 1; k < d; k++) a[i * d + k] -= l * a[j * d + k];
+      }
+  }
+  for (c = 0; c < d; c++) {
+    for (i = 0; i < d; i++) x[i] = (perm[i] == c) ? 1.0 : 0.0;
+    for (i = 1; i < d; i++)
+      for (k = 0; k < i; k++) x[i] -= a[i * d + k] * x[k];
+    for (i = d - 1; i >= 0; i--) {
+      for (k = i + 1; k < d; k++) x[i] -= a[i * d + k] * x[k];
+      x[i] /= a[i * d + i];
+    }
+    for (i = 0; i < d; i++) inv[i * d + c] = x[i];
+  }
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++) IM[i * d + j] = inv[j * d + i];
+}
+
+/* A heterodyne grid and its set-up arrays (struct Het, QuickCBC.h:27-50),
+ * flat and row-major:
+ *   fgrid, freq  M          bin indices and frequencies of the grid
+ *   amp, phase   nifo x M   fullphaseamp of the reference point on the grid
+ *   IP           NR x (J+1) x (J+1)
+ *   SL, lc, ls   NR x nifo x (J+1)
+ *   rc, rs, aa   nifo x NR */
+typedef struct {
+  int M, J, NR, MN, MM, nifo;
+  const int *fgrid;
+  const double *freq;
+  double *amp, *phase, *IP, *SL, *lc, *ls, *rc, *rs, *aa;
+  double logLb;
+} qo_het;
+
+/* heterodyne, QuickCBC.c:5112-5299, on a given grid (het_space's output:
+ * fgrid, M = NR J + 1, MN, MM).  D: nifo rows of N, SN: nifo rows of N/2,
+ * freq: the N/2 analysis frequencies (freq[0] = 1/Tobs).  Returns 0, the
+ * IMRPhenomD status, or QO_EFAULT for a grid the reference would read
+ * outside its arrays with. */
+int qo_heterodyne(int nifo, const int *labels, double gmst, double fref, double **D, double **SN,
+                  const double *params, const double *freq, int N, double Tobs, qo_het *h) {
+  const int M = h->M, J = h->J, NR = h->NR, MN = h->MN, MM = h->MM, n2 = N / 2;
+  double **hb, **rb, **ampb, **phaseb, **RC, **RS, **AA;
+  double logL = 0.0;
+  int i, id, j, k, n, st = 0;
+  if (J < 1 || J > 8 || M != NR * J + 1 || NR < 1 || MN < 1 || MM > n2 || MN >= MM || h->fgrid[0] < 1 ||
+      h->fgrid[M - 1] >= n2)
+    return QO_EFAULT;
+  for (j = 1; j < M; j++)
+    if (h->fgrid[j] <= h->fgrid[j - 1]) return QO_EFAULT;
+  hb = malloc(sizeof(double *) * nifo);
+  rb = malloc(sizeof(double *) * nifo);
+  ampb = malloc(sizeof(double *) * nifo);
+  phaseb = malloc(sizeof(double *) * nifo);
+  RC = malloc(sizeof(double *) * nifo);
+  RS = malloc(sizeof(double *) * nifo);
+  AA = malloc(sizeof(double *) * nifo);
+  for (id = 0; id < nifo; id++) {
+    hb[id] = malloc(sizeof(double) * N);
+    rb[id] = malloc(sizeof(double) * N);
+    ampb[id] = malloc(sizeof(double) * n2);
+    phaseb[id] = malloc(sizeof(double) * n2);
+    RC[id] = calloc(n2, sizeof(double));
+    RS[id] = calloc(n2, sizeof(double));
+    AA[id] = calloc(n2, sizeof(double));
+  }
+  /* full frequency template, amplitude and phase (:5150-5152) */
+  st = qo_fulltemplates(nifo, labels, gmst, fref, hb, freq, params, N);
+  if (st == 0) st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ampb, phaseb, freq, params, N);
+  if (st == 0) {
+    /* downsampled reference amplitude and phase (:5154-5163) */
+    for (id = 0; id < nifo; id++)
+      for (j = 0; j < M; j++) {
+        n = h->fgrid[j];
+        h->amp[id * M + j] = ampb[id][n];
+        h->phase[id * M + j] = phaseb[id][n];
+      }
+    /* residual and the reference likelihood over [MN, MM) (:5165-5180) */
+    for (id = 0; id < nifo; id++)
+      for (j = 0; j < N; j++) rb[id][j] = D[id][j] - hb[id][j];
+    for (id = 0; id < nifo; id++) {
+      double HH = qo_fourier_nwip(hb[id], hb[id], SN[id], MN, MM, N);
+      double HR = qo_fourier_nwip(rb[id], hb[id], SN[id], MN, MM, N);
+      logL += HR + 0.5 * HH;
+    }
+    h->logLb = logL;
+    /* per-bin residual products and template power (:5186-5200); AA[0] is
+     * never set by the reference and never read with a grid starting at 1 */
+    for (id = 0; id < nifo; id++)
+      for (j = 1; j < n2; j++) {
+        double cp = cos(phaseb[id][j]), sp = sin(phaseb[id][j]);
+        double x = ampb[id][j] / SN[id][j];
+        RC[id][j] = 2.0 * (rb[id][j] * cp + rb[id][N - j] * sp) * x;
+        RS[id][j] = 2.0 * (-rb[id][j] * sp + rb[id][N - j] * cp) * x;
+        AA[id][j] = ampb[id][j] * ampb[id][j] / SN[id][j];
+      }
+    /* values at the region starts, for the overcount correction (:5202-5212) */
+    for (id = 0; id < nifo; id++)
+      for (i = 0; i < NR; ++i) {
+        k = h->fgrid[i * J];
+        h->rc[id * NR + i] = RC[id][k];
+        h->rs[id * NR + i] = RS[id][k];
+        h->aa[id * NR + i] = AA[id][k];
+      }
+    for (i = 0; i < NR * nifo * (J + 1); i++) h->SL[i] = h->lc[i] = h->ls[i] = 0.0;
+    /* per region: Legendre moments and the inverse of the downsampled
+     * Legendre matrix (:5232-5290) */
+    for (i = 0; i < NR; ++i) {
+      if ((i + 1) * J < M) {
+        const int U = h->fgrid[(i + 1) * J] - h->fgrid[i * J], W = U + 1;
+        double *P = malloc(sizeof(double) * (J + 1) * W), PP[81];
+        qo_legendre(J, U, P);
+        for (j = 0; j <= J; ++j)
+          for (k = 0; k <= J; ++k) PP[j * (J + 1) + k] = P[j * W + h->fgrid[i * J + k] - h->fgrid[i * J]];
+        qo_inverse(PP, h->IP + (size_t)i * (J + 1) * (J + 1), J + 1);
+        for (id = 0; id < nifo; id++) {
+          double *SLr = h->SL + ((size_t)i * nifo + id) * (J + 1);
+          double *lcr = h->lc + ((size_t)i * nifo + id) * (J + 1);
+          double *lsr = h->ls + ((size_t)i * nifo + id) * (J + 1);
+          for (n = 0; n <= U; n++) {
+            k = h->fgrid[i * J] + n;
+            for (j = 0; j <= J; j++) {
+              SLr[j] += AA[id][k] * P[j * W + n];
+              lcr[j] += RC[id][k] * P[j * W + n];
+              lsr[j] += RS[id][k] * P[j * W + n];
+            }
+          }
+        }
+        free(P);
+      }
+    }
+  }
+  for (id = 0; id < nifo; id++) {
+    free(hb[id]);
+    free(rb[id]);
+    free(ampb[id]);
+    free(phaseb[id]);
+    free(RC[id]);
+    free(RS[id]);
+    free(AA[id]);
+  }
+  free(hb);
+  free(rb);
+  free(ampb);
+  free(phaseb);
+  free(RC);
+  free(RS);
+  free(AA);
+  return st;
+}
+
+/* log_likelihood_het, QuickCBC.c:5303-5444 (lhold = 0, ConstCBC.h:63).
+ * Returns logL; *status gets the IMRPhenomD status (logL = 0 on error). */
+double qo_loglike_het(int nifo, const int *labels, double gmst, double fref, double Tobs, const qo_het *h,
+                      const double *params, int *status) {
+  const int M = h->M, J = h->J, NR = h->NR;
+  double *amp = malloc(sizeof(double) * nifo * M), *phase = malloc(sizeof(double) * nifo * M);
+  double *cc = malloc(sizeof(double) * nifo * M), *hc = malloc(sizeof(double) * nifo * M);
+  double *hs = malloc(sizeof(double) * nifo * M);
+  double *ar[3], *pr[3];
+  double logL = 0.0, uu[9], vv[9];
+  int id, i, j, k, st;
+  for (id = 0; id < nifo; id++) {
+    ar[id] = amp + (size_t)id * M;
+    pr[id] = phase + (size_t)id * M;
+  }
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, h->freq, params, 2 * M);
+  if (status) *status = st;
+  if (st == 0) {
+    for (id = 0; id < nifo; id++)
+      for (j = 0; j < M; j++) {
+        const double ha = h->amp[id * M + j], hp = h->phase[id * M + j];
+        double c = 2.0 * (ha - ar[id][j] * cos(hp - pr[id][j]));
+        double s = 2.0 * ar[id][j] * sin(hp - pr[id][j]);
+        hc[id * M + j] = c / ha;
+        hs[id * M + j] = s / ha;
+        cc[id * M + j] = hc[id * M + j] * hc[id * M + j] + hs[id * M + j] * hs[id * M + j];
+      }
+    logL = h->logLb;
+    for (id = 0; id < nifo; id++) {
+      double HH = 0.0, HR = 0.0, x = 0.0, y = 0.0;
+      for (i = 0; i < NR; i++) {
+        const double *IPi = h->IP + (size_t)i * (J + 1) * (J + 1);
+        const double *SLr = h->SL + ((size_t)i * nifo + id) * (J + 1);
+        const double *lcr = h->lc + ((size_t)i * nifo + id) * (J + 1);
+        const double *lsr = h->ls + ((size_t)i * nifo + id) * (J + 1);
+        /* Legendre coefficients of the slow term, then its moment */
+        for (j = 0; j <= J; j++) {
+          vv[j] = cc[id * M + i * J + j];
+          uu[j] = 0.0;
+        }
+        for (j = 0; j <= J; j++)
+          for (k = 0; k <= J; k++) uu[j] += IPi[j * (J + 1) + k] * vv[k];
+        for (j = 0; j <= J; j++) HH += uu[j] * SLr[j];
+        if (i < NR - 2) x += h->aa[id * NR + i + 1] * vv[J]; /* correction for overcount */
+        for (j = 0; j <= J; j++) {
+          vv[j] = hc[id * M + i * J + j];
+          uu[j] = 0.0;
+        }
+        for (j = 0; j <= J; j++)
+          for (k = 0; k <= J; k++) uu[j] += IPi[j * (J + 1) + k] * vv[k];
+        for (j = 0; j <= J; j++) HR += uu[j] * lcr[j];
+        if (i < NR - 2) y += h->rc[id * NR + i + 1] * vv[J];
+        for (j = 0; j <= J; j++) {
+          vv[j] = hs[id * M + i * J + j];
+          uu[j] = 0.0;
+        }
+        for (j = 0; j <= J; j++)
+          for (k = 0; k <= J; k++) uu[j] += IPi[j * (J + 1) + k] * vv[k];
+        for (j = 0; j <= J; j++) HR += uu[j] * lsr[j];
+        if (i < NR - 2) y += h->rs[id * NR + i + 1] * vv[J];
+      }
+      HH -= x;
+      HR -= y;
+      logL -= (HR + 0.5 * HH);
+    }
+  }
+  free(amp);
+  free(phase);
+  free(cc);
+  free(hc);
+  free(hs);
+  return logL;
+}
+
+/* Flat-array conveniences for the ctypes wrapper. */
+int qo_fullphaseamp_flat(int nifo, const int *labels, double gmst, double fref, double Tobs,
+                         const double *freq, int n, const double *params, double *amp, double *phase) {
+  double *ar[3], *pr[3];
+  int id;
+  for (id = 0; id < nifo; id++) {
+    ar[id] = amp + (size_t)id * n;
+    pr[id] = phase + (size_t)id * n;
+  }
+  qo_spline_init();
+  return qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, freq, params, 2 * n);
+}
+
+int qo_heterodyne_flat(int nifo, const int *labels, double gmst, double fref, const double *D, const double *SN,
+                       const double *params, int N, double Tobs, const int *fgrid, int M, int J, int MN, int MM,
+                       double *amp, double *phase, double *IP, double *SL, double *lc, double *ls, double *rc,
+                       double *rs, double *aa, double *logLb) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs);
+  qo_het h;
+  int i, st;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+  h.M = M;
+  h.J = J;
+  h.NR = J > 0 ? (M - 1) / J : 0;
+  h.MN = MN;
+  h.MM = MM;
+  h.nifo = nifo;
+  h.fgrid = fgrid;
+  h.freq = NULL;
+  h.amp = amp;
+  h.phase = phase;
+  h.IP = IP;
+  h.SL = SL;
+  h.lc = lc;
+  h.ls = ls;
+  h.rc = rc;
+  h.rs = rs;
+  h.aa = aa;
+  h.logLb = 0.0;
+  st = qo_heterodyne(nifo, labels, gmst, fref, Dr, Sr, params, freq, N, Tobs, &h);
+  *logLb = h.logLb;
+  free(freq);
+  return st;
+}
+
+void qo_loglike_het_batch_flat(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
+                               const double *freq, const double *amp, const double *phase, const double *IP,
+                               const double *SL, const double *lc, const double *ls, const double *rc,
+                               const double *rs, const double *aa, double logLb, const double *params, int B,
+                               double *logL, int *status, int nthreads) {
+  qo_het h;
+  int b;
+  (void)nthreads;
+  qo_spline_init();
+  h.M = M;
+  h.J = J;
+  h.NR = (M - 1) / J;
+  h.MN = h.MM = 0;
+  h.nifo = nifo;
+  h.fgrid = NULL;
+  h.freq = freq;
+  h.amp = (double *)amp;
+  h.phase = (double *)phase;
+  h.IP = (double *)IP;
+  h.SL = (double *)SL;
+  h.lc = (double *)lc;
+  h.ls = (double *)ls;
+  h.rc = (double *)rc;
+  h.rs = (double *)rs;
+  h.aa = (double *)aa;
+  h.logLb = logLb;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (b = 0; b < B; b++) {
+    int st = 0;
+    logL[b] = qo_loglike_het(nifo, labels, gmst, fref, Tobs, &h, params + 11 * (size_t)b, &st);
+    if (status) status[b] = st;
+  }
+}

@@ -109,6 +109,25 @@ double qo_pow_cr(double x, double y);
 double qo_log_cr(double x);
 int qo_num_threads(void);
 
+/* Heterodyned likelihood (QuickCBC.c:3780-3869, 5055-5444; Utilities.c:1862).
+ * Grids and set-up arrays are flat and row-major, layouts in qcbc_oracle.c. */
+void qo_legendre(int J, int U, double *P);
+void qo_inverse(const double *M, double *IM, int d);
+int qo_fullphaseamp_flat(int nifo, const int *labels, double gmst, double fref, double Tobs,
+                         const double *freq, int n, const double *params, double *amp, double *phase);
+int qo_heterodyne_flat(int nifo, const int *labels, double gmst, double fref, const double *D,
+                       const double *SN, const double *params, int N, double Tobs, const int *fgrid,
+                       int M, int J, int MN, int MM, double *amp, double *phase, double *IP,
+                       double *SL, double *lc, double *ls, double *rc, double *rs, double *aa,
+                       double *logLb);
+void qo_loglike_het_batch_flat(int nifo, const int *labels, double gmst, double fref, double Tobs,
+                               int M, int J, const double *freq, const double *amp,
+                               const double *phase, const double *IP, const double *SL,
+                               const double *lc, const double *ls, const double *rc,
+                               const double *rs, const double *aa, double logLb,
+                               const double *params, int B, double *logL, int *status,
+                               int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -173,3 +173,24 @@ def make_data(an: Analysis, template_fn, inj, noise_seed=2024, with_noise=True):
     if with_noise:
         h = h + noise(an, noise_seed)
     return np.ascontiguousarray(h)
+
+
+def het_grid(MN, MM, Tobs, J=2, frac=1.0 / 64.0, max_hz=100.0):
+    """A heterodyne grid of the shape het_space builds (QuickCBC.c:4784-4908):
+    bin indices from MN in regions of J equal steps, M = NR J + 1 of them,
+    the last one below MM.  het_space sizes the steps from the parameter
+    space's Fisher directions; here a region starting at bin k steps by
+    frac * k bins (at least 1, at most max_hz * Tobs, het_space's own cap)."""
+    cap = max(1, int(max_hz * Tobs))
+    g = [int(MN)]
+    k = int(MN)
+    while True:
+        s = max(1, min(cap, int(frac * k)))
+        if k + J * s > MM - 1:
+            s = (MM - 1 - k) // J
+            if s >= 1:
+                g += [k + s * (i + 1) for i in range(J)]
+            break
+        g += [k + s * (i + 1) for i in range(J)]
+        k += J * s
+    return np.asarray(g, dtype=np.int32)

@@ -31,3 +31,34 @@ def points(which, n, seed=5):
     if which == 0:
         return W.prior_draws0(n, seed=seed)
     return W.ensemble_around(inj, n, seed=seed)
+
+
+def het_band(an, ref, SN):
+    """[MN, MM] of limits() (QuickCBC.c:4275-4385): the bins where the
+    reference point's cumulative network SNR^2 passes 0.001 and where it comes
+    within 0.001 of its total, from max(fmin, fbegin) up."""
+    st, A, _ = O.fullphaseamp(an.labels, an.gmst, an.fref, an.Tobs, an.freq(), ref)
+    assert st == 0
+    jmin = int(an.Tobs * max(an.fmin, O.fbegin(ref)))
+    x = (4.0 * A ** 2 / SN).sum(0)
+    x[:jmin] = 0.0
+    c = np.cumsum(x)
+    j0 = int(np.argmax(c > 0.001))
+    j1 = int(np.argmax(c[-1] - c < 0.001))
+    return max(j0, jmin), j1
+
+
+@functools.lru_cache(maxsize=None)
+def het(which, J=2, frac=1.0 / 64.0):
+    """The oracle's heterodyne at the injection, on het_grid over het_band."""
+    an, inj, D, SN = analysis(which)
+    MN, MM = het_band(an, inj, SN)
+    g = W.het_grid(MN, MM, an.Tobs, J=J, frac=frac)
+    st, h = O.heterodyne(an.labels, an.gmst, an.fref, D, SN, an.N, an.Tobs, inj, g, J, MN, MM)
+    assert st == 0
+    return h
+
+
+def oracle_loglike_het(which, P, J=2, frac=1.0 / 64.0):
+    an, inj, D, SN = analysis(which)
+    return O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, het(which, J, frac), P)
