@@ -7,7 +7,8 @@
  * reference objects.  Part 2 is the batch API the drop-ins are built on: one
  * analysis (detector set, data, PSD) is made resident on the GPU once and
  * then evaluated for batches of parameter points.  Parts 3-5 add the PT
- * ensemble step, the multi-analysis persistent batch and the serving loop.
+ * ensemble step, the multi-analysis persistent batch and the serving loop;
+ * Part 6 the heterodyned likelihood.
  *
  * Every computation runs on the GPU; if no GPU (or no gfx950 code object) is
  * usable every entry point reports QCBC_ENODEV (the drop-ins print the error
@@ -52,6 +53,32 @@ typedef struct tagAmpPhaseFDWaveform {
   double *phase;
   size_t length;
 } AmpPhaseFDWaveform;
+/* QuickCBC.h:27-50: a heterodyne.  Arrays are the reference's
+ * double_matrix / double_tensor shapes (Utilities.c:2981-3025: n+1 rows of
+ * m+1, each malloc'd), so its freehet releases them. */
+struct Het {
+  int M;
+  int MM;
+  int MN;
+  int ell;
+  int J;
+  int NR;
+  double hh;
+  double hr;
+  double logLb;
+  double ***IP;
+  double ***SL;
+  double **amp;
+  double **phase;
+  double **rc;
+  double **rs;
+  double **aa;
+  double ***lc;
+  double ***ls;
+  double *pref;
+  int *fgrid;
+  RealVector *freq;
+};
 #endif
 
 /* ------------------------------------------------------------------ *
@@ -89,6 +116,27 @@ void F_ant(double psi, double alpha, double sindelta, double GMST, double *Fplus
            int id);
 void Times(double alpha, double sindelta, double GMST, double *dtimes);
 double fbegin(double *param);
+
+/* Replaces fullphaseamp, QuickCBC.c:3780-3869 (prototype QuickCBC.h:98): per
+ * detector the projected amplitude (tapered by exp(10 (f - fs)/fs) below
+ * fbegin) and phase 2 pi f (ts - td) + lambda - Psi on freq->data[0..N/2).
+ * amp, phase: net->Nifo rows of at least N/2. */
+void fullphaseamp(struct Net *net, double **amp, double **phase, RealVector *freq, double *params, int N);
+
+/* Replaces heterodyne, QuickCBC.c:5112-5299 (prototype QuickCBC.h:106): the
+ * set-up of the heterodyned likelihood at params on the grid het_space chose
+ * (het->M, MN, MM, J, NR, fgrid, freq).  Allocates and fills het's IP, SL,
+ * amp, phase, rc, rs, aa, lc, ls and pref in the reference's shapes, sets
+ * logLb and prints it as the reference does.  The GPU copy stays cached for
+ * log_likelihood_het. */
+void heterodyne(struct Net *net, struct Het *het, double **D, double *params, RealVector *freq, double **SN,
+                int N, double Tobs);
+
+/* Replaces log_likelihood_het, QuickCBC.c:5303-5444 (prototype QuickCBC.h:107).
+ * het may come from heterodyne() above or from the reference's own: it is
+ * copied to the GPU on first use and cached by its array pointers and pref;
+ * call qcbc_invalidate() after changing its arrays in place. */
+double log_likelihood_het(struct Net *net, struct Het *het, double *params, double Tobs);
 
 /* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
 AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
@@ -285,6 +333,48 @@ int qcbc_server_wait(qcbc_server *s, int64_t request_id, double *logL, double *r
 int qcbc_server_stats(qcbc_server *s, size_t *bytes_in_use, int *n_analyses, long long *evals,
                       long long *batches);
 int qcbc_server_destroy(qcbc_server *s);
+
+/* ------------------------------------------------------------------ *
+ * Part 6: heterodyned likelihood (QuickCBC.c:5112-5444)               *
+ * ------------------------------------------------------------------ */
+/* A heterodyne lives on the GPU and belongs to the analysis it was made
+ * for (destroy it before the analysis).  Its grid is M = NR J + 1 bin
+ * indices fgrid[] (strictly increasing, inside (0, N/2)), frequencies
+ * fgrid / Tobs, in NR regions of J steps; QuickCBC's het_space chooses it.
+ * Set-up arrays, row-major as flat copies of struct Het:
+ *   amp, phase   n_det x M
+ *   IP           NR x (J+1) x (J+1)
+ *   SL, lc, ls   NR x n_det x (J+1)
+ *   rc, rs, aa   n_det x NR */
+typedef struct qcbc_het qcbc_het;
+/* heterodyne() at params (11) on a bound analysis; logLb over bins [MN, MM).
+ * J <= 8.  QCBC_EDOM if the reference point is outside the model. */
+int qcbc_het_create(qcbc_ctx *ctx, const double *params, const int *fgrid, int M, int J, int MN,
+                    int MM, qcbc_het **out);
+/* A heterodyne from set-up arrays made elsewhere (the reference's struct
+ * Het, flattened); freq has M entries.  The analysis need not be bound. */
+int qcbc_het_import(qcbc_ctx *ctx, int M, int J, const double *freq, const double *amp,
+                    const double *phase, const double *IP, const double *SL, const double *lc,
+                    const double *ls, const double *rc, const double *rs, const double *aa,
+                    double logLb, qcbc_het **out);
+int qcbc_het_info(qcbc_het *het, int *M, int *J, int *NR, int *n_det, double *logLb);
+/* Copies of the set-up arrays (any pointer may be NULL). */
+int qcbc_het_export(qcbc_het *het, double *freq, double *amp, double *phase, double *IP, double *SL,
+                    double *lc, double *ls, double *rc, double *rs, double *aa);
+/* The evaluation's per-sample weights (3 x n_det x M: wH, wC, wS): the
+ * linear form log_likelihood_het applies to the samples' cc, hc, hs. */
+int qcbc_het_weights(qcbc_het *het, double *w);
+/* log_likelihood_het for B points (B x 11, host memory).  status[b] is 0 or
+ * QCBC_EDOM (then logL[b] = 0). */
+int qcbc_het_loglike(qcbc_het *het, const double *params, int B, double *logL, int *status);
+/* The same on device memory, asynchronous on the heterodyne's stream; wait
+ * with qcbc_het_sync.  d_status may be NULL. */
+int qcbc_het_loglike_device(qcbc_het *het, const double *d_params, int B, double *d_logL,
+                            int *d_status);
+int qcbc_het_sync(qcbc_het *het);
+/* HIP-event timing of the evaluation kernel, as qcbc_ctx_timing. */
+int qcbc_het_timing(qcbc_het *het, int enable, double *kernel_ms, long long *launches);
+int qcbc_het_destroy(qcbc_het *het);
 
 #ifdef __cplusplus
 }

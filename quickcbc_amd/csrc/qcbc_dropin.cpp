@@ -8,6 +8,7 @@
 // copied to the GPU once, not on every call.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
+#include "qcbc_het_internal.h"
 #include "qcbc_internal.h"
 
 using namespace qcbc;
@@ -291,6 +293,255 @@ double fbegin(double* param) {
   double out = 0.0;
   geometry(2, param, 11, &out, 1, "fbegin");
   return out;
+}
+
+void fullphaseamp(struct Net* net, double** amp, double** phase, RealVector* freq, double* params, int N) {
+  const int n = N / 2;
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !amp || !phase || !freq || !freq->data || !params || n < 1 ||
+      freq->length < (size_t)n)
+    fail("fullphaseamp", QCBC_EINVAL);
+  for (int d = 0; d < net->Nifo; d++)
+    if (!amp[d] || !phase[d]) fail("fullphaseamp", QCBC_EINVAL);
+  // the analysis's sky and detector constants; Tobs is net->Tobs (QuickCBC.c:3799)
+  qcbc_desc desc = desc_from(net, 2 * n < 4 ? 4 : 2 * n, net->Tobs);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc, nullptr, nullptr, &st);
+  if (!ctx) fail("fullphaseamp", st);
+  DropState& s = t_drop;
+  const int nd = net->Nifo;
+  if ((st = drop_buf(11 + (size_t)n * (1 + 2 * nd)))) fail("fullphaseamp", st);
+  double* d_p = s.d_buf;
+  double* d_f = d_p + 11;
+  double* d_a = d_f + n;
+  double* d_ph = d_a + (size_t)nd * n;
+  // IMRPhenomDGenerateh22FDAmpPhase's f_min is the list's first frequency
+  AnalysisConst an = ctx->an;
+  an.fmin0 = freq->data[0];
+  std::vector<double> a((size_t)nd * n), p((size_t)nd * n);
+  int pst = 0;
+  if (hipMemcpyAsync(d_p, params, sizeof(double) * 11, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+      hipMemcpyAsync(d_f, freq->data, sizeof(double) * n, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+      launch_setup(d_p, 1, an, s.d_pt, s.d_st, s.stream) != hipSuccess ||
+      launch_phaseamp(s.d_pt, d_f, n, nd, net->Tobs, std::sqrt(net->Tobs), d_a, d_ph, s.stream) != hipSuccess ||
+      hipMemcpyAsync(&pst, s.d_st, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(a.data(), d_a, sizeof(double) * a.size(), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(p.data(), d_ph, sizeof(double) * p.size(), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipStreamSynchronize(s.stream) != hipSuccess)
+    fail("fullphaseamp", QCBC_EHIP);
+  if (domain_error("fullphaseamp", pst)) {
+    std::fill(a.begin(), a.end(), 0.0);
+    std::fill(p.begin(), p.end(), 0.0);
+  }
+  for (int d = 0; d < nd; d++) {
+    std::memcpy(amp[d], a.data() + (size_t)d * n, sizeof(double) * n);
+    std::memcpy(phase[d], p.data() + (size_t)d * n, sizeof(double) * n);
+  }
+}
+
+}  // extern "C"
+
+namespace {
+
+// The heterodynes a thread has on the GPU, keyed by the struct, its array
+// pointers and the reference point it was made at.
+struct HetEntry {
+  const struct Het* het = nullptr;
+  const void* arrays[4] = {};
+  int M = 0, J = 0, NR = 0;
+  double pref[11] = {};
+  bool has_pref = false;
+  unsigned long long gen = 0, last_use = 0;
+  qcbc_het* h = nullptr;
+};
+thread_local std::vector<HetEntry> t_hets;
+
+HetEntry key_of(const struct Het* het) {
+  HetEntry e;
+  e.het = het;
+  e.arrays[0] = het->IP;
+  e.arrays[1] = het->SL;
+  e.arrays[2] = het->amp;
+  e.arrays[3] = het->freq ? (const void*)het->freq->data : nullptr;
+  e.M = het->M;
+  e.J = het->J;
+  e.NR = het->NR;
+  e.has_pref = het->pref != nullptr;
+  if (e.has_pref) std::memcpy(e.pref, het->pref, sizeof(e.pref));
+  e.gen = g_generation.load();
+  return e;
+}
+
+bool same_key(const HetEntry& a, const HetEntry& b) {
+  if (a.het != b.het || a.M != b.M || a.J != b.J || a.NR != b.NR || a.has_pref != b.has_pref || a.gen != b.gen)
+    return false;
+  for (int i = 0; i < 4; i++)
+    if (a.arrays[i] != b.arrays[i]) return false;
+  return !a.has_pref || std::memcmp(a.pref, b.pref, sizeof(a.pref)) == 0;
+}
+
+void het_cache_put(HetEntry e) {
+  for (auto it = t_hets.begin(); it != t_hets.end();)
+    if (it->het == e.het) {  // a newer heterodyne of the same struct
+      qcbc_het_destroy(it->h);
+      it = t_hets.erase(it);
+    } else {
+      ++it;
+    }
+  if (t_hets.size() >= 4) {
+    size_t lru = 0;
+    for (size_t i = 1; i < t_hets.size(); i++)
+      if (t_hets[i].last_use < t_hets[lru].last_use) lru = i;
+    qcbc_het_destroy(t_hets[lru].h);
+    t_hets.erase(t_hets.begin() + lru);
+  }
+  e.last_use = ++t_drop.tick;
+  t_hets.push_back(e);
+}
+
+double*** tensor(int n, int m, int l) {  // double_tensor's shape (Utilities.c:3001-3016)
+  double*** t = (double***)std::malloc(sizeof(double**) * (n + 1));
+  if (!t) fail("heterodyne", QCBC_ENOMEM);
+  for (int i = 0; i < n + 1; i++) {
+    t[i] = (double**)std::malloc(sizeof(double*) * (m + 1));
+    if (!t[i]) fail("heterodyne", QCBC_ENOMEM);
+    for (int j = 0; j < m + 1; j++) {
+      t[i][j] = (double*)std::malloc(sizeof(double) * (l + 1));
+      if (!t[i][j]) fail("heterodyne", QCBC_ENOMEM);
+    }
+  }
+  return t;
+}
+
+double** matrix(int n, int m) {  // double_matrix's shape (Utilities.c:2981-2992)
+  double** a = (double**)std::malloc(sizeof(double*) * (n + 1));
+  if (!a) fail("heterodyne", QCBC_ENOMEM);
+  for (int i = 0; i < n + 1; i++) {
+    a[i] = (double*)std::malloc(sizeof(double) * (m + 1));
+    if (!a[i]) fail("heterodyne", QCBC_ENOMEM);
+  }
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+void heterodyne(struct Net* net, struct Het* het, double** D, double* params, RealVector* freq, double** SN, int N,
+                double Tobs) {
+  (void)freq;  // the analysis grid f_i = i/Tobs, freq[0] = 1/Tobs (QuickCBC.c:418-419)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !D || !SN || !params || !het->fgrid)
+    fail("heterodyne", QCBC_EINVAL);
+  const int M = het->M, J = het->J, NR = het->NR, nd = net->Nifo;
+  if (J < 1 || M != NR * J + 1) fail("heterodyne", QCBC_EINVAL);
+  qcbc_desc desc = desc_from(net, N, Tobs);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc, D, SN, &st);
+  if (!ctx) fail("heterodyne", st);
+  qcbc_het* h = nullptr;
+  st = qcbc_het_create(ctx, params, het->fgrid, M, J, het->MN, het->MM, &h);
+  if (st == QCBC_EDOM) {
+    domain_error("heterodyne", st);
+    return;
+  }
+  if (st) fail("heterodyne", st);
+  const size_t J1 = J + 1;
+  std::vector<double> amp((size_t)nd * M), phase((size_t)nd * M), IP(NR * J1 * J1), SL(NR * nd * J1),
+      lc(NR * nd * J1), ls(NR * nd * J1), rc((size_t)nd * NR), rs((size_t)nd * NR), aa((size_t)nd * NR);
+  double logLb = 0.0;
+  if ((st = qcbc_het_export(h, nullptr, amp.data(), phase.data(), IP.data(), SL.data(), lc.data(), ls.data(),
+                            rc.data(), rs.data(), aa.data())) ||
+      (st = qcbc_het_info(h, nullptr, nullptr, nullptr, nullptr, &logLb)))
+    fail("heterodyne", st);
+  // the struct's arrays, allocated as QuickCBC.c:5128-5138 does
+  het->IP = tensor(NR, J + 1, J + 1);
+  het->SL = tensor(NR, nd, J + 1);
+  het->amp = matrix(nd, M);
+  het->phase = matrix(nd, M);
+  het->rc = matrix(nd, NR);
+  het->rs = matrix(nd, NR);
+  het->aa = matrix(nd, NR);
+  het->lc = tensor(NR, nd, J + 1);
+  het->ls = tensor(NR, nd, J + 1);
+  het->pref = (double*)std::malloc(sizeof(double) * (11 + 1));
+  if (!het->pref) fail("heterodyne", QCBC_ENOMEM);
+  std::memcpy(het->pref, params, sizeof(double) * 11);
+  for (int d = 0; d < nd; d++) {
+    std::memcpy(het->amp[d], amp.data() + (size_t)d * M, sizeof(double) * M);
+    std::memcpy(het->phase[d], phase.data() + (size_t)d * M, sizeof(double) * M);
+    std::memcpy(het->rc[d], rc.data() + (size_t)d * NR, sizeof(double) * NR);
+    std::memcpy(het->rs[d], rs.data() + (size_t)d * NR, sizeof(double) * NR);
+    std::memcpy(het->aa[d], aa.data() + (size_t)d * NR, sizeof(double) * NR);
+  }
+  for (int i = 0; i < NR; i++) {
+    for (size_t j = 0; j < J1; j++) std::memcpy(het->IP[i][j], IP.data() + (i * J1 + j) * J1, sizeof(double) * J1);
+    for (int d = 0; d < nd; d++) {
+      const size_t o = ((size_t)i * nd + d) * J1;
+      std::memcpy(het->SL[i][d], SL.data() + o, sizeof(double) * J1);
+      std::memcpy(het->lc[i][d], lc.data() + o, sizeof(double) * J1);
+      std::memcpy(het->ls[i][d], ls.data() + o, sizeof(double) * J1);
+    }
+  }
+  het->logLb = logLb;
+  std::printf("ref logL = %e\n", logLb);  // QuickCBC.c:5180
+  HetEntry e = key_of(het);
+  e.h = h;
+  het_cache_put(e);
+}
+
+double log_likelihood_het(struct Net* net, struct Het* het, double* params, double Tobs) {
+  (void)Tobs;  // fullphaseamp takes Tobs from net (QuickCBC.c:3799)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !params || !het->IP || !het->SL || !het->lc || !het->ls ||
+      !het->amp || !het->phase || !het->rc || !het->rs || !het->aa || !het->freq || !het->freq->data)
+    fail("log_likelihood_het", QCBC_EINVAL);
+  const int M = het->M, J = het->J, NR = het->NR, nd = net->Nifo;
+  if (J < 1 || NR < 1 || M < NR * J + 1 || het->freq->length < (size_t)M) fail("log_likelihood_het", QCBC_EINVAL);
+  HetEntry key = key_of(het);
+  qcbc_het* h = nullptr;
+  for (auto& e : t_hets)
+    if (same_key(e, key)) {
+      e.last_use = ++t_drop.tick;
+      h = e.h;
+      break;
+    }
+  if (!h) {
+    // a heterodyne made by the reference's own set-up: flatten and import it
+    // (samples beyond NR J + 1 are never read, QuickCBC.c:5358-5417)
+    const int Mu = NR * J + 1;
+    const size_t J1 = J + 1;
+    qcbc_desc desc = desc_from(net, 2 * Mu < 4 ? 4 : 2 * Mu, net->Tobs);
+    int st;
+    qcbc_ctx* ctx = get_ctx(desc, nullptr, nullptr, &st);
+    if (!ctx) fail("log_likelihood_het", st);
+    std::vector<double> amp((size_t)nd * Mu), phase((size_t)nd * Mu), IP(NR * J1 * J1), SL(NR * nd * J1),
+        lc(NR * nd * J1), ls(NR * nd * J1), rc((size_t)nd * NR), rs((size_t)nd * NR), aa((size_t)nd * NR);
+    for (int d = 0; d < nd; d++) {
+      std::memcpy(amp.data() + (size_t)d * Mu, het->amp[d], sizeof(double) * Mu);
+      std::memcpy(phase.data() + (size_t)d * Mu, het->phase[d], sizeof(double) * Mu);
+      std::memcpy(rc.data() + (size_t)d * NR, het->rc[d], sizeof(double) * NR);
+      std::memcpy(rs.data() + (size_t)d * NR, het->rs[d], sizeof(double) * NR);
+      std::memcpy(aa.data() + (size_t)d * NR, het->aa[d], sizeof(double) * NR);
+    }
+    for (int i = 0; i < NR; i++) {
+      for (size_t j = 0; j < J1; j++) std::memcpy(IP.data() + (i * J1 + j) * J1, het->IP[i][j], sizeof(double) * J1);
+      for (int d = 0; d < nd; d++) {
+        const size_t o = ((size_t)i * nd + d) * J1;
+        std::memcpy(SL.data() + o, het->SL[i][d], sizeof(double) * J1);
+        std::memcpy(lc.data() + o, het->lc[i][d], sizeof(double) * J1);
+        std::memcpy(ls.data() + o, het->ls[i][d], sizeof(double) * J1);
+      }
+    }
+    st = qcbc_het_import(ctx, Mu, J, het->freq->data, amp.data(), phase.data(), IP.data(), SL.data(), lc.data(),
+                         ls.data(), rc.data(), rs.data(), aa.data(), het->logLb, &h);
+    if (st) fail("log_likelihood_het", st);
+    key.h = h;
+    het_cache_put(key);
+  }
+  double logL = 0.0;
+  int pst = 0;
+  int st = qcbc_het_loglike(h, params, 1, &logL, &pst);
+  if (st) fail("log_likelihood_het", st);
+  if (domain_error("log_likelihood_het", pst)) return 0.0;
+  return logL;
 }
 
 // IMRPhenomD_internals.c:141-181

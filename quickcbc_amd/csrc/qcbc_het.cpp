@@ -1,0 +1,361 @@
+// Heterodyned likelihood (QuickCBC.c:5112-5444) on the GPU: the set-up of a
+// heterodyne (heterodyne(), on a grid the caller chose, as het_space does)
+// from a bound analysis, or imported from set-up arrays computed elsewhere,
+// then batched evaluation of log_likelihood_het.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/qcbc.h"
+#include "qcbc_api_internal.h"
+#include "qcbc_het_internal.h"
+#include "qcbc_internal.h"
+
+using namespace qcbc;
+
+#define HIPCHK(x)                \
+  do {                           \
+    hipError_t e_ = (x);         \
+    if (e_ != hipSuccess) {      \
+      qcbc::t_last_hip = e_;     \
+      return QCBC_EHIP;          \
+    }                            \
+  } while (0)
+
+struct qcbc_het {
+  int device = 0;
+  int M = 0, J = 0, NR = 0, nd = 0;
+  double logLb = 0.0;
+  AnalysisConst an{};  // the analysis's, with freq[0] of the grid as the model's f_min
+  hipStream_t stream = nullptr;
+  double* d_freq = nullptr;   // M
+  double* d_amp = nullptr;    // nd x M
+  double* d_phase = nullptr;  // nd x M
+  double* d_w = nullptr;      // 3 x nd x M
+  double* d_raw = nullptr;    // IP | SL | lc | ls | rc | rs | aa
+  size_t off[7] = {}, len[7] = {};
+  // batch scratch
+  int cap = 0;
+  double* d_params = nullptr;
+  Point* d_pts = nullptr;
+  int* d_status = nullptr;
+  double* d_logL = nullptr;
+  // kernel timing
+  bool timing = false;
+  double kernel_ms = 0.0;
+  long long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::mutex mu;
+};
+
+namespace {
+
+enum { R_IP = 0, R_SL, R_LC, R_LS, R_RC, R_RS, R_AA };
+
+void het_layout(qcbc_het* h) {
+  const size_t J1 = h->J + 1, NR = h->NR, nd = h->nd;
+  const size_t L[7] = {NR * J1 * J1, NR * nd * J1, NR * nd * J1, NR * nd * J1, nd * NR, nd * NR, nd * NR};
+  size_t o = 0;
+  for (int i = 0; i < 7; i++) {
+    h->off[i] = o;
+    h->len[i] = L[i];
+    o += L[i];
+  }
+}
+
+size_t raw_total(const qcbc_het* h) { return h->off[6] + h->len[6]; }
+
+void het_free(qcbc_het* h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& e : h->ev) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  void* bufs[] = {h->d_freq, h->d_amp, h->d_phase, h->d_w, h->d_raw, h->d_params, h->d_pts, h->d_status, h->d_logL};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+bool shape_ok(int M, int J) { return J >= 1 && J <= HET_MAX_J && M >= J + 1 && (M - 1) % J == 0; }
+
+int het_alloc(qcbc_ctx* ctx, int M, int J, qcbc_het** out) {
+  qcbc_het* h = new qcbc_het();
+  h->device = ctx->device;
+  h->M = M;
+  h->J = J;
+  h->NR = (M - 1) / J;
+  h->nd = ctx->desc.n_det;
+  h->an = ctx->an;
+  het_layout(h);
+  const size_t nd = h->nd;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&h->d_freq, sizeof(double) * M) != hipSuccess ||
+      hipMalloc(&h->d_amp, sizeof(double) * nd * M) != hipSuccess ||
+      hipMalloc(&h->d_phase, sizeof(double) * nd * M) != hipSuccess ||
+      hipMalloc(&h->d_w, sizeof(double) * 3 * nd * M) != hipSuccess ||
+      hipMalloc(&h->d_raw, sizeof(double) * raw_total(h)) != hipSuccess) {
+    het_free(h);
+    return QCBC_ENOMEM;
+  }
+  *out = h;
+  return QCBC_OK;
+}
+
+int het_reserve(qcbc_het* h, int B) {
+  if (B <= h->cap) return QCBC_OK;
+  const int nb = std::max(B, 2 * h->cap);
+  hipFree(h->d_params);
+  hipFree(h->d_pts);
+  hipFree(h->d_status);
+  hipFree(h->d_logL);
+  h->d_params = nullptr;
+  h->d_pts = nullptr;
+  h->d_status = nullptr;
+  h->d_logL = nullptr;
+  h->cap = 0;
+  HIPCHK(hipMalloc(&h->d_params, sizeof(double) * 11 * nb));
+  HIPCHK(hipMalloc(&h->d_pts, sizeof(Point) * nb));
+  HIPCHK(hipMalloc(&h->d_status, sizeof(int) * nb));
+  HIPCHK(hipMalloc(&h->d_logL, sizeof(double) * nb));
+  h->cap = nb;
+  return QCBC_OK;
+}
+
+void harvest(qcbc_het* h) {
+  for (auto& e : h->ev) {
+    float ms = 0.f;
+    if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
+      h->kernel_ms += ms;
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  h->ev.clear();
+}
+
+// set-up of B points then the evaluation kernel, on the het's stream (caller holds mu)
+int het_eval(qcbc_het* h, const double* d_params, int B, double* d_logL, int* d_status) {
+  int st = het_reserve(h, B);
+  if (st) return st;
+  HIPCHK(launch_setup(d_params, B, h->an, h->d_pts, d_status, h->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, h->stream));
+  }
+  HIPCHK(launch_het_ll(h->d_pts, B, h->nd, h->d_freq, h->d_amp, h->d_phase, h->d_w, h->M, h->an.sqT, h->logLb, d_logL,
+                       h->stream));
+  if (h->timing) {
+    HIPCHK(hipEventRecord(e1, h->stream));
+    h->ev.emplace_back(e0, e1);
+    h->launches++;
+    if (h->ev.size() >= 512) harvest(h);
+  }
+  return QCBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcbc_het_create(qcbc_ctx* ctx, const double* params, const int* fgrid, int M, int J, int MN, int MM,
+                    qcbc_het** out) {
+  if (!out) return QCBC_EINVAL;
+  *out = nullptr;
+  if (!ctx || !params || !fgrid || !shape_ok(M, J)) return QCBC_EINVAL;
+  const int n2 = ctx->nhalf, nd = ctx->desc.n_det, N = ctx->desc.N;
+  // the bins heterodyne() reads: the grid inside (0, N/2), strictly increasing
+  // (every region spans at least J bins), and the logLb band inside the data
+  if (fgrid[0] < 1 || fgrid[M - 1] >= n2 || MN < 1 || MM > n2 || MN >= MM) return QCBC_EINVAL;
+  for (int m = 1; m < M; m++)
+    if (fgrid[m] <= fgrid[m - 1]) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  if (!ctx->bound) return QCBC_ENOTBOUND;
+  qcbc_het* h = nullptr;
+  int st = het_alloc(ctx, M, J, &h);
+  if (st) return st;
+  // scratch: params, point, status, logLb, h22 amp/phase, per-bin planes, partials, grid
+  const size_t nbins = 5 * (size_t)nd * n2, npart = (size_t)HET_SETUP_BLOCKS * 2 * nd;
+  const size_t ndbl = 11 + 1 + 2 * (size_t)n2 + nbins + npart;
+  double* d_tmp = nullptr;
+  Point* d_pt = nullptr;
+  int* d_i = nullptr;
+  auto cleanup = [&](int code) {
+    if (d_tmp) hipFree(d_tmp);
+    if (d_pt) hipFree(d_pt);
+    if (d_i) hipFree(d_i);
+    if (code != QCBC_OK) het_free(h);
+    return code;
+  };
+  if (hipMalloc(&d_tmp, sizeof(double) * ndbl) != hipSuccess || hipMalloc(&d_pt, sizeof(Point)) != hipSuccess ||
+      hipMalloc(&d_i, sizeof(int) * (1 + (size_t)M)) != hipSuccess)
+    return cleanup(QCBC_ENOMEM);
+  double* d_p = d_tmp;
+  double* d_lb = d_p + 11;
+  double* d_wa = d_lb + 1;
+  double* d_wp = d_wa + n2;
+  double* d_bins = d_wp + n2;
+  double* d_part = d_bins + nbins;
+  int* d_st = d_i;
+  int* d_fg = d_i + 1;
+  hipStream_t s = ctx->stream;
+  double* r = h->d_raw;
+  int pst = 0;
+  double lb = 0.0;
+  if (hipMemcpyAsync(d_p, params, sizeof(double) * 11, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(d_fg, fgrid, sizeof(int) * M, hipMemcpyHostToDevice, s) != hipSuccess ||
+      launch_setup(d_p, 1, ctx->an, d_pt, d_st, s) != hipSuccess ||
+      launch_waveform(d_pt, 1, n2, ctx->desc.Tobs, d_wa, d_wp, s) != hipSuccess ||
+      launch_het_setup(d_pt, d_wa, d_wp, ctx->d_D, ctx->d_SN, nd, N, ctx->desc.Tobs, ctx->an.sqT, d_fg, M, J, MN, MM,
+                       d_bins, d_part, d_lb, h->d_freq, h->d_amp, h->d_phase, r + h->off[R_IP], r + h->off[R_SL],
+                       r + h->off[R_LC], r + h->off[R_LS], r + h->off[R_RC], r + h->off[R_RS], r + h->off[R_AA],
+                       h->d_w, s) != hipSuccess ||
+      hipMemcpyAsync(&pst, d_st, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&lb, d_lb, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return cleanup(QCBC_EHIP);
+  if (pst != QCBC_OK) return cleanup(pst);
+  h->logLb = lb;
+  h->an.fmin0 = ((double)fgrid[0]) / ctx->desc.Tobs;  // the grid's first frequency, as het->freq holds it
+  cleanup(QCBC_OK);
+  *out = h;
+  return QCBC_OK;
+}
+
+int qcbc_het_import(qcbc_ctx* ctx, int M, int J, const double* freq, const double* amp, const double* phase,
+                    const double* IP, const double* SL, const double* lc, const double* ls, const double* rc,
+                    const double* rs, const double* aa, double logLb, qcbc_het** out) {
+  if (!out) return QCBC_EINVAL;
+  *out = nullptr;
+  if (!ctx || !shape_ok(M, J) || !freq || !amp || !phase || !IP || !SL || !lc || !ls || !rc || !rs || !aa)
+    return QCBC_EINVAL;
+  if (!(freq[0] > 0)) return QCBC_EINVAL;
+  hipSetDevice(ctx->device);
+  qcbc_het* h = nullptr;
+  int st = het_alloc(ctx, M, J, &h);
+  if (st) return st;
+  h->logLb = logLb;
+  h->an.fmin0 = freq[0];
+  const size_t nd = h->nd;
+  const double* src[7] = {IP, SL, lc, ls, rc, rs, aa};
+  hipStream_t s = h->stream;
+  bool ok = hipMemcpyAsync(h->d_freq, freq, sizeof(double) * M, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(h->d_amp, amp, sizeof(double) * nd * M, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(h->d_phase, phase, sizeof(double) * nd * M, hipMemcpyHostToDevice, s) == hipSuccess;
+  for (int i = 0; ok && i < 7; i++)
+    ok = hipMemcpyAsync(h->d_raw + h->off[i], src[i], sizeof(double) * h->len[i], hipMemcpyHostToDevice, s) ==
+         hipSuccess;
+  double* r = h->d_raw;
+  ok = ok &&
+       launch_het_weights(r + h->off[R_IP], r + h->off[R_SL], r + h->off[R_LC], r + h->off[R_LS], r + h->off[R_RC],
+                          r + h->off[R_RS], r + h->off[R_AA], M, J, h->nd, h->d_w, s) == hipSuccess &&
+       hipStreamSynchronize(s) == hipSuccess;
+  if (!ok) {
+    het_free(h);
+    return QCBC_EHIP;
+  }
+  *out = h;
+  return QCBC_OK;
+}
+
+int qcbc_het_info(qcbc_het* h, int* M, int* J, int* NR, int* n_det, double* logLb) {
+  if (!h) return QCBC_EINVAL;
+  if (M) *M = h->M;
+  if (J) *J = h->J;
+  if (NR) *NR = h->NR;
+  if (n_det) *n_det = h->nd;
+  if (logLb) *logLb = h->logLb;
+  return QCBC_OK;
+}
+
+int qcbc_het_export(qcbc_het* h, double* freq, double* amp, double* phase, double* IP, double* SL, double* lc,
+                    double* ls, double* rc, double* rs, double* aa) {
+  if (!h) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  const size_t nd = h->nd, M = h->M;
+  hipStream_t s = h->stream;
+  if (freq) HIPCHK(hipMemcpyAsync(freq, h->d_freq, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+  if (amp) HIPCHK(hipMemcpyAsync(amp, h->d_amp, sizeof(double) * nd * M, hipMemcpyDeviceToHost, s));
+  if (phase) HIPCHK(hipMemcpyAsync(phase, h->d_phase, sizeof(double) * nd * M, hipMemcpyDeviceToHost, s));
+  double* dst[7] = {IP, SL, lc, ls, rc, rs, aa};
+  for (int i = 0; i < 7; i++)
+    if (dst[i])
+      HIPCHK(hipMemcpyAsync(dst[i], h->d_raw + h->off[i], sizeof(double) * h->len[i], hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return QCBC_OK;
+}
+
+int qcbc_het_weights(qcbc_het* h, double* w) {
+  if (!h || !w) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  HIPCHK(hipMemcpyAsync(w, h->d_w, sizeof(double) * 3 * h->nd * h->M, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return QCBC_OK;
+}
+
+int qcbc_het_loglike(qcbc_het* h, const double* params, int B, double* logL, int* status) {
+  if (!h || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = het_reserve(h, B);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(h->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, h->stream));
+  st = het_eval(h, h->d_params, B, h->d_logL, h->d_status);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(logL, h->d_logL, sizeof(double) * B, hipMemcpyDeviceToHost, h->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, h->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return QCBC_OK;
+}
+
+int qcbc_het_loglike_device(qcbc_het* h, const double* d_params, int B, double* d_logL, int* d_status) {
+  if (!h || B < 0 || (B > 0 && (!d_params || !d_logL))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = het_reserve(h, B);
+  if (st) return st;
+  return het_eval(h, d_params, B, d_logL, d_status ? d_status : h->d_status);
+}
+
+int qcbc_het_sync(qcbc_het* h) {
+  if (!h) return QCBC_EINVAL;
+  hipSetDevice(h->device);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return QCBC_OK;
+}
+
+int qcbc_het_timing(qcbc_het* h, int enable, double* kernel_ms, long long* launches) {
+  if (!h) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  harvest(h);
+  if (kernel_ms) *kernel_ms = h->kernel_ms;
+  if (launches) *launches = h->launches;
+  if (enable >= 0) {
+    h->timing = enable != 0;
+    h->kernel_ms = 0.0;
+    h->launches = 0;
+  }
+  return QCBC_OK;
+}
+
+int qcbc_het_destroy(qcbc_het* h) {
+  het_free(h);
+  return QCBC_OK;
+}
+
+}  // extern "C"

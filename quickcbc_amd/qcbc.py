@@ -48,6 +48,19 @@ class AmpPhaseFDWaveform(ctypes.Structure):
     _fields_ = [("freq", _dp), ("amp", _dp), ("phase", _dp), ("length", ctypes.c_size_t)]
 
 
+_dpp = ctypes.POINTER(_dp)
+
+
+class HetStruct(ctypes.Structure):
+    """struct Het of QuickCBC.h:27-50."""
+    _fields_ = [("M", ctypes.c_int), ("MM", ctypes.c_int), ("MN", ctypes.c_int), ("ell", ctypes.c_int),
+                ("J", ctypes.c_int), ("NR", ctypes.c_int), ("hh", ctypes.c_double), ("hr", ctypes.c_double),
+                ("logLb", ctypes.c_double), ("IP", ctypes.POINTER(_dpp)), ("SL", ctypes.POINTER(_dpp)),
+                ("amp", _dpp), ("phase", _dpp), ("rc", _dpp), ("rs", _dpp), ("aa", _dpp),
+                ("lc", ctypes.POINTER(_dpp)), ("ls", ctypes.POINTER(_dpp)), ("pref", _dp), ("fgrid", _ip),
+                ("freq", ctypes.POINTER(RealVector))]
+
+
 _lib = None
 
 
@@ -97,6 +110,26 @@ def lib():
         L.qcbc_set_band.argtypes = [ctypes.c_double, ctypes.c_double]
         L.qcbc_set_fref.argtypes = [ctypes.c_double]
         L.qcbc_set_exit_on_error.argtypes = [ctypes.c_int]
+        L.fullphaseamp.restype = None
+        L.fullphaseamp.argtypes = [ctypes.POINTER(Net), _dpp, _dpp, ctypes.POINTER(RealVector), _dp, ctypes.c_int]
+        L.heterodyne.restype = None
+        L.heterodyne.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dpp, _dp,
+                                 ctypes.POINTER(RealVector), _dpp, ctypes.c_int, ctypes.c_double]
+        L.log_likelihood_het.restype = ctypes.c_double
+        L.log_likelihood_het.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dp, ctypes.c_double]
+        # heterodyned likelihood
+        L.qcbc_het_create.argtypes = [_vp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(_vp)]
+        L.qcbc_het_import.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_dp] * 10 + [ctypes.c_double,
+                                                                                        ctypes.POINTER(_vp)]
+        L.qcbc_het_info.argtypes = [_vp, _ip, _ip, _ip, _ip, _dp]
+        L.qcbc_het_export.argtypes = [_vp] + [_dp] * 10
+        L.qcbc_het_weights.argtypes = [_vp, _dp]
+        L.qcbc_het_loglike.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.qcbc_het_loglike_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+        L.qcbc_het_sync.argtypes = [_vp]
+        L.qcbc_het_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_het_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -609,3 +642,159 @@ def point_records(ctx, params):
     recs = (PointRecord * p.shape[0])()
     _check(L.qcbc_point_records(ctx.h, _d(p), p.shape[0], recs), "qcbc_point_records")
     return list(recs)
+
+
+# ------------------------------------------------- heterodyned likelihood
+HET_KEYS = ("amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")
+
+
+class Het:
+    """A heterodyne on the GPU (qcbc_het): made by heterodyne() on a bound
+    analysis (``Het(ctx, params=..., fgrid=..., J=..., MN=..., MM=...)``) or
+    imported from set-up arrays (``Het(ctx, arrays=dict)``, the layout of
+    the oracle's heterodyne)."""
+
+    def __init__(self, ctx, params=None, fgrid=None, J=2, MN=None, MM=None, arrays=None):
+        self.ctx = ctx
+        nd = ctx.an.nifo
+        h = _vp()
+        if arrays is None:
+            p = np.ascontiguousarray(params, np.float64)
+            g = np.ascontiguousarray(fgrid, np.int32)
+            _check(lib().qcbc_het_create(ctx.h, _d(p), _i(g), g.size, J, MN, MM, ctypes.byref(h)),
+                   "qcbc_het_create")
+        else:
+            a = [np.ascontiguousarray(arrays[k], np.float64) for k in ("freq",) + HET_KEYS]
+            _check(lib().qcbc_het_import(ctx.h, int(arrays["M"]), int(arrays["J"]), *[_d(x) for x in a],
+                                         float(arrays["logLb"]), ctypes.byref(h)), "qcbc_het_import")
+        self.h = h
+        M, Jo, NR, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lb = ctypes.c_double()
+        _check(lib().qcbc_het_info(h, ctypes.byref(M), ctypes.byref(Jo), ctypes.byref(NR), ctypes.byref(n),
+                                   ctypes.byref(lb)), "qcbc_het_info")
+        self.M, self.J, self.NR, self.nd, self.logLb = M.value, Jo.value, NR.value, nd, lb.value
+
+    def export(self):
+        M, J1, NR, nd = self.M, self.J + 1, self.NR, self.nd
+        out = dict(M=M, J=self.J, NR=NR, logLb=self.logLb, freq=np.zeros(M), amp=np.zeros((nd, M)),
+                   phase=np.zeros((nd, M)), IP=np.zeros((NR, J1, J1)), SL=np.zeros((NR, nd, J1)),
+                   lc=np.zeros((NR, nd, J1)), ls=np.zeros((NR, nd, J1)), rc=np.zeros((nd, NR)),
+                   rs=np.zeros((nd, NR)), aa=np.zeros((nd, NR)))
+        _check(lib().qcbc_het_export(self.h, _d(out["freq"]), *[_d(out[k]) for k in HET_KEYS]), "qcbc_het_export")
+        return out
+
+    def weights(self):
+        w = np.zeros((3, self.nd, self.M))
+        _check(lib().qcbc_het_weights(self.h, _d(w)), "qcbc_het_weights")
+        return w
+
+    def loglike(self, params):
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        logL = np.zeros(B)
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_het_loglike(self.h, _d(p), B, _d(logL), _i(st)), "qcbc_het_loglike")
+        return logL, st
+
+    def loglike_device(self, d_params, B, d_logL, d_status=0):
+        _check(lib().qcbc_het_loglike_device(self.h, d_params, B, d_logL, d_status), "qcbc_het_loglike_device")
+
+    def sync(self):
+        _check(lib().qcbc_het_sync(self.h), "qcbc_het_sync")
+
+    def timing(self, enable):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(lib().qcbc_het_timing(self.h, enable, ctypes.byref(ms), ctypes.byref(n)), "qcbc_het_timing")
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().qcbc_het_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+def dropin_fullphaseamp(an, freq, params):
+    net = make_net(an)
+    f = np.ascontiguousarray(freq, np.float64)
+    rv = RealVector()
+    rv.data = _d(f)
+    rv.length = f.size
+    amp = RowMatrix(np.zeros((an.nifo, f.size)))
+    ph = RowMatrix(np.zeros((an.nifo, f.size)))
+    p = np.ascontiguousarray(params, np.float64)
+    lib().fullphaseamp(ctypes.byref(net), amp.p, ph.p, ctypes.byref(rv), _d(p), 2 * f.size)
+    return amp.arr, ph.arr
+
+
+class HetView:
+    """A struct Het as QuickCBC holds it: either filled by the drop-in
+    heterodyne() (``read()`` copies its arrays out) or built here from
+    set-up arrays in the reference's double_matrix / double_tensor shapes."""
+
+    def __init__(self, M, J, NR, MN, MM, fgrid, Tobs):
+        self.s = HetStruct()
+        self.s.M, self.s.J, self.s.NR, self.s.MN, self.s.MM = M, J, NR, MN, MM
+        self._g = np.ascontiguousarray(fgrid, np.int32)
+        self._f = np.ascontiguousarray(self._g / Tobs, np.float64)
+        self._rv = RealVector()
+        self._rv.data = _d(self._f)
+        self._rv.length = self._f.size
+        self.s.fgrid = _i(self._g)
+        self.s.freq = ctypes.pointer(self._rv)
+        self._keep = []
+
+    @classmethod
+    def from_arrays(cls, h, Tobs, nd):
+        v = cls(h["M"], h["J"], h["NR"], h.get("MN", 0), h.get("MM", 0), h["fgrid"], Tobs)
+        v.s.logLb = h["logLb"]
+        for k in ("amp", "phase", "rc", "rs", "aa"):
+            m = RowMatrix(h[k])
+            v._keep.append(m)
+            setattr(v.s, k, m.p)
+        for k in ("IP", "SL", "lc", "ls"):
+            rows = [RowMatrix(h[k][i]) for i in range(h[k].shape[0])]
+            v._keep += rows
+            t = (_dpp * len(rows))(*[r.p for r in rows])
+            v._keep.append(t)
+            setattr(v.s, k, ctypes.cast(t, ctypes.POINTER(_dpp)))
+        if "pref" in h:
+            pr = np.ascontiguousarray(h["pref"], np.float64)
+            v._keep.append(pr)
+            v.s.pref = _d(pr)
+        return v
+
+    def read(self, nd):
+        s = self.s
+        M, J1, NR = s.M, s.J + 1, s.NR
+        mat = lambda p, n, m: np.array([[p[i][j] for j in range(m)] for i in range(n)])
+        return dict(M=M, J=s.J, NR=NR, logLb=s.logLb, freq=self._f.copy(), fgrid=self._g.copy(),
+                    amp=mat(s.amp, nd, M), phase=mat(s.phase, nd, M), rc=mat(s.rc, nd, NR), rs=mat(s.rs, nd, NR),
+                    aa=mat(s.aa, nd, NR), IP=np.array([mat(s.IP[i], J1, J1) for i in range(NR)]),
+                    SL=np.array([mat(s.SL[i], nd, J1) for i in range(NR)]),
+                    lc=np.array([mat(s.lc[i], nd, J1) for i in range(NR)]),
+                    ls=np.array([mat(s.ls[i], nd, J1) for i in range(NR)]),
+                    pref=np.array([s.pref[i] for i in range(11)]))
+
+
+def dropin_heterodyne(an, Drows, SNrows, params, fgrid, J, MN, MM):
+    g = np.ascontiguousarray(fgrid, np.int32)
+    v = HetView(g.size, J, (g.size - 1) // J, MN, MM, g, an.Tobs)
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.ascontiguousarray(params, np.float64)
+    lib().heterodyne(ctypes.byref(net), ctypes.byref(v.s), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, an.N,
+                     an.Tobs)
+    return v
+
+
+def dropin_log_likelihood_het(an, view, params):
+    net = make_net(an)
+    p = np.ascontiguousarray(params, np.float64)
+    return lib().log_likelihood_het(ctypes.byref(net), ctypes.byref(view.s), _d(p), an.Tobs)

@@ -33,7 +33,8 @@ def exported_symbols():
 def test_header_declares_the_reference_entry_points():
     names = declared_functions()
     for ref in ("log_likelihood_full", "fulltemplates", "IMRPhenomDGenerateh22FDAmpPhase", "fourier_nwip",
-                "F_ant", "Times", "fbegin", "CreateAmpPhaseFDWaveform", "DestroyAmpPhaseFDWaveform"):
+                "F_ant", "Times", "fbegin", "CreateAmpPhaseFDWaveform", "DestroyAmpPhaseFDWaveform",
+                "fullphaseamp", "heterodyne", "log_likelihood_het"):
         assert ref in names
 
 
@@ -94,3 +95,24 @@ def test_point_record_mirror_matches_the_struct(tmp_path):
     assert int(got.pop("sizeof")) == ctypes.sizeof(PointRecord)
     for n in names:
         assert getattr(PointRecord, n).offset == int(got[n]), n
+
+
+def test_reference_struct_mirrors_match_the_header(tmp_path):
+    """struct Net and struct Het as the ctypes bindings lay them out are the
+    header's (QuickCBC.h:13-50), field by field."""
+    from quickcbc_amd.qcbc import HetStruct, Net
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "qcbc.h"', "int main(void) {"]
+    for tag, cls in (("Net", Net), ("Het", HetStruct)):
+        for f in cls._fields_:
+            src.append('  printf("%s.%s %%zu\\n", offsetof(struct %s, %s));' % (tag, f[0], tag, f[0]))
+        src.append('  printf("%s.sizeof %%zu\\n", sizeof(struct %s));' % (tag, tag))
+    src.append("  return 0;\n}")
+    c, exe = tmp_path / "off.c", tmp_path / "off"
+    c.write_text("\n".join(src) + "\n")
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(c)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True)
+               .stdout.splitlines())
+    for tag, cls in (("Net", Net), ("Het", HetStruct)):
+        assert int(got[tag + ".sizeof"]) == ctypes.sizeof(cls), tag
+        for f in cls._fields_:
+            assert getattr(cls, f[0]).offset == int(got[tag + "." + f[0]]), (tag, f[0])
