@@ -61,6 +61,60 @@ def cpu_baseline(an, D, SN, P, seconds):
     return n2 / dt, n2, dt, O.lib().qo_num_threads()
 
 
+def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
+    """The heterodyned likelihood (log_likelihood_het, QuickCBC.c:5303-5444)
+    for the same 2048-point ensemble: heterodyne at the injection on a grid
+    over limits()' SNR band, then batched evaluation with the points resident
+    on the GPU (per launch: IMRPhenomD set-up of every point, then one wave per
+    point over the grid samples)."""
+    A, _ = Q.dropin_fullphaseamp(an, an.freq(), inj)
+    MN, MM = W.snr_band(an, A, SN, Q.dropin_fbegin(inj))
+    g = W.het_grid(MN, MM, an.Tobs)
+    t0 = time.perf_counter()
+    het = Q.Het(ctx, params=inj, fgrid=g, J=2, MN=MN, MM=MM)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    B = P.shape[0]
+    dp = Q.DeviceBuffer.from_array(np.ascontiguousarray(P, np.float64))
+    dl = Q.DeviceBuffer(8 * B)
+    for _ in range(20):
+        het.loglike_device(dp.ptr, B, dl.ptr)
+    het.sync()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        het.loglike_device(dp.ptr, B, dl.ptr)
+    het.sync()
+    T = time.perf_counter() - t0
+    het.timing(1)
+    for _ in range(launches):
+        het.loglike_device(dp.ptr, B, dl.ptr)
+    ms, n = het.timing(0)
+    Lh, sh = het.loglike(P)
+    Lf, _, _ = ctx.loglike(P, Q.FP64)
+    out = {"logl_per_s": B * launches / T, "launch_ms": T / launches * 1e3, "kernel_ms": ms / max(n, 1),
+           "kernel": "k_het_ll<%d>" % an.nifo, "points": B, "grid_samples": het.M, "regions": het.NR,
+           "band_bins": [MN, MM], "setup_ms": setup_ms,
+           "het_vs_full_abs_dlogl_median": percentile(np.abs(Lh - Lf), 50)}
+    het.close()
+    dp.free()
+    dl.free()
+    if not no_cpu:
+        from oracle import oracle as O
+        st, oh = O.heterodyne(an.labels, an.gmst, an.fref, D, SN, an.N, an.Tobs, inj, g, 2, MN, MM)
+        n0 = 2048
+        t0 = time.perf_counter()
+        O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, oh, P[:n0])
+        dt = time.perf_counter() - t0
+        n2 = int(min(P.shape[0] * 512, max(n0, cpu_seconds / max(dt / n0, 1e-9))))
+        idx = np.arange(n2) % P.shape[0]
+        t0 = time.perf_counter()
+        O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, oh, P[idx])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n2 / dt, "unit": "logL evals/s", "cores": O.lib().qo_num_threads(),
+                               "kind": "port", "sample": "%d points of the same ensemble, %.1f s, gcc -O2 -fopenmp, "
+                                                         "OpenMP over points" % (n2, dt)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +135,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=200, help="PT-step requests per analysis")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="analysis arrivals per second")
     ap.add_argument("--draws", type=int, default=100000)
+    ap.add_argument("--het-launches", type=int, default=500)
+    ap.add_argument("--only-het", action="store_true", help="time the heterodyned likelihood only (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how the "
@@ -121,6 +177,11 @@ def main():
     sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
     plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/3), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
+
+    if args.only_het:
+        r = het_bench(Q, W, an, ctx, inj, D, SN, init, args.het_launches, 5.0, True)
+        print(json.dumps({"config": "heterodyned likelihood", **r}), flush=True)
+        return
 
     if args.autotune:
         for prec in (Q.FP64, Q.MIXED):
@@ -246,6 +307,8 @@ def main():
         extra["fp64_fused_over_plain"] = value / extra["plain_fp64_logl_per_s"]
         extra["launch_shape_fp64"] = ctx.get_shape(Q.FP64)
         extra["launch_shape_mixed"] = ctx.get_shape(Q.MIXED)
+        extra["het"] = het_bench(Q, W, an, ctx, inj, D, SN, init, args.het_launches, args.cpu_seconds / 3,
+                                 args.no_cpu)
         log(json.dumps(extra))
 
     cpu = None

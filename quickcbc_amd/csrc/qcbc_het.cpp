@@ -37,6 +37,8 @@ struct qcbc_het {
   double* d_amp = nullptr;    // nd x M
   double* d_phase = nullptr;  // nd x M
   double* d_w = nullptr;      // 3 x nd x M
+  double* d_wf = nullptr;     // 3 x nd x M, folded with the reference amplitude
+  double* d_pl = nullptr;     // 4 x M sample powers
   double* d_raw = nullptr;    // IP | SL | lc | ls | rc | rs | aa
   size_t off[7] = {}, len[7] = {};
   // batch scratch
@@ -78,7 +80,8 @@ void het_free(qcbc_het* h) {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
   }
-  void* bufs[] = {h->d_freq, h->d_amp, h->d_phase, h->d_w, h->d_raw, h->d_params, h->d_pts, h->d_status, h->d_logL};
+  void* bufs[] = {h->d_freq, h->d_amp, h->d_phase, h->d_w,      h->d_wf,
+                  h->d_pl,   h->d_raw, h->d_params, h->d_pts, h->d_status, h->d_logL};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -102,6 +105,8 @@ int het_alloc(qcbc_ctx* ctx, int M, int J, qcbc_het** out) {
       hipMalloc(&h->d_amp, sizeof(double) * nd * M) != hipSuccess ||
       hipMalloc(&h->d_phase, sizeof(double) * nd * M) != hipSuccess ||
       hipMalloc(&h->d_w, sizeof(double) * 3 * nd * M) != hipSuccess ||
+      hipMalloc(&h->d_wf, sizeof(double) * 3 * nd * M) != hipSuccess ||
+      hipMalloc(&h->d_pl, sizeof(double) * 4 * M) != hipSuccess ||
       hipMalloc(&h->d_raw, sizeof(double) * raw_total(h)) != hipSuccess) {
     het_free(h);
     return QCBC_ENOMEM;
@@ -152,7 +157,7 @@ int het_eval(qcbc_het* h, const double* d_params, int B, double* d_logL, int* d_
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, h->stream));
   }
-  HIPCHK(launch_het_ll(h->d_pts, B, h->nd, h->d_freq, h->d_amp, h->d_phase, h->d_w, h->M, h->an.sqT, h->logLb, d_logL,
+  HIPCHK(launch_het_ll(h->d_pts, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_phase, h->d_wf, h->M, h->logLb, d_logL,
                        h->stream));
   if (h->timing) {
     HIPCHK(hipEventRecord(e1, h->stream));
@@ -185,7 +190,7 @@ int qcbc_het_create(qcbc_ctx* ctx, const double* params, const int* fgrid, int M
   int st = het_alloc(ctx, M, J, &h);
   if (st) return st;
   // scratch: params, point, status, logLb, h22 amp/phase, per-bin planes, partials, grid
-  const size_t nbins = 5 * (size_t)nd * n2, npart = (size_t)HET_SETUP_BLOCKS * 2 * nd;
+  const size_t nbins = 3 * (size_t)nd * n2, npart = (size_t)HET_SETUP_BLOCKS * 2 * nd;
   const size_t ndbl = 11 + 1 + 2 * (size_t)n2 + nbins + npart;
   double* d_tmp = nullptr;
   Point* d_pt = nullptr;
@@ -219,7 +224,7 @@ int qcbc_het_create(qcbc_ctx* ctx, const double* params, const int* fgrid, int M
       launch_het_setup(d_pt, d_wa, d_wp, ctx->d_D, ctx->d_SN, nd, N, ctx->desc.Tobs, ctx->an.sqT, d_fg, M, J, MN, MM,
                        d_bins, d_part, d_lb, h->d_freq, h->d_amp, h->d_phase, r + h->off[R_IP], r + h->off[R_SL],
                        r + h->off[R_LC], r + h->off[R_LS], r + h->off[R_RC], r + h->off[R_RS], r + h->off[R_AA],
-                       h->d_w, s) != hipSuccess ||
+                       h->d_w, h->d_wf, h->d_pl, s) != hipSuccess ||
       hipMemcpyAsync(&pst, d_st, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipMemcpyAsync(&lb, d_lb, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
@@ -257,8 +262,9 @@ int qcbc_het_import(qcbc_ctx* ctx, int M, int J, const double* freq, const doubl
          hipSuccess;
   double* r = h->d_raw;
   ok = ok &&
-       launch_het_weights(r + h->off[R_IP], r + h->off[R_SL], r + h->off[R_LC], r + h->off[R_LS], r + h->off[R_RC],
-                          r + h->off[R_RS], r + h->off[R_AA], M, J, h->nd, h->d_w, s) == hipSuccess &&
+       launch_het_weights(h->d_freq, h->d_amp, r + h->off[R_IP], r + h->off[R_SL], r + h->off[R_LC],
+                          r + h->off[R_LS], r + h->off[R_RC], r + h->off[R_RS], r + h->off[R_AA], M, J, h->nd, h->d_w,
+                          h->d_wf, h->d_pl, s) == hipSuccess &&
        hipStreamSynchronize(s) == hipSuccess;
   if (!ok) {
     het_free(h);

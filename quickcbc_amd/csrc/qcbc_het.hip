@@ -11,6 +11,8 @@
 //   SL, lc, ls                              NR x nd x (J+1)
 //   rc, rs, aa                              nd x NR
 //   w (evaluation weights)                  3 x nd x M: wH, wC, wS
+//   wf (the same over a_ref^2, a_ref, a_ref) 3 x nd x M
+//   pl (sample powers)                       4 x M: f^(1/6), f^(-1/6), 1/f, log f
 #include "qcbc_device.h"
 #include "qcbc_het_internal.h"
 
@@ -41,6 +43,53 @@ __device__ __forceinline__ void h22_at(const Point& c, double f, double* a, doub
     *a = c.amp0 * amplitude(c, p);
     *ph = phase(c, p);
   }
+}
+
+// The frequency powers of one sample from its planes and the point's
+// Msec^(1/6), Msec^(-1/6), 1/Msec and log Msec: the products and sums
+// pow_direct forms with pow, log and divisions, to a few ulps of its values.
+__device__ __forceinline__ Pow pow_planes(const Point& c, double i_Msec, double f, const double* __restrict__ pl,
+                                          int M, int m) {
+  Pow p;
+  p.Mf = c.Msec * f;
+  p.s6 = c.s6_scale * pl[m];
+  p.third = p.s6 * p.s6;
+  p.two_thirds = p.third * p.third;
+  p.four_thirds = p.Mf * p.third;
+  p.five_thirds = p.four_thirds * p.third;
+  p.two = p.Mf * p.Mf;
+  p.seven_thirds = p.third * p.two;
+  p.eight_thirds = p.two_thirds * p.two;
+  p.i_s6 = c.is6_scale * pl[M + m];
+  p.i_third = p.i_s6 * p.i_s6;
+  p.i_two_thirds = p.i_third * p.i_third;
+  p.i_Mf = i_Msec * pl[2 * M + m];
+  p.i_five_thirds = p.i_Mf * p.i_two_thirds;
+  p.logMf = c.logMsec + pl[3 * M + m];
+  p.logv = (LOG_PI + p.logMf) / 3.0;  // log v, v = (pi Mf)^(1/3)
+  return p;
+}
+
+// The h22 amplitude without amp0 (a) and phase (p) of point c at grid
+// sample m, zero above fCut (IMRPhenomD.c:529-555), powers from the planes
+__device__ __forceinline__ void h22_planes(const Point& c, double i_Msec, double f, const double* __restrict__ pl,
+                                           int M, int m, double* a, double* p) {
+  *a = 0.0;
+  *p = 0.0;
+  if (!(f > c.fCut)) {
+    const Pow pw = pow_planes(c, i_Msec, f, pl, M, m);
+    *a = amplitude(c, pw);
+    *p = phase(c, pw);
+  }
+}
+
+// fullphaseamp's projected amplitude ad and phase x (QuickCBC.c:3849-3862)
+// for detector d: ad = Fs h22fac amp0 a / sqrt(Tobs) y with the point's
+// detector scale Fs h22fac amp0 / sqrt(Tobs) (Ascale) and the taper y
+__device__ __forceinline__ void project(const Point& c, int d, double f, double a, double p, double y, double* ad,
+                                        double* x) {
+  *x = TPI * f * c.tshift[d] + c.lambda[d] - p;
+  *ad = c.Ascale[d] * a * y;
 }
 
 // legendre_maker's recurrence (QuickCBC.c:5066-5088) at one point k of 0..U
@@ -127,10 +176,10 @@ __global__ void k_phaseamp(const Point* __restrict__ pts, const double* __restri
 
 // heterodyne's per-bin work (QuickCBC.c:5150-5200) at the reference point,
 // from the h22 amplitude and phase of every bin (k_waveform): the template
-// hb (fulltemplates), residual rb = D - hb, the products RC, RS, AA and the
-// bins' fullphaseamp amplitude and phase; and per block the partial sums of
-// (h|h) and (r|h) over [MN, MM) (fourier_nwip).  Planes nd x N/2:
-// out = [RC | RS | AA | ampb | phaseb].
+// hb (fulltemplates), residual rb = D - hb and, with fullphaseamp's amplitude
+// and phase of the bin, the products RC, RS, AA; and per block the partial
+// sums of (h|h) and (r|h) over [MN, MM) (fourier_nwip).  Planes nd x N/2:
+// out = [RC | RS | AA].
 __global__ void k_het_bins(const Point* __restrict__ pts, const double* __restrict__ wa, const double* __restrict__ wp,
                            const double* __restrict__ D, const double* __restrict__ SN, int nd, int N, double Tobs,
                            double sqT, int MN, int MM, double* __restrict__ out, double* __restrict__ partial) {
@@ -147,8 +196,6 @@ __global__ void k_het_bins(const Point* __restrict__ pts, const double* __restri
       const double A = c.Fs[d] * H22FAC * wa[j] / sqT;
       const double x = TPI * f * c.tshift[d] + c.lambda[d] - wp[j];
       const double cp = cos(x), sp = sin(x);
-      out[3 * P + o] = A * y;
-      out[4 * P + o] = x;
       if (j == 0) {  // RC[0] = RS[0] = 0 (:5188-5189); AA[0] is never read
         out[o] = out[P + o] = out[2 * P + o] = 0.0;
         continue;
@@ -200,20 +247,39 @@ __global__ void k_het_logLb(const double* __restrict__ partial, int nblk, int nd
   logLb[0] = logL;
 }
 
-// The reference amplitude and phase on the grid (:5154-5163) and the grid
-// frequencies fgrid / Tobs (het_space, :4901)
-__global__ void k_het_samples(const double* __restrict__ bins, const int* __restrict__ fgrid, int M, int nd, int n2,
-                              double Tobs, double* __restrict__ freq, double* __restrict__ amp,
-                              double* __restrict__ phs) {
+// The grid frequencies fgrid / Tobs (het_space, QuickCBC.c:4901)
+__global__ void k_het_freq(const int* __restrict__ fgrid, int M, double Tobs, double* __restrict__ freq) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
-  const size_t P = (size_t)nd * n2;
-  const int k = fgrid[m];
-  freq[m] = ((double)k) / Tobs;
-  for (int d = 0; d < nd; d++) {
-    amp[(size_t)d * M + m] = bins[3 * P + (size_t)d * n2 + k];
-    phs[(size_t)d * M + m] = bins[4 * P + (size_t)d * n2 + k];
-  }
+  freq[m] = ((double)fgrid[m]) / Tobs;
+}
+
+// The reference amplitude and phase on the grid (:5154-5163).  The
+// reference takes them from fullphaseamp of every bin; here they come from
+// the evaluation kernel's own per-sample code, so that, as in the reference,
+// the heterodyne ratios of the reference point are exactly 0.
+__global__ void k_het_ref(const Point* __restrict__ pts, const double* __restrict__ freq,
+                          const double* __restrict__ pl, int M, int nd, double* __restrict__ amp,
+                          double* __restrict__ phs) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const Point& c = pts[0];
+  const double f = freq[m];
+  double a, p;
+  h22_planes(c, 1.0 / c.Msec, f, pl, M, m, &a, &p);
+  const double y = taper(f, c.fs);
+  for (int d = 0; d < nd; d++) project(c, d, f, a, p, y, &amp[(size_t)d * M + m], &phs[(size_t)d * M + m]);
+}
+
+// The sample powers of the evaluation (4 x M)
+__global__ void k_het_planes(const double* __restrict__ freq, int M, double* __restrict__ pl) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const double f = freq[m];
+  pl[m] = pow(f, 1.0 / 6.0);
+  pl[M + m] = pow(f, -1.0 / 6.0);
+  pl[2 * M + m] = 1.0 / f;
+  pl[3 * M + m] = log(f);
 }
 
 // One 64-lane block per region i (:5202-5290): the Legendre moments SL, lc,
@@ -277,10 +343,12 @@ __global__ __launch_bounds__(64) void k_het_regions(const double* __restrict__ b
 // regions i = 1 .. NR-2 is taken off once with aa[d][i] (the overcount
 // correction); likewise HR with lc, rc (hc) and ls, rs (hs).  A boundary
 // sample belongs to two regions: the earlier region's term comes first.
+// wf folds the ratios' 1/a_ref in: wH/a_ref^2, wC/a_ref, wS/a_ref.
 __global__ void k_het_weights(const double* __restrict__ IP, const double* __restrict__ SL,
                               const double* __restrict__ lc, const double* __restrict__ ls,
                               const double* __restrict__ rc, const double* __restrict__ rs,
-                              const double* __restrict__ aa, int M, int J, int NR, int nd, double* __restrict__ w) {
+                              const double* __restrict__ aa, const double* __restrict__ amp, int M, int J, int NR,
+                              int nd, double* __restrict__ w, double* __restrict__ wf) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int J1 = J + 1;
@@ -317,57 +385,79 @@ __global__ void k_het_weights(const double* __restrict__ IP, const double* __res
     w[(size_t)(0 * nd + d) * M + m] = wh;
     w[(size_t)(1 * nd + d) * M + m] = wc;
     w[(size_t)(2 * nd + d) * M + m] = ws;
+    const double ar = amp[(size_t)d * M + m];
+    wf[(size_t)(0 * nd + d) * M + m] = wh / (ar * ar);
+    wf[(size_t)(1 * nd + d) * M + m] = wc / ar;
+    wf[(size_t)(2 * nd + d) * M + m] = ws / ar;
   }
 }
 
-// log_likelihood_het (:5303-5444) for a batch: one wave per point, the grid
-// samples on the lanes.  Per sample and detector fullphaseamp's amplitude and
-// phase (:3849-3862), the heterodyne ratios (:5340-5345)
-//   hc = 2 (a_ref - a cos(phi_ref - phi)) / a_ref, hs = 2 a sin(phi_ref - phi) / a_ref,
-// then HR_d = sum_m (wC hc + wS hs), HH_d = sum_m wH (hc^2 + hs^2) and
-// logL = logLb - sum_d (HR_d + HH_d / 2).
-template <int ND>
+// log_likelihood_het (:5303-5444) for a batch: WPP waves per point, the
+// grid samples on their lanes.  Per sample and detector fullphaseamp's
+// amplitude a and phase phi (:3849-3862) and the heterodyne differences
+// (:5340-5345) cc = 2 (a_ref - a cos(phi_ref - phi)), ss = 2 a sin(phi_ref - phi);
+// the ratios hc = cc / a_ref, hs = ss / a_ref enter through the folded
+// weights: HR_d = sum_m (wC hc + wS hs) = sum_m (wfC cc + wfS ss), HH_d =
+// sum_m wH (hc^2 + hs^2) = sum_m wfH (cc^2 + ss^2), and
+// logL = logLb - sum_d (HR_d + HH_d / 2).  256-thread blocks, 4 / WPP points each.
+template <int ND, int WPP>
 __global__ __launch_bounds__(256) void k_het_ll(const Point* __restrict__ pts, int B, const double* __restrict__ freq,
-                                                const double* __restrict__ ha, const double* __restrict__ hp,
-                                                const double* __restrict__ w, int M, double sqT, double logLb,
-                                                double* __restrict__ logL) {
-  const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (b >= B) return;
-  const Point& c = pts[b];
-  if (c.status != ST_OK) {
-    if (lane == 0) logL[b] = 0.0;
-    return;
-  }
+                                                const double* __restrict__ pl, const double* __restrict__ ha,
+                                                const double* __restrict__ hp, const double* __restrict__ wf, int M,
+                                                double logLb, double* __restrict__ logL) {
+  __shared__ double red[4][2 * ND];
+  // the wave index made visibly uniform: the point's constants are then scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int b = blockIdx.x * (4 / WPP) + wave / WPP, sub = wave % WPP;
   double hr[ND], hh[ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) hr[d] = hh[d] = 0.0;
-  for (int m = lane; m < M; m += 64) {
-    const double f = freq[m];
-    double a, p;
-    h22_at(c, f, &a, &p);
-    const double y = taper(f, c.fs);
+  const bool live = b < B && pts[b < B ? b : 0].status == ST_OK;
+  if (live) {
+    const Point& c = pts[b];
+    const double i_Msec = 1.0 / c.Msec;
+    for (int m = sub * 64 + lane; m < M; m += 64 * WPP) {
+      const double f = freq[m];
+      double a, p;
+      h22_planes(c, i_Msec, f, pl, M, m, &a, &p);
+      const double y = taper(f, c.fs);
 #pragma unroll
-    for (int d = 0; d < ND; d++) {
-      const double A = c.Fs[d] * H22FAC * a / sqT;
-      const double x = TPI * f * c.tshift[d] + c.lambda[d] - p;
-      const double ad = A * y;
-      const double ar = ha[(size_t)d * M + m];
-      double sn, cs;
-      sincos(hp[(size_t)d * M + m] - x, &sn, &cs);
-      const double cc = 2.0 * (ar - ad * cs);
-      const double ss = 2.0 * ad * sn;
-      const double hc = cc / ar, hs = ss / ar;
-      hr[d] += w[(size_t)(1 * ND + d) * M + m] * hc + w[(size_t)(2 * ND + d) * M + m] * hs;
-      hh[d] += w[(size_t)(0 * ND + d) * M + m] * (hc * hc + hs * hs);
+      for (int d = 0; d < ND; d++) {
+        double ad, x, sn, cs;
+        project(c, d, f, a, p, y, &ad, &x);
+        sincos(hp[(size_t)d * M + m] - x, &sn, &cs);
+        const double cc = 2.0 * (ha[(size_t)d * M + m] - ad * cs);
+        const double ss = 2.0 * ad * sn;
+        hr[d] += wf[(size_t)(1 * ND + d) * M + m] * cc + wf[(size_t)(2 * ND + d) * M + m] * ss;
+        hh[d] += wf[(size_t)(0 * ND + d) * M + m] * (cc * cc + ss * ss);
+      }
     }
   }
-  double L = logLb;
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    const double HR = het_wave_sum(hr[d]), HH = het_wave_sum(hh[d]);
-    L -= (HR + 0.5 * HH);
+    const double r = het_wave_sum(hr[d]), q = het_wave_sum(hh[d]);
+    if (lane == 0) {
+      red[wave][2 * d] = r;
+      red[wave][2 * d + 1] = q;
+    }
   }
-  if (lane == 0) logL[b] = L;
+  __syncthreads();
+  if (sub == 0 && lane == 0 && b < B) {
+    double L = 0.0;
+    if (live) {
+      L = logLb;
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        double HR = 0.0, HH = 0.0;
+        for (int k = 0; k < WPP; k++) {
+          HR += red[wave + k][2 * d];
+          HH += red[wave + k][2 * d + 1];
+        }
+        L -= (HR + 0.5 * HH);
+      }
+    }
+    logL[b] = L;
+  }
 }
 
 // ------------------------------------------------------------ launchers
@@ -382,38 +472,64 @@ hipError_t launch_het_setup(const Point* pts, const double* wa, const double* wp
                             int nd, int N, double Tobs, double sqT, const int* fgrid, int M, int J, int MN, int MM,
                             double* bins, double* partial, double* logLb, double* freq, double* amp, double* phs,
                             double* IP, double* SL, double* lc, double* ls, double* rc, double* rs, double* aa,
-                            double* w, hipStream_t s) {
+                            double* w, double* wf, double* pl, hipStream_t s) {
   const int NR = (M - 1) / J;
   hipLaunchKernelGGL(k_het_bins, dim3(HET_SETUP_BLOCKS), dim3(256), 0, s, pts, wa, wp, D, SN, nd, N, Tobs, sqT, MN,
                      MM, bins, partial);
   hipLaunchKernelGGL(k_het_logLb, dim3(1), dim3(64), 0, s, partial, HET_SETUP_BLOCKS, nd, logLb);
-  hipLaunchKernelGGL(k_het_samples, dim3((M + 255) / 256), dim3(256), 0, s, bins, fgrid, M, nd, N / 2, Tobs, freq,
-                     amp, phs);
+  hipLaunchKernelGGL(k_het_freq, dim3((M + 255) / 256), dim3(256), 0, s, fgrid, M, Tobs, freq);
+  hipLaunchKernelGGL(k_het_planes, dim3((M + 255) / 256), dim3(256), 0, s, freq, M, pl);
+  hipLaunchKernelGGL(k_het_ref, dim3((M + 255) / 256), dim3(256), 0, s, pts, freq, pl, M, nd, amp, phs);
   hipLaunchKernelGGL(k_het_regions, dim3(NR), dim3(64), 0, s, bins, fgrid, J, NR, nd, N / 2, IP, SL, lc, ls, rc, rs,
                      aa);
-  hipLaunchKernelGGL(k_het_weights, dim3((M + 255) / 256), dim3(256), 0, s, IP, SL, lc, ls, rc, rs, aa, M, J, NR,
-                     nd, w);
+  hipLaunchKernelGGL(k_het_weights, dim3((M + 255) / 256), dim3(256), 0, s, IP, SL, lc, ls, rc, rs, aa, amp, M, J,
+                     NR, nd, w, wf);
   return hipGetLastError();
 }
 
-hipError_t launch_het_weights(const double* IP, const double* SL, const double* lc, const double* ls,
-                              const double* rc, const double* rs, const double* aa, int M, int J, int nd, double* w,
+hipError_t launch_het_weights(const double* freq, const double* amp, const double* IP, const double* SL,
+                              const double* lc, const double* ls, const double* rc, const double* rs,
+                              const double* aa, int M, int J, int nd, double* w, double* wf, double* pl,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_het_weights, dim3((M + 255) / 256), dim3(256), 0, s, IP, SL, lc, ls, rc, rs, aa, M, J,
-                     (M - 1) / J, nd, w);
+  hipLaunchKernelGGL(k_het_planes, dim3((M + 255) / 256), dim3(256), 0, s, freq, M, pl);
+  hipLaunchKernelGGL(k_het_weights, dim3((M + 255) / 256), dim3(256), 0, s, IP, SL, lc, ls, rc, rs, aa, amp, M, J,
+                     (M - 1) / J, nd, w, wf);
   return hipGetLastError();
 }
 
-hipError_t launch_het_ll(const Point* pts, int B, int nd, const double* freq, const double* ha, const double* hp,
-                         const double* w, int M, double sqT, double logLb, double* logL, hipStream_t s) {
-  if (B <= 0) return hipSuccess;
-  const dim3 g((B + 3) / 4), t(256);
-  if (nd == 1)
-    hipLaunchKernelGGL(k_het_ll<1>, g, t, 0, s, pts, B, freq, ha, hp, w, M, sqT, logLb, logL);
-  else if (nd == 2)
-    hipLaunchKernelGGL(k_het_ll<2>, g, t, 0, s, pts, B, freq, ha, hp, w, M, sqT, logLb, logL);
+// One wave per point once the batch gives every SIMD two waves (the kernel's
+// ~150 VGPRs allow no more than three); smaller batches split each point's
+// samples over up to four waves.
+int het_waves_per_point(int M, int B) {
+  int wpp = 1;
+  while (wpp < 4 && B * wpp < 2048 && M > 64 * wpp) wpp *= 2;
+  return wpp;
+}
+
+template <int ND>
+static void het_ll_nd(int wpp, dim3 g, hipStream_t s, const Point* pts, int B, const double* freq, const double* pl,
+                      const double* ha, const double* hp, const double* wf, int M, double logLb,
+                      double* logL) {
+  if (wpp == 1)
+    hipLaunchKernelGGL((k_het_ll<ND, 1>), g, dim3(256), 0, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
+  else if (wpp == 2)
+    hipLaunchKernelGGL((k_het_ll<ND, 2>), g, dim3(256), 0, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
   else
-    hipLaunchKernelGGL(k_het_ll<3>, g, t, 0, s, pts, B, freq, ha, hp, w, M, sqT, logLb, logL);
+    hipLaunchKernelGGL((k_het_ll<ND, 4>), g, dim3(256), 0, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
+}
+
+hipError_t launch_het_ll(const Point* pts, int B, int nd, const double* freq, const double* pl, const double* ha,
+                         const double* hp, const double* wf, int M, double logLb, double* logL,
+                         hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  const int wpp = het_waves_per_point(M, B), ppb = 4 / wpp;
+  const dim3 g((B + ppb - 1) / ppb);
+  if (nd == 1)
+    het_ll_nd<1>(wpp, g, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
+  else if (nd == 2)
+    het_ll_nd<2>(wpp, g, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
+  else
+    het_ll_nd<3>(wpp, g, s, pts, B, freq, pl, ha, hp, wf, M, logLb, logL);
   return hipGetLastError();
 }
 

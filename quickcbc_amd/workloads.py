@@ -175,6 +175,21 @@ def make_data(an: Analysis, template_fn, inj, noise_seed=2024, with_noise=True):
     return np.ascontiguousarray(h)
 
 
+def snr_band(an: Analysis, amp, SN, fs):
+    """[MN, MM] as limits() chooses them (QuickCBC.c:4275-4385): from bin
+    Tobs max(fmin, fbegin) up, the first bin where the reference point's
+    cumulative network SNR^2 passes 0.001 and the first where it comes within
+    0.001 of its total.  amp: fullphaseamp's amplitude rows (nifo x N/2) at
+    the reference point, fs its fbegin."""
+    jmin = int(an.Tobs * max(an.fmin, fs))
+    x = (4.0 * np.asarray(amp) ** 2 / SN).sum(0)
+    x[:jmin] = 0.0
+    c = np.cumsum(x)
+    j0 = int(np.argmax(c > 0.001))
+    j1 = int(np.argmax(c[-1] - c < 0.001))
+    return max(j0, jmin), j1
+
+
 def het_grid(MN, MM, Tobs, J=2, frac=1.0 / 64.0, max_hz=100.0):
     """A heterodyne grid of the shape het_space builds (QuickCBC.c:4784-4908):
     bin indices from MN in regions of J equal steps, M = NR J + 1 of them,

@@ -34,18 +34,10 @@ def points(which, n, seed=5):
 
 
 def het_band(an, ref, SN):
-    """[MN, MM] of limits() (QuickCBC.c:4275-4385): the bins where the
-    reference point's cumulative network SNR^2 passes 0.001 and where it comes
-    within 0.001 of its total, from max(fmin, fbegin) up."""
+    """[MN, MM] of limits() (QuickCBC.c:4275-4385) from the oracle's fullphaseamp."""
     st, A, _ = O.fullphaseamp(an.labels, an.gmst, an.fref, an.Tobs, an.freq(), ref)
     assert st == 0
-    jmin = int(an.Tobs * max(an.fmin, O.fbegin(ref)))
-    x = (4.0 * A ** 2 / SN).sum(0)
-    x[:jmin] = 0.0
-    c = np.cumsum(x)
-    j0 = int(np.argmax(c > 0.001))
-    j1 = int(np.argmax(c[-1] - c < 0.001))
-    return max(j0, jmin), j1
+    return W.snr_band(an, A, SN, O.fbegin(ref))
 
 
 @functools.lru_cache(maxsize=None)
