@@ -374,6 +374,9 @@ int qcbc_het_loglike_device(qcbc_het *het, const double *d_params, int B, double
 int qcbc_het_sync(qcbc_het *het);
 /* HIP-event timing of the evaluation kernel, as qcbc_ctx_timing. */
 int qcbc_het_timing(qcbc_het *het, int enable, double *kernel_ms, long long *launches);
+/* The evaluation kernel alone: B points (device memory) set up once, then
+ * reps launches back to back between two HIP events; the mean per launch. */
+int qcbc_het_time_kernel(qcbc_het *het, const double *d_params, int B, int reps, double *ms_per_launch);
 int qcbc_het_destroy(qcbc_het *het);
 
 #ifdef __cplusplus

@@ -359,6 +359,30 @@ int qcbc_het_timing(qcbc_het* h, int enable, double* kernel_ms, long long* launc
   return QCBC_OK;
 }
 
+int qcbc_het_time_kernel(qcbc_het* h, const double* d_params, int B, int reps, double* ms_per_launch) {
+  if (!h || B < 1 || !d_params || reps < 1 || !ms_per_launch) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = het_reserve(h, B);
+  if (st) return st;
+  HIPCHK(launch_setup(d_params, B, h->an, h->d_pts, h->d_status, h->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; r++)
+    HIPCHK(launch_het_ll(h->d_pts, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_phase, h->d_wf, h->M, h->logLb,
+                         h->d_logL, h->stream));
+  HIPCHK(hipEventRecord(e1, h->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *ms_per_launch = ms / reps;
+  return QCBC_OK;
+}
+
 int qcbc_het_destroy(qcbc_het* h) {
   het_free(h);
   return QCBC_OK;

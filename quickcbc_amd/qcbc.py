@@ -129,6 +129,7 @@ def lib():
         L.qcbc_het_loglike_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
         L.qcbc_het_sync.argtypes = [_vp]
         L.qcbc_het_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_het_time_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
         L.qcbc_het_destroy.argtypes = [_vp]
         _lib = L
     return _lib
@@ -707,6 +708,11 @@ class Het:
         n = ctypes.c_longlong()
         _check(lib().qcbc_het_timing(self.h, enable, ctypes.byref(ms), ctypes.byref(n)), "qcbc_het_timing")
         return ms.value, n.value
+
+    def time_kernel(self, d_params, B, reps):
+        ms = ctypes.c_double()
+        _check(lib().qcbc_het_time_kernel(self.h, d_params, B, reps, ctypes.byref(ms)), "qcbc_het_time_kernel")
+        return ms.value
 
     def close(self):
         if self.h:

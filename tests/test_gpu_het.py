@@ -221,3 +221,20 @@ def test_timing_counts_launches(setup):
         g.loglike(P)
     ms, n = g.timing(0)
     assert n == 3 and ms > 0
+
+
+def test_device_entry_points(setup):
+    """qcbc_het_loglike_device on resident points gives qcbc_het_loglike's
+    numbers; the kernel-alone timing runs."""
+    which, an, inj, D, SN, h, c, g = setup
+    P = W.ensemble_around(inj, 300, seed=6, scale=0.3)
+    L, _ = g.loglike(P)
+    dp = Q.DeviceBuffer.from_array(np.ascontiguousarray(P))
+    dl = Q.DeviceBuffer(8 * P.shape[0])
+    g.loglike_device(dp.ptr, P.shape[0], dl.ptr)
+    g.sync()
+    Ld = dl.download((P.shape[0],))
+    assert np.array_equal(Ld, L)
+    assert g.time_kernel(dp.ptr, P.shape[0], 5) > 0
+    dp.free()
+    dl.free()
