@@ -244,6 +244,7 @@ int qcbc_device_info(char *name, size_t len, int *cu_count, double *mem_gb);
  * Part 3: parallel-tempered ensemble step (resident state)            *
  * ------------------------------------------------------------------ */
 typedef struct qcbc_pt qcbc_pt;
+typedef struct qcbc_het qcbc_het; /* a heterodyne, Part 6 */
 /* n_temps x n_chains points; temperatures ladder^t (t = 0..n_temps-1).
  * init: (n_temps*n_chains) x 11.  sigma: 11 per-parameter jump scales at
  * temperature 1 (scaled by sqrt(T)). */
@@ -258,6 +259,12 @@ int qcbc_pt_create_groups(qcbc_ctx *ctx, int n_temps, int n_chains, double ladde
                           const double *sigma, uint64_t seed, int precision, int groups,
                           qcbc_pt **out);
 int qcbc_pt_groups(qcbc_pt *pt, int *groups);
+/* The same ensemble on the heterodyned likelihood of het (Part 6), which must
+ * belong to ctx and outlive the ensemble: the production sampler's
+ * likelihood (QuickCBC.c:2372) in the resident step.  FP64. */
+int qcbc_pt_create_het(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, double ladder,
+                       const double *init, const double *sigma, uint64_t seed, int groups,
+                       qcbc_pt **out);
 /* nsteps PT steps: propose, evaluate, Metropolis accept, adjacent-temperature
  * swaps, all on the GPU.  use_graph: replay one captured step as a hipGraph. */
 int qcbc_pt_run(qcbc_pt *pt, int nsteps, int use_graph);
@@ -346,7 +353,6 @@ int qcbc_server_destroy(qcbc_server *s);
  *   IP           NR x (J+1) x (J+1)
  *   SL, lc, ls   NR x n_det x (J+1)
  *   rc, rs, aa   n_det x NR */
-typedef struct qcbc_het qcbc_het;
 /* heterodyne() at params (11) on a bound analysis; logLb over bins [MN, MM).
  * J <= 8.  QCBC_EDOM if the reference point is outside the model. */
 int qcbc_het_create(qcbc_ctx *ctx, const double *params, const int *fgrid, int M, int J, int MN,

@@ -66,4 +66,8 @@ int loglike_partials(qcbc_ctx* c, const double* d_params, int B, int prec, int* 
 // setup + fused likelihood + finalize on the context stream (caller holds mu)
 int loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d_logL, double* d_rho,
                  int* d_status);
+// the heterodyned likelihood of B points on stream s: their set-up into pts,
+// status into d_status, logL into d_logL (the heterodyne's data are read only)
+int het_enqueue(qcbc_het* h, const double* d_params, int B, Point* pts, int* d_status, double* d_logL, hipStream_t s);
+int het_device(const qcbc_het* h);
 }  // namespace qcbc

@@ -74,17 +74,17 @@ size_t raw_total(const qcbc_het* h) { return h->off[6] + h->len[6]; }
 
 void het_free(qcbc_het* h) {
   if (!h) return;
-  hipSetDevice(h->device);
-  if (h->stream) hipStreamSynchronize(h->stream);
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto& e : h->ev) {
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
   }
   void* bufs[] = {h->d_freq, h->d_amp, h->d_phase, h->d_w,      h->d_wf,
                   h->d_pl,   h->d_raw, h->d_params, h->d_pts, h->d_status, h->d_logL};
   for (void* b : bufs)
-    if (b) hipFree(b);
-  if (h->stream) hipStreamDestroy(h->stream);
+    if (b) (void)hipFree(b);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -118,10 +118,10 @@ int het_alloc(qcbc_ctx* ctx, int M, int J, qcbc_het** out) {
 int het_reserve(qcbc_het* h, int B) {
   if (B <= h->cap) return QCBC_OK;
   const int nb = std::max(B, 2 * h->cap);
-  hipFree(h->d_params);
-  hipFree(h->d_pts);
-  hipFree(h->d_status);
-  hipFree(h->d_logL);
+  (void)hipFree(h->d_params);
+  (void)hipFree(h->d_pts);
+  (void)hipFree(h->d_status);
+  (void)hipFree(h->d_logL);
   h->d_params = nullptr;
   h->d_pts = nullptr;
   h->d_status = nullptr;
@@ -140,8 +140,8 @@ void harvest(qcbc_het* h) {
     float ms = 0.f;
     if (hipEventSynchronize(e.second) == hipSuccess && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess)
       h->kernel_ms += ms;
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
   }
   h->ev.clear();
 }
@@ -169,6 +169,15 @@ int het_eval(qcbc_het* h, const double* d_params, int B, double* d_logL, int* d_
 }
 
 }  // namespace
+
+int qcbc::het_enqueue(qcbc_het* h, const double* d_params, int B, Point* pts, int* d_status, double* d_logL,
+                      hipStream_t s) {
+  HIPCHK(launch_setup(d_params, B, h->an, pts, d_status, s));
+  HIPCHK(launch_het_ll(pts, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_phase, h->d_wf, h->M, h->logLb, d_logL, s));
+  return QCBC_OK;
+}
+
+int qcbc::het_device(const qcbc_het* h) { return h->device; }
 
 extern "C" {
 

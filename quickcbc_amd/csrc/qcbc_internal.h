@@ -24,8 +24,9 @@ hipError_t launch_ll(int precision, const LLInst& in, int cpb, int tile, hipStre
 hipError_t launch_ll_queue(int precision, const LLInst* d_insts, const LLItem* d_items, int n_items, int* d_counter,
                            int max_nd, int cpb, int tile, int blocks, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s);
-// partial sums as written by: the plain path (raw), k_ll / k_ll_queue FP64, mixed
-enum { FIN_RAW = 0, FIN_FP64 = 1, FIN_MIXED = 2 };
+// partial sums as written by: the plain path (raw), k_ll / k_ll_queue FP64,
+// mixed; FIN_DONE: logL is already final (the heterodyned likelihood)
+enum { FIN_RAW = 0, FIN_FP64 = 1, FIN_MIXED = 2, FIN_DONE = 3 };
 hipError_t launch_finalize(const double* partial, int splits, int nd, const Point* pts, int B, int mode,
                            const double* wref, double* logL, double* rho, hipStream_t s);
 hipError_t launch_waveform(const Point* pts, int B, int nhalf, double Tobs, double* amp, double* phs, hipStream_t s);

@@ -982,7 +982,7 @@ __global__ __launch_bounds__(1024) void k_pt_post(
   for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
     const int lp = t * ncols + b;
     const uint64_t gp = (uint64_t)t * n_chains + c;
-    finalize_point(partial, splits, nd, pts[lp], lp, mode, wref, Ly, rho);
+    if (mode != FIN_DONE) finalize_point(partial, splits, nd, pts[lp], lp, mode, wref, Ly, rho);
     const double u = u01(seed ^ 0x5bd1e995ULL, s * P + gp, 99);
     const double* q = y + 11 * lp;
     if (st_y[lp] == 0 && in_prior(q) && log(u) < tc[3 * t + 1] * (Ly[lp] - L[lp])) {

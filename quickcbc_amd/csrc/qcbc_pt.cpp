@@ -39,6 +39,7 @@ struct Group {
 
 struct qcbc_pt {
   qcbc_ctx* ctx = nullptr;
+  qcbc_het* het = nullptr;  // the heterodyned likelihood instead of the full one
   int n_temps = 0, n_chains = 0, P = 0, precision = 0, G = 1;
   double ladder = 1.0;
   uint64_t seed = 0;
@@ -76,6 +77,17 @@ static int group_step(qcbc_pt* pt, int k) {
   Group& gr = pt->g[k];
   const int prec = pt->precision;
   const size_t o = gr.off;
+  if (pt->het) {  // set-up and heterodyned logL of y, then the same post kernel
+    int st = het_enqueue(pt->het, pt->d_y + 11 * o, gr.P, gr.d_pts, pt->d_st + o, pt->d_Ly + o, gr.s);
+    if (st) return st;
+    LLInst in{};
+    in.pts = gr.d_pts;
+    in.nd = c->desc.n_det;
+    HIPCHK(launch_pt_post(in, FIN_DONE, c->d_wref, pt->d_x + 11 * o, pt->d_L + o, pt->d_y + 11 * o, pt->d_Ly + o,
+                          pt->d_rho + 3 * o, pt->d_st + o, pt->d_sigma, pt->d_tc, pt->n_temps, pt->n_chains, gr.c0,
+                          gr.ncols, pt->seed, gr.d_step, pt->d_acc + o, pt->d_swaps, gr.d_done, gr.s));
+    return QCBC_OK;
+  }
   HIPCHK(launch_setup(pt->d_y + 11 * o, gr.P, c->an, gr.d_pts, pt->d_st + o, gr.s));
   const LLInst in = make_inst(c, c->tile[prec], gr.P, effective_splits(c, prec, pt->P), gr.d_pts, gr.d_partial);
   int st = ll_launch_timed(c, prec, in, gr.s);
@@ -119,18 +131,20 @@ static void pt_free(qcbc_pt* pt) {
 
 extern "C" {
 
-int qcbc_pt_create_groups(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init,
-                          const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out) {
+static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, double ladder, const double* init,
+                     const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out) {
   if (!ctx || !out || !init || !sigma || n_temps < 1 || n_chains < 1 || n_chains > 1024 || !(ladder >= 1.0) ||
       (precision != QCBC_FP64 && precision != QCBC_MIXED) || groups < 0 || groups > QCBC_PT_MAX_GROUPS ||
       groups > n_chains)
     return QCBC_EINVAL;
-  if (!ctx->bound) return QCBC_ENOTBOUND;
+  if (het && het_device(het) != ctx->device) return QCBC_EINVAL;
+  if (!het && !ctx->bound) return QCBC_ENOTBOUND;
   *out = nullptr;
   std::lock_guard<std::mutex> lk(ctx->mu);
   hipSetDevice(ctx->device);
   qcbc_pt* pt = new qcbc_pt();
   pt->ctx = ctx;
+  pt->het = het;
   pt->n_temps = n_temps;
   pt->n_chains = n_chains;
   pt->P = n_temps * n_chains;
@@ -202,7 +216,15 @@ int qcbc_pt_create_groups(qcbc_ctx* ctx, int n_temps, int n_chains, double ladde
     if (launch_fill_u64(pt->g[k].d_step, 1, 0ULL, s) != hipSuccess ||
         launch_fill_i32(reinterpret_cast<int*>(pt->g[k].d_done), 1, 0, s) != hipSuccess)
       return fail(QCBC_EHIP);
-  int st = loglike_core(ctx, pt->d_x, pt->P, precision, pt->d_L, pt->d_rho, pt->d_st);
+  int st = QCBC_OK;
+  if (het) {  // each group's points on the group's own set-up records
+    for (int k = 0; k < pt->G && !st; k++) {
+      const Group& gr = pt->g[k];
+      st = het_enqueue(het, pt->d_x + 11 * (size_t)gr.off, gr.P, gr.d_pts, pt->d_st + gr.off, pt->d_L + gr.off, s);
+    }
+  } else {
+    st = loglike_core(ctx, pt->d_x, pt->P, precision, pt->d_L, pt->d_rho, pt->d_st);
+  }
   if (st) return fail(st);
   // proposals of step 0
   for (int k = 0; k < pt->G; k++) {
@@ -214,6 +236,17 @@ int qcbc_pt_create_groups(qcbc_ctx* ctx, int n_temps, int n_chains, double ladde
   if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
   *out = pt;
   return QCBC_OK;
+}
+
+int qcbc_pt_create_groups(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init,
+                          const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out) {
+  return pt_create(ctx, nullptr, n_temps, n_chains, ladder, init, sigma, seed, precision, groups, out);
+}
+
+int qcbc_pt_create_het(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, double ladder, const double* init,
+                       const double* sigma, uint64_t seed, int groups, qcbc_pt** out) {
+  if (!het) return QCBC_EINVAL;
+  return pt_create(ctx, het, n_temps, n_chains, ladder, init, sigma, seed, QCBC_FP64, groups, out);
 }
 
 int qcbc_pt_create(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init, const double* sigma,

@@ -368,10 +368,12 @@ def dropin_fbegin(params):
 class PT:
     """Resident parallel-tempered ensemble (qcbc_pt)."""
 
-    def __init__(self, ctx, n_temps, n_chains, ladder, init, sigma, seed=1, precision=FP64, groups=0):
+    def __init__(self, ctx, n_temps, n_chains, ladder, init, sigma, seed=1, precision=FP64, groups=0, het=None):
         L = lib()
         L.qcbc_pt_create_groups.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                             ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.qcbc_pt_create_het.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                         ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]
         L.qcbc_pt_groups.argtypes = [_vp, _ip]
         L.qcbc_pt_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.qcbc_pt_get.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
@@ -383,8 +385,13 @@ class PT:
         init = np.ascontiguousarray(init, np.float64).reshape(self.P, 11)
         sigma = np.ascontiguousarray(sigma, np.float64).reshape(11)
         h = _vp()
-        _check(L.qcbc_pt_create_groups(ctx.h, n_temps, n_chains, ladder, _d(init), _d(sigma), seed, precision,
-                                       groups, ctypes.byref(h)), "qcbc_pt_create_groups")
+        self.het = het
+        if het is None:
+            _check(L.qcbc_pt_create_groups(ctx.h, n_temps, n_chains, ladder, _d(init), _d(sigma), seed, precision,
+                                           groups, ctypes.byref(h)), "qcbc_pt_create_groups")
+        else:
+            _check(L.qcbc_pt_create_het(ctx.h, het.h, n_temps, n_chains, ladder, _d(init), _d(sigma), seed, groups,
+                                        ctypes.byref(h)), "qcbc_pt_create_het")
         self.h = h
         g = ctypes.c_int()
         _check(L.qcbc_pt_groups(h, ctypes.byref(g)), "qcbc_pt_groups")

@@ -238,3 +238,30 @@ def test_device_entry_points(setup):
     assert g.time_kernel(dp.ptr, P.shape[0], 5) > 0
     dp.free()
     dl.free()
+
+
+def test_pt_on_the_heterodyned_likelihood(setup):
+    """The resident PT step on the heterodyned likelihood: one column group,
+    three groups and the graph replay give the same chains, and the
+    ensemble's logL is the heterodyned logL of its points (GPU and oracle)."""
+    which, an, inj, D, SN, h, c, g = setup
+    nt, nc = 8, 48
+    init = W.ensemble_around(inj, nt * nc, seed=5, scale=0.3)
+    sigma = np.array([2e-5, 1e-4, 2e-3, 2e-3, 0.05, 2e-5, 0.01, 0.01, 0.01, 0.02, 0.01]) * 0.3
+    one = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=1, het=g)
+    three = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=3, het=g)
+    gr = Q.PT(c, nt, nc, 1.2, init, sigma, seed=13, groups=3, het=g)
+    one.run(40)
+    three.run(40)
+    gr.run(40, graph=True)
+    r1, r3, rg = one.get(), three.get(), gr.get()
+    for r in (r3, rg):
+        assert np.array_equal(r1[0], r[0]) and np.array_equal(r1[1], r[1])
+        assert np.array_equal(r1[2], r[2]) and r1[3] == r[3]
+    assert r1[2].sum() > 0
+    L, s = g.loglike(r1[0])
+    assert (s == 0).all() and np.allclose(L, r1[1], rtol=1e-12, atol=1e-9)
+    Lo, _ = _data.oracle_loglike_het(which, r1[0])
+    assert (np.abs(r1[1] - Lo) <= 1e-8 + 1e-10 * np.abs(Lo - h["logLb"])).all()
+    for pt in (one, three, gr):
+        pt.close()
