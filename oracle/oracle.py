@@ -289,3 +289,37 @@ def loglike_het(labels, gmst, fref, Tobs, het, params, nthreads=0):
     L.qo_loglike_het_batch_flat(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"],
                                 *[_d(a) for a in arr], het["logLb"], _d(p), B, _d(logL), _i(st), nthreads)
     return logL, st
+
+
+def fisher_het(labels, gmst, fref, Tobs, het, params, nthreads=0):
+    """Fisher_Het (QuickCBC.c:5446) for a batch: (fish B x 11 x 11, status)."""
+    L = lib()
+    L.qo_fisher_het_batch_flat.restype = None
+    L.qo_fisher_het_batch_flat.argtypes = ([ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_dp] * 7 +
+                                           [ctypes.c_int, _dp, _ip, ctypes.c_int])
+    labels = np.ascontiguousarray(labels, np.int32)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+    B = p.shape[0]
+    fish = np.zeros((B, 11, 11))
+    st = np.zeros(B, np.int32)
+    arr = [np.ascontiguousarray(het[k], np.float64) for k in ("freq", "amp", "phase", "IP", "SL", "aa")]
+    L.qo_fisher_het_batch_flat(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"],
+                               *[_d(a) for a in arr[:5]], _d(arr[5]), _d(p), B, _d(fish), _i(st), nthreads)
+    return fish, st
+
+
+def fisher_evec(fish):
+    """FisherEvec (QuickCBC.c:2449) for a batch of d x d matrices: (ej B x d, ev B x d x d, sweeps)."""
+    L = lib()
+    L.qo_fisher_evec_batch.restype = None
+    L.qo_fisher_evec_batch.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+    F = np.ascontiguousarray(fish, np.float64)
+    d = F.shape[-1]
+    F = F.reshape(-1, d, d)
+    B = F.shape[0]
+    ej = np.zeros((B, d))
+    ev = np.zeros((B, d, d))
+    sw = np.zeros(B, np.int32)
+    L.qo_fisher_evec_batch(_d(F), B, d, _d(ej), _d(ev), _i(sw))
+    return ej, ev, sw

@@ -1432,3 +1432,250 @@ void qo_loglike_het_batch_flat(int nifo, const int *labels, double gmst, double 
     if (status) status[b] = st;
   }
 }
+
+/* Fisher_Het, QuickCBC.c:5446-5617: the Fisher matrix of the heterodyned
+ * likelihood at params, NP = 11.  Central differences of fullphaseamp on the
+ * grid with epsilon = 1e-5 in every parameter; per pair (i, j) and detector
+ * the slow term cc = 4 (dA_i dA_j + A^2 dphi_i dphi_j) / a_ref^2 on the
+ * grid, its Legendre coefficients uu = IP vv per region, summed against SL,
+ * less the overcount of the region boundaries 1 .. NR-2 (aa).  With
+ * |chi1| or |chi2| >= 0.9999 the matrix is 1e4 times the identity.  fish is
+ * row-major 11 x 11.  Returns 0 or the first failing fullphaseamp status. */
+int qo_fisher_het(int nifo, const int *labels, double gmst, double fref, double Tobs, const qo_het *h,
+                  const double *params, double *fish) {
+  enum { NP = 11 };
+  const int M = h->M, J = h->J, NR = h->NR;
+  const double epsilon = 1.0e-5;
+  double *amp, *phase, *ampP, *phaseP, *ampM, *phaseM, *cc;
+  double *ar[3], *pr[3], uu[9], vv[9], paramsP[NP], paramsM[NP];
+  int i, j, k, ii, jj, kk, id, st = 0;
+  for (i = 0; i < NP; i++)
+    for (j = 0; j < NP; j++) fish[j * NP + i] = 0.0;
+  for (i = 0; i < NP; i++) fish[i * NP + i] = 1.0e4;
+  if (!(fabs(params[2]) < 0.9999 && fabs(params[3]) < 0.9999)) return 0;
+  amp = malloc(sizeof(double) * nifo * M);
+  phase = malloc(sizeof(double) * nifo * M);
+  ampP = malloc(sizeof(double) * NP * nifo * M);
+  phaseP = malloc(sizeof(double) * NP * nifo * M);
+  ampM = malloc(sizeof(double) * NP * nifo * M);
+  phaseM = malloc(sizeof(double) * NP * nifo * M);
+  cc = malloc(sizeof(double) * nifo * M);
+  if (!amp || !phase || !ampP || !phaseP || !ampM || !phaseM || !cc) {
+    st = QO_ENOMEM;
+    goto out;
+  }
+  for (id = 0; id < nifo; id++) {
+    ar[id] = amp + (size_t)id * M;
+    pr[id] = phase + (size_t)id * M;
+  }
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, h->freq, params, 2 * M);
+  for (i = 0; i < NP && st == 0; i++) {
+    double *aP[3], *pP[3], *aM[3], *pM[3];
+    for (k = 0; k < NP; k++) {
+      paramsP[k] = params[k];
+      paramsM[k] = params[k];
+    }
+    paramsP[i] += epsilon;
+    paramsM[i] -= epsilon;
+    for (id = 0; id < nifo; id++) {
+      aP[id] = ampP + ((size_t)i * nifo + id) * M;
+      pP[id] = phaseP + ((size_t)i * nifo + id) * M;
+      aM[id] = ampM + ((size_t)i * nifo + id) * M;
+      pM[id] = phaseM + ((size_t)i * nifo + id) * M;
+    }
+    st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, aP, pP, h->freq, paramsP, 2 * M);
+    if (st == 0) st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, aM, pM, h->freq, paramsM, 2 * M);
+    if (st) break;
+    /* the central difference, kept in the plus arrays */
+    for (id = 0; id < nifo; id++)
+      for (k = 0; k < M; k++) {
+        aP[id][k] -= aM[id][k];
+        pP[id][k] -= pM[id][k];
+        aP[id][k] /= (2.0 * epsilon);
+        pP[id][k] /= (2.0 * epsilon);
+      }
+  }
+  if (st == 0) {
+    for (i = 0; i < NP; i++)
+      for (j = 0; j < NP; j++) fish[j * NP + i] = 0.0;
+    for (i = 0; i < NP; i++)
+      for (j = i; j < NP; j++)
+        for (id = 0; id < nifo; id++) {
+          const double *dAi = ampP + ((size_t)i * nifo + id) * M, *dAj = ampP + ((size_t)j * nifo + id) * M;
+          const double *dPi = phaseP + ((size_t)i * nifo + id) * M, *dPj = phaseP + ((size_t)j * nifo + id) * M;
+          double HH = 0.0, x = 0.0;
+          for (jj = 0; jj < M; jj++) {
+            const double ha = h->amp[id * M + jj];
+            cc[id * M + jj] = 4.0 * (dAi[jj] * dAj[jj] + ar[id][jj] * ar[id][jj] * dPi[jj] * dPj[jj]);
+            cc[id * M + jj] /= (ha * ha);
+          }
+          for (ii = 0; ii < NR; ii++) {
+            const double *IPi = h->IP + (size_t)ii * (J + 1) * (J + 1);
+            const double *SLr = h->SL + ((size_t)ii * nifo + id) * (J + 1);
+            for (jj = 0; jj <= J; jj++) {
+              vv[jj] = cc[id * M + ii * J + jj];
+              uu[jj] = 0.0;
+            }
+            for (jj = 0; jj <= J; jj++)
+              for (kk = 0; kk <= J; kk++) uu[jj] += IPi[jj * (J + 1) + kk] * vv[kk];
+            for (jj = 0; jj <= J; jj++) HH += uu[jj] * SLr[jj];
+            if (ii < NR - 2) x += h->aa[id * NR + ii + 1] * vv[J]; /* correction for overcount */
+          }
+          HH -= x;
+          fish[i * NP + j] += HH;
+        }
+    for (i = 0; i < NP; i++)
+      for (j = i + 1; j < NP; j++) fish[j * NP + i] = fish[i * NP + j];
+  }
+out:
+  free(amp);
+  free(phase);
+  free(ampP);
+  free(phaseP);
+  free(ampM);
+  free(phaseM);
+  free(cc);
+  return st;
+}
+
+/* FisherEvec, QuickCBC.c:2449-2546: eigenvalues ej and eigenvectors ev (row
+ * i = eigenvector i) of the symmetric d x d matrix fish, ordered by |ej|
+ * ascending (GSL_EIGEN_SORT_ABS_ASC), then turned into 1-sigma jump sizes
+ * ej = 1/sqrt(max(ej, 10)).  A diagonal entry below 1e-16 in magnitude gives
+ * ej = 1e4 and ev = I.  The reference decomposes with GSL's eigen_symmv
+ * (Householder tridiagonalisation and implicit QR); GSL is absent here, so
+ * this is the cyclic Jacobi method (Golub & Van Loan, Matrix Computations,
+ * Alg. 8.5.3, row-cyclic order), which converges to the same eigenpairs:
+ * the eigenvalues to rounding, the eigenvectors to rounding over the gap
+ * and up to sign.  GSL's sign is whatever its rotations leave; here each
+ * eigenvector is scaled so its largest-magnitude component (the first on a
+ * tie) is positive.  A rotation is skipped once |a_pq| <= 2^-53 sqrt(|a_pp
+ * a_qq|) (and a_pq set to 0), the relative criterion of Demmel and Veselic
+ * (1992); sweeps stop when one performs no rotation, at most 60.  d <= 16. */
+int qo_fisher_evec(const double *fish, double *ej, double *ev, int d) {
+  double a[256], v[256], lam[16];
+  int idx[16];
+  int i, j, k, p, q, sweep, ec = 0, sweeps = 0;
+  for (i = 0; i < d; i++)
+    if (fabs(fish[i * d + i]) < 1.0e-16) ec = 1;
+  if (ec) {
+    for (i = 0; i < d; i++) {
+      ej[i] = 10000.0;
+      for (j = 0; j < d; j++) ev[i * d + j] = (i == j) ? 1.0 : 0.0;
+    }
+    return 0;
+  }
+  for (i = 0; i < d * d; i++) a[i] = fish[i];
+  for (i = 0; i < d; i++)
+    for (j = 0; j < d; j++) v[i * d + j] = (i == j) ? 1.0 : 0.0;
+  for (sweep = 0; sweep < 60; sweep++) {
+    int rot = 0;
+    for (p = 0; p < d - 1; p++)
+      for (q = p + 1; q < d; q++) {
+        const double apq = a[p * d + q], app = a[p * d + p], aqq = a[q * d + q];
+        double tau, t, c, s;
+        if (fabs(apq) <= 0x1p-53 * sqrt(fabs(app * aqq))) {
+          a[p * d + q] = a[q * d + p] = 0.0;
+          continue;
+        }
+        rot++;
+        /* the symmetric Schur rotation that zeroes a_pq (GvL Alg. 8.5.1) */
+        tau = (aqq - app) / (2.0 * apq);
+        if (fabs(tau) > 1.0e150)
+          t = 0.5 / tau;
+        else
+          t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = t * c;
+        for (k = 0; k < d; k++) {
+          if (k == p || k == q) continue;
+          {
+            const double akp = a[k * d + p], akq = a[k * d + q];
+            const double np = c * akp - s * akq, nq = s * akp + c * akq;
+            a[k * d + p] = a[p * d + k] = np;
+            a[k * d + q] = a[q * d + k] = nq;
+          }
+        }
+        a[p * d + p] = app - t * apq;
+        a[q * d + q] = aqq + t * apq;
+        a[p * d + q] = a[q * d + p] = 0.0;
+        for (k = 0; k < d; k++) {
+          const double vkp = v[k * d + p], vkq = v[k * d + q];
+          v[k * d + p] = c * vkp - s * vkq;
+          v[k * d + q] = s * vkp + c * vkq;
+        }
+      }
+    sweeps = sweep + 1;
+    if (rot == 0) break;
+  }
+  /* order by |lambda| ascending, stable */
+  for (i = 0; i < d; i++) {
+    lam[i] = a[i * d + i];
+    idx[i] = i;
+  }
+  for (i = 1; i < d; i++) {
+    int t = idx[i];
+    j = i - 1;
+    while (j >= 0 && fabs(lam[idx[j]]) > fabs(lam[t])) {
+      idx[j + 1] = idx[j];
+      j--;
+    }
+    idx[j + 1] = t;
+  }
+  for (i = 0; i < d; i++) {
+    const int col = idx[i];
+    int kmax = 0;
+    double vmax = -1.0, sg;
+    for (k = 0; k < d; k++)
+      if (fabs(v[k * d + col]) > vmax) {
+        vmax = fabs(v[k * d + col]);
+        kmax = k;
+      }
+    sg = v[kmax * d + col] < 0.0 ? -1.0 : 1.0;
+    ej[i] = lam[col];
+    for (k = 0; k < d; k++) ev[i * d + k] = sg * v[k * d + col];
+  }
+  for (i = 0; i < d; i++) {
+    if (ej[i] < 10.0) ej[i] = 10.0; /* no eigenvalue too small */
+    ej[i] = 1.0 / sqrt(ej[i]);      /* 1-sigma jump amplitudes */
+  }
+  return sweeps;
+}
+
+/* Flat conveniences: a batch of B Fisher matrices (status per point) and
+ * their eigen-decompositions. */
+void qo_fisher_het_batch_flat(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
+                              const double *freq, const double *amp, const double *phase, const double *IP,
+                              const double *SL, const double *aa, const double *params, int B, double *fish,
+                              int *status, int nthreads) {
+  qo_het h;
+  int b;
+  qo_spline_init();
+  memset(&h, 0, sizeof h);
+  h.M = M;
+  h.J = J;
+  h.NR = (M - 1) / J;
+  h.nifo = nifo;
+  h.freq = freq;
+  h.amp = (double *)amp;
+  h.phase = (double *)phase;
+  h.IP = (double *)IP;
+  h.SL = (double *)SL;
+  h.aa = (double *)aa;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (b = 0; b < B; b++) {
+    int st = qo_fisher_het(nifo, labels, gmst, fref, Tobs, &h, params + 11 * (size_t)b, fish + 121 * (size_t)b);
+    if (status) status[b] = st;
+  }
+}
+
+void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *ev, int *sweeps) {
+  int b;
+  for (b = 0; b < B; b++) {
+    int s = qo_fisher_evec(fish + (size_t)b * d * d, ej + (size_t)b * d, ev + (size_t)b * d * d, d);
+    if (sweeps) sweeps[b] = s;
+  }
+}

@@ -128,6 +128,16 @@ void qo_loglike_het_batch_flat(int nifo, const int *labels, double gmst, double 
                                const double *params, int B, double *logL, int *status,
                                int nthreads);
 
+/* Fisher_Het (QuickCBC.c:5446-5617) and FisherEvec (QuickCBC.c:2449-2546):
+ * fish row-major 11 x 11 per point; ej d, ev d x d (row i = eigenvector i). */
+void qo_fisher_het_batch_flat(int nifo, const int *labels, double gmst, double fref, double Tobs,
+                              int M, int J, const double *freq, const double *amp,
+                              const double *phase, const double *IP, const double *SL,
+                              const double *aa, const double *params, int B, double *fish,
+                              int *status, int nthreads);
+int qo_fisher_evec(const double *fish, double *ej, double *ev, int d);
+void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *ev, int *sweeps);
+
 #ifdef __cplusplus
 }
 #endif
