@@ -138,6 +138,23 @@ void heterodyne(struct Net *net, struct Het *het, double **D, double *params, Re
  * call qcbc_invalidate() after changing its arrays in place. */
 double log_likelihood_het(struct Net *net, struct Het *het, double *params, double Tobs);
 
+/* Replaces Fisher_Het, QuickCBC.c:5446-5617 (prototype QuickCBC.h:151): the
+ * Fisher matrix of the heterodyned likelihood at params, central differences
+ * of fullphaseamp with epsilon 1e-5 on het's grid.  fish: 11 rows of 11.
+ * With |chi1| or |chi2| >= 0.9999, 1e4 times the identity.  het is found or
+ * imported as log_likelihood_het finds it. */
+void Fisher_Het(struct Net *net, struct Het *het, double **fish, double *params, double Tobs);
+
+/* Replaces FisherEvec, QuickCBC.c:2449-2546 (prototype QuickCBC.h:164): the
+ * eigen-decomposition of the symmetric d x d matrix fish (d <= 11) ordered by
+ * |eigenvalue| ascending, eval[i] = 1/sqrt(max(lambda_i, 10)) (the 1-sigma
+ * jump along eigenvector i) and evec[i][j] its component j.  GSL's
+ * eigen_symmv is replaced by the cyclic Jacobi method: the same eigenpairs
+ * to rounding, each eigenvector signed so that its largest-magnitude
+ * component is positive.  A diagonal entry below 1e-16 gives 1e4 and the
+ * unit vectors, as in the reference. */
+void FisherEvec(double **fish, double *eval, double **evec, int d);
+
 /* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
 AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
 void DestroyAmpPhaseFDWaveform(AmpPhaseFDWaveform *wf);
@@ -383,6 +400,28 @@ int qcbc_het_timing(qcbc_het *het, int enable, double *kernel_ms, long long *lau
 /* The evaluation kernel alone: B points (device memory) set up once, then
  * reps launches back to back between two HIP events; the mean per launch. */
 int qcbc_het_time_kernel(qcbc_het *het, const double *d_params, int B, int reps, double *ms_per_launch);
+/* Fisher_Het (QuickCBC.c:5446-5617) at B points (B x 11, host memory) on
+ * the heterodyne: fish B x 11 x 11 row-major.  When ej and ev are both given,
+ * also FisherEvec (QuickCBC.c:2449-2546) of each matrix on the GPU: ej B x 11
+ * jump sizes, ev B x 11 x 11 (row i = eigenvector i, as qcbc_fisher_evec).
+ * status[b] (may be NULL) is 0 or QCBC_EDOM (a perturbed point outside the
+ * model; the matrix is then 1e4 I). */
+int qcbc_het_fisher(qcbc_het *het, const double *params, int B, double *fish, double *ej, double *ev,
+                    int *status);
+/* The same on device memory, asynchronous on the heterodyne's stream. */
+int qcbc_het_fisher_device(qcbc_het *het, const double *d_params, int B, double *d_fish, double *d_ej,
+                           double *d_ev, int *d_status);
+/* Kernel timing of the Fisher matrix for B points (device memory): mean ms
+ * per launch over reps back-to-back launches of each stage, ms[0] the set-up
+ * of the 23 B perturbed points, ms[1] the Fisher kernel, ms[2] the eigen
+ * kernel. */
+int qcbc_het_fisher_time(qcbc_het *het, const double *d_params, int B, int reps, double *ms);
+/* FisherEvec (QuickCBC.c:2449-2546) of B symmetric d x d matrices (d <= 11,
+ * host memory, row-major) on the analysis's GPU: ej B x d, ev B x d x d
+ * (row i = eigenvector i), ordered by |eigenvalue| ascending; cyclic Jacobi
+ * in place of GSL's eigen_symmv, each eigenvector signed so that its
+ * largest-magnitude component is positive. */
+int qcbc_fisher_evec(qcbc_ctx *ctx, const double *fish, int B, int d, double *ej, double *ev);
 int qcbc_het_destroy(qcbc_het *het);
 
 #ifdef __cplusplus

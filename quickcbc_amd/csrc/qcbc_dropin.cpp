@@ -18,6 +18,7 @@
 
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
+#include "qcbc_fisher_internal.h"
 #include "qcbc_het_internal.h"
 #include "qcbc_internal.h"
 
@@ -488,60 +489,98 @@ void heterodyne(struct Net* net, struct Het* het, double** D, double* params, Re
   het_cache_put(e);
 }
 
-double log_likelihood_het(struct Net* net, struct Het* het, double* params, double Tobs) {
-  (void)Tobs;  // fullphaseamp takes Tobs from net (QuickCBC.c:3799)
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !params || !het->IP || !het->SL || !het->lc || !het->ls ||
+// The GPU heterodyne of a struct Het: the cached one heterodyne() made, or
+// one made by the reference's own set-up, flattened and imported on first use
+// (samples beyond NR J + 1 are never read, QuickCBC.c:5358-5417)
+static qcbc_het* het_for(struct Net* net, struct Het* het, const char* where) {
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !het->IP || !het->SL || !het->lc || !het->ls ||
       !het->amp || !het->phase || !het->rc || !het->rs || !het->aa || !het->freq || !het->freq->data)
-    fail("log_likelihood_het", QCBC_EINVAL);
+    fail(where, QCBC_EINVAL);
   const int M = het->M, J = het->J, NR = het->NR, nd = net->Nifo;
-  if (J < 1 || NR < 1 || M < NR * J + 1 || het->freq->length < (size_t)M) fail("log_likelihood_het", QCBC_EINVAL);
+  if (J < 1 || NR < 1 || M < NR * J + 1 || het->freq->length < (size_t)M) fail(where, QCBC_EINVAL);
   HetEntry key = key_of(het);
-  qcbc_het* h = nullptr;
   for (auto& e : t_hets)
     if (same_key(e, key)) {
       e.last_use = ++t_drop.tick;
-      h = e.h;
-      break;
+      return e.h;
     }
-  if (!h) {
-    // a heterodyne made by the reference's own set-up: flatten and import it
-    // (samples beyond NR J + 1 are never read, QuickCBC.c:5358-5417)
-    const int Mu = NR * J + 1;
-    const size_t J1 = J + 1;
-    qcbc_desc desc = desc_from(net, 2 * Mu < 4 ? 4 : 2 * Mu, net->Tobs);
-    int st;
-    qcbc_ctx* ctx = get_ctx(desc, nullptr, nullptr, &st);
-    if (!ctx) fail("log_likelihood_het", st);
-    std::vector<double> amp((size_t)nd * Mu), phase((size_t)nd * Mu), IP(NR * J1 * J1), SL(NR * nd * J1),
-        lc(NR * nd * J1), ls(NR * nd * J1), rc((size_t)nd * NR), rs((size_t)nd * NR), aa((size_t)nd * NR);
-    for (int d = 0; d < nd; d++) {
-      std::memcpy(amp.data() + (size_t)d * Mu, het->amp[d], sizeof(double) * Mu);
-      std::memcpy(phase.data() + (size_t)d * Mu, het->phase[d], sizeof(double) * Mu);
-      std::memcpy(rc.data() + (size_t)d * NR, het->rc[d], sizeof(double) * NR);
-      std::memcpy(rs.data() + (size_t)d * NR, het->rs[d], sizeof(double) * NR);
-      std::memcpy(aa.data() + (size_t)d * NR, het->aa[d], sizeof(double) * NR);
-    }
-    for (int i = 0; i < NR; i++) {
-      for (size_t j = 0; j < J1; j++) std::memcpy(IP.data() + (i * J1 + j) * J1, het->IP[i][j], sizeof(double) * J1);
-      for (int d = 0; d < nd; d++) {
-        const size_t o = ((size_t)i * nd + d) * J1;
-        std::memcpy(SL.data() + o, het->SL[i][d], sizeof(double) * J1);
-        std::memcpy(lc.data() + o, het->lc[i][d], sizeof(double) * J1);
-        std::memcpy(ls.data() + o, het->ls[i][d], sizeof(double) * J1);
-      }
-    }
-    st = qcbc_het_import(ctx, Mu, J, het->freq->data, amp.data(), phase.data(), IP.data(), SL.data(), lc.data(),
-                         ls.data(), rc.data(), rs.data(), aa.data(), het->logLb, &h);
-    if (st) fail("log_likelihood_het", st);
-    key.h = h;
-    het_cache_put(key);
+  const int Mu = NR * J + 1;
+  const size_t J1 = J + 1;
+  qcbc_desc desc = desc_from(net, 2 * Mu < 4 ? 4 : 2 * Mu, net->Tobs);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc, nullptr, nullptr, &st);
+  if (!ctx) fail(where, st);
+  std::vector<double> amp((size_t)nd * Mu), phase((size_t)nd * Mu), IP(NR * J1 * J1), SL(NR * nd * J1),
+      lc(NR * nd * J1), ls(NR * nd * J1), rc((size_t)nd * NR), rs((size_t)nd * NR), aa((size_t)nd * NR);
+  for (int d = 0; d < nd; d++) {
+    std::memcpy(amp.data() + (size_t)d * Mu, het->amp[d], sizeof(double) * Mu);
+    std::memcpy(phase.data() + (size_t)d * Mu, het->phase[d], sizeof(double) * Mu);
+    std::memcpy(rc.data() + (size_t)d * NR, het->rc[d], sizeof(double) * NR);
+    std::memcpy(rs.data() + (size_t)d * NR, het->rs[d], sizeof(double) * NR);
+    std::memcpy(aa.data() + (size_t)d * NR, het->aa[d], sizeof(double) * NR);
   }
+  for (int i = 0; i < NR; i++) {
+    for (size_t j = 0; j < J1; j++) std::memcpy(IP.data() + (i * J1 + j) * J1, het->IP[i][j], sizeof(double) * J1);
+    for (int d = 0; d < nd; d++) {
+      const size_t o = ((size_t)i * nd + d) * J1;
+      std::memcpy(SL.data() + o, het->SL[i][d], sizeof(double) * J1);
+      std::memcpy(lc.data() + o, het->lc[i][d], sizeof(double) * J1);
+      std::memcpy(ls.data() + o, het->ls[i][d], sizeof(double) * J1);
+    }
+  }
+  qcbc_het* h = nullptr;
+  st = qcbc_het_import(ctx, Mu, J, het->freq->data, amp.data(), phase.data(), IP.data(), SL.data(), lc.data(),
+                       ls.data(), rc.data(), rs.data(), aa.data(), het->logLb, &h);
+  if (st) fail(where, st);
+  key.h = h;
+  het_cache_put(key);
+  return h;
+}
+
+double log_likelihood_het(struct Net* net, struct Het* het, double* params, double Tobs) {
+  (void)Tobs;  // fullphaseamp takes Tobs from net (QuickCBC.c:3799)
+  if (!params) fail("log_likelihood_het", QCBC_EINVAL);
+  qcbc_het* h = het_for(net, het, "log_likelihood_het");
   double logL = 0.0;
   int pst = 0;
   int st = qcbc_het_loglike(h, params, 1, &logL, &pst);
   if (st) fail("log_likelihood_het", st);
   if (domain_error("log_likelihood_het", pst)) return 0.0;
   return logL;
+}
+
+void Fisher_Het(struct Net* net, struct Het* het, double** fish, double* params, double Tobs) {
+  (void)Tobs;  // fullphaseamp takes Tobs from net (QuickCBC.c:3799)
+  if (!fish || !params) fail("Fisher_Het", QCBC_EINVAL);
+  qcbc_het* h = het_for(net, het, "Fisher_Het");
+  double F[11 * 11];
+  int pst = 0;
+  int st = qcbc_het_fisher(h, params, 1, F, nullptr, nullptr, &pst);
+  if (st) fail("Fisher_Het", st);
+  domain_error("Fisher_Het", pst);  // the matrix is then 1e4 I
+  for (int i = 0; i < 11; i++) std::memcpy(fish[i], F + 11 * i, sizeof(double) * 11);
+}
+
+void FisherEvec(double** fish, double* eval, double** evec, int d) {
+  if (!fish || !eval || !evec || d < 1 || d > FISHER_EVEC_MAX_D) fail("FisherEvec", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("FisherEvec", st);
+  DropState& s = t_drop;
+  const size_t dd = (size_t)d * d;
+  if ((st = drop_buf(2 * dd + d))) fail("FisherEvec", st);
+  double F[FISHER_EVEC_MAX_D * FISHER_EVEC_MAX_D], E[FISHER_EVEC_MAX_D * FISHER_EVEC_MAX_D], J[FISHER_EVEC_MAX_D];
+  for (int i = 0; i < d; i++) std::memcpy(F + (size_t)i * d, fish[i], sizeof(double) * d);
+  double *d_f = s.d_buf, *d_ev = d_f + dd, *d_ej = d_ev + dd;
+  if (hipMemcpyAsync(d_f, F, sizeof(double) * dd, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
+      launch_fisher_evec(d_f, 1, d, d_ej, d_ev, nullptr, s.stream) != hipSuccess ||
+      hipMemcpyAsync(E, d_ev, sizeof(double) * dd, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipMemcpyAsync(J, d_ej, sizeof(double) * d, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipStreamSynchronize(s.stream) != hipSuccess)
+    fail("FisherEvec", QCBC_EHIP);
+  for (int i = 0; i < d; i++) {
+    eval[i] = J[i];
+    std::memcpy(evec[i], E + (size_t)i * d, sizeof(double) * d);
+  }
 }
 
 // IMRPhenomD_internals.c:141-181

@@ -1,0 +1,347 @@
+// HIP kernels of the heterodyned Fisher matrix (Fisher_Het, QuickCBC.c:5446-5617)
+// and its eigen-decomposition into jump directions (FisherEvec,
+// QuickCBC.c:2449-2546), compiled with -ffp-contract=off: every per-sample
+// product and every rotation is the reference's double arithmetic, operation
+// for operation; only the sum over a heterodyne's samples is regrouped (one
+// weight per sample, as in the heterodyned likelihood).
+//
+// A Fisher matrix needs fullphaseamp at 23 parameter sets per point: the
+// point itself, then params[k] + 1e-5 and params[k] - 1e-5 for k = 0..10
+// (variant 2k+1 and 2k+2).  k_fisher_params writes them, the ordinary set-up
+// kernel turns them into records, and k_het_fisher takes one block per point:
+// per chunk of 64 grid samples, the 23 variants' h22 amplitude and phase
+// (one variant per wave), the 22 central differences per detector, then the
+// 66 (i <= j) sums over the chunk's samples.
+#include "qcbc_device.h"
+#include "qcbc_fisher_internal.h"
+#include "qcbc_het_device.h"
+
+namespace qcbc {
+
+namespace {
+
+constexpr int NP = FISHER_NP, NV = FISHER_NV;
+constexpr int NPAIR = NP * (NP + 1) / 2;  // 66 entries of the upper triangle
+constexpr int NQ = 4;                     // each pair's sum split over four sample quarters
+constexpr double EPS = 1.0e-5;            // Fisher_Het's epsilon (QuickCBC.c:5486)
+
+// The 23 parameter sets of each point: q[(b NV + v) 11 + k]
+__global__ void k_fisher_params(const double* __restrict__ p, int B, double* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * NV) return;
+  const int b = i / NV, v = i % NV;
+  double x[NP];
+#pragma unroll
+  for (int k = 0; k < NP; k++) x[k] = p[(size_t)b * NP + k];
+  if (v > 0) {
+    const int k = (v - 1) >> 1;
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+      if (j == k) x[j] = (v & 1) ? x[j] + EPS : x[j] - EPS;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) q[(size_t)i * NP + k] = x[k];
+}
+
+// (i, j) of upper-triangle entry n in row order
+__device__ __forceinline__ void pair_of(int n, int* i, int* j) {
+  int r = 0;
+  while (n >= NP - r) {
+    n -= NP - r;
+    r++;
+  }
+  *i = r;
+  *j = r + n;
+}
+
+// Fisher_Het for B points, one 256-thread block each.  pts holds NV records
+// per point (k_fisher_params' order), ha the reference amplitude (nd x M), w
+// the raw evaluation weights, whose first nd x M are wH: sample m of region
+// i enters HH with sum_j IP[i][j][k] SL[i][d][j], less aa[d][i] at the
+// region starts 1 .. NR-2 (log_likelihood_het's and Fisher_Het's overcount
+// correction).  The slow term per sample is the reference's
+//   cc = 4 (dA_i dA_j + A^2 dphi_i dphi_j) / (a_ref a_ref),
+// so fish[i][j] = sum_d sum_m wH cc, evaluated as sum (wH / a_ref^2) 4 (...):
+// one division per sample instead of one per sample and entry.  With
+// |chi1| or |chi2| >= 0.9999 the
+// matrix is 1e4 I and nothing is evaluated (:5467-5469); a record the set-up
+// rejected leaves 1e4 I and its status (the first failing variant's, in
+// Fisher_Het's order of evaluation).
+template <int ND>
+__global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pts, const double* __restrict__ params,
+                                                    int B, const double* __restrict__ freq,
+                                                    const double* __restrict__ pl, const double* __restrict__ ha,
+                                                    const double* __restrict__ w, int M, double* __restrict__ fish,
+                                                    int* __restrict__ status) {
+  __shared__ double V[NV][3][64];       // h22 amplitude, phase, taper per variant and sample
+  __shared__ double U[ND][2 * NP][64];  // dA_k (k < 11) and dphi_k per detector and sample
+  __shared__ double S[ND][2][64];       // A^2 and wH / a_ref^2 per detector and sample
+  __shared__ double red[NQ][NPAIR];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
+  if (b >= B) return;
+  const Point* P = pts + (size_t)b * NV;
+  int st = ST_OK;
+  for (int v = 0; v < NV && st == ST_OK; v++) st = P[v].status;
+  const bool spins = fabs(params[(size_t)b * NP + 2]) < 0.9999 && fabs(params[(size_t)b * NP + 3]) < 0.9999;
+  double* F = fish + (size_t)b * NP * NP;
+  if (!spins || st != ST_OK) {  // block-uniform
+    if (t < NP * NP) F[t] = (t % (NP + 1) == 0) ? 1.0e4 : 0.0;
+    if (t == 0) status[b] = spins ? st : ST_OK;
+    return;
+  }
+  // the pair sums this thread owns: items t and t + 256 of NQ x NPAIR
+  int pi0, pj0, pi1 = 0, pj1 = 0;
+  const int q0 = t / NPAIR, q1 = (t + 256) / NPAIR;
+  pair_of(t % NPAIR, &pi0, &pj0);
+  const bool two = t + 256 < NQ * NPAIR;
+  if (two) pair_of((t + 256) % NPAIR, &pi1, &pj1);
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int m0 = 0; m0 < M; m0 += 64) {
+    // the 23 variants' h22 amplitude and phase at the chunk's samples
+    for (int v = wave; v < NV; v += 4) {
+      const Point& c = P[v];
+      const int m = m0 + lane;
+      double a = 0.0, p = 0.0, y = 0.0;
+      if (m < M) {
+        const double f = freq[m];
+        h22_planes(c, 1.0 / c.Msec, f, pl, M, m, &a, &p);
+        y = taper(f, c.fs);
+      }
+      V[v][0][lane] = a;
+      V[v][1][lane] = p;
+      V[v][2][lane] = y;
+    }
+    __syncthreads();
+    // central differences (:5497-5512): per detector, 22 rows of 64 samples
+    for (int r = wave; r < ND * NP; r += 4) {
+      const int d = r / NP, k = r % NP, m = m0 + lane;
+      double dA = 0.0, dp = 0.0;
+      if (m < M) {
+        const double f = freq[m];
+        double aP, xP, aM, xM;
+        project(P[2 * k + 1], d, f, V[2 * k + 1][0][lane], V[2 * k + 1][1][lane], V[2 * k + 1][2][lane], &aP, &xP);
+        project(P[2 * k + 2], d, f, V[2 * k + 2][0][lane], V[2 * k + 2][1][lane], V[2 * k + 2][2][lane], &aM, &xM);
+        aP -= aM;
+        xP -= xM;
+        dA = aP / (2.0 * EPS);
+        dp = xP / (2.0 * EPS);
+      }
+      U[d][k][lane] = dA;
+      U[d][NP + k][lane] = dp;
+    }
+    for (int d = wave; d < ND; d += 4) {
+      const int m = m0 + lane;
+      double A2 = 0.0, wh = 0.0;
+      if (m < M) {
+        double a0, x0;
+        project(P[0], d, freq[m], V[0][0][lane], V[0][1][lane], V[0][2][lane], &a0, &x0);
+        const double h = ha[(size_t)d * M + m];
+        A2 = a0 * a0;
+        wh = w[(size_t)d * M + m] / (h * h);
+      }
+      S[d][0][lane] = A2;
+      S[d][1][lane] = wh;
+    }
+    __syncthreads();
+    // the chunk's share of each owned entry: 16 samples of every detector
+    {
+      const int s0 = q0 * 16;
+#pragma unroll 1
+      for (int d = 0; d < ND; d++)
+#pragma unroll 4
+        for (int s = s0; s < s0 + 16; s++) {
+          const double cc = 4.0 * (U[d][pi0][s] * U[d][pj0][s] + S[d][0][s] * U[d][NP + pi0][s] * U[d][NP + pj0][s]);
+          acc0 += S[d][1][s] * cc;
+        }
+    }
+    if (two) {
+      const int s0 = q1 * 16;
+#pragma unroll 1
+      for (int d = 0; d < ND; d++)
+#pragma unroll 4
+        for (int s = s0; s < s0 + 16; s++) {
+          const double cc = 4.0 * (U[d][pi1][s] * U[d][pj1][s] + S[d][0][s] * U[d][NP + pi1][s] * U[d][NP + pj1][s]);
+          acc1 += S[d][1][s] * cc;
+        }
+    }
+    __syncthreads();
+  }
+  red[q0][t % NPAIR] = acc0;
+  if (two) red[q1][(t + 256) % NPAIR] = acc1;
+  __syncthreads();
+  if (t < NPAIR) {
+    int i, j;
+    pair_of(t, &i, &j);
+    double x = red[0][t];
+    for (int q = 1; q < NQ; q++) x += red[q][t];
+    F[i * NP + j] = x;
+    F[j * NP + i] = x;
+  }
+  if (t == 0) status[b] = ST_OK;
+}
+
+// FisherEvec (QuickCBC.c:2449-2546) of d x d matrices (d <= 11: the
+// reference calls it with NP = 11, NX = 7 and NS = 7, ConstCBC.h:67-69): the
+// cyclic Jacobi method (Golub & Van Loan, Alg. 8.5.3, row-cyclic order) in
+// place of GSL's eigen_symmv, which is absent.  A rotation is skipped, and
+// a_pq set to 0, once |a_pq| <= 2^-53 sqrt(|a_pp a_qq|) (the relative
+// criterion of Demmel and Veselic); sweeps stop when one rotates nothing, at
+// most 60.  Then the eigenpairs ordered by |lambda| ascending
+// (GSL_EIGEN_SORT_ABS_ASC, stable), each eigenvector's largest-magnitude
+// component made positive (the first on a tie), and the jump sizes
+// 1/sqrt(max(lambda, 10)).  A diagonal entry below 1e-16 in magnitude gives
+// 1e4 and the unit vectors.
+//
+// EV_G lanes per matrix, EV_PER matrices per 64-lane block, matrix and
+// eigenvectors in LDS.  Every lane of a group computes the rotation (t, c, s)
+// from the same three entries; lane k then updates row and column k of A and
+// row k of V (entries no other lane of the rotation touches), lane 0 the
+// diagonal pair: per element the same operations as a sequential sweep, so
+// the same results.  The rotation sequence stays serial; the block steps all
+// its groups through it together and a group that has converged idles.
+constexpr int EV_D = NP, EV_G = 16, EV_PER = 64 / EV_G;
+
+__global__ __launch_bounds__(64) void k_fisher_evec(const double* __restrict__ fish, int B, int d,
+                                                    double* __restrict__ ej, double* __restrict__ ev,
+                                                    int* __restrict__ sweeps) {
+  __shared__ double sa[EV_PER][EV_D * EV_D];
+  __shared__ double sv[EV_PER][EV_D * EV_D];
+  const int g = threadIdx.x / EV_G, k = threadIdx.x % EV_G;
+  const int b = blockIdx.x * EV_PER + g;
+  const bool live = b < B;
+  double* A = sa[g];
+  double* V = sv[g];
+  const int dd = d * d;
+  for (int i = k; i < dd; i += EV_G) {
+    A[i] = live ? fish[(size_t)b * dd + i] : 1.0;
+    V[i] = (i / d == i % d) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  bool ec = false;
+  for (int i = 0; i < d; i++)
+    if (fabs(A[i * d + i]) < 1.0e-16) ec = true;
+  bool done = !live || ec;
+  int nsweep = 0;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    if (__syncthreads_and(done)) break;  // block-uniform
+    int rot = 0;
+    for (int p = 0; p < d - 1; p++)
+      for (int q = p + 1; q < d; q++) {
+        const double apq = A[p * d + q], app = A[p * d + p], aqq = A[q * d + q];
+        const bool skip = fabs(apq) <= 0x1p-53 * sqrt(fabs(app * aqq));
+        double t = 0.0, c = 1.0, s = 0.0;
+        if (!done && !skip) {
+          const double tau = (aqq - app) / (2.0 * apq);
+          if (fabs(tau) > 1.0e150)
+            t = 0.5 / tau;
+          else
+            t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = t * c;
+        }
+        __syncthreads();  // every lane has read a_pq, a_pp, a_qq
+        if (!done) {
+          if (skip) {
+            if (k == 0) {
+              A[p * d + q] = 0.0;
+              A[q * d + p] = 0.0;
+            }
+          } else {
+            rot++;
+            if (k < d && k != p && k != q) {
+              const double akp = A[k * d + p], akq = A[k * d + q];
+              const double np = c * akp - s * akq, nq = s * akp + c * akq;
+              A[k * d + p] = np;
+              A[p * d + k] = np;
+              A[k * d + q] = nq;
+              A[q * d + k] = nq;
+            }
+            if (k == 0) {
+              A[p * d + p] = app - t * apq;
+              A[q * d + q] = aqq + t * apq;
+              A[p * d + q] = 0.0;
+              A[q * d + p] = 0.0;
+            }
+            if (k < d) {
+              const double vkp = V[k * d + p], vkq = V[k * d + q];
+              V[k * d + p] = c * vkp - s * vkq;
+              V[k * d + q] = s * vkp + c * vkq;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    if (!done) {
+      nsweep = sweep + 1;
+      if (rot == 0) done = true;
+    }
+  }
+  if (!live || k != 0) return;
+  double* e = ej + (size_t)b * d;
+  double* E = ev + (size_t)b * dd;
+  if (sweeps) sweeps[b] = ec ? 0 : nsweep;
+  if (ec) {
+    for (int i = 0; i < d; i++) {
+      e[i] = 10000.0;
+      for (int j = 0; j < d; j++) E[i * d + j] = (i == j) ? 1.0 : 0.0;
+    }
+    return;
+  }
+  int idx[EV_D];
+  for (int i = 0; i < d; i++) idx[i] = i;
+  for (int i = 1; i < d; i++) {
+    const int t = idx[i];
+    int j = i - 1;
+    while (j >= 0 && fabs(A[idx[j] * d + idx[j]]) > fabs(A[t * d + t])) {
+      idx[j + 1] = idx[j];
+      j--;
+    }
+    idx[j + 1] = t;
+  }
+  for (int i = 0; i < d; i++) {
+    const int col = idx[i];
+    int kmax = 0;
+    double vmax = -1.0;
+    for (int r = 0; r < d; r++)
+      if (fabs(V[r * d + col]) > vmax) {
+        vmax = fabs(V[r * d + col]);
+        kmax = r;
+      }
+    const double sg = V[kmax * d + col] < 0.0 ? -1.0 : 1.0;
+    double l = A[col * d + col];
+    if (l < 10.0) l = 10.0;
+    e[i] = 1.0 / sqrt(l);
+    for (int r = 0; r < d; r++) E[i * d + r] = sg * V[r * d + col];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_fisher_params(const double* d_params, int B, double* d_q, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fisher_params, dim3((B * NV + 255) / 256), dim3(256), 0, s, d_params, B, d_q);
+  return hipGetLastError();
+}
+
+hipError_t launch_het_fisher(const Point* pts, const double* d_params, int B, int nd, const double* freq,
+                             const double* pl, const double* ha, const double* w, int M, double* fish, int* status,
+                             hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  if (nd == 1)
+    hipLaunchKernelGGL(k_het_fisher<1>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+  else if (nd == 2)
+    hipLaunchKernelGGL(k_het_fisher<2>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+  else
+    hipLaunchKernelGGL(k_het_fisher<3>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_fisher_evec(const double* fish, int B, int d, double* ej, double* ev, int* sweeps, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  if (d < 1 || d > EV_D) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fisher_evec, dim3((B + EV_PER - 1) / EV_PER), dim3(64), 0, s, fish, B, d, ej, ev, sweeps);
+  return hipGetLastError();
+}
+
+}  // namespace qcbc

@@ -1,0 +1,27 @@
+// Launch wrappers of the Fisher-matrix and eigen-decomposition kernels
+// (qcbc_fisher.hip).  Host-safe: no device code in here.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "qcbc_types.h"
+
+namespace qcbc {
+
+constexpr int FISHER_NP = 11;                 // parameters of a point (NP, ConstCBC.h:68)
+constexpr int FISHER_NV = 2 * FISHER_NP + 1;  // parameter sets per point: itself, then +eps / -eps each
+constexpr int FISHER_EVEC_MAX_D = FISHER_NP;  // largest matrix the eigen kernel takes
+
+// the FISHER_NV parameter sets of each of B points: q is B x FISHER_NV x 11
+hipError_t launch_fisher_params(const double* d_params, int B, double* d_q, hipStream_t s);
+// Fisher_Het for B points from the set-up records of their FISHER_NV
+// parameter sets (pts, B x FISHER_NV), on a heterodyne's grid: fish is
+// B x 11 x 11, status B
+hipError_t launch_het_fisher(const Point* pts, const double* d_params, int B, int nd, const double* freq,
+                             const double* pl, const double* ha, const double* w, int M, double* fish, int* status,
+                             hipStream_t s);
+// FisherEvec of B d x d matrices (d <= FISHER_EVEC_MAX_D): ej B x d, ev B x d x d,
+// sweeps B (optional)
+hipError_t launch_fisher_evec(const double* fish, int B, int d, double* ej, double* ev, int* sweeps, hipStream_t s);
+
+}  // namespace qcbc

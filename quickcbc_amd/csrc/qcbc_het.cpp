@@ -13,6 +13,7 @@
 
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
+#include "qcbc_fisher_internal.h"
 #include "qcbc_het_internal.h"
 #include "qcbc_internal.h"
 
@@ -47,6 +48,17 @@ struct qcbc_het {
   Point* d_pts = nullptr;
   int* d_status = nullptr;
   double* d_logL = nullptr;
+  // Fisher-matrix scratch: FISHER_NV parameter sets, records and statuses per
+  // point, then the matrices, jump sizes, eigenvectors and point statuses
+  int fcap = 0;
+  double* d_fq = nullptr;
+  Point* d_fpts = nullptr;
+  int* d_fst = nullptr;
+  double* d_fp = nullptr;
+  double* d_fish = nullptr;
+  double* d_ej = nullptr;
+  double* d_ev = nullptr;
+  int* d_fstatus = nullptr;
   // kernel timing
   bool timing = false;
   double kernel_ms = 0.0;
@@ -80,8 +92,9 @@ void het_free(qcbc_het* h) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
   }
-  void* bufs[] = {h->d_freq, h->d_amp, h->d_phase, h->d_w,      h->d_wf,
-                  h->d_pl,   h->d_raw, h->d_params, h->d_pts, h->d_status, h->d_logL};
+  void* bufs[] = {h->d_freq, h->d_amp,  h->d_phase, h->d_w,    h->d_wf,   h->d_pl,   h->d_raw,
+                  h->d_params, h->d_pts, h->d_status, h->d_logL, h->d_fq, h->d_fpts, h->d_fst,
+                  h->d_fp,   h->d_fish, h->d_ej,    h->d_ev,   h->d_fstatus};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -132,6 +145,43 @@ int het_reserve(qcbc_het* h, int B) {
   HIPCHK(hipMalloc(&h->d_status, sizeof(int) * nb));
   HIPCHK(hipMalloc(&h->d_logL, sizeof(double) * nb));
   h->cap = nb;
+  return QCBC_OK;
+}
+
+int fisher_reserve(qcbc_het* h, int B) {
+  if (B <= h->fcap) return QCBC_OK;
+  const int nb = std::max(B, 2 * h->fcap);
+  void** bufs[] = {(void**)&h->d_fq,   (void**)&h->d_fpts, (void**)&h->d_fst, (void**)&h->d_fp,
+                   (void**)&h->d_fish, (void**)&h->d_ej,   (void**)&h->d_ev,  (void**)&h->d_fstatus};
+  for (void** b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+  h->fcap = 0;
+  const size_t n = nb, nv = (size_t)nb * FISHER_NV, np = FISHER_NP;
+  HIPCHK(hipMalloc(&h->d_fq, sizeof(double) * np * nv));
+  HIPCHK(hipMalloc(&h->d_fpts, sizeof(Point) * nv));
+  HIPCHK(hipMalloc(&h->d_fst, sizeof(int) * nv));
+  HIPCHK(hipMalloc(&h->d_fp, sizeof(double) * np * n));
+  HIPCHK(hipMalloc(&h->d_fish, sizeof(double) * np * np * n));
+  HIPCHK(hipMalloc(&h->d_ej, sizeof(double) * np * n));
+  HIPCHK(hipMalloc(&h->d_ev, sizeof(double) * np * np * n));
+  HIPCHK(hipMalloc(&h->d_fstatus, sizeof(int) * n));
+  h->fcap = nb;
+  return QCBC_OK;
+}
+
+// Fisher_Het of B points (device params) into d_fish, and FisherEvec of each
+// into d_ej / d_ev when they are given, on the het's stream (caller holds mu)
+int fisher_eval(qcbc_het* h, const double* d_params, int B, double* d_fish, double* d_ej, double* d_ev,
+                int* d_status) {
+  int st = fisher_reserve(h, B);
+  if (st) return st;
+  HIPCHK(launch_fisher_params(d_params, B, h->d_fq, h->stream));
+  HIPCHK(launch_setup(h->d_fq, B * FISHER_NV, h->an, h->d_fpts, h->d_fst, h->stream));
+  HIPCHK(launch_het_fisher(h->d_fpts, d_params, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_w, h->M, d_fish,
+                           d_status, h->stream));
+  if (d_ej && d_ev) HIPCHK(launch_fisher_evec(d_fish, B, FISHER_NP, d_ej, d_ev, nullptr, h->stream));
   return QCBC_OK;
 }
 
@@ -344,6 +394,86 @@ int qcbc_het_loglike_device(qcbc_het* h, const double* d_params, int B, double* 
   int st = het_reserve(h, B);
   if (st) return st;
   return het_eval(h, d_params, B, d_logL, d_status ? d_status : h->d_status);
+}
+
+int qcbc_het_fisher(qcbc_het* h, const double* params, int B, double* fish, double* ej, double* ev, int* status) {
+  if (!h || B < 0 || (B > 0 && (!params || !fish)) || (!ej) != (!ev)) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = fisher_reserve(h, B);
+  if (st) return st;
+  const size_t np = FISHER_NP;
+  HIPCHK(hipMemcpyAsync(h->d_fp, params, sizeof(double) * np * B, hipMemcpyHostToDevice, h->stream));
+  st = fisher_eval(h, h->d_fp, B, h->d_fish, ej ? h->d_ej : nullptr, ev ? h->d_ev : nullptr, h->d_fstatus);
+  if (st) return st;
+  HIPCHK(hipMemcpyAsync(fish, h->d_fish, sizeof(double) * np * np * B, hipMemcpyDeviceToHost, h->stream));
+  if (ej) {
+    HIPCHK(hipMemcpyAsync(ej, h->d_ej, sizeof(double) * np * B, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(ev, h->d_ev, sizeof(double) * np * np * B, hipMemcpyDeviceToHost, h->stream));
+  }
+  if (status) HIPCHK(hipMemcpyAsync(status, h->d_fstatus, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return QCBC_OK;
+}
+
+int qcbc_het_fisher_device(qcbc_het* h, const double* d_params, int B, double* d_fish, double* d_ej, double* d_ev,
+                           int* d_status) {
+  if (!h || B < 0 || (B > 0 && (!d_params || !d_fish)) || (!d_ej) != (!d_ev)) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = fisher_reserve(h, B);
+  if (st) return st;
+  return fisher_eval(h, d_params, B, d_fish, d_ej, d_ev, d_status ? d_status : h->d_fstatus);
+}
+
+int qcbc_het_fisher_time(qcbc_het* h, const double* d_params, int B, int reps, double* ms) {
+  if (!h || B < 1 || !d_params || reps < 1 || !ms) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  int st = fisher_reserve(h, B);
+  if (st) return st;
+  hipEvent_t e[4];
+  for (auto& x : e) HIPCHK(hipEventCreate(&x));
+  HIPCHK(launch_fisher_params(d_params, B, h->d_fq, h->stream));
+  HIPCHK(hipEventRecord(e[0], h->stream));
+  for (int r = 0; r < reps; r++) HIPCHK(launch_setup(h->d_fq, B * FISHER_NV, h->an, h->d_fpts, h->d_fst, h->stream));
+  HIPCHK(hipEventRecord(e[1], h->stream));
+  for (int r = 0; r < reps; r++)
+    HIPCHK(launch_het_fisher(h->d_fpts, d_params, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_w, h->M, h->d_fish,
+                             h->d_fstatus, h->stream));
+  HIPCHK(hipEventRecord(e[2], h->stream));
+  for (int r = 0; r < reps; r++)
+    HIPCHK(launch_fisher_evec(h->d_fish, B, FISHER_NP, h->d_ej, h->d_ev, nullptr, h->stream));
+  HIPCHK(hipEventRecord(e[3], h->stream));
+  HIPCHK(hipEventSynchronize(e[3]));
+  for (int i = 0; i < 3; i++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e[i], e[i + 1]));
+    ms[i] = t / reps;
+  }
+  for (auto& x : e) hipEventDestroy(x);
+  return QCBC_OK;
+}
+
+int qcbc_fisher_evec(qcbc_ctx* ctx, const double* fish, int B, int d, double* ej, double* ev) {
+  if (!ctx || B < 0 || d < 1 || d > FISHER_EVEC_MAX_D || (B > 0 && (!fish || !ej || !ev))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  const size_t dd = (size_t)d * d, n = (size_t)B * (2 * dd + d);
+  double* buf = nullptr;
+  HIPCHK(hipMalloc(&buf, sizeof(double) * n));
+  double *d_f = buf, *d_ev = buf + dd * B, *d_ej = buf + 2 * dd * B;
+  hipError_t e = hipMemcpyAsync(d_f, fish, sizeof(double) * dd * B, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_fisher_evec(d_f, B, d, d_ej, d_ev, nullptr, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ej, d_ej, sizeof(double) * d * B, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ev, d_ev, sizeof(double) * dd * B, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(buf);
+  HIPCHK(e);
+  return QCBC_OK;
 }
 
 int qcbc_het_sync(qcbc_het* h) {

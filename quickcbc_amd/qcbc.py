@@ -131,6 +131,15 @@ def lib():
         L.qcbc_het_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_het_time_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
         L.qcbc_het_destroy.argtypes = [_vp]
+        # Fisher matrix and jump directions
+        L.qcbc_het_fisher.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _ip]
+        L.qcbc_het_fisher_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.qcbc_het_fisher_time.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
+        L.qcbc_fisher_evec.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.Fisher_Het.restype = None
+        L.Fisher_Het.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dpp, _dp, ctypes.c_double]
+        L.FisherEvec.restype = None
+        L.FisherEvec.argtypes = [_dpp, _dp, _dpp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -188,6 +197,17 @@ class Context:
         self.h = h
         if D is not None:
             self.bind(D, SN if SN is not None else an.SN())
+
+    def fisher_evec(self, fish):
+        """FisherEvec of a batch of d x d matrices on this analysis's GPU: (ej B x d, ev B x d x d)."""
+        F = np.ascontiguousarray(fish, np.float64)
+        d = F.shape[-1]
+        F = F.reshape(-1, d, d)
+        B = F.shape[0]
+        ej = np.zeros((B, d))
+        ev = np.zeros((B, d, d))
+        _check(lib().qcbc_fisher_evec(self.h, _d(F), B, d, _d(ej), _d(ev)), "qcbc_fisher_evec")
+        return ej, ev
 
     def close(self):
         if self.h:
@@ -710,6 +730,29 @@ class Het:
     def sync(self):
         _check(lib().qcbc_het_sync(self.h), "qcbc_het_sync")
 
+    def fisher(self, params, evec=True):
+        """Fisher_Het at each point: (fish B x 11 x 11, ej B x 11, ev B x 11 x 11, status); ej and ev are
+        None unless evec."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        F = np.zeros((B, 11, 11))
+        st = np.zeros(B, np.int32)
+        ej = np.zeros((B, 11)) if evec else None
+        ev = np.zeros((B, 11, 11)) if evec else None
+        _check(lib().qcbc_het_fisher(self.h, _d(p), B, _d(F), _d(ej) if evec else None, _d(ev) if evec else None,
+                                     _i(st)), "qcbc_het_fisher")
+        return F, ej, ev, st
+
+    def fisher_device(self, d_params, B, d_fish, d_ej=0, d_ev=0, d_status=0):
+        _check(lib().qcbc_het_fisher_device(self.h, d_params, B, d_fish, d_ej, d_ev, d_status),
+               "qcbc_het_fisher_device")
+
+    def fisher_time(self, d_params, B, reps):
+        """Mean ms per launch: (set-up of the 23 B perturbed points, Fisher kernel, eigen kernel)."""
+        ms = np.zeros(3)
+        _check(lib().qcbc_het_fisher_time(self.h, d_params, B, reps, _d(ms)), "qcbc_het_fisher_time")
+        return tuple(float(x) for x in ms)
+
     def timing(self, enable):
         ms = ctypes.c_double()
         n = ctypes.c_longlong()
@@ -811,3 +854,20 @@ def dropin_log_likelihood_het(an, view, params):
     net = make_net(an)
     p = np.ascontiguousarray(params, np.float64)
     return lib().log_likelihood_het(ctypes.byref(net), ctypes.byref(view.s), _d(p), an.Tobs)
+
+
+def dropin_fisher_het(an, view, params):
+    net = make_net(an)
+    p = np.ascontiguousarray(params, np.float64)
+    F = RowMatrix(np.zeros((11, 11)))
+    lib().Fisher_Het(ctypes.byref(net), ctypes.byref(view.s), F.p, _d(p), an.Tobs)
+    return F.arr
+
+
+def dropin_fisher_evec(fish):
+    F = RowMatrix(np.array(fish, np.float64))
+    d = F.arr.shape[0]
+    ej = np.zeros(d)
+    ev = RowMatrix(np.zeros((d, d)))
+    lib().FisherEvec(F.p, _d(ej), ev.p, d)
+    return ej, ev.arr
