@@ -282,6 +282,42 @@ int qcbc_pt_groups(qcbc_pt *pt, int *groups);
 int qcbc_pt_create_het(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, double ladder,
                        const double *init, const double *sigma, uint64_t seed, int groups,
                        qcbc_pt **out);
+/* QuickCBC's own PT step on the heterodyned likelihood: update()'s proposal
+ * mix (QuickCBC.c:2238-2405) and MCMC_all's step (:2029-2092), resident on
+ * the GPU.  Each chain column is one of the reference's temperature ladders:
+ * per step and column, with probability 0.2 one swap of a random adjacent
+ * pair of temperatures, else update() at every temperature (Fisher
+ * eigen-jumps above alpha > a, differential evolution from the
+ * temperature's history above b, jiggle above c), accepted by
+ * H = dlogL/T + the mass Jacobian of a prior flat in m1, m2 + 3 dlnDL, inside
+ * the prior box with eta in [etamin, 1/4].  The sky ring (alpha <= c) is not
+ * built, so c must be 0 (the reference's alternative setting,
+ * :2256-2260).  The Fisher jumps (Fisher_Het + FisherEvec) are computed at
+ * creation and by qcbc_pt_refresh_fisher (the reference refreshes them every
+ * 50000 iterations, :2019-2027; efix is not built). */
+typedef struct {
+  double min[11], max[11]; /* prior box (update()'s min, max) */
+  double a, b, c;          /* proposal thresholds: 0.5, 0.3, 0 */
+  double etamin;           /* lower limit of eta (0.05, ConstCBC.h:57) */
+  int n_hist;              /* history rows per temperature (NH = 1000, ConstCBC.h:72) */
+  int hist_every;          /* update steps between history entries (20, QuickCBC.c:2071) */
+} qcbc_update_desc;
+int qcbc_pt_create_update(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, double ladder,
+                          const double *init, const qcbc_update_desc *desc, uint64_t seed, int groups,
+                          qcbc_pt **out);
+/* Fisher jumps at every chain's current state, and the pending proposals
+ * remade with them. */
+int qcbc_pt_refresh_fisher(qcbc_pt *pt);
+/* State of a QuickCBC-step ensemble, in point order (t*n_chains + c): jump
+ * sizes (P x 11), directions (P x 11 x 11), the pending proposals (P x 11)
+ * and their types (1 Fisher, 2 DE, 3 jiggle); stats n_chains x 9 per column:
+ * update steps, swap steps, swaps accepted, then proposed and accepted for
+ * each type.  Any pointer may be NULL. */
+int qcbc_pt_get_update(qcbc_pt *pt, double *ej, double *ev, double *proposals, int *types,
+                       long long *stats);
+/* The histories (n_chains x n_temps x n_hist x 11) and their fill counts per
+ * column (row count % n_hist is written next). */
+int qcbc_pt_get_history(qcbc_pt *pt, double *hist, long long *counts);
 /* nsteps PT steps: propose, evaluate, Metropolis accept, adjacent-temperature
  * swaps, all on the GPU.  use_graph: replay one captured step as a hipGraph. */
 int qcbc_pt_run(qcbc_pt *pt, int nsteps, int use_graph);

@@ -323,3 +323,71 @@ def fisher_evec(fish):
     sw = np.zeros(B, np.int32)
     L.qo_fisher_evec_batch(_d(F), B, d, _d(ej), _d(ev), _i(sw))
     return ej, ev, sw
+
+
+def pt_update_hist_init(x, n_temps, n_chains, n_hist, seed):
+    """The initial histories of QuickCBC's PT step (n_chains x n_temps x n_hist x 11)."""
+    L = lib()
+    L.qo_update_hist_init.restype = None
+    L.qo_update_hist_init.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _dp]
+    x = np.ascontiguousarray(x, np.float64)
+    hist = np.zeros((n_chains, n_temps, n_hist, 11))
+    L.qo_update_hist_init(_d(x), n_temps, n_chains, n_hist, seed, _d(hist))
+    return hist
+
+
+def update_propose(x, ej, ev, hist, sT, abc, seed, ctr):
+    """update()'s proposal from x with the counters (seed, ctr): (type, y)."""
+    L = lib()
+    L.qo_update_propose.restype = ctypes.c_int
+    L.qo_update_propose.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_uint64,
+                                    ctypes.c_uint64, _dp]
+    a = [np.ascontiguousarray(v, np.float64) for v in (x, ej, ev, hist, abc)]
+    y = np.zeros(11)
+    t = L.qo_update_propose(_d(a[0]), _d(a[1]), _d(a[2]), _d(a[3]), a[3].size // 11, sT, _d(a[4]), seed, ctr, _d(y))
+    return t, y
+
+
+def pt_update_run(labels, gmst, fref, Tobs, het, uconst, n_hist, hist_every, n_temps, n_chains, ladder, seed,
+                  nsteps, state):
+    """QuickCBC's PT step, nsteps times, on the state dict (x, L, ej, ev, hist, hcount, y, typ, stats, step),
+    updated in place; uconst = min[11] + max[11] + [a, b, c, etamin]."""
+    L = lib()
+    L.qo_pt_update_run.restype = None
+    L.qo_pt_update_run.argtypes = ([ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int] + [_dp] * 10 +
+                                   [ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp, _dp, _dp,
+                                    ctypes.POINTER(ctypes.c_longlong), _dp, _ip,
+                                    ctypes.POINTER(ctypes.c_longlong)])
+    labels = np.ascontiguousarray(labels, np.int32)
+    arr = [np.ascontiguousarray(het[k], np.float64)
+           for k in ("freq", "amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")]
+    uc = np.ascontiguousarray(uconst, np.float64)
+    for k in ("x", "L", "ej", "ev", "hist", "y"):
+        state[k] = np.ascontiguousarray(state[k], np.float64)
+    state["typ"] = np.ascontiguousarray(state["typ"], np.int32)
+    hc = (ctypes.c_longlong * n_chains)(*[int(v) for v in state["hcount"]])
+    st = (ctypes.c_longlong * (9 * n_chains))(*[int(v) for v in np.asarray(state["stats"]).ravel()])
+    step = ctypes.c_longlong(int(state["step"]))
+    L.qo_pt_update_run(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"], *[_d(a) for a in arr],
+                       het["logLb"], _d(uc), n_hist, hist_every, n_temps, n_chains, ladder, seed, nsteps,
+                       ctypes.byref(step), *[_d(state[k]) for k in ("x", "L", "ej", "ev", "hist")], hc,
+                       _d(state["y"]), _i(state["typ"]), st)
+    state["hcount"] = np.array(hc[:], np.int64)
+    state["stats"] = np.array(st[:], np.int64).reshape(n_chains, 9)
+    state["step"] = step.value
+    return state
+
+
+def update_prior(y, uconst):
+    """update()'s prior check: (flag, mass Jacobian term)."""
+    L = lib()
+    L.qo_update_prior.restype = ctypes.c_int
+    L.qo_update_prior.argtypes = [_dp, _dp, _dp]
+    y = np.ascontiguousarray(y, np.float64)
+    uc = np.ascontiguousarray(uconst, np.float64)
+    j = np.zeros(1)
+    f = L.qo_update_prior(_d(y), _d(uc), _d(j))
+    return f, float(j[0])

@@ -43,6 +43,7 @@
 #include "qcbc_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1678,4 +1679,273 @@ void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *
     int s = qo_fisher_evec(fish + (size_t)b * d * d, ej + (size_t)b * d, ev + (size_t)b * d * d, d);
     if (sweeps) sweeps[b] = s;
   }
+}
+
+/* ------------------------------------------------------------------ *
+ * QuickCBC's PT step: update() (QuickCBC.c:2238-2405) and MCMC_all's   *
+ * iteration (:2029-2092) for an ensemble of independent chain columns  *
+ * (each column one temperature ladder), with the counter-based random  *
+ * numbers the GPU uses, so that the chains can be replayed.  Arrays in *
+ * point order p = t n_chains + c; histories column-major (column c,    *
+ * temperature t, row i at ((c n_temps + t) n_hist + i) 11).            *
+ * ------------------------------------------------------------------ */
+#define QO_SALT_PROPOSE 0x6a09e667f3bcc908ULL
+#define QO_SALT_ACCEPT 0xbb67ae8584caa73bULL
+#define QO_SALT_SWAP 0x3c6ef372fe94f82bULL
+#define QO_SALT_HIST 0xa54ff53a5f1d36f1ULL
+#define QO_D_ALPHA 0
+#define QO_D_FISHER 1
+#define QO_D_DE 4
+#define QO_DE_TRIES 32
+#define QO_D_DE_SCALE 40
+#define QO_D_JIGGLE 50
+#define QO_ANGLE_WRAP_MAX 1.0e4
+#define QO_UPD_STATS 9
+
+static uint64_t qo_mix(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+/* uniform in (0, 1): 53 bits of SplitMix64 of (seed ^ salt, counter a, draw k) */
+double qo_uni(uint64_t seed, uint64_t salt, uint64_t a, uint64_t k) {
+  const uint64_t r = qo_mix((seed ^ salt) ^ qo_mix(a * 0x2545F4914F6CDD1DULL + k));
+  return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+static double qo_gauss(uint64_t seed, uint64_t salt, uint64_t a, uint64_t k) {
+  const double u1 = qo_uni(seed, salt, a, k), u2 = qo_uni(seed, salt, a, k + 1);
+  return sqrt(-2.0 * log(u1)) * cos(QO_TPI * u2);
+}
+
+typedef struct {
+  double min[11], max[11], a, b, c, etamin;
+  int n_hist, hist_every;
+} qo_update_const;
+
+/* update()'s prior checks and mass Jacobian (QuickCBC.c:2345-2363): 1 when
+ * y is outside the prior */
+static int qo_prior_flag(const double *y, const qo_update_const *u, double *pMcM) {
+  int i, flag = 0;
+  double leta;
+  for (i = 0; i < 11; i++)
+    if (!(y[i] <= u->max[i] && y[i] >= u->min[i])) flag = 1;
+  leta = (5.0 / 3.0) * (y[0] - y[1]);
+  if (leta > log(0.25)) flag = 1;
+  *pMcM = 0.0;
+  if (flag == 0) {
+    double eta = exp(leta);
+    *pMcM = 2.0 * y[1] + leta - 0.5 * log(1.0 - 4.0 * eta);
+    if (eta < u->etamin) flag = 1;
+  }
+  return flag;
+}
+
+static double qo_mass_jacobian(const double *x) {
+  double leta = (5.0 / 3.0) * (x[0] - x[1]);
+  double eta = exp(leta);
+  return 2.0 * x[1] + leta - 0.5 * log(1.0 - 4.0 * eta);
+}
+
+/* update()'s proposal (QuickCBC.c:2268-2342, sky ring not built: c = 0) and
+ * de_jump (:2407-2425); returns the type, 1 Fisher, 2 DE, 3 jiggle */
+int qo_update_propose(const double *x, const double *ej, const double *ev, const double *hist, int n_hist,
+                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y) {
+  const double alpha = qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_ALPHA);
+  int typ, j, k;
+  for (j = 0; j < 11; j++) y[j] = x[j];
+  if (alpha > abc[0]) {
+    int i = (int)(qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_FISHER) * 11.0);
+    double beta = sT * ej[i] * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_FISHER + 1);
+    typ = 1;
+    for (j = 0; j < 11; j++) y[j] = x[j] + beta * ev[11 * i + j];
+  } else if (alpha > abc[1]) {
+    int i = (int)((double)n_hist * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_DE)), r;
+    double g = 1.0;
+    typ = 2;
+    j = i;
+    for (r = 0; r < QO_DE_TRIES && j == i; r++)
+      j = (int)((double)n_hist * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_DE + 1 + r));
+    if (j == i) j = (i + 1) % n_hist;
+    if (qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_DE_SCALE) < 0.9)
+      g = 0.5 * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_DE_SCALE + 1);
+    for (k = 0; k < 11; k++) y[k] = x[k] + g * (hist[11 * i + k] - hist[11 * j + k]);
+  } else {
+    double beta = 0.01 * pow(10.0, -floor(3.0 * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_JIGGLE))) * sT;
+    typ = 3;
+    for (k = 0; k < 11; k++) y[k] = x[k] + beta * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_JIGGLE + 1 + 2 * k);
+  }
+  if (fabs(y[4]) <= QO_ANGLE_WRAP_MAX) {
+    while (y[4] > QO_TPI) y[4] -= QO_TPI;
+    while (y[4] < 0.0) y[4] += QO_TPI;
+  }
+  if (fabs(y[7]) <= QO_ANGLE_WRAP_MAX) {
+    while (y[7] > QO_TPI) y[7] -= QO_TPI;
+    while (y[7] < 0.0) y[7] += QO_TPI;
+  }
+  if (fabs(y[9]) <= QO_ANGLE_WRAP_MAX) {
+    while (y[9] > QO_PI) y[9] -= QO_PI;
+    while (y[9] < 0.0) y[9] += QO_PI;
+  }
+  return typ;
+}
+
+/* The initial history of each column (MCMC_all, :1946-1954): row i of
+ * temperature t holds the state of a temperature q drawn from the column's. */
+void qo_update_hist_init(const double *x, int n_temps, int n_chains, int n_hist, uint64_t seed, double *hist) {
+  int c, t, i, k;
+  for (c = 0; c < n_chains; c++)
+    for (t = 0; t < n_temps; t++)
+      for (i = 0; i < n_hist; i++) {
+        int q = (int)((double)n_temps * qo_uni(seed, QO_SALT_HIST, (uint64_t)c * n_temps + t, (uint64_t)i));
+        double *h = hist + (((size_t)c * n_temps + t) * n_hist + i) * 11;
+        for (k = 0; k < 11; k++) h[k] = x[11 * ((size_t)q * n_chains + c) + k];
+      }
+}
+
+/* nsteps steps of the ensemble from step *step (the proposals y, types typ of
+ * that step already made): the heterodyned logL of y, then per column a swap
+ * or update() at every temperature, the history, the next proposals.
+ * uconst: min[11], max[11], a, b, c, etamin; T_t = ladder^t.  stats:
+ * n_chains x 9 (update steps, swap steps, swaps accepted, proposed and
+ * accepted per type), accumulated. */
+void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
+                      const double *freq, const double *amp, const double *phase, const double *IP,
+                      const double *SL, const double *lc, const double *ls, const double *rc, const double *rs,
+                      const double *aa, double logLb, const double *uconst, int n_hist, int hist_every, int n_temps,
+                      int n_chains, double ladder, uint64_t seed, int nsteps, long long *step, double *x, double *L,
+                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ,
+                      long long *stats) {
+  const size_t P = (size_t)n_temps * n_chains;
+  qo_update_const u;
+  qo_het h;
+  double *Ly = malloc(sizeof(double) * P), *sT = malloc(sizeof(double) * n_temps);
+  double *iT = malloc(sizeof(double) * n_temps);
+  int *sty = malloc(sizeof(int) * P);
+  int i, t, c, n;
+  for (i = 0; i < 11; i++) {
+    u.min[i] = uconst[i];
+    u.max[i] = uconst[11 + i];
+  }
+  u.a = uconst[22];
+  u.b = uconst[23];
+  u.c = uconst[24];
+  u.etamin = uconst[25];
+  u.n_hist = n_hist;
+  u.hist_every = hist_every;
+  for (t = 0; t < n_temps; t++) {
+    double T = pow(ladder, (double)t);
+    sT[t] = sqrt(T);
+    iT[t] = 1.0 / T;
+  }
+  qo_spline_init();
+  memset(&h, 0, sizeof h);
+  h.M = M;
+  h.J = J;
+  h.NR = (M - 1) / J;
+  h.nifo = nifo;
+  h.freq = freq;
+  h.amp = (double *)amp;
+  h.phase = (double *)phase;
+  h.IP = (double *)IP;
+  h.SL = (double *)SL;
+  h.lc = (double *)lc;
+  h.ls = (double *)ls;
+  h.rc = (double *)rc;
+  h.rs = (double *)rs;
+  h.aa = (double *)aa;
+  h.logLb = logLb;
+  for (n = 0; n < nsteps; n++) {
+    const uint64_t s = (uint64_t)*step;
+    long p;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (p = 0; p < (long)P; p++) {
+      int st = 0;
+      Ly[p] = qo_loglike_het(nifo, labels, gmst, fref, Tobs, &h, y + 11 * p, &st);
+      sty[p] = st;
+    }
+    for (c = 0; c < n_chains; c++) {
+      const uint64_t cs = s * (uint64_t)n_chains + c;
+      long long *cst = stats + (size_t)c * QO_UPD_STATS;
+      const int swap_step = n_temps > 1 && qo_uni(seed, QO_SALT_SWAP, cs, 0) < 0.2;
+      if (!swap_step) {
+        for (t = 0; t < n_temps; t++) {
+          const size_t q = (size_t)t * n_chains + c;
+          double pMcMy;
+          const int flag = qo_prior_flag(y + 11 * q, &u, &pMcMy) || sty[q] != 0;
+          const int k = typ[q];
+          cst[3 + 2 * (k - 1)]++;
+          if (!flag) {
+            const double *xp = x + 11 * q, *yp = y + 11 * q;
+            const double H =
+                (Ly[q] - L[q]) * iT[t] + pMcMy + 3.0 * yp[6] - 3.0 * xp[6] - qo_mass_jacobian(xp);
+            if (H > log(qo_uni(seed, QO_SALT_ACCEPT, s * P + q, 0))) {
+              memcpy(x + 11 * q, yp, sizeof(double) * 11);
+              L[q] = Ly[q];
+              cst[4 + 2 * (k - 1)]++;
+            }
+          }
+        }
+        if (s % (uint64_t)hist_every == 0) {
+          const long long m = hcount[c];
+          for (t = 0; t < n_temps; t++)
+            memcpy(hist + (((size_t)c * n_temps + t) * n_hist + (size_t)(m % n_hist)) * 11,
+                   x + 11 * ((size_t)t * n_chains + c), sizeof(double) * 11);
+          hcount[c] = m + 1;
+        }
+        cst[0]++;
+      } else {
+        const int j = (int)((double)(n_temps - 1) * qo_uni(seed, QO_SALT_SWAP, cs, 1));
+        const size_t la = (size_t)j * n_chains + c, lb = (size_t)(j + 1) * n_chains + c;
+        const double dL = L[la] - L[lb];
+        const double beta = exp(dL * iT[j + 1] - dL * iT[j]);
+        cst[1]++;
+        if (beta > qo_uni(seed, QO_SALT_SWAP, cs, 2)) {
+          double tmp[121];
+          cst[2]++;
+          memcpy(tmp, x + 11 * la, sizeof(double) * 11);
+          memcpy(x + 11 * la, x + 11 * lb, sizeof(double) * 11);
+          memcpy(x + 11 * lb, tmp, sizeof(double) * 11);
+          memcpy(tmp, ej + 11 * la, sizeof(double) * 11);
+          memcpy(ej + 11 * la, ej + 11 * lb, sizeof(double) * 11);
+          memcpy(ej + 11 * lb, tmp, sizeof(double) * 11);
+          memcpy(tmp, ev + 121 * la, sizeof(double) * 121);
+          memcpy(ev + 121 * la, ev + 121 * lb, sizeof(double) * 121);
+          memcpy(ev + 121 * lb, tmp, sizeof(double) * 121);
+          tmp[0] = L[la];
+          L[la] = L[lb];
+          L[lb] = tmp[0];
+        }
+      }
+      for (t = 0; t < n_temps; t++) {
+        const size_t q = (size_t)t * n_chains + c;
+        const double *hr = hist + ((size_t)c * n_temps + t) * (size_t)n_hist * 11;
+        typ[q] = qo_update_propose(x + 11 * q, ej + 11 * q, ev + 121 * q, hr, n_hist, sT[t], uconst + 22, seed,
+                                   (s + 1) * P + q, y + 11 * q);
+      }
+    }
+    *step = (long long)(s + 1);
+  }
+  free(Ly);
+  free(sT);
+  free(iT);
+  free(sty);
+}
+
+/* update()'s prior check of one point: returns the flag, *pMcM the Jacobian
+ * term (uconst as qo_pt_update_run's) */
+int qo_update_prior(const double *y, const double *uconst, double *pMcM) {
+  qo_update_const u;
+  int i;
+  for (i = 0; i < 11; i++) {
+    u.min[i] = uconst[i];
+    u.max[i] = uconst[11 + i];
+  }
+  u.a = uconst[22];
+  u.b = uconst[23];
+  u.c = uconst[24];
+  u.etamin = uconst[25];
+  u.n_hist = 2;
+  u.hist_every = 1;
+  return qo_prior_flag(y, &u, pMcM);
 }

@@ -12,6 +12,8 @@
 
 #include <stddef.h>
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -137,6 +139,21 @@ void qo_fisher_het_batch_flat(int nifo, const int *labels, double gmst, double f
                               int *status, int nthreads);
 int qo_fisher_evec(const double *fish, double *ej, double *ev, int d);
 void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *ev, int *sweeps);
+
+/* QuickCBC's PT step (update(), QuickCBC.c:2238-2405; MCMC_all, :2029-2092)
+ * with the counter-based random numbers of the GPU step. */
+double qo_uni(uint64_t seed, uint64_t salt, uint64_t a, uint64_t k);
+int qo_update_propose(const double *x, const double *ej, const double *ev, const double *hist, int n_hist,
+                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y);
+void qo_update_hist_init(const double *x, int n_temps, int n_chains, int n_hist, uint64_t seed, double *hist);
+int qo_update_prior(const double *y, const double *uconst, double *pMcM);
+void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
+                      const double *freq, const double *amp, const double *phase, const double *IP,
+                      const double *SL, const double *lc, const double *ls, const double *rc, const double *rs,
+                      const double *aa, double logLb, const double *uconst, int n_hist, int hist_every, int n_temps,
+                      int n_chains, double ladder, uint64_t seed, int nsteps, long long *step, double *x, double *L,
+                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ,
+                      long long *stats);
 
 #ifdef __cplusplus
 }

@@ -70,4 +70,8 @@ int loglike_core(qcbc_ctx* c, const double* d_params, int B, int prec, double* d
 // status into d_status, logL into d_logL (the heterodyne's data are read only)
 int het_enqueue(qcbc_het* h, const double* d_params, int B, Point* pts, int* d_status, double* d_logL, hipStream_t s);
 int het_device(const qcbc_het* h);
+// Fisher_Het and FisherEvec of B points (device params) into d_ej (B x 11)
+// and d_ev (B x 11 x 11) on stream s, with the heterodyne's scratch; returns
+// after the stream has finished them
+int het_fisher_enqueue(qcbc_het* h, const double* d_params, int B, double* d_ej, double* d_ev, hipStream_t s);
 }  // namespace qcbc

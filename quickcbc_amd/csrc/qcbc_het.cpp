@@ -172,16 +172,16 @@ int fisher_reserve(qcbc_het* h, int B) {
 }
 
 // Fisher_Het of B points (device params) into d_fish, and FisherEvec of each
-// into d_ej / d_ev when they are given, on the het's stream (caller holds mu)
+// into d_ej / d_ev when they are given, on stream s (caller holds mu)
 int fisher_eval(qcbc_het* h, const double* d_params, int B, double* d_fish, double* d_ej, double* d_ev,
-                int* d_status) {
+                int* d_status, hipStream_t s) {
   int st = fisher_reserve(h, B);
   if (st) return st;
-  HIPCHK(launch_fisher_params(d_params, B, h->d_fq, h->stream));
-  HIPCHK(launch_setup(h->d_fq, B * FISHER_NV, h->an, h->d_fpts, h->d_fst, h->stream));
+  HIPCHK(launch_fisher_params(d_params, B, h->d_fq, s));
+  HIPCHK(launch_setup(h->d_fq, B * FISHER_NV, h->an, h->d_fpts, h->d_fst, s));
   HIPCHK(launch_het_fisher(h->d_fpts, d_params, B, h->nd, h->d_freq, h->d_pl, h->d_amp, h->d_w, h->M, d_fish,
-                           d_status, h->stream));
-  if (d_ej && d_ev) HIPCHK(launch_fisher_evec(d_fish, B, FISHER_NP, d_ej, d_ev, nullptr, h->stream));
+                           d_status, s));
+  if (d_ej && d_ev) HIPCHK(launch_fisher_evec(d_fish, B, FISHER_NP, d_ej, d_ev, nullptr, s));
   return QCBC_OK;
 }
 
@@ -228,6 +228,18 @@ int qcbc::het_enqueue(qcbc_het* h, const double* d_params, int B, Point* pts, in
 }
 
 int qcbc::het_device(const qcbc_het* h) { return h->device; }
+
+int qcbc::het_fisher_enqueue(qcbc_het* h, const double* d_params, int B, double* d_ej, double* d_ev, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(h->mu);
+  int st = fisher_reserve(h, B);
+  if (st) return st;
+  // the het's own Fisher scratch: wait for any earlier use of it on the het's stream
+  HIPCHK(hipStreamSynchronize(h->stream));
+  st = fisher_eval(h, d_params, B, h->d_fish, d_ej, d_ev, h->d_fstatus, s);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(s));  // the scratch is free again when this returns
+  return QCBC_OK;
+}
 
 extern "C" {
 
@@ -405,7 +417,8 @@ int qcbc_het_fisher(qcbc_het* h, const double* params, int B, double* fish, doub
   if (st) return st;
   const size_t np = FISHER_NP;
   HIPCHK(hipMemcpyAsync(h->d_fp, params, sizeof(double) * np * B, hipMemcpyHostToDevice, h->stream));
-  st = fisher_eval(h, h->d_fp, B, h->d_fish, ej ? h->d_ej : nullptr, ev ? h->d_ev : nullptr, h->d_fstatus);
+  st = fisher_eval(h, h->d_fp, B, h->d_fish, ej ? h->d_ej : nullptr, ev ? h->d_ev : nullptr, h->d_fstatus,
+                   h->stream);
   if (st) return st;
   HIPCHK(hipMemcpyAsync(fish, h->d_fish, sizeof(double) * np * np * B, hipMemcpyDeviceToHost, h->stream));
   if (ej) {
@@ -425,7 +438,7 @@ int qcbc_het_fisher_device(qcbc_het* h, const double* d_params, int B, double* d
   hipSetDevice(h->device);
   int st = fisher_reserve(h, B);
   if (st) return st;
-  return fisher_eval(h, d_params, B, d_fish, d_ej, d_ev, d_status ? d_status : h->d_fstatus);
+  return fisher_eval(h, d_params, B, d_fish, d_ej, d_ev, d_status ? d_status : h->d_fstatus, h->stream);
 }
 
 int qcbc_het_fisher_time(qcbc_het* h, const double* d_params, int B, int reps, double* ms) {

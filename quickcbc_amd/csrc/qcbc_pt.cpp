@@ -22,6 +22,7 @@
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
 #include "qcbc_internal.h"
+#include "qcbc_update_internal.h"
 
 using namespace qcbc;
 
@@ -57,6 +58,17 @@ struct qcbc_pt {
   hipGraphExec_t exec[2] = {};
   bool step_timing = false;
   std::vector<double> lat_ms;
+  // QuickCBC's own step (update() and MCMC_all's swaps) instead of the
+  // jiggle step: Fisher jump sizes and directions per point (storage order,
+  // moved with the state on swaps), the history of each (column,
+  // temperature) (column-major: column c's rows at c n_temps n_hist), its
+  // fill count per column, the proposal type of each point and per-column
+  // statistics
+  bool upd = false;
+  UpdateConst uc{};
+  double *d_ej = nullptr, *d_ev = nullptr, *d_hist = nullptr;
+  unsigned long long *d_hcount = nullptr, *d_ustats = nullptr;
+  int* d_typ = nullptr;
 };
 
 #define HIPCHK(x)                \
@@ -77,6 +89,16 @@ static int group_step(qcbc_pt* pt, int k) {
   Group& gr = pt->g[k];
   const int prec = pt->precision;
   const size_t o = gr.off;
+  if (pt->upd) {  // heterodyned logL of y, then QuickCBC's step
+    int st = het_enqueue(pt->het, pt->d_y + 11 * o, gr.P, gr.d_pts, pt->d_st + o, pt->d_Ly + o, gr.s);
+    if (st) return st;
+    HIPCHK(launch_pt_update(pt->d_x + 11 * o, pt->d_L + o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
+                            pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_hcount,
+                            pt->d_y + 11 * o, pt->d_Ly + o, pt->d_st + o, pt->d_typ + o, pt->d_tc, pt->uc,
+                            pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_ustats, gr.d_done,
+                            gr.s));
+    return QCBC_OK;
+  }
   if (pt->het) {  // set-up and heterodyned logL of y, then the same post kernel
     int st = het_enqueue(pt->het, pt->d_y + 11 * o, gr.P, gr.d_pts, pt->d_st + o, pt->d_Ly + o, gr.s);
     if (st) return st;
@@ -122,8 +144,9 @@ static void pt_free(qcbc_pt* pt) {
     if (pt->exec[i]) (void)hipGraphExecDestroy(pt->exec[i]);
     if (pt->graph[i]) (void)hipGraphDestroy(pt->graph[i]);
   }
-  void* bufs[] = {pt->d_x, pt->d_L, pt->d_y, pt->d_Ly, pt->d_rho, pt->d_sigma, pt->d_tc, pt->d_st, pt->d_acc,
-                  pt->d_swaps};
+  void* bufs[] = {pt->d_x,     pt->d_L,  pt->d_y,  pt->d_Ly,   pt->d_rho,    pt->d_sigma,  pt->d_tc,
+                  pt->d_st,    pt->d_acc, pt->d_swaps, pt->d_ej, pt->d_ev, pt->d_hist, pt->d_hcount,
+                  pt->d_ustats, pt->d_typ};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete pt;
@@ -131,8 +154,30 @@ static void pt_free(qcbc_pt* pt) {
 
 extern "C" {
 
+static int pt_sync(qcbc_pt* pt);
+
+// Proposals of the current step of every group, QuickCBC's kind (on stream s)
+static int update_propose_all(qcbc_pt* pt, hipStream_t s) {
+  for (int k = 0; k < pt->G; k++) {
+    const Group& gr = pt->g[k];
+    const size_t o = gr.off;
+    HIPCHK(launch_update_propose(pt->d_x + 11 * o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
+                                 pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_tc, pt->uc,
+                                 pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_y + 11 * o,
+                                 pt->d_typ + o, s));
+  }
+  return QCBC_OK;
+}
+
 static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, double ladder, const double* init,
-                     const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out) {
+                     const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out,
+                     const qcbc_update_desc* ud = nullptr) {
+  if (ud && (!het || !(ud->a >= ud->b && ud->b >= ud->c && ud->c == 0.0 && ud->a <= 1.0 && ud->b >= 0.0) ||
+             ud->n_hist < 2 || ud->hist_every < 1 || !(ud->etamin >= 0.0)))
+    return QCBC_EINVAL;
+  if (ud)
+    for (int i = 0; i < 11; i++)
+      if (!(ud->min[i] < ud->max[i])) return QCBC_EINVAL;
   if (!ctx || !out || !init || !sigma || n_temps < 1 || n_chains < 1 || n_chains > 1024 || !(ladder >= 1.0) ||
       (precision != QCBC_FP64 && precision != QCBC_MIXED) || groups < 0 || groups > QCBC_PT_MAX_GROUPS ||
       groups > n_chains)
@@ -152,6 +197,19 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
   pt->seed = seed;
   pt->precision = precision;
   pt->G = groups ? groups : (n_chains >= 3 && pt->P >= 1536 ? 3 : 1);
+  if (ud) {
+    pt->upd = true;
+    for (int i = 0; i < 11; i++) {
+      pt->uc.min[i] = ud->min[i];
+      pt->uc.max[i] = ud->max[i];
+    }
+    pt->uc.a = ud->a;
+    pt->uc.b = ud->b;
+    pt->uc.c = ud->c;
+    pt->uc.etamin = ud->etamin;
+    pt->uc.n_hist = ud->n_hist;
+    pt->uc.hist_every = ud->hist_every;
+  }
   const size_t P = pt->P;
   int c0 = 0, off = 0;
   for (int k = 0; k < pt->G; k++) {
@@ -179,6 +237,13 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
             hipMalloc(&pt->d_st, sizeof(int) * P) == hipSuccess &&
             hipMalloc(&pt->d_acc, sizeof(unsigned long long) * P) == hipSuccess &&
             hipMalloc(&pt->d_swaps, sizeof(unsigned long long) * n_chains) == hipSuccess;
+  if (ok && pt->upd)
+    ok = hipMalloc(&pt->d_ej, sizeof(double) * 11 * P) == hipSuccess &&
+         hipMalloc(&pt->d_ev, sizeof(double) * 121 * P) == hipSuccess &&
+         hipMalloc(&pt->d_hist, sizeof(double) * 11 * P * (size_t)pt->uc.n_hist) == hipSuccess &&
+         hipMalloc(&pt->d_hcount, sizeof(unsigned long long) * n_chains) == hipSuccess &&
+         hipMalloc(&pt->d_ustats, sizeof(unsigned long long) * UPD_STATS * n_chains) == hipSuccess &&
+         hipMalloc(&pt->d_typ, sizeof(int) * P) == hipSuccess;
   for (int k = 0; k < pt->G && ok; k++) {
     Group& gr = pt->g[k];
     ok = hipMalloc(&gr.d_pts, sizeof(Point) * gr.P) == hipSuccess &&
@@ -226,6 +291,28 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
     st = loglike_core(ctx, pt->d_x, pt->P, precision, pt->d_L, pt->d_rho, pt->d_st);
   }
   if (st) return fail(st);
+  if (pt->upd) {
+    // MCMC_all's start (QuickCBC.c:1918-1954): Fisher jumps at every state,
+    // each column's history from its own states, then the proposals of step 0
+    if (launch_fill_u64(pt->d_hcount, n_chains, 0ULL, s) != hipSuccess ||
+        launch_fill_u64(pt->d_ustats, UPD_STATS * n_chains, 0ULL, s) != hipSuccess)
+      return fail(QCBC_EHIP);
+    for (int k = 0; k < pt->G; k++) {
+      const Group& gr = pt->g[k];
+      if (launch_update_hist_init(pt->d_x + 11 * (size_t)gr.off, n_temps, n_chains, gr.c0, gr.ncols,
+                                  pt->uc.n_hist, seed,
+                                  pt->d_hist + (size_t)gr.c0 * n_temps * pt->uc.n_hist * 11, s) != hipSuccess)
+        return fail(QCBC_EHIP);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
+    st = het_fisher_enqueue(het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, s);
+    if (st) return fail(st);
+    st = update_propose_all(pt, s);
+    if (st) return fail(st);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
+    *out = pt;
+    return QCBC_OK;
+  }
   // proposals of step 0
   for (int k = 0; k < pt->G; k++) {
     const Group& gr = pt->g[k];
@@ -247,6 +334,80 @@ int qcbc_pt_create_het(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, 
                        const double* sigma, uint64_t seed, int groups, qcbc_pt** out) {
   if (!het) return QCBC_EINVAL;
   return pt_create(ctx, het, n_temps, n_chains, ladder, init, sigma, seed, QCBC_FP64, groups, out);
+}
+
+int qcbc_pt_create_update(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, double ladder,
+                          const double* init, const qcbc_update_desc* desc, uint64_t seed, int groups,
+                          qcbc_pt** out) {
+  if (!het || !desc) return QCBC_EINVAL;
+  static const double unit[11] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};  // unused by this step
+  return pt_create(ctx, het, n_temps, n_chains, ladder, init, unit, seed, QCBC_FP64, groups, out, desc);
+}
+
+int qcbc_pt_refresh_fisher(qcbc_pt* pt) {
+  if (!pt || !pt->upd) return QCBC_EINVAL;
+  qcbc_ctx* c = pt->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = pt_sync(pt);
+  if (st) return st;
+  st = het_fisher_enqueue(pt->het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, c->stream);
+  if (st) return st;
+  // the pending proposals were made with the old directions (the reference
+  // refreshes before its update, QuickCBC.c:2019-2027)
+  st = update_propose_all(pt, c->stream);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QCBC_OK;
+}
+
+int qcbc_pt_get_update(qcbc_pt* pt, double* ej, double* ev, double* proposals, int* types, long long* stats) {
+  if (!pt || !pt->upd) return QCBC_EINVAL;
+  qcbc_ctx* c = pt->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = pt_sync(pt);
+  if (st) return st;
+  hipStream_t s = c->stream;
+  const size_t P = pt->P;
+  std::vector<double> e(11 * P), v(121 * P), y(11 * P);
+  std::vector<int> ty(P);
+  std::vector<unsigned long long> us((size_t)UPD_STATS * pt->n_chains);
+  HIPCHK(hipMemcpyAsync(e.data(), pt->d_ej, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(v.data(), pt->d_ev, sizeof(double) * 121 * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(y.data(), pt->d_y, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(ty.data(), pt->d_typ, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(us.data(), pt->d_ustats, sizeof(unsigned long long) * us.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < P; i++) {
+    const size_t p = pt->perm[i];
+    if (ej) std::memcpy(ej + 11 * p, &e[11 * i], sizeof(double) * 11);
+    if (ev) std::memcpy(ev + 121 * p, &v[121 * i], sizeof(double) * 121);
+    if (proposals) std::memcpy(proposals + 11 * p, &y[11 * i], sizeof(double) * 11);
+    if (types) types[p] = ty[i];
+  }
+  if (stats)
+    for (size_t i = 0; i < us.size(); i++) stats[i] = (long long)us[i];
+  return QCBC_OK;
+}
+
+int qcbc_pt_get_history(qcbc_pt* pt, double* hist, long long* counts) {
+  if (!pt || !pt->upd) return QCBC_EINVAL;
+  qcbc_ctx* c = pt->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = pt_sync(pt);
+  if (st) return st;
+  hipStream_t s = c->stream;
+  std::vector<unsigned long long> hc(pt->n_chains);
+  if (hist)
+    HIPCHK(hipMemcpyAsync(hist, pt->d_hist, sizeof(double) * 11 * (size_t)pt->P * pt->uc.n_hist,
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hc.data(), pt->d_hcount, sizeof(unsigned long long) * pt->n_chains, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (counts)
+    for (int i = 0; i < pt->n_chains; i++) counts[i] = (long long)hc[i];
+  return QCBC_OK;
 }
 
 int qcbc_pt_create(qcbc_ctx* ctx, int n_temps, int n_chains, double ladder, const double* init, const double* sigma,

@@ -1,0 +1,50 @@
+// QuickCBC's PT step (update(), QuickCBC.c:2238-2405; MCMC_all's step,
+// :2029-2092) on the GPU: constants, random-draw layout and launch wrappers
+// (qcbc_update.hip).  Host-safe: no device code in here.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace qcbc {
+
+// Prior box and proposal mix of one ensemble.
+struct UpdateConst {
+  double min[11], max[11];  // prior bounds per parameter (update()'s min, max)
+  double a, b, c;           // proposal thresholds on alpha (0.5, 0.3, 0.2 in the reference; c = 0 here)
+  double etamin;            // lower limit of the symmetric mass ratio (ConstCBC.h:57)
+  int n_hist;               // history rows per temperature (NH, ConstCBC.h:72)
+  int hist_every;           // steps between history entries (20, QuickCBC.c:2071)
+};
+
+// Random streams (xor-ed into the seed) and draw indices within a stream.
+constexpr uint64_t SALT_PROPOSE = 0x6a09e667f3bcc908ULL;
+constexpr uint64_t SALT_ACCEPT = 0xbb67ae8584caa73bULL;
+constexpr uint64_t SALT_SWAP = 0x3c6ef372fe94f82bULL;
+constexpr uint64_t SALT_HIST = 0xa54ff53a5f1d36f1ULL;
+constexpr int D_ALPHA = 0;     // proposal type
+constexpr int D_FISHER = 1;    // eigendirection, then a normal (2 draws)
+constexpr int D_DE = 4;        // history row i, then row j: one draw per try
+constexpr int DE_TRIES = 32;
+constexpr int D_DE_SCALE = 40; // whether the scale is N(0, 0.5), then a normal
+constexpr int D_JIGGLE = 50;   // the scale's exponent, then 11 normals (2 draws each)
+
+// angles wrapped into range after a jump only within this magnitude (rad)
+constexpr double ANGLE_WRAP_MAX = 1.0e4;
+
+// per column: update steps, swap steps, swaps accepted, then (proposed,
+// accepted) for the Fisher, DE and jiggle proposals
+constexpr int UPD_STATS = 9;
+
+hipError_t launch_update_hist_init(const double* x, int n_temps, int n_chains, int c0, int ncols, int n_hist,
+                                   uint64_t seed, double* hist, hipStream_t s);
+hipError_t launch_update_propose(const double* x, const double* ej, const double* ev, const double* hist,
+                                 const double* tc, const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols,
+                                 uint64_t seed, const unsigned long long* step, double* y, int* typ, hipStream_t s);
+hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
+                            double* y, const double* Ly, const int* st_y, int* typ, const double* tc,
+                            const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
+                            unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s);
+
+}  // namespace qcbc

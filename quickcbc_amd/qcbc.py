@@ -450,6 +450,73 @@ class PT:
             pass
 
 
+class UpdateDesc(ctypes.Structure):
+    """qcbc_update_desc: prior box and proposal mix of QuickCBC's own PT step."""
+    _fields_ = [("min", ctypes.c_double * 11), ("max", ctypes.c_double * 11), ("a", ctypes.c_double),
+                ("b", ctypes.c_double), ("c", ctypes.c_double), ("etamin", ctypes.c_double),
+                ("n_hist", ctypes.c_int), ("hist_every", ctypes.c_int)]
+
+
+def update_desc(pmin, pmax, a=0.5, b=0.3, c=0.0, etamin=0.05, n_hist=1000, hist_every=20):
+    d = UpdateDesc()
+    d.min[:] = [float(v) for v in pmin]
+    d.max[:] = [float(v) for v in pmax]
+    d.a, d.b, d.c, d.etamin, d.n_hist, d.hist_every = a, b, c, etamin, n_hist, hist_every
+    return d
+
+
+class PTUpdate(PT):
+    """A resident ensemble stepped by QuickCBC's own update() and MCMC_all
+    step on the heterodyned likelihood (qcbc_pt_create_update)."""
+
+    def __init__(self, ctx, het, n_temps, n_chains, ladder, init, desc, seed=1, groups=0):
+        L = lib()
+        L.qcbc_pt_create_update.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp,
+                                            ctypes.POINTER(UpdateDesc), ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.POINTER(_vp)]
+        L.qcbc_pt_refresh_fisher.argtypes = [_vp]
+        L.qcbc_pt_get_update.argtypes = [_vp, _dp, _dp, _dp, _ip, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_pt_get_history.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_pt_groups.argtypes = [_vp, _ip]
+        L.qcbc_pt_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.qcbc_pt_get.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_pt_latencies.argtypes = [_vp, _dp, ctypes.c_int, _ip]
+        L.qcbc_pt_set_step_timing.argtypes = [_vp, ctypes.c_int]
+        L.qcbc_pt_destroy.argtypes = [_vp]
+        self.ctx = ctx
+        self.het = het
+        self.n_temps, self.n_chains = n_temps, n_chains
+        self.P = n_temps * n_chains
+        self.desc = desc
+        init = np.ascontiguousarray(init, np.float64).reshape(self.P, 11)
+        h = _vp()
+        _check(L.qcbc_pt_create_update(ctx.h, het.h, n_temps, n_chains, ladder, _d(init), ctypes.byref(desc), seed,
+                                       groups, ctypes.byref(h)), "qcbc_pt_create_update")
+        self.h = h
+        g = ctypes.c_int()
+        _check(L.qcbc_pt_groups(h, ctypes.byref(g)), "qcbc_pt_groups")
+        self.groups = g.value
+
+    def refresh_fisher(self):
+        _check(lib().qcbc_pt_refresh_fisher(self.h), "qcbc_pt_refresh_fisher")
+
+    def get_update(self):
+        """(ej P x 11, ev P x 11 x 11, proposals P x 11, types P, stats n_chains x 9), point order."""
+        ej = np.zeros((self.P, 11))
+        ev = np.zeros((self.P, 11, 11))
+        y = np.zeros((self.P, 11))
+        ty = np.zeros(self.P, np.int32)
+        st = (ctypes.c_longlong * (9 * self.n_chains))()
+        _check(lib().qcbc_pt_get_update(self.h, _d(ej), _d(ev), _d(y), _i(ty), st), "qcbc_pt_get_update")
+        return ej, ev, y, ty, np.array(st[:], np.int64).reshape(self.n_chains, 9)
+
+    def get_history(self):
+        hist = np.zeros((self.n_chains, self.n_temps, self.desc.n_hist, 11))
+        cnt = (ctypes.c_longlong * self.n_chains)()
+        _check(lib().qcbc_pt_get_history(self.h, _d(hist), cnt), "qcbc_pt_get_history")
+        return hist, np.array(cnt[:], np.int64)
+
+
 # ------------------------------------------------------- device buffers
 class DeviceBuffer:
     """Device memory from qcbc_dev_alloc (for the device-pointer entry points)."""

@@ -209,3 +209,29 @@ def het_grid(MN, MM, Tobs, J=2, frac=1.0 / 64.0, max_hz=100.0):
         g += [k + s * (i + 1) for i in range(J)]
         k += J * s
     return np.asarray(g, dtype=np.int32)
+
+
+def prior_box(an: Analysis):
+    """MCMC_all's prior box (QuickCBC.c:1869-1892) with ConstCBC.h's limits:
+    chirp mass 0.5-160 Msun, total mass 0.5-300 Msun, |chi| <= smax = 0.85,
+    DL 1e6-1e10 pc, and the peak-time window [Tobs - 3.5 (at least 1.5),
+    Tobs - 0.5] of a trigger at Tobs - 2 (QuickCBC.c:182-184)."""
+    smax = 0.85
+    tmax = an.Tobs - 0.5
+    tmin = max(an.Tobs - 3.5, 1.5)
+    lo = np.array([np.log(0.5 * MSUN_SI), np.log(0.5 * MSUN_SI), -smax, -smax, 0.0, tmin, np.log(1e6 * PC_SI),
+                   0.0, -1.0, 0.0, -1.0])
+    hi = np.array([np.log(160.0 * MSUN_SI), np.log(300.0 * MSUN_SI), smax, smax, 2 * np.pi, tmax,
+                   np.log(1e10 * PC_SI), 2 * np.pi, 1.0, np.pi, 1.0])
+    return lo, hi
+
+
+def ensemble_in_prior(center, n, lo, hi, seed=99, scale=1.0):
+    """ensemble_around, with every point moved inside the prior box (angles
+    wrapped, the rest clipped just inside)."""
+    P = ensemble_around(center, n, seed=seed, scale=scale)
+    P[:, 4] = np.mod(P[:, 4], 2 * np.pi)
+    P[:, 7] = np.mod(P[:, 7], 2 * np.pi)
+    P[:, 9] = np.mod(P[:, 9], np.pi)
+    eps = 1e-6 * (hi - lo)
+    return np.clip(P, lo + eps, hi - eps)
