@@ -91,6 +91,8 @@ def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
            "kernel": "k_het_ll<%d>" % an.nifo, "points": B, "grid_samples": het.M, "regions": het.NR,
            "band_bins": [MN, MM], "setup_ms": setup_ms,
            "het_vs_full_abs_dlogl_median": percentile(np.abs(Lh - Lf), 50)}
+    out["fisher"] = fisher_bench(Q, het, P, dp, launches, no_cpu, an, D, SN, inj, g, MN, MM)
+    out["pt_update"] = update_bench(Q, W, an, ctx, het, inj, P.shape[0], no_cpu, D, SN, g, MN, MM)
     het.close()
     dp.free()
     dl.free()
@@ -109,6 +111,110 @@ def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
         out["cpu_baseline"] = {"value": n2 / dt, "unit": "logL evals/s", "cores": O.lib().qo_num_threads(),
                                "kind": "port", "sample": "%d points of the same ensemble, %.1f s, gcc -O2 -fopenmp, "
                                                          "OpenMP over points" % (n2, dt)}
+    return out
+
+
+def fisher_bench(Q, het, P, dp, launches, no_cpu, an, D, SN, inj, g, MN, MM):
+    """Fisher_Het + FisherEvec (QuickCBC.c:5446-5617, :2449-2546) at every
+    point of the ensemble on the same heterodyne: the set-up of the 23
+    perturbed parameter sets per point, the Fisher kernel and the eigen kernel,
+    each timed alone (HIP events around back-to-back launches), and the whole
+    host call."""
+    B = P.shape[0]
+    reps = max(5, launches // 50)
+    het.fisher_time(dp.ptr, B, 2)
+    setup_ms, fisher_ms, evec_ms = het.fisher_time(dp.ptr, B, reps)
+    het.fisher(P)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        het.fisher(P)
+    call_ms = (time.perf_counter() - t0) / 5 * 1e3
+    out = {"points": B, "setup_ms": setup_ms, "fisher_kernel_ms": fisher_ms, "evec_kernel_ms": evec_ms,
+           "call_ms": call_ms, "points_per_s": B / (call_ms * 1e-3),
+           "kernels": ["k_setup x 23 points", "k_het_fisher<%d>" % an.nifo, "k_fisher_evec"]}
+    if not no_cpu:
+        from oracle import oracle as O
+        oh = _oracle_het(O, an, D, SN, inj, g, MN, MM)
+        n = 1024
+        t0 = time.perf_counter()
+        F, _ = O.fisher_het(an.labels, an.gmst, an.fref, an.Tobs, oh, P[:n])
+        O.fisher_evec(F)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "points/s (Fisher_Het + FisherEvec)",
+                               "cores": O.lib().qo_num_threads(), "kind": "port",
+                               "sample": "%d points of the same ensemble, %.2f s, OpenMP over points (Fisher), "
+                                         "one thread (eigen)" % (n, dt)}
+    return out
+
+
+def _oracle_het(O, an, D, SN, inj, g, MN, MM):
+    st, oh = O.heterodyne(an.labels, an.gmst, an.fref, D, SN, an.N, an.Tobs, inj, g, 2, MN, MM)
+    return oh
+
+
+def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=1000, lat_steps=200):
+    """QuickCBC's own PT step on the heterodyned likelihood (update()'s
+    Fisher / differential-evolution / jiggle proposals with the reference's
+    priors, MCMC_all's swap-or-update step, QuickCBC.c:2029-2092 and
+    :2238-2405) for the 32 x 64 ensemble: steps per second over `steps`
+    resident steps (eager and graph replay), the synchronised step latency,
+    the acceptance per proposal type, and the Fisher refresh."""
+    lo, hi = W.prior_box(an)
+    n_temps, n_chains = 32, B // 32
+    init = W.ensemble_in_prior(inj, n_temps * n_chains, lo, hi, seed=99)
+    desc = Q.update_desc(lo, hi)
+    t0 = time.perf_counter()
+    pt = Q.PTUpdate(ctx, het, n_temps, n_chains, 1.15, init, desc, seed=5)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    pt.run(20)
+    t0 = time.perf_counter()
+    pt.run(steps)
+    T = time.perf_counter() - t0
+    pt.run(64, graph=True)
+    t0 = time.perf_counter()
+    pt.run(steps, graph=True)
+    Tg = time.perf_counter() - t0
+    pt.step_timing(True)
+    pt.run(lat_steps)
+    lat = pt.latencies()
+    pt.step_timing(False)
+    t0 = time.perf_counter()
+    pt.refresh_fisher()
+    refresh_ms = (time.perf_counter() - t0) * 1e3
+    _, _, _, _, st = pt.get_update()
+    tot = st.sum(0)
+    out = {"points": n_temps * n_chains, "groups": pt.groups, "steps_per_s": steps / T,
+           "step_ms": T / steps * 1e3, "graph_steps_per_s": steps / Tg,
+           "logl_per_s": n_temps * n_chains * steps / T,
+           "step_ms_p50": percentile(lat * 1.0, 50), "step_ms_p99": percentile(lat * 1.0, 99),
+           "swap_step_fraction": float(tot[1] / max(1, tot[0] + tot[1])),
+           "swap_accept": float(tot[2] / max(1, tot[1])),
+           "accept": {k: float(tot[4 + 2 * i] / max(1, tot[3 + 2 * i])) for i, k in
+                      enumerate(("fisher", "de", "jiggle"))},
+           "proposed": {k: int(tot[3 + 2 * i]) for i, k in enumerate(("fisher", "de", "jiggle"))},
+           "fisher_refresh_ms": refresh_ms, "create_ms": create_ms}
+    pt.close()
+    if not no_cpu:
+        from oracle import oracle as O
+        oh = _oracle_het(O, an, D, SN, inj, g, MN, MM)
+        P = n_temps * n_chains
+        x = init.copy()
+        L, _ = O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, oh, x)
+        F, _ = O.fisher_het(an.labels, an.gmst, an.fref, an.Tobs, oh, x)
+        ej, ev, _ = O.fisher_evec(F)
+        hist = O.pt_update_hist_init(x, n_temps, n_chains, 1000, 5)
+        y = x.copy()
+        S = dict(x=x, L=L, ej=ej, ev=ev, hist=hist, hcount=np.zeros(n_chains, np.int64), y=y,
+                 typ=np.full(P, 3, np.int32), stats=np.zeros((n_chains, 9), np.int64), step=0)
+        uc = np.concatenate([lo, hi, [0.5, 0.3, 0.0, 0.05]])
+        n = 100
+        t0 = time.perf_counter()
+        O.pt_update_run(an.labels, an.gmst, an.fref, an.Tobs, oh, uc, 1000, 20, n_temps, n_chains, 1.15, 5, n, S)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": P * n / dt, "unit": "logL evals/s (whole PT steps)",
+                               "cores": O.lib().qo_num_threads(), "kind": "port",
+                               "sample": "%d steps of the same %d-point ensemble, %.2f s, OpenMP over points for "
+                                         "the likelihood" % (n, P, dt)}
     return out
 
 
@@ -134,6 +240,8 @@ def main():
     ap.add_argument("--draws", type=int, default=100000)
     ap.add_argument("--het-launches", type=int, default=500)
     ap.add_argument("--only-het", action="store_true", help="time the heterodyned likelihood only (for profiling)")
+    ap.add_argument("--only-update", action="store_true",
+                    help="time QuickCBC's own PT step on the heterodyned likelihood only (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how the "
@@ -174,6 +282,20 @@ def main():
     sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
     plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/3), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
+
+    if args.only_update:
+        A, _ = Q.dropin_fullphaseamp(an, an.freq(), inj)
+        MN, MM = W.snr_band(an, A, SN, Q.dropin_fbegin(inj))
+        g = W.het_grid(MN, MM, an.Tobs)
+        het = Q.Het(ctx, params=inj, fgrid=g, J=2, MN=MN, MM=MM)
+        dp = Q.DeviceBuffer.from_array(np.ascontiguousarray(init, np.float64))
+        r = {"fisher": fisher_bench(Q, het, init, dp, 100, True, an, D, SN, inj, g, MN, MM),
+             "pt_update": update_bench(Q, W, an, ctx, het, inj, B, True, D, SN, g, MN, MM, steps=args.steps,
+                                       lat_steps=20)}
+        print(json.dumps({"config": "QuickCBC PT step, heterodyned likelihood", **r}), flush=True)
+        het.close()
+        dp.free()
+        return
 
     if args.only_het:
         r = het_bench(Q, W, an, ctx, inj, D, SN, init, args.het_launches, 5.0, True)

@@ -80,71 +80,92 @@ __device__ double mass_jacobian(const double* x) {
   return 2.0 * x[1] + leta - 0.5 * log(1.0 - 4.0 * eta);
 }
 
-// update()'s proposal for the state x at temperature t (storage index lp,
-// point number gp) for step s: y, and the proposal type (1 Fisher, 2 DE,
-// 3 jiggle) returned
-__device__ int propose(const double* __restrict__ x, const double* __restrict__ ej, const double* __restrict__ ev,
-                       const double* __restrict__ hist, int n_hist, double sT, const UpdateConst& u, uint64_t seed,
-                       uint64_t ctr, double* __restrict__ y) {
+// Component j of update()'s proposal for the state x at temperature T
+// (sT = sqrt(T)) from the counters (seed, ctr); *typ gets the proposal type
+// (1 Fisher, 2 DE, 3 jiggle).  One lane per component: the draws a proposal
+// shares (its type, eigendirection, history rows, scale) are the same
+// counters on every lane, so each component is computed exactly as a
+// sequential loop over the components would compute it.
+__device__ double propose_component(const double* __restrict__ x, const double* __restrict__ ej,
+                                    const double* __restrict__ ev, const double* __restrict__ hist, int n_hist,
+                                    double sT, const UpdateConst& u, uint64_t seed, uint64_t ctr, int j, int* typ) {
   const double alpha = uni(seed, SALT_PROPOSE, ctr, D_ALPHA);
-  int typ;
-  for (int j = 0; j < 11; j++) y[j] = x[j];
+  double y;
   if (alpha > u.a) {  // Fisher jump along an eigendirection
-    typ = 1;
+    *typ = 1;
     const int i = (int)(uni(seed, SALT_PROPOSE, ctr, D_FISHER) * 11.0);
     const double beta = sT * ej[i] * gauss(seed, SALT_PROPOSE, ctr, D_FISHER + 1);
-    for (int j = 0; j < 11; j++) y[j] = x[j] + beta * ev[11 * i + j];
+    y = x[j] + beta * ev[11 * i + j];
   } else if (alpha > u.b) {  // differential evolution from the temperature's history
-    typ = 2;
+    *typ = 2;
     const int i = (int)((double)n_hist * uni(seed, SALT_PROPOSE, ctr, D_DE));
-    int j = i;
-    for (int r = 0; r < DE_TRIES && j == i; r++) j = (int)((double)n_hist * uni(seed, SALT_PROPOSE, ctr, D_DE + 1 + r));
-    if (j == i) j = (i + 1) % n_hist;
+    int k = i;
+    for (int r = 0; r < DE_TRIES && k == i; r++) k = (int)((double)n_hist * uni(seed, SALT_PROPOSE, ctr, D_DE + 1 + r));
+    if (k == i) k = (i + 1) % n_hist;
     double g = 1.0;
     if (uni(seed, SALT_PROPOSE, ctr, D_DE_SCALE) < 0.9) g = 0.5 * gauss(seed, SALT_PROPOSE, ctr, D_DE_SCALE + 1);
-    for (int k = 0; k < 11; k++) y[k] = x[k] + g * (hist[11 * i + k] - hist[11 * j + k]);
+    y = x[j] + g * (hist[11 * i + j] - hist[11 * k + j]);
   } else {  // jiggle (alpha > c; the sky ring below c is not built and c is 0)
-    typ = 3;
+    *typ = 3;
     const double beta = 0.01 * pow(10.0, -floor(3.0 * uni(seed, SALT_PROPOSE, ctr, D_JIGGLE))) * sT;
-    for (int k = 0; k < 11; k++) y[k] = x[k] + beta * gauss(seed, SALT_PROPOSE, ctr, D_JIGGLE + 1 + 2 * k);
+    y = x[j] + beta * gauss(seed, SALT_PROPOSE, ctr, D_JIGGLE + 1 + 2 * j);
   }
   // angles back into range (:2336-2342).  Only an angle within +-ANGLE_WRAP_MAX
   // rad is wrapped, so each loop ends within ~1600 turns; one beyond (or not
   // finite) stays out of its prior range and the proposal is rejected.
-  if (fabs(y[4]) <= ANGLE_WRAP_MAX) {
-    while (y[4] > TPI_U) y[4] -= TPI_U;
-    while (y[4] < 0.0) y[4] += TPI_U;
+  if ((j == 4 || j == 7) && fabs(y) <= ANGLE_WRAP_MAX) {
+    while (y > TPI_U) y -= TPI_U;
+    while (y < 0.0) y += TPI_U;
+  } else if (j == 9 && fabs(y) <= ANGLE_WRAP_MAX) {
+    while (y > PI_U) y -= PI_U;
+    while (y < 0.0) y += PI_U;
   }
-  if (fabs(y[7]) <= ANGLE_WRAP_MAX) {
-    while (y[7] > TPI_U) y[7] -= TPI_U;
-    while (y[7] < 0.0) y[7] += TPI_U;
-  }
-  if (fabs(y[9]) <= ANGLE_WRAP_MAX) {
-    while (y[9] > PI_U) y[9] -= PI_U;
-    while (y[9] < 0.0) y[9] += PI_U;
-  }
-  return typ;
+  return y;
 }
 
-// Proposals of step *step for the group's points (used at creation; later
-// steps get theirs from k_pt_update).  One thread per point.
-__global__ void k_update_propose(const double* __restrict__ x, const double* __restrict__ ej,
-                                 const double* __restrict__ ev, const double* __restrict__ hist,
-                                 const double* __restrict__ tc, UpdateConst u, int n_temps, int n_chains, int c0,
-                                 int ncols, uint64_t seed, const unsigned long long* __restrict__ step,
-                                 double* __restrict__ y, int* __restrict__ typ) {
-  const int lp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lp >= n_temps * ncols) return;
-  const int t = lp / ncols, c = c0 + lp % ncols;
-  const uint64_t P = (uint64_t)n_temps * n_chains, gp = (uint64_t)t * n_chains + c;
-  const double* h = hist + ((size_t)(c - c0) * n_temps + t) * (size_t)u.n_hist * 11;
-  typ[lp] = propose(x + 11 * lp, ej + 11 * lp, ev + 121 * lp, h, u.n_hist, tc[3 * t], u, seed, *step * P + gp,
-                    y + 11 * lp);
+// lanes per point of the proposal loops (11 used)
+constexpr int PROP_LANES = 16;
+
+// The proposals of step s for every temperature of the group's column b
+// (storage lp = t ncols + b): thread l of the caller's nthreads is lane
+// l % PROP_LANES of temperature l / PROP_LANES, then of every nthreads /
+// PROP_LANES-th temperature after it.
+__device__ void propose_column(const double* __restrict__ x, const double* __restrict__ ej,
+                               const double* __restrict__ ev, const double* __restrict__ hist,
+                               const double* __restrict__ tc, const UpdateConst& u, int n_temps, int n_chains,
+                               int c0, int ncols, int b, uint64_t seed, uint64_t s, int l, int nthreads,
+                               double* __restrict__ y, int* __restrict__ typ) {
+  const uint64_t P = (uint64_t)n_temps * n_chains;
+  const int per = nthreads / PROP_LANES, sub = l / PROP_LANES, j = l % PROP_LANES;
+  if (j >= 11) return;
+  for (int t = sub; t < n_temps; t += per) {
+    const int lp = t * ncols + b;
+    const uint64_t gp = (uint64_t)t * n_chains + c0 + b;
+    const double* h = hist + ((size_t)b * n_temps + t) * (size_t)u.n_hist * 11;
+    int ty;
+    const double yj = propose_component(x + 11 * lp, ej + 11 * lp, ev + 121 * lp, h, u.n_hist, tc[3 * t], u, seed,
+                                        s * P + gp, j, &ty);
+    y[11 * lp + j] = yj;
+    if (j == 0) typ[lp] = ty;
+  }
+}
+
+// Proposals of step *step for the group's points (used at creation and after
+// a Fisher refresh; later steps get theirs from k_pt_update).  One block per
+// column.
+__global__ __launch_bounds__(512) void k_update_propose(const double* __restrict__ x, const double* __restrict__ ej,
+                                                        const double* __restrict__ ev, const double* __restrict__ hist,
+                                                        const double* __restrict__ tc, UpdateConst u, int n_temps,
+                                                        int n_chains, int c0, int ncols, uint64_t seed,
+                                                        const unsigned long long* __restrict__ step,
+                                                        double* __restrict__ y, int* __restrict__ typ) {
+  propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, blockIdx.x, seed, *step, threadIdx.x,
+                 blockDim.x, y, typ);
 }
 
 // One PT step of a group of chain columns after the heterodyned logL of the
 // proposals (Ly, status st_y): block b owns column c0 + b.
-__global__ __launch_bounds__(256) void k_pt_update(
+__global__ __launch_bounds__(512) void k_pt_update(
     double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej, double* __restrict__ ev,
     double* __restrict__ hist, unsigned long long* __restrict__ hcount, double* __restrict__ y,
     const double* __restrict__ Ly, const int* __restrict__ st_y, int* __restrict__ typ, const double* __restrict__ tc,
@@ -225,13 +246,8 @@ __global__ __launch_bounds__(256) void k_pt_update(
   }
   __syncthreads();
   // proposals of step s + 1
-  for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
-    const int lp = t * ncols + b;
-    const uint64_t gp = (uint64_t)t * n_chains + c;
-    const double* h = hist + ((size_t)b * n_temps + t) * (size_t)u.n_hist * 11;
-    typ[lp] = propose(x + 11 * lp, ej + 11 * lp, ev + 121 * lp, h, u.n_hist, tc[3 * t], u, seed, (s + 1) * P + gp,
-                      y + 11 * lp);
-  }
+  propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, b, seed, s + 1, threadIdx.x, blockDim.x, y,
+                 typ);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -261,6 +277,12 @@ __global__ void k_update_hist_init(const double* __restrict__ x, int n_temps, in
 
 }  // namespace
 
+// threads per column block: PROP_LANES per temperature, whole waves, at most 512
+static int update_threads(int n_temps) {
+  const int n = n_temps * PROP_LANES;
+  return n >= 512 ? 512 : ((n + 63) / 64) * 64;
+}
+
 hipError_t launch_update_hist_init(const double* x, int n_temps, int n_chains, int c0, int ncols, int n_hist,
                                    uint64_t seed, double* hist, hipStream_t s) {
   const size_t n = (size_t)ncols * n_temps * n_hist;
@@ -272,9 +294,8 @@ hipError_t launch_update_hist_init(const double* x, int n_temps, int n_chains, i
 hipError_t launch_update_propose(const double* x, const double* ej, const double* ev, const double* hist,
                                  const double* tc, const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols,
                                  uint64_t seed, const unsigned long long* step, double* y, int* typ, hipStream_t s) {
-  const int n = n_temps * ncols;
-  hipLaunchKernelGGL(k_update_propose, dim3((n + 127) / 128), dim3(128), 0, s, x, ej, ev, hist, tc, u, n_temps,
-                     n_chains, c0, ncols, seed, step, y, typ);
+  hipLaunchKernelGGL(k_update_propose, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, ej, ev, hist, tc, u,
+                     n_temps, n_chains, c0, ncols, seed, step, y, typ);
   return hipGetLastError();
 }
 
@@ -282,8 +303,7 @@ hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double
                             double* y, const double* Ly, const int* st_y, int* typ, const double* tc,
                             const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
                             unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s) {
-  const int threads = n_temps >= 256 ? 256 : ((n_temps + 63) / 64) * 64;
-  hipLaunchKernelGGL(k_pt_update, dim3(ncols), dim3(threads), 0, s, x, L, ej, ev, hist, hcount, y, Ly, st_y, typ, tc,
+  hipLaunchKernelGGL(k_pt_update, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, L, ej, ev, hist, hcount, y, Ly, st_y, typ, tc,
                      u, n_temps, n_chains, c0, ncols, seed, step, stats, done);
   return hipGetLastError();
 }
