@@ -154,9 +154,9 @@ def _oracle_het(O, an, D, SN, inj, g, MN, MM):
 
 def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=1000, lat_steps=200):
     """QuickCBC's own PT step on the heterodyned likelihood (update()'s
-    Fisher / differential-evolution / jiggle proposals with the reference's
-    priors, MCMC_all's swap-or-update step, QuickCBC.c:2029-2092 and
-    :2238-2405) for the 32 x 64 ensemble: steps per second over `steps`
+    Fisher / differential-evolution / jiggle / sky-ring proposals, 0.5 / 0.2 /
+    0.1 / 0.2, with the reference's priors, MCMC_all's swap-or-update step,
+    QuickCBC.c:2029-2092 and :2238-2405) for the 32 x 64 ensemble: steps per second over `steps`
     resident steps (eager and graph replay), the synchronised step latency,
     the acceptance per proposal type, and the Fisher refresh."""
     lo, hi = W.prior_box(an)
@@ -181,7 +181,7 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
     t0 = time.perf_counter()
     pt.refresh_fisher()
     refresh_ms = (time.perf_counter() - t0) * 1e3
-    _, _, _, _, st = pt.get_update()
+    _, _, _, _, _, st = pt.get_update()
     tot = st.sum(0)
     out = {"points": n_temps * n_chains, "groups": pt.groups, "steps_per_s": steps / T,
            "step_ms": T / steps * 1e3, "graph_steps_per_s": steps / Tg,
@@ -190,8 +190,8 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
            "swap_step_fraction": float(tot[1] / max(1, tot[0] + tot[1])),
            "swap_accept": float(tot[2] / max(1, tot[1])),
            "accept": {k: float(tot[4 + 2 * i] / max(1, tot[3 + 2 * i])) for i, k in
-                      enumerate(("fisher", "de", "jiggle"))},
-           "proposed": {k: int(tot[3 + 2 * i]) for i, k in enumerate(("fisher", "de", "jiggle"))},
+                      enumerate(("fisher", "de", "jiggle", "sky_ring"))},
+           "proposed": {k: int(tot[3 + 2 * i]) for i, k in enumerate(("fisher", "de", "jiggle", "sky_ring"))},
            "fisher_refresh_ms": refresh_ms, "create_ms": create_ms}
     pt.close()
     if not no_cpu:
@@ -205,8 +205,8 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
         hist = O.pt_update_hist_init(x, n_temps, n_chains, 1000, 5)
         y = x.copy()
         S = dict(x=x, L=L, ej=ej, ev=ev, hist=hist, hcount=np.zeros(n_chains, np.int64), y=y,
-                 typ=np.full(P, 3, np.int32), stats=np.zeros((n_chains, 9), np.int64), step=0)
-        uc = np.concatenate([lo, hi, [0.5, 0.3, 0.0, 0.05]])
+                 typ=np.full(P, 3, np.int32), jac=np.zeros(P), stats=np.zeros((n_chains, 11), np.int64), step=0)
+        uc = np.concatenate([lo, hi, [0.5, 0.3, 0.2, 0.05, an.gmst, an.nifo], list(an.labels) + [0] * (3 - an.nifo)])
         n = 100
         t0 = time.perf_counter()
         O.pt_update_run(an.labels, an.gmst, an.fref, an.Tobs, oh, uc, 1000, 20, n_temps, n_chains, 1.15, 5, n, S)

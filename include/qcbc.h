@@ -288,16 +288,18 @@ int qcbc_pt_create_het(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, 
  * per step and column, with probability 0.2 one swap of a random adjacent
  * pair of temperatures, else update() at every temperature (Fisher
  * eigen-jumps above alpha > a, differential evolution from the
- * temperature's history above b, jiggle above c), accepted by
- * H = dlogL/T + the mass Jacobian of a prior flat in m1, m2 + 3 dlnDL, inside
- * the prior box with eta in [etamin, 1/4].  The sky ring (alpha <= c) is not
- * built, so c must be 0 (the reference's alternative setting,
- * :2256-2260).  The Fisher jumps (Fisher_Het + FisherEvec) are computed at
- * creation and by qcbc_pt_refresh_fisher (the reference refreshes them every
- * 50000 iterations, :2019-2027; efix is not built). */
+ * temperature's history above b, jiggle above c, below c the sky ring:
+ * Ring_all, skymap_all and skydensity_all's log density), accepted by
+ * H = dlogL/T + the mass Jacobian of a prior flat in m1, m2 + 3 dlnDL (+ the
+ * sky-ring density), inside the prior box with eta in [etamin, 1/4].  The
+ * sky ring needs two detectors (c = 0 turns it off: the reference's
+ * alternative setting, :2256-2260).  The Fisher jumps (Fisher_Het +
+ * FisherEvec) are computed at creation and by qcbc_pt_refresh_fisher (the
+ * reference refreshes them every 50000 iterations, :2019-2027; efix is not
+ * built). */
 typedef struct {
   double min[11], max[11]; /* prior box (update()'s min, max) */
-  double a, b, c;          /* proposal thresholds: 0.5, 0.3, 0 */
+  double a, b, c;          /* proposal thresholds: 0.5, 0.3, 0.2 */
   double etamin;           /* lower limit of eta (0.05, ConstCBC.h:57) */
   int n_hist;              /* history rows per temperature (NH = 1000, ConstCBC.h:72) */
   int hist_every;          /* update steps between history entries (20, QuickCBC.c:2071) */
@@ -309,11 +311,12 @@ int qcbc_pt_create_update(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chain
  * remade with them. */
 int qcbc_pt_refresh_fisher(qcbc_pt *pt);
 /* State of a QuickCBC-step ensemble, in point order (t*n_chains + c): jump
- * sizes (P x 11), directions (P x 11 x 11), the pending proposals (P x 11)
- * and their types (1 Fisher, 2 DE, 3 jiggle); stats n_chains x 9 per column:
- * update steps, swap steps, swaps accepted, then proposed and accepted for
- * each type.  Any pointer may be NULL. */
-int qcbc_pt_get_update(qcbc_pt *pt, double *ej, double *ev, double *proposals, int *types,
+ * sizes (P x 11), directions (P x 11 x 11), the pending proposals (P x 11),
+ * their types (1 Fisher, 2 DE, 3 jiggle, 4 sky ring) and sky-ring log
+ * densities (P, 0 for the others); stats n_chains x 11 per column: update
+ * steps, swap steps, swaps accepted, then proposed and accepted for each
+ * type.  Any pointer may be NULL. */
+int qcbc_pt_get_update(qcbc_pt *pt, double *ej, double *ev, double *proposals, int *types, double *jac,
                        long long *stats);
 /* The histories (n_chains x n_temps x n_hist x 11) and their fill counts per
  * column (row count % n_hist is written next). */

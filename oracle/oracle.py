@@ -336,22 +336,61 @@ def pt_update_hist_init(x, n_temps, n_chains, n_hist, seed):
     return hist
 
 
-def update_propose(x, ej, ev, hist, sT, abc, seed, ctr):
-    """update()'s proposal from x with the counters (seed, ctr): (type, y)."""
+def update_propose(x, ej, ev, hist, sT, mix, seed, ctr):
+    """update()'s proposal from x with the counters (seed, ctr); mix = a, b, c, etamin, gmst, n_det,
+    labels[3]: (type, y, log density of a sky-ring map or 0)."""
     L = lib()
     L.qo_update_propose.restype = ctypes.c_int
     L.qo_update_propose.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_uint64,
-                                    ctypes.c_uint64, _dp]
-    a = [np.ascontiguousarray(v, np.float64) for v in (x, ej, ev, hist, abc)]
+                                    ctypes.c_uint64, _dp, _dp]
+    m = np.zeros(9)
+    mm = np.asarray(mix, np.float64)
+    m[:mm.size] = mm
+    a = [np.ascontiguousarray(v, np.float64) for v in (x, ej, ev, hist, m)]
     y = np.zeros(11)
-    t = L.qo_update_propose(_d(a[0]), _d(a[1]), _d(a[2]), _d(a[3]), a[3].size // 11, sT, _d(a[4]), seed, ctr, _d(y))
-    return t, y
+    j = np.zeros(1)
+    t = L.qo_update_propose(_d(a[0]), _d(a[1]), _d(a[2]), _d(a[3]), a[3].size // 11, sT, _d(a[4]), seed, ctr, _d(y),
+                            _d(j))
+    return t, y, float(j[0])
+
+
+def ring_all(px, d1, d2, gmst, u_rot):
+    """Ring_all (Utilities.c:1445): y = x with the sky location moved on the d1-d2 ring."""
+    L = lib()
+    L.qo_ring_all.restype = None
+    L.qo_ring_all.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    x = np.ascontiguousarray(px, np.float64)
+    y = x.copy()
+    L.qo_ring_all(_d(x), _d(y), d1, d2, gmst, u_rot)
+    return y
+
+
+def skymap_all(px, py, gmst, ifo1, ifo2):
+    """skymap_all (QuickCBC.c:7085): y's extrinsics for its sky location."""
+    L = lib()
+    L.qo_skymap_all.restype = None
+    L.qo_skymap_all.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    x = np.ascontiguousarray(px, np.float64)
+    y = np.ascontiguousarray(py, np.float64).copy()
+    L.qo_skymap_all(_d(x), _d(y), gmst, ifo1, ifo2)
+    return y
+
+
+def skydensity_all(px, py, gmst, ifo1, ifo2):
+    """skydensity_all (QuickCBC.c:7411): (density, y after skymap_all)."""
+    L = lib()
+    L.qo_skydensity_all.restype = ctypes.c_double
+    L.qo_skydensity_all.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    x = np.ascontiguousarray(px, np.float64)
+    y = np.ascontiguousarray(py, np.float64).copy()
+    d = L.qo_skydensity_all(_d(x), _d(y), gmst, ifo1, ifo2)
+    return d, y
 
 
 def pt_update_run(labels, gmst, fref, Tobs, het, uconst, n_hist, hist_every, n_temps, n_chains, ladder, seed,
                   nsteps, state):
-    """QuickCBC's PT step, nsteps times, on the state dict (x, L, ej, ev, hist, hcount, y, typ, stats, step),
-    updated in place; uconst = min[11] + max[11] + [a, b, c, etamin]."""
+    """QuickCBC's PT step, nsteps times, on the state dict (x, L, ej, ev, hist, hcount, y, typ, jac, stats,
+    step), updated in place; uconst = min[11] + max[11] + [a, b, c, etamin, gmst, n_det, labels[3]]."""
     L = lib()
     L.qo_pt_update_run.restype = None
     L.qo_pt_update_run.argtypes = ([ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -359,24 +398,24 @@ def pt_update_run(labels, gmst, fref, Tobs, het, uconst, n_hist, hist_every, n_t
                                    [ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_uint64, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp, _dp, _dp,
-                                    ctypes.POINTER(ctypes.c_longlong), _dp, _ip,
+                                    ctypes.POINTER(ctypes.c_longlong), _dp, _ip, _dp,
                                     ctypes.POINTER(ctypes.c_longlong)])
     labels = np.ascontiguousarray(labels, np.int32)
     arr = [np.ascontiguousarray(het[k], np.float64)
            for k in ("freq", "amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")]
     uc = np.ascontiguousarray(uconst, np.float64)
-    for k in ("x", "L", "ej", "ev", "hist", "y"):
+    for k in ("x", "L", "ej", "ev", "hist", "y", "jac"):
         state[k] = np.ascontiguousarray(state[k], np.float64)
     state["typ"] = np.ascontiguousarray(state["typ"], np.int32)
     hc = (ctypes.c_longlong * n_chains)(*[int(v) for v in state["hcount"]])
-    st = (ctypes.c_longlong * (9 * n_chains))(*[int(v) for v in np.asarray(state["stats"]).ravel()])
+    st = (ctypes.c_longlong * (11 * n_chains))(*[int(v) for v in np.asarray(state["stats"]).ravel()])
     step = ctypes.c_longlong(int(state["step"]))
     L.qo_pt_update_run(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"], *[_d(a) for a in arr],
                        het["logLb"], _d(uc), n_hist, hist_every, n_temps, n_chains, ladder, seed, nsteps,
                        ctypes.byref(step), *[_d(state[k]) for k in ("x", "L", "ej", "ev", "hist")], hc,
-                       _d(state["y"]), _i(state["typ"]), st)
+                       _d(state["y"]), _i(state["typ"]), _d(state["jac"]), st)
     state["hcount"] = np.array(hc[:], np.int64)
-    state["stats"] = np.array(st[:], np.int64).reshape(n_chains, 9)
+    state["stats"] = np.array(st[:], np.int64).reshape(n_chains, 11)
     state["step"] = step.value
     return state
 

@@ -1700,7 +1700,7 @@ void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *
 #define QO_D_DE_SCALE 40
 #define QO_D_JIGGLE 50
 #define QO_ANGLE_WRAP_MAX 1.0e4
-#define QO_UPD_STATS 9
+#define QO_UPD_STATS 11
 
 static uint64_t qo_mix(uint64_t z) {
   z += 0x9e3779b97f4a7c15ULL;
@@ -1747,13 +1747,227 @@ static double qo_mass_jacobian(const double *x) {
   return 2.0 * x[1] + leta - 0.5 * log(1.0 - 4.0 * eta);
 }
 
-/* update()'s proposal (QuickCBC.c:2268-2342, sky ring not built: c = 0) and
- * de_jump (:2407-2425); returns the type, 1 Fisher, 2 DE, 3 jiggle */
+/* Ring_all, Utilities.c:1445-1594: a new sky location (y[7], y[8]) on the
+ * ring of constant time delay between detectors d1 and d2, the rotation
+ * angle 2 pi u_rot about their baseline.  The vertex vectors are Times'
+ * (Utilities.c:1092-1104), in seconds. */
+void qo_ring_all(const double *px, double *py, int d1, int d2, double gmst, double u_rot) {
+  double zv[3], uv[3], vv[3], kv[3], kvr[3];
+  double zmag = 0.0, alpha, sind, cosd, calpha, salpha, cmu = 0.0, smu, rot, cr, sr;
+  int i;
+  for (i = 0; i < 3; i++) zv[i] = qcbc_vertex[d1][i] / QO_CLIGHT - qcbc_vertex[d2][i] / QO_CLIGHT;
+  for (i = 0; i < 3; i++) zmag += zv[i] * zv[i];
+  zmag = sqrt(zmag);
+  for (i = 0; i < 3; i++) zv[i] /= zmag; /* unit vector between the sites */
+  alpha = px[7];
+  sind = px[8];
+  cosd = sqrt(1.0 - sind * sind);
+  calpha = cos(alpha - gmst);
+  salpha = -sin(alpha - gmst);
+  kv[0] = cosd * calpha; /* propagation direction */
+  kv[1] = cosd * salpha;
+  kv[2] = sind;
+  for (i = 0; i < 3; i++) cmu += zv[i] * kv[i];
+  if (cmu > 0.999999) cmu = 0.999999;
+  if (cmu < -0.999999) cmu = -0.999999;
+  smu = sqrt(1.0 - cmu * cmu);
+  for (i = 0; i < 3; i++) uv[i] = (kv[i] - cmu * zv[i]) / smu;
+  vv[0] = (zv[1] * kv[2] - zv[2] * kv[1]) / smu;
+  vv[1] = (zv[2] * kv[0] - zv[0] * kv[2]) / smu;
+  vv[2] = (zv[0] * kv[1] - zv[1] * kv[0]) / smu;
+  rot = QO_TPI * u_rot; /* rotating about zv keeps the time delay between the sites */
+  cr = cos(rot);
+  sr = sin(rot);
+  for (i = 0; i < 3; i++) kvr[i] = cmu * zv[i] + smu * (cr * uv[i] + sr * vv[i]);
+  py[8] = kvr[2];
+  rot = atan2(kvr[1], kvr[0]);
+  rot = gmst - rot;
+  if (rot < 0.0) rot += QO_TPI;
+  py[7] = rot;
+}
+
+/* uvwz_all, QuickCBC.c:7295-7323: the four quadratures of the signal */
+static void qo_uvwz_all(double *u, double *v, double *w, double *z, const double *params) {
+  const double psi = params[9], ciota = params[10];
+  const double Ap = 0.5 * (1.0 + ciota * ciota), Ac = -ciota;
+  const double ecc = Ac / Ap, Amp = Ap / exp(params[6]), phi = params[4];
+  const double cphi = cos(phi), sphi = sin(phi), c2p = cos(2.0 * psi), s2p = sin(2.0 * psi);
+  *u = Amp * (cphi * c2p + ecc * sphi * s2p);
+  *v = Amp * (cphi * s2p - ecc * sphi * c2p);
+  *w = Amp * (sphi * c2p - ecc * cphi * s2p);
+  *z = Amp * (sphi * s2p + ecc * cphi * c2p);
+}
+
+/* uvwz_sol, QuickCBC.c:7526-7544: the quadratures at the new sky location
+ * that give detectors 1 and 2 the same response */
+static void qo_uvwz_sol(double *uy, double *vy, double *wy, double *zy, double ux, double vx, double wx, double zx,
+                        double fp1x, double fp1y, double fc1x, double fc1y, double fp2x, double fp2y, double fc2x,
+                        double fc2y) {
+  const double den = fc1y * fp2y - fc2y * fp1y;
+  (void)fp1x;
+  *uy = (vx * (fc1y * fc2x - fc1x * fc2y) + ux * (fc1y * fp2x - fc2y * fp1x)) / den;
+  *vy = (vx * (fc1x * fp2y - fc2x * fp1y) + ux * (fp2y * fp1x - fp1y * fp2x)) / den;
+  *wy = (zx * (fc1y * fc2x - fc1x * fc2y) + wx * (fc1y * fp2x - fc2y * fp1x)) / den;
+  *zy = (zx * (fc1x * fp2y - fc2x * fp1y) + wx * (fp2y * fp1x - fp1y * fp2x)) / den;
+}
+
+/* exsolve_all, QuickCBC.c:7325-7407: phase, polarisation, distance and
+ * inclination from the quadratures */
+static void qo_exsolve_all(double *phiy, double *psiy, double *DLy, double *ciotay, double uy, double vy, double wy,
+                           double zy) {
+  const double q = 2.0 * (uy * wy + vy * zy) / ((uy * uy + vy * vy) - (wy * wy + zy * zy));
+  const double rad = sqrt(1.0 + q * q);
+  double x = atan2(2.0 * (uy * wy + vy * zy), ((uy * uy + vy * vy) - (wy * wy + zy * zy)));
+  double p, ecc, ci, Ay;
+  int flag1, flag2, flag3;
+  if (x < 0.0) x += QO_TPI;
+  *phiy = 0.5 * x;
+  flag1 = cos(x) > 0.0;
+  flag2 = cos(x / 2.0) > 0.0;
+  flag3 = sin(x / 2.0) > 0.0;
+  p = ((rad + 1.0) * (uy * uy + vy * vy) + (rad - 1.0) * (wy * wy + zy * zy) + 2.0 * q * (uy * wy + vy * zy)) /
+      (2.0 * rad);
+  if (flag1)
+    ecc = (uy * zy - vy * wy) / p;
+  else
+    ecc = p / (uy * zy - vy * wy);
+  ci = -(1.0 - sqrt(1.0 - ecc * ecc)) / ecc;
+  *ciotay = ci;
+  if (flag1 && flag2)
+    x = atan2((1.0 + rad) * vy + q * zy, (1.0 + rad) * uy + q * wy);
+  else if (flag1 && !flag2)
+    x = atan2(-((1.0 + rad) * vy + q * zy), -((1.0 + rad) * uy + q * wy));
+  else if (!flag1 && flag3)
+    x = atan2(ecc * ((1.0 + rad) * uy + q * wy), -ecc * ((1.0 + rad) * vy + q * zy));
+  else
+    x = atan2(-ecc * ((1.0 + rad) * uy + q * wy), ecc * ((1.0 + rad) * vy + q * zy));
+  if (x < 0.0) x += QO_TPI;
+  *psiy = 0.5 * x;
+  if (flag1)
+    Ay = sqrt(p);
+  else
+    Ay = sqrt(p) / fabs(ecc);
+  *DLy = (1.0 + ci * ci) / (2.0 * Ay);
+}
+
+/* skymap_all, QuickCBC.c:7085-7146: the extrinsic parameters at the new sky
+ * location py[7], py[8] that keep the signal in detectors ifo1 and ifo2 */
+void qo_skymap_all(const double *px, double *py, double gmst, int ifo1, int ifo2) {
+  double dtx[3], dty[3], fxp1, fxc1, fxp2, fxc2, fyp1, fyc1, fyp2, fyc2;
+  double ux, vx, wx, zx, uy, vy, wy, zy, phiy, psiy, DLy, ciotay;
+  qo_times(px[7], px[8], gmst, dtx);
+  qo_f_ant(0.0, px[7], px[8], gmst, &fxp1, &fxc1, ifo1);
+  qo_f_ant(0.0, px[7], px[8], gmst, &fxp2, &fxc2, ifo2);
+  qo_times(py[7], py[8], gmst, dty);
+  qo_f_ant(0.0, py[7], py[8], gmst, &fyp1, &fyc1, ifo1);
+  qo_f_ant(0.0, py[7], py[8], gmst, &fyp2, &fyc2, ifo2);
+  qo_uvwz_all(&ux, &vx, &wx, &zx, px);
+  qo_uvwz_sol(&uy, &vy, &wy, &zy, ux, vx, wx, zx, fxp1, fyp1, fxc1, fyc1, fxp2, fyp2, fxc2, fyc2);
+  qo_exsolve_all(&phiy, &psiy, &DLy, &ciotay, uy, vy, wy, zy);
+  py[9] = psiy;
+  py[10] = ciotay;
+  py[6] = log(DLy);
+  py[4] = phiy;
+  py[5] = px[5] + dtx[ifo1] - dty[ifo1];
+}
+
+/* |det| of a 4 x 4 matrix (det, Utilities.c:953-988: GSL's LU with row
+ * pivoting on the transposed matrix, the product of the pivots), written out */
+static double qo_det4_abs(const double *A) {
+  double m[16], dx = 1.0;
+  int i, j, k;
+  for (i = 0; i < 4; i++)
+    for (j = 0; j < 4; j++) m[i * 4 + j] = A[j * 4 + i];
+  for (j = 0; j < 3; j++) {
+    int ip = j;
+    double amax = fabs(m[j * 4 + j]);
+    for (i = j + 1; i < 4; i++)
+      if (fabs(m[i * 4 + j]) > amax) {
+        amax = fabs(m[i * 4 + j]);
+        ip = i;
+      }
+    if (ip != j)
+      for (k = 0; k < 4; k++) {
+        double t = m[j * 4 + k];
+        m[j * 4 + k] = m[ip * 4 + k];
+        m[ip * 4 + k] = t;
+      }
+    if (m[j * 4 + j] != 0.0)
+      for (i = j + 1; i < 4; i++) {
+        double l = m[i * 4 + j] / m[j * 4 + j];
+        m[i * 4 + j] = l;
+        for (k = j + 1; k < 4; k++) m[i * 4 + k] -= l * m[j * 4 + k];
+      }
+  }
+  for (i = 0; i < 4; i++) dx *= m[i * 4 + i];
+  return fabs(dx);
+}
+
+/* skydensity_all, QuickCBC.c:7411-7490: skymap_all of x onto y's sky
+ * location, then |det| of the finite-difference Jacobian (epsilon 1e-6) of
+ * (ln DL, psi, phi, cos iota) at y with respect to x, each perturbed image
+ * mapped back to the solution branch of y (psi +- pi/2, phi +- pi) */
+double qo_skydensity_all(const double *px, double *py, double gmst, int ifo1, int ifo2) {
+  const double ep = 1.0e-6;
+  const int idx[4] = {6, 9, 4, 10};
+  double J[16], xp[11], yp[11];
+  int i, j;
+  qo_skymap_all(px, py, gmst, ifo1, ifo2);
+  for (i = 0; i <= 3; i++) {
+    for (j = 0; j < 11; j++) {
+      xp[j] = px[j];
+      yp[j] = py[j];
+    }
+    xp[idx[i]] += ep;
+    qo_skymap_all(xp, yp, gmst, ifo1, ifo2);
+    if (fabs(yp[9] + QO_PI / 2.0 - py[9]) < fabs(yp[9] - py[9])) yp[9] += QO_PI / 2.0;
+    if (fabs(yp[9] - QO_PI / 2.0 - py[9]) < fabs(yp[9] - py[9])) yp[9] -= QO_PI / 2.0;
+    if (fabs(yp[4] + QO_PI - py[4]) < fabs(yp[4] - py[4])) yp[4] += QO_PI;
+    if (fabs(yp[4] - QO_PI - py[4]) < fabs(yp[4] - py[4])) yp[4] -= QO_PI;
+    for (j = 0; j <= 3; j++) J[i * 4 + j] = (yp[idx[j]] - py[idx[j]]) / ep;
+  }
+  return qo_det4_abs(J);
+}
+
+#define QO_D_SKY 80
+#define QO_D_SKY_ROT 120
+#define QO_D_SKY_FLIP 121
+
+/* update()'s sky-ring proposal (QuickCBC.c:2293-2330) from x into y (a copy
+ * of x on entry): the detector pair, Ring_all, skymap_all, the log density
+ * of the map (*jac), and the half-period shift of phase and polarisation
+ * with probability 1/2.  mix = a, b, c, etamin, gmst, n_det, labels[3]. */
+static void qo_sky_proposal(const double *x, double *y, const double *mix, uint64_t seed, uint64_t ctr, double *jac) {
+  const double gmst = mix[4];
+  const int n_det = (int)mix[5];
+  int id1 = 0, id2 = 1, r;
+  if (n_det > 2) {
+    id1 = (int)((double)n_det * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_SKY));
+    id2 = id1;
+    for (r = 0; r < QO_DE_TRIES && id2 == id1; r++)
+      id2 = (int)((double)n_det * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_SKY + 1 + r));
+    if (id2 == id1) id2 = (id1 + 1) % n_det;
+  }
+  id1 = (int)mix[6 + id1];
+  id2 = (int)mix[6 + id2];
+  qo_ring_all(x, y, id1, id2, gmst, qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_SKY_ROT));
+  qo_skymap_all(x, y, gmst, id1, id2);
+  *jac = log(qo_skydensity_all(x, y, gmst, id1, id2));
+  if (qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_SKY_FLIP) > 0.5) {
+    y[4] += QO_PI;
+    y[9] += QO_PI / 2.0;
+  }
+}
+
+/* update()'s proposal (QuickCBC.c:2268-2342) and de_jump (:2407-2425);
+ * returns the type, 1 Fisher, 2 DE, 3 jiggle, 4 sky ring (*jac its log
+ * density, 0 for the others).  mix = a, b, c, etamin, gmst, n_det, labels[3]. */
 int qo_update_propose(const double *x, const double *ej, const double *ev, const double *hist, int n_hist,
-                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y) {
+                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y, double *jac) {
   const double alpha = qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_ALPHA);
   int typ, j, k;
   for (j = 0; j < 11; j++) y[j] = x[j];
+  *jac = 0.0;
   if (alpha > abc[0]) {
     int i = (int)(qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_FISHER) * 11.0);
     double beta = sT * ej[i] * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_FISHER + 1);
@@ -1770,10 +1984,13 @@ int qo_update_propose(const double *x, const double *ej, const double *ev, const
     if (qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_DE_SCALE) < 0.9)
       g = 0.5 * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_DE_SCALE + 1);
     for (k = 0; k < 11; k++) y[k] = x[k] + g * (hist[11 * i + k] - hist[11 * j + k]);
-  } else {
+  } else if (alpha > abc[2]) {
     double beta = 0.01 * pow(10.0, -floor(3.0 * qo_uni(seed, QO_SALT_PROPOSE, ctr, QO_D_JIGGLE))) * sT;
     typ = 3;
     for (k = 0; k < 11; k++) y[k] = x[k] + beta * qo_gauss(seed, QO_SALT_PROPOSE, ctr, QO_D_JIGGLE + 1 + 2 * k);
+  } else {
+    typ = 4;
+    qo_sky_proposal(x, y, abc, seed, ctr, jac);
   }
   if (fabs(y[4]) <= QO_ANGLE_WRAP_MAX) {
     while (y[4] > QO_TPI) y[4] -= QO_TPI;
@@ -1804,17 +2021,18 @@ void qo_update_hist_init(const double *x, int n_temps, int n_chains, int n_hist,
 }
 
 /* nsteps steps of the ensemble from step *step (the proposals y, types typ of
- * that step already made): the heterodyned logL of y, then per column a swap
- * or update() at every temperature, the history, the next proposals.
- * uconst: min[11], max[11], a, b, c, etamin; T_t = ladder^t.  stats:
- * n_chains x 9 (update steps, swap steps, swaps accepted, proposed and
- * accepted per type), accumulated. */
+ * that step already made, with the sky-ring log densities jac): the
+ * heterodyned logL of y, then per column a swap or update() at every
+ * temperature, the history, the next proposals.  uconst: min[11], max[11],
+ * a, b, c, etamin, gmst, n_det, labels[3]; T_t = ladder^t.  stats:
+ * n_chains x 11 (update steps, swap steps, swaps accepted, proposed and
+ * accepted per type 1..4), accumulated. */
 void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
                       const double *freq, const double *amp, const double *phase, const double *IP,
                       const double *SL, const double *lc, const double *ls, const double *rc, const double *rs,
                       const double *aa, double logLb, const double *uconst, int n_hist, int hist_every, int n_temps,
                       int n_chains, double ladder, uint64_t seed, int nsteps, long long *step, double *x, double *L,
-                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ,
+                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ, double *jac,
                       long long *stats) {
   const size_t P = (size_t)n_temps * n_chains;
   qo_update_const u;
@@ -1878,7 +2096,7 @@ void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, dou
           if (!flag) {
             const double *xp = x + 11 * q, *yp = y + 11 * q;
             const double H =
-                (Ly[q] - L[q]) * iT[t] + pMcMy + 3.0 * yp[6] - 3.0 * xp[6] - qo_mass_jacobian(xp);
+                jac[q] + (Ly[q] - L[q]) * iT[t] + pMcMy + 3.0 * yp[6] - 3.0 * xp[6] - qo_mass_jacobian(xp);
             if (H > log(qo_uni(seed, QO_SALT_ACCEPT, s * P + q, 0))) {
               memcpy(x + 11 * q, yp, sizeof(double) * 11);
               L[q] = Ly[q];
@@ -1921,7 +2139,7 @@ void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, dou
         const size_t q = (size_t)t * n_chains + c;
         const double *hr = hist + ((size_t)c * n_temps + t) * (size_t)n_hist * 11;
         typ[q] = qo_update_propose(x + 11 * q, ej + 11 * q, ev + 121 * q, hr, n_hist, sT[t], uconst + 22, seed,
-                                   (s + 1) * P + q, y + 11 * q);
+                                   (s + 1) * P + q, y + 11 * q, jac + q);
       }
     }
     *step = (long long)(s + 1);

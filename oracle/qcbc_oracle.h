@@ -144,7 +144,10 @@ void qo_fisher_evec_batch(const double *fish, int B, int d, double *ej, double *
  * with the counter-based random numbers of the GPU step. */
 double qo_uni(uint64_t seed, uint64_t salt, uint64_t a, uint64_t k);
 int qo_update_propose(const double *x, const double *ej, const double *ev, const double *hist, int n_hist,
-                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y);
+                      double sT, const double *abc, uint64_t seed, uint64_t ctr, double *y, double *jac);
+void qo_ring_all(const double *px, double *py, int d1, int d2, double gmst, double u_rot);
+void qo_skymap_all(const double *px, double *py, double gmst, int ifo1, int ifo2);
+double qo_skydensity_all(const double *px, double *py, double gmst, int ifo1, int ifo2);
 void qo_update_hist_init(const double *x, int n_temps, int n_chains, int n_hist, uint64_t seed, double *hist);
 int qo_update_prior(const double *y, const double *uconst, double *pMcM);
 void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
@@ -152,7 +155,7 @@ void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, dou
                       const double *SL, const double *lc, const double *ls, const double *rc, const double *rs,
                       const double *aa, double logLb, const double *uconst, int n_hist, int hist_every, int n_temps,
                       int n_chains, double ladder, uint64_t seed, int nsteps, long long *step, double *x, double *L,
-                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ,
+                      double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ, double *jac,
                       long long *stats);
 
 #ifdef __cplusplus

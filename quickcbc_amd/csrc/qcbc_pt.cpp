@@ -69,6 +69,7 @@ struct qcbc_pt {
   double *d_ej = nullptr, *d_ev = nullptr, *d_hist = nullptr;
   unsigned long long *d_hcount = nullptr, *d_ustats = nullptr;
   int* d_typ = nullptr;
+  double* d_jac = nullptr;  // log density of each pending sky-ring proposal (0 for the others)
 };
 
 #define HIPCHK(x)                \
@@ -94,7 +95,8 @@ static int group_step(qcbc_pt* pt, int k) {
     if (st) return st;
     HIPCHK(launch_pt_update(pt->d_x + 11 * o, pt->d_L + o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
                             pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_hcount,
-                            pt->d_y + 11 * o, pt->d_Ly + o, pt->d_st + o, pt->d_typ + o, pt->d_tc, pt->uc,
+                            pt->d_y + 11 * o, pt->d_Ly + o, pt->d_st + o, pt->d_typ + o, pt->d_jac + o, pt->d_tc,
+                            pt->uc,
                             pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_ustats, gr.d_done,
                             gr.s));
     return QCBC_OK;
@@ -146,7 +148,7 @@ static void pt_free(qcbc_pt* pt) {
   }
   void* bufs[] = {pt->d_x,     pt->d_L,  pt->d_y,  pt->d_Ly,   pt->d_rho,    pt->d_sigma,  pt->d_tc,
                   pt->d_st,    pt->d_acc, pt->d_swaps, pt->d_ej, pt->d_ev, pt->d_hist, pt->d_hcount,
-                  pt->d_ustats, pt->d_typ};
+                  pt->d_ustats, pt->d_typ, pt->d_jac};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete pt;
@@ -164,7 +166,7 @@ static int update_propose_all(qcbc_pt* pt, hipStream_t s) {
     HIPCHK(launch_update_propose(pt->d_x + 11 * o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
                                  pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_tc, pt->uc,
                                  pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_y + 11 * o,
-                                 pt->d_typ + o, s));
+                                 pt->d_typ + o, pt->d_jac + o, s));
   }
   return QCBC_OK;
 }
@@ -172,9 +174,9 @@ static int update_propose_all(qcbc_pt* pt, hipStream_t s) {
 static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, double ladder, const double* init,
                      const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out,
                      const qcbc_update_desc* ud = nullptr) {
-  if (ud && (!het || !(ud->a >= ud->b && ud->b >= ud->c && ud->c == 0.0 && ud->a <= 1.0 && ud->b >= 0.0) ||
-             ud->n_hist < 2 || ud->hist_every < 1 || !(ud->etamin >= 0.0)))
-    return QCBC_EINVAL;
+  if (ud && (!het || !(ud->a >= ud->b && ud->b >= ud->c && ud->c >= 0.0 && ud->a <= 1.0) || ud->n_hist < 2 ||
+             ud->hist_every < 1 || !(ud->etamin >= 0.0) || (ud->c > 0.0 && ctx && ctx->desc.n_det < 2)))
+    return QCBC_EINVAL;  // (a sky ring needs two detectors)
   if (ud)
     for (int i = 0; i < 11; i++)
       if (!(ud->min[i] < ud->max[i])) return QCBC_EINVAL;
@@ -209,6 +211,9 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
     pt->uc.etamin = ud->etamin;
     pt->uc.n_hist = ud->n_hist;
     pt->uc.hist_every = ud->hist_every;
+    pt->uc.gmst = ctx->desc.gmst;
+    pt->uc.n_det = ctx->desc.n_det;
+    for (int d = 0; d < 3; d++) pt->uc.labels[d] = d < ctx->desc.n_det ? ctx->desc.labels[d] : 0;
   }
   const size_t P = pt->P;
   int c0 = 0, off = 0;
@@ -243,7 +248,8 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
          hipMalloc(&pt->d_hist, sizeof(double) * 11 * P * (size_t)pt->uc.n_hist) == hipSuccess &&
          hipMalloc(&pt->d_hcount, sizeof(unsigned long long) * n_chains) == hipSuccess &&
          hipMalloc(&pt->d_ustats, sizeof(unsigned long long) * UPD_STATS * n_chains) == hipSuccess &&
-         hipMalloc(&pt->d_typ, sizeof(int) * P) == hipSuccess;
+         hipMalloc(&pt->d_typ, sizeof(int) * P) == hipSuccess &&
+         hipMalloc(&pt->d_jac, sizeof(double) * P) == hipSuccess;
   for (int k = 0; k < pt->G && ok; k++) {
     Group& gr = pt->g[k];
     ok = hipMalloc(&gr.d_pts, sizeof(Point) * gr.P) == hipSuccess &&
@@ -361,7 +367,8 @@ int qcbc_pt_refresh_fisher(qcbc_pt* pt) {
   return QCBC_OK;
 }
 
-int qcbc_pt_get_update(qcbc_pt* pt, double* ej, double* ev, double* proposals, int* types, long long* stats) {
+int qcbc_pt_get_update(qcbc_pt* pt, double* ej, double* ev, double* proposals, int* types, double* jac,
+                       long long* stats) {
   if (!pt || !pt->upd) return QCBC_EINVAL;
   qcbc_ctx* c = pt->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
@@ -370,13 +377,14 @@ int qcbc_pt_get_update(qcbc_pt* pt, double* ej, double* ev, double* proposals, i
   if (st) return st;
   hipStream_t s = c->stream;
   const size_t P = pt->P;
-  std::vector<double> e(11 * P), v(121 * P), y(11 * P);
+  std::vector<double> e(11 * P), v(121 * P), y(11 * P), jc(P);
   std::vector<int> ty(P);
   std::vector<unsigned long long> us((size_t)UPD_STATS * pt->n_chains);
   HIPCHK(hipMemcpyAsync(e.data(), pt->d_ej, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(v.data(), pt->d_ev, sizeof(double) * 121 * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(y.data(), pt->d_y, sizeof(double) * 11 * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(ty.data(), pt->d_typ, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(jc.data(), pt->d_jac, sizeof(double) * P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(us.data(), pt->d_ustats, sizeof(unsigned long long) * us.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   for (size_t i = 0; i < P; i++) {
@@ -385,6 +393,7 @@ int qcbc_pt_get_update(qcbc_pt* pt, double* ej, double* ev, double* proposals, i
     if (ev) std::memcpy(ev + 121 * p, &v[121 * i], sizeof(double) * 121);
     if (proposals) std::memcpy(proposals + 11 * p, &y[11 * i], sizeof(double) * 11);
     if (types) types[p] = ty[i];
+    if (jac) jac[p] = jc[i];
   }
   if (stats)
     for (size_t i = 0; i < us.size(); i++) stats[i] = (long long)us[i];

@@ -18,8 +18,11 @@
 //                   i != j from the temperature's history, g = N(0, 0.5) with
 //                   probability 0.9, else 1
 //   alpha > c       jiggle: y = x + 0.01 10^-floor(3u) sqrt(T) N(0,1) per parameter
-//   otherwise       the sky ring (Ring_all / skymap_all, :2298-2330), not built: c
-//                   must be 0 (the reference's alternative setting, :2256-2260)
+//   otherwise       the sky ring (:2293-2330): a new sky location on the ring of
+//                   constant time delay between two detectors (Ring_all), the
+//                   extrinsics that keep both detectors' signals (skymap_all), the
+//                   log density of that map (skydensity_all) added to H, and phase
+//                   and polarisation shifted by half a period with probability 1/2
 // then the angles wrapped into range (:2336-2342).
 //
 // Random numbers are counter-based (SplitMix64 of (seed, step, point,
@@ -30,6 +33,7 @@
 
 #include <cstdint>
 
+#include "qcbc_device.h"
 #include "qcbc_update_internal.h"
 
 namespace qcbc {
@@ -80,17 +84,251 @@ __device__ double mass_jacobian(const double* x) {
   return 2.0 * x[1] + leta - 0.5 * log(1.0 - 4.0 * eta);
 }
 
+// ---------------------------------------------------------------- sky ring
+// The parameters a sky-ring move changes: phase [4], tc [5], ln DL [6],
+// alpha [7], sin delta [8], psi [9], cos iota [10], passed by value so that
+// nothing of the (rarely taken) sky path lives in memory.
+struct Ext {
+  double p4, p5, p6, p7, p8, p9, p10;
+};
+
+// Ring_all (Utilities.c:1445-1594): y's sky location rotated by 2 pi u_rot
+// about the baseline of detectors d1, d2 (the vertex vectors of Times, in s).
+__device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot) {
+  double zv[3], uv[3], vv[3], kv[3], kvr[3];
+  double zmag = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {  // vertex rows picked without a run-time index
+      if (d == d1) a = qcbc_vertex[d][i] / CLIGHT;
+      if (d == d2) b = qcbc_vertex[d][i] / CLIGHT;
+    }
+    zv[i] = a - b;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) zmag += zv[i] * zv[i];
+  zmag = sqrt(zmag);
+#pragma unroll
+  for (int i = 0; i < 3; i++) zv[i] /= zmag;
+  const double sind = x.p8, cosd = sqrt(1.0 - sind * sind);
+  const double calpha = cos(x.p7 - gmst), salpha = -sin(x.p7 - gmst);
+  kv[0] = cosd * calpha;
+  kv[1] = cosd * salpha;
+  kv[2] = sind;
+  double cmu = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) cmu += zv[i] * kv[i];
+  if (cmu > 0.999999) cmu = 0.999999;
+  if (cmu < -0.999999) cmu = -0.999999;
+  const double smu = sqrt(1.0 - cmu * cmu);
+#pragma unroll
+  for (int i = 0; i < 3; i++) uv[i] = (kv[i] - cmu * zv[i]) / smu;
+  vv[0] = (zv[1] * kv[2] - zv[2] * kv[1]) / smu;
+  vv[1] = (zv[2] * kv[0] - zv[0] * kv[2]) / smu;
+  vv[2] = (zv[0] * kv[1] - zv[1] * kv[0]) / smu;
+  double rot = TPI_U * u_rot;
+  const double cr = cos(rot), sr = sin(rot);
+#pragma unroll
+  for (int i = 0; i < 3; i++) kvr[i] = cmu * zv[i] + smu * (cr * uv[i] + sr * vv[i]);
+  Ext y = x;
+  y.p8 = kvr[2];
+  rot = atan2(kvr[1], kvr[0]);
+  rot = gmst - rot;
+  if (rot < 0.0) rot += TPI_U;
+  y.p7 = rot;
+  return y;
+}
+
+// skymap_all (QuickCBC.c:7085-7146) with uvwz_all (:7295-7323), uvwz_sol
+// (:7526-7544) and exsolve_all (:7325-7407): the extrinsics at y's sky
+// location that keep the signal in detectors ifo1 and ifo2
+__device__ __noinline__ Ext skymap_all(Ext x, Ext y, double gmst, int ifo1, int ifo2) {
+  double fxp1, fxc1, fxp2, fxc2, fyp1, fyc1, fyp2, fyc2;
+  antenna(0.0, x.p7, x.p8, gmst, ifo1, &fxp1, &fxc1);
+  antenna(0.0, x.p7, x.p8, gmst, ifo2, &fxp2, &fxc2);
+  antenna(0.0, y.p7, y.p8, gmst, ifo1, &fyp1, &fyc1);
+  antenna(0.0, y.p7, y.p8, gmst, ifo2, &fyp2, &fyc2);
+  const double dtx = delay(x.p7, x.p8, gmst, ifo1), dty = delay(y.p7, y.p8, gmst, ifo1);
+  // uvwz_all
+  const double ciota = x.p10, Ap = 0.5 * (1.0 + ciota * ciota), Ac = -ciota;
+  const double ecc0 = Ac / Ap, Amp = Ap / exp(x.p6), phi = x.p4;
+  const double cphi = cos(phi), sphi = sin(phi), c2p = cos(2.0 * x.p9), s2p = sin(2.0 * x.p9);
+  const double ux = Amp * (cphi * c2p + ecc0 * sphi * s2p);
+  const double vx = Amp * (cphi * s2p - ecc0 * sphi * c2p);
+  const double wx = Amp * (sphi * c2p - ecc0 * cphi * s2p);
+  const double zx = Amp * (sphi * s2p + ecc0 * cphi * c2p);
+  // uvwz_sol
+  const double den = fyc1 * fyp2 - fyc2 * fyp1;
+  const double uy = (vx * (fyc1 * fxc2 - fxc1 * fyc2) + ux * (fyc1 * fxp2 - fyc2 * fxp1)) / den;
+  const double vy = (vx * (fxc1 * fyp2 - fxc2 * fyp1) + ux * (fyp2 * fxp1 - fyp1 * fxp2)) / den;
+  const double wy = (zx * (fyc1 * fxc2 - fxc1 * fyc2) + wx * (fyc1 * fxp2 - fyc2 * fxp1)) / den;
+  const double zy = (zx * (fxc1 * fyp2 - fxc2 * fyp1) + wx * (fyp2 * fxp1 - fyp1 * fxp2)) / den;
+  // exsolve_all
+  const double q = 2.0 * (uy * wy + vy * zy) / ((uy * uy + vy * vy) - (wy * wy + zy * zy));
+  const double rad = sqrt(1.0 + q * q);
+  double a = atan2(2.0 * (uy * wy + vy * zy), ((uy * uy + vy * vy) - (wy * wy + zy * zy)));
+  if (a < 0.0) a += TPI_U;
+  const double phiy = 0.5 * a;
+  const bool flag1 = cos(a) > 0.0, flag2 = cos(a / 2.0) > 0.0, flag3 = sin(a / 2.0) > 0.0;
+  const double p =
+      ((rad + 1.0) * (uy * uy + vy * vy) + (rad - 1.0) * (wy * wy + zy * zy) + 2.0 * q * (uy * wy + vy * zy)) /
+      (2.0 * rad);
+  const double ecc = flag1 ? (uy * zy - vy * wy) / p : p / (uy * zy - vy * wy);
+  const double ci = -(1.0 - sqrt(1.0 - ecc * ecc)) / ecc;
+  if (flag1 && flag2)
+    a = atan2((1.0 + rad) * vy + q * zy, (1.0 + rad) * uy + q * wy);
+  else if (flag1 && !flag2)
+    a = atan2(-((1.0 + rad) * vy + q * zy), -((1.0 + rad) * uy + q * wy));
+  else if (!flag1 && flag3)
+    a = atan2(ecc * ((1.0 + rad) * uy + q * wy), -ecc * ((1.0 + rad) * vy + q * zy));
+  else
+    a = atan2(-ecc * ((1.0 + rad) * uy + q * wy), ecc * ((1.0 + rad) * vy + q * zy));
+  if (a < 0.0) a += TPI_U;
+  const double Ay = flag1 ? sqrt(p) : sqrt(p) / fabs(ecc);
+  y.p9 = 0.5 * a;
+  y.p10 = ci;
+  y.p6 = log((1.0 + ci * ci) / (2.0 * Ay));
+  y.p4 = phiy;
+  y.p5 = x.p5 + dtx - dty;
+  return y;
+}
+
+// |det| of a 4 x 4 matrix as det (Utilities.c:953-988): LU with row pivoting
+// on the transposed matrix, the product of the pivots (row exchanges as
+// predicated swaps, so every index is a compile-time constant)
+__device__ double det4_abs(const double (&A)[16]) {
+  double m[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) m[i][j] = A[j * 4 + i];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    int ip = j;
+    double amax = fabs(m[j][j]);
+#pragma unroll
+    for (int i = j + 1; i < 4; i++)
+      if (fabs(m[i][j]) > amax) {
+        amax = fabs(m[i][j]);
+        ip = i;
+      }
+#pragma unroll
+    for (int i = j + 1; i < 4; i++)
+      if (i == ip)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double t = m[j][k];
+          m[j][k] = m[i][k];
+          m[i][k] = t;
+        }
+    if (m[j][j] != 0.0)
+#pragma unroll
+      for (int i = j + 1; i < 4; i++) {
+        const double l = m[i][j] / m[j][j];
+        m[i][j] = l;
+#pragma unroll
+        for (int k = j + 1; k < 4; k++) m[i][k] -= l * m[j][k];
+      }
+  }
+  double dx = 1.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) dx *= m[i][i];
+  return fabs(dx);
+}
+
+// the component of the Jacobian's order (ln DL, psi, phi, cos iota)
+__device__ __forceinline__ double jcomp(const Ext& e, int j) {
+  return j == 0 ? e.p6 : (j == 1 ? e.p9 : (j == 2 ? e.p4 : e.p10));
+}
+
+// update()'s sky-ring proposal (QuickCBC.c:2293-2330) with skydensity_all
+// (:7411-7490): the pair of detectors, Ring_all, skymap_all, the log of |det|
+// of the one-sided (epsilon 1e-6) Jacobian of (ln DL, psi, phi, cos iota) at
+// y with respect to x (each perturbed image mapped to y's solution branch),
+// then the half-period shift of phase and polarisation with probability 1/2.
+// Called by the 11 lanes of one point (lane j of its 16-lane segment): lane 0
+// maps x onto the new sky location (skydensity_all's own second mapping of x
+// onto y repeats the same operations on the same values, so it is made
+// once) while lanes 1-4 each map one perturbed x; y and the four Jacobian
+// rows are then exchanged by shuffles within the segment.
+struct SkyOut {
+  Ext y;
+  double jac;
+};
+__device__ __noinline__ SkyOut sky_proposal(Ext x, double gmst, int n_det, int l0, int l1, int l2, uint64_t seed,
+                                           uint64_t ctr, int j) {
+  int i1 = 0, i2 = 1;
+  if (n_det > 2) {
+    i1 = (int)((double)n_det * uni(seed, SALT_PROPOSE, ctr, D_SKY));
+    i2 = i1;
+    for (int r = 0; r < DE_TRIES && i2 == i1; r++) i2 = (int)((double)n_det * uni(seed, SALT_PROPOSE, ctr, D_SKY + 1 + r));
+    if (i2 == i1) i2 = (i1 + 1) % n_det;
+  }
+  const int id1 = i1 == 0 ? l0 : (i1 == 1 ? l1 : l2);
+  const int id2 = i2 == 0 ? l0 : (i2 == 1 ? l1 : l2);
+  const Ext yr = ring_all(x, id1, id2, gmst, uni(seed, SALT_PROPOSE, ctr, D_SKY_ROT));
+  // skymap_all reads only the sky location of its second argument, so the
+  // map of x (lane 0) and the perturbed maps (lanes 1-4) run side by side
+  constexpr double ep = 1.0e-6;
+  Ext xp = x;
+  if (j == 1) xp.p6 += ep;
+  if (j == 2) xp.p9 += ep;
+  if (j == 3) xp.p4 += ep;
+  if (j == 4) xp.p10 += ep;
+  const Ext ys = skymap_all(xp, yr, gmst, id1, id2);
+  Ext y = yr;  // lane 0's map of x, to every lane
+  y.p4 = __shfl(ys.p4, 0, 16);
+  y.p5 = __shfl(ys.p5, 0, 16);
+  y.p6 = __shfl(ys.p6, 0, 16);
+  y.p9 = __shfl(ys.p9, 0, 16);
+  y.p10 = __shfl(ys.p10, 0, 16);
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;  // this lane's Jacobian row (lanes 1-4)
+  if (j >= 1 && j <= 4) {
+    Ext yp = ys;
+    if (fabs(yp.p9 + PI_U / 2.0 - y.p9) < fabs(yp.p9 - y.p9)) yp.p9 += PI_U / 2.0;
+    if (fabs(yp.p9 - PI_U / 2.0 - y.p9) < fabs(yp.p9 - y.p9)) yp.p9 -= PI_U / 2.0;
+    if (fabs(yp.p4 + PI_U - y.p4) < fabs(yp.p4 - y.p4)) yp.p4 += PI_U;
+    if (fabs(yp.p4 - PI_U - y.p4) < fabs(yp.p4 - y.p4)) yp.p4 -= PI_U;
+    r0 = (jcomp(yp, 0) - jcomp(y, 0)) / ep;
+    r1 = (jcomp(yp, 1) - jcomp(y, 1)) / ep;
+    r2 = (jcomp(yp, 2) - jcomp(y, 2)) / ep;
+    r3 = (jcomp(yp, 3) - jcomp(y, 3)) / ep;
+  }
+  double J[16];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    J[i * 4 + 0] = __shfl(r0, 1 + i, 16);
+    J[i * 4 + 1] = __shfl(r1, 1 + i, 16);
+    J[i * 4 + 2] = __shfl(r2, 1 + i, 16);
+    J[i * 4 + 3] = __shfl(r3, 1 + i, 16);
+  }
+  SkyOut o;
+  o.jac = log(det4_abs(J));
+  if (uni(seed, SALT_PROPOSE, ctr, D_SKY_FLIP) > 0.5) {
+    y.p4 += PI_U;
+    y.p9 += PI_U / 2.0;
+  }
+  o.y = y;
+  return o;
+}
+
 // Component j of update()'s proposal for the state x at temperature T
 // (sT = sqrt(T)) from the counters (seed, ctr); *typ gets the proposal type
 // (1 Fisher, 2 DE, 3 jiggle).  One lane per component: the draws a proposal
 // shares (its type, eigendirection, history rows, scale) are the same
 // counters on every lane, so each component is computed exactly as a
-// sequential loop over the components would compute it.
+// sequential loop over the components would compute it.  All of lanes 0-10
+// of a point's 16-lane segment call it together (the sky ring shuffles within
+// the segment).
 __device__ double propose_component(const double* __restrict__ x, const double* __restrict__ ej,
                                     const double* __restrict__ ev, const double* __restrict__ hist, int n_hist,
-                                    double sT, const UpdateConst& u, uint64_t seed, uint64_t ctr, int j, int* typ) {
+                                    double sT, const UpdateConst& u, uint64_t seed, uint64_t ctr, int j, int* typ,
+                                    double* jac) {
   const double alpha = uni(seed, SALT_PROPOSE, ctr, D_ALPHA);
   double y;
+  *jac = 0.0;
   if (alpha > u.a) {  // Fisher jump along an eigendirection
     *typ = 1;
     const int i = (int)(uni(seed, SALT_PROPOSE, ctr, D_FISHER) * 11.0);
@@ -105,10 +343,26 @@ __device__ double propose_component(const double* __restrict__ x, const double* 
     double g = 1.0;
     if (uni(seed, SALT_PROPOSE, ctr, D_DE_SCALE) < 0.9) g = 0.5 * gauss(seed, SALT_PROPOSE, ctr, D_DE_SCALE + 1);
     y = x[j] + g * (hist[11 * i + j] - hist[11 * k + j]);
-  } else {  // jiggle (alpha > c; the sky ring below c is not built and c is 0)
+  } else if (alpha > u.c) {  // jiggle
     *typ = 3;
     const double beta = 0.01 * pow(10.0, -floor(3.0 * uni(seed, SALT_PROPOSE, ctr, D_JIGGLE))) * sT;
     y = x[j] + beta * gauss(seed, SALT_PROPOSE, ctr, D_JIGGLE + 1 + 2 * j);
+  } else {  // sky ring: every lane maps the whole point and keeps its component
+    *typ = 4;
+    Ext e;
+    e.p4 = x[4];
+    e.p5 = x[5];
+    e.p6 = x[6];
+    e.p7 = x[7];
+    e.p8 = x[8];
+    e.p9 = x[9];
+    e.p10 = x[10];
+    const SkyOut o = sky_proposal(e, u.gmst, u.n_det, u.labels[0], u.labels[1], u.labels[2], seed, ctr, j);
+    *jac = o.jac;
+    y = j == 4 ? o.y.p4
+               : j == 5 ? o.y.p5
+               : j == 6 ? o.y.p6
+               : j == 7 ? o.y.p7 : j == 8 ? o.y.p8 : j == 9 ? o.y.p9 : j == 10 ? o.y.p10 : x[j];
   }
   // angles back into range (:2336-2342).  Only an angle within +-ANGLE_WRAP_MAX
   // rad is wrapped, so each loop ends within ~1600 turns; one beyond (or not
@@ -134,7 +388,7 @@ __device__ void propose_column(const double* __restrict__ x, const double* __res
                                const double* __restrict__ ev, const double* __restrict__ hist,
                                const double* __restrict__ tc, const UpdateConst& u, int n_temps, int n_chains,
                                int c0, int ncols, int b, uint64_t seed, uint64_t s, int l, int nthreads,
-                               double* __restrict__ y, int* __restrict__ typ) {
+                               double* __restrict__ y, int* __restrict__ typ, double* __restrict__ jac) {
   const uint64_t P = (uint64_t)n_temps * n_chains;
   const int per = nthreads / PROP_LANES, sub = l / PROP_LANES, j = l % PROP_LANES;
   if (j >= 11) return;
@@ -143,10 +397,14 @@ __device__ void propose_column(const double* __restrict__ x, const double* __res
     const uint64_t gp = (uint64_t)t * n_chains + c0 + b;
     const double* h = hist + ((size_t)b * n_temps + t) * (size_t)u.n_hist * 11;
     int ty;
+    double jc;
     const double yj = propose_component(x + 11 * lp, ej + 11 * lp, ev + 121 * lp, h, u.n_hist, tc[3 * t], u, seed,
-                                        s * P + gp, j, &ty);
+                                        s * P + gp, j, &ty, &jc);
     y[11 * lp + j] = yj;
-    if (j == 0) typ[lp] = ty;
+    if (j == 0) {
+      typ[lp] = ty;
+      jac[lp] = jc;
+    }
   }
 }
 
@@ -158,9 +416,10 @@ __global__ __launch_bounds__(512) void k_update_propose(const double* __restrict
                                                         const double* __restrict__ tc, UpdateConst u, int n_temps,
                                                         int n_chains, int c0, int ncols, uint64_t seed,
                                                         const unsigned long long* __restrict__ step,
-                                                        double* __restrict__ y, int* __restrict__ typ) {
+                                                        double* __restrict__ y, int* __restrict__ typ,
+                                                        double* __restrict__ jac) {
   propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, blockIdx.x, seed, *step, threadIdx.x,
-                 blockDim.x, y, typ);
+                 blockDim.x, y, typ, jac);
 }
 
 // One PT step of a group of chain columns after the heterodyned logL of the
@@ -168,7 +427,8 @@ __global__ __launch_bounds__(512) void k_update_propose(const double* __restrict
 __global__ __launch_bounds__(512) void k_pt_update(
     double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej, double* __restrict__ ev,
     double* __restrict__ hist, unsigned long long* __restrict__ hcount, double* __restrict__ y,
-    const double* __restrict__ Ly, const int* __restrict__ st_y, int* __restrict__ typ, const double* __restrict__ tc,
+    const double* __restrict__ Ly, const int* __restrict__ st_y, int* __restrict__ typ, double* __restrict__ jac,
+    const double* __restrict__ tc,
     UpdateConst u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed, unsigned long long* __restrict__ step,
     unsigned long long* __restrict__ stats, unsigned int* __restrict__ done) {
   const int b = blockIdx.x, c = c0 + b;
@@ -189,7 +449,8 @@ __global__ __launch_bounds__(512) void k_pt_update(
       atomicAdd(&cst[3 + 2 * (k - 1)], 1ULL);
       if (!flag) {
         const double* xp = x + 11 * lp;
-        const double H = (Ly[lp] - L[lp]) * tc[3 * t + 1] + pMcMy + 3.0 * q[6] - 3.0 * xp[6] - mass_jacobian(xp);
+        const double H =
+            jac[lp] + (Ly[lp] - L[lp]) * tc[3 * t + 1] + pMcMy + 3.0 * q[6] - 3.0 * xp[6] - mass_jacobian(xp);
         if (H > log(uni(seed, SALT_ACCEPT, s * P + gp, 0))) {
           for (int i = 0; i < 11; i++) x[11 * lp + i] = q[i];
           L[lp] = Ly[lp];
@@ -247,7 +508,7 @@ __global__ __launch_bounds__(512) void k_pt_update(
   __syncthreads();
   // proposals of step s + 1
   propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, b, seed, s + 1, threadIdx.x, blockDim.x, y,
-                 typ);
+                 typ, jac);
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -293,17 +554,18 @@ hipError_t launch_update_hist_init(const double* x, int n_temps, int n_chains, i
 
 hipError_t launch_update_propose(const double* x, const double* ej, const double* ev, const double* hist,
                                  const double* tc, const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols,
-                                 uint64_t seed, const unsigned long long* step, double* y, int* typ, hipStream_t s) {
+                                 uint64_t seed, const unsigned long long* step, double* y, int* typ, double* jac,
+                                 hipStream_t s) {
   hipLaunchKernelGGL(k_update_propose, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, ej, ev, hist, tc, u,
-                     n_temps, n_chains, c0, ncols, seed, step, y, typ);
+                     n_temps, n_chains, c0, ncols, seed, step, y, typ, jac);
   return hipGetLastError();
 }
 
 hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                            double* y, const double* Ly, const int* st_y, int* typ, const double* tc,
+                            double* y, const double* Ly, const int* st_y, int* typ, double* jac, const double* tc,
                             const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
                             unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_pt_update, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, L, ej, ev, hist, hcount, y, Ly, st_y, typ, tc,
+  hipLaunchKernelGGL(k_pt_update, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, L, ej, ev, hist, hcount, y, Ly, st_y, typ, jac, tc,
                      u, n_temps, n_chains, c0, ncols, seed, step, stats, done);
   return hipGetLastError();
 }

@@ -16,6 +16,9 @@ struct UpdateConst {
   double etamin;            // lower limit of the symmetric mass ratio (ConstCBC.h:57)
   int n_hist;               // history rows per temperature (NH, ConstCBC.h:72)
   int hist_every;           // steps between history entries (20, QuickCBC.c:2071)
+  // the analysis's detectors, for the sky ring
+  double gmst;
+  int n_det, labels[3];
 };
 
 // Random streams (xor-ed into the seed) and draw indices within a stream.
@@ -29,21 +32,25 @@ constexpr int D_DE = 4;        // history row i, then row j: one draw per try
 constexpr int DE_TRIES = 32;
 constexpr int D_DE_SCALE = 40; // whether the scale is N(0, 0.5), then a normal
 constexpr int D_JIGGLE = 50;   // the scale's exponent, then 11 normals (2 draws each)
+constexpr int D_SKY = 80;      // first detector of the pair, then the second: one draw per try
+constexpr int D_SKY_ROT = 120; // the rotation about the pair's baseline
+constexpr int D_SKY_FLIP = 121;// the half-period shift of phase and polarisation
 
 // angles wrapped into range after a jump only within this magnitude (rad)
 constexpr double ANGLE_WRAP_MAX = 1.0e4;
 
 // per column: update steps, swap steps, swaps accepted, then (proposed,
-// accepted) for the Fisher, DE and jiggle proposals
-constexpr int UPD_STATS = 9;
+// accepted) for the Fisher, DE, jiggle and sky-ring proposals
+constexpr int UPD_STATS = 11;
 
 hipError_t launch_update_hist_init(const double* x, int n_temps, int n_chains, int c0, int ncols, int n_hist,
                                    uint64_t seed, double* hist, hipStream_t s);
 hipError_t launch_update_propose(const double* x, const double* ej, const double* ev, const double* hist,
                                  const double* tc, const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols,
-                                 uint64_t seed, const unsigned long long* step, double* y, int* typ, hipStream_t s);
+                                 uint64_t seed, const unsigned long long* step, double* y, int* typ, double* jac,
+                                 hipStream_t s);
 hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                            double* y, const double* Ly, const int* st_y, int* typ, const double* tc,
+                            double* y, const double* Ly, const int* st_y, int* typ, double* jac, const double* tc,
                             const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
                             unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s);
 

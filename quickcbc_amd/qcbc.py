@@ -457,7 +457,7 @@ class UpdateDesc(ctypes.Structure):
                 ("n_hist", ctypes.c_int), ("hist_every", ctypes.c_int)]
 
 
-def update_desc(pmin, pmax, a=0.5, b=0.3, c=0.0, etamin=0.05, n_hist=1000, hist_every=20):
+def update_desc(pmin, pmax, a=0.5, b=0.3, c=0.2, etamin=0.05, n_hist=1000, hist_every=20):
     d = UpdateDesc()
     d.min[:] = [float(v) for v in pmin]
     d.max[:] = [float(v) for v in pmax]
@@ -475,7 +475,7 @@ class PTUpdate(PT):
                                             ctypes.POINTER(UpdateDesc), ctypes.c_uint64, ctypes.c_int,
                                             ctypes.POINTER(_vp)]
         L.qcbc_pt_refresh_fisher.argtypes = [_vp]
-        L.qcbc_pt_get_update.argtypes = [_vp, _dp, _dp, _dp, _ip, ctypes.POINTER(ctypes.c_longlong)]
+        L.qcbc_pt_get_update.argtypes = [_vp, _dp, _dp, _dp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_pt_get_history.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_pt_groups.argtypes = [_vp, _ip]
         L.qcbc_pt_run.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
@@ -501,14 +501,16 @@ class PTUpdate(PT):
         _check(lib().qcbc_pt_refresh_fisher(self.h), "qcbc_pt_refresh_fisher")
 
     def get_update(self):
-        """(ej P x 11, ev P x 11 x 11, proposals P x 11, types P, stats n_chains x 9), point order."""
+        """(ej P x 11, ev P x 11 x 11, proposals P x 11, types P, sky-ring log densities P,
+        stats n_chains x 11), point order."""
         ej = np.zeros((self.P, 11))
         ev = np.zeros((self.P, 11, 11))
         y = np.zeros((self.P, 11))
         ty = np.zeros(self.P, np.int32)
-        st = (ctypes.c_longlong * (9 * self.n_chains))()
-        _check(lib().qcbc_pt_get_update(self.h, _d(ej), _d(ev), _d(y), _i(ty), st), "qcbc_pt_get_update")
-        return ej, ev, y, ty, np.array(st[:], np.int64).reshape(self.n_chains, 9)
+        jc = np.zeros(self.P)
+        st = (ctypes.c_longlong * (11 * self.n_chains))()
+        _check(lib().qcbc_pt_get_update(self.h, _d(ej), _d(ev), _d(y), _i(ty), _d(jc), st), "qcbc_pt_get_update")
+        return ej, ev, y, ty, jc, np.array(st[:], np.int64).reshape(self.n_chains, 11)
 
     def get_history(self):
         hist = np.zeros((self.n_chains, self.n_temps, self.desc.n_hist, 11))

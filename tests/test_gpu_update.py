@@ -1,13 +1,14 @@
 """QuickCBC's own PT step on the GPU against the oracle's replay.
 
-update()'s proposal mix (QuickCBC.c:2238-2405) and MCMC_all's step
-(:2029-2092) on the heterodyned likelihood, resident on the GPU
-(qcbc_pt_create_update).  The oracle replays the same chains from the same
+update()'s proposal mix (QuickCBC.c:2238-2405: Fisher, differential
+evolution, jiggle and the sky ring) and MCMC_all's step (:2029-2092) on the
+heterodyned likelihood, resident on the GPU (qcbc_pt_create_update).  The oracle replays the same chains from the same
 counter-based random numbers (its own heterodyned logL, the GPU's Fisher
 jumps): proposal types, accepted moves, swaps and history fills must agree
 exactly, states to rounding (the GPU's log, cos and pow are within a few ulps
-of glibc's, so a proposal differs from the oracle's by ~1e-15 relative;
-logL by <= 1e-8 as the heterodyned-likelihood tests).  Column groups and
+of glibc's, so a proposal differs from the oracle's by ~1e-15 relative; a
+sky-ring move's finite-difference density amplifies that to ~1e-9 of its
+log; logL by <= 1e-8 as the heterodyned-likelihood tests).  Column groups and
 graph replay give bit-identical chains.
 """
 import numpy as np
@@ -23,10 +24,10 @@ pytestmark = pytest.mark.gpu
 N_TEMPS, N_CHAINS, N_HIST, EVERY, LADDER, SEED = 4, 6, 16, 2, 1.5, 11
 
 
-@pytest.fixture(scope="module")
-def setup():
-    an, inj, D, SN = _data.analysis(0)
-    h = _data.het(0)
+@pytest.fixture(scope="module", params=[0, 1])
+def setup(request):
+    an, inj, D, SN = _data.analysis(request.param)
+    h = _data.het(request.param)
     c = Q.Context(an, D, SN)
     g = Q.Het(c, arrays=h)
     lo, hi = W.prior_box(an)
@@ -36,21 +37,22 @@ def setup():
     c.close()
 
 
-def _make(setup, groups, n_hist=N_HIST):
-    an, h, c, g, lo, hi, init = setup
-    desc = Q.update_desc(lo, hi, n_hist=n_hist, hist_every=EVERY)
-    return Q.PTUpdate(c, g, N_TEMPS, N_CHAINS, LADDER, init, desc, seed=SEED, groups=groups)
+def _make(setup, groups, n_hist=N_HIST, c=0.2):
+    an, h, ctx, g, lo, hi, init = setup
+    desc = Q.update_desc(lo, hi, c=c, n_hist=n_hist, hist_every=EVERY)
+    return Q.PTUpdate(ctx, g, N_TEMPS, N_CHAINS, LADDER, init, desc, seed=SEED, groups=groups)
 
 
-def _uconst(lo, hi):
-    return np.concatenate([lo, hi, [0.5, 0.3, 0.0, 0.05]])
+def _uconst(lo, hi, an, c=0.2):
+    sky = [an.gmst, an.nifo] + list(an.labels) + [0] * (3 - an.nifo)
+    return np.concatenate([lo, hi, [0.5, 0.3, c, 0.05], sky])
 
 
 def test_start_matches_the_oracle(setup):
     an, h, c, g, lo, hi, init = setup
     pt = _make(setup, 2)
     x, L, _, _ = pt.get()
-    ej, ev, y, typ, stats = pt.get_update()
+    ej, ev, y, typ, jac, stats = pt.get_update()
     hist, cnt = pt.get_history()
     assert np.array_equal(x, init)
     Lo, so = O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, h, x)
@@ -60,41 +62,46 @@ def test_start_matches_the_oracle(setup):
     assert np.array_equal(ej, ej2) and np.array_equal(ev, ev2)
     assert np.array_equal(hist, O.pt_update_hist_init(x, N_TEMPS, N_CHAINS, N_HIST, SEED))
     assert (cnt == 0).all() and (stats == 0).all()
-    uc = _uconst(lo, hi)
+    uc = _uconst(lo, hi, an)
     for q in range(N_TEMPS * N_CHAINS):
         t, col = divmod(q, N_CHAINS)
-        to, yo = O.update_propose(x[q], ej[q], ev[q], hist[col, t], np.sqrt(LADDER ** t), uc[22:25], SEED, q)
+        to, yo, jo = O.update_propose(x[q], ej[q], ev[q], hist[col, t], np.sqrt(LADDER ** t), uc[22:], SEED, q)
         assert typ[q] == to
-        assert np.abs(y[q] - yo).max() <= 1e-13 * (1.0 + np.abs(x[q]).max()), (q, typ[q], y[q] - yo)
+        assert np.abs(y[q] - yo).max() <= 1e-12 * (1.0 + np.abs(x[q]).max()), (q, typ[q], y[q] - yo)
+        assert abs(jac[q] - jo) <= 1e-6 * (1.0 + abs(jo)), (q, jac[q], jo)
     pt.close()
 
 
-def test_chains_match_the_oracle_replay(setup):
-    an, h, c, g, lo, hi, init = setup
-    pt = _make(setup, 2)
+@pytest.mark.parametrize("c", [0.0, 0.2])
+def test_chains_match_the_oracle_replay(setup, c):
+    an, h, ctx, g, lo, hi, init = setup
+    pt = _make(setup, 2, c=c)
     x, L, _, _ = pt.get()
-    ej, ev, y, typ, stats = pt.get_update()
+    ej, ev, y, typ, jac, stats = pt.get_update()
     hist, cnt = pt.get_history()
     Lo, _ = O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, h, x)
     S = dict(x=x.copy(), L=Lo, ej=ej.copy(), ev=ev.copy(), hist=hist.copy(), hcount=cnt.copy(), y=y.copy(),
-             typ=typ.copy(), stats=stats.copy(), step=0)
+             typ=typ.copy(), jac=jac.copy(), stats=stats.copy(), step=0)
     nsteps = 40
-    print("update replay: %d x %d points, %d steps" % (N_TEMPS, N_CHAINS, nsteps), flush=True)
+    print("update replay: %d det, %d x %d points, %d steps, c = %g" % (an.nifo, N_TEMPS, N_CHAINS, nsteps, c),
+          flush=True)
     pt.run(nsteps)
-    O.pt_update_run(an.labels, an.gmst, an.fref, an.Tobs, h, _uconst(lo, hi), N_HIST, EVERY, N_TEMPS, N_CHAINS,
-                    LADDER, SEED, nsteps, S)
+    O.pt_update_run(an.labels, an.gmst, an.fref, an.Tobs, h, _uconst(lo, hi, an, c), N_HIST, EVERY, N_TEMPS,
+                    N_CHAINS, LADDER, SEED, nsteps, S)
     x1, L1, _, _ = pt.get()
-    ej1, ev1, y1, typ1, stats1 = pt.get_update()
+    ej1, ev1, y1, typ1, jac1, stats1 = pt.get_update()
     hist1, cnt1 = pt.get_history()
     assert np.array_equal(stats1, S["stats"]), (stats1, S["stats"])
     assert np.array_equal(cnt1, S["hcount"]) and np.array_equal(typ1, S["typ"])
     assert stats1[:, 4::2].sum() > 0 and stats1[:, 2].sum() > 0  # moves and swaps were accepted
+    assert (stats1[:, 9].sum() > 0) == (c > 0)  # sky-ring proposals with c > 0
     tol = 1e-12 * (1.0 + np.abs(x1))
     assert (np.abs(x1 - S["x"]) <= tol).all()
     assert np.abs(L1 - S["L"]).max() <= 1e-8
     assert np.array_equal(ej1, S["ej"]) and np.array_equal(ev1, S["ev"])  # swapped with their states
     assert (np.abs(hist1 - S["hist"]) <= 1e-12 * (1.0 + np.abs(hist1))).all()
     assert (np.abs(y1 - S["y"]) <= 1e-12 * (1.0 + np.abs(y1))).all()
+    assert (np.abs(jac1 - S["jac"]) <= 1e-6 * (1.0 + np.abs(jac1))).all()
     pt.close()
 
 
@@ -112,7 +119,7 @@ def test_groups_and_graph_replay_give_the_same_chains(setup):
     ua, ub, ug = a.get_update(), b.get_update(), gph.get_update()
     for k in (0, 1):
         assert np.array_equal(ra[k], rb[k]) and np.array_equal(ra[k], rg[k])
-    for k in range(5):
+    for k in range(6):
         assert np.array_equal(ua[k], ub[k]) and np.array_equal(ua[k], ug[k])
     for p in (a, b, gph):
         p.close()
@@ -124,23 +131,23 @@ def test_refresh_fisher(setup):
     pt.run(12)
     pt.refresh_fisher()
     x, L, _, _ = pt.get()
-    ej, ev, y, typ, stats = pt.get_update()
+    ej, ev, y, typ, jac, stats = pt.get_update()
     hist, cnt = pt.get_history()
     F, ej2, ev2, _ = g.fisher(x)
     assert np.array_equal(ej, ej2) and np.array_equal(ev, ev2)
-    uc = _uconst(lo, hi)
+    uc = _uconst(lo, hi, an)
     P = N_TEMPS * N_CHAINS
     for q in range(P):  # the pending proposals of step 12, remade with the new jumps
         t, col = divmod(q, N_CHAINS)
-        to, yo = O.update_propose(x[q], ej[q], ev[q], hist[col, t], np.sqrt(LADDER ** t), uc[22:25], SEED,
-                                  12 * P + q)
-        assert typ[q] == to and np.abs(y[q] - yo).max() <= 1e-13 * (1.0 + np.abs(x[q]).max())
+        to, yo, jo = O.update_propose(x[q], ej[q], ev[q], hist[col, t], np.sqrt(LADDER ** t), uc[22:], SEED,
+                                      12 * P + q)
+        assert typ[q] == to and np.abs(y[q] - yo).max() <= 1e-12 * (1.0 + np.abs(x[q]).max())
     pt.close()
 
 
 def test_invalid_descriptions_are_rejected(setup):
     an, h, c, g, lo, hi, init = setup
-    for kw in (dict(c=0.2), dict(a=0.2, b=0.3), dict(n_hist=1), dict(hist_every=0)):
+    for kw in (dict(c=0.4), dict(a=0.2, b=0.3), dict(c=-0.1), dict(n_hist=1), dict(hist_every=0)):
         desc = Q.update_desc(lo, hi, **kw)
         with pytest.raises(RuntimeError):
             Q.PTUpdate(c, g, N_TEMPS, N_CHAINS, LADDER, init, desc, seed=SEED)
