@@ -117,6 +117,21 @@ void F_ant(double psi, double alpha, double sindelta, double GMST, double *Fplus
 void Times(double alpha, double sindelta, double GMST, double *dtimes);
 double fbegin(double *param);
 
+/* Replaces templates, QuickCBC.c:3527-3611 (prototype QuickCBC.h:95): the
+ * reference-frame templates of MCMC_intrinsic.  params: the seven
+ * quasi-intrinsic parameters (ln Mc, ln Mt, chi1, chi2, phi0 = params[4]
+ * itself, tc, ln DL) then, for detector j = 1, 2, its phase offset, time
+ * offset and amplitude ratio relative to detector 0 at params[7 + 3 (j-1)].
+ * hwave as fulltemplates. */
+void templates(struct Net *net, double **hwave, RealVector *freq, double *params, int N);
+
+/* Replaces log_likelihood_intrinsic, QuickCBC.c:4093-4145 (prototype
+ * QuickCBC.h:109): sum over detectors of (d|h) - (h|h)/2 with templates();
+ * with more than one detector 0 unless at least two have 2 ((d|h) - (h|h)/2)
+ * above 25. */
+double log_likelihood_intrinsic(struct Net *net, double **D, double *params, RealVector *freq, double **SN,
+                                int N, double Tobs);
+
 /* Replaces fullphaseamp, QuickCBC.c:3780-3869 (prototype QuickCBC.h:98): per
  * detector the projected amplitude (tapered by exp(10 (f - fs)/fs) below
  * fbegin) and phase 2 pi f (ts - td) + lambda - Psi on freq->data[0..N/2).
@@ -253,6 +268,15 @@ int qcbc_autotune(qcbc_ctx *ctx, int precision, int B);
  * kernel, measured with HIP events on the context's stream. */
 int qcbc_ctx_timing(qcbc_ctx *ctx, int enable, double *kernel_ms, long long *launches);
 int qcbc_ctx_sync(qcbc_ctx *ctx);
+
+/* log_likelihood_intrinsic (QuickCBC.c:4093-4145) for B points of
+ * templates' reference-frame parameters (B x (7 + 3 (n_det - 1)), host
+ * memory), fused FP64 path on the bound data: logL[b], and snrsq (B x n_det,
+ * may be NULL) the per-detector 2 ((d|h) - (h|h)/2). */
+int qcbc_loglike_intrinsic(qcbc_ctx *ctx, const double *params, int B, double *logL, double *snrsq,
+                           int *status);
+/* templates() (QuickCBC.c:3527-3611) for B points: hwave B x n_det x N. */
+int qcbc_templates_intrinsic(qcbc_ctx *ctx, const double *params, int B, double *hwave, int *status);
 
 const char *qcbc_strerror(int code);
 int qcbc_device_info(char *name, size_t len, int *cu_count, double *mem_gb);

@@ -430,3 +430,34 @@ def update_prior(y, uconst):
     j = np.zeros(1)
     f = L.qo_update_prior(_d(y), _d(uc), _d(j))
     return f, float(j[0])
+
+
+def templates(nifo, fref, N, Tobs, params):
+    """templates (QuickCBC.c:3527): (status, hwave nifo x N) for the reference-frame parameters."""
+    L = lib()
+    L.qo_templates_flat.restype = ctypes.c_int
+    L.qo_templates_flat.argtypes = [ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double, _dp]
+    p = np.ascontiguousarray(params, np.float64)
+    h = np.zeros((nifo, N))
+    st = L.qo_templates_flat(nifo, fref, _d(h), N, Tobs, _d(p))
+    return st, h
+
+
+def loglike_intrinsic(nifo, fref, fmin, fmax, D, SN, N, Tobs, params):
+    """log_likelihood_intrinsic (QuickCBC.c:4093) for a batch: (logL, snrsq B x nifo, status)."""
+    L = lib()
+    L.qo_loglike_intrinsic_batch_flat.restype = None
+    L.qo_loglike_intrinsic_batch_flat.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                  _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int, _dp,
+                                                  _dp, _ip]
+    ips = 7 + 3 * (nifo - 1)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, ips)
+    B = p.shape[0]
+    logL = np.zeros(B)
+    snr = np.zeros((B, nifo))
+    st = np.zeros(B, np.int32)
+    D = np.ascontiguousarray(D, np.float64)
+    SN = np.ascontiguousarray(SN, np.float64)
+    L.qo_loglike_intrinsic_batch_flat(nifo, fref, fmin, fmax, _d(D), _d(SN), N, Tobs, _d(p), B, _d(logL), _d(snr),
+                                      _i(st))
+    return logL, snr, st

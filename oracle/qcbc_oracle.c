@@ -2167,3 +2167,131 @@ int qo_update_prior(const double *y, const double *uconst, double *pMcM) {
   u.hist_every = 1;
   return qo_prior_flag(y, &u, pMcM);
 }
+
+/* templates, QuickCBC.c:3527-3611: the reference-frame templates of
+ * MCMC_intrinsic.  params: ln Mc, ln Mt, chi1, chi2, phi0 (= params[4]
+ * itself), tc, ln DL, then for detector j = 1, 2 its phase offset, time
+ * offset and amplitude ratio at params[7 + 3 (j-1)]; detector 0 has 0, 0, 1. */
+int qo_templates(int nifo, double fref, double **hwave, const double *freq, const double *params, int N) {
+  double fs = qo_fbegin(params);
+  double Tobs = 1.0 / freq[1];
+  double sqT = sqrt(Tobs);
+  double mc = qo_exp_cr(params[0]), mt = qo_exp_cr(params[1]);
+  double eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
+  double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  double phi0 = params[4];
+  double ts = Tobs - params[5];
+  double distance = qo_exp_cr(params[6]);
+  double *amp = malloc(sizeof(double) * (N / 2)), *phase = malloc(sizeof(double) * (N / 2));
+  int st, j, i;
+  if (!amp || !phase) {
+    free(amp);
+    free(phase);
+    return QO_ENOMEM;
+  }
+  st = qo_h22_ampphase(freq, N / 2, phi0, fref, m1_SI, m2_SI, params[2], params[3], distance, amp, phase);
+  if (st) {
+    free(amp);
+    free(phase);
+    return st;
+  }
+  for (j = 0; j < nifo; j++) {
+    double pd = 0.0, td = 0.0, Ar = 1.0; /* phase offset, time offset, amplitude ratio */
+    if (j > 0) {
+      pd = params[(j - 1) * 3 + 7];
+      td = params[(j - 1) * 3 + 7 + 1];
+      Ar = params[(j - 1) * 3 + 7 + 2];
+    }
+    hwave[j][0] = 0.0;
+    hwave[j][N / 2] = 0.0;
+    for (i = 1; i < N / 2; i++) {
+      double f = freq[i];
+      if (f > fs) {
+        double A = Ar * QO_H22FAC * amp[i] / sqT;
+        double x = QO_TPI * f * (ts - td) + pd - phase[i];
+        hwave[j][i] = A * cos(x);
+        hwave[j][N - i] = A * sin(x);
+      } else {
+        hwave[j][i] = 0.0;
+        hwave[j][N - i] = 0.0;
+      }
+    }
+  }
+  free(amp);
+  free(phase);
+  return QO_OK;
+}
+
+/* log_likelihood_intrinsic, QuickCBC.c:4093-4145 (lhold = 0; fmin/fmax at
+ * run time): sum_d [(d|h) - (h|h)/2] with templates; with more than one
+ * detector 0 unless at least two have snrsq = 2 ((d|h) - (h|h)/2) > 25.
+ * snrsq (nifo) may be NULL. */
+double qo_log_likelihood_intrinsic(int nifo, double fref, double fmin, double fmax, double **D,
+                                   const double *params, const double *freq, double **SN, double *snrsq, int N,
+                                   double Tobs, int *status) {
+  int imin = (int)(Tobs * fmin), imax = (int)(Tobs * fmax);
+  double logL = 0.0, s2[3] = {0, 0, 0};
+  double **tw;
+  int i, j, st;
+  if (imax > N / 2) imax = N / 2;
+  tw = malloc(sizeof(double *) * (nifo + 1));
+  for (i = 0; i < nifo + 1; i++) tw[i] = malloc(sizeof(double) * (N + 1));
+  st = qo_templates(nifo, fref, tw, freq, params, N);
+  if (status) *status = st;
+  if (st == 0) {
+    for (i = 0; i < nifo; i++) {
+      double HH = qo_fourier_nwip(tw[i], tw[i], SN[i], imin, imax, N);
+      double HD = qo_fourier_nwip(D[i], tw[i], SN[i], imin, imax, N);
+      double x = HD - 0.5 * HH;
+      s2[i] = 2.0 * x;
+      logL += x;
+    }
+    if (nifo > 1) {
+      j = 0; /* detectors with snr > 5 */
+      for (i = 0; i < nifo; i++)
+        if (s2[i] > 25.0) j++;
+      if (j < 2) logL = 0.0; /* a signal in at least two */
+    }
+  }
+  if (snrsq)
+    for (i = 0; i < nifo; i++) snrsq[i] = s2[i];
+  for (i = 0; i < nifo + 1; i++) free(tw[i]);
+  free(tw);
+  return logL;
+}
+
+int qo_templates_flat(int nifo, double fref, double *hwave, int N, double Tobs, const double *params) {
+  double *rows[3];
+  double *freq = qo_make_freq(N, Tobs);
+  int i, st;
+  for (i = 0; i < nifo; i++) rows[i] = hwave + (size_t)i * N;
+  qo_spline_init();
+  st = qo_templates(nifo, fref, rows, freq, params, N);
+  free(freq);
+  return st;
+}
+
+void qo_loglike_intrinsic_batch_flat(int nifo, double fref, double fmin, double fmax, const double *D,
+                                     const double *SN, int N, double Tobs, const double *params, int B,
+                                     double *logL, double *snrsq, int *status) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs);
+  const int ips = 7 + 3 * (nifo - 1);
+  int b, i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (b = 0; b < B; b++) {
+    int st = 0;
+    logL[b] = qo_log_likelihood_intrinsic(nifo, fref, fmin, fmax, Dr, params + (size_t)ips * b, freq, Sr,
+                                          snrsq + (size_t)nifo * b, N, Tobs, &st);
+    if (status) status[b] = st;
+  }
+  free(freq);
+}

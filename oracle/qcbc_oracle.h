@@ -158,6 +158,13 @@ void qo_pt_update_run(int nifo, const int *labels, double gmst, double fref, dou
                       double *ej, double *ev, double *hist, long long *hcount, double *y, int *typ, double *jac,
                       long long *stats);
 
+/* templates (QuickCBC.c:3527-3611) and log_likelihood_intrinsic (:4093-4145):
+ * params 7 + 3 (nifo - 1) per point (reference-frame form). */
+int qo_templates_flat(int nifo, double fref, double *hwave, int N, double Tobs, const double *params);
+void qo_loglike_intrinsic_batch_flat(int nifo, double fref, double fmin, double fmax, const double *D,
+                                     const double *SN, int N, double Tobs, const double *params, int B,
+                                     double *logL, double *snrsq, int *status);
+
 #ifdef __cplusplus
 }
 #endif

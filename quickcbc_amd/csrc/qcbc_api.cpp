@@ -73,7 +73,7 @@ static int ctx_reserve(qcbc_ctx* c, int B) {
   c->d_params = nullptr; c->d_pts = nullptr; c->d_status = nullptr;
   c->d_logL = nullptr; c->d_rho = nullptr; c->d_partial = nullptr;
   c->cap_B = 0;
-  HIPCHK(hipMalloc(&c->d_params, sizeof(double) * 11 * nb));
+  HIPCHK(hipMalloc(&c->d_params, sizeof(double) * 13 * nb));  // 11, or templates' 7 + 3 (n_det - 1)
   HIPCHK(hipMalloc(&c->d_pts, sizeof(Point) * nb));
   HIPCHK(hipMalloc(&c->d_status, sizeof(int) * nb));
   HIPCHK(hipMalloc(&c->d_logL, sizeof(double) * nb));
@@ -375,8 +375,10 @@ int qcbc_loglike_plain(qcbc_ctx* c, const double* params, int B, double* logL, d
 }
 
 // ------------------------------------------------- waveform / templates
+// ips = 0: the 11 parameters of fulltemplates; ips > 0: templates' ips
+// reference-frame parameters per point
 static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, double* phase, double* hwave,
-                      int* status) {
+                      int* status, int ips = 0) {
   if (!c || B < 0 || (B > 0 && !params)) return QCBC_EINVAL;
   if (B == 0) return QCBC_OK;
   std::lock_guard<std::mutex> lk(c->mu);
@@ -389,8 +391,12 @@ static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, dou
   HIPCHK(hipMalloc(&d_amp, sizeof(double) * nh * chunk));
   HIPCHK(hipMalloc(&d_ph, sizeof(double) * nh * chunk));
   if (hwave) HIPCHK(hipMalloc(&d_hw, sizeof(double) * nd * N * chunk));
-  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * 11 * B, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(launch_setup(c->d_params, B, c->an, c->d_pts, c->d_status, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * (ips ? ips : 11) * B, hipMemcpyHostToDevice,
+                        c->stream));
+  if (ips)
+    HIPCHK(launch_setup_intrinsic(c->d_params, ips, B, c->an, c->d_pts, c->d_status, c->stream));
+  else
+    HIPCHK(launch_setup(c->d_params, B, c->an, c->d_pts, c->d_status, c->stream));
   for (int b0 = 0; b0 < B; b0 += chunk) {
     int n = std::min(chunk, B - b0);
     HIPCHK(launch_waveform(c->d_pts + b0, n, nh, c->desc.Tobs, d_amp, d_ph, c->stream));
@@ -418,6 +424,33 @@ int qcbc_waveform(qcbc_ctx* c, const double* params, int B, double* amp, double*
 int qcbc_templates(qcbc_ctx* c, const double* params, int B, double* hwave, int* status) {
   if (!hwave) return QCBC_EINVAL;
   return wave_batch(c, params, B, nullptr, nullptr, hwave, status);
+}
+
+int qcbc_templates_intrinsic(qcbc_ctx* c, const double* params, int B, double* hwave, int* status) {
+  if (!c || !hwave) return QCBC_EINVAL;
+  return wave_batch(c, params, B, nullptr, nullptr, hwave, status, 7 + 3 * (c->desc.n_det - 1));
+}
+
+int qcbc_loglike_intrinsic(qcbc_ctx* c, const double* params, int B, double* logL, double* snrsq, int* status) {
+  if (!c || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  if (!c->bound) return QCBC_ENOTBOUND;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  const int nd = c->desc.n_det, ips = 7 + 3 * (nd - 1);
+  HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * ips * B, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_setup_intrinsic(c->d_params, ips, B, c->an, c->d_pts, c->d_status, c->stream));
+  const LLInst in = make_inst(c, c->tile[QCBC_FP64], B, effective_splits(c, QCBC_FP64, B), c->d_pts, c->d_partial);
+  st = ll_launch_timed(c, QCBC_FP64, in, c->stream);
+  if (st) return st;
+  HIPCHK(launch_finalize_intrinsic(c->d_partial, in.splits, nd, c->d_pts, B, c->d_logL, c->d_rho, c->stream));
+  HIPCHK(hipMemcpyAsync(logL, c->d_logL, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
+  if (snrsq) HIPCHK(hipMemcpyAsync(snrsq, c->d_rho, sizeof(double) * nd * B, hipMemcpyDeviceToHost, c->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, c->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return QCBC_OK;
 }
 
 // ------------------------------------------------------ shapes / timing

@@ -918,6 +918,34 @@ __device__ inline void point_setup_projection(const double* prm, const AnalysisC
   }
 }
 
+// templates' projection (QuickCBC.c:3571-3607), the reference-frame form of
+// MCMC_intrinsic: detector 0 unprojected (amplitude ratio 1, phase and time
+// offsets 0), detector j > 0 with amplitude ratio, phase offset and time
+// offset q[7 + 3 (j-1) + {2, 0, 1}] relative to it; A = Ar h22fac amp / sqrt(Tobs)
+// and x = 2 pi f (ts - td) + pd - Psi, so Ar, pd and td take the places of
+// Fs, lambda and the light-travel delay.
+__device__ inline void point_setup_projection_offsets(const double* q, const AnalysisConst& an, Point& c) {
+  const double ts = an.Tobs - q[5];
+  for (int d = 0; d < 3; d++) {
+    c.Fs[d] = c.tshift[d] = c.lambda[d] = c.Ascale[d] = 0.0;
+    if (d >= an.n_det) continue;
+    double Ar = 1.0, pd = 0.0, td = 0.0;
+    if (d == 1) {
+      pd = q[7];
+      td = q[8];
+      Ar = q[9];
+    } else if (d == 2) {
+      pd = q[10];
+      td = q[11];
+      Ar = q[12];
+    }
+    c.Fs[d] = Ar;
+    c.lambda[d] = pd;
+    c.tshift[d] = ts - td;
+    c.Ascale[d] = Ar * H22FAC;
+  }
+}
+
 // After the model, fbegin (c.fs) and the projection: the detector scales,
 // the contributing bins and the fused kernels' constants.
 __device__ inline void point_setup_finish(const AnalysisConst& an, Point& c) {

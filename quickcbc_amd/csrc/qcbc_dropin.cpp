@@ -197,6 +197,37 @@ void fulltemplates(struct Net* net, double** hwave, RealVector* freq, double* pa
   for (int d = 0; d < net->Nifo; d++) std::memcpy(hwave[d], tmp.data() + (size_t)d * N, sizeof(double) * N);
 }
 
+void templates(struct Net* net, double** hwave, RealVector* freq, double* params, int N) {
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !hwave || !freq || !freq->data || !params)
+    fail("templates", QCBC_EINVAL);
+  qcbc_desc desc = desc_from(net, N, 1.0 / freq->data[1]);  // QuickCBC.c:3539
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc, nullptr, nullptr, &st);
+  if (!ctx) fail("templates", st);
+  std::vector<double> tmp((size_t)net->Nifo * N);
+  int pst = 0;
+  st = qcbc_templates_intrinsic(ctx, params, 1, tmp.data(), &pst);
+  if (st) fail("templates", st);
+  if (domain_error("templates", pst)) std::fill(tmp.begin(), tmp.end(), 0.0);
+  for (int d = 0; d < net->Nifo; d++) std::memcpy(hwave[d], tmp.data() + (size_t)d * N, sizeof(double) * N);
+}
+
+double log_likelihood_intrinsic(struct Net* net, double** D, double* params, RealVector* freq, double** SN, int N,
+                                double Tobs) {
+  (void)freq;  // the grid is f_i = i/Tobs, freq[0] = 1/Tobs (QuickCBC.c:418-419)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params) fail("log_likelihood_intrinsic", QCBC_EINVAL);
+  qcbc_desc desc = desc_from(net, N, Tobs);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc, D, SN, &st);
+  if (!ctx) fail("log_likelihood_intrinsic", st);
+  double logL = 0.0;
+  int pst = 0;
+  st = qcbc_loglike_intrinsic(ctx, params, 1, &logL, nullptr, &pst);
+  if (st) fail("log_likelihood_intrinsic", st);
+  if (domain_error("log_likelihood_intrinsic", pst)) return 0.0;
+  return logL;
+}
+
 int IMRPhenomDGenerateh22FDAmpPhase(AmpPhaseFDWaveform** h22, RealVector* freq, double phi0, double fRef_in,
                                     double m1_SI, double m2_SI, double chi1, double chi2, double distance) {
   // IMRPhenomD.c:197-210: pointer checks

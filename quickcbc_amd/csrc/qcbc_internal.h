@@ -11,6 +11,12 @@
 namespace qcbc {
 
 hipError_t launch_qnm_init(hipStream_t s);
+// k_setup for templates' reference-frame parameters, ips per point (7 + 3 (n_det - 1))
+hipError_t launch_setup_intrinsic(const double* d_params, int ips, int B, const AnalysisConst& an, Point* d_pts,
+                                  int* d_status, hipStream_t s);
+// log_likelihood_intrinsic's logL and per-detector snrsq from FP64 partials
+hipError_t launch_finalize_intrinsic(const double* partial, int splits, int nd, const Point* pts, int B,
+                                     double* logL, double* snrsq, hipStream_t s);
 hipError_t launch_setup(const double* d_params, int B, const AnalysisConst& an, Point* d_pts, int* d_status,
                         hipStream_t s);
 hipError_t launch_setup_h22(double m1, double m2, double c1, double c2, double dist, double phi0, double fref,

@@ -824,6 +824,50 @@ __global__ void k_finalize(const double* __restrict__ partial, int splits, int n
   finalize_point(partial, splits, nd, pts[b], b, mode, wref, logL, rho);
 }
 
+// log_likelihood_intrinsic's sums (QuickCBC.c:4113-4136) from the FP64
+// partials: per detector x = HD - HH/2 and snrsq = 2 x, logL the sum of the x;
+// with more than one detector logL = 0 unless at least two have snrsq > 25.
+__global__ void k_finalize_intrinsic(const double* __restrict__ partial, int splits, int nd,
+                                     const Point* __restrict__ pts, int B, double* __restrict__ logL,
+                                     double* __restrict__ snrsq) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const Point& c = pts[b];
+  if (c.status != ST_OK) {
+    logL[b] = 0.0;
+    for (int d = 0; d < nd; d++) snrsq[(size_t)b * nd + d] = 0.0;
+    return;
+  }
+  double L = 0.0;
+  int loud = 0;
+  for (int d = 0; d < nd; d++) {
+    double HH = 0.0, HD = 0.0;
+    for (int s = 0; s < splits; s++) {
+      HH += partial[((size_t)b * splits + s) * (2 * nd) + 2 * d];
+      HD += partial[((size_t)b * splits + s) * (2 * nd) + 2 * d + 1];
+    }
+    const double A = c.Ascale[d];
+    HH *= A * A;
+    HD *= A;
+    HH *= 4.0;  // fourier_nwip returns 4 sum (Utilities.c:734)
+    HD *= 4.0;
+    const double x = HD - 0.5 * HH;
+    snrsq[(size_t)b * nd + d] = 2.0 * x;
+    if (2.0 * x > 25.0) loud++;
+    L += x;
+  }
+  if (nd > 1 && loud < 2) L = 0.0;  // a signal in at least two detectors
+  logL[b] = L;
+}
+
+hipError_t launch_finalize_intrinsic(const double* partial, int splits, int nd, const Point* pts, int B,
+                                     double* logL, double* snrsq, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize_intrinsic, dim3((B + 127) / 128), dim3(128), 0, s, partial, splits, nd, pts, B, logL,
+                     snrsq);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------ launchers
 size_t ll_lds_bytes(int precision, int nd, int tile) {
   if (precision == 0) return (size_t)(4 + 3 * nd) * tile * sizeof(double);

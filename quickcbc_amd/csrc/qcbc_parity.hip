@@ -102,6 +102,49 @@ __global__ __launch_bounds__(256) void k_setup(const double* __restrict__ params
   }
 }
 
+// k_setup for templates' parameters (QuickCBC.c:3527-3611): ips doubles per
+// point, the quasi-intrinsic seven (ln Mc, ln Mt, chi1, chi2, phi0, tc,
+// ln DL, with phi0 = params[4] itself: the model sees 2 params[4], which
+// k_setup's model halves exactly) then (phase, time, amplitude) offsets of
+// detectors 1, 2.  Same four-wave split as k_setup.
+__global__ __launch_bounds__(256) void k_setup_intrinsic(const double* __restrict__ params, int ips, int B,
+                                                         AnalysisConst an, Point* __restrict__ pts,
+                                                         int* __restrict__ status) {
+  const int lane = threadIdx.x & 63, role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x * 64 + lane;
+  const bool on = b < B;
+  int st = ST_OK;
+  double prm[11], q[13];
+  if (on) {
+#pragma unroll
+    for (int i = 0; i < 13; i++) q[i] = i < ips ? params[(size_t)ips * b + i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 11; i++) prm[i] = i < 7 ? q[i] : 0.0;
+    prm[4] = 2.0 * q[4];
+    Point& c = pts[b];
+    if (role == 0)
+      st = point_setup_model_half(prm, an, c, 0);
+    else if (role == 3)
+      point_setup_model_half(prm, an, c, 1);
+    else if (role == 1)
+      c.fs = fbegin(prm);
+    else
+      point_setup_projection_offsets(q, an, c);
+  }
+  __syncthreads();
+  if (on && role == 0) {
+    Point& c = pts[b];
+    if (st == ST_OK) {
+      point_setup_refphase(prm, an, c);
+      point_setup_finish(an, c);
+    } else {
+      c.kstart = c.kend = an.imin;
+    }
+    c.status = st;
+    if (status) status[b] = st;
+  }
+}
+
 // One point from explicit IMRPhenomD arguments (the h22 drop-in).
 __global__ __launch_bounds__(64) void k_setup_h22(double m1_SI, double m2_SI, double chi1, double chi2,
                                                   double distance, double phi0, double fref, double fmin0,
@@ -290,6 +333,13 @@ __global__ void k_geometry(int what, const double* __restrict__ in, int n, doubl
 // ------------------------------------------------------------ launchers
 hipError_t launch_qnm_init(hipStream_t s) {
   hipLaunchKernelGGL(k_qnm_init, dim3(1), dim3(64), 0, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_setup_intrinsic(const double* d_params, int ips, int B, const AnalysisConst& an, Point* d_pts,
+                                  int* d_status, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup_intrinsic, dim3((B + 63) / 64), dim3(256), 0, s, d_params, ips, B, an, d_pts, d_status);
   return hipGetLastError();
 }
 

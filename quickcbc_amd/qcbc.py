@@ -131,6 +131,16 @@ def lib():
         L.qcbc_het_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_het_time_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
         L.qcbc_het_destroy.argtypes = [_vp]
+        # reference-frame likelihood
+        L.qcbc_loglike_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
+        L.qcbc_templates_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.templates.restype = None
+        L.templates.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(_dp), ctypes.POINTER(RealVector), _dp,
+                                ctypes.c_int]
+        L.log_likelihood_intrinsic.restype = ctypes.c_double
+        L.log_likelihood_intrinsic.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(_dp), _dp,
+                                               ctypes.POINTER(RealVector), ctypes.POINTER(_dp), ctypes.c_int,
+                                               ctypes.c_double]
         # Fisher matrix and jump directions
         L.qcbc_het_fisher.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _ip]
         L.qcbc_het_fisher_device.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp]
@@ -197,6 +207,27 @@ class Context:
         self.h = h
         if D is not None:
             self.bind(D, SN if SN is not None else an.SN())
+
+    def loglike_intrinsic(self, params):
+        """log_likelihood_intrinsic for a batch of templates' parameters: (logL, snrsq B x n_det, status)."""
+        ips = 7 + 3 * (self.an.nifo - 1)
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, ips)
+        B = p.shape[0]
+        logL = np.zeros(B)
+        snr = np.zeros((B, self.an.nifo))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_loglike_intrinsic(self.h, _d(p), B, _d(logL), _d(snr), _i(st)), "qcbc_loglike_intrinsic")
+        return logL, snr, st
+
+    def templates_intrinsic(self, params):
+        """templates() for a batch: (hwave B x n_det x N, status)."""
+        ips = 7 + 3 * (self.an.nifo - 1)
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, ips)
+        B = p.shape[0]
+        hw = np.zeros((B, self.an.nifo, self.an.N))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_templates_intrinsic(self.h, _d(p), B, _d(hw), _i(st)), "qcbc_templates_intrinsic")
+        return hw, st
 
     def fisher_evec(self, fish):
         """FisherEvec of a batch of d x d matrices on this analysis's GPU: (ej B x d, ev B x d x d)."""
@@ -940,3 +971,20 @@ def dropin_fisher_evec(fish):
     ev = RowMatrix(np.zeros((d, d)))
     lib().FisherEvec(F.p, _d(ej), ev.p, d)
     return ej, ev.arr
+
+
+def dropin_templates(an, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    h = RowMatrix(np.zeros((an.nifo, an.N)))
+    p = np.ascontiguousarray(params, np.float64)
+    lib().templates(ctypes.byref(net), h.p, ctypes.byref(fr), _d(p), an.N)
+    return h.arr
+
+
+def dropin_log_likelihood_intrinsic(an, Drows, SNrows, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.ascontiguousarray(params, np.float64)
+    return lib().log_likelihood_intrinsic(ctypes.byref(net), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, an.N,
+                                          an.Tobs)

@@ -33,6 +33,7 @@ def exported_symbols():
 def test_header_declares_the_reference_entry_points():
     names = declared_functions()
     for ref in ("log_likelihood_full", "fulltemplates", "IMRPhenomDGenerateh22FDAmpPhase", "fourier_nwip",
+                "templates", "log_likelihood_intrinsic",
                 "F_ant", "Times", "fbegin", "CreateAmpPhaseFDWaveform", "DestroyAmpPhaseFDWaveform",
                 "fullphaseamp", "heterodyne", "log_likelihood_het", "Fisher_Het", "FisherEvec"):
         assert ref in names
