@@ -218,6 +218,48 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
     return out
 
 
+def intrinsic_bench(Q, an, ctx, P, D, SN, no_cpu, reps=50):
+    """log_likelihood_intrinsic (QuickCBC.c:4093-4145), the reference-frame
+    likelihood of MCMC_intrinsic, for the same 2048 points mapped to detector
+    0's frame (phase, time and amplitude offsets of the other detectors),
+    through the fused FP64 kernel; host calls (parameters in, logL out)."""
+    from oracle import oracle as O
+    Qp = []
+    for p in P:
+        ciota = p[10]
+        Ap, Ac = (1 + ciota ** 2) / 2, -ciota
+        dt = O.times(p[7], p[8], an.gmst)
+        Fs, lam = [], []
+        for d in an.labels:
+            Fp, Fc = O.f_ant(p[9], p[7], p[8], an.gmst, d)
+            Fs.append(np.sqrt(Ap ** 2 * Fp ** 2 + Ac ** 2 * Fc ** 2))
+            l = np.arctan2(Ac * Fc, Ap * Fp)
+            lam.append(l + 2 * np.pi if l < 0 else l)
+        q = [p[0], p[1], p[2], p[3], 0.5 * (lam[0] + p[4]), p[5] + dt[an.labels[0]], p[6] - np.log(Fs[0])]
+        for d in range(1, an.nifo):
+            q += [lam[d] - lam[0], dt[an.labels[d]] - dt[an.labels[0]], Fs[d] / Fs[0]]
+        Qp.append(q)
+    Qp = np.array(Qp)
+    ctx.loglike_intrinsic(Qp)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L, s2, st = ctx.loglike_intrinsic(Qp)
+    T = (time.perf_counter() - t0) / reps
+    Lf, _, _ = ctx.loglike(P, Q.FP64)
+    loud = (s2 > 25).sum(1) >= 2
+    out = {"points": Qp.shape[0], "logl_per_s": Qp.shape[0] / T, "call_ms": T * 1e3,
+           "loud_fraction": float(loud.mean()),
+           "vs_full_max_abs_dlogl_when_loud": float(np.abs(L - Lf)[loud].max()) if loud.any() else None}
+    if not no_cpu:
+        n = 256
+        t0 = time.perf_counter()
+        O.loglike_intrinsic(an.nifo, an.fref, an.fmin, an.fmax, D, SN, an.N, an.Tobs, Qp[:n])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "logL evals/s", "cores": O.lib().qo_num_threads(),
+                               "kind": "port", "sample": "%d points, %.2f s, OpenMP over points" % (n, dt)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,6 +470,7 @@ def main():
         extra["launch_shape_mixed"] = ctx.get_shape(Q.MIXED)
         extra["het"] = het_bench(Q, W, an, ctx, inj, D, SN, init, args.het_launches, args.cpu_seconds / 3,
                                  args.no_cpu)
+        extra["intrinsic"] = intrinsic_bench(Q, an, ctx, init, D, SN, args.no_cpu)
         log(json.dumps(extra))
 
     cpu = None
