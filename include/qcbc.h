@@ -278,6 +278,15 @@ int qcbc_loglike_intrinsic(qcbc_ctx *ctx, const double *params, int B, double *l
 /* templates() (QuickCBC.c:3527-3611) for B points: hwave B x n_det x N. */
 int qcbc_templates_intrinsic(qcbc_ctx *ctx, const double *params, int B, double *hwave, int *status);
 
+/* Data conditioning (QuickCBC.c:268-318) on the GPU, then binding: each
+ * detector's N time samples x (n_det x N, host memory) windowed by tukey()
+ * (Utilities.c:635-654, alpha = 2 t_rise / Tobs; the reference uses t_rise
+ * = 0.4 s), Fourier transformed (N a power of 2, as its radix-2 FFT), in the
+ * half-complex layout and scaled by sqrt(Tobs)/N, bound with the PSD SN
+ * (n_det x N/2) as qcbc_bind_data.  D (n_det x N, may be NULL) receives the
+ * conditioned data. */
+int qcbc_bind_time_series(qcbc_ctx *ctx, const double *x, const double *SN, double t_rise, double *D);
+
 const char *qcbc_strerror(int code);
 int qcbc_device_info(char *name, size_t len, int *cu_count, double *mem_gb);
 

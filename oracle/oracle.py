@@ -461,3 +461,37 @@ def loglike_intrinsic(nifo, fref, fmin, fmax, D, SN, N, Tobs, params):
     L.qo_loglike_intrinsic_batch_flat(nifo, fref, fmin, fmax, _d(D), _d(SN), N, Tobs, _d(p), B, _d(logL), _d(snr),
                                       _i(st))
     return logL, snr, st
+
+
+def tukey(N, alpha):
+    """tukey() (Utilities.c:635-654): the window values, flat for (1 - alpha) of the N samples."""
+    i = np.arange(N, dtype=np.float64)
+    imin = int(alpha * (N - 1) / 2.0)
+    imax = int((N - 1) * (1.0 - alpha / 2.0))
+    Nwin = N - imax
+    w = np.ones(N)
+    lo = i < imin
+    hi = i > imax
+    w[lo] = 0.5 * (1.0 + np.cos(np.pi * (i[lo] / imin - 1.0)))
+    w[hi] = 0.5 * (1.0 + np.cos(np.pi * ((i[hi] - imax) / Nwin)))
+    return w
+
+
+def halfcomplex(X, N):
+    """GSL's half-complex layout of the N/2 + 1 coefficients of a real FFT."""
+    out = np.zeros(N)
+    h = N // 2
+    out[0] = X[0].real
+    out[h] = X[h].real
+    out[1:h] = X[1:h].real
+    out[N - np.arange(1, h)] = X[1:h].imag
+    return out
+
+
+def condition(x, Tobs, t_rise=0.4):
+    """QuickCBC.c:268-313: tukey(alpha = 2 t_rise / Tobs), the real FFT in the half-complex layout
+    (gsl_fft_real_radix2_transform, forward sign), times sqrt(Tobs) / N; x is n_det x N."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    N = x.shape[1]
+    w = tukey(N, 2.0 * t_rise / Tobs)
+    return np.array([halfcomplex(np.fft.rfft(row * w), N) for row in x]) * (np.sqrt(Tobs) / N)

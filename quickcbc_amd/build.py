@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqcbc.so")
-SOURCES = ["qcbc_parity.hip", "qcbc_kernels.hip", "qcbc_het.hip", "qcbc_fisher.hip", "qcbc_update.hip", "qcbc_het.cpp", "qcbc_api.cpp", "qcbc_dropin.cpp", "qcbc_pt.cpp", "qcbc_multi.cpp", "qcbc_serve.cpp"]
+SOURCES = ["qcbc_parity.hip", "qcbc_kernels.hip", "qcbc_het.hip", "qcbc_fisher.hip", "qcbc_update.hip", "qcbc_condition.hip", "qcbc_het.cpp", "qcbc_api.cpp", "qcbc_dropin.cpp", "qcbc_pt.cpp", "qcbc_multi.cpp", "qcbc_serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -32,6 +32,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-u
 PER_SOURCE = {"qcbc_parity.hip": ["-ffp-contract=off"], "qcbc_het.hip": ["-ffp-contract=off"],
               "qcbc_fisher.hip": ["-ffp-contract=off"],
               "qcbc_update.hip": ["-ffp-contract=off"],
+              "qcbc_condition.hip": ["-ffp-contract=off"],
               "qcbc_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
@@ -54,7 +55,8 @@ def build(verbose=False):
             raise RuntimeError("hipcc failed on %s" % s)
         objs.append(o)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stderr)

@@ -2,6 +2,7 @@
 // drop-in entry points.  Every numerical step runs in the HIP kernels; this
 // file only moves data, sizes launches and maps errors.
 #include <hip/hip_runtime_api.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <chrono>
@@ -16,6 +17,7 @@
 
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
+#include "qcbc_condition_internal.h"
 #include "qcbc_internal.h"
 
 using namespace qcbc;
@@ -206,6 +208,48 @@ int qcbc_bind_rows(qcbc_ctx* c, double* const* D, double* const* SN) {
   if (!c) return QCBC_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
   return bind_impl(c, (const double* const*)D, (const double* const*)SN);
+}
+
+int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double t_rise, double* D) {
+  if (!c || !x || !SN || !(t_rise >= 0.0)) return QCBC_EINVAL;
+  const int nd = c->desc.n_det, N = c->desc.N, nh = c->nhalf;
+  if (N & (N - 1)) return QCBC_EINVAL;  // radix 2, as gsl_fft_real_radix2_transform
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  double* d_x = nullptr;
+  void* d_X = nullptr;
+  HIPCHK(hipMalloc(&d_x, sizeof(double) * nd * N));
+  if (hipMalloc(&d_X, sizeof(double) * 2 * nd * (N / 2 + 1)) != hipSuccess) {
+    hipFree(d_x);
+    return QCBC_ENOMEM;
+  }
+  hipfftHandle plan = 0;
+  hipError_t e = hipMemcpyAsync(d_x, x, sizeof(double) * nd * N, hipMemcpyHostToDevice, c->stream);
+  // QuickCBC.c:268-284: alpha = 2 t_rise / Tobs; D = FFT(tukey(x)) sqrt(Tobs) / N (:309-313)
+  if (e == hipSuccess) e = launch_tukey(d_x, nd, N, 2.0 * t_rise / c->desc.Tobs, c->stream);
+  int fst = hipfftPlan1d(&plan, N, HIPFFT_D2Z, nd);
+  if (fst == HIPFFT_SUCCESS) fst = hipfftSetStream(plan, c->stream);
+  if (fst == HIPFFT_SUCCESS && e == hipSuccess)
+    fst = hipfftExecD2Z(plan, d_x, static_cast<hipfftDoubleComplex*>(d_X));
+  if (fst == HIPFFT_SUCCESS && e == hipSuccess)
+    e = launch_pack_halfcomplex(d_X, nd, N, sqrt(c->desc.Tobs) / (double)N, c->d_D, c->stream);
+  if (e == hipSuccess && D) e = hipMemcpyAsync(D, c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (plan) hipfftDestroy(plan);
+  hipFree(d_x);
+  hipFree(d_X);
+  if (fst != HIPFFT_SUCCESS) return QCBC_EHIP;
+  HIPCHK(e);
+  // then bind as qcbc_bind_data does, with D already on the device
+  std::vector<double> Dh((size_t)nd * N);
+  HIPCHK(hipMemcpy(Dh.data(), c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost));
+  const double* rd[3];
+  const double* rs[3];
+  for (int d = 0; d < nd; d++) {
+    rd[d] = Dh.data() + (size_t)d * N;
+    rs[d] = SN + (size_t)d * nh;
+  }
+  return bind_impl(c, rd, rs);
 }
 
 int qcbc_bind_data(qcbc_ctx* c, const double* D, const double* SN) {

@@ -1,0 +1,14 @@
+// Launch wrappers of the data-conditioning kernels (qcbc_condition.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+namespace qcbc {
+
+// tukey() (Utilities.c:635-654) in place on nd rows of N samples
+hipError_t launch_tukey(double* x, int nd, int N, double alpha, hipStream_t s);
+// nd rows of N/2 + 1 complex coefficients (double2) into GSL's half-complex
+// layout, nd rows of N, times fac
+hipError_t launch_pack_halfcomplex(const void* X, int nd, int N, double fac, double* D, hipStream_t s);
+
+}  // namespace qcbc

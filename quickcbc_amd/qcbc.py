@@ -131,6 +131,7 @@ def lib():
         L.qcbc_het_timing.argtypes = [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_het_time_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
         L.qcbc_het_destroy.argtypes = [_vp]
+        L.qcbc_bind_time_series.argtypes = [_vp, _dp, _dp, ctypes.c_double, _dp]
         # reference-frame likelihood
         L.qcbc_loglike_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_templates_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
@@ -207,6 +208,14 @@ class Context:
         self.h = h
         if D is not None:
             self.bind(D, SN if SN is not None else an.SN())
+
+    def bind_time_series(self, x, SN, t_rise=0.4):
+        """Tukey window, FFT and scaling on the GPU, then bind: returns the conditioned D (n_det x N)."""
+        x = np.ascontiguousarray(x, np.float64)
+        SN = np.ascontiguousarray(SN, np.float64)
+        D = np.zeros((self.an.nifo, self.an.N))
+        _check(lib().qcbc_bind_time_series(self.h, _d(x), _d(SN), t_rise, _d(D)), "qcbc_bind_time_series")
+        return D
 
     def loglike_intrinsic(self, params):
         """log_likelihood_intrinsic for a batch of templates' parameters: (logL, snrsq B x n_det, status)."""
