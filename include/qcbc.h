@@ -287,6 +287,18 @@ int qcbc_templates_intrinsic(qcbc_ctx *ctx, const double *params, int B, double 
  * conditioned data. */
 int qcbc_bind_time_series(qcbc_ctx *ctx, const double *x, const double *SN, double t_rise, double *D);
 
+/* makespec (Utilities.c:1908-1959) on the GPU for the analysis's N and Tobs:
+ * SN (N/2) = exp(natural cubic spline of ln S through (ffit, Sspline)) times
+ * (1 + the Lorentzian lines at linef with heights lineh, quality lineQ,
+ * half-widths linew and taper scales deltafmax), SN[0] and SN[N/2-1] the end
+ * knots' values.  The knots must rise and cover bins 1 .. N/2-2 (the
+ * reference's GSL spline aborts outside them); QCBC_EINVAL otherwise.
+ * Rebind with qcbc_bind_data to use the draw (noise marginalisation,
+ * QuickCBC.c:1994-2010). */
+int qcbc_makespec(qcbc_ctx *ctx, int Nspline, int Nlines, const double *ffit, const double *Sspline,
+                  const double *linef, const double *lineh, const double *lineQ, const double *linew,
+                  const double *deltafmax, double *SN);
+
 const char *qcbc_strerror(int code);
 int qcbc_device_info(char *name, size_t len, int *cu_count, double *mem_gb);
 

@@ -495,3 +495,17 @@ def condition(x, Tobs, t_rise=0.4):
     N = x.shape[1]
     w = tukey(N, 2.0 * t_rise / Tobs)
     return np.array([halfcomplex(np.fft.rfft(row * w), N) for row in x]) * (np.sqrt(Tobs) / N)
+
+
+def makespec(ffit, Sspline, lines, Tobs, N):
+    """makespec (Utilities.c:1908): (status, SN of N/2); lines = (linef, lineh, lineQ, linew, deltafmax) or None."""
+    L = lib()
+    L.qo_makespec.restype = ctypes.c_int
+    L.qo_makespec.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 8 + [ctypes.c_double, ctypes.c_int]
+    x = np.ascontiguousarray(ffit, np.float64)
+    y = np.ascontiguousarray(Sspline, np.float64)
+    Ls = [np.ascontiguousarray(v, np.float64) for v in lines] if lines is not None else [np.zeros(1)] * 5
+    nl = Ls[0].size if lines is not None else 0
+    SN = np.zeros(N // 2)
+    st = L.qo_makespec(x.size, nl, _d(x), _d(y), *[_d(v) for v in Ls], _d(SN), Tobs, N)
+    return st, SN

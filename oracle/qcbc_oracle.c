@@ -2295,3 +2295,52 @@ void qo_loglike_intrinsic_batch_flat(int nifo, double fref, double fmin, double 
   }
   free(freq);
 }
+
+/* makespec, Utilities.c:1908-1959: SN (N/2) from a natural cubic spline of
+ * ln S through (ffit, Sspline) (GSL's cspline: the Thomas solve above, its
+ * evaluation form) and the Lorentzian lines.  Returns QO_EDOM when the knots
+ * do not cover bins 1 .. N/2-2 (GSL's spline aborts there). */
+int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspline, const double *linef,
+                const double *lineh, const double *lineQ, const double *linew, const double *deltafmax, double *SN,
+                double Tobs, int N) {
+  double *c = malloc(sizeof(double) * Nspline);
+  int i, j;
+  if (N / 2 > 3 && (ffit[0] > 1.0 / Tobs || ffit[Nspline - 1] < (double)(N / 2 - 2) / Tobs)) {
+    free(c);
+    return QO_EDOM;
+  }
+  qo_spline_solve(ffit, Sspline, Nspline, c);
+  SN[0] = exp(Sspline[0]);
+  SN[N / 2 - 1] = exp(Sspline[Nspline - 1]);
+  for (i = 1; i < N / 2 - 1; ++i) {
+    double f = (double)(i) / Tobs, y = 0.0, sm;
+    int lo = 0, hi = Nspline - 1;
+    while (hi > lo + 1) {
+      int mid = (lo + hi) / 2;
+      if (ffit[mid] > f)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    {
+      double h = ffit[lo + 1] - ffit[lo], dy = Sspline[lo + 1] - Sspline[lo], dx = f - ffit[lo];
+      double b = dy / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0, d = (c[lo + 1] - c[lo]) / (3.0 * h);
+      sm = exp(Sspline[lo] + dx * (b + dx * (c[lo] + dx * d)));
+    }
+    for (j = 0; j < Nlines; ++j) {
+      double x = fabs(f - linef[j]);
+      if (x < linew[j]) {
+        double z = 1.0, f2, ff2, df2, df4;
+        if (x > deltafmax[j]) z = exp(-(x - deltafmax[j]) / deltafmax[j]);
+        f2 = linef[j] * linef[j];
+        ff2 = f * f;
+        df2 = lineQ[j] * (1.0 - ff2 / f2);
+        df4 = df2 * df2;
+        y += z * lineh[j] / (ff2 / f2 + df4);
+      }
+    }
+    SN[i] = sm * (1.0 + y);
+  }
+  free(c);
+  return QO_OK;
+}

@@ -165,6 +165,11 @@ void qo_loglike_intrinsic_batch_flat(int nifo, double fref, double fmin, double 
                                      const double *SN, int N, double Tobs, const double *params, int B,
                                      double *logL, double *snrsq, int *status);
 
+/* makespec (Utilities.c:1908-1959) */
+int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspline, const double *linef,
+                const double *lineh, const double *lineQ, const double *linew, const double *deltafmax, double *SN,
+                double Tobs, int N);
+
 #ifdef __cplusplus
 }
 #endif

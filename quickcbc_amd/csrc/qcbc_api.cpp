@@ -252,6 +252,39 @@ int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double
   return bind_impl(c, rd, rs);
 }
 
+int qcbc_makespec(qcbc_ctx* c, int Nspline, int Nlines, const double* ffit, const double* Sspline,
+                  const double* linef, const double* lineh, const double* lineQ, const double* linew,
+                  const double* deltafmax, double* SN) {
+  if (!c || Nspline < 2 || Nlines < 0 || !ffit || !Sspline || !SN ||
+      (Nlines > 0 && (!linef || !lineh || !lineQ || !linew || !deltafmax)))
+    return QCBC_EINVAL;
+  const int N = c->desc.N, h = N / 2;
+  const double Tobs = c->desc.Tobs;
+  for (int i = 1; i < Nspline; i++)
+    if (!(ffit[i] > ffit[i - 1])) return QCBC_EINVAL;
+  // gsl_spline_eval is defined on [ffit[0], ffit[n-1]] only (the reference
+  // aborts outside it); bins 1 .. N/2 - 2 are evaluated
+  if (h > 3 && (ffit[0] > 1.0 / Tobs || ffit[Nspline - 1] < (double)(h - 2) / Tobs)) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  const size_t nl = (size_t)Nlines, ns = (size_t)Nspline;
+  const size_t total = 2 * ns + 5 * nl + ns + 3 * ns + (size_t)h;
+  double* buf = nullptr;
+  HIPCHK(hipMalloc(&buf, sizeof(double) * std::max<size_t>(total, 1)));
+  double *xa = buf, *ya = xa + ns, *lines = ya + ns, *cc = lines + 5 * nl, *w = cc + ns, *out = w + 3 * ns;
+  hipError_t e = hipMemcpyAsync(xa, ffit, sizeof(double) * ns, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ya, Sspline, sizeof(double) * ns, hipMemcpyHostToDevice, c->stream);
+  const double* lrows[5] = {linef, lineh, lineQ, linew, deltafmax};
+  for (int r = 0; r < 5 && e == hipSuccess && nl > 0; r++)
+    e = hipMemcpyAsync(lines + r * nl, lrows[r], sizeof(double) * nl, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_makespec(xa, ya, Nspline, lines, Nlines, Tobs, N, cc, w, out, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(SN, out, sizeof(double) * h, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(buf);
+  HIPCHK(e);
+  return QCBC_OK;
+}
+
 int qcbc_bind_data(qcbc_ctx* c, const double* D, const double* SN) {
   if (!c || !D || !SN) return QCBC_EINVAL;
   const double* rd[3];

@@ -132,6 +132,7 @@ def lib():
         L.qcbc_het_time_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
         L.qcbc_het_destroy.argtypes = [_vp]
         L.qcbc_bind_time_series.argtypes = [_vp, _dp, _dp, ctypes.c_double, _dp]
+        L.qcbc_makespec.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_dp] * 8
         # reference-frame likelihood
         L.qcbc_loglike_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_templates_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
@@ -216,6 +217,16 @@ class Context:
         D = np.zeros((self.an.nifo, self.an.N))
         _check(lib().qcbc_bind_time_series(self.h, _d(x), _d(SN), t_rise, _d(D)), "qcbc_bind_time_series")
         return D
+
+    def makespec(self, ffit, Sspline, lines=None):
+        """makespec on the GPU: SN (N/2); lines = (linef, lineh, lineQ, linew, deltafmax) arrays or None."""
+        x = np.ascontiguousarray(ffit, np.float64)
+        y = np.ascontiguousarray(Sspline, np.float64)
+        L = [np.ascontiguousarray(v, np.float64) for v in lines] if lines is not None else [np.zeros(1)] * 5
+        nl = L[0].size if lines is not None else 0
+        SN = np.zeros(self.an.N // 2)
+        _check(lib().qcbc_makespec(self.h, x.size, nl, _d(x), _d(y), *[_d(v) for v in L], _d(SN)), "qcbc_makespec")
+        return SN
 
     def loglike_intrinsic(self, params):
         """log_likelihood_intrinsic for a batch of templates' parameters: (logL, snrsq B x n_det, status)."""
