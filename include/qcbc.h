@@ -284,7 +284,8 @@ int qcbc_templates_intrinsic(qcbc_ctx *ctx, const double *params, int B, double 
  * = 0.4 s), Fourier transformed (N a power of 2, as its radix-2 FFT), in the
  * half-complex layout and scaled by sqrt(Tobs)/N, bound with the PSD SN
  * (n_det x N/2) as qcbc_bind_data.  D (n_det x N, may be NULL) receives the
- * conditioned data. */
+ * conditioned data.  Allocates its device buffers and FFT plan per call (a
+ * once-per-analysis step). */
 int qcbc_bind_time_series(qcbc_ctx *ctx, const double *x, const double *SN, double t_rise, double *D);
 
 /* makespec (Utilities.c:1908-1959) on the GPU for the analysis's N and Tobs:
@@ -294,7 +295,7 @@ int qcbc_bind_time_series(qcbc_ctx *ctx, const double *x, const double *SN, doub
  * knots' values.  The knots must rise and cover bins 1 .. N/2-2 (the
  * reference's GSL spline aborts outside them); QCBC_EINVAL otherwise.
  * Rebind with qcbc_bind_data to use the draw (noise marginalisation,
- * QuickCBC.c:1994-2010). */
+ * QuickCBC.c:1994-2010).  Allocates its device buffers per call. */
 int qcbc_makespec(qcbc_ctx *ctx, int Nspline, int Nlines, const double *ffit, const double *Sspline,
                   const double *linef, const double *lineh, const double *lineQ, const double *linew,
                   const double *deltafmax, double *SN);
@@ -504,7 +505,8 @@ int qcbc_het_fisher_time(qcbc_het *het, const double *d_params, int B, int reps,
  * host memory, row-major) on the analysis's GPU: ej B x d, ev B x d x d
  * (row i = eigenvector i), ordered by |eigenvalue| ascending; cyclic Jacobi
  * in place of GSL's eigen_symmv, each eigenvector signed so that its
- * largest-magnitude component is positive. */
+ * largest-magnitude component is positive.  Allocates its device buffers per
+ * call (qcbc_het_fisher reuses the heterodyne's). */
 int qcbc_fisher_evec(qcbc_ctx *ctx, const double *fish, int B, int d, double *ej, double *ev);
 int qcbc_het_destroy(qcbc_het *het);
 
