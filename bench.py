@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--draws", type=int, default=100000)
     ap.add_argument("--het-launches", type=int, default=500)
     ap.add_argument("--only-het", action="store_true", help="time the heterodyned likelihood only (for profiling)")
+    ap.add_argument("--only-aux", action="store_true",
+                    help="time the reference-frame likelihood, data conditioning and makespec only (for profiling)")
     ap.add_argument("--only-update", action="store_true",
                     help="time QuickCBC's own PT step on the heterodyned likelihood only (for profiling)")
     args = ap.parse_args()
@@ -324,6 +326,30 @@ def main():
     sigma = np.array([5e-6, 2.5e-5, 5e-4, 5e-4, 0.0125, 5e-6, 0.0025, 0.0025, 0.0025, 0.005, 0.0025])
     nb = an.imax - an.imin
     plane_bytes = (4 + 3 * an.nifo) * nb * 8    # f, f^(1/3), f^(-1/6), log f; Re d/S, Im d/S, 1/S per detector
+
+    if args.only_aux:
+        r = {"intrinsic": intrinsic_bench(Q, an, ctx, init, D, SN, True)}
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal((an.nifo, an.N)) * 1e-21
+        c2 = Q.Context(an)
+        c2.bind_time_series(x, SN)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            c2.bind_time_series(x, SN)
+        r["bind_time_series_ms"] = (time.perf_counter() - t0) / 10 * 1e3
+        fk = np.concatenate([[0.5 / an.Tobs], np.linspace(5.0, an.N / 2 / an.Tobs - 1.0, 40),
+                             [an.N / 2 / an.Tobs]])
+        yk = np.log(W.psd(np.maximum(fk, 10.0)))
+        lines = (np.array([60.0, 331.5, 1000.0]), np.array([50.0, 8.0, 3.0]), np.array([1e3, 4e2, 2e2]),
+                 np.array([2.0, 1.5, 3.0]), np.array([0.3, 0.2, 0.5]))
+        c2.makespec(fk, yk, lines)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            c2.makespec(fk, yk, lines)
+        r["makespec_ms"] = (time.perf_counter() - t0) / 20 * 1e3
+        c2.close()
+        print(json.dumps({"config": "reference-frame likelihood, data conditioning, makespec", **r}), flush=True)
+        return
 
     if args.only_update:
         A, _ = Q.dropin_fullphaseamp(an, an.freq(), inj)
