@@ -40,6 +40,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def warm_clocks(pt, seconds):
+    """Untimed PT steps for `seconds` of wall time.  After even half a second
+    of idle the GPU steps ~20% slower for its first tens of milliseconds of
+    work (its clocks ramp), so the W warm-up steps alone leave a 20-step timed
+    region below the steady rate."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        pt.run(50)
+
+
 def percentile(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
@@ -170,7 +180,7 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
     t0 = time.perf_counter()
     pt.run(steps)
     T = time.perf_counter() - t0
-    pt.run(64, graph=True)
+    pt.run(steps, graph=True)  # captures the graphs of this step count
     t0 = time.perf_counter()
     pt.run(steps, graph=True)
     Tg = time.perf_counter() - t0
@@ -265,6 +275,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warm-seconds", type=float, default=0.5,
+                    help="untimed PT steps before the warm-up steps, so the GPU leaves its idle clock state")
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--kernel-steps", type=int, default=500, help="PT steps of the kernel-alone timing")
     ap.add_argument("--plain-points", type=int, default=256)
@@ -387,6 +399,7 @@ def main():
     # ---- headline: FP64 fused path, PT steps (the library's column groups,
     # overlapping on their own streams), wall clock over the timed steps
     pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=args.pt_groups)
+    warm_clocks(pt, args.warm_seconds)
     pt.run(args.warmup)
     ctx.sync()
     t0 = time.perf_counter()
@@ -417,8 +430,8 @@ def main():
     achieved_tf = flops / (kernel_ms * 1e-3) / 1e12
     extra = {}
     if not args.only_main:
-        # graph-replayed steps
-        pt.run(2, graph=True)
+        # graph-replayed steps (the same step count once untimed: its graphs are captured then)
+        pt.run(args.steps, graph=True)
         ctx.sync()
         t0 = time.perf_counter()
         pt.run(args.steps, graph=True)
@@ -437,7 +450,7 @@ def main():
         extra["fp64_graph_step_ms_p50"] = percentile(lat, 50)
         extra["fp64_graph_step_ms_p99"] = percentile(lat, 99)
         x, Lx, acc, sw = pt.get()
-        extra["pt_accept_rate"] = float(acc.sum()) / (B * (args.warmup + 2 * args.steps + 2 + 2 * args.latency_steps))
+        extra["pt_accept_rate"] = float(acc.sum()) / (B * pt.steps_run)
         extra["pt_groups"] = pt.groups
         pt.close()
         # latency-first configuration: one column group (three launches per
@@ -453,6 +466,7 @@ def main():
         extra["fp64_one_group_step_ms_p99"] = percentile(lat, 99)
         # mixed precision on the same shape
         ptm = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.MIXED, groups=args.pt_groups)
+        warm_clocks(ptm, 0.2)
         ptm.run(args.warmup)
         ctx.sync()
         t0 = time.perf_counter()
@@ -491,6 +505,7 @@ def main():
         extra["plain_fp64_logl_per_s"] = npl / Tp
         extra["plain_vs_fused_max_abs_dlogl"] = float(np.abs(Lp - L64[:npl]).max())
         extra["mixed_over_plain"] = extra["mixed_logl_per_s"] / extra["plain_fp64_logl_per_s"]
+        extra["mixed_over_fp64_fused"] = extra["mixed_logl_per_s"] / value
         extra["fp64_fused_over_plain"] = value / extra["plain_fp64_logl_per_s"]
         extra["launch_shape_fp64"] = ctx.get_shape(Q.FP64)
         extra["launch_shape_mixed"] = ctx.get_shape(Q.MIXED)
@@ -523,10 +538,13 @@ def main():
         "config": {"workload": "batched PT ensemble, Tobs=16 s, fs=4096 Hz, 3 det (H/L/V), band 20-2048 Hz "
                                "(32448 bins), 32 temps x 64 chains = 2048 points/step, FP64 fused launch",
                    "points_per_step": B, "bins": nb, "detectors": an.nifo},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TF, "traffic": K_LL_FP64_TRAFFIC_BYTES,
-                     "ceiling": "FP64 arithmetic: the kernel runs on the FP64 vector ALU, no MFMA; MI355X's "
-                                "FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
+                     "traffic_source": "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
+                                       "workload, scaled by a 1 GiB copy (profiles/k_ll_fp64_pt_step_hbm_traffic.md);"
+                                       " not measured in this run",
+                     "ceiling": "FP64 vector ALU issue: the kernel has no MFMA; MI355X's FP64 vector peak "
+                                "(78.6 TFLOP/s, an FMA counted as 2)",
                      "kernel": "k_ll<3,FP64,%d>" % tile, "kernel_ms": kernel_ms, "points_per_block": cpb,
                      "flops_per_launch": flops, "bin_points_per_launch": bin_points,
                      "flops_per_bin_point": flops / max(bin_points, 1)},

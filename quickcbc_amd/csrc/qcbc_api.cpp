@@ -169,8 +169,10 @@ int qcbc_ctx_destroy(qcbc_ctx* c) {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
   }
+  if (c->cond_plan) hipfftDestroy(static_cast<hipfftHandle>(c->cond_plan));
   void* bufs[] = {c->d_p64, c->d_p32, c->d_D, c->d_SN, c->d_wref, c->d_params, c->d_pts, c->d_status,
-                  c->d_logL, c->d_rho, c->d_partial, c->d_amp, c->d_phase, c->d_hw, c->d_plain_partial};
+                  c->d_logL, c->d_rho, c->d_partial, c->d_amp, c->d_phase, c->d_hw, c->d_plain_partial,
+                  c->d_cond_x, c->d_cond_X, c->d_spec};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -178,15 +180,13 @@ int qcbc_ctx_destroy(qcbc_ctx* c) {
   return QCBC_OK;
 }
 
-static int bind_impl(qcbc_ctx* c, const double* const* D, const double* const* SN) {
-  if (!c || !D || !SN) return QCBC_EINVAL;
-  hipSetDevice(c->device);
+// The band planes from the data already in c->d_D and the PSD rows SN (host
+// memory): SN is copied to the device, the FP32 scales chosen, the planes
+// built.
+static int bind_finish(qcbc_ctx* c, const double* const* SN) {
   const int nd = c->desc.n_det, N = c->desc.N, nh = c->nhalf;
-  for (int d = 0; d < nd; d++) {
-    if (!D[d] || !SN[d]) return QCBC_EINVAL;
-    HIPCHK(hipMemcpyAsync(c->d_D + (size_t)d * N, D[d], sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  for (int d = 0; d < nd; d++)
     HIPCHK(hipMemcpyAsync(c->d_SN + (size_t)d * nh, SN[d], sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
-  }
   // per-detector FP32 scale: 1/S at the middle of the band
   double wref[3] = {1.0, 1.0, 1.0};
   int kmid = (c->an.imin + c->an.imax) / 2;
@@ -204,6 +204,18 @@ static int bind_impl(qcbc_ctx* c, const double* const* D, const double* const* S
   return QCBC_OK;
 }
 
+static int bind_impl(qcbc_ctx* c, const double* const* D, const double* const* SN) {
+  if (!c || !D || !SN) return QCBC_EINVAL;
+  hipSetDevice(c->device);
+  const int nd = c->desc.n_det, N = c->desc.N;
+  for (int d = 0; d < nd; d++)
+    if (!D[d] || !SN[d]) return QCBC_EINVAL;
+  c->bound = false;
+  for (int d = 0; d < nd; d++)
+    HIPCHK(hipMemcpyAsync(c->d_D + (size_t)d * N, D[d], sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  return bind_finish(c, SN);
+}
+
 int qcbc_bind_rows(qcbc_ctx* c, double* const* D, double* const* SN) {
   if (!c) return QCBC_EINVAL;
   std::lock_guard<std::mutex> lk(c->mu);
@@ -216,40 +228,28 @@ int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double
   if (N & (N - 1)) return QCBC_EINVAL;  // radix 2, as gsl_fft_real_radix2_transform
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  double* d_x = nullptr;
-  void* d_X = nullptr;
-  HIPCHK(hipMalloc(&d_x, sizeof(double) * nd * N));
-  if (hipMalloc(&d_X, sizeof(double) * 2 * nd * (N / 2 + 1)) != hipSuccess) {
-    hipFree(d_x);
-    return QCBC_ENOMEM;
+  // scratch and plan are made on the first call and kept with the analysis
+  if (!c->d_cond_x) HIPCHK(hipMalloc(&c->d_cond_x, sizeof(double) * nd * N));
+  if (!c->d_cond_X) HIPCHK(hipMalloc(&c->d_cond_X, sizeof(double) * 2 * nd * (N / 2 + 1)));
+  if (!c->cond_plan) {
+    hipfftHandle plan = nullptr;
+    if (hipfftPlan1d(&plan, N, HIPFFT_D2Z, nd) != HIPFFT_SUCCESS) return QCBC_EHIP;
+    c->cond_plan = plan;
   }
-  hipfftHandle plan = 0;
-  hipError_t e = hipMemcpyAsync(d_x, x, sizeof(double) * nd * N, hipMemcpyHostToDevice, c->stream);
+  hipfftHandle plan = static_cast<hipfftHandle>(c->cond_plan);
+  if (hipfftSetStream(plan, c->stream) != HIPFFT_SUCCESS) return QCBC_EHIP;
+  c->bound = false;
+  HIPCHK(hipMemcpyAsync(c->d_cond_x, x, sizeof(double) * nd * N, hipMemcpyHostToDevice, c->stream));
   // QuickCBC.c:268-284: alpha = 2 t_rise / Tobs; D = FFT(tukey(x)) sqrt(Tobs) / N (:309-313)
-  if (e == hipSuccess) e = launch_tukey(d_x, nd, N, 2.0 * t_rise / c->desc.Tobs, c->stream);
-  int fst = hipfftPlan1d(&plan, N, HIPFFT_D2Z, nd);
-  if (fst == HIPFFT_SUCCESS) fst = hipfftSetStream(plan, c->stream);
-  if (fst == HIPFFT_SUCCESS && e == hipSuccess)
-    fst = hipfftExecD2Z(plan, d_x, static_cast<hipfftDoubleComplex*>(d_X));
-  if (fst == HIPFFT_SUCCESS && e == hipSuccess)
-    e = launch_pack_halfcomplex(d_X, nd, N, sqrt(c->desc.Tobs) / (double)N, c->d_D, c->stream);
-  if (e == hipSuccess && D) e = hipMemcpyAsync(D, c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (plan) hipfftDestroy(plan);
-  hipFree(d_x);
-  hipFree(d_X);
-  if (fst != HIPFFT_SUCCESS) return QCBC_EHIP;
-  HIPCHK(e);
+  HIPCHK(launch_tukey(c->d_cond_x, nd, N, 2.0 * t_rise / c->desc.Tobs, c->stream));
+  if (hipfftExecD2Z(plan, c->d_cond_x, static_cast<hipfftDoubleComplex*>(c->d_cond_X)) != HIPFFT_SUCCESS)
+    return QCBC_EHIP;
+  HIPCHK(launch_pack_halfcomplex(c->d_cond_X, nd, N, sqrt(c->desc.Tobs) / (double)N, c->d_D, c->stream));
+  if (D) HIPCHK(hipMemcpyAsync(D, c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost, c->stream));
   // then bind as qcbc_bind_data does, with D already on the device
-  std::vector<double> Dh((size_t)nd * N);
-  HIPCHK(hipMemcpy(Dh.data(), c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost));
-  const double* rd[3];
   const double* rs[3];
-  for (int d = 0; d < nd; d++) {
-    rd[d] = Dh.data() + (size_t)d * N;
-    rs[d] = SN + (size_t)d * nh;
-  }
-  return bind_impl(c, rd, rs);
+  for (int d = 0; d < nd; d++) rs[d] = SN + (size_t)d * nh;
+  return bind_finish(c, rs);
 }
 
 int qcbc_makespec(qcbc_ctx* c, int Nspline, int Nlines, const double* ffit, const double* Sspline,
@@ -268,9 +268,15 @@ int qcbc_makespec(qcbc_ctx* c, int Nspline, int Nlines, const double* ffit, cons
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   const size_t nl = (size_t)Nlines, ns = (size_t)Nspline;
-  const size_t total = 2 * ns + 5 * nl + ns + 3 * ns + (size_t)h;
-  double* buf = nullptr;
-  HIPCHK(hipMalloc(&buf, sizeof(double) * std::max<size_t>(total, 1)));
+  const size_t total = std::max<size_t>(2 * ns + 5 * nl + ns + 3 * ns + (size_t)h, 1);
+  if (total > c->spec_cap) {  // grown on demand and kept with the analysis
+    if (c->d_spec) hipFree(c->d_spec);
+    c->d_spec = nullptr;
+    c->spec_cap = 0;
+    HIPCHK(hipMalloc(&c->d_spec, sizeof(double) * total));
+    c->spec_cap = total;
+  }
+  double* buf = c->d_spec;
   double *xa = buf, *ya = xa + ns, *lines = ya + ns, *cc = lines + 5 * nl, *w = cc + ns, *out = w + 3 * ns;
   hipError_t e = hipMemcpyAsync(xa, ffit, sizeof(double) * ns, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(ya, Sspline, sizeof(double) * ns, hipMemcpyHostToDevice, c->stream);
@@ -280,7 +286,6 @@ int qcbc_makespec(qcbc_ctx* c, int Nspline, int Nlines, const double* ffit, cons
   if (e == hipSuccess) e = launch_makespec(xa, ya, Nspline, lines, Nlines, Tobs, N, cc, w, out, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(SN, out, sizeof(double) * h, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(buf);
   HIPCHK(e);
   return QCBC_OK;
 }
@@ -452,6 +457,21 @@ int qcbc_loglike_plain(qcbc_ctx* c, const double* params, int B, double* logL, d
 }
 
 // ------------------------------------------------- waveform / templates
+namespace {
+// Device temporaries of one call, freed on every return path.
+struct DevTemp {
+  void* p = nullptr;
+  DevTemp() = default;
+  DevTemp(const DevTemp&) = delete;
+  DevTemp& operator=(const DevTemp&) = delete;
+  ~DevTemp() {
+    if (p) hipFree(p);
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+}  // namespace
+
 // ips = 0: the 11 parameters of fulltemplates; ips > 0: templates' ips
 // reference-frame parameters per point
 static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, double* phase, double* hwave,
@@ -464,10 +484,11 @@ static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, dou
   const int chunk = std::max(1, std::min(B, (int)((size_t)1 << 28) / std::max(1, (nd + 2) * N)));
   int st = ctx_reserve(c, B);
   if (st) return st;
-  double *d_amp = nullptr, *d_ph = nullptr, *d_hw = nullptr;
-  HIPCHK(hipMalloc(&d_amp, sizeof(double) * nh * chunk));
-  HIPCHK(hipMalloc(&d_ph, sizeof(double) * nh * chunk));
-  if (hwave) HIPCHK(hipMalloc(&d_hw, sizeof(double) * nd * N * chunk));
+  DevTemp t_amp, t_ph, t_hw;
+  HIPCHK(hipMalloc(&t_amp.p, sizeof(double) * nh * chunk));
+  HIPCHK(hipMalloc(&t_ph.p, sizeof(double) * nh * chunk));
+  if (hwave) HIPCHK(hipMalloc(&t_hw.p, sizeof(double) * nd * N * chunk));
+  double *d_amp = t_amp.as<double>(), *d_ph = t_ph.as<double>(), *d_hw = t_hw.as<double>();
   HIPCHK(hipMemcpyAsync(c->d_params, params, sizeof(double) * (ips ? ips : 11) * B, hipMemcpyHostToDevice,
                         c->stream));
   if (ips)
@@ -488,9 +509,6 @@ static int wave_batch(qcbc_ctx* c, const double* params, int B, double* amp, dou
   }
   if (status) HIPCHK(hipMemcpyAsync(status, c->d_status, sizeof(int) * B, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(d_amp);
-  hipFree(d_ph);
-  if (d_hw) hipFree(d_hw);
   return QCBC_OK;
 }
 
@@ -511,8 +529,8 @@ int qcbc_templates_intrinsic(qcbc_ctx* c, const double* params, int B, double* h
 int qcbc_loglike_intrinsic(qcbc_ctx* c, const double* params, int B, double* logL, double* snrsq, int* status) {
   if (!c || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
   if (B == 0) return QCBC_OK;
-  if (!c->bound) return QCBC_ENOTBOUND;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->bound) return QCBC_ENOTBOUND;
   hipSetDevice(c->device);
   int st = ctx_reserve(c, B);
   if (st) return st;

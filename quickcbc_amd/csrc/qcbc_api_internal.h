@@ -42,6 +42,14 @@ struct qcbc_ctx {
   double* d_phase = nullptr;
   double* d_hw = nullptr;
   double* d_plain_partial = nullptr;
+  // data-conditioning scratch, kept for the next call of the same shape:
+  // time series (n_det x N), its transform (n_det x (N/2 + 1) complex) and
+  // the real-to-complex plan; makespec's knots, lines and output
+  double* d_cond_x = nullptr;
+  void* d_cond_X = nullptr;
+  void* cond_plan = nullptr;  // hipfftHandle
+  double* d_spec = nullptr;
+  size_t spec_cap = 0;  // doubles
   // fused-kernel launch shape per precision
   int cpb[2] = {8, 8}, tile[2] = {512, 512}, splits[2] = {0, 0};
   // kernel timing

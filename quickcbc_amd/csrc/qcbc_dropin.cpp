@@ -98,8 +98,9 @@ bool same_desc(const qcbc_desc& a, const qcbc_desc& b) {
 }
 
 // Find or create the thread's analysis for this shape; bind D/SN if given and
-// not already bound (same row pointers, same generation).
-qcbc_ctx* get_ctx(const qcbc_desc& desc, double** D, double** SN, int* status) {
+// not already bound (same row pointers, same generation), or always when
+// rebind is set.
+qcbc_ctx* get_ctx(const qcbc_desc& desc, double** D, double** SN, int* status, bool rebind = false) {
   DropState& s = t_drop;
   *status = drop_init();
   if (*status) return nullptr;
@@ -126,7 +127,7 @@ qcbc_ctx* get_ctx(const qcbc_desc& desc, double** D, double** SN, int* status) {
   }
   hit->last_use = ++s.tick;
   if (D && SN) {
-    bool same = hit->gen == g_generation.load() && (int)hit->drows.size() == desc.n_det;
+    bool same = !rebind && hit->gen == g_generation.load() && (int)hit->drows.size() == desc.n_det;
     for (int d = 0; same && d < desc.n_det; d++) same = hit->drows[d] == D[d] && hit->srows[d] == SN[d];
     if (!same) {
       *status = qcbc_bind_rows(hit->ctx, D, SN);
@@ -137,6 +138,44 @@ qcbc_ctx* get_ctx(const qcbc_desc& desc, double** D, double** SN, int* status) {
     }
   }
   return hit->ctx;
+}
+
+// The kernels evaluate the analysis grid f_i = i / Tobs (freq[0] = 1/Tobs),
+// the grid QuickCBC builds (QuickCBC.c:417-419, :635-636).  A freq that is
+// not that grid, to a few ulps, is rejected rather than ignored.  A list
+// once checked is remembered by its pointer, length and Tobs until
+// qcbc_invalidate().  NULL stands for the grid itself.
+struct GridSeen {
+  const double* data = nullptr;
+  size_t length = 0;
+  int nhalf = 0;
+  double Tobs = 0.0;
+  unsigned long long gen = 0;
+};
+thread_local GridSeen t_grid_seen[4];
+thread_local unsigned t_grid_next = 0;
+
+bool grid_ok(const RealVector* freq, int nhalf, double Tobs) {
+  if (!freq) return true;
+  if (!freq->data || freq->length < (size_t)nhalf || !(Tobs > 0)) return false;
+  const unsigned long long gen = g_generation.load();
+  for (const auto& g : t_grid_seen)
+    if (g.data == freq->data && g.length == freq->length && g.nhalf == nhalf && g.Tobs == Tobs && g.gen == gen)
+      return true;
+  const double* f = freq->data;
+  const double tol = 1e-15;  // about 4.5 ulp
+  if (nhalf > 0 && !(std::fabs(f[0] - 1.0 / Tobs) <= tol / Tobs)) return false;
+  for (int i = 1; i < nhalf; i++) {
+    const double want = (double)i / Tobs;
+    if (!(std::fabs(f[i] - want) <= tol * want)) return false;
+  }
+  GridSeen& g = t_grid_seen[t_grid_next++ & 3];
+  g.data = freq->data;
+  g.length = freq->length;
+  g.nhalf = nhalf;
+  g.Tobs = Tobs;
+  g.gen = gen;
+  return true;
 }
 
 qcbc_desc desc_from(const struct Net* net, int N, double Tobs) {
@@ -166,8 +205,8 @@ void qcbc_invalidate(void) { g_generation++; }
 
 double log_likelihood_full(struct Net* net, double** D, double* params, RealVector* freq, double** SN, double* rho,
                            int N, double Tobs) {
-  (void)freq;  // the grid is f_i = i/Tobs, freq[0] = 1/Tobs (QuickCBC.c:418-419)
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params) fail("log_likelihood_full", QCBC_EINVAL);
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params || !grid_ok(freq, N / 2, Tobs))
+    fail("log_likelihood_full", QCBC_EINVAL);
   qcbc_desc desc = desc_from(net, N, Tobs);
   int st;
   qcbc_ctx* ctx = get_ctx(desc, D, SN, &st);
@@ -183,7 +222,8 @@ double log_likelihood_full(struct Net* net, double** D, double* params, RealVect
 }
 
 void fulltemplates(struct Net* net, double** hwave, RealVector* freq, double* params, int N) {
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !hwave || !freq || !freq->data || !params)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !hwave || !freq || !freq->data || freq->length < 2 || !params ||
+      !grid_ok(freq, N / 2, 1.0 / freq->data[1]))
     fail("fulltemplates", QCBC_EINVAL);
   qcbc_desc desc = desc_from(net, N, 1.0 / freq->data[1]);  // QuickCBC.c:3702
   int st;
@@ -198,7 +238,8 @@ void fulltemplates(struct Net* net, double** hwave, RealVector* freq, double* pa
 }
 
 void templates(struct Net* net, double** hwave, RealVector* freq, double* params, int N) {
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !hwave || !freq || !freq->data || !params)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !hwave || !freq || !freq->data || freq->length < 2 || !params ||
+      !grid_ok(freq, N / 2, 1.0 / freq->data[1]))
     fail("templates", QCBC_EINVAL);
   qcbc_desc desc = desc_from(net, N, 1.0 / freq->data[1]);  // QuickCBC.c:3539
   int st;
@@ -214,8 +255,8 @@ void templates(struct Net* net, double** hwave, RealVector* freq, double* params
 
 double log_likelihood_intrinsic(struct Net* net, double** D, double* params, RealVector* freq, double** SN, int N,
                                 double Tobs) {
-  (void)freq;  // the grid is f_i = i/Tobs, freq[0] = 1/Tobs (QuickCBC.c:418-419)
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params) fail("log_likelihood_intrinsic", QCBC_EINVAL);
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params || !grid_ok(freq, N / 2, Tobs))
+    fail("log_likelihood_intrinsic", QCBC_EINVAL);
   qcbc_desc desc = desc_from(net, N, Tobs);
   int st;
   qcbc_ctx* ctx = get_ctx(desc, D, SN, &st);
@@ -460,14 +501,16 @@ extern "C" {
 
 void heterodyne(struct Net* net, struct Het* het, double** D, double* params, RealVector* freq, double** SN, int N,
                 double Tobs) {
-  (void)freq;  // the analysis grid f_i = i/Tobs, freq[0] = 1/Tobs (QuickCBC.c:418-419)
-  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !D || !SN || !params || !het->fgrid)
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !het || !D || !SN || !params || !het->fgrid ||
+      !grid_ok(freq, N / 2, Tobs))
     fail("heterodyne", QCBC_EINVAL);
   const int M = het->M, J = het->J, NR = het->NR, nd = net->Nifo;
   if (J < 1 || M != NR * J + 1) fail("heterodyne", QCBC_EINVAL);
   qcbc_desc desc = desc_from(net, N, Tobs);
   int st;
-  qcbc_ctx* ctx = get_ctx(desc, D, SN, &st);
+  // heterodyne runs rarely and often right after SN was redrawn in place
+  // (noise marginalisation, QuickCBC.c:2001-2006): always bind D/SN afresh
+  qcbc_ctx* ctx = get_ctx(desc, D, SN, &st, true);
   if (!ctx) fail("heterodyne", st);
   qcbc_het* h = nullptr;
   st = qcbc_het_create(ctx, params, het->fgrid, M, J, het->MN, het->MM, &h);

@@ -52,10 +52,13 @@ struct qcbc_pt {
   unsigned long long *d_acc = nullptr, *d_swaps = nullptr;
   std::vector<int> perm;  // storage index -> point number t * n_chains + c
   hipEvent_t ev_fork = nullptr, ev_join[QCBC_PT_MAX_GROUPS] = {};
-  // captured steps: [0] one step, [1] kGraphSteps steps (the column groups
-  // run through them without meeting, as the free-running streams do)
-  hipGraph_t graph[2] = {};
-  hipGraphExec_t exec[2] = {};
+  // captured steps: [0] one step, [1] kGraphSteps steps, [2] the last
+  // run's remainder of rem_steps (< kGraphSteps) steps (in [1] and [2] the
+  // column groups run through the steps without meeting, as the
+  // free-running streams do)
+  hipGraph_t graph[3] = {};
+  hipGraphExec_t exec[3] = {};
+  int rem_steps = 0;
   bool step_timing = false;
   std::vector<double> lat_ms;
   // QuickCBC's own step (update() and MCMC_all's swaps) instead of the
@@ -142,7 +145,7 @@ static void pt_free(qcbc_pt* pt) {
   if (pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
   for (hipEvent_t e : pt->ev_join)
     if (e) (void)hipEventDestroy(e);
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < 3; i++) {
     if (pt->exec[i]) (void)hipGraphExecDestroy(pt->exec[i]);
     if (pt->graph[i]) (void)hipGraphDestroy(pt->graph[i]);
   }
@@ -483,20 +486,35 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
   hipSetDevice(c->device);
   hipStream_t s = c->stream;
   pt->lat_ms.clear();
-  // no memset nodes in either graph: every fill is a kernel
-  const bool multi = use_graph && !pt->step_timing && nsteps >= kGraphSteps;
-  if (use_graph && !pt->exec[0]) {
+  // no memset nodes in any graph: every fill is a kernel.  Timed steps
+  // replay the one-step graph; otherwise the run is kGraphSteps-step graphs
+  // and one graph of the remaining steps, captured for that count.
+  const bool multi = use_graph && !pt->step_timing;
+  const int full = multi ? nsteps / kGraphSteps : 0, rem = multi ? nsteps % kGraphSteps : 0;
+  if (use_graph && !multi && !pt->exec[0]) {
     int st = pt_capture(pt, 0, 1);
     if (st) return st;
   }
-  if (multi && !pt->exec[1]) {
+  if (full > 0 && !pt->exec[1]) {
     int st = pt_capture(pt, 1, kGraphSteps);
     if (st) return st;
   }
-  int i = 0;
-  if (multi)
-    for (; i + kGraphSteps <= nsteps; i += kGraphSteps) HIPCHK(hipGraphLaunch(pt->exec[1], s));
-  for (; i < nsteps; i++) {
+  if (rem > 0 && (!pt->exec[2] || pt->rem_steps != rem)) {
+    if (pt->exec[2]) (void)hipGraphExecDestroy(pt->exec[2]);
+    if (pt->graph[2]) (void)hipGraphDestroy(pt->graph[2]);
+    pt->exec[2] = nullptr;
+    pt->graph[2] = nullptr;
+    pt->rem_steps = 0;
+    int st = pt_capture(pt, 2, rem);
+    if (st) return st;
+    pt->rem_steps = rem;
+  }
+  if (multi) {
+    for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(pt->exec[1], s));
+    if (rem > 0) HIPCHK(hipGraphLaunch(pt->exec[2], s));
+    return pt_sync(pt);
+  }
+  for (int i = 0; i < nsteps; i++) {
     auto t0 = std::chrono::steady_clock::now();
     if (use_graph) {
       HIPCHK(hipGraphLaunch(pt->exec[0], s));

@@ -467,9 +467,11 @@ class PT:
         g = ctypes.c_int()
         _check(L.qcbc_pt_groups(h, ctypes.byref(g)), "qcbc_pt_groups")
         self.groups = g.value
+        self.steps_run = 0
 
     def run(self, nsteps, graph=False):
         _check(lib().qcbc_pt_run(self.h, nsteps, 1 if graph else 0), "qcbc_pt_run")
+        self.steps_run += nsteps
 
     def step_timing(self, on):
         _check(lib().qcbc_pt_set_step_timing(self.h, 1 if on else 0), "qcbc_pt_set_step_timing")

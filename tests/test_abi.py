@@ -117,3 +117,40 @@ def test_reference_struct_mirrors_match_the_header(tmp_path):
         assert int(got[tag + ".sizeof"]) == ctypes.sizeof(cls), tag
         for f in cls._fields_:
             assert getattr(cls, f[0]).offset == int(got[tag + "." + f[0]]), (tag, f[0])
+
+
+def _dropin_with_freq(shift):
+    """Calls log_likelihood_full in a child process with the analysis grid
+    f_i = i/Tobs, or with that grid shifted by `shift` Hz from bin 2 on, and
+    returns the process's exit code and stderr."""
+    code = (
+        "import ctypes,sys\n"
+        "sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "from quickcbc_amd import qcbc as Q, workloads as W\n"
+        "an = W.config0()\n"
+        "net = Q.make_net(an)\n"
+        "fr = Q.make_freq(an)\n"
+        "fr._keep[2:] += %r\n"
+        "D = Q.RowMatrix(np.zeros((an.nifo, an.N)))\n"
+        "S = Q.RowMatrix(np.ones((an.nifo, an.N // 2)))\n"
+        "p = W.injection0()\n"
+        "rho = np.zeros(an.nifo)\n"
+        "Q.lib().log_likelihood_full(ctypes.byref(net), D.p, Q._d(p), ctypes.byref(fr), S.p, Q._d(rho), an.N, an.Tobs)\n"
+        % (ROOT, shift))
+    env = dict(os.environ)
+    env["HIP_VISIBLE_DEVICES"] = "-1"
+    env["ROCR_VISIBLE_DEVICES"] = "-1"
+    r = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
+    return r.returncode, r.stderr
+
+
+def test_dropin_rejects_a_frequency_list_that_is_not_the_analysis_grid():
+    """The drop-ins evaluate on f_i = i/Tobs (QuickCBC.c:417-419): a freq that
+    differs is an invalid argument (the reference's error convention: print,
+    exit(-1)), checked before any device work; the grid itself passes the
+    check (and then, with no GPU, stops at the device)."""
+    rc, err = _dropin_with_freq(0.5)
+    assert rc == 255 and "log_likelihood_full: invalid argument" in err, (rc, err)
+    rc, err = _dropin_with_freq(0.0)
+    assert rc == 255 and "no usable gfx950 device" in err, (rc, err)
