@@ -18,7 +18,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 ERRORS = {0: "QCBC_OK", 1: "QCBC_EINVAL", 12: "QCBC_ENOMEM", 14: "QCBC_EFAULT", 33: "QCBC_EDOM",
-          100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY", 104: "QCBC_ECANCELED"}
+          100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY",
+          104: "QCBC_ECANCELED", 105: "QCBC_ETIMEDOUT"}
 
 
 class QcbcError(RuntimeError):
@@ -198,7 +199,33 @@ def make_desc(an):
     return d
 
 
-class Context:
+class _Handle:
+    """Lifetime links between handles: a handle made on another (an ensemble
+    or a heterodyne on an analysis) must be destroyed first.  Owners hold
+    their dependents strongly, so even when the garbage collector finalises
+    both in one cycle (in any order), closing the owner closes the dependents
+    before it."""
+
+    def _depend_on(self, *owners):
+        self._owners = [o for o in owners if o is not None]
+        for o in self._owners:
+            o.__dict__.setdefault("_deps", []).append(self)
+
+    def _release(self):
+        for o in self.__dict__.get("_owners", []):
+            deps = o.__dict__.get("_deps")
+            if deps is not None:
+                o._deps = [d for d in deps if d is not self]
+        self._owners = []
+
+    def _close_deps(self):
+        deps = self.__dict__.get("_deps", [])
+        self._deps = []
+        for d in reversed(deps):
+            d.close()
+
+
+class Context(_Handle):
     """One resident analysis on the GPU (qcbc_ctx)."""
 
     def __init__(self, an, D=None, SN=None):
@@ -261,6 +288,7 @@ class Context:
         return ej, ev
 
     def close(self):
+        self._close_deps()
         if self.h:
             lib().qcbc_ctx_destroy(self.h)
             self.h = None
@@ -436,7 +464,7 @@ def dropin_fbegin(params):
     return lib().fbegin(_d(p))
 
 
-class PT:
+class PT(_Handle):
     """Resident parallel-tempered ensemble (qcbc_pt)."""
 
     def __init__(self, ctx, n_temps, n_chains, ladder, init, sigma, seed=1, precision=FP64, groups=0, het=None):
@@ -464,6 +492,7 @@ class PT:
             _check(L.qcbc_pt_create_het(ctx.h, het.h, n_temps, n_chains, ladder, _d(init), _d(sigma), seed, groups,
                                         ctypes.byref(h)), "qcbc_pt_create_het")
         self.h = h
+        self._depend_on(ctx, het)
         g = ctypes.c_int()
         _check(L.qcbc_pt_groups(h, ctypes.byref(g)), "qcbc_pt_groups")
         self.groups = g.value
@@ -495,6 +524,7 @@ class PT:
         if self.h:
             lib().qcbc_pt_destroy(self.h)
             self.h = None
+        self._release()
 
     def __del__(self):
         try:
@@ -546,6 +576,8 @@ class PTUpdate(PT):
         _check(L.qcbc_pt_create_update(ctx.h, het.h, n_temps, n_chains, ladder, _d(init), ctypes.byref(desc), seed,
                                        groups, ctypes.byref(h)), "qcbc_pt_create_update")
         self.h = h
+        self._depend_on(ctx, het)
+        self.steps_run = 0
         g = ctypes.c_int()
         _check(L.qcbc_pt_groups(h, ctypes.byref(g)), "qcbc_pt_groups")
         self.groups = g.value
@@ -621,7 +653,7 @@ def _ptr_array(ptrs):
     return arr
 
 
-class Multi:
+class Multi(_Handle):
     """Persistent multi-analysis plan (qcbc_multi)."""
 
     def __init__(self, ctxs, B, precision=FP64):
@@ -642,6 +674,7 @@ class Multi:
         h = _vp()
         _check(L.qcbc_multi_create(hs, bb, len(ctxs), precision, ctypes.byref(h)), "qcbc_multi_create")
         self.h = h
+        self._depend_on(*self.ctxs)
         self.params = [DeviceBuffer(8 * 11 * b) for b in self.B]
         self.logL = [DeviceBuffer(8 * b) for b in self.B]
         self.rho = [DeviceBuffer(8 * 3 * b) for b in self.B]
@@ -685,6 +718,7 @@ class Multi:
         if self.h:
             lib().qcbc_multi_destroy(self.h)
             self.h = None
+        self._release()
 
     def __del__(self):
         try:
@@ -798,7 +832,7 @@ def point_records(ctx, params):
 HET_KEYS = ("amp", "phase", "IP", "SL", "lc", "ls", "rc", "rs", "aa")
 
 
-class Het:
+class Het(_Handle):
     """A heterodyne on the GPU (qcbc_het): made by heterodyne() on a bound
     analysis (``Het(ctx, params=..., fgrid=..., J=..., MN=..., MM=...)``) or
     imported from set-up arrays (``Het(ctx, arrays=dict)``, the layout of
@@ -818,6 +852,7 @@ class Het:
             _check(lib().qcbc_het_import(ctx.h, int(arrays["M"]), int(arrays["J"]), *[_d(x) for x in a],
                                          float(arrays["logLb"]), ctypes.byref(h)), "qcbc_het_import")
         self.h = h
+        self._depend_on(ctx)
         M, Jo, NR, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lb = ctypes.c_double()
         _check(lib().qcbc_het_info(h, ctypes.byref(M), ctypes.byref(Jo), ctypes.byref(NR), ctypes.byref(n),
@@ -887,9 +922,11 @@ class Het:
         return ms.value
 
     def close(self):
+        self._close_deps()
         if self.h:
             lib().qcbc_het_destroy(self.h)
             self.h = None
+        self._release()
 
     def __del__(self):
         try:
