@@ -200,7 +200,8 @@ enum {
   QCBC_ENOTBOUND = 102,
   QCBC_EBUSY = 103,   /* admission refused: memory budget or slots */
   QCBC_ECANCELED = 104,
-  QCBC_ETIMEDOUT = 105
+  QCBC_ETIMEDOUT = 105,
+  QCBC_ERANGE = 34    /* as ERANGE: a batch larger than a fixed capacity */
 };
 
 enum { QCBC_FP64 = 0, QCBC_MIXED = 1 };
@@ -237,6 +238,19 @@ int qcbc_loglike(qcbc_ctx *ctx, const double *params, int B, int precision, doub
 /* Same with device pointers, asynchronous on the context's stream. */
 int qcbc_loglike_device(qcbc_ctx *ctx, const double *d_params, int B, int precision,
                         double *d_logL, double *d_rho, int *d_status);
+/* No counterpart in the reference: qcbc_loglike_device for callers that
+ * cannot allocate device memory (code under HIP stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels and
+ * results, but the point records and partial sums live in a fixed device
+ * array of the library (one per GPU, about 17 MB, held until the process
+ * ends) instead of the context's scratch, so the call allocates nothing.
+ * At most QCBC_NOALLOC_MAX_POINTS points per call (QCBC_ERANGE beyond).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from different contexts or threads, before the previous call's work
+ * on its stream has finished) race and must be serialised by the caller. */
+#define QCBC_NOALLOC_MAX_POINTS 4096
+int qcbc_loglike_device_noalloc(qcbc_ctx *ctx, const double *d_params, int B, int precision,
+                                double *d_logL, double *d_rho, int *d_status);
 /* The plain FP64 GPU path: per point, the reference's three loops as three
  * launches (waveform -> amp/phase in HBM, projection -> templates in HBM,
  * inner products), one point after another.  Host pointers. */

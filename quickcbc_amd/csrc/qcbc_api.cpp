@@ -47,6 +47,9 @@ int ensure_device() {
     g_ready_dev[dev] = -1;
     return QCBC_ENODEV;
   }
+  // the no-allocation entry point's array; a failure here leaves that one
+  // entry point reporting QCBC_ENODEV and the rest of the library working
+  (void)noalloc_init_device(dev);
   g_ready_dev[dev] = 1;
   return QCBC_OK;
 }
@@ -663,6 +666,7 @@ const char* qcbc_strerror(int code) {
     case QCBC_EBUSY: return "admission refused: memory budget or slots exhausted";
     case QCBC_ECANCELED: return "request canceled";
     case QCBC_ETIMEDOUT: return "timed out waiting for a request";
+    case QCBC_ERANGE: return "batch larger than the fixed capacity of this entry point";
     default: return "unknown error";
   }
 }

@@ -63,6 +63,10 @@ struct qcbc_ctx {
 namespace qcbc {
 extern thread_local hipError_t t_last_hip;
 int ensure_device();
+// the no-allocation entry point's device array: its address on device dev
+// (called once per device, with dev current), and its size in bytes
+int noalloc_init_device(int dev);
+size_t noalloc_scratch_bytes();
 int effective_splits(const qcbc_ctx* c, int prec, int B);
 LLInst make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial);
 // the fused likelihood launch on stream s, bracketed by timing events when

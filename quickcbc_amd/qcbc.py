@@ -18,7 +18,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 ERRORS = {0: "QCBC_OK", 1: "QCBC_EINVAL", 12: "QCBC_ENOMEM", 14: "QCBC_EFAULT", 33: "QCBC_EDOM",
-          100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY",
+          34: "QCBC_ERANGE", 100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY",
           104: "QCBC_ECANCELED", 105: "QCBC_ETIMEDOUT"}
 
 
@@ -80,6 +80,7 @@ def lib():
         L.qcbc_bind_data.argtypes = [_vp, _dp, _dp]
         L.qcbc_loglike.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_loglike_device.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+        L.qcbc_loglike_device_noalloc.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
         L.qcbc_loglike_plain.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_waveform.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_templates.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
