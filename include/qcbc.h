@@ -170,6 +170,43 @@ void Fisher_Het(struct Net *net, struct Het *het, double **fish, double *params,
  * unit vectors, as in the reference. */
 void FisherEvec(double **fish, double *eval, double **evec, int d);
 
+/* The heterodyne's grid choice.  SN/SM: net->Nifo rows of N/2 (the PSD);
+ * freq the analysis grid (as log_likelihood_full's).  fmin/fmax are the
+ * drop-in band (qcbc_set_band).  The reference's prints and its diagnostic
+ * files (ratio.dat, fspace.dat, df.dat, df_flat.dat) are not written. */
+/* Replaces limits, QuickCBC.c:4275-4385 (prototype QuickCBC.h:101): stst[0],
+ * stst[1] the band over which the network SNR^2 of params passes from 0.001
+ * to within 0.001 of its total; ratio[j] = 1 / (SNR^2 density at bin j over
+ * the band average) for j = 1 .. M-1 (M twice the ringdown frequency, at
+ * least 64 Hz, in bins), 1 beyond. */
+void limits(struct Net *net, double *params, RealVector *freq, double **SN, int *stst, double *ratio, int N,
+            double Tobs);
+/* Replaces Fisher_All, QuickCBC.c:2695-2817 (prototype QuickCBC.h:99): the
+ * Fisher matrix of the full likelihood over the band, central differences of
+ * fullphaseamp with epsilon 1e-5.  fish: 11 rows of 11. */
+void Fisher_All(struct Net *net, double **fish, double *params, RealVector *freq, double **SN, int N, double Tobs);
+/* Replaces diff, QuickCBC.c:4913-4964 (prototype QuickCBC.h:103): sum over
+ * detectors and bins [max(fmin, fbegin) Tobs, 2 f_ringdown Tobs) of
+ * 4 |h(params) - h(dparams)|^2 / S. */
+double diff(struct Net *net, double *params, double *dparams, RealVector *freq, double **SM, int N, double Tobs);
+/* Replaces het_diff, QuickCBC.c:4966-5053 (prototype QuickCBC.h:104): diff's
+ * sum on the heterodyne het (found or imported as log_likelihood_het finds it). */
+double het_diff(struct Net *net, struct Het *het, double *params, double *dparams);
+/* Replaces efix, QuickCBC.c:4387-4567 (prototype QuickCBC.h:100): each jump
+ * eval[i] along evec[i] (11 rows of 11) rescaled so that the displaced
+ * point's distance (diff when hflag is 0, het_diff on het otherwise) comes
+ * to zs; a direction that never comes within zs gets 0.  The prior box
+ * min/max and etamin 0.05 (ConstCBC.h:57) bound the displaced points. */
+void efix(struct Net *net, struct Het *het, double *params, double *min, double *max, RealVector *freq, double **SM,
+          int N, double Tobs, double *eval, double **evec, double zs, int hflag);
+/* Replaces het_space, QuickCBC.c:4570-4910 (prototype QuickCBC.h:105): the
+ * heterodyne grid for params (limits, Fisher_All, FisherEvec, efix to 30,
+ * the linear-span scan with tolerance 0.003, J = 2): sets het->MN, MM, J,
+ * NR, M and allocates het->fgrid (M ints) and het->freq (CreateRealVector(M)),
+ * as the reference does; heterodyne() then sets it up. */
+void het_space(struct Net *net, struct Het *het, double *params, double *min, double *max, RealVector *freq,
+               double **SN, int N, double Tobs);
+
 /* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
 AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
 void DestroyAmpPhaseFDWaveform(AmpPhaseFDWaveform *wf);
@@ -523,6 +560,37 @@ int qcbc_het_fisher_time(qcbc_het *het, const double *d_params, int B, int reps,
  * call (qcbc_het_fisher reuses the heterodyne's). */
 int qcbc_fisher_evec(qcbc_ctx *ctx, const double *fish, int B, int d, double *ej, double *ev);
 int qcbc_het_destroy(qcbc_het *het);
+
+/* ------------------------------------------------------------------ *
+ * Part 7: the heterodyne's grid choice (QuickCBC.c:4275-5053, 2695-2817)
+ * ------------------------------------------------------------------ */
+/* Waveforms, Fisher sums and distances run on the GPU; limits' running sum,
+ * efix's search over the jump scale (every direction of every point in
+ * lockstep, one batched evaluation per step) and het_space's span scan run
+ * on the host.  These allocate their device buffers per call (set-up steps,
+ * run once per heterodyne).  The analysis must be bound (its PSD is used).
+ * pmin, pmax: the prior box (11 each); etamin: 0.05 in the reference. */
+/* limits at params: band[2] = {MN, MM}; ratio (N/2 entries, may be NULL). */
+int qcbc_limits(qcbc_ctx *ctx, const double *params, int *band, double *ratio);
+/* Fisher_All at B points (B x 11): fish B x 11 x 11, status B (may be NULL;
+ * QCBC_EDOM for a perturbed point outside the model, the matrix then 1e4 I). */
+int qcbc_fisher_all(qcbc_ctx *ctx, const double *params, int B, double *fish, int *status);
+/* diff between start point params[of[k]] (of B) and px[k], k < K: z (K). */
+int qcbc_diff(qcbc_ctx *ctx, const double *params, int B, const double *px, const int *of, int K, double *z);
+/* efix on the full data for B points: ej (B x 11) rescaled in place along
+ * ev (B x 11 x 11, row i = direction i) to distance zs; trials (B x 11,
+ * may be NULL) the distances evaluated per direction. */
+int qcbc_efix(qcbc_ctx *ctx, const double *params, int B, double *ej, const double *ev, const double *pmin,
+              const double *pmax, double etamin, double zs, int *trials);
+/* het_space at params: MN, MM, M = NR J + 1 (J = 2) grid bins into fgrid (room
+ * for cap; QCBC_ERANGE if more), and efix's jump sizes and directions
+ * (eval 11, evec 11 x 11, may be NULL).  qcbc_het_create then sets it up. */
+int qcbc_het_space(qcbc_ctx *ctx, const double *params, const double *pmin, const double *pmax, double etamin,
+                   int *MN, int *MM, int *M, int *NR, int *fgrid, int cap, double *eval, double *evec);
+/* het_diff and efix (het_diff distance) on a heterodyne, as above. */
+int qcbc_het_diff(qcbc_het *het, const double *params, int B, const double *px, const int *of, int K, double *z);
+int qcbc_het_efix(qcbc_het *het, const double *params, int B, double *ej, const double *ev, const double *pmin,
+                  const double *pmax, double etamin, double zs, int *trials);
 
 #ifdef __cplusplus
 }

@@ -509,3 +509,135 @@ def makespec(ffit, Sspline, lines, Tobs, N):
     SN = np.zeros(N // 2)
     st = L.qo_makespec(x.size, nl, _d(x), _d(y), *[_d(v) for v in Ls], _d(SN), Tobs, N)
     return st, SN
+
+
+# ----------------------------------------------- heterodyne grid choice
+ETAMIN = 0.05  # ConstCBC.h:57
+
+
+def fringdown(params):
+    """fringdown (Utilities.c:567): ringdown frequency (Hz)."""
+    L = lib()
+    L.qo_fringdown.restype = ctypes.c_double
+    L.qo_fringdown.argtypes = [_dp]
+    p = np.ascontiguousarray(params, np.float64)
+    return L.qo_fringdown(_d(p))
+
+
+def limits(labels, gmst, fref, fmin, SN, N, Tobs, params):
+    """limits (QuickCBC.c:4275): (status, [MN, MM], ratio (N/2))."""
+    L = lib()
+    L.qo_limits_flat.restype = ctypes.c_int
+    L.qo_limits_flat.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                 ctypes.c_int, ctypes.c_double, _ip, _dp]
+    labels = np.ascontiguousarray(labels, np.int32)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    stst = np.zeros(2, np.int32)
+    ratio = np.ones(N // 2)
+    st = L.qo_limits_flat(labels.size, _i(labels), gmst, fref, fmin, _d(SN), _d(p), N, Tobs, _i(stst), _d(ratio))
+    return st, (int(stst[0]), int(stst[1])), ratio
+
+
+def fisher_all(labels, gmst, fref, fmin, fmax, SN, N, Tobs, params):
+    """Fisher_All (QuickCBC.c:2695): (status, fish 11 x 11)."""
+    L = lib()
+    L.qo_fisher_all_flat.restype = ctypes.c_int
+    L.qo_fisher_all_flat.argtypes = [ctypes.c_int, _ip] + [ctypes.c_double] * 4 + [_dp, _dp, ctypes.c_int,
+                                                                                   ctypes.c_double, _dp]
+    labels = np.ascontiguousarray(labels, np.int32)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    fish = np.zeros((11, 11))
+    st = L.qo_fisher_all_flat(labels.size, _i(labels), gmst, fref, fmin, fmax, _d(SN), _d(p), N, Tobs, _d(fish))
+    return st, fish
+
+
+def diff(labels, gmst, fref, fmin, SN, N, Tobs, params, dparams):
+    """diff (QuickCBC.c:4913)."""
+    L = lib()
+    L.qo_diff_flat.restype = ctypes.c_double
+    L.qo_diff_flat.argtypes = [ctypes.c_int, _ip] + [ctypes.c_double] * 3 + [_dp, _dp, _dp, ctypes.c_int,
+                                                                             ctypes.c_double]
+    labels = np.ascontiguousarray(labels, np.int32)
+    SN = np.ascontiguousarray(SN, np.float64)
+    a = np.ascontiguousarray(params, np.float64)
+    b = np.ascontiguousarray(dparams, np.float64)
+    return L.qo_diff_flat(labels.size, _i(labels), gmst, fref, fmin, _d(SN), _d(a), _d(b), N, Tobs)
+
+
+def het_diff(labels, gmst, fref, Tobs, het, params, dparams):
+    """het_diff (QuickCBC.c:4966)."""
+    L = lib()
+    L.qo_het_diff_flat.restype = ctypes.c_double
+    L.qo_het_diff_flat.argtypes = ([ctypes.c_int, _ip] + [ctypes.c_double] * 3 + [ctypes.c_int] * 2 +
+                                   [_dp] * 7)
+    labels = np.ascontiguousarray(labels, np.int32)
+    arr = [np.ascontiguousarray(het[k], np.float64) for k in ("freq", "amp", "IP", "SL", "aa")]
+    a = np.ascontiguousarray(params, np.float64)
+    b = np.ascontiguousarray(dparams, np.float64)
+    return L.qo_het_diff_flat(labels.size, _i(labels), gmst, fref, Tobs, het["M"], het["J"], *[_d(x) for x in arr],
+                              _d(a), _d(b))
+
+
+def displace(params, direction, step, pmin, pmax, etamin=ETAMIN):
+    """The displaced point of efix and het_space (QuickCBC.c:4418-4450)."""
+    L = lib()
+    L.qo_displace.restype = None
+    L.qo_displace.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, ctypes.c_double, _dp]
+    a = [np.ascontiguousarray(x, np.float64) for x in (params, direction, pmin, pmax)]
+    out = np.zeros(11)
+    L.qo_displace(_d(a[0]), _d(a[1]), step, _d(a[2]), _d(a[3]), etamin, _d(out))
+    return out
+
+
+def efix(labels, gmst, fref, Tobs, params, ej, ev, pmin, pmax, zs, het=None, fmin=None, SN=None, N=None,
+         etamin=ETAMIN):
+    """efix (QuickCBC.c:4387) for a batch of points: the scaled jump sizes (B x 11) and the
+    distances evaluated per direction (B x 11).  het given: het_diff on it (hflag 1), else diff
+    on the full data SN (hflag 0)."""
+    L = lib()
+    L.qo_efix_batch_flat.restype = None
+    L.qo_efix_batch_flat.argtypes = ([ctypes.c_int, _ip] + [ctypes.c_double] * 3 + [_dp, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_dp] * 6 +
+                                     [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp, _dp, ctypes.c_double, _ip])
+    labels = np.ascontiguousarray(labels, np.int32)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+    B = p.shape[0]
+    e = np.array(ej, np.float64).reshape(B, 11)
+    v = np.ascontiguousarray(ev, np.float64).reshape(B, 11, 11)
+    lo = np.ascontiguousarray(pmin, np.float64)
+    hi = np.ascontiguousarray(pmax, np.float64)
+    trials = np.zeros((B, 11), np.int32)
+    if het is not None:
+        arr = [np.ascontiguousarray(het[k], np.float64) for k in ("freq", "amp", "IP", "SL", "aa")]
+        M, J, ptrs = het["M"], het["J"], [_d(x) for x in arr]
+        S, NN, fm = None, 0, 0.0
+    else:
+        S = np.ascontiguousarray(SN, np.float64)
+        M, J, ptrs = 0, 1, [None] * 5
+        NN, fm = N, fmin
+    L.qo_efix_batch_flat(labels.size, _i(labels), gmst, fref, fm, _d(S) if S is not None else None, NN, Tobs, M, J,
+                         *ptrs, _d(p), B, _d(lo), _d(hi), etamin, _d(e), _d(v), zs, _i(trials))
+    return e, trials
+
+
+def het_space(labels, gmst, fref, fmin, fmax, SN, N, Tobs, params, pmin, pmax, etamin=ETAMIN):
+    """het_space (QuickCBC.c:4570): (status, dict with MN, MM, M, NR, J, fgrid, eval, evec)."""
+    L = lib()
+    L.qo_het_space_flat.restype = ctypes.c_int
+    L.qo_het_space_flat.argtypes = ([ctypes.c_int, _ip] + [ctypes.c_double] * 4 + [_dp, _dp, _dp, _dp,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp])
+    labels = np.ascontiguousarray(labels, np.int32)
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    lo = np.ascontiguousarray(pmin, np.float64)
+    hi = np.ascontiguousarray(pmax, np.float64)
+    out4 = np.zeros(4, np.int32)
+    fgrid = np.zeros(N // 2, np.int32)
+    ev = np.zeros(11)
+    evec = np.zeros((11, 11))
+    st = L.qo_het_space_flat(labels.size, _i(labels), gmst, fref, fmin, fmax, _d(SN), _d(p), _d(lo), _d(hi),
+                             etamin, N, Tobs, _i(out4), _i(fgrid), _d(ev), _d(evec))
+    MN, MM, M, NR = (int(x) for x in out4)
+    return st, dict(MN=MN, MM=MM, M=M, NR=NR, J=2, fgrid=fgrid[:M].copy(), eval=ev, evec=evec)

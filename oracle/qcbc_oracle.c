@@ -2344,3 +2344,646 @@ int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspli
   free(c);
   return QO_OK;
 }
+
+/* ------------------------------------------ heterodyne grid choice (f1) */
+/* The reference's het_space picks the heterodyne's grid from the reference
+ * point alone: limits() gives the band and the per-bin SNR weighting,
+ * Fisher_All and FisherEvec the principal directions, efix scales each to a
+ * fixed likelihood change, and the grid is the coarsest on which the
+ * waveform ratios to the 11 displaced points are linear to a tolerance. */
+
+/* fQNM, Utilities.c:313-354: ringdown frequency from the final-mass and
+ * final-spin fits and Berti's fit of the fundamental QNM (gr-qc/0512160);
+ * masses in seconds. */
+static double qo_fqnm(double m1, double m2, double chi1, double chi2) {
+  const double p0 = 0.04827, p1 = 0.01707, scri = 0.41616;
+  const double k00 = -3.82, k01 = -1.2019, k02 = -1.20764, k10 = 3.79245, k11 = 1.18385, k12 = 4.90494;
+  const double q = (m1 < m2) ? m1 / m2 : m2 / m1;
+  const double eta = q / ((1.0 + q) * (1.0 + q));
+  const double atot = (chi1 + q * q * chi2) / ((1.0 + q) * (1.0 + q));
+  const double aeff = atot + scri * eta * (chi1 + chi2);
+  const double Z1 = 1.0 + pow((1.0 - aeff * aeff), (1. / 3.)) * (pow((1. + aeff), (1. / 3.)) + pow((1. - aeff), (1. / 3.)));
+  const double Z2 = sqrt(3.0 * aeff * aeff + Z1 * Z1);
+  const double risco = 3.0 + Z2 - (aeff / (fabs(aeff) + 1.0e-20)) * sqrt((3.0 - Z1) * (3.0 + Z1 + 2.0 * Z2));
+  const double Eisco = sqrt(1.0 - 2. / (3.0 * risco));
+  const double Lisco = (2. / (3.0 * sqrt(3.))) * (1. + 2.0 * sqrt(3.0 * risco - 2.0));
+  const double Mf =
+      (1.0 - eta * (1.0 - Eisco) - 4.0 * eta * eta * (4.0 * p0 + 16.0 * p1 * atot * (atot + 1.0) + Eisco - 1.0)) *
+      (m1 + m2);
+  const double e2 = eta * eta, e3 = e2 * eta;
+  const double af = atot + eta * (Lisco - 2 * atot * (Eisco - 1.0)) + k00 * e2 + k01 * e2 * aeff + k02 * e2 * aeff * aeff +
+                    k10 * e3 + k11 * e3 * aeff + k12 * e3 * aeff * aeff;
+  return (1.5251 - 1.1568 * pow((1.0 - af), 0.1292)) / (QO_TPI * Mf);
+}
+
+/* fringdown, Utilities.c:567-584: the masses in seconds from ln Mc, ln Mt
+ * with TSUN (eta capped at 1/4), then fQNM. */
+double qo_fringdown(const double *param) {
+  const double Mchirp = exp(param[0]) / QO_MSUN_SI * QO_TSUN;
+  const double M = exp(param[1]) / QO_MSUN_SI * QO_TSUN;
+  double eta = pow((Mchirp / M), (5.0 / 3.0));
+  if (eta > 0.25) eta = 0.25;
+  {
+    const double dm = sqrt(1.0 - 4.0 * eta);
+    return qo_fqnm(M * (1.0 + dm) / 2.0, M * (1.0 - dm) / 2.0, param[2], param[3]);
+  }
+}
+
+/* The number of bins limits() and diff() cover: twice the ringdown
+ * frequency, at least 64 Hz, times Tobs (QuickCBC.c:4288-4290, :4933-4935). */
+static int qo_signal_bins(const double *params, double Tobs) {
+  double fstop = 2.0 * qo_fringdown(params);
+  if (fstop < 64.0) fstop = 64.0;
+  return (int)(fstop * Tobs);
+}
+
+/* The first bin of limits() and diff(): max(fmin, fbegin) Tobs, truncated
+ * (QuickCBC.c:4296-4303, :4923-4930). */
+static int qo_first_bin(const double *params, double fmin, double Tobs) {
+  const double fs = qo_fbegin(params);
+  return (fmin > fs) ? (int)(Tobs * fmin) : (int)(Tobs * fs);
+}
+
+/* limits, QuickCBC.c:4275-4385.  With M = min(signal bins, N/2): the network
+ * SNR^2 density of the reference point, x_j = sum_d 4 A_d(j)^2 / S_d(j) over
+ * j in [jmin, M); stst[0] the first bin where its running sum exceeds 0.001
+ * (jmin if none, never below jmin), stst[1] the first where it is within
+ * 0.001 of the total (M - 1 if none); ratio[j] = 1 / (x_j / average x over
+ * the band), 1 where x_j = 0, for j = 1 .. M-1 (ratio has N/2 entries; the
+ * rest are set to 1).  SN: nifo rows of N/2.  The reference's prints and its
+ * ratio.dat are left out.  Returns 0 or the fullphaseamp status. */
+int qo_limits(int nifo, const int *labels, double gmst, double fref, double fmin, double **SN,
+              const double *params, int N, double Tobs, int *stst, double *ratio) {
+  int M = qo_signal_bins(params, Tobs), jmin, j, id, st, found0 = 0, found1 = 0;
+  double *freq, *buf, *ar[3], *pr[3], *dens, *cum, run = 0.0, total, avg;
+  if (M > N / 2) M = N / 2;
+  jmin = qo_first_bin(params, fmin, Tobs);
+  freq = qo_make_freq(N, Tobs);
+  buf = malloc(sizeof(double) * (2 * (size_t)nifo * M + 2 * (size_t)M));
+  if (!freq || !buf) {
+    free(freq);
+    free(buf);
+    return QO_ENOMEM;
+  }
+  for (id = 0; id < nifo; id++) {
+    ar[id] = buf + (size_t)id * M;
+    pr[id] = buf + (size_t)(nifo + id) * M;
+  }
+  dens = buf + 2 * (size_t)nifo * M;
+  cum = dens + M;
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, freq, params, 2 * M);
+  if (st == 0) {
+    for (j = 0; j < M; j++) {
+      dens[j] = 0.0;
+      cum[j] = 0.0;
+    }
+    for (j = jmin; j < M; j++) {
+      for (id = 0; id < nifo; id++) {
+        const double x = 4.0 * (ar[id][j] * ar[id][j]) / SN[id][j];
+        dens[j] += x;
+        run += x;
+      }
+      cum[j] = run;
+    }
+    total = M > 0 ? cum[M - 1] : 0.0;
+    stst[0] = jmin;
+    stst[1] = M - 1;
+    for (j = jmin; j < M; j++) {
+      if (!found0 && cum[j] > 0.001) {
+        found0 = 1;
+        stst[0] = j;
+      }
+      if (!found1 && total - cum[j] < 0.001) {
+        found1 = 1;
+        stst[1] = j;
+      }
+    }
+    if (stst[0] < jmin) stst[0] = jmin;
+    avg = total / (double)(stst[1] - stst[0]);  /* the band's average SNR^2 per bin */
+    for (j = 0; j < N / 2; j++) ratio[j] = 1.0;
+    for (j = 1; j < M; j++) {
+      const double r = dens[j] / avg;
+      if (r > 0.0) ratio[j] = 1.0 / r;
+    }
+  }
+  free(freq);
+  free(buf);
+  return st;
+}
+
+/* Fisher_All, QuickCBC.c:2695-2817: the Fisher matrix of the full
+ * likelihood over [imin, imax) = [(int)(Tobs fmin), min((int)(Tobs fmax),
+ * N/2)), central differences of fullphaseamp with epsilon 1e-5 on the
+ * analysis grid: fish[i][j] = sum_d sum_k 4 (dA_i dA_j + A^2 dphi_i dphi_j)
+ * / S_d(k), detector by detector, bins in order.  |chi1| or |chi2| >= 0.9999
+ * gives 1e4 I.  fish row-major 11 x 11.  Returns 0 or the first failing
+ * fullphaseamp status (the matrix is then 1e4 I). */
+int qo_fisher_all(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax, double **SN,
+                  const double *params, int N, double Tobs, double *fish) {
+  enum { NP = 11 };
+  const double epsilon = 1.0e-5;
+  const int n = N / 2;
+  int imin = (int)(Tobs * fmin), imax = (int)(Tobs * fmax), i, j, k, id, st = 0;
+  double *freq, *buf, *a0[3], *p0[3], *aP[3], *pP[3], *aM[3], *pM[3], *dA, *dP, pp[NP], pm[NP];
+  if (imax > n) imax = n;
+  for (i = 0; i < NP * NP; i++) fish[i] = (i % (NP + 1) == 0) ? 1.0e4 : 0.0;
+  if (!(fabs(params[2]) < 0.9999 && fabs(params[3]) < 0.9999)) return 0;
+  freq = qo_make_freq(N, Tobs);
+  buf = malloc(sizeof(double) * (size_t)nifo * n * (6 + 2 * NP));
+  if (!freq || !buf) {
+    free(freq);
+    free(buf);
+    return QO_ENOMEM;
+  }
+  for (id = 0; id < nifo; id++) {
+    a0[id] = buf + (size_t)(0 * nifo + id) * n;
+    p0[id] = buf + (size_t)(1 * nifo + id) * n;
+    aP[id] = buf + (size_t)(2 * nifo + id) * n;
+    pP[id] = buf + (size_t)(3 * nifo + id) * n;
+    aM[id] = buf + (size_t)(4 * nifo + id) * n;
+    pM[id] = buf + (size_t)(5 * nifo + id) * n;
+  }
+  dA = buf + (size_t)6 * nifo * n; /* [i][id][k] */
+  dP = dA + (size_t)NP * nifo * n;
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, a0, p0, freq, params, N);
+  for (i = 0; i < NP && st == 0; i++) {
+    for (k = 0; k < NP; k++) pp[k] = pm[k] = params[k];
+    pp[i] += epsilon;
+    pm[i] -= epsilon;
+    st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, aP, pP, freq, pp, N);
+    if (st == 0) st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, aM, pM, freq, pm, N);
+    if (st) break;
+    for (id = 0; id < nifo; id++)
+      for (k = 1; k < n; k++) {
+        double da = aP[id][k], dp = pP[id][k];
+        da -= aM[id][k];
+        dp -= pM[id][k];
+        dA[((size_t)i * nifo + id) * n + k] = da / (2.0 * epsilon);
+        dP[((size_t)i * nifo + id) * n + k] = dp / (2.0 * epsilon);
+      }
+  }
+  if (st == 0) {
+    for (i = 0; i < NP * NP; i++) fish[i] = 0.0;
+    for (id = 0; id < nifo; id++)
+      for (i = 0; i < NP; i++)
+        for (j = i; j < NP; j++) {
+          const double *ai = dA + ((size_t)i * nifo + id) * n, *aj = dA + ((size_t)j * nifo + id) * n;
+          const double *pi = dP + ((size_t)i * nifo + id) * n, *pj = dP + ((size_t)j * nifo + id) * n;
+          for (k = imin; k < imax; k++)
+            fish[i * NP + j] += 4.0 * (ai[k] * aj[k] + a0[id][k] * a0[id][k] * pi[k] * pj[k]) / SN[id][k];
+        }
+    for (i = 0; i < NP; i++)
+      for (j = i + 1; j < NP; j++) fish[j * NP + i] = fish[i * NP + j];
+  } else {
+    for (i = 0; i < NP * NP; i++) fish[i] = (i % (NP + 1) == 0) ? 1.0e4 : 0.0;
+  }
+  free(freq);
+  free(buf);
+  return st;
+}
+
+/* diff, QuickCBC.c:4913-4964: sum_d sum_j 4 |h_d(params) - h_d(dparams)|^2 /
+ * S_d(j), the waveforms as fullphaseamp's amplitude and phase, over bins
+ * [first bin of params, M/2) with M = min(2 signal bins of params, N).  A
+ * failing set-up of either point gives NaN (the reference's arithmetic on
+ * an unset waveform). */
+double qo_diff(int nifo, const int *labels, double gmst, double fref, double fmin, double **SN, const double *params,
+               const double *dparams, int N, double Tobs) {
+  int M = 2 * qo_signal_bins(params, Tobs), imin = qo_first_bin(params, fmin, Tobs), h, j, id, st;
+  double *freq, *buf, *ar[3], *pr[3], *ap[3], *pp[3], dx = 0.0;
+  if (M > N) M = N;
+  h = M / 2;
+  freq = qo_make_freq(N, Tobs);
+  buf = malloc(sizeof(double) * 4 * (size_t)nifo * (h > 0 ? h : 1));
+  if (!freq || !buf || h < 1) {
+    free(freq);
+    free(buf);
+    return NAN;
+  }
+  for (id = 0; id < nifo; id++) {
+    ar[id] = buf + (size_t)(0 * nifo + id) * h;
+    pr[id] = buf + (size_t)(1 * nifo + id) * h;
+    ap[id] = buf + (size_t)(2 * nifo + id) * h;
+    pp[id] = buf + (size_t)(3 * nifo + id) * h;
+  }
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, freq, params, M);
+  if (st == 0) st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ap, pp, freq, dparams, M);
+  if (st == 0) {
+    for (id = 0; id < nifo; id++)
+      for (j = imin; j < h; j++) {
+        const double Re = (ar[id][j] - ap[id][j] * cos(pr[id][j] - pp[id][j]));
+        const double Im = ap[id][j] * sin(pr[id][j] - pp[id][j]);
+        dx += 4.0 * (Re * Re + Im * Im) / SN[id][j];
+      }
+  } else {
+    dx = NAN;
+  }
+  free(freq);
+  free(buf);
+  return dx;
+}
+
+/* het_diff, QuickCBC.c:4966-5053: diff's sum on the heterodyne: per sample
+ * cc = 4 |h(params) - h(dparams)|^2 / a_ref^2, its Legendre coefficients
+ * IP cc per region summed against SL, less the overcount of the region
+ * starts 1 .. NR-2 (aa).  NaN on a failing set-up, as qo_diff. */
+double qo_het_diff(int nifo, const int *labels, double gmst, double fref, double Tobs, const qo_het *h,
+                   const double *params, const double *dparams) {
+  const int M = h->M, J = h->J, NR = h->NR;
+  int id, jj, kk, ii, st;
+  double *buf = malloc(sizeof(double) * (4 * (size_t)nifo + 1) * M), *ar[3], *pr[3], *ap[3], *pp[3], *cc;
+  double uu[16], vv[16], dx = 0.0;
+  if (!buf || J + 1 > 16) {
+    free(buf);
+    return NAN;
+  }
+  for (id = 0; id < nifo; id++) {
+    ar[id] = buf + (size_t)(0 * nifo + id) * M;
+    pr[id] = buf + (size_t)(1 * nifo + id) * M;
+    ap[id] = buf + (size_t)(2 * nifo + id) * M;
+    pp[id] = buf + (size_t)(3 * nifo + id) * M;
+  }
+  cc = buf + (size_t)4 * nifo * M;
+  st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, h->freq, params, 2 * M);
+  if (st == 0) st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ap, pp, h->freq, dparams, 2 * M);
+  if (st) {
+    free(buf);
+    return NAN;
+  }
+  for (id = 0; id < nifo; id++) {
+    double HH = 0.0, x = 0.0;
+    for (jj = 0; jj < M; jj++) {
+      const double Re = (ar[id][jj] - ap[id][jj] * cos(pr[id][jj] - pp[id][jj]));
+      const double Im = ap[id][jj] * sin(pr[id][jj] - pp[id][jj]);
+      const double ha = h->amp[id * M + jj];
+      cc[jj] = 4.0 * (Re * Re + Im * Im) / (ha * ha);
+    }
+    for (ii = 0; ii < NR; ii++) {
+      const double *IPi = h->IP + (size_t)ii * (J + 1) * (J + 1);
+      const double *SLr = h->SL + ((size_t)ii * nifo + id) * (J + 1);
+      for (jj = 0; jj <= J; jj++) {
+        vv[jj] = cc[ii * J + jj];
+        uu[jj] = 0.0;
+      }
+      for (jj = 0; jj <= J; jj++)
+        for (kk = 0; kk <= J; kk++) uu[jj] += IPi[jj * (J + 1) + kk] * vv[kk];
+      for (jj = 0; jj <= J; jj++) HH += uu[jj] * SLr[jj];
+      if (ii < NR - 2) x += h->aa[id * NR + ii + 1] * vv[J]; /* overcount of the region starts */
+    }
+    HH -= x;
+    dx += HH;
+  }
+  free(buf);
+  return dx;
+}
+
+/* The displaced point of efix and het_space (QuickCBC.c:4418-4450,
+ * :4628-4662): params + step dir, or params - step dir if that puts eta
+ * above 1/4; the angles phi (period 2 pi), alpha (2 pi) and psi (pi) brought
+ * into range as the reference does it (x = p/P, p = (x - floor(x)) P, one
+ * radian added when x < 0, then whole periods taken off or added until p
+ * lies in [0, P]); clipped to [min, max]; ln Mc reset so that eta is at
+ * most 0.2499 and at least etamin. */
+void qo_displace(const double *params, const double *dir, double step, const double *min, const double *max,
+                 double etamin, double *px) {
+  enum { NP = 11 };
+  static const int idx[3] = {4, 7, 9};
+  const double per[3] = {QO_TPI, QO_TPI, QO_PI};
+  int j, a;
+  double eta;
+  for (j = 0; j < NP; j++) px[j] = params[j] + step * dir[j];
+  eta = exp((5.0 / 3.0) * (px[0] - px[1]));
+  if (eta > 0.25)
+    for (j = 0; j < NP; j++) px[j] = params[j] - step * dir[j];
+  for (a = 0; a < 3; a++) {
+    const double x = px[idx[a]] / per[a];
+    px[idx[a]] = (x - floor(x)) * per[a];
+    if (x < 0.0) px[idx[a]] += 1.0;
+  }
+  for (a = 0; a < 3; a++) {
+    while (px[idx[a]] > per[a]) px[idx[a]] -= per[a];
+    while (px[idx[a]] < 0.0) px[idx[a]] += per[a];
+  }
+  for (j = 0; j < NP; j++)
+    if (px[j] > max[j]) px[j] = max[j];
+  for (j = 0; j < NP; j++)
+    if (px[j] < min[j]) px[j] = min[j];
+  eta = exp((5.0 / 3.0) * (px[0] - px[1]));
+  if (eta > 0.25) px[0] = px[1] + 3.0 / 5.0 * log(0.2499);
+  if (eta < etamin) px[0] = px[1] + 3.0 / 5.0 * log(etamin);
+}
+
+/* efix, QuickCBC.c:4387-4567: for each eigen-direction i, the scale alpha of
+ * its jump eval[i] evec[i] at which the displaced point's distance z (diff
+ * on the full data when het is NULL, het_diff on the heterodyne otherwise)
+ * comes to zs: doubling alpha while z < zs/2 and dividing it by 1.9 while
+ * z > 2 zs (at most 15 trials, alpha < 1e4), then secant steps from 1.1
+ * alpha until |z - zs| <= 0.2 (at most 10, alpha < 1e4); the trial closest
+ * to zs wins, none if it missed by more than zs.  eval[i] *= that alpha (0
+ * for a failed direction).  evec row-major 11 x 11 (row i = direction i);
+ * SN nifo rows of N/2 (diff only).  trials (may be NULL) receives the number
+ * of distances evaluated per direction. */
+void qo_efix(int nifo, const int *labels, double gmst, double fref, double fmin, double **SN, int N, double Tobs,
+             const qo_het *het, const double *params, const double *min, const double *max, double etamin,
+             double *eval, const double *evec, double zs, int *trials) {
+  enum { NP = 11 };
+  const double zmx = zs * 2.0, zmn = zs / 2.0;
+  double px[NP];
+  int i;
+#define QO_DIST(p)                                                                                   \
+  (het ? qo_het_diff(nifo, labels, gmst, fref, Tobs, het, params, (p))                               \
+       : qo_diff(nifo, labels, gmst, fref, fmin, SN, params, (p), N, Tobs))
+  for (i = 0; i < NP; i++) {
+    const double *dir = evec + (size_t)i * NP;
+    double dzmin = 1.0e20, alpham = 1.0, alpha0 = 1.0, alpha, alphanew, z0, z, dz;
+    int k = 0, n = 0;
+    do {
+      qo_displace(params, dir, alpha0 * eval[i], min, max, etamin, px);
+      z0 = QO_DIST(px);
+      n++;
+      dz = fabs(z0 - zs);
+      if (dz < dzmin) {
+        dzmin = dz;
+        alpham = alpha0;
+      }
+      if (z0 < zmn) alpha0 *= 2.0;
+      if (z0 > zmx) alpha0 /= 1.9;
+      k++;
+    } while ((z0 > zmx || z0 < zmn) && k < 15 && alpha0 < 1.0e4);
+    alpha = alpha0 * 1.1;
+    k = 0;
+    do {
+      qo_displace(params, dir, alpha * eval[i], min, max, etamin, px);
+      z = QO_DIST(px);
+      n++;
+      if (alpha > alpha0)
+        alphanew = alpha0 + (zs - z0) / (z - z0) * (alpha - alpha0);
+      else
+        alphanew = alpha + (zs - z) / (z0 - z) * (alpha0 - alpha);
+      dz = fabs(z - zs);
+      if (dz < dzmin) {
+        dzmin = dz;
+        alpham = alpha;
+      }
+      z0 = z;
+      alpha0 = alpha;
+      alpha = alphanew;
+      k++;
+    } while (fabs(z - zs) > 0.2 && k < 10 && alpha < 1.0e4);
+    if (dzmin / zs > 1.0) alpham = 0.0; /* a direction that never came near zs */
+    eval[i] *= alpham;
+    if (trials) trials[i] = n;
+  }
+#undef QO_DIST
+}
+
+/* het_space, QuickCBC.c:4570-4910, for the reference point params:
+ *  1. limits: the band [MN, MM] and the SNR weighting ratio;
+ *  2. Fisher_All, FisherEvec and efix to zs = 30 on the full data: eleven
+ *     displaced points dparams[i] (qo_displace by eval[i] along evec[i]);
+ *  3. on [MN, MM) and per detector and displaced point, the ratio terms
+ *     Cs = 1 - (A_i/A) cos(dphi), Sn = (A_i/A) sin(dphi), DH = Cs^2 + Sn^2;
+ *     a new grid point wherever one of them, at the midpoint of the current
+ *     span, departs from the straight line between its ends by more than
+ *     max(0.003, 0.003 ratio), or the span exceeds 100 Tobs bins;
+ *  4. each bin's span length, then the flat grid: from MN, J = 2 equal steps
+ *     per region, each as long as the shortest span it crosses, the last
+ *     region refitted to end at MM when that leaves a step of at least 1.
+ * fgrid has room for N/2 entries; *M = NR J + 1 of them are written.  eval
+ * and evec (may be NULL) receive efix's scaled jump sizes and the
+ * directions (11, 11 x 11).  The reference's prints and its fspace.dat,
+ * df.dat and df_flat.dat are left out; it copies the grid from an array it
+ * has just freed, here the values are used as computed.  Returns 0 or a
+ * fullphaseamp status. */
+int qo_het_space(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax, double **SN,
+                 const double *params, const double *min, const double *max, double etamin, int N, double Tobs,
+                 int *MN_out, int *MM_out, int *M_out, int *NR_out, int *fgrid, double *eval_out, double *evec_out) {
+  enum { NP = 11, J = 2 };
+  const double tol = 0.003;
+  const int n2 = N / 2;
+  int stst[2], MN, MM, st, i, j, id, count, ii, jmin, k;
+  double fish[NP * NP], ej[NP], ev[NP * NP], dpar[NP * NP];
+  double *ratio = malloc(sizeof(double) * n2), *freq = NULL, *buf = NULL, *Cs, *Sn, *DH;
+  int *marks = NULL, *span = NULL;
+  if (!ratio) return QO_ENOMEM;
+  st = qo_limits(nifo, labels, gmst, fref, fmin, SN, params, N, Tobs, stst, ratio);
+  if (st) goto out;
+  MN = stst[0];
+  MM = stst[1];
+  st = qo_fisher_all(nifo, labels, gmst, fref, fmin, fmax, SN, params, N, Tobs, fish);
+  if (st) goto out;
+  qo_fisher_evec(fish, ej, ev, NP);
+  qo_efix(nifo, labels, gmst, fref, fmin, SN, N, Tobs, NULL, params, min, max, etamin, ej, ev, 30.0, NULL);
+  if (eval_out) memcpy(eval_out, ej, sizeof(ej));
+  if (evec_out) memcpy(evec_out, ev, sizeof(ev));
+  for (i = 0; i < NP; i++) qo_displace(params, ev + (size_t)i * NP, ej[i], min, max, etamin, dpar + (size_t)i * NP);
+  /* the reference point's and the displaced points' waveforms on [0, MM) */
+  freq = qo_make_freq(N, Tobs);
+  buf = malloc(sizeof(double) * (size_t)MM * nifo * (2 + 2 * NP + 3 * NP));
+  marks = malloc(sizeof(int) * (n2 + 1));
+  span = malloc(sizeof(int) * (n2 + 1));
+  if (!freq || !buf || !marks || !span || MM < 1) {
+    st = QO_ENOMEM;
+    goto out;
+  }
+  {
+    double *ar[3], *pr[3], *ap[3], *pp[3];
+    double *A0 = buf, *P0 = A0 + (size_t)nifo * MM, *Ad = P0 + (size_t)nifo * MM, *Pd = Ad + (size_t)NP * nifo * MM;
+    Cs = Pd + (size_t)NP * nifo * MM;
+    Sn = Cs + (size_t)NP * nifo * MM;
+    DH = Sn + (size_t)NP * nifo * MM;
+    for (id = 0; id < nifo; id++) {
+      ar[id] = A0 + (size_t)id * MM;
+      pr[id] = P0 + (size_t)id * MM;
+    }
+    st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ar, pr, freq, params, 2 * MM);
+    for (i = 0; i < NP && st == 0; i++) {
+      for (id = 0; id < nifo; id++) {
+        ap[id] = Ad + ((size_t)i * nifo + id) * MM;
+        pp[id] = Pd + ((size_t)i * nifo + id) * MM;
+      }
+      st = qo_fullphaseamp(nifo, labels, gmst, fref, Tobs, ap, pp, freq, dpar + (size_t)i * NP, 2 * MM);
+    }
+    if (st) goto out;
+    /* [i][id][j] */
+    for (i = 0; i < NP; i++)
+      for (id = 0; id < nifo; id++)
+        for (j = MN; j < MM; j++) {
+          const size_t o = ((size_t)i * nifo + id) * MM + j;
+          const double r = Ad[o] / ar[id][j], dphi = pr[id][j] - Pd[o];
+          Cs[o] = 1.0 - r * cos(dphi);
+          Sn[o] = r * sin(dphi);
+          DH[o] = Cs[o] * Cs[o] + Sn[o] * Sn[o];
+        }
+  }
+  /* the spans on which every ratio term is linear to the tolerance */
+  jmin = MN;
+  count = 0;
+  marks[0] = jmin;
+  {
+    const int djmax = (int)(100.0 * Tobs);
+    for (j = MN; j < MM; j++) {
+      int flag = 0;
+      double tl = tol * ratio[j];
+      if (tl < tol) tl = tol;
+      for (id = 0; id < nifo; id++)
+        for (i = 0; i < NP; i++) {
+          const size_t o = ((size_t)i * nifo + id) * MM;
+          const int jmid = (j + jmin) / 2;
+          /* at j = jmin the midpoint line is 0/0, NaN, and no test fires */
+          const double wa = (double)(j - jmid), wb = (double)(jmid - jmin), wd = (double)(j - jmin);
+          const double cmid = (Cs[o + jmin] * wa + Cs[o + j] * wb) / wd;
+          const double smid = (Sn[o + jmin] * wa + Sn[o + j] * wb) / wd;
+          const double dmid = (DH[o + jmin] * wa + DH[o + j] * wb) / wd;
+          if (fabs(Cs[o + jmid] - cmid) > tl || fabs(Sn[o + jmid] - smid) > tl || fabs(DH[o + jmid] - dmid) > tl)
+            flag = 1;
+        }
+      if ((j - jmin) > djmax) flag = 1;
+      if (flag) {
+        jmin = j;
+        count++;
+        marks[count] = jmin;
+      }
+    }
+  }
+  /* every bin's span length (100 outside [MN, MM)) */
+  for (i = 0; i < n2; i++) span[i] = 100;
+  for (ii = 0; ii < count; ii++)
+    for (i = marks[ii]; i < marks[ii + 1]; i++) span[i] = marks[ii + 1] - marks[ii];
+  for (i = marks[count]; i < MM; i++) span[i] = MM - marks[count];
+  /* the flat grid: J equal steps per region, each the shortest span the
+   * region crosses (a crossing past N/2 stops the search, as in the reference) */
+  j = MN;
+  ii = 0;
+  fgrid[0] = j;
+  do {
+    int step = span[j], again;
+    do {
+      again = 0;
+      k = j + step * J;
+      for (i = j; i <= k; i++) {
+        if (i < n2) {
+          if (span[i] < step) {
+            step = span[i];
+            again = 1;
+          }
+        } else {
+          again = 0;
+        }
+      }
+    } while (again);
+    j = k;
+    for (i = 0; i < J; i++) {
+      ii++;
+      if (ii >= n2) {
+        st = QO_EFAULT;
+        goto out;
+      }
+      fgrid[ii] = fgrid[ii - 1] + step;
+    }
+  } while (j < MM);
+  /* the last region refitted to end at MM */
+  {
+    const int step = (MM - fgrid[ii - J]) / J;
+    if (step >= 1) {
+      ii -= J;
+      for (i = 0; i < J; i++) {
+        ii++;
+        fgrid[ii] = fgrid[ii - 1] + step;
+      }
+    }
+  }
+  *MN_out = MN;
+  *MM_out = MM;
+  *M_out = ii + 1;
+  *NR_out = ii / J;
+out:
+  free(ratio);
+  free(freq);
+  free(buf);
+  free(marks);
+  free(span);
+  return st;
+}
+
+/* Flat-array forms for the ctypes wrapper: SN nifo x N/2. */
+int qo_limits_flat(int nifo, const int *labels, double gmst, double fref, double fmin, const double *SN,
+                   const double *params, int N, double Tobs, int *stst, double *ratio) {
+  double *Sr[3];
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  return qo_limits(nifo, labels, gmst, fref, fmin, Sr, params, N, Tobs, stst, ratio);
+}
+
+int qo_fisher_all_flat(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax,
+                       const double *SN, const double *params, int N, double Tobs, double *fish) {
+  double *Sr[3];
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  return qo_fisher_all(nifo, labels, gmst, fref, fmin, fmax, Sr, params, N, Tobs, fish);
+}
+
+double qo_diff_flat(int nifo, const int *labels, double gmst, double fref, double fmin, const double *SN,
+                    const double *params, const double *dparams, int N, double Tobs) {
+  double *Sr[3];
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  return qo_diff(nifo, labels, gmst, fref, fmin, Sr, params, dparams, N, Tobs);
+}
+
+static void qo_het_view(qo_het *h, int nifo, int M, int J, const double *freq, const double *amp, const double *IP,
+                        const double *SL, const double *aa) {
+  memset(h, 0, sizeof(*h));
+  h->M = M;
+  h->J = J;
+  h->NR = (M - 1) / J;
+  h->nifo = nifo;
+  h->freq = freq;
+  h->amp = (double *)amp;
+  h->IP = (double *)IP;
+  h->SL = (double *)SL;
+  h->aa = (double *)aa;
+}
+
+double qo_het_diff_flat(int nifo, const int *labels, double gmst, double fref, double Tobs, int M, int J,
+                        const double *freq, const double *amp, const double *IP, const double *SL, const double *aa,
+                        const double *params, const double *dparams) {
+  qo_het h;
+  qo_spline_init();
+  qo_het_view(&h, nifo, M, J, freq, amp, IP, SL, aa);
+  return qo_het_diff(nifo, labels, gmst, fref, Tobs, &h, params, dparams);
+}
+
+/* efix for B points at once (OpenMP over points): het arrays NULL for the
+ * full-data distance (SN, fmin used), else the heterodyne's (SN unused).
+ * eval B x 11 in place, evec B x 11 x 11, trials B x 11 (may be NULL). */
+void qo_efix_batch_flat(int nifo, const int *labels, double gmst, double fref, double fmin, const double *SN, int N,
+                        double Tobs, int M, int J, const double *freq, const double *amp, const double *IP,
+                        const double *SL, const double *aa, const double *params, int B, const double *min,
+                        const double *max, double etamin, double *eval, const double *evec, double zs, int *trials) {
+  double *Sr[3] = {NULL, NULL, NULL};
+  qo_het h;
+  int i, b;
+  qo_spline_init();
+  if (SN)
+    for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  if (amp) qo_het_view(&h, nifo, M, J, freq, amp, IP, SL, aa);
+#pragma omp parallel for schedule(dynamic)
+  for (b = 0; b < B; b++)
+    qo_efix(nifo, labels, gmst, fref, fmin, Sr, N, Tobs, amp ? &h : NULL, params + (size_t)b * 11, min, max,
+            etamin, eval + (size_t)b * 11, evec + (size_t)b * 121, zs, trials ? trials + (size_t)b * 11 : NULL);
+}
+
+int qo_het_space_flat(int nifo, const int *labels, double gmst, double fref, double fmin, double fmax,
+                      const double *SN, const double *params, const double *min, const double *max, double etamin,
+                      int N, double Tobs, int *out4, int *fgrid, double *eval, double *evec) {
+  double *Sr[3];
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  return qo_het_space(nifo, labels, gmst, fref, fmin, fmax, Sr, params, min, max, etamin, N, Tobs, out4, out4 + 1,
+                      out4 + 2, out4 + 3, fgrid, eval, evec);
+}

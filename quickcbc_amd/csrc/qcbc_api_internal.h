@@ -86,4 +86,36 @@ int het_device(const qcbc_het* h);
 // and d_ev (B x 11 x 11) on stream s, with the heterodyne's scratch; returns
 // after the stream has finished them
 int het_fisher_enqueue(qcbc_het* h, const double* d_params, int B, double* d_ej, double* d_ev, hipStream_t s);
+
+// A heterodyne's grid as the grid-choice kernels read it: frequencies (M),
+// sample powers (4 x M), folded weights (3 x nd x M, first wH / a_ref^2) and
+// the reference amplitude (nd x M), all device memory
+struct HetGridView {
+  int device, M, J, NR, nd;
+  AnalysisConst an;
+  const double *freq, *pl, *wf, *amp;
+};
+int het_grid_view(const qcbc_het* h, HetGridView* out);
+
+// The heterodyne's grid choice (qcbc_gridchoice.cpp), on stream s; PSD rows
+// SN on the host, d_SN (n_det x N/2) on the device.
+// limits (QuickCBC.c:4275-4385): stst[2], ratio (N/2, may be null)
+int limits_impl(qcbc_ctx* c, const double* const* SN, const double* params, int* stst, double* ratio, hipStream_t s);
+// Fisher_All (QuickCBC.c:2695-2817) of B points (host memory)
+int fisher_all_impl(qcbc_ctx* c, const double* d_SN, const double* params, int B, double* fish, int* status,
+                    hipStream_t s);
+// diff (QuickCBC.c:4913-4964), or het_diff (:4966-5053) when h is given, of
+// K displaced points px (K x 11) against start points params[of[k]]
+int distances_impl(qcbc_ctx* c, const double* d_SN, qcbc_het* h, const double* params, int B, const double* px,
+                   const int* of, int K, double* z, hipStream_t s);
+// efix (QuickCBC.c:4387-4567) of B points: ej (B x 11) scaled in place
+int efix_impl(qcbc_ctx* c, const double* d_SN, qcbc_het* h, const double* params, int B, double* ej, const double* ev,
+              const double* mn, const double* mx, double etamin, double zs, int* trials, hipStream_t s);
+// the same on device states, jump sizes and directions (the PT step's)
+int het_efix_device(qcbc_het* h, const double* d_x, int B, double* d_ej, const double* d_ev, const double* mn,
+                    const double* mx, double etamin, double zs, hipStream_t s);
+// het_space (QuickCBC.c:4570-4910): fgrid has room for cap entries
+int het_space_impl(qcbc_ctx* c, const double* const* SN, const double* d_SN, const double* params, const double* mn,
+                   const double* mx, double etamin, int* MN, int* MM, int* M, int* NR, int* fgrid, int cap,
+                   double* eval, double* evec, hipStream_t s);
 }  // namespace qcbc

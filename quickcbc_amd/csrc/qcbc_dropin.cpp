@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/qcbc.h"
@@ -654,6 +655,136 @@ void FisherEvec(double** fish, double* eval, double** evec, int d) {
   for (int i = 0; i < d; i++) {
     eval[i] = J[i];
     std::memcpy(evec[i], E + (size_t)i * d, sizeof(double) * d);
+  }
+}
+
+// ------------------------------------------------- the heterodyne's grid choice
+// The PSD rows of a call on the thread's device buffer (these entry points
+// take SN without D, so the analysis's bound data are left alone).
+static const double* psd_on_device(const struct Net* net, double** SN, int N, const char* where) {
+  const int n2 = N / 2, nd = net->Nifo;
+  for (int d = 0; d < nd; d++)
+    if (!SN[d]) fail(where, QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail(where, st);
+  DropState& s = t_drop;
+  if ((st = drop_buf((size_t)nd * n2))) fail(where, st);
+  for (int d = 0; d < nd; d++)
+    if (hipMemcpyAsync(s.d_buf + (size_t)d * n2, SN[d], sizeof(double) * n2, hipMemcpyHostToDevice, s.stream) !=
+        hipSuccess)
+      fail(where, QCBC_EHIP);
+  if (hipStreamSynchronize(s.stream) != hipSuccess) fail(where, QCBC_EHIP);
+  return s.d_buf;
+}
+
+static qcbc_ctx* grid_ctx(struct Net* net, RealVector* freq, int N, double Tobs, const char* where) {
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || N < 4 || !grid_ok(freq, N / 2, Tobs)) fail(where, QCBC_EINVAL);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc_from(net, N, Tobs), nullptr, nullptr, &st);
+  if (!ctx) fail(where, st);
+  return ctx;
+}
+
+void limits(struct Net* net, double* params, RealVector* freq, double** SN, int* stst, double* ratio, int N,
+            double Tobs) {
+  if (!params || !SN || !stst) fail("limits", QCBC_EINVAL);
+  qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "limits");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int st = limits_impl(ctx, (const double* const*)SN, params, stst, ratio, ctx->stream);
+  if (domain_error("limits", st)) stst[0] = stst[1] = 0;
+}
+
+void Fisher_All(struct Net* net, double** fish, double* params, RealVector* freq, double** SN, int N, double Tobs) {
+  if (!fish || !params || !SN) fail("Fisher_All", QCBC_EINVAL);
+  qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "Fisher_All");
+  const double* d_SN = psd_on_device(net, SN, N, "Fisher_All");
+  double F[11 * 11];
+  int pst = 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int st = fisher_all_impl(ctx, d_SN, params, 1, F, &pst, ctx->stream);
+  if (st) fail("Fisher_All", st);
+  domain_error("Fisher_All", pst);  // the matrix is then 1e4 I
+  for (int i = 0; i < 11; i++) std::memcpy(fish[i], F + 11 * i, sizeof(double) * 11);
+}
+
+double diff(struct Net* net, double* params, double* dparams, RealVector* freq, double** SM, int N, double Tobs) {
+  if (!params || !dparams || !SM) fail("diff", QCBC_EINVAL);
+  qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "diff");
+  const double* d_SN = psd_on_device(net, SM, N, "diff");
+  const int of = 0;
+  double z = 0.0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int st = distances_impl(ctx, d_SN, nullptr, params, 1, dparams, &of, 1, &z, ctx->stream);
+  if (st) fail("diff", st);
+  return z;
+}
+
+double het_diff(struct Net* net, struct Het* het, double* params, double* dparams) {
+  if (!params || !dparams) fail("het_diff", QCBC_EINVAL);
+  qcbc_het* h = het_for(net, het, "het_diff");
+  const int of = 0;
+  double z = 0.0;
+  int st = drop_init();
+  if (st) fail("het_diff", st);
+  st = distances_impl(nullptr, nullptr, h, params, 1, dparams, &of, 1, &z, t_drop.stream);
+  if (st) fail("het_diff", st);
+  return z;
+}
+
+void efix(struct Net* net, struct Het* het, double* params, double* min, double* max, RealVector* freq, double** SM,
+          int N, double Tobs, double* eval, double** evec, double zs, int hflag) {
+  if (!params || !min || !max || !eval || !evec || !(zs > 0.0)) fail("efix", QCBC_EINVAL);
+  double ev[11 * 11];
+  for (int i = 0; i < 11; i++) {
+    if (!evec[i]) fail("efix", QCBC_EINVAL);
+    std::memcpy(ev + 11 * i, evec[i], sizeof(double) * 11);
+  }
+  const double etamin = 0.05;  // ConstCBC.h:57
+  int st;
+  if (hflag == 0) {
+    if (!SM) fail("efix", QCBC_EINVAL);
+    qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "efix");
+    const double* d_SN = psd_on_device(net, SM, N, "efix");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    st = efix_impl(ctx, d_SN, nullptr, params, 1, eval, ev, min, max, etamin, zs, nullptr, ctx->stream);
+  } else {
+    qcbc_het* h = het_for(net, het, "efix");
+    if ((st = drop_init())) fail("efix", st);
+    st = efix_impl(nullptr, nullptr, h, params, 1, eval, ev, min, max, etamin, zs, nullptr, t_drop.stream);
+  }
+  if (st) fail("efix", st);
+}
+
+void het_space(struct Net* net, struct Het* het, double* params, double* min, double* max, RealVector* freq,
+               double** SN, int N, double Tobs) {
+  if (!het || !params || !min || !max || !SN) fail("het_space", QCBC_EINVAL);
+  qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "het_space");
+  const double* d_SN = psd_on_device(net, SN, N, "het_space");
+  std::vector<int> g(N / 2);
+  int MN = 0, MM = 0, M = 0, NR = 0;
+  int st;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    st = het_space_impl(ctx, (const double* const*)SN, d_SN, params, min, max, 0.05, &MN, &MM, &M, &NR, g.data(),
+                        (int)g.size(), nullptr, nullptr, ctx->stream);
+  }
+  if (st == QCBC_EDOM) {
+    domain_error("het_space", st);
+    return;
+  }
+  if (st) fail("het_space", st);
+  het->MN = MN;
+  het->MM = MM;
+  het->J = 2;
+  het->NR = NR;
+  het->M = M;
+  // QuickCBC.c:4887-4891
+  het->freq = CreateRealVector(M);
+  het->fgrid = (int*)std::malloc(sizeof(int) * (M + 1));
+  if (!het->fgrid) fail("het_space", QCBC_ENOMEM);
+  for (int i = 0; i < M; i++) {
+    het->fgrid[i] = g[i];
+    het->freq->data[i] = ((double)g[i]) / Tobs;
   }
 }
 

@@ -67,12 +67,20 @@ __device__ __forceinline__ void pair_of(int n, int* i, int* j) {
 // matrix is 1e4 I and nothing is evaluated (:5467-5469); a record the set-up
 // rejected leaves 1e4 I and its status (the first failing variant's, in
 // Fisher_Het's order of evaluation).
+//
+// The grid may be split over gridDim.y blocks per point (Fisher_All's full
+// band): block (b, s) sums the chunks of samples [s len, (s + 1) len), len a
+// multiple of 64, into partial[(b splits + s) 66 + n] (upper-triangle entry
+// n in row order), and k_fisher_reduce adds the splits in order.  With one
+// split the block writes the matrix itself.  ha null: a_ref = 1 (w is then
+// the per-sample weight itself, as 1/S for the full band).
 template <int ND>
 __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pts, const double* __restrict__ params,
                                                     int B, const double* __restrict__ freq,
                                                     const double* __restrict__ pl, const double* __restrict__ ha,
                                                     const double* __restrict__ w, int M, double* __restrict__ fish,
-                                                    int* __restrict__ status) {
+                                                    int* __restrict__ status, int len,
+                                                    double* __restrict__ partial) {
   __shared__ double V[NV][3][64];       // h22 amplitude, phase, taper per variant and sample
   __shared__ double U[ND][2 * NP][64];  // dA_k (k < 11) and dphi_k per detector and sample
   __shared__ double S[ND][2][64];       // A^2 and wH / a_ref^2 per detector and sample
@@ -85,11 +93,15 @@ __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pt
   for (int v = 0; v < NV && st == ST_OK; v++) st = P[v].status;
   const bool spins = fabs(params[(size_t)b * NP + 2]) < 0.9999 && fabs(params[(size_t)b * NP + 3]) < 0.9999;
   double* F = fish + (size_t)b * NP * NP;
+  const int split = blockIdx.y, splits = gridDim.y;
   if (!spins || st != ST_OK) {  // block-uniform
-    if (t < NP * NP) F[t] = (t % (NP + 1) == 0) ? 1.0e4 : 0.0;
-    if (t == 0) status[b] = spins ? st : ST_OK;
+    if (split == 0) {
+      if (t < NP * NP) F[t] = (t % (NP + 1) == 0) ? 1.0e4 : 0.0;
+      if (t == 0) status[b] = spins ? st : ST_OK;
+    }
     return;
   }
+  const int m_begin = split * len, m_end = min(M, m_begin + len);
   // the pair sums this thread owns: items t and t + 256 of NQ x NPAIR
   int pi0, pj0, pi1 = 0, pj1 = 0;
   const int q0 = t / NPAIR, q1 = (t + 256) / NPAIR;
@@ -97,13 +109,13 @@ __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pt
   const bool two = t + 256 < NQ * NPAIR;
   if (two) pair_of((t + 256) % NPAIR, &pi1, &pj1);
   double acc0 = 0.0, acc1 = 0.0;
-  for (int m0 = 0; m0 < M; m0 += 64) {
+  for (int m0 = m_begin; m0 < m_end; m0 += 64) {
     // the 23 variants' h22 amplitude and phase at the chunk's samples
     for (int v = wave; v < NV; v += 4) {
       const Point& c = P[v];
       const int m = m0 + lane;
       double a = 0.0, p = 0.0, y = 0.0;
-      if (m < M) {
+      if (m < m_end) {
         const double f = freq[m];
         h22_planes(c, 1.0 / c.Msec, f, pl, M, m, &a, &p);
         y = taper(f, c.fs);
@@ -117,7 +129,7 @@ __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pt
     for (int r = wave; r < ND * NP; r += 4) {
       const int d = r / NP, k = r % NP, m = m0 + lane;
       double dA = 0.0, dp = 0.0;
-      if (m < M) {
+      if (m < m_end) {
         const double f = freq[m];
         double aP, xP, aM, xM;
         project(P[2 * k + 1], d, f, V[2 * k + 1][0][lane], V[2 * k + 1][1][lane], V[2 * k + 1][2][lane], &aP, &xP);
@@ -133,12 +145,15 @@ __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pt
     for (int d = wave; d < ND; d += 4) {
       const int m = m0 + lane;
       double A2 = 0.0, wh = 0.0;
-      if (m < M) {
+      if (m < m_end) {
         double a0, x0;
         project(P[0], d, freq[m], V[0][0][lane], V[0][1][lane], V[0][2][lane], &a0, &x0);
-        const double h = ha[(size_t)d * M + m];
         A2 = a0 * a0;
-        wh = w[(size_t)d * M + m] / (h * h);
+        wh = w[(size_t)d * M + m];
+        if (ha) {
+          const double h = ha[(size_t)d * M + m];
+          wh = wh / (h * h);
+        }
       }
       S[d][0][lane] = A2;
       S[d][1][lane] = wh;
@@ -175,10 +190,35 @@ __global__ __launch_bounds__(256) void k_het_fisher(const Point* __restrict__ pt
     pair_of(t, &i, &j);
     double x = red[0][t];
     for (int q = 1; q < NQ; q++) x += red[q][t];
-    F[i * NP + j] = x;
-    F[j * NP + i] = x;
+    if (splits == 1) {
+      F[i * NP + j] = x;
+      F[j * NP + i] = x;
+    } else {
+      partial[((size_t)b * splits + split) * NPAIR + t] = x;
+    }
   }
-  if (t == 0) status[b] = ST_OK;
+  if (t == 0 && split == 0) status[b] = ST_OK;
+}
+
+// The sum over the splits of k_het_fisher's partial entries, in split order,
+// for the points whose matrix k_het_fisher evaluated (the others already
+// hold 1e4 I).  One thread per point and entry.
+__global__ void k_fisher_reduce(const Point* __restrict__ pts, const double* __restrict__ params, int B, int splits,
+                                const double* __restrict__ partial, double* __restrict__ fish) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * NPAIR) return;
+  const int b = i / NPAIR, n = i % NPAIR;
+  const Point* P = pts + (size_t)b * NV;
+  int st = ST_OK;
+  for (int v = 0; v < NV && st == ST_OK; v++) st = P[v].status;
+  const bool spins = fabs(params[(size_t)b * NP + 2]) < 0.9999 && fabs(params[(size_t)b * NP + 3]) < 0.9999;
+  if (!spins || st != ST_OK) return;
+  double x = 0.0;
+  for (int sp = 0; sp < splits; sp++) x += partial[((size_t)b * splits + sp) * NPAIR + n];
+  int r, c;
+  pair_of(n, &r, &c);
+  fish[(size_t)b * NP * NP + r * NP + c] = x;
+  fish[(size_t)b * NP * NP + c * NP + r] = x;
 }
 
 // FisherEvec (QuickCBC.c:2449-2546) of d x d matrices (d <= 11: the
@@ -326,14 +366,24 @@ hipError_t launch_fisher_params(const double* d_params, int B, double* d_q, hipS
 
 hipError_t launch_het_fisher(const Point* pts, const double* d_params, int B, int nd, const double* freq,
                              const double* pl, const double* ha, const double* w, int M, double* fish, int* status,
-                             hipStream_t s) {
+                             hipStream_t s, int splits, double* partial) {
   if (B <= 0) return hipSuccess;
+  if (splits < 1 || (splits > 1 && !partial)) return hipErrorInvalidValue;
+  int len = (M + splits - 1) / splits;
+  len = (len + 63) / 64 * 64;
+  const dim3 grid(B, splits);
   if (nd == 1)
-    hipLaunchKernelGGL(k_het_fisher<1>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+    hipLaunchKernelGGL(k_het_fisher<1>, grid, dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status,
+                       len, partial);
   else if (nd == 2)
-    hipLaunchKernelGGL(k_het_fisher<2>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+    hipLaunchKernelGGL(k_het_fisher<2>, grid, dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status,
+                       len, partial);
   else
-    hipLaunchKernelGGL(k_het_fisher<3>, dim3(B), dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status);
+    hipLaunchKernelGGL(k_het_fisher<3>, grid, dim3(256), 0, s, pts, d_params, B, freq, pl, ha, w, M, fish, status,
+                       len, partial);
+  if (splits > 1)
+    hipLaunchKernelGGL(k_fisher_reduce, dim3((B * NPAIR + 255) / 256), dim3(256), 0, s, pts, d_params, B, splits,
+                       partial, fish);
   return hipGetLastError();
 }
 

@@ -14,12 +14,17 @@ constexpr int FISHER_EVEC_MAX_D = FISHER_NP;  // largest matrix the eigen kernel
 
 // the FISHER_NV parameter sets of each of B points: q is B x FISHER_NV x 11
 hipError_t launch_fisher_params(const double* d_params, int B, double* d_q, hipStream_t s);
+constexpr int FISHER_NPAIR = FISHER_NP * (FISHER_NP + 1) / 2;  // entries of the upper triangle
+
 // Fisher_Het for B points from the set-up records of their FISHER_NV
 // parameter sets (pts, B x FISHER_NV), on a heterodyne's grid: fish is
-// B x 11 x 11, status B
+// B x 11 x 11, status B.  The same kernel gives Fisher_All on a grid of
+// every band bin (ha null, w = 1/S per detector and bin), split over
+// `splits` blocks per point with partial (B x splits x FISHER_NPAIR)
+// scratch when splits > 1.
 hipError_t launch_het_fisher(const Point* pts, const double* d_params, int B, int nd, const double* freq,
                              const double* pl, const double* ha, const double* w, int M, double* fish, int* status,
-                             hipStream_t s);
+                             hipStream_t s, int splits = 1, double* partial = nullptr);
 // FisherEvec of B d x d matrices (d <= FISHER_EVEC_MAX_D): ej B x d, ev B x d x d,
 // sweeps B (optional)
 hipError_t launch_fisher_evec(const double* fish, int B, int d, double* ej, double* ev, int* sweeps, hipStream_t s);

@@ -241,6 +241,21 @@ int qcbc::het_fisher_enqueue(qcbc_het* h, const double* d_params, int B, double*
   return QCBC_OK;
 }
 
+int qcbc::het_grid_view(const qcbc_het* h, HetGridView* out) {
+  if (!h || !out) return QCBC_EINVAL;
+  out->device = h->device;
+  out->M = h->M;
+  out->J = h->J;
+  out->NR = h->NR;
+  out->nd = h->nd;
+  out->an = h->an;
+  out->freq = h->d_freq;
+  out->pl = h->d_pl;
+  out->wf = h->d_wf;
+  out->amp = h->d_amp;
+  return QCBC_OK;
+}
+
 extern "C" {
 
 int qcbc_het_create(qcbc_ctx* ctx, const double* params, const int* fgrid, int M, int J, int MN, int MM,

@@ -416,6 +416,12 @@ hipError_t launch_het_setup(const Point* pts, const double* wa, const double* wp
   return hipGetLastError();
 }
 
+hipError_t launch_het_planes(const double* freq, int M, double* pl, hipStream_t s) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_het_planes, dim3((M + 255) / 256), dim3(256), 0, s, freq, M, pl);
+  return hipGetLastError();
+}
+
 hipError_t launch_het_weights(const double* freq, const double* amp, const double* IP, const double* SL,
                               const double* lc, const double* ls, const double* rc, const double* rs,
                               const double* aa, int M, int J, int nd, double* w, double* wf, double* pl,

@@ -23,6 +23,8 @@ hipError_t launch_het_setup(const Point* pts, const double* wa, const double* wp
                             double* bins, double* partial, double* logLb, double* freq, double* amp, double* phs,
                             double* IP, double* SL, double* lc, double* ls, double* rc, double* rs, double* aa,
                             double* w, double* wf, double* pl, hipStream_t s);
+// the sample powers pl (4 x M: f^(1/6), f^(-1/6), 1/f, log f) of M frequencies
+hipError_t launch_het_planes(const double* freq, int M, double* pl, hipStream_t s);
 // evaluation weights w and wf (3 nd M each) and sample powers pl (4 M) from
 // imported set-up arrays
 hipError_t launch_het_weights(const double* freq, const double* amp, const double* IP, const double* SL,

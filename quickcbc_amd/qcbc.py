@@ -154,6 +154,33 @@ def lib():
         L.Fisher_Het.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dpp, _dp, ctypes.c_double]
         L.FisherEvec.restype = None
         L.FisherEvec.argtypes = [_dpp, _dp, _dpp, ctypes.c_int]
+        # the heterodyne's grid choice
+        L.qcbc_limits.argtypes = [_vp, _dp, _ip, _dp]
+        L.qcbc_fisher_all.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.qcbc_diff.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp]
+        L.qcbc_efix.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip]
+        L.qcbc_het_space.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_double, _ip, _ip, _ip, _ip, _ip, ctypes.c_int,
+                                     _dp, _dp]
+        L.qcbc_het_diff.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp]
+        L.qcbc_het_efix.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                    _ip]
+        L.limits.restype = None
+        L.limits.argtypes = [ctypes.POINTER(Net), _dp, ctypes.POINTER(RealVector), _dpp, _ip, _dp, ctypes.c_int,
+                             ctypes.c_double]
+        L.Fisher_All.restype = None
+        L.Fisher_All.argtypes = [ctypes.POINTER(Net), _dpp, _dp, ctypes.POINTER(RealVector), _dpp, ctypes.c_int,
+                                 ctypes.c_double]
+        L.diff.restype = ctypes.c_double
+        L.diff.argtypes = [ctypes.POINTER(Net), _dp, _dp, ctypes.POINTER(RealVector), _dpp, ctypes.c_int,
+                           ctypes.c_double]
+        L.het_diff.restype = ctypes.c_double
+        L.het_diff.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dp, _dp]
+        L.efix.restype = None
+        L.efix.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dp, _dp, _dp, ctypes.POINTER(RealVector),
+                           _dpp, ctypes.c_int, ctypes.c_double, _dp, _dpp, ctypes.c_double, ctypes.c_int]
+        L.het_space.restype = None
+        L.het_space.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dp, _dp, _dp,
+                                ctypes.POINTER(RealVector), _dpp, ctypes.c_int, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -287,6 +314,61 @@ class Context(_Handle):
         ev = np.zeros((B, d, d))
         _check(lib().qcbc_fisher_evec(self.h, _d(F), B, d, _d(ej), _d(ev)), "qcbc_fisher_evec")
         return ej, ev
+
+    # ---- the heterodyne's grid choice (QuickCBC.c:4275-5053, 2695-2817)
+    def limits(self, params):
+        """limits: ((MN, MM), ratio N/2)."""
+        p = np.ascontiguousarray(params, np.float64)
+        band = np.zeros(2, np.int32)
+        ratio = np.ones(self.an.N // 2)
+        _check(lib().qcbc_limits(self.h, _d(p), _i(band), _d(ratio)), "qcbc_limits")
+        return (int(band[0]), int(band[1])), ratio
+
+    def fisher_all(self, params):
+        """Fisher_All for a batch: (fish B x 11 x 11, status)."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        F = np.zeros((B, 11, 11))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_fisher_all(self.h, _d(p), B, _d(F), _i(st)), "qcbc_fisher_all")
+        return F, st
+
+    def diff(self, params, px, of=None):
+        """diff between start points params[of[k]] and px[k]."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        x = np.ascontiguousarray(px, np.float64).reshape(-1, 11)
+        o = np.ascontiguousarray(np.zeros(x.shape[0]) if of is None else of, np.int32)
+        z = np.zeros(x.shape[0])
+        _check(lib().qcbc_diff(self.h, _d(p), p.shape[0], _d(x), _i(o), x.shape[0], _d(z)), "qcbc_diff")
+        return z
+
+    def efix(self, params, ej, ev, pmin, pmax, zs, etamin=0.05):
+        """efix on the full data for a batch: (scaled ej B x 11, trials B x 11)."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        e = np.array(ej, np.float64).reshape(B, 11)
+        v = np.ascontiguousarray(ev, np.float64).reshape(B, 11, 11)
+        lo = np.ascontiguousarray(pmin, np.float64)
+        hi = np.ascontiguousarray(pmax, np.float64)
+        tr = np.zeros((B, 11), np.int32)
+        _check(lib().qcbc_efix(self.h, _d(p), B, _d(e), _d(v), _d(lo), _d(hi), etamin, zs, _i(tr)), "qcbc_efix")
+        return e, tr
+
+    def het_space(self, params, pmin, pmax, etamin=0.05):
+        """het_space: dict with MN, MM, M, NR, J, fgrid, eval, evec."""
+        p = np.ascontiguousarray(params, np.float64)
+        lo = np.ascontiguousarray(pmin, np.float64)
+        hi = np.ascontiguousarray(pmax, np.float64)
+        cap = self.an.N // 2
+        g = np.zeros(cap, np.int32)
+        ev = np.zeros(11)
+        evec = np.zeros((11, 11))
+        MN, MM, M, NR = (ctypes.c_int() for _ in range(4))
+        _check(lib().qcbc_het_space(self.h, _d(p), _d(lo), _d(hi), etamin, ctypes.byref(MN), ctypes.byref(MM),
+                                    ctypes.byref(M), ctypes.byref(NR), _i(g), cap, _d(ev), _d(evec)),
+               "qcbc_het_space")
+        return dict(MN=MN.value, MM=MM.value, M=M.value, NR=NR.value, J=2, fgrid=g[:M.value].copy(), eval=ev,
+                    evec=evec)
 
     def close(self):
         self._close_deps()
@@ -922,6 +1004,28 @@ class Het(_Handle):
         _check(lib().qcbc_het_time_kernel(self.h, d_params, B, reps, ctypes.byref(ms)), "qcbc_het_time_kernel")
         return ms.value
 
+    def diff(self, params, px, of=None):
+        """het_diff between start points params[of[k]] and px[k] on this heterodyne."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        x = np.ascontiguousarray(px, np.float64).reshape(-1, 11)
+        o = np.ascontiguousarray(np.zeros(x.shape[0]) if of is None else of, np.int32)
+        z = np.zeros(x.shape[0])
+        _check(lib().qcbc_het_diff(self.h, _d(p), p.shape[0], _d(x), _i(o), x.shape[0], _d(z)), "qcbc_het_diff")
+        return z
+
+    def efix(self, params, ej, ev, pmin, pmax, zs, etamin=0.05):
+        """efix with the heterodyne distance for a batch: (scaled ej B x 11, trials B x 11)."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, 11)
+        B = p.shape[0]
+        e = np.array(ej, np.float64).reshape(B, 11)
+        v = np.ascontiguousarray(ev, np.float64).reshape(B, 11, 11)
+        lo = np.ascontiguousarray(pmin, np.float64)
+        hi = np.ascontiguousarray(pmax, np.float64)
+        tr = np.zeros((B, 11), np.int32)
+        _check(lib().qcbc_het_efix(self.h, _d(p), B, _d(e), _d(v), _d(lo), _d(hi), etamin, zs, _i(tr)),
+               "qcbc_het_efix")
+        return e, tr
+
     def close(self):
         self._close_deps()
         if self.h:
@@ -1048,3 +1152,67 @@ def dropin_log_likelihood_intrinsic(an, Drows, SNrows, params):
     p = np.ascontiguousarray(params, np.float64)
     return lib().log_likelihood_intrinsic(ctypes.byref(net), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, an.N,
                                           an.Tobs)
+
+
+# ------------------------------------------- drop-ins of the grid choice
+def dropin_limits(an, SNrows, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.ascontiguousarray(params, np.float64)
+    stst = np.zeros(2, np.int32)
+    ratio = np.ones(an.N // 2)
+    lib().limits(ctypes.byref(net), _d(p), ctypes.byref(fr), SNrows.p, _i(stst), _d(ratio), an.N, an.Tobs)
+    return (int(stst[0]), int(stst[1])), ratio
+
+
+def dropin_fisher_all(an, SNrows, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.ascontiguousarray(params, np.float64)
+    F = RowMatrix(np.zeros((11, 11)))
+    lib().Fisher_All(ctypes.byref(net), F.p, _d(p), ctypes.byref(fr), SNrows.p, an.N, an.Tobs)
+    return F.arr
+
+
+def dropin_diff(an, SNrows, params, dparams):
+    net = make_net(an)
+    fr = make_freq(an)
+    a = np.ascontiguousarray(params, np.float64)
+    b = np.ascontiguousarray(dparams, np.float64)
+    return lib().diff(ctypes.byref(net), _d(a), _d(b), ctypes.byref(fr), SNrows.p, an.N, an.Tobs)
+
+
+def dropin_het_diff(an, view, params, dparams):
+    net = make_net(an)
+    a = np.ascontiguousarray(params, np.float64)
+    b = np.ascontiguousarray(dparams, np.float64)
+    return lib().het_diff(ctypes.byref(net), ctypes.byref(view.s), _d(a), _d(b))
+
+
+def dropin_efix(an, SNrows, view, params, ej, ev, pmin, pmax, zs, hflag):
+    net = make_net(an)
+    fr = make_freq(an)
+    a = np.ascontiguousarray(params, np.float64)
+    e = np.array(ej, np.float64)
+    V = RowMatrix(np.array(ev, np.float64))
+    lo = np.ascontiguousarray(pmin, np.float64)
+    hi = np.ascontiguousarray(pmax, np.float64)
+    lib().efix(ctypes.byref(net), ctypes.byref(view.s) if view is not None else None, _d(a), _d(lo), _d(hi),
+               ctypes.byref(fr), SNrows.p if SNrows is not None else None, an.N, an.Tobs, _d(e), V.p, zs, hflag)
+    return e
+
+
+def dropin_het_space(an, SNrows, params, pmin, pmax):
+    """het_space() into a fresh struct Het: (MN, MM, M, NR, J, fgrid, freq)."""
+    net = make_net(an)
+    fr = make_freq(an)
+    a = np.ascontiguousarray(params, np.float64)
+    lo = np.ascontiguousarray(pmin, np.float64)
+    hi = np.ascontiguousarray(pmax, np.float64)
+    h = HetStruct()
+    lib().het_space(ctypes.byref(net), ctypes.byref(h), _d(a), _d(lo), _d(hi), ctypes.byref(fr), SNrows.p, an.N,
+                    an.Tobs)
+    M = h.M
+    g = np.array([h.fgrid[i] for i in range(M)], np.int32)
+    f = np.array([h.freq.contents.data[i] for i in range(M)])
+    return dict(MN=h.MN, MM=h.MM, M=M, NR=h.NR, J=h.J, fgrid=g, freq=f)
