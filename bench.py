@@ -77,9 +77,12 @@ def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
     over limits()' SNR band, then batched evaluation with the points resident
     on the GPU (per launch: IMRPhenomD set-up of every point, then one wave per
     point over the grid samples)."""
-    A, _ = Q.dropin_fullphaseamp(an, an.freq(), inj)
-    MN, MM = W.snr_band(an, A, SN, Q.dropin_fbegin(inj))
-    g = W.het_grid(MN, MM, an.Tobs)
+    lo, hi = W.prior_box(an)
+    ctx.het_space(inj, lo, hi)
+    t0 = time.perf_counter()
+    sp = ctx.het_space(inj, lo, hi)  # the reference's grid choice (QuickCBC.c:4570-4910)
+    space_ms = (time.perf_counter() - t0) * 1e3
+    g, MN, MM = sp["fgrid"], sp["MN"], sp["MM"]
     t0 = time.perf_counter()
     het = Q.Het(ctx, params=inj, fgrid=g, J=2, MN=MN, MM=MM)
     setup_ms = (time.perf_counter() - t0) * 1e3
@@ -97,9 +100,15 @@ def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
     kernel_ms = het.time_kernel(dp.ptr, B, launches)
     Lh, sh = het.loglike(P)
     Lf, _, _ = ctx.loglike(P, Q.FP64)
+    # het_space's own accuracy target: the eleven points efix put at distance 30
+    Lhd, _ = het.loglike(sp["dparams"])
+    Lfd, _, _ = ctx.loglike(sp["dparams"], Q.FP64)
+    near = float(np.abs(Lhd - Lfd).max())
     out = {"logl_per_s": B * launches / T, "launch_ms": T / launches * 1e3, "kernel_ms": kernel_ms,
            "kernel": "k_het_ll<%d>" % an.nifo, "points": B, "grid_samples": het.M, "regions": het.NR,
-           "band_bins": [MN, MM], "setup_ms": setup_ms,
+           "grid": "het_space (limits, Fisher_All, FisherEvec, efix to 30, tolerance 0.003)",
+           "band_bins": [MN, MM], "het_space_ms": space_ms, "setup_ms": setup_ms,
+           "het_vs_full_abs_dlogl_at_efix_points_max": near,
            "het_vs_full_abs_dlogl_median": percentile(np.abs(Lh - Lf), 50)}
     out["fisher"] = fisher_bench(Q, het, P, dp, launches, no_cpu, an, D, SN, inj, g, MN, MM)
     out["pt_update"] = update_bench(Q, W, an, ctx, het, inj, P.shape[0], no_cpu, D, SN, g, MN, MM)
@@ -172,7 +181,7 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
     lo, hi = W.prior_box(an)
     n_temps, n_chains = 32, B // 32
     init = W.ensemble_in_prior(inj, n_temps * n_chains, lo, hi, seed=99)
-    desc = Q.update_desc(lo, hi)
+    desc = Q.update_desc(lo, hi, efix_zs=1.0)  # efix of the Fisher jumps, QuickCBC.c:1927
     t0 = time.perf_counter()
     pt = Q.PTUpdate(ctx, het, n_temps, n_chains, 1.15, init, desc, seed=5)
     create_ms = (time.perf_counter() - t0) * 1e3
@@ -364,9 +373,9 @@ def main():
         return
 
     if args.only_update:
-        A, _ = Q.dropin_fullphaseamp(an, an.freq(), inj)
-        MN, MM = W.snr_band(an, A, SN, Q.dropin_fbegin(inj))
-        g = W.het_grid(MN, MM, an.Tobs)
+        lo, hi = W.prior_box(an)
+        sp = ctx.het_space(inj, lo, hi)
+        g, MN, MM = sp["fgrid"], sp["MN"], sp["MM"]
         het = Q.Het(ctx, params=inj, fgrid=g, J=2, MN=MN, MM=MM)
         dp = Q.DeviceBuffer.from_array(np.ascontiguousarray(init, np.float64))
         r = {"fisher": fisher_bench(Q, het, init, dp, 100, True, an, D, SN, inj, g, MN, MM),

@@ -391,15 +391,18 @@ int qcbc_pt_create_het(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, 
  * sky-ring density), inside the prior box with eta in [etamin, 1/4].  The
  * sky ring needs two detectors (c = 0 turns it off: the reference's
  * alternative setting, :2256-2260).  The Fisher jumps (Fisher_Het +
- * FisherEvec) are computed at creation and by qcbc_pt_refresh_fisher (the
- * reference refreshes them every 50000 iterations, :2019-2027; efix is not
- * built). */
+ * FisherEvec, then efix when desc->efix_zs > 0) are computed at creation and
+ * by qcbc_pt_refresh_fisher (the reference refreshes them every 50000
+ * iterations, :2019-2027). */
 typedef struct {
   double min[11], max[11]; /* prior box (update()'s min, max) */
   double a, b, c;          /* proposal thresholds: 0.5, 0.3, 0.2 */
   double etamin;           /* lower limit of eta (0.05, ConstCBC.h:57) */
   int n_hist;              /* history rows per temperature (NH = 1000, ConstCBC.h:72) */
   int hist_every;          /* update steps between history entries (20, QuickCBC.c:2071) */
+  double efix_zs;          /* > 0: after each Fisher refresh, efix with the heterodyne
+                              distance to this target (the reference's 1.0,
+                              QuickCBC.c:1927, :2025); 0: the jumps as FisherEvec gives them */
 } qcbc_update_desc;
 int qcbc_pt_create_update(qcbc_ctx *ctx, qcbc_het *het, int n_temps, int n_chains, double ladder,
                           const double *init, const qcbc_update_desc *desc, uint64_t seed, int groups,
@@ -583,10 +586,13 @@ int qcbc_diff(qcbc_ctx *ctx, const double *params, int B, const double *px, cons
 int qcbc_efix(qcbc_ctx *ctx, const double *params, int B, double *ej, const double *ev, const double *pmin,
               const double *pmax, double etamin, double zs, int *trials);
 /* het_space at params: MN, MM, M = NR J + 1 (J = 2) grid bins into fgrid (room
- * for cap; QCBC_ERANGE if more), and efix's jump sizes and directions
- * (eval 11, evec 11 x 11, may be NULL).  qcbc_het_create then sets it up. */
+ * for cap; QCBC_ERANGE if more), efix's jump sizes and directions (eval 11,
+ * evec 11 x 11) and the eleven displaced points the grid was fitted to
+ * (dparams 11 x 11); the last three may be NULL.  qcbc_het_create then
+ * sets the heterodyne up. */
 int qcbc_het_space(qcbc_ctx *ctx, const double *params, const double *pmin, const double *pmax, double etamin,
-                   int *MN, int *MM, int *M, int *NR, int *fgrid, int cap, double *eval, double *evec);
+                   int *MN, int *MM, int *M, int *NR, int *fgrid, int cap, double *eval, double *evec,
+                   double *dparams);
 /* het_diff and efix (het_diff distance) on a heterodyne, as above. */
 int qcbc_het_diff(qcbc_het *het, const double *params, int B, const double *px, const int *of, int K, double *z);
 int qcbc_het_efix(qcbc_het *het, const double *params, int B, double *ej, const double *ev, const double *pmin,

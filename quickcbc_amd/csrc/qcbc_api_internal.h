@@ -117,5 +117,5 @@ int het_efix_device(qcbc_het* h, const double* d_x, int B, double* d_ej, const d
 // het_space (QuickCBC.c:4570-4910): fgrid has room for cap entries
 int het_space_impl(qcbc_ctx* c, const double* const* SN, const double* d_SN, const double* params, const double* mn,
                    const double* mx, double etamin, int* MN, int* MM, int* M, int* NR, int* fgrid, int cap,
-                   double* eval, double* evec, hipStream_t s);
+                   double* eval, double* evec, double* dparams, hipStream_t s);
 }  // namespace qcbc

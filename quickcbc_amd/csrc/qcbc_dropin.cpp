@@ -766,7 +766,7 @@ void het_space(struct Net* net, struct Het* het, double* params, double* min, do
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     st = het_space_impl(ctx, (const double* const*)SN, d_SN, params, min, max, 0.05, &MN, &MM, &M, &NR, g.data(),
-                        (int)g.size(), nullptr, nullptr, ctx->stream);
+                        (int)g.size(), nullptr, nullptr, nullptr, ctx->stream);
   }
   if (st == QCBC_EDOM) {
     domain_error("het_space", st);

@@ -477,7 +477,7 @@ int het_efix_device(qcbc_het* h, const double* d_x, int B, double* d_ej, const d
 // the device.  fgrid has room for cap entries.
 int het_space_impl(qcbc_ctx* c, const double* const* SN, const double* d_SN, const double* params, const double* mn,
                    const double* mx, double etamin, int* MN_out, int* MM_out, int* M_out, int* NR_out, int* fgrid,
-                   int cap, double* eval_out, double* evec_out, hipStream_t s) {
+                   int cap, double* eval_out, double* evec_out, double* dpar_out, hipStream_t s) {
   constexpr int J = 2;
   const double tol = 0.003;
   const int nd = c->desc.n_det, n2 = c->nhalf;
@@ -511,6 +511,7 @@ int het_space_impl(qcbc_ctx* c, const double* const* SN, const double* d_SN, con
   if (eval_out) std::memcpy(eval_out, ej, sizeof(ej));
   if (evec_out) std::memcpy(evec_out, ev, sizeof(ev));
   for (int i = 0; i < NP; i++) displace(params, ev + NP * i, ej[i], mn, mx, etamin, dpar + NP * i);
+  if (dpar_out) std::memcpy(dpar_out, dpar, sizeof(dpar));
   // 3. the reference point's and the displaced points' waveforms on [0, MM)
   std::vector<double> A((size_t)(NP + 1) * nd * MM), P((size_t)(NP + 1) * nd * MM);
   {
@@ -706,7 +707,8 @@ int qcbc_efix(qcbc_ctx* c, const double* params, int B, double* ej, const double
 }
 
 int qcbc_het_space(qcbc_ctx* c, const double* params, const double* pmin, const double* pmax, double etamin,
-                   int* MN, int* MM, int* M, int* NR, int* fgrid, int cap, double* eval, double* evec) {
+                   int* MN, int* MM, int* M, int* NR, int* fgrid, int cap, double* eval, double* evec,
+                   double* dparams) {
   if (!c || !params || !MN || !MM || !M || !NR || !fgrid || cap < 3) return QCBC_EINVAL;
   int st = check_box(pmin, pmax, etamin);
   if (st) return st;
@@ -718,7 +720,7 @@ int qcbc_het_space(qcbc_ctx* c, const double* params, const double* pmin, const 
   st = host_psd(c, SNh, rows);
   if (st) return st;
   return het_space_impl(c, rows.data(), c->d_SN, params, pmin, pmax, etamin, MN, MM, M, NR, fgrid, cap, eval, evec,
-                        c->stream);
+                        dparams, c->stream);
 }
 
 int qcbc_het_diff(qcbc_het* h, const double* params, int B, const double* px, const int* of, int K, double* z) {

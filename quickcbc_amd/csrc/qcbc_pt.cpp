@@ -69,6 +69,7 @@ struct qcbc_pt {
   // statistics
   bool upd = false;
   UpdateConst uc{};
+  double efix_zs = 0.0;  // > 0: efix of the Fisher jumps after each refresh
   double *d_ej = nullptr, *d_ev = nullptr, *d_hist = nullptr;
   unsigned long long *d_hcount = nullptr, *d_ustats = nullptr;
   int* d_typ = nullptr;
@@ -178,7 +179,8 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
                      const double* sigma, uint64_t seed, int precision, int groups, qcbc_pt** out,
                      const qcbc_update_desc* ud = nullptr) {
   if (ud && (!het || !(ud->a >= ud->b && ud->b >= ud->c && ud->c >= 0.0 && ud->a <= 1.0) || ud->n_hist < 2 ||
-             ud->hist_every < 1 || !(ud->etamin >= 0.0) || (ud->c > 0.0 && ctx && ctx->desc.n_det < 2)))
+             ud->hist_every < 1 || !(ud->etamin >= 0.0) || !(ud->efix_zs >= 0.0 && ud->efix_zs < 1e300) ||
+             (ud->efix_zs > 0.0 && !(ud->etamin > 0.0)) || (ud->c > 0.0 && ctx && ctx->desc.n_det < 2)))
     return QCBC_EINVAL;  // (a sky ring needs two detectors)
   if (ud)
     for (int i = 0; i < 11; i++)
@@ -214,6 +216,7 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
     pt->uc.etamin = ud->etamin;
     pt->uc.n_hist = ud->n_hist;
     pt->uc.hist_every = ud->hist_every;
+    pt->efix_zs = ud->efix_zs;
     pt->uc.gmst = ctx->desc.gmst;
     pt->uc.n_det = ctx->desc.n_det;
     for (int d = 0; d < 3; d++) pt->uc.labels[d] = d < ctx->desc.n_det ? ctx->desc.labels[d] : 0;
@@ -316,6 +319,11 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
     if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
     st = het_fisher_enqueue(het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, s);
     if (st) return fail(st);
+    if (pt->efix_zs > 0.0) {
+      st = het_efix_device(het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, pt->uc.min, pt->uc.max, pt->uc.etamin,
+                           pt->efix_zs, s);
+      if (st) return fail(st);
+    }
     st = update_propose_all(pt, s);
     if (st) return fail(st);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(QCBC_EHIP);
@@ -362,6 +370,11 @@ int qcbc_pt_refresh_fisher(qcbc_pt* pt) {
   if (st) return st;
   st = het_fisher_enqueue(pt->het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, c->stream);
   if (st) return st;
+  if (pt->efix_zs > 0.0) {
+    st = het_efix_device(pt->het, pt->d_x, pt->P, pt->d_ej, pt->d_ev, pt->uc.min, pt->uc.max, pt->uc.etamin,
+                         pt->efix_zs, c->stream);
+    if (st) return st;
+  }
   // the pending proposals were made with the old directions (the reference
   // refreshes before its update, QuickCBC.c:2019-2027)
   st = update_propose_all(pt, c->stream);

@@ -160,7 +160,7 @@ def lib():
         L.qcbc_diff.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp]
         L.qcbc_efix.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip]
         L.qcbc_het_space.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_double, _ip, _ip, _ip, _ip, _ip, ctypes.c_int,
-                                     _dp, _dp]
+                                     _dp, _dp, _dp]
         L.qcbc_het_diff.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp]
         L.qcbc_het_efix.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                     _ip]
@@ -355,7 +355,7 @@ class Context(_Handle):
         return e, tr
 
     def het_space(self, params, pmin, pmax, etamin=0.05):
-        """het_space: dict with MN, MM, M, NR, J, fgrid, eval, evec."""
+        """het_space: dict with MN, MM, M, NR, J, fgrid, eval, evec and the displaced points dparams."""
         p = np.ascontiguousarray(params, np.float64)
         lo = np.ascontiguousarray(pmin, np.float64)
         hi = np.ascontiguousarray(pmax, np.float64)
@@ -363,12 +363,13 @@ class Context(_Handle):
         g = np.zeros(cap, np.int32)
         ev = np.zeros(11)
         evec = np.zeros((11, 11))
+        dpar = np.zeros((11, 11))
         MN, MM, M, NR = (ctypes.c_int() for _ in range(4))
         _check(lib().qcbc_het_space(self.h, _d(p), _d(lo), _d(hi), etamin, ctypes.byref(MN), ctypes.byref(MM),
-                                    ctypes.byref(M), ctypes.byref(NR), _i(g), cap, _d(ev), _d(evec)),
+                                    ctypes.byref(M), ctypes.byref(NR), _i(g), cap, _d(ev), _d(evec), _d(dpar)),
                "qcbc_het_space")
         return dict(MN=MN.value, MM=MM.value, M=M.value, NR=NR.value, J=2, fgrid=g[:M.value].copy(), eval=ev,
-                    evec=evec)
+                    evec=evec, dparams=dpar)
 
     def close(self):
         self._close_deps()
@@ -620,14 +621,16 @@ class UpdateDesc(ctypes.Structure):
     """qcbc_update_desc: prior box and proposal mix of QuickCBC's own PT step."""
     _fields_ = [("min", ctypes.c_double * 11), ("max", ctypes.c_double * 11), ("a", ctypes.c_double),
                 ("b", ctypes.c_double), ("c", ctypes.c_double), ("etamin", ctypes.c_double),
-                ("n_hist", ctypes.c_int), ("hist_every", ctypes.c_int)]
+                ("n_hist", ctypes.c_int), ("hist_every", ctypes.c_int), ("efix_zs", ctypes.c_double)]
 
 
-def update_desc(pmin, pmax, a=0.5, b=0.3, c=0.2, etamin=0.05, n_hist=1000, hist_every=20):
+def update_desc(pmin, pmax, a=0.5, b=0.3, c=0.2, etamin=0.05, n_hist=1000, hist_every=20, efix_zs=0.0):
+    """efix_zs = 1.0 rescales the Fisher jumps as MCMC_all does (QuickCBC.c:1927)."""
     d = UpdateDesc()
     d.min[:] = [float(v) for v in pmin]
     d.max[:] = [float(v) for v in pmax]
     d.a, d.b, d.c, d.etamin, d.n_hist, d.hist_every = a, b, c, etamin, n_hist, hist_every
+    d.efix_zs = efix_zs
     return d
 
 

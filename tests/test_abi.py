@@ -99,21 +99,24 @@ def test_point_record_mirror_matches_the_struct(tmp_path):
 
 
 def test_reference_struct_mirrors_match_the_header(tmp_path):
-    """struct Net and struct Het as the ctypes bindings lay them out are the
-    header's (QuickCBC.h:13-50), field by field."""
-    from quickcbc_amd.qcbc import HetStruct, Net
+    """struct Net and struct Het (QuickCBC.h:13-50), qcbc_desc and
+    qcbc_update_desc as the ctypes bindings lay them out are the header's,
+    field by field."""
+    from quickcbc_amd.qcbc import Desc, HetStruct, Net, UpdateDesc
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "qcbc.h"', "int main(void) {"]
-    for tag, cls in (("Net", Net), ("Het", HetStruct)):
+    types = (("Net", "struct Net", Net), ("Het", "struct Het", HetStruct), ("Desc", "qcbc_desc", Desc),
+             ("UpdateDesc", "qcbc_update_desc", UpdateDesc))
+    for tag, ctype, cls in types:
         for f in cls._fields_:
-            src.append('  printf("%s.%s %%zu\\n", offsetof(struct %s, %s));' % (tag, f[0], tag, f[0]))
-        src.append('  printf("%s.sizeof %%zu\\n", sizeof(struct %s));' % (tag, tag))
+            src.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (tag, f[0], ctype, f[0]))
+        src.append('  printf("%s.sizeof %%zu\\n", sizeof(%s));' % (tag, ctype))
     src.append("  return 0;\n}")
     c, exe = tmp_path / "off.c", tmp_path / "off"
     c.write_text("\n".join(src) + "\n")
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(c)], check=True)
     got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True)
                .stdout.splitlines())
-    for tag, cls in (("Net", Net), ("Het", HetStruct)):
+    for tag, _, cls in types:
         assert int(got[tag + ".sizeof"]) == ctypes.sizeof(cls), tag
         for f in cls._fields_:
             assert getattr(cls, f[0]).offset == int(got[tag + "." + f[0]]), (tag, f[0])

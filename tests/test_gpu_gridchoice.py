@@ -161,6 +161,9 @@ def test_het_space(setup):
         assert got[k] == sp[k], k
     assert np.array_equal(got["fgrid"], sp["fgrid"])
     assert np.allclose(got["eval"], sp["eval"], rtol=1e-6, atol=0)
+    # the displaced points are its own jumps applied as the reference applies them
+    Pd = np.array([O.displace(inj, got["evec"][i], got["eval"][i], lo, hi) for i in range(11)])
+    assert np.array_equal(got["dparams"], Pd)
     d = Q.dropin_het_space(an, Q.RowMatrix(SN), inj, lo, hi)
     assert d["M"] == sp["M"] and d["J"] == 2 and np.array_equal(d["fgrid"], sp["fgrid"])
     assert np.array_equal(d["freq"], sp["fgrid"] / an.Tobs)

@@ -155,3 +155,30 @@ def test_invalid_descriptions_are_rejected(setup):
     bad_hi[3] = lo[3]
     with pytest.raises(RuntimeError):
         Q.PTUpdate(c, g, N_TEMPS, N_CHAINS, LADDER, init, Q.update_desc(lo, bad_hi), seed=SEED)
+
+
+def test_efix_rescales_the_fisher_jumps(setup):
+    """With efix_zs = 1 the ensemble's jumps are FisherEvec's rescaled by
+    efix on the heterodyne distance (MCMC_all, QuickCBC.c:1922-1928), at
+    creation and on refresh; the oracle's efix gives the same scales."""
+    an, h, ctx, g, lo, hi, init = setup
+    desc = Q.update_desc(lo, hi, c=0.2, n_hist=N_HIST, hist_every=EVERY, efix_zs=1.0)
+    pt = Q.PTUpdate(ctx, g, N_TEMPS, N_CHAINS, LADDER, init, desc, seed=SEED, groups=2)
+    x, _, _, _ = pt.get()
+    ej, ev, _, _, _, _ = pt.get_update()
+    F, ej0, ev0, _ = g.fisher(x)
+    assert np.array_equal(ev, ev0)
+    want, _ = g.efix(x, ej0, ev0, lo, hi, 1.0)
+    assert np.array_equal(ej, want)
+    eo, _ = O.efix(an.labels, an.gmst, an.fref, an.Tobs, x, ej0, ev0, lo, hi, 1.0, het=h)
+    close = np.isclose(ej, eo, rtol=1e-6, atol=0).all(1)
+    assert close.mean() >= 0.9, close
+    assert (ej <= ej0 * 1.0e4).all() and (ej >= 0).all()
+    pt.run(6)
+    pt.refresh_fisher()
+    x, _, _, _ = pt.get()
+    ej, ev, _, _, _, _ = pt.get_update()
+    F, ej0, ev0, _ = g.fisher(x)
+    want, _ = g.efix(x, ej0, ev0, lo, hi, 1.0)
+    assert np.array_equal(ev, ev0) and np.array_equal(ej, want)
+    pt.close()
