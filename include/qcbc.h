@@ -207,6 +207,26 @@ void efix(struct Net *net, struct Het *het, double *params, double *min, double 
 void het_space(struct Net *net, struct Het *het, double *params, double *min, double *max, RealVector *freq,
                double **SN, int N, double Tobs);
 
+/* The search likelihoods of CBC_start.  params: the reference-frame
+ * parameters of templates() (7 + 3 (Nifo - 1)), moved in place onto the
+ * maximum as the reference moves them.  The band is the drop-in band
+ * (qcbc_set_band).  pflag = 1's diagnostic prints are not made. */
+/* Replaces log_likelihood_APmax, QuickCBC.c:5672-5762 (prototype QuickCBC.h:112). */
+double log_likelihood_APmax(struct Net *net, double **D, double *params, RealVector *freq, double **SN, int N,
+                            double Tobs);
+/* Replaces log_likelihood_max, QuickCBC.c:5765-6231 (prototype QuickCBC.h:113):
+ * tc in [tmin, tmax]; uses net->tmax (PDwave's mask) and net->delays. */
+double log_likelihood_max(struct Net *net, double **D, double *params, RealVector *freq, double **SN, int N,
+                          double Tobs, double tmin, double tmax, int pflag);
+/* Replaces log_likelihood_penalized, QuickCBC.c:6233-6408 (prototype QuickCBC.h:114):
+ * one detector's D, twave (half-complex N) and SN (N/2). */
+double log_likelihood_penalized(double *params, double *D, double *twave, double *SN, double *Ax, double *Px, int N,
+                                double Tobs, int pflag);
+/* Replaces PDwave, QuickCBC.c:3448-3525 (prototype QuickCBC.h:160). */
+void PDwave(double *wavef, RealVector *freq, double tcmin, double *params, int N);
+/* Replaces pbt_shift, Utilities.c:687-713 (prototype Utilities.h:37): N a power of 2. */
+void pbt_shift(double *corr, double *corrf, double *data1, double *data2, double *Sn, int imin, int imax, int N);
+
 /* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
 AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
 void DestroyAmpPhaseFDWaveform(AmpPhaseFDWaveform *wf);
@@ -593,6 +613,23 @@ int qcbc_efix(qcbc_ctx *ctx, const double *params, int B, double *ej, const doub
 int qcbc_het_space(qcbc_ctx *ctx, const double *params, const double *pmin, const double *pmax, double etamin,
                    int *MN, int *MM, int *M, int *NR, int *fgrid, int cap, double *eval, double *evec,
                    double *dparams);
+/* ------------------------------------------------------------------ *
+ * Part 8: search likelihoods (QuickCBC.c:5672-6408)
+ * ------------------------------------------------------------------ */
+/* log_likelihood_APmax for B points (params B x (7 + 3 (n_det - 1)),
+ * reference-frame, host memory, moved in place): templates, the penalized
+ * likelihood per detector (8 Hz bands, those with excess power cut) with
+ * amplitude and phase maximised, and a second pass for points that moved.
+ * status[b] (may be NULL) 0 or QCBC_EDOM. */
+int qcbc_loglike_apmax(qcbc_ctx *ctx, double *params, int B, double *logL, int *status);
+/* log_likelihood_max for B points (as above): tc set to Tobs/2, the template
+ * PDwave (masked at fbegin for tc = net_tmax), its matched filter over every
+ * lag in [tmin, tmax] (one backward FFT per point and detector), the peaks,
+ * each refined by the penalized likelihood, and the network maximum over
+ * the detectors' peaks within delays (n_det x n_det light travel times). */
+int qcbc_loglike_max(qcbc_ctx *ctx, double *params, int B, double tmin, double tmax, double net_tmax,
+                     const double *delays, double *logL, int *status);
+
 /* het_diff and efix (het_diff distance) on a heterodyne, as above. */
 int qcbc_het_diff(qcbc_het *het, const double *params, int B, const double *px, const int *of, int K, double *z);
 int qcbc_het_efix(qcbc_het *het, const double *params, int B, double *ej, const double *ev, const double *pmin,

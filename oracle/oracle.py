@@ -641,3 +641,88 @@ def het_space(labels, gmst, fref, fmin, fmax, SN, N, Tobs, params, pmin, pmax, e
                              etamin, N, Tobs, _i(out4), _i(fgrid), _d(ev), _d(evec))
     MN, MM, M, NR = (int(x) for x in out4)
     return st, dict(MN=MN, MM=MM, M=M, NR=NR, J=2, fgrid=fgrid[:M].copy(), eval=ev, evec=evec)
+
+
+# ------------------------------------------------- search likelihoods
+LIGHT_TIMES = {(0, 1): 0.0106, (0, 2): 0.0278, (1, 2): 0.0257}  # ConstCBC.h:19-21
+
+
+def delays(labels):
+    """net->delays as QuickCBC's set-up fills it (QuickCBC.c:7776-7790): nifo x nifo."""
+    n = len(labels)
+    d = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                d[i, j] = LIGHT_TIMES[tuple(sorted((labels[i], labels[j])))]
+    return d
+
+
+def penalized(params, D, twave, SN, fmin, fmax, N, Tobs):
+    """log_likelihood_penalized (QuickCBC.c:6233) of one detector: (logL, A, P)."""
+    L = lib()
+    L.qo_penalized_flat.restype = ctypes.c_double
+    L.qo_penalized_flat.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_double, _dp]
+    a = [np.ascontiguousarray(x, np.float64) for x in (params, D, twave, SN)]
+    AP = np.zeros(2)
+    v = L.qo_penalized_flat(*[_d(x) for x in a], fmin, fmax, N, Tobs, _d(AP))
+    return v, AP[0], AP[1]
+
+
+def pdwave(N, Tobs, tcmin, params, fref=100.0):
+    """PDwave (QuickCBC.c:3448): (status, half-complex N)."""
+    L = lib()
+    L.qo_pdwave_flat.restype = ctypes.c_int
+    L.qo_pdwave_flat.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double]
+    w = np.zeros(N)
+    p = np.ascontiguousarray(params, np.float64)
+    st = L.qo_pdwave_flat(_d(w), N, Tobs, tcmin, _d(p), fref)
+    return st, w
+
+
+def pbt_shift(data1, data2, Sn, imin, imax, N):
+    """pbt_shift (Utilities.c:687): (corr, corrf)."""
+    L = lib()
+    L.qo_pbt_shift.restype = None
+    L.qo_pbt_shift.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    a = [np.ascontiguousarray(x, np.float64) for x in (data1, data2, Sn)]
+    c, cf = np.zeros(N), np.zeros(N)
+    L.qo_pbt_shift(_d(c), _d(cf), *[_d(x) for x in a], imin, imax, N)
+    return c, cf
+
+
+def apmax(nifo, fref, fmin, fmax, D, SN, N, Tobs, params):
+    """log_likelihood_APmax (QuickCBC.c:5672) for a batch: (logL, moved params, status)."""
+    L = lib()
+    L.qo_apmax_batch_flat.restype = None
+    L.qo_apmax_batch_flat.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp, ctypes.c_int,
+                                                                             ctypes.c_double, _dp, ctypes.c_int, _dp,
+                                                                             _ip]
+    ips = 7 + 3 * (nifo - 1)
+    p = np.array(params, np.float64).reshape(-1, ips)
+    B = p.shape[0]
+    logL = np.zeros(B)
+    st = np.zeros(B, np.int32)
+    L.qo_apmax_batch_flat(nifo, fref, fmin, fmax, _d(np.ascontiguousarray(D, np.float64)),
+                          _d(np.ascontiguousarray(SN, np.float64)), N, Tobs, _d(p), B, _d(logL), _i(st))
+    return logL, p, st
+
+
+def llmax(labels, fref, fmin, fmax, D, SN, N, Tobs, tmin, tmax, net_tmax, params):
+    """log_likelihood_max (QuickCBC.c:5765) for a batch: (logL, moved params, status)."""
+    L = lib()
+    L.qo_llmax_batch_flat.restype = None
+    L.qo_llmax_batch_flat.argtypes = ([ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp, ctypes.c_int] +
+                                      [ctypes.c_double] * 4 + [_dp, _dp, ctypes.c_int, _dp, _ip])
+    nifo = len(labels)
+    ips = 7 + 3 * (nifo - 1)
+    p = np.array(params, np.float64).reshape(-1, ips)
+    B = p.shape[0]
+    logL = np.zeros(B)
+    st = np.zeros(B, np.int32)
+    dl = np.ascontiguousarray(delays(labels), np.float64)
+    L.qo_llmax_batch_flat(nifo, fref, fmin, fmax, _d(np.ascontiguousarray(D, np.float64)),
+                          _d(np.ascontiguousarray(SN, np.float64)), N, Tobs, tmin, tmax, net_tmax, _d(dl), _d(p), B,
+                          _d(logL), _i(st))
+    return logL, p, st

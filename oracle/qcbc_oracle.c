@@ -2987,3 +2987,485 @@ int qo_het_space_flat(int nifo, const int *labels, double gmst, double fref, dou
   return qo_het_space(nifo, labels, gmst, fref, fmin, fmax, Sr, params, min, max, etamin, N, Tobs, out4, out4 + 1,
                       out4 + 2, out4 + 3, fgrid, eval, evec);
 }
+
+/* ------------------------------------------------ search likelihoods (f3) */
+#define QO_FBW 8.0    /* band width of the penalized likelihood (ConstCBC.h:26) */
+#define QO_RHOCUT 4.0 /* its band threshold (ConstCBC.h:27) */
+#define QO_LPMIN 9.0  /* least logL of a peak (ConstCBC.h:25) */
+#define QO_DTP 0.05   /* least time between peaks (ConstCBC.h:24) */
+#define QO_NX 7
+
+/* log_likelihood_penalized, QuickCBC.c:6233-6408, for one detector (its
+ * pflag = 1 prints are left out).  Over [max(fmin, fbegin) Tobs, min(fmax
+ * Tobs, N/2)) in bands of bw = (int)(8 Tobs) bins: per band the template
+ * power y = sum 4|h|^2/S and x = |sum 4 D conj(h)/S|; rho = (x - y)/sqrt(y) (0
+ * where y = 0).  A band's sums are recorded when the scan enters the next
+ * band, so the last band (and any band the scan never leaves) keeps rho = 0
+ * and y = 0, where the reference leaves y unset.  The bands from the first
+ * with y > 0.1 to the first after it with y < 0.1 are kept, less those with
+ * rho > 4; over them HH = 1e-4 + sum 4|h|^2/S, HC + i HS = sum 4 D conj(h)/S, and
+ * A = |HC + i HS| / HH, P = atan2(HS, HC), logL = |HC + i HS|^2 / HH / 2. */
+double qo_log_likelihood_penalized(const double *params, const double *D, const double *twave, const double *SN,
+                                   double fmin, double fmax, int N, double Tobs, double *Ax, double *Px) {
+  const int bw = (int)(QO_FBW * Tobs);
+  const double fs = qo_fbegin(params);
+  int imin = (fmin > fs) ? (int)(Tobs * fmin) : (int)(Tobs * fs);
+  int imax = (int)(Tobs * fmax), nb, i, j, k, kstart, kend, started;
+  double *rho, *snr, x = 0.0, xs = 0.0, y = 0.0, HH, HC, HS, HD;
+  if (imax > N / 2) imax = N / 2;
+  nb = (imax - imin) / bw;
+  if (nb < 1) nb = 1;
+  rho = calloc((size_t)nb + 1, sizeof(double));
+  snr = calloc((size_t)nb + 1, sizeof(double));
+  j = 0;
+  k = -1;
+  for (i = imin; i < imax; i++) {
+    if (j % bw == 0) { /* entering the next band: record the last one */
+      if (k >= 0 && k < nb) {
+        if (y > 0.0) rho[k] = (sqrt(x * x + xs * xs) - y) / sqrt(y);
+        snr[k] = y;
+      }
+      k++;
+      x = xs = y = 0.0;
+    }
+    x += 4.0 * (D[i] * twave[i] + D[N - i] * twave[N - i]) / SN[i];
+    xs += 4.0 * (D[i] * twave[N - i] - D[N - i] * twave[i]) / SN[i];
+    y += 4.0 * (twave[i] * twave[i] + twave[N - i] * twave[N - i]) / SN[i];
+    j++;
+  }
+  kstart = 0;
+  started = 0;
+  for (k = 0; k < nb; k++)
+    if (snr[k] > 0.1 && !started) {
+      started = 1;
+      kstart = k;
+    }
+  kend = nb;
+  for (k = kstart; k < nb; k++)
+    if (snr[k] < 0.1) {
+      kend = k;
+      break;
+    }
+  HH = 0.0001;
+  HC = 0.0;
+  HS = 0.0;
+  for (k = kstart; k < kend; k++) {
+    if (rho[k] > QO_RHOCUT) continue; /* a band the signal model does not explain */
+    for (i = imin + k * bw; i < imin + k * bw + bw; i++) {
+      HH += 4.0 * (twave[i] * twave[i] + twave[N - i] * twave[N - i]) / SN[i];
+      HC += 4.0 * (D[i] * twave[i] + D[N - i] * twave[N - i]) / SN[i];
+      HS += 4.0 * (D[i] * twave[N - i] - D[N - i] * twave[i]) / SN[i];
+    }
+  }
+  HD = sqrt(HC * HC + HS * HS);
+  *Ax = HD / HH;
+  *Px = atan2(HS, HC);
+  free(rho);
+  free(snr);
+  return (HD * HD / HH) / 2.0;
+}
+
+/* log_likelihood_APmax, QuickCBC.c:5672-5762: templates() of the
+ * reference-frame params, the penalized likelihood per detector with its
+ * amplitude and phase maximised, and params moved onto the maximum: ln DL
+ * by -ln A_0, phi0 by -P_0/2 (into [0, pi]), each other detector's phase
+ * offset by -(P_d - P_0) (into [0, 2 pi]) and amplitude ratio by A_d/A_0;
+ * done again once if some A_d is outside [0.9, 1.1] or some |P_d| > 0.1.
+ * params (7 + 3 (nifo - 1)) in place.  status: the templates' IMRPhenomD status. */
+double qo_log_likelihood_apmax(int nifo, double fref, double fmin, double fmax, double **D, double *params,
+                               const double *freq, double **SN, int N, double Tobs, int *status) {
+  double *buf = malloc(sizeof(double) * (size_t)nifo * N), *tw[3], A[3], P[3], logL = 0.0;
+  int id, round, st = 0;
+  for (id = 0; id < nifo; id++) tw[id] = buf + (size_t)id * N;
+  for (round = 0; round < 2; round++) {
+    int moved = 0;
+    st = qo_templates(nifo, fref, tw, freq, params, N);
+    if (st) break;
+    logL = 0.0;
+    for (id = 0; id < nifo; id++)
+      logL += qo_log_likelihood_penalized(params, D[id], tw[id], SN[id], fmin, fmax, N, Tobs, &A[id], &P[id]);
+    params[6] -= log(A[0]);
+    params[4] -= P[0] / 2.0; /* the template phase is the GW phase, twice the orbital one */
+    if (params[4] < 0.0) params[4] += QO_PI;
+    if (params[4] > QO_PI) params[4] -= QO_PI;
+    for (id = 1; id < nifo; id++) {
+      double *off = params + QO_NX + (id - 1) * 3;
+      off[0] -= P[id] - P[0];
+      off[2] *= A[id] / A[0];
+      if (off[0] < 0.0) off[0] += QO_TPI;
+      if (off[0] > QO_TPI) off[0] -= QO_TPI;
+    }
+    for (id = 0; id < nifo; id++)
+      if (A[id] > 1.1 || A[id] < 0.9 || fabs(P[id]) > 0.1) moved = 1;
+    if (!moved) break;
+  }
+  free(buf);
+  if (status) *status = st;
+  return st ? 0.0 : logL;
+}
+
+/* PDwave, QuickCBC.c:3448-3525: the h22 template h22fac A/sqrt(Tobs)
+ * e^{i (2 pi f ts - Psi)} with phi0 = params[4] (not halved), ts = Tobs -
+ * params[5], zero at f <= fbegin of params with tc = tcmin; half-complex N. */
+int qo_pdwave(double *wavef, const double *freq, double tcmin, const double *params, double fref, int N) {
+  const double Tobs = 1.0 / freq[1], sqT = sqrt(Tobs);
+  double q[11], fs, mc, mt, eta, dm, *amp, *phase;
+  int i, st;
+  memcpy(q, params, sizeof(q));
+  q[5] = tcmin;
+  fs = qo_fbegin(q);
+  mc = qo_exp_cr(params[0]);
+  mt = qo_exp_cr(params[1]);
+  eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
+  dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  amp = malloc(sizeof(double) * (N / 2));
+  phase = malloc(sizeof(double) * (N / 2));
+  st = qo_h22_ampphase(freq, N / 2, params[4], fref, mt * (1.0 + dm) / 2.0, mt * (1.0 - dm) / 2.0, params[2], params[3],
+                       qo_exp_cr(params[6]), amp, phase);
+  if (st == 0) {
+    const double ts = Tobs - params[5];
+    for (i = 1; i < N / 2; i++) {
+      const double f = freq[i];
+      wavef[i] = 0.0;
+      wavef[N - i] = 0.0;
+      if (f > fs) {
+        const double p = QO_TPI * f * ts, cp = cos(p), sp = sin(p);
+        const double x = cos(phase[i]), y = sin(phase[i]), Amp = QO_H22FAC * amp[i] / sqT;
+        wavef[i] = Amp * (x * cp + y * sp);
+        wavef[N - i] = Amp * (x * sp - y * cp);
+      }
+    }
+    wavef[0] = 0.0;
+    wavef[N / 2] = 0.0;
+  }
+  free(amp);
+  free(phase);
+  return st;
+}
+
+/* In-place complex FFT of length n (a power of 2), e^{+2 pi i jk/n}, no
+ * normalisation: iterative radix-2 decimation in time (Cooley-Tukey), the
+ * twiddles from sin/cos of each angle. */
+static void qo_fft_backward(double *re, double *im, int n) {
+  int i, j, len;
+  for (i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double t = re[i];
+      re[i] = re[j];
+      re[j] = t;
+      t = im[i];
+      im[i] = im[j];
+      im[j] = t;
+    }
+  }
+  for (len = 2; len <= n; len <<= 1) {
+    const int h = len >> 1;
+    for (i = 0; i < n; i += len)
+      for (j = 0; j < h; j++) {
+        const double ang = QO_TPI * (double)j / (double)len;
+        const double wr = cos(ang), wi = sin(ang);
+        const double ur = re[i + j], ui = im[i + j];
+        const double vr = re[i + j + h] * wr - im[i + j + h] * wi;
+        const double vi = re[i + j + h] * wi + im[i + j + h] * wr;
+        re[i + j] = ur + vr;
+        im[i + j] = ui + vi;
+        re[i + j + h] = ur - vr;
+        im[i + j + h] = ui - vi;
+      }
+  }
+}
+
+/* pbt_shift, Utilities.c:687-713: the cross spectrum C_k = D_k conj(h_k) / S_k
+ * for k in [imin, imax) (half-complex inputs) and its inverse transforms:
+ * corr = the half-complex inverse FFT of C (GSL's, normalised by 1/N), corrf
+ * that of -i C.  Both are (2/N) Re and Im of z(t) = sum_k C_k e^{2 pi i kt/N}
+ * (the bins 0 and N/2 are never in the band), computed here with one complex
+ * transform. */
+void qo_pbt_shift(double *corr, double *corrf, const double *data1, const double *data2, const double *Sn, int imin,
+                  int imax, int N) {
+  int i;
+  for (i = 0; i < N; i++) corr[i] = corrf[i] = 0.0;
+  for (i = imin; i < imax; i++) {
+    const int k = N - i;
+    corr[i] = (data1[i] * data2[i] + data1[k] * data2[k]) / Sn[i];  /* Re C_i */
+    corrf[i] = (data1[k] * data2[i] - data1[i] * data2[k]) / Sn[i]; /* Im C_i */
+  }
+  qo_fft_backward(corr, corrf, N);
+  for (i = 0; i < N; i++) {
+    corr[i] *= 2.0 / (double)N;
+    corrf[i] *= 2.0 / (double)N;
+  }
+}
+
+typedef struct {
+  double L, t;
+  int idx;
+} qo_lag;
+
+/* descending in logL, ascending in lag index on a tie */
+static int qo_lag_cmp(const void *a, const void *b) {
+  const qo_lag *x = a, *y = b;
+  if (x->L > y->L) return -1;
+  if (x->L < y->L) return 1;
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+/* log_likelihood_max, QuickCBC.c:5765-6231 (pflag 0).  params
+ * (reference-frame, 7 + 3 (nifo - 1)) in place: tc set to Tobs/2 and the
+ * template PDwave (fbegin at tc = net_tmax); per detector the matched
+ * filter over all time lags (pbt_shift), logL(lag) = |z|^2 / (h|h) / 2
+ * (the reference's (2 N |corr + i corrf|)^2 / HH / 2) over lag indices
+ * [mn, mx); its peaks in descending order, each kept if above 9 (the first
+ * always) and more than 0.05 s from the kept ones; each kept peak's template
+ * shifted in time and phase and scaled, then the penalized likelihood
+ * refines its amplitude, phase and logL; with several detectors the best
+ * pair of peaks within the pair's light travel time delays[i][j] (a third
+ * detector adds its best peak consistent with both, from its first NU - 1
+ * peaks, as the reference loops), else the best single peak; params then
+ * moved onto it.  GSL's heapsort orders equal logLs arbitrarily; ties go by
+ * lag index here.  Returns logL; status the IMRPhenomD status. */
+double qo_log_likelihood_max(int nifo, double fref, double fmin, double fmax, double **D, double *params,
+                             const double *freq, double **SN, int N, double Tobs, double tmin, double tmax,
+                             double net_tmax, const double *delays, int *status) {
+  enum { MAXP = 4096 };
+  const double dt = Tobs / (double)N;
+  double *h = malloc(sizeof(double) * N), *tw = malloc(sizeof(double) * N);
+  double *corr = malloc(sizeof(double) * N), *corrf = malloc(sizeof(double) * N), HH[3];
+  int imin = (int)(Tobs * fmin), imax = (int)(Tobs * fmax), mn, mx, M, k, i, j, kk, st;
+  int NU[3] = {0, 0, 0};
+  double *LLU[3], *TDU[3], *PU[3], *AU[3], logL = 0.0, delt[3] = {0, 0, 0}, pshift[3] = {0, 0, 0},
+                                           norm[3] = {1, 1, 1};
+  qo_lag *lags;
+  if (status) *status = 0;
+  if (!(qo_fbegin(params) < fmax)) {
+    free(h);
+    free(tw);
+    free(corr);
+    free(corrf);
+    return 0.0;
+  }
+  if (imin < 1) imin = 1;
+  if (imax > N / 2) imax = N / 2;
+  mn = (int)(tmin / Tobs * (double)N) - 1;
+  mx = (int)(tmax / Tobs * (double)N) + 1;
+  if (mn < 0) mn = 0;
+  if (mx > N) mx = N;
+  M = mx - mn;
+  lags = malloc(sizeof(qo_lag) * (M > 0 ? M : 1));
+  for (k = 0; k < nifo; k++) {
+    LLU[k] = malloc(sizeof(double) * MAXP);
+    TDU[k] = malloc(sizeof(double) * MAXP);
+    PU[k] = malloc(sizeof(double) * MAXP);
+    AU[k] = malloc(sizeof(double) * MAXP);
+  }
+  params[5] = Tobs / 2.0; /* the reference time moved to the centre */
+  st = qo_pdwave(h, freq, net_tmax, params, fref, N);
+  if (st) {
+    if (status) *status = st;
+    goto out;
+  }
+  for (k = 0; k < nifo; k++) HH[k] = qo_fourier_nwip(h, h, SN[k], imin, imax, N);
+  for (k = 0; k < nifo; k++) {
+    int n;
+    qo_pbt_shift(corr, corrf, D[k], h, SN[k], imin, imax, N);
+    /* lag index i <-> lag (i - N/2) dt: lags >= 0 at corr[i - N/2], < 0 at corr[i + N/2] */
+    for (n = 0, i = mn; i < mx; i++, n++) {
+      const int src = (i >= N / 2) ? i - N / 2 : i + N / 2;
+      const double HD = 2.0 * (double)N * sqrt(corr[src] * corr[src] + corrf[src] * corrf[src]);
+      lags[n].L = (HD * HD / HH[k]) / 2.0;
+      lags[n].t = dt * (double)(i - N / 2);
+      lags[n].idx = i;
+    }
+    qsort(lags, M, sizeof(qo_lag), qo_lag_cmp);
+    NU[k] = 0;
+    for (n = 0; n < M && NU[k] < MAXP; n++) {
+      int keep = (n == 0);
+      if (n > 0 && lags[n].L > QO_LPMIN) {
+        keep = 1;
+        for (j = 0; j < NU[k]; j++)
+          if (fabs(lags[n].t - TDU[k][j]) < QO_DTP) keep = 0;
+      }
+      if (keep) {
+        LLU[k][NU[k]] = lags[n].L;
+        TDU[k][NU[k]] = lags[n].t;
+        NU[k]++;
+      }
+    }
+    for (j = 0; j < NU[k]; j++) {
+      const int jj = (int)(rint(TDU[k][j] / dt)) + N / 2;
+      const int src = (jj >= N / 2) ? jj - N / 2 : jj + N / 2;
+      const double HD = 2.0 * (double)N * sqrt(corr[src] * corr[src] + corrf[src] * corrf[src]);
+      double Ax, Px;
+      const double nm = HD / HH[k], psh = atan2(corrf[src], corr[src]), tsh = TDU[k][j];
+      PU[k][j] = psh;
+      AU[k][j] = nm;
+      for (i = imin; i < imax; i++) {
+        const double f = (double)(i) / Tobs, ph = psh - QO_TPI * f * tsh, cph = cos(ph), sph = sin(ph);
+        tw[i] = nm * (h[i] * cph - h[N - i] * sph);
+        tw[N - i] = nm * (h[N - i] * cph + h[i] * sph);
+      }
+      LLU[k][j] = qo_log_likelihood_penalized(params, D[k], tw, SN[k], fmin, fmax, N, Tobs, &Ax, &Px);
+      AU[k][j] *= Ax;
+      PU[k][j] -= Px;
+    }
+  }
+  if (nifo > 1) {
+    double maxL[3][3] = {{0}}, tx[3][3][3], px[3][3][3], ax[3][3][3], best = 0.0;
+    int bi = 0, bj = 1, a, b, c;
+    for (a = 0; a < nifo; a++)
+      for (b = a + 1; b < nifo; b++) {
+        maxL[a][b] = 0.0;
+        for (k = 0; k < NU[a]; k++) {
+          if (!(LLU[a][k] > QO_LPMIN)) continue;
+          for (kk = 0; kk < NU[b]; kk++) {
+            double x;
+            if (!(LLU[b][kk] > QO_LPMIN) || !(fabs(TDU[a][k] - TDU[b][kk]) < delays[a * nifo + b])) continue;
+            x = LLU[a][k] + LLU[b][kk];
+            if (x > maxL[a][b]) {
+              maxL[a][b] = x;
+              tx[a][b][a] = TDU[a][k];
+              px[a][b][a] = PU[a][k];
+              ax[a][b][a] = AU[a][k];
+              tx[a][b][b] = TDU[b][kk];
+              px[a][b][b] = PU[b][kk];
+              ax[a][b][b] = AU[b][kk];
+            }
+          }
+        }
+      }
+    if (nifo > 2)
+      for (a = 0; a < nifo; a++)
+        for (b = a + 1; b < nifo; b++) {
+          if (!(maxL[a][b] > 0.0)) continue;
+          for (c = 0; c < nifo; c++) {
+            double x = 0.0;
+            const double t0 = tx[a][b][a], t1 = tx[a][b][b];
+            if (c == a || c == b) continue;
+            tx[a][b][c] = (t0 + t1) / 2.0;
+            px[a][b][c] = 0.0;
+            ax[a][b][c] = 1.0;
+            for (k = 0; k < NU[c] - 1; k++) { /* the reference's loop stops before the last peak */
+              const double t2 = TDU[c][k];
+              if (fabs(t0 - t2) < delays[a * nifo + c] && fabs(t1 - t2) < delays[b * nifo + c] && LLU[c][k] > x) {
+                x = LLU[c][k];
+                tx[a][b][c] = TDU[c][k];
+                px[a][b][c] = PU[c][k];
+                ax[a][b][c] = AU[c][k];
+              }
+            }
+            maxL[a][b] += x;
+          }
+        }
+    for (a = 0; a < nifo; a++)
+      for (b = a + 1; b < nifo; b++)
+        if (maxL[a][b] > best) {
+          best = maxL[a][b];
+          bi = a;
+          bj = b;
+        }
+    if (best > 0.0) {
+      logL = maxL[bi][bj];
+      for (k = 0; k < nifo; k++) {
+        delt[k] = tx[bi][bj][k];
+        pshift[k] = px[bi][bj][k];
+        norm[k] = ax[bi][bj][k];
+      }
+    }
+  } else {
+    double best = 0.0;
+    int bk = 0;
+    for (k = 0; k < NU[0]; k++)
+      if (LLU[0][k] > best) {
+        best = LLU[0][k];
+        bk = k;
+      }
+    if (best > 0.0) {
+      delt[0] = TDU[0][bk];
+      norm[0] = AU[0][bk];
+      pshift[0] = PU[0][bk];
+      logL = LLU[0][bk];
+    }
+  }
+  params[6] -= log(norm[0]);
+  params[5] += delt[0];
+  params[4] += pshift[0] / 2.0;
+  if (params[4] < 0.0) params[4] += QO_PI;
+  if (params[4] > QO_PI) params[4] -= QO_PI;
+  for (k = 1; k < nifo; k++) {
+    double *off = params + QO_NX + (k - 1) * 3;
+    off[0] = pshift[k] - pshift[0];
+    off[1] = delt[k] - delt[0];
+    off[2] = norm[k] / norm[0];
+    if (off[0] < 0.0) off[0] += QO_TPI;
+    if (off[0] > QO_TPI) off[0] -= QO_TPI;
+  }
+out:
+  for (k = 0; k < nifo; k++) {
+    free(LLU[k]);
+    free(TDU[k]);
+    free(PU[k]);
+    free(AU[k]);
+  }
+  free(lags);
+  free(h);
+  free(tw);
+  free(corr);
+  free(corrf);
+  return logL;
+}
+
+/* Flat-array forms (D nifo x N, SN nifo x N/2, params B x (7 + 3 (nifo - 1)) in place). */
+void qo_apmax_batch_flat(int nifo, double fref, double fmin, double fmax, const double *D, const double *SN, int N,
+                         double Tobs, double *params, int B, double *logL, int *status) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs);
+  const int ips = QO_NX + 3 * (nifo - 1);
+  int b, i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (b = 0; b < B; b++)
+    logL[b] = qo_log_likelihood_apmax(nifo, fref, fmin, fmax, Dr, params + (size_t)ips * b, freq, Sr, N, Tobs,
+                                      status ? status + b : NULL);
+  free(freq);
+}
+
+void qo_llmax_batch_flat(int nifo, double fref, double fmin, double fmax, const double *D, const double *SN, int N,
+                         double Tobs, double tmin, double tmax, double net_tmax, const double *delays, double *params,
+                         int B, double *logL, int *status) {
+  double *Dr[3], *Sr[3];
+  double *freq = qo_make_freq(N, Tobs);
+  const int ips = QO_NX + 3 * (nifo - 1);
+  int b, i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) {
+    Dr[i] = (double *)D + (size_t)i * N;
+    Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (b = 0; b < B; b++)
+    logL[b] = qo_log_likelihood_max(nifo, fref, fmin, fmax, Dr, params + (size_t)ips * b, freq, Sr, N, Tobs, tmin,
+                                    tmax, net_tmax, delays, status ? status + b : NULL);
+  free(freq);
+}
+
+double qo_penalized_flat(const double *params, const double *D, const double *twave, const double *SN, double fmin,
+                         double fmax, int N, double Tobs, double *AP) {
+  qo_spline_init();
+  return qo_log_likelihood_penalized(params, D, twave, SN, fmin, fmax, N, Tobs, AP, AP + 1);
+}
+
+int qo_pdwave_flat(double *wavef, int N, double Tobs, double tcmin, const double *params, double fref) {
+  double *freq = qo_make_freq(N, Tobs);
+  int st;
+  qo_spline_init();
+  st = qo_pdwave(wavef, freq, tcmin, params, fref, N);
+  free(freq);
+  return st;
+}

@@ -118,4 +118,14 @@ int het_efix_device(qcbc_het* h, const double* d_x, int B, double* d_ej, const d
 int het_space_impl(qcbc_ctx* c, const double* const* SN, const double* d_SN, const double* params, const double* mn,
                    const double* mx, double etamin, int* MN, int* MM, int* M, int* NR, int* fgrid, int cap,
                    double* eval, double* evec, double* dparams, hipStream_t s);
+// The search likelihoods (qcbc_search.cpp), params in place (7 + 3 (n_det - 1) each)
+int search_apmax(qcbc_ctx* c, double* params, int B, double* logL, int* status);
+int search_llmax(qcbc_ctx* c, double* params, int B, double tmin, double tmax, double net_tmax, const double* delays,
+                 double* logL, int* status);
+int search_penalized_one(const double* params, const double* D, const double* twave, const double* SN, int N,
+                         double Tobs, double fmin, double fmax, double* logL, double* A, double* P, hipStream_t s);
+int search_pdwave(const double* params, double tcmin, int N, double Tobs, double fref, double fmin0, double* wavef,
+                  hipStream_t s);
+int search_pbt_shift(double* corr, double* corrf, const double* data1, const double* data2, const double* Sn, int imin,
+                     int imax, int N, hipStream_t s);
 }  // namespace qcbc

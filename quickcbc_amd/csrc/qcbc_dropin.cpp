@@ -788,6 +788,86 @@ void het_space(struct Net* net, struct Het* het, double* params, double* min, do
   }
 }
 
+// ------------------------------------------------- search likelihoods
+double log_likelihood_APmax(struct Net* net, double** D, double* params, RealVector* freq, double** SN, int N,
+                            double Tobs) {
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params || !grid_ok(freq, N / 2, Tobs))
+    fail("log_likelihood_APmax", QCBC_EINVAL);
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc_from(net, N, Tobs), D, SN, &st);
+  if (!ctx) fail("log_likelihood_APmax", st);
+  double logL = 0.0;
+  int pst = 0;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    st = search_apmax(ctx, params, 1, &logL, &pst);
+  }
+  if (st) fail("log_likelihood_APmax", st);
+  if (domain_error("log_likelihood_APmax", pst)) return 0.0;
+  return logL;
+}
+
+double log_likelihood_max(struct Net* net, double** D, double* params, RealVector* freq, double** SN, int N,
+                          double Tobs, double tmin, double tmax, int pflag) {
+  (void)pflag;  // its prints are not made
+  if (!net || net->Nifo < 1 || net->Nifo > 3 || !D || !SN || !params || !grid_ok(freq, N / 2, Tobs) ||
+      (net->Nifo > 1 && !net->delays) || !(tmin < tmax))
+    fail("log_likelihood_max", QCBC_EINVAL);
+  const int nd = net->Nifo;
+  double dl[9] = {0};
+  for (int i = 0; i < nd && nd > 1; i++) {
+    if (!net->delays[i]) fail("log_likelihood_max", QCBC_EINVAL);
+    for (int j = 0; j < nd; j++) dl[i * nd + j] = net->delays[i][j];
+  }
+  int st;
+  qcbc_ctx* ctx = get_ctx(desc_from(net, N, Tobs), D, SN, &st);
+  if (!ctx) fail("log_likelihood_max", st);
+  double logL = 0.0;
+  int pst = 0;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    st = search_llmax(ctx, params, 1, tmin, tmax, net->tmax, dl, &logL, &pst);
+  }
+  if (st) fail("log_likelihood_max", st);
+  if (domain_error("log_likelihood_max", pst)) return 0.0;
+  return logL;
+}
+
+double log_likelihood_penalized(double* params, double* D, double* twave, double* SN, double* Ax, double* Px, int N,
+                                double Tobs, int pflag) {
+  (void)pflag;  // its prints are not made
+  if (!params || !D || !twave || !SN || !Ax || !Px || N < 4 || !(Tobs > 0)) fail("log_likelihood_penalized", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("log_likelihood_penalized", st);
+  double logL = 0.0;
+  st = search_penalized_one(params, D, twave, SN, N, Tobs, g_fmin.load(), g_fmax.load(), &logL, Ax, Px, t_drop.stream);
+  if (st) fail("log_likelihood_penalized", st);
+  return logL;
+}
+
+void PDwave(double* wavef, RealVector* freq, double tcmin, double* params, int N) {
+  if (!wavef || !freq || !freq->data || freq->length < 2 || !params || N < 4) fail("PDwave", QCBC_EINVAL);
+  const double Tobs = 1.0 / freq->data[1];  // QuickCBC.c:3459
+  if (!grid_ok(freq, N / 2, Tobs)) fail("PDwave", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("PDwave", st);
+  st = search_pdwave(params, tcmin, N, Tobs, g_fref.load(), freq->data[0], wavef, t_drop.stream);
+  if (st == QCBC_EDOM) {
+    if (domain_error("PDwave", st)) std::fill(wavef, wavef + N, 0.0);
+    return;
+  }
+  if (st) fail("PDwave", st);
+}
+
+void pbt_shift(double* corr, double* corrf, double* data1, double* data2, double* Sn, int imin, int imax, int N) {
+  if (!corr || !corrf || !data1 || !data2 || !Sn || N < 4 || (N & (N - 1)) || imin < 1 || imax > N / 2)
+    fail("pbt_shift", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("pbt_shift", st);
+  st = search_pbt_shift(corr, corrf, data1, data2, Sn, imin, imax, N, t_drop.stream);
+  if (st) fail("pbt_shift", st);
+}
+
 // IMRPhenomD_internals.c:141-181
 AmpPhaseFDWaveform* CreateAmpPhaseFDWaveform(size_t length) {
   AmpPhaseFDWaveform* wf = (AmpPhaseFDWaveform*)std::malloc(sizeof(AmpPhaseFDWaveform));

@@ -154,6 +154,24 @@ def lib():
         L.Fisher_Het.argtypes = [ctypes.POINTER(Net), ctypes.POINTER(HetStruct), _dpp, _dp, ctypes.c_double]
         L.FisherEvec.restype = None
         L.FisherEvec.argtypes = [_dpp, _dp, _dpp, ctypes.c_int]
+        # search likelihoods
+        L.qcbc_loglike_apmax.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.qcbc_loglike_max.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       _dp, _dp, _ip]
+        L.log_likelihood_APmax.restype = ctypes.c_double
+        L.log_likelihood_APmax.argtypes = [ctypes.POINTER(Net), _dpp, _dp, ctypes.POINTER(RealVector), _dpp,
+                                           ctypes.c_int, ctypes.c_double]
+        L.log_likelihood_max.restype = ctypes.c_double
+        L.log_likelihood_max.argtypes = [ctypes.POINTER(Net), _dpp, _dp, ctypes.POINTER(RealVector), _dpp,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int]
+        L.log_likelihood_penalized.restype = ctypes.c_double
+        L.log_likelihood_penalized.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_int]
+        L.PDwave.restype = None
+        L.PDwave.argtypes = [_dp, ctypes.POINTER(RealVector), ctypes.c_double, _dp, ctypes.c_int]
+        L.pbt_shift.restype = None
+        L.pbt_shift.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         # the heterodyne's grid choice
         L.qcbc_limits.argtypes = [_vp, _dp, _ip, _dp]
         L.qcbc_fisher_all.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
@@ -314,6 +332,29 @@ class Context(_Handle):
         ev = np.zeros((B, d, d))
         _check(lib().qcbc_fisher_evec(self.h, _d(F), B, d, _d(ej), _d(ev)), "qcbc_fisher_evec")
         return ej, ev
+
+    # ---- search likelihoods (QuickCBC.c:5672-6408)
+    def apmax(self, params):
+        """log_likelihood_APmax for a batch of reference-frame points: (logL, moved params, status)."""
+        ips = 7 + 3 * (self.an.nifo - 1)
+        p = np.array(params, np.float64).reshape(-1, ips)
+        B = p.shape[0]
+        L = np.zeros(B)
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_loglike_apmax(self.h, _d(p), B, _d(L), _i(st)), "qcbc_loglike_apmax")
+        return L, p, st
+
+    def llmax(self, params, tmin, tmax, net_tmax, delays):
+        """log_likelihood_max for a batch of reference-frame points: (logL, moved params, status)."""
+        ips = 7 + 3 * (self.an.nifo - 1)
+        p = np.array(params, np.float64).reshape(-1, ips)
+        B = p.shape[0]
+        L = np.zeros(B)
+        st = np.zeros(B, np.int32)
+        dl = np.ascontiguousarray(delays, np.float64)
+        _check(lib().qcbc_loglike_max(self.h, _d(p), B, tmin, tmax, net_tmax, _d(dl), _d(L), _i(st)),
+               "qcbc_loglike_max")
+        return L, p, st
 
     # ---- the heterodyne's grid choice (QuickCBC.c:4275-5053, 2695-2817)
     def limits(self, params):
@@ -1219,3 +1260,48 @@ def dropin_het_space(an, SNrows, params, pmin, pmax):
     g = np.array([h.fgrid[i] for i in range(M)], np.int32)
     f = np.array([h.freq.contents.data[i] for i in range(M)])
     return dict(MN=h.MN, MM=h.MM, M=M, NR=h.NR, J=h.J, fgrid=g, freq=f)
+
+
+# ------------------------------------------- drop-ins of the search likelihoods
+def dropin_apmax(an, Drows, SNrows, params):
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.array(params, np.float64)
+    v = lib().log_likelihood_APmax(ctypes.byref(net), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, an.N, an.Tobs)
+    return v, p
+
+
+def dropin_llmax(an, Drows, SNrows, params, tmin, tmax, net_tmax, delays):
+    net = make_net(an)
+    net.tmax = net_tmax
+    dl = RowMatrix(np.array(delays, np.float64))
+    net.delays = dl.p
+    net._keep2 = dl
+    fr = make_freq(an)
+    p = np.array(params, np.float64)
+    v = lib().log_likelihood_max(ctypes.byref(net), Drows.p, _d(p), ctypes.byref(fr), SNrows.p, an.N, an.Tobs, tmin,
+                                 tmax, 0)
+    return v, p
+
+
+def dropin_penalized(params, D, twave, SN, N, Tobs):
+    a = [np.ascontiguousarray(x, np.float64) for x in (params, D, twave, SN)]
+    A = ctypes.c_double(0)
+    P = ctypes.c_double(0)
+    v = lib().log_likelihood_penalized(*[_d(x) for x in a], ctypes.byref(A), ctypes.byref(P), N, Tobs, 0)
+    return v, A.value, P.value
+
+
+def dropin_pdwave(an, params, tcmin):
+    fr = make_freq(an)
+    w = np.zeros(an.N)
+    p = np.ascontiguousarray(params, np.float64)
+    lib().PDwave(_d(w), ctypes.byref(fr), tcmin, _d(p), an.N)
+    return w
+
+
+def dropin_pbt_shift(data1, data2, Sn, imin, imax, N):
+    a = [np.ascontiguousarray(x, np.float64) for x in (data1, data2, Sn)]
+    c, cf = np.zeros(N), np.zeros(N)
+    lib().pbt_shift(_d(c), _d(cf), *[_d(x) for x in a], imin, imax, N)
+    return c, cf

@@ -54,3 +54,20 @@ def het(which, J=2, frac=1.0 / 64.0):
 def oracle_loglike_het(which, P, J=2, frac=1.0 / 64.0):
     an, inj, D, SN = analysis(which)
     return O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, het(which, J, frac), P)
+
+
+def ref_frame(an, p):
+    """The full point's reference-frame parameters (detector 0's frame)."""
+    ciota = p[10]
+    Ap, Ac = (1 + ciota ** 2) / 2, -ciota
+    dt = O.times(p[7], p[8], an.gmst)
+    Fs, lam = [], []
+    for d in an.labels:
+        Fp, Fc = O.f_ant(p[9], p[7], p[8], an.gmst, d)
+        Fs.append(np.sqrt(Ap ** 2 * Fp ** 2 + Ac ** 2 * Fc ** 2))
+        l = np.arctan2(Ac * Fc, Ap * Fp)
+        lam.append(l + 2 * np.pi if l < 0 else l)
+    q = [p[0], p[1], p[2], p[3], 0.5 * (lam[0] + p[4]), p[5] + dt[an.labels[0]], p[6] - np.log(Fs[0])]
+    for d in range(1, an.nifo):
+        q += [lam[d] - lam[0], dt[an.labels[d]] - dt[an.labels[0]], Fs[d] / Fs[0]]
+    return np.array(q)
