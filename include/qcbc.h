@@ -226,6 +226,16 @@ double log_likelihood_penalized(double *params, double *D, double *twave, double
 void PDwave(double *wavef, RealVector *freq, double tcmin, double *params, int N);
 /* Replaces pbt_shift, Utilities.c:687-713 (prototype Utilities.h:37): N a power of 2. */
 void pbt_shift(double *corr, double *corrf, double *data1, double *data2, double *Sn, int imin, int imax, int N);
+/* Replaces tukey, Utilities.c:635-654 (prototype Utilities.h:35): the window
+ * applied in place to N samples. */
+void tukey(double *data, double alpha, int N);
+/* Replaces tukey_scale, Utilities.c:656-683 (prototype Utilities.h:36): the
+ * window's mean (s1) and root mean square (s2). */
+void tukey_scale(double *s1, double *s2, double alpha, int N);
+/* Replaces whiten, Utilities.c:294-309 (prototype Utilities.h:24): bins 1 ..
+ * N/2 - 1 of a half-complex row divided by sqrt(Sn), the DC and Nyquist
+ * bins set to 0. */
+void whiten(double *data, double *Sn, int N);
 
 /* Allocators of the ABI types, IMRPhenomD_internals.c:141-181. */
 AmpPhaseFDWaveform *CreateAmpPhaseFDWaveform(size_t length);
@@ -258,7 +268,8 @@ enum {
   QCBC_EBUSY = 103,   /* admission refused: memory budget or slots */
   QCBC_ECANCELED = 104,
   QCBC_ETIMEDOUT = 105,
-  QCBC_ERANGE = 34    /* as ERANGE: a batch larger than a fixed capacity */
+  QCBC_ERANGE = 34,   /* as ERANGE: a batch or file larger than a capacity */
+  QCBC_EIO = 5        /* as EIO: a file that cannot be read or written */
 };
 
 enum { QCBC_FP64 = 0, QCBC_MIXED = 1 };
@@ -358,6 +369,23 @@ int qcbc_templates_intrinsic(qcbc_ctx *ctx, const double *params, int B, double 
  * conditioned data.  Allocates its device buffers and FFT plan per call (a
  * once-per-analysis step). */
 int qcbc_bind_time_series(qcbc_ctx *ctx, const double *x, const double *SN, double t_rise, double *D);
+
+/* The conditioning of QuickCBC's main (QuickCBC.c:250-318) and of CBC_start
+ * after glitch cleaning (:807-823) on the GPU, then binding.  x (n_det x N
+ * time samples, host) windowed by tukey (alpha = 2 t_rise / Tobs) and
+ * transformed; D (n_det x N, may be NULL) receives the half-complex data
+ * times sqrt(Tobs)/N, which is bound with the PSD SN (n_det x N/2).
+ * fmin_floor > 0 binds the PSD as 1 below bin (int)(fmin_floor Tobs), as
+ * CBC_start does (SN itself is not written).  main's outputs from the same
+ * transform (each may be NULL, n_det rows): DW (N) the whitened series of
+ * dataw.dat (whiten, the half-complex inverse, / (sqrt(N/2)/sqrt(Tobs))),
+ * pspec (N/2) the periodogram of pspec_*.dat, (2 Tobs/N^2)(Re^2 + Im^2),
+ * and wcheck (N/2) = pspec / SN of wcheck_*.dat.  The bin-0 entry of pspec
+ * takes the DC term only (the reference adds the square of its row's spare
+ * element).  qcbc_bind_time_series(ctx, x, SN, t_rise, D) is
+ * qcbc_condition(ctx, x, SN, t_rise, 0, D, NULL, NULL, NULL). */
+int qcbc_condition(qcbc_ctx *ctx, const double *x, const double *SN, double t_rise, double fmin_floor, double *D,
+                   double *DW, double *pspec, double *wcheck);
 
 /* makespec (Utilities.c:1908-1959) on the GPU for the analysis's N and Tobs:
  * SN (N/2) = exp(natural cubic spline of ln S through (ffit, Sspline)) times
@@ -629,6 +657,32 @@ int qcbc_loglike_apmax(qcbc_ctx *ctx, double *params, int B, double *logL, int *
  * the detectors' peaks within delays (n_det x n_det light travel times). */
 int qcbc_loglike_max(qcbc_ctx *ctx, double *params, int B, double tmin, double tmax, double net_tmax,
                      const double *delays, double *logL, int *status);
+
+/* ------------------------------------------------------------------ *
+ * Part 9: the input and output text formats (QuickCBC.c:195-248,
+ * :267-306), host code
+ * ------------------------------------------------------------------ */
+/* The file name QuickCBC builds, "<kind>_<(int)Tobs>_<(int)ttrig>_<label>.dat"
+ * (kind "framed" or "spec", QuickCBC.c:200, :225, :244).  QCBC_ERANGE when it
+ * does not fit len bytes. */
+int qcbc_data_file_name(char *buf, size_t len, const char *kind, double Tobs, double ttrig, int label);
+/* A framed_* file (QuickCBC.c:200-230): whitespace-separated pairs (time,
+ * strain).  *N is the reference's count: the pairs read before the end of
+ * the file is seen, so a file ending right after its last number counts one
+ * pair fewer.  time and x (each may be NULL) receive the first *N pairs;
+ * QCBC_ERANGE when *N > cap (with *N set), QCBC_EINVAL on a token that is not
+ * a number (the reference's loop would not end), QCBC_EIO when the file
+ * cannot be read. */
+int qcbc_read_framed(const char *path, int cap, double *time, double *x, int *N);
+/* A spec_* file (QuickCBC.c:242-248): n2 rows of four numbers, the second
+ * and third the full and smooth PSDs, into SN and SM (each may be NULL).
+ * QCBC_EINVAL when the file holds fewer numbers. */
+int qcbc_read_spec(const char *path, int n2, double *SN, double *SM);
+/* A table as main writes dataw.dat, pspec_*.dat and wcheck_*.dat
+ * (QuickCBC.c:280-306): nrows rows of ncols columns (cols[c][r]), each value
+ * printed "%.*e " with digits digits (6 for dataw.dat's "%e", 15 for the
+ * spectra's "%.15e"), then a newline.  QCBC_EIO on a write error. */
+int qcbc_write_table(const char *path, int nrows, int ncols, const double *const *cols, int digits);
 
 /* het_diff and efix (het_diff distance) on a heterodyne, as above. */
 int qcbc_het_diff(qcbc_het *het, const double *params, int B, const double *px, const int *of, int K, double *z);

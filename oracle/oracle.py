@@ -497,6 +497,85 @@ def condition(x, Tobs, t_rise=0.4):
     return np.array([halfcomplex(np.fft.rfft(row * w), N) for row in x]) * (np.sqrt(Tobs) / N)
 
 
+def tukey_scale(alpha, N):
+    """tukey_scale() (Utilities.c:656-683): (mean, root mean square) of the window."""
+    w = tukey(N, alpha)
+    return w.sum() / N, np.sqrt((w * w).sum() / N)
+
+
+def whiten(data, Sn):
+    """whiten() (Utilities.c:294-309) of a half-complex row: bins 1 .. N/2 - 1 times 1/sqrt(S), DC and Nyquist 0."""
+    d = np.array(data, np.float64)
+    N = d.size
+    h = N // 2
+    x = 1.0 / np.sqrt(np.asarray(Sn, np.float64)[1:h])
+    d[1:h] *= x
+    d[N - np.arange(1, h)] *= x
+    d[0] = d[h] = 0.0
+    return d
+
+
+def condition_outputs(x, SN, Tobs, t_rise=0.4):
+    """main's conditioning (QuickCBC.c:250-313) of x (n_det x N) with the PSD SN (n_det x N/2):
+    D (the scaled half-complex data), DW (dataw.dat's whitened series), pspec and wcheck (the
+    spectra of pspec_*.dat and wcheck_*.dat; bin 0 takes the DC term only)."""
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    SN = np.atleast_2d(np.asarray(SN, np.float64))
+    nd, N = x.shape
+    h = N // 2
+    w = tukey(N, 2.0 * t_rise / Tobs)
+    raw = np.array([halfcomplex(np.fft.rfft(row * w), N) for row in x])  # unscaled, GSL's layout
+    fac = np.sqrt(N // 2) / np.sqrt(Tobs)
+    DW = np.zeros((nd, N))
+    for d in range(nd):
+        W = whiten(raw[d], SN[d])
+        # GSL's half-complex inverse: the backward sum times 1/N
+        Z = np.zeros(h + 1, complex)
+        Z[:h + 1] = W[:h + 1]
+        Z[1:h] += 1j * W[N - np.arange(1, h)]
+        DW[d] = np.fft.irfft(Z, N) / fac
+    pfac = Tobs / (float(N) * float(N))
+    sq = raw[:, :h] ** 2
+    sq[:, 1:] += raw[:, N - np.arange(1, h)] ** 2
+    pspec = pfac * 2.0 * sq
+    return {"D": raw * (np.sqrt(Tobs) / N), "DW": DW, "pspec": pspec, "wcheck": pspec / SN}
+
+
+def framed_count(text):
+    """The count of QuickCBC.c:204-210 for a file of T whitespace-separated numbers: each fscanf takes two,
+    and the end of the file is seen by the one that finds no further number, or whose last number ends
+    the file (no character after it); the count is the calls made less one."""
+    T = len(text.split())
+    if T == 0:
+        return 0
+    if text[-1].isspace() or T % 2 == 1:
+        return T // 2
+    return T // 2 - 1
+
+
+def read_framed(path, cap=1 << 24):
+    """A framed_* file through stdio as QuickCBC.c:204-220 reads it: (N, time, strain)."""
+    L = lib()
+    L.qo_read_framed.restype = ctypes.c_int
+    L.qo_read_framed.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+    N = L.qo_read_framed(str(path).encode(), 0, None, None)
+    if N < 0:
+        return N, None, None
+    t, x = np.zeros(max(N, 1)), np.zeros(max(N, 1))
+    L.qo_read_framed(str(path).encode(), min(N, cap), _d(t), _d(x))
+    return N, t[:N], x[:N]
+
+
+def read_spec(path, n2):
+    """A spec_* file through stdio as QuickCBC.c:242-248 reads it: (rows read, SN, SM)."""
+    L = lib()
+    L.qo_read_spec.restype = ctypes.c_int
+    L.qo_read_spec.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+    SN, SM = np.zeros(n2), np.zeros(n2)
+    r = L.qo_read_spec(str(path).encode(), n2, _d(SN), _d(SM))
+    return r, SN, SM
+
+
 def makespec(ffit, Sspline, lines, Tobs, N):
     """makespec (Utilities.c:1908): (status, SN of N/2); lines = (linef, lineh, lineQ, linew, deltafmax) or None."""
     L = lib()

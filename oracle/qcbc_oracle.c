@@ -44,6 +44,7 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
@@ -3468,4 +3469,50 @@ int qo_pdwave_flat(double *wavef, int N, double Tobs, double tcmin, const double
   st = qo_pdwave(wavef, freq, tcmin, params, fref, N);
   free(freq);
   return st;
+}
+
+/* ------------------------------------------------ text formats (f4) */
+/* A framed_* file read as QuickCBC.c:204-220 reads it, through stdio: the
+ * count loop `while (!feof(in)) { fscanf(in, "%lf%lf", &a, &b); N++; }`
+ * from N = -1, then, after a rewind, the first N pairs.  Returns N; -1 when
+ * the file cannot be opened, -2 on a token that is not a number (where the
+ * reference's loop would not end).  time and x (each may be NULL) receive
+ * the first min(N, cap) pairs. */
+int qo_read_framed(const char *path, int cap, double *time, double *x) {
+  FILE *in = fopen(path, "r");
+  double a, b;
+  int N = -1, i;
+  if (!in) return -1;
+  while (!feof(in)) {
+    if (fscanf(in, "%lf%lf", &a, &b) == 0) {
+      fclose(in);
+      return -2;
+    }
+    N++;
+  }
+  rewind(in);
+  for (i = 0; i < N && i < cap; i++) {
+    if (fscanf(in, "%lf%lf", &a, &b) != 2) break;
+    if (time) time[i] = a;
+    if (x) x[i] = b;
+  }
+  fclose(in);
+  return N;
+}
+
+/* A spec_* file read as QuickCBC.c:242-248: rows of four numbers, the second
+ * into SN and the third into SM.  Returns the number of whole rows read (at
+ * most n2); -1 when the file cannot be opened. */
+int qo_read_spec(const char *path, int n2, double *SN, double *SM) {
+  FILE *in = fopen(path, "r");
+  double v[4];
+  int i;
+  if (!in) return -1;
+  for (i = 0; i < n2; i++) {
+    if (fscanf(in, "%lf%lf%lf%lf", &v[0], &v[1], &v[2], &v[3]) != 4) break;
+    SN[i] = v[1];
+    SM[i] = v[2];
+  }
+  fclose(in);
+  return i;
 }

@@ -170,6 +170,9 @@ int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspli
                 const double *lineh, const double *lineQ, const double *linew, const double *deltafmax, double *SN,
                 double Tobs, int N);
 
+int qo_read_framed(const char *path, int cap, double *time, double *x);
+int qo_read_spec(const char *path, int n2, double *SN, double *SM);
+
 #ifdef __cplusplus
 }
 #endif

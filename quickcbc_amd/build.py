@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqcbc.so")
-SOURCES = ["qcbc_parity.hip", "qcbc_kernels.hip", "qcbc_het.hip", "qcbc_fisher.hip", "qcbc_update.hip", "qcbc_condition.hip", "qcbc_noalloc.hip", "qcbc_gridchoice.hip", "qcbc_search.hip", "qcbc_het.cpp", "qcbc_api.cpp", "qcbc_dropin.cpp", "qcbc_pt.cpp", "qcbc_multi.cpp", "qcbc_serve.cpp", "qcbc_gridchoice.cpp", "qcbc_search.cpp"]
+SOURCES = ["qcbc_parity.hip", "qcbc_kernels.hip", "qcbc_het.hip", "qcbc_fisher.hip", "qcbc_update.hip", "qcbc_condition.hip", "qcbc_noalloc.hip", "qcbc_gridchoice.hip", "qcbc_search.hip", "qcbc_het.cpp", "qcbc_api.cpp", "qcbc_dropin.cpp", "qcbc_pt.cpp", "qcbc_multi.cpp", "qcbc_serve.cpp", "qcbc_gridchoice.cpp", "qcbc_search.cpp", "qcbc_ingest.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -173,6 +173,7 @@ int qcbc_ctx_destroy(qcbc_ctx* c) {
     hipEventDestroy(e.second);
   }
   if (c->cond_plan) hipfftDestroy(static_cast<hipfftHandle>(c->cond_plan));
+  if (c->cond_iplan) hipfftDestroy(static_cast<hipfftHandle>(c->cond_iplan));
   void* bufs[] = {c->d_p64, c->d_p32, c->d_D, c->d_SN, c->d_wref, c->d_params, c->d_pts, c->d_status,
                   c->d_logL, c->d_rho, c->d_partial, c->d_amp, c->d_phase, c->d_hw, c->d_plain_partial,
                   c->d_cond_x, c->d_cond_X, c->d_spec};
@@ -225,13 +226,31 @@ int qcbc_bind_rows(qcbc_ctx* c, double* const* D, double* const* SN) {
   return bind_impl(c, (const double* const*)D, (const double* const*)SN);
 }
 
-int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double t_rise, double* D) {
-  if (!c || !x || !SN || !(t_rise >= 0.0)) return QCBC_EINVAL;
+namespace {
+struct CondTemp {
+  std::vector<void*> bufs;
+  ~CondTemp() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
+  int alloc(size_t bytes, void** out) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) return QCBC_ENOMEM;
+    bufs.push_back(p);
+    *out = p;
+    return QCBC_OK;
+  }
+};
+}  // namespace
+
+int qcbc_condition(qcbc_ctx* c, const double* x, const double* SN, double t_rise, double fmin_floor, double* D,
+                   double* DW, double* pspec, double* wcheck) {
+  if (!c || !x || !SN || !(t_rise >= 0.0) || !(fmin_floor >= 0.0)) return QCBC_EINVAL;
   const int nd = c->desc.n_det, N = c->desc.N, nh = c->nhalf;
+  const double Tobs = c->desc.Tobs;
   if (N & (N - 1)) return QCBC_EINVAL;  // radix 2, as gsl_fft_real_radix2_transform
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  // scratch and plan are made on the first call and kept with the analysis
+  // scratch and plans are made on the first call and kept with the analysis
   if (!c->d_cond_x) HIPCHK(hipMalloc(&c->d_cond_x, sizeof(double) * nd * N));
   if (!c->d_cond_X) HIPCHK(hipMalloc(&c->d_cond_X, sizeof(double) * 2 * nd * (N / 2 + 1)));
   if (!c->cond_plan) {
@@ -239,20 +258,65 @@ int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double
     if (hipfftPlan1d(&plan, N, HIPFFT_D2Z, nd) != HIPFFT_SUCCESS) return QCBC_EHIP;
     c->cond_plan = plan;
   }
+  const bool spectra = DW || pspec || wcheck;
+  if (DW && !c->cond_iplan) {
+    hipfftHandle plan = nullptr;
+    if (hipfftPlan1d(&plan, N, HIPFFT_Z2D, nd) != HIPFFT_SUCCESS) return QCBC_EHIP;
+    c->cond_iplan = plan;
+  }
   hipfftHandle plan = static_cast<hipfftHandle>(c->cond_plan);
   if (hipfftSetStream(plan, c->stream) != HIPFFT_SUCCESS) return QCBC_EHIP;
   c->bound = false;
   HIPCHK(hipMemcpyAsync(c->d_cond_x, x, sizeof(double) * nd * N, hipMemcpyHostToDevice, c->stream));
-  // QuickCBC.c:268-284: alpha = 2 t_rise / Tobs; D = FFT(tukey(x)) sqrt(Tobs) / N (:309-313)
-  HIPCHK(launch_tukey(c->d_cond_x, nd, N, 2.0 * t_rise / c->desc.Tobs, c->stream));
+  // QuickCBC.c:257-265: alpha = 2 t_rise / Tobs, tukey, the forward transform
+  HIPCHK(launch_tukey(c->d_cond_x, nd, N, 2.0 * t_rise / Tobs, c->stream));
   if (hipfftExecD2Z(plan, c->d_cond_x, static_cast<hipfftDoubleComplex*>(c->d_cond_X)) != HIPFFT_SUCCESS)
     return QCBC_EHIP;
-  HIPCHK(launch_pack_halfcomplex(c->d_cond_X, nd, N, sqrt(c->desc.Tobs) / (double)N, c->d_D, c->stream));
+  // D = the transform in the half-complex layout times sqrt(Tobs)/N (:309-313)
+  HIPCHK(launch_pack_halfcomplex(c->d_cond_X, nd, N, sqrt(Tobs) / (double)N, c->d_D, c->stream));
   if (D) HIPCHK(hipMemcpyAsync(D, c->d_D, sizeof(double) * nd * N, hipMemcpyDeviceToHost, c->stream));
-  // then bind as qcbc_bind_data does, with D already on the device
+  CondTemp tmp;
+  if (spectra) {
+    // main's whitened series and spectra (:267-306) of the unscaled transform
+    void *d_S, *d_W, *d_p = nullptr, *d_w = nullptr;
+    int st;
+    if ((st = tmp.alloc(sizeof(double) * nd * nh, &d_S)) ||
+        (st = tmp.alloc(sizeof(double) * 2 * nd * (N / 2 + 1), &d_W)) ||
+        (pspec && (st = tmp.alloc(sizeof(double) * nd * nh, &d_p))) ||
+        (wcheck && (st = tmp.alloc(sizeof(double) * nd * nh, &d_w))))
+      return st;
+    HIPCHK(hipMemcpyAsync(d_S, SN, sizeof(double) * nd * nh, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_spectra(c->d_cond_X, static_cast<double*>(d_S), nd, N, Tobs, d_W, static_cast<double*>(d_p),
+                          static_cast<double*>(d_w), c->stream));
+    if (pspec) HIPCHK(hipMemcpyAsync(pspec, d_p, sizeof(double) * nd * nh, hipMemcpyDeviceToHost, c->stream));
+    if (wcheck) HIPCHK(hipMemcpyAsync(wcheck, d_w, sizeof(double) * nd * nh, hipMemcpyDeviceToHost, c->stream));
+    if (DW) {
+      hipfftHandle ip = static_cast<hipfftHandle>(c->cond_iplan);
+      if (hipfftSetStream(ip, c->stream) != HIPFFT_SUCCESS) return QCBC_EHIP;
+      // the time series is no longer needed: the backward transform lands there
+      if (hipfftExecZ2D(ip, static_cast<hipfftDoubleComplex*>(d_W), c->d_cond_x) != HIPFFT_SUCCESS) return QCBC_EHIP;
+      HIPCHK(launch_whitened_series(c->d_cond_x, nd, N, Tobs, c->d_cond_x, c->stream));
+      HIPCHK(hipMemcpyAsync(DW, c->d_cond_x, sizeof(double) * nd * N, hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  // then bind as qcbc_bind_data does, with D already on the device; CBC_start's
+  // re-conditioning sets the PSD to 1 below fmin (:817-823)
+  std::vector<double> floored;
   const double* rs[3];
   for (int d = 0; d < nd; d++) rs[d] = SN + (size_t)d * nh;
+  if (fmin_floor > 0.0) {
+    floored.assign(SN, SN + (size_t)nd * nh);
+    const int j = std::min((int)(fmin_floor * Tobs), nh);
+    for (int d = 0; d < nd; d++) {
+      for (int i = 0; i < j; i++) floored[(size_t)d * nh + i] = 1.0;
+      rs[d] = floored.data() + (size_t)d * nh;
+    }
+  }
   return bind_finish(c, rs);
+}
+
+int qcbc_bind_time_series(qcbc_ctx* c, const double* x, const double* SN, double t_rise, double* D) {
+  return qcbc_condition(c, x, SN, t_rise, 0.0, D, nullptr, nullptr, nullptr);
 }
 
 int qcbc_makespec(qcbc_ctx* c, int Nspline, int Nlines, const double* ffit, const double* Sspline,
@@ -666,7 +730,8 @@ const char* qcbc_strerror(int code) {
     case QCBC_EBUSY: return "admission refused: memory budget or slots exhausted";
     case QCBC_ECANCELED: return "request canceled";
     case QCBC_ETIMEDOUT: return "timed out waiting for a request";
-    case QCBC_ERANGE: return "batch larger than the fixed capacity of this entry point";
+    case QCBC_ERANGE: return "batch or data larger than the capacity of this entry point";
+    case QCBC_EIO: return "file cannot be read or written";
     default: return "unknown error";
   }
 }

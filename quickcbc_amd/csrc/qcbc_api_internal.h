@@ -48,6 +48,7 @@ struct qcbc_ctx {
   double* d_cond_x = nullptr;
   void* d_cond_X = nullptr;
   void* cond_plan = nullptr;  // hipfftHandle
+  void* cond_iplan = nullptr;  // complex-to-real plan of the whitened series
   double* d_spec = nullptr;
   size_t spec_cap = 0;  // doubles
   // fused-kernel launch shape per precision
@@ -128,4 +129,11 @@ int search_pdwave(const double* params, double tcmin, int N, double Tobs, double
                   hipStream_t s);
 int search_pbt_shift(double* corr, double* corrf, const double* data1, const double* data2, const double* Sn, int imin,
                      int imax, int N, hipStream_t s);
+
+// the small conditioning drop-ins on the GPU (qcbc_ingest.cpp): tukey in
+// place on N samples, tukey_scale into s12 = {s1, s2}, whiten in place on a
+// half-complex row of N with Sn (N/2); synchronous on s
+int ingest_tukey(double* data, double alpha, int N, hipStream_t s);
+int ingest_tukey_scale(double alpha, int N, double* s12, hipStream_t s);
+int ingest_whiten(double* data, const double* Sn, int N, hipStream_t s);
 }  // namespace qcbc

@@ -49,7 +49,119 @@ __global__ void k_pack_halfcomplex(const double2* __restrict__ X, int nd, int N,
   }
 }
 
+// main's spectra of the windowed data (QuickCBC.c:286-306) from the
+// transform X (nd rows of N/2 + 1, unscaled): per bin i < N/2 the
+// periodogram pspec = (2 Tobs / N^2) (Re^2 + Im^2) and wcheck = pspec / S
+// (the Nyquist slot N - 0 of bin 0 is the row's spare element, 0 here); and
+// W, whiten() (Utilities.c:294-309) of X: bins 1 .. N/2 - 1 times 1/sqrt(S),
+// the DC and Nyquist bins 0
+__global__ void k_spectra(const double2* __restrict__ X, const double* __restrict__ SN, int nd, int N, double pfac,
+                          double2* __restrict__ W, double* __restrict__ pspec, double* __restrict__ wcheck) {
+  const int h = N / 2;
+  const size_t n = (size_t)nd * (h + 1);
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int d = (int)(r / (h + 1)), i = (int)(r % (h + 1));
+  const double2 c = X[r];
+  if (i < h) {
+    const double re = c.x, im = (i == 0) ? 0.0 : c.y;
+    const double p = pfac * (re * re + im * im);
+    const size_t k = (size_t)d * h + i;
+    if (pspec) pspec[k] = p;
+    if (wcheck) wcheck[k] = p / SN[k];
+  }
+  double2 w = make_double2(0.0, 0.0);
+  if (i > 0 && i < h) {
+    const double x = 1.0 / sqrt(SN[(size_t)d * h + i]);
+    w = make_double2(c.x * x, c.y * x);
+  }
+  W[r] = w;
+}
+
+// the whitened series: GSL's half-complex inverse (the backward sum times
+// 1/N) of W, divided by sqrt(N/2)/sqrt(Tobs) (QuickCBC.c:270-276)
+__global__ void k_whitened_series(const double* __restrict__ w, size_t n, double invN, double fac,
+                                  double* __restrict__ DW) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  DW[r] = (w[r] * invN) / fac;
+}
+
+// whiten() (Utilities.c:294-309) in place on one half-complex row of N
+__global__ void k_whiten(double* __restrict__ data, const double* __restrict__ Sn, int N) {
+  const int h = N / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > h) return;
+  if (i == 0 || i == h) {
+    data[i] = 0.0;
+    return;
+  }
+  const double x = 1.0 / sqrt(Sn[i]);
+  data[i] *= x;
+  data[N - i] *= x;
+}
+
+// tukey_scale() (Utilities.c:656-683): the window's mean and root mean
+// square over N samples; one block, lane-strided sums reduced in LDS
+__global__ __launch_bounds__(256) void k_tukey_scale(int N, double alpha, double* __restrict__ s12) {
+  __shared__ double a[256], b[256];
+  const int imin = (int)(alpha * (double)(N - 1) / 2.0);
+  const int imax = (int)((double)(N - 1) * (1.0 - alpha / 2.0));
+  const int Nwin = N - imax;
+  double x1 = 0.0, x2 = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    double filter = 1.0;
+    if (i < imin) filter = 0.5 * (1.0 + cos(M_PI * ((double)(i) / (double)(imin)-1.0)));
+    if (i > imax) filter = 0.5 * (1.0 + cos(M_PI * ((double)(i - imax) / (double)(Nwin))));
+    x1 += filter;
+    x2 += filter * filter;
+  }
+  a[threadIdx.x] = x1;
+  b[threadIdx.x] = x2;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      b[threadIdx.x] += b[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    s12[0] = a[0] / (double)N;
+    s12[1] = sqrt(b[0] / (double)N);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_spectra(const void* X, const double* SN, int nd, int N, double Tobs, void* W, double* pspec,
+                          double* wcheck, hipStream_t s) {
+  const size_t n = (size_t)nd * (N / 2 + 1);
+  // fac 2 of the reference's fac * 2.0 * (...), fac = Tobs / (N N)
+  const double pfac = (Tobs / ((double)N * (double)N)) * 2.0;
+  hipLaunchKernelGGL(k_spectra, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, static_cast<const double2*>(X),
+                     SN, nd, N, pfac, static_cast<double2*>(W), pspec, wcheck);
+  return hipGetLastError();
+}
+
+hipError_t launch_whitened_series(const double* w, int nd, int N, double Tobs, double* DW, hipStream_t s) {
+  const size_t n = (size_t)nd * N;
+  const double fac = sqrt((double)(N / 2)) / sqrt(Tobs);
+  hipLaunchKernelGGL(k_whitened_series, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, n, 1.0 / (double)N,
+                     fac, DW);
+  return hipGetLastError();
+}
+
+hipError_t launch_whiten(double* data, const double* Sn, int N, hipStream_t s) {
+  const int n = N / 2 + 1;
+  hipLaunchKernelGGL(k_whiten, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, data, Sn, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_tukey_scale(int N, double alpha, double* s12, hipStream_t s) {
+  hipLaunchKernelGGL(k_tukey_scale, dim3(1), dim3(256), 0, s, N, alpha, s12);
+  return hipGetLastError();
+}
 
 hipError_t launch_tukey(double* x, int nd, int N, double alpha, hipStream_t s) {
   const size_t n = (size_t)nd * N;

@@ -859,6 +859,33 @@ void PDwave(double* wavef, RealVector* freq, double tcmin, double* params, int N
   if (st) fail("PDwave", st);
 }
 
+void tukey(double* data, double alpha, int N) {
+  if (!data || N < 1) fail("tukey", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("tukey", st);
+  st = ingest_tukey(data, alpha, N, t_drop.stream);
+  if (st) fail("tukey", st);
+}
+
+void tukey_scale(double* s1, double* s2, double alpha, int N) {
+  if (!s1 || !s2 || N < 1) fail("tukey_scale", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("tukey_scale", st);
+  double s12[2];
+  st = ingest_tukey_scale(alpha, N, s12, t_drop.stream);
+  if (st) fail("tukey_scale", st);
+  *s1 = s12[0];
+  *s2 = s12[1];
+}
+
+void whiten(double* data, double* Sn, int N) {
+  if (!data || !Sn || N < 2 || (N & 1)) fail("whiten", QCBC_EINVAL);
+  int st = drop_init();
+  if (st) fail("whiten", st);
+  st = ingest_whiten(data, Sn, N, t_drop.stream);
+  if (st) fail("whiten", st);
+}
+
 void pbt_shift(double* corr, double* corrf, double* data1, double* data2, double* Sn, int imin, int imax, int N) {
   if (!corr || !corrf || !data1 || !data2 || !Sn || N < 4 || (N & (N - 1)) || imin < 1 || imax > N / 2)
     fail("pbt_shift", QCBC_EINVAL);

@@ -18,7 +18,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 ERRORS = {0: "QCBC_OK", 1: "QCBC_EINVAL", 12: "QCBC_ENOMEM", 14: "QCBC_EFAULT", 33: "QCBC_EDOM",
-          34: "QCBC_ERANGE", 100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY",
+          34: "QCBC_ERANGE", 5: "QCBC_EIO", 100: "QCBC_EHIP", 101: "QCBC_ENODEV", 102: "QCBC_ENOTBOUND", 103: "QCBC_EBUSY",
           104: "QCBC_ECANCELED", 105: "QCBC_ETIMEDOUT"}
 
 
@@ -135,6 +135,18 @@ def lib():
         L.qcbc_het_destroy.argtypes = [_vp]
         L.qcbc_bind_time_series.argtypes = [_vp, _dp, _dp, ctypes.c_double, _dp]
         L.qcbc_makespec.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_dp] * 8
+        L.qcbc_condition.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.qcbc_data_file_name.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int]
+        L.qcbc_read_framed.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp, _dp, _ip]
+        L.qcbc_read_spec.argtypes = [ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        L.qcbc_write_table.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dpp, ctypes.c_int]
+        L.tukey.restype = None
+        L.tukey.argtypes = [_dp, ctypes.c_double, ctypes.c_int]
+        L.tukey_scale.restype = None
+        L.tukey_scale.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.whiten.restype = None
+        L.whiten.argtypes = [_dp, _dp, ctypes.c_int]
         # reference-frame likelihood
         L.qcbc_loglike_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _ip]
         L.qcbc_templates_intrinsic.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
@@ -290,6 +302,19 @@ class Context(_Handle):
         D = np.zeros((self.an.nifo, self.an.N))
         _check(lib().qcbc_bind_time_series(self.h, _d(x), _d(SN), t_rise, _d(D)), "qcbc_bind_time_series")
         return D
+
+    def condition(self, x, SN, t_rise=0.4, fmin_floor=0.0, outputs=True):
+        """main's conditioning (and CBC_start's with fmin_floor) on the GPU, then bind: a dict of D and,
+        with outputs, DW (n_det x N), pspec and wcheck (n_det x N/2)."""
+        x = np.ascontiguousarray(x, np.float64)
+        SN = np.ascontiguousarray(SN, np.float64)
+        nd, N = self.an.nifo, self.an.N
+        out = {"D": np.zeros((nd, N))}
+        if outputs:
+            out.update(DW=np.zeros((nd, N)), pspec=np.zeros((nd, N // 2)), wcheck=np.zeros((nd, N // 2)))
+        g = [_d(out[k]) if k in out else None for k in ("D", "DW", "pspec", "wcheck")]
+        _check(lib().qcbc_condition(self.h, _d(x), _d(SN), t_rise, fmin_floor, *g), "qcbc_condition")
+        return out
 
     def makespec(self, ffit, Sspline, lines=None):
         """makespec on the GPU: SN (N/2); lines = (linef, lineh, lineQ, linew, deltafmax) arrays or None."""
@@ -1298,6 +1323,57 @@ def dropin_pdwave(an, params, tcmin):
     p = np.ascontiguousarray(params, np.float64)
     lib().PDwave(_d(w), ctypes.byref(fr), tcmin, _d(p), an.N)
     return w
+
+
+def dropin_tukey(data, alpha):
+    x = np.array(data, np.float64)
+    lib().tukey(_d(x), alpha, x.size)
+    return x
+
+
+def dropin_tukey_scale(alpha, N):
+    s1, s2 = ctypes.c_double(0), ctypes.c_double(0)
+    lib().tukey_scale(ctypes.byref(s1), ctypes.byref(s2), alpha, N)
+    return s1.value, s2.value
+
+
+def dropin_whiten(data, Sn):
+    x = np.array(data, np.float64)
+    S = np.ascontiguousarray(Sn, np.float64)
+    lib().whiten(_d(x), _d(S), x.size)
+    return x
+
+
+def data_file_name(kind, Tobs, ttrig, label):
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().qcbc_data_file_name(buf, 256, kind.encode(), Tobs, ttrig, label), "qcbc_data_file_name")
+    return buf.value.decode()
+
+
+def read_framed(path):
+    """A framed_* file: (time, strain) of the N pairs the reference counts."""
+    n = ctypes.c_int(0)
+    p = str(path).encode()
+    _check(lib().qcbc_read_framed(p, 0, None, None, ctypes.byref(n)), "qcbc_read_framed")
+    t, x = np.zeros(max(n.value, 1)), np.zeros(max(n.value, 1))
+    _check(lib().qcbc_read_framed(p, n.value, _d(t), _d(x), ctypes.byref(n)), "qcbc_read_framed")
+    return t[:n.value], x[:n.value]
+
+
+def read_spec(path, n2):
+    """A spec_* file: (SN, SM) of its first n2 rows."""
+    SN, SM = np.zeros(n2), np.zeros(n2)
+    _check(lib().qcbc_read_spec(str(path).encode(), n2, _d(SN), _d(SM)), "qcbc_read_spec")
+    return SN, SM
+
+
+def write_table(path, cols, digits):
+    """Columns written as main writes dataw.dat and the spectra: "%.{digits}e " per value, a newline per row."""
+    cs = [np.ascontiguousarray(c, np.float64) for c in cols]
+    nrows = cs[0].size if cs else 0
+    arr = (_dp * max(len(cs), 1))(*[c.ctypes.data_as(_dp) for c in cs])
+    _check(lib().qcbc_write_table(str(path).encode(), nrows, len(cs), ctypes.cast(arr, _dpp), digits),
+           "qcbc_write_table")
 
 
 def dropin_pbt_shift(data1, data2, Sn, imin, imax, N):
