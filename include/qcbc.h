@@ -185,6 +185,14 @@ void limits(struct Net *net, double *params, RealVector *freq, double **SN, int 
  * Fisher matrix of the full likelihood over the band, central differences of
  * fullphaseamp with epsilon 1e-5.  fish: 11 rows of 11. */
 void Fisher_All(struct Net *net, double **fish, double *params, RealVector *freq, double **SN, int N, double Tobs);
+/* Replaces Fisher_Full, QuickCBC.c:2551-2688 (prototype QuickCBC.h:150):
+ * MCMC_intrinsic's Fisher matrix of the first d (<= 7) reference-frame
+ * parameters over the band, central differences of AmpPhase (the bare h22,
+ * :3395-3444) with epsilon 1e-5, detector l weighted by 4 (l = 0) or 4 r_l^2
+ * (r_l = params[7 + 3 (l-1) + 2]); diagonal entries at least 1; 1e4 I when a
+ * spin is at 0.9999 or more.  fish: d rows of d. */
+void Fisher_Full(struct Net *net, double **fish, int d, double *params, RealVector *freq, double **SN, int N,
+                 double Tobs);
 /* Replaces diff, QuickCBC.c:4913-4964 (prototype QuickCBC.h:103): sum over
  * detectors and bins [max(fmin, fbegin) Tobs, 2 f_ringdown Tobs) of
  * 4 |h(params) - h(dparams)|^2 / S. */
@@ -626,6 +634,12 @@ int qcbc_limits(qcbc_ctx *ctx, const double *params, int *band, double *ratio);
 /* Fisher_All at B points (B x 11): fish B x 11 x 11, status B (may be NULL;
  * QCBC_EDOM for a perturbed point outside the model, the matrix then 1e4 I). */
 int qcbc_fisher_all(qcbc_ctx *ctx, const double *params, int B, double *fish, int *status);
+/* Fisher_Full (QuickCBC.c:2551-2688) of B reference-frame points (B x (7 + 3
+ * (n_det - 1)), host memory) on the bound PSD over the analysis's band:
+ * fish B x d x d (d <= 7), status B (may be NULL; QCBC_EDOM for a point
+ * outside the model, whose matrix is 1e4 I). */
+int qcbc_fisher_full(qcbc_ctx *ctx, const double *params, int B, int d, double *fish, int *status);
+
 /* diff between start point params[of[k]] (of B) and px[k], k < K: z (K). */
 int qcbc_diff(qcbc_ctx *ctx, const double *params, int B, const double *px, const int *of, int K, double *z);
 /* efix on the full data for B points: ej (B x 11) rescaled in place along

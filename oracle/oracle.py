@@ -632,6 +632,19 @@ def fisher_all(labels, gmst, fref, fmin, fmax, SN, N, Tobs, params):
     return st, fish
 
 
+def fisher_full(nifo, fref, fmin, fmax, SN, N, Tobs, params, d=7):
+    """Fisher_Full (QuickCBC.c:2551) of a reference-frame point: (status, fish d x d)."""
+    L = lib()
+    L.qo_fisher_full_flat.restype = ctypes.c_int
+    L.qo_fisher_full_flat.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                                              ctypes.c_double, _dp]
+    SN = np.ascontiguousarray(SN, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    fish = np.zeros((d, d))
+    st = L.qo_fisher_full_flat(nifo, fref, fmin, fmax, _d(SN), _d(p), d, N, Tobs, _d(fish))
+    return st, fish
+
+
 def diff(labels, gmst, fref, fmin, SN, N, Tobs, params, dparams):
     """diff (QuickCBC.c:4913)."""
     L = lib()

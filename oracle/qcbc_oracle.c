@@ -2543,6 +2543,112 @@ int qo_fisher_all(int nifo, const int *labels, double gmst, double fref, double 
   return st;
 }
 
+/* AmpPhase, QuickCBC.c:3395-3444: the bare h22 of the reference-frame
+ * parameters (phi0 = params[4] itself) on the grid: Af = h22fac amp and Pf =
+ * 2 pi f (Tobs - params[5]) - phase for bins 1 .. N/2 - 1 (bin 0 left 0). */
+int qo_amp_phase(double *Af, double *Pf, const double *freq, const double *params, double fref, int N) {
+  double Tobs = 1.0 / freq[1];
+  double mc = qo_exp_cr(params[0]), mt = qo_exp_cr(params[1]);
+  double eta = qo_pow_cr((mc / mt), (5.0 / 3.0));
+  double dm = eta > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta);
+  double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  double ts = Tobs - params[5];
+  double *amp = malloc(sizeof(double) * (N / 2)), *phase = malloc(sizeof(double) * (N / 2));
+  int st, i;
+  if (!amp || !phase) {
+    free(amp);
+    free(phase);
+    return QO_ENOMEM;
+  }
+  st = qo_h22_ampphase(freq, N / 2, params[4], fref, m1_SI, m2_SI, params[2], params[3], qo_exp_cr(params[6]), amp,
+                       phase);
+  Af[0] = Pf[0] = 0.0;
+  for (i = 1; i < N / 2 && st == 0; i++) {
+    double f = freq[i];
+    Pf[i] = QO_TPI * f * ts - phase[i];
+    Af[i] = QO_H22FAC * amp[i];
+  }
+  free(amp);
+  free(phase);
+  return st;
+}
+
+/* Fisher_Full, QuickCBC.c:2551-2688, for d <= 7 parameters of a
+ * reference-frame point (7 + 3 (nifo - 1) entries): 1e4 I, and when both
+ * spins are below 0.9999 in magnitude, the central differences (epsilon
+ * 1e-5) of AmpPhase in params[0 .. d-1] summed over [(int)(Tobs fmin),
+ * min((int)(Tobs fmax), N/2)) as Scale (dA_i dA_j + A^2 dP_i dP_j) / S_l
+ * per detector l, Scale 4 for detector 0 and 4 r_l^2 (r_l =
+ * params[7 + 3 (l-1) + 2]) for the others, detector by detector in the
+ * reference's order; mirrored; diagonal entries below 1 set to 1.  A
+ * failing set-up (where the reference would read a null waveform) leaves
+ * 1e4 I and returns its status. */
+int qo_fisher_full(int nifo, double fref, double fmin, double fmax, double **SN, const double *params, int d, int N,
+                   double Tobs, double *fish) {
+  const double epsilon = 1.0e-5;
+  const int n = N / 2;
+  int imin = (int)(Tobs * fmin), imax = (int)(Tobs * fmax), i, j, k, l, st = 0;
+  double *freq, *AH, *PH, *AHP, *PHP, *AHM, *PHM, pp[7], pm[7];
+  if (imax > n) imax = n;
+  for (i = 0; i < d * d; i++) fish[i] = (i % (d + 1) == 0) ? 1.0e4 : 0.0;
+  if (!(fabs(params[2]) < 0.9999 && fabs(params[3]) < 0.9999)) return 0;
+  freq = qo_make_freq(N, Tobs);
+  AH = malloc(sizeof(double) * (size_t)n * (2 + 4 * d));
+  if (!freq || !AH) {
+    free(freq);
+    free(AH);
+    return QO_ENOMEM;
+  }
+  PH = AH + n;
+  AHP = PH + n; /* [i][k] */
+  PHP = AHP + (size_t)d * n;
+  AHM = PHP + (size_t)d * n;
+  PHM = AHM + (size_t)d * n;
+  st = qo_amp_phase(AH, PH, freq, params, fref, N);
+  for (i = 0; i < d && st == 0; i++) {
+    for (k = 0; k < 7; k++) pp[k] = pm[k] = params[k];
+    pp[i] += epsilon;
+    pm[i] -= epsilon;
+    st = qo_amp_phase(AHP + (size_t)i * n, PHP + (size_t)i * n, freq, pp, fref, N);
+    if (st == 0) st = qo_amp_phase(AHM + (size_t)i * n, PHM + (size_t)i * n, freq, pm, fref, N);
+    for (k = 0; k < n && st == 0; k++) {
+      AHP[(size_t)i * n + k] -= AHM[(size_t)i * n + k];
+      PHP[(size_t)i * n + k] -= PHM[(size_t)i * n + k];
+      AHP[(size_t)i * n + k] /= (2.0 * epsilon);
+      PHP[(size_t)i * n + k] /= (2.0 * epsilon);
+    }
+  }
+  if (st == 0) {
+    for (i = 0; i < d * d; i++) fish[i] = 0.0;
+    for (l = 0; l < nifo; l++) {
+      const double Scale = (l == 0) ? 4.0 : 4.0 * params[(l - 1) * 3 + 7 + 2] * params[(l - 1) * 3 + 7 + 2];
+      for (i = 0; i < d; i++)
+        for (j = i; j < d; j++) {
+          const double *ai = AHP + (size_t)i * n, *aj = AHP + (size_t)j * n;
+          const double *pi = PHP + (size_t)i * n, *pj = PHP + (size_t)j * n;
+          for (k = imin; k < imax; k++)
+            fish[i * d + j] += Scale * (ai[k] * aj[k] + AH[k] * AH[k] * pi[k] * pj[k]) / SN[l][k];
+        }
+    }
+    for (i = 0; i < d; i++)
+      for (j = i + 1; j < d; j++) fish[j * d + i] = fish[i * d + j];
+    for (i = 0; i < d; i++)
+      if (fish[i * d + i] < 1.0) fish[i * d + i] = 1.0;
+  }
+  free(freq);
+  free(AH);
+  return st;
+}
+
+int qo_fisher_full_flat(int nifo, double fref, double fmin, double fmax, const double *SN, const double *params, int d,
+                        int N, double Tobs, double *fish) {
+  double *Sr[3];
+  int i;
+  qo_spline_init();
+  for (i = 0; i < nifo; i++) Sr[i] = (double *)SN + (size_t)i * (N / 2);
+  return qo_fisher_full(nifo, fref, fmin, fmax, Sr, params, d, N, Tobs, fish);
+}
+
 /* diff, QuickCBC.c:4913-4964: sum_d sum_j 4 |h_d(params) - h_d(dparams)|^2 /
  * S_d(j), the waveforms as fullphaseamp's amplitude and phase, over bins
  * [first bin of params, M/2) with M = min(2 signal bins of params, N).  A

@@ -170,6 +170,9 @@ int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspli
                 const double *lineh, const double *lineQ, const double *linew, const double *deltafmax, double *SN,
                 double Tobs, int N);
 
+int qo_amp_phase(double *Af, double *Pf, const double *freq, const double *params, double fref, int N);
+int qo_fisher_full_flat(int nifo, double fref, double fmin, double fmax, const double *SN, const double *params, int d,
+                        int N, double Tobs, double *fish);
 int qo_read_framed(const char *path, int cap, double *time, double *x);
 int qo_read_spec(const char *path, int n2, double *SN, double *SM);
 

@@ -103,6 +103,8 @@ int het_grid_view(const qcbc_het* h, HetGridView* out);
 // limits (QuickCBC.c:4275-4385): stst[2], ratio (N/2, may be null)
 int limits_impl(qcbc_ctx* c, const double* const* SN, const double* params, int* stst, double* ratio, hipStream_t s);
 // Fisher_All (QuickCBC.c:2695-2817) of B points (host memory)
+int fisher_full_impl(qcbc_ctx* c, const double* d_SN, const double* params, int B, int d, double* fish, int* status,
+                     hipStream_t s);
 int fisher_all_impl(qcbc_ctx* c, const double* d_SN, const double* params, int B, double* fish, int* status,
                     hipStream_t s);
 // diff (QuickCBC.c:4913-4964), or het_diff (:4966-5053) when h is given, of

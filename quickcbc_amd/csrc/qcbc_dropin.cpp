@@ -707,6 +707,20 @@ void Fisher_All(struct Net* net, double** fish, double* params, RealVector* freq
   for (int i = 0; i < 11; i++) std::memcpy(fish[i], F + 11 * i, sizeof(double) * 11);
 }
 
+void Fisher_Full(struct Net* net, double** fish, int d, double* params, RealVector* freq, double** SN, int N,
+                 double Tobs) {
+  if (!fish || !params || !SN || d < 1 || d > 7) fail("Fisher_Full", QCBC_EINVAL);
+  qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "Fisher_Full");
+  const double* d_SN = psd_on_device(net, SN, N, "Fisher_Full");
+  double F[7 * 7];
+  int pst = 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int st = fisher_full_impl(ctx, d_SN, params, 1, d, F, &pst, ctx->stream);
+  if (st) fail("Fisher_Full", st);
+  domain_error("Fisher_Full", pst);  // the matrix is then 1e4 I
+  for (int i = 0; i < d; i++) std::memcpy(fish[i], F + d * i, sizeof(double) * d);
+}
+
 double diff(struct Net* net, double* params, double* dparams, RealVector* freq, double** SM, int N, double Tobs) {
   if (!params || !dparams || !SM) fail("diff", QCBC_EINVAL);
   qcbc_ctx* ctx = grid_ctx(net, freq, N, Tobs, "diff");

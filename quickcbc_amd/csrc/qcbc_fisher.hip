@@ -221,6 +221,151 @@ __global__ void k_fisher_reduce(const Point* __restrict__ pts, const double* __r
   fish[(size_t)b * NP * NP + c * NP + r] = x;
 }
 
+
+// ---------------------------------------------------------------- Fisher_Full
+// Fisher_Full (QuickCBC.c:2551-2688), MCMC_intrinsic's Fisher matrix of the
+// seven reference-frame parameters over the band [imin, imax) of every bin:
+// AmpPhase (:3395-3444) at the point and at params[k] +- 1e-5, k < 7, gives
+// the bare h22 A = h22fac amp and phase P = 2 pi f ts - phase (no taper, no
+// projection, ts = Tobs - params[5]), and
+//   fish[i][j] = sum_l Scale_l sum_k (dA_i dA_j + A^2 dP_i dP_j) / S_l[k]
+// with Scale_0 = 4 and Scale_l = 4 r_l^2 (r_l the amplitude ratio
+// params[7 + 3 (l-1) + 2]); diagonal entries below 1 become 1.  With |chi1|
+// or |chi2| >= 0.9999 the matrix is 1e4 I.  Here the detector sum is one
+// weight per bin, sum_l Scale_l / S_l[k] (a regrouping of the reference's
+// sums).  A d < 7 matrix of the reference is the top-left block of this one.
+constexpr int FF_NP = FISHER_FULL_NP, FF_NV = 2 * FF_NP + 1, FF_NPAIR = FF_NP * (FF_NP + 1) / 2;
+
+__device__ __forceinline__ void ff_pair_of(int n, int* i, int* j) {
+  int r = 0;
+  while (n >= FF_NP - r) {
+    n -= FF_NP - r;
+    r++;
+  }
+  *i = r;
+  *j = r + n;
+}
+
+// the FF_NV parameter sets (seven entries each) of B points of ips entries
+__global__ void k_fisher_full_params(const double* __restrict__ p, int ips, int B, double* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * FF_NV) return;
+  const int b = i / FF_NV, v = i % FF_NV;
+  double x[FF_NP];
+#pragma unroll
+  for (int k = 0; k < FF_NP; k++) x[k] = p[(size_t)b * ips + k];
+  if (v > 0) {
+    const int k = (v - 1) >> 1;
+#pragma unroll
+    for (int j = 0; j < FF_NP; j++)
+      if (j == k) x[j] = (v & 1) ? x[j] + EPS : x[j] - EPS;
+  }
+#pragma unroll
+  for (int k = 0; k < FF_NP; k++) q[(size_t)i * FF_NP + k] = x[k];
+}
+
+__device__ __forceinline__ bool ff_evaluated(const Point* P, const double* prm, int* st) {
+  *st = ST_OK;
+  for (int v = 0; v < FF_NV && *st == ST_OK; v++) *st = P[v].status;
+  const bool spins = fabs(prm[2]) < 0.9999 && fabs(prm[3]) < 0.9999;
+  if (!spins) *st = ST_OK;
+  return spins && *st == ST_OK;
+}
+
+// one 256-thread block per point and band slice [split len, (split+1) len):
+// each thread owns samples t, t + 256, ..., evaluates the 15 variants there
+// and accumulates the 28 upper-triangle sums; waves reduce by shuffles, the
+// block in LDS; partial[(b splits + split) 28 + n]
+template <int ND>
+__global__ __launch_bounds__(256) void k_fisher_full(const Point* __restrict__ pts, const double* __restrict__ q,
+                                                     const double* __restrict__ params, int ips, int B,
+                                                     const double* __restrict__ SN, int n2, int imin, int nb,
+                                                     double Tobs, int len, double* __restrict__ partial) {
+  __shared__ double red[4][FF_NPAIR];
+  __shared__ double sdA[FF_NP][256], sdP[FF_NP][256];  // each thread's own column
+  const int b = blockIdx.x, split = blockIdx.y, splits = gridDim.y, t = threadIdx.x;
+  if (b >= B) return;
+  const Point* P = pts + (size_t)b * FF_NV;
+  const double* prm = params + (size_t)b * ips;
+  int st;
+  if (!ff_evaluated(P, prm, &st)) return;  // block-uniform; the reduce writes 1e4 I
+  double scale[ND];
+  scale[0] = 4.0;
+#pragma unroll
+  for (int l = 1; l < ND; l++) {
+    const double r = prm[(l - 1) * 3 + FF_NP + 2];
+    scale[l] = 4.0 * r * r;
+  }
+  double acc[FF_NPAIR];
+#pragma unroll
+  for (int n = 0; n < FF_NPAIR; n++) acc[n] = 0.0;
+  const int m_end = min(nb, (split + 1) * len);
+  for (int m = split * len + t; m < m_end; m += 256) {
+    const int k = imin + m;
+    const double f = (double)k / Tobs;
+    double A0 = 0.0;
+    {
+      double a, ph;
+      h22_at(P[0], f, &a, &ph);
+      A0 = H22FAC * a;
+    }
+#pragma unroll 1
+    for (int i = 0; i < FF_NP; i++) {
+      double aP, pP, aM, pM;
+      h22_at(P[2 * i + 1], f, &aP, &pP);
+      h22_at(P[2 * i + 2], f, &aM, &pM);
+      const double tsP = Tobs - q[((size_t)b * FF_NV + 2 * i + 1) * FF_NP + 5];
+      const double tsM = Tobs - q[((size_t)b * FF_NV + 2 * i + 2) * FF_NP + 5];
+      double A = H22FAC * aP - H22FAC * aM;
+      double X = (TPI * f * tsP - pP) - (TPI * f * tsM - pM);
+      sdA[i][t] = A / (2.0 * EPS);
+      sdP[i][t] = X / (2.0 * EPS);
+    }
+    double wk = 0.0;
+#pragma unroll
+    for (int l = 0; l < ND; l++) wk += scale[l] / SN[(size_t)l * n2 + k];
+    const double A2 = A0 * A0;
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < FF_NP; i++)
+#pragma unroll
+      for (int j = i; j < FF_NP; j++, n++) acc[n] += wk * (sdA[i][t] * sdA[j][t] + A2 * sdP[i][t] * sdP[j][t]);
+  }
+  const int wave = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int n = 0; n < FF_NPAIR; n++) {
+    const double v = het_wave_sum(acc[n]);
+    if (lane == 0) red[wave][n] = v;
+  }
+  __syncthreads();
+  if (t < FF_NPAIR) partial[((size_t)b * splits + split) * FF_NPAIR + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+}
+
+// the matrices: 1e4 I where nothing was evaluated, else the splits' sums in
+// split order, mirrored, the diagonal at least 1; fish B x 7 x 7
+__global__ void k_fisher_full_reduce(const Point* __restrict__ pts, const double* __restrict__ params, int ips, int B,
+                                     int splits, const double* __restrict__ partial, double* __restrict__ fish,
+                                     int* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * FF_NPAIR) return;
+  const int b = i / FF_NPAIR, n = i % FF_NPAIR;
+  int r, c;
+  ff_pair_of(n, &r, &c);
+  double* F = fish + (size_t)b * FF_NP * FF_NP;
+  int st;
+  double x;
+  if (ff_evaluated(pts + (size_t)b * FF_NV, params + (size_t)b * ips, &st)) {
+    x = 0.0;
+    for (int sp = 0; sp < splits; sp++) x += partial[((size_t)b * splits + sp) * FF_NPAIR + n];
+    if (r == c && x < 1.0) x = 1.0;
+  } else {
+    x = (r == c) ? 1.0e4 : 0.0;
+  }
+  F[r * FF_NP + c] = x;
+  F[c * FF_NP + r] = x;
+  if (n == 0) status[b] = st;
+}
+
 // FisherEvec (QuickCBC.c:2449-2546) of d x d matrices (d <= 11: the
 // reference calls it with NP = 11, NX = 7 and NS = 7, ConstCBC.h:67-69): the
 // cyclic Jacobi method (Golub & Van Loan, Alg. 8.5.3, row-cyclic order) in
@@ -384,6 +529,33 @@ hipError_t launch_het_fisher(const Point* pts, const double* d_params, int B, in
   if (splits > 1)
     hipLaunchKernelGGL(k_fisher_reduce, dim3((B * NPAIR + 255) / 256), dim3(256), 0, s, pts, d_params, B, splits,
                        partial, fish);
+  return hipGetLastError();
+}
+
+hipError_t launch_fisher_full_params(const double* d_params, int ips, int B, double* d_q, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fisher_full_params, dim3((B * FF_NV + 255) / 256), dim3(256), 0, s, d_params, ips, B, d_q);
+  return hipGetLastError();
+}
+
+hipError_t launch_fisher_full(const Point* pts, const double* d_q, const double* d_params, int ips, int B, int nd,
+                              const double* SN, int n2, int imin, int nb, double Tobs, int splits, double* partial,
+                              double* fish, int* status, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  if (splits < 1 || nb < 1 || !partial || nd < 1 || nd > 3) return hipErrorInvalidValue;
+  const int len = (nb + splits - 1) / splits;
+  const dim3 grid(B, splits);
+  if (nd == 1)
+    hipLaunchKernelGGL(k_fisher_full<1>, grid, dim3(256), 0, s, pts, d_q, d_params, ips, B, SN, n2, imin, nb, Tobs,
+                       len, partial);
+  else if (nd == 2)
+    hipLaunchKernelGGL(k_fisher_full<2>, grid, dim3(256), 0, s, pts, d_q, d_params, ips, B, SN, n2, imin, nb, Tobs,
+                       len, partial);
+  else
+    hipLaunchKernelGGL(k_fisher_full<3>, grid, dim3(256), 0, s, pts, d_q, d_params, ips, B, SN, n2, imin, nb, Tobs,
+                       len, partial);
+  hipLaunchKernelGGL(k_fisher_full_reduce, dim3((B * FF_NPAIR + 255) / 256), dim3(256), 0, s, pts, d_params, ips, B,
+                     splits, partial, fish, status);
   return hipGetLastError();
 }
 

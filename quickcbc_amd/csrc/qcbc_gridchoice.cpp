@@ -648,9 +648,54 @@ int fisher_all_impl(qcbc_ctx* c, const double* d_SN, const double* params, int B
   return QCBC_OK;
 }
 
+// Fisher_Full (QuickCBC.c:2551-2688) of B reference-frame points (host
+// memory, 7 + 3 (n_det - 1) each) on the band bins of c, SN on the device:
+// fish B x d x d (the top-left d x d block of the seven-parameter matrix),
+// status B (may be null)
+int fisher_full_impl(qcbc_ctx* c, const double* d_SN, const double* params, int B, int d, double* fish, int* status,
+                     hipStream_t s) {
+  const int nd = c->desc.n_det, ips = 7 + 3 * (nd - 1), nb = c->nb, imin = c->an.imin;
+  if (nb < 1 || d < 1 || d > FISHER_FULL_NP) return QCBC_EINVAL;
+  const int splits = std::max(1, std::min((nb + 1023) / 1024, (2048 + B - 1) / B));
+  Scratch mem;
+  double *d_par, *q, *partial, *d_fish;
+  Point* pts;
+  int *pst, *d_st;
+  int st;
+  if ((st = mem.alloc((size_t)B * ips, &d_par)) || (st = mem.alloc((size_t)B * FISHER_FULL_NV * FISHER_FULL_NP, &q)) ||
+      (st = mem.alloc((size_t)B * FISHER_FULL_NV, &pts)) || (st = mem.alloc((size_t)B * FISHER_FULL_NV, &pst)) ||
+      (st = mem.alloc((size_t)B * splits * FISHER_FULL_NPAIR, &partial)) ||
+      (st = mem.alloc((size_t)B * FISHER_FULL_NP * FISHER_FULL_NP, &d_fish)) || (st = mem.alloc((size_t)B, &d_st)))
+    return st;
+  HIPCHK(hipMemcpyAsync(d_par, params, sizeof(double) * ips * B, hipMemcpyHostToDevice, s));
+  HIPCHK(launch_fisher_full_params(d_par, ips, B, q, s));
+  // AmpPhase's model: phi0 = params[4] itself, as templates() hands it over
+  HIPCHK(launch_setup_intrinsic(q, FISHER_FULL_NP, B * FISHER_FULL_NV, c->an, pts, pst, s));
+  HIPCHK(launch_fisher_full(pts, q, d_par, ips, B, nd, d_SN, c->nhalf, imin, nb, c->desc.Tobs, splits, partial, d_fish,
+                            d_st, s));
+  std::vector<double> F((size_t)B * FISHER_FULL_NP * FISHER_FULL_NP);
+  HIPCHK(hipMemcpyAsync(F.data(), d_fish, sizeof(double) * F.size(), hipMemcpyDeviceToHost, s));
+  if (status) HIPCHK(hipMemcpyAsync(status, d_st, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int b = 0; b < B; b++)
+    for (int i = 0; i < d; i++)
+      for (int j = 0; j < d; j++)
+        fish[((size_t)b * d + i) * d + j] = F[((size_t)b * FISHER_FULL_NP + i) * FISHER_FULL_NP + j];
+  return QCBC_OK;
+}
+
 }  // namespace qcbc
 
 extern "C" {
+
+int qcbc_fisher_full(qcbc_ctx* c, const double* params, int B, int d, double* fish, int* status) {
+  if (!c || B < 0 || (B > 0 && (!params || !fish)) || d < 1 || d > FISHER_FULL_NP) return QCBC_EINVAL;
+  if (B == 0) return QCBC_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->bound) return QCBC_ENOTBOUND;
+  hipSetDevice(c->device);
+  return fisher_full_impl(c, c->d_SN, params, B, d, fish, status, c->stream);
+}
 
 int qcbc_fisher_all(qcbc_ctx* c, const double* params, int B, double* fish, int* status) {
   if (!c || B < 0 || (B > 0 && (!params || !fish))) return QCBC_EINVAL;

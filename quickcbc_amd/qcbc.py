@@ -187,6 +187,10 @@ def lib():
         # the heterodyne's grid choice
         L.qcbc_limits.argtypes = [_vp, _dp, _ip, _dp]
         L.qcbc_fisher_all.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip]
+        L.qcbc_fisher_full.argtypes = [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _ip]
+        L.Fisher_Full.restype = None
+        L.Fisher_Full.argtypes = [ctypes.POINTER(Net), _dpp, ctypes.c_int, _dp, ctypes.POINTER(RealVector), _dpp,
+                                  ctypes.c_int, ctypes.c_double]
         L.qcbc_diff.argtypes = [_vp, _dp, ctypes.c_int, _dp, _ip, ctypes.c_int, _dp]
         L.qcbc_efix.argtypes = [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _ip]
         L.qcbc_het_space.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_double, _ip, _ip, _ip, _ip, _ip, ctypes.c_int,
@@ -397,6 +401,16 @@ class Context(_Handle):
         F = np.zeros((B, 11, 11))
         st = np.zeros(B, np.int32)
         _check(lib().qcbc_fisher_all(self.h, _d(p), B, _d(F), _i(st)), "qcbc_fisher_all")
+        return F, st
+
+    def fisher_full(self, params, d=7):
+        """Fisher_Full for a batch of reference-frame points: (fish B x d x d, status)."""
+        ips = 7 + 3 * (self.an.nifo - 1)
+        p = np.ascontiguousarray(params, np.float64).reshape(-1, ips)
+        B = p.shape[0]
+        F = np.zeros((B, d, d))
+        st = np.zeros(B, np.int32)
+        _check(lib().qcbc_fisher_full(self.h, _d(p), B, d, _d(F), _i(st)), "qcbc_fisher_full")
         return F, st
 
     def diff(self, params, px, of=None):
@@ -1240,6 +1254,15 @@ def dropin_fisher_all(an, SNrows, params):
     p = np.ascontiguousarray(params, np.float64)
     F = RowMatrix(np.zeros((11, 11)))
     lib().Fisher_All(ctypes.byref(net), F.p, _d(p), ctypes.byref(fr), SNrows.p, an.N, an.Tobs)
+    return F.arr
+
+
+def dropin_fisher_full(an, SNrows, params, d=7):
+    net = make_net(an)
+    fr = make_freq(an)
+    p = np.ascontiguousarray(params, np.float64)
+    F = RowMatrix(np.zeros((d, d)))
+    lib().Fisher_Full(ctypes.byref(net), F.p, d, _d(p), ctypes.byref(fr), SNrows.p, an.N, an.Tobs)
     return F.arr
 
 
