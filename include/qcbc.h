@@ -551,6 +551,27 @@ int qcbc_server_wait(qcbc_server *s, int64_t request_id, double *logL, double *r
                      double *latency_ms, int timeout_ms);
 int qcbc_server_stats(qcbc_server *s, size_t *bytes_in_use, int *n_analyses, long long *evals,
                       long long *batches);
+/* Admit an analysis with a resident parallel-tempered ensemble: n_temps x
+ * n_chains points (init (n_temps*n_chains) x 11, point t*n_chains + c,
+ * temperatures ladder^t, jiggle scales sigma (11) at temperature 1, random
+ * numbers from seed and the ensemble's step counter), charged to the budget
+ * with the analysis.  Its states, logL, pending proposals and counters stay
+ * on the GPU.  At most 65536 points. */
+int qcbc_server_admit_pt(qcbc_server *s, const qcbc_desc *desc, const double *D, const double *SN,
+                         int n_temps, int n_chains, double ladder, const double *init,
+                         const double *sigma, uint64_t seed, int *analysis_id);
+/* Advance an admitted ensemble by nsteps PT steps (propose, evaluate,
+ * Metropolis accept, adjacent-temperature swaps; the steps of one ensemble
+ * run in order, one per batch, beside other analyses' steps and likelihood
+ * requests).  Wait with qcbc_server_wait (logL, rho, status NULL; latency is
+ * submit to the last step's completion).  An ensemble's chain does not
+ * depend on what else the server runs. */
+int qcbc_server_submit_steps(qcbc_server *s, int analysis_id, int nsteps, int64_t *request_id);
+/* An ensemble's state between steps: params ((n_temps*n_chains) x 11), logL,
+ * accepted moves per point, swaps per chain column, steps run (each may be
+ * NULL). */
+int qcbc_server_get_pt(qcbc_server *s, int analysis_id, double *params, double *logL, long long *accepted,
+                       long long *swaps, long long *steps);
 int qcbc_server_destroy(qcbc_server *s);
 
 /* ------------------------------------------------------------------ *

@@ -88,7 +88,7 @@ static int ctx_reserve(qcbc_ctx* c, int B) {
   return QCBC_OK;
 }
 
-static int make_analysis(const qcbc_desc* d, AnalysisConst* an, int* nhalf, int* nb, int* nbp) {
+int qcbc::make_analysis(const qcbc_desc* d, AnalysisConst* an, int* nhalf, int* nb, int* nbp) {
   if (!d || d->n_det < 1 || d->n_det > 3 || d->N < 4 || (d->N & 1) || !(d->Tobs > 0)) return QCBC_EINVAL;
   for (int i = 0; i < d->n_det; i++)
     if (d->labels[i] < 0 || d->labels[i] > 2) return QCBC_EINVAL;

@@ -68,6 +68,9 @@ int ensure_device();
 // (called once per device, with dev current), and its size in bytes
 int noalloc_init_device(int dev);
 size_t noalloc_scratch_bytes();
+// the analysis constants, half length, band bins and padded band bins of a
+// description (QCBC_EINVAL for an invalid one)
+int make_analysis(const qcbc_desc* d, AnalysisConst* an, int* nhalf, int* nb, int* nbp);
 int effective_splits(const qcbc_ctx* c, int prec, int B);
 LLInst make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial);
 // the fused likelihood launch on stream s, bracketed by timing events when

@@ -906,6 +906,11 @@ class Server:
         L.qcbc_server_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t), _ip,
                                         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
         L.qcbc_server_destroy.argtypes = [_vp]
+        L.qcbc_server_admit_pt.argtypes = [_vp, ctypes.POINTER(Desc), _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _dp, _dp, ctypes.c_uint64, _ip]
+        L.qcbc_server_submit_steps.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.qcbc_server_get_pt.argtypes = [_vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_longlong),
+                                         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
         h = _vp()
         _check(L.qcbc_server_create(budget_bytes, precision, ctypes.byref(h)), "qcbc_server_create")
         self.h = h
@@ -922,6 +927,39 @@ class Server:
         self.nd[aid.value] = an.nifo
         return 0, aid.value
 
+    def admit_pt(self, an, D, SN, n_temps, n_chains, ladder, init, sigma, seed):
+        """Admit an analysis with a resident PT ensemble: (code, analysis id or None)."""
+        d = make_desc(an)
+        D = np.ascontiguousarray(D, np.float64)
+        SN = np.ascontiguousarray(SN, np.float64)
+        x0 = np.ascontiguousarray(init, np.float64).reshape(n_temps * n_chains, 11)
+        sg = np.ascontiguousarray(sigma, np.float64)
+        aid = ctypes.c_int(0)
+        code = lib().qcbc_server_admit_pt(self.h, ctypes.byref(d), _d(D), _d(SN), n_temps, n_chains, ladder, _d(x0),
+                                          _d(sg), seed, ctypes.byref(aid))
+        if code != 0:
+            return code, None
+        self.nd[aid.value] = an.nifo
+        self.P = getattr(self, "P", {})
+        self.P[aid.value] = (n_temps * n_chains, n_chains)
+        return 0, aid.value
+
+    def submit_steps(self, aid, nsteps=1):
+        rid = ctypes.c_int64(0)
+        _check(lib().qcbc_server_submit_steps(self.h, aid, nsteps, ctypes.byref(rid)), "qcbc_server_submit_steps")
+        return rid.value, 0, aid
+
+    def get_pt(self, aid):
+        """An ensemble's state: dict of params, logL, accepted, swaps, steps."""
+        P, nc = self.P[aid]
+        x = np.zeros((P, 11))
+        L = np.zeros(P)
+        acc = (ctypes.c_longlong * P)()
+        sw = (ctypes.c_longlong * nc)()
+        n = ctypes.c_longlong(0)
+        _check(lib().qcbc_server_get_pt(self.h, aid, _d(x), _d(L), acc, sw, ctypes.byref(n)), "qcbc_server_get_pt")
+        return {"params": x, "logL": L, "accepted": np.array(acc[:]), "swaps": np.array(sw[:]), "steps": n.value}
+
     def evict(self, aid):
         return lib().qcbc_server_evict(self.h, aid)
 
@@ -934,9 +972,9 @@ class Server:
     def wait(self, ticket, timeout_ms=-1):
         rid, B, aid = ticket
         nd = self.nd.get(aid, 3)
-        logL = np.zeros(B)
-        rho = np.zeros((B, nd))
-        st = np.zeros(B, np.int32)
+        logL = np.zeros(max(B, 1))
+        rho = np.zeros((max(B, 1), nd))
+        st = np.zeros(max(B, 1), np.int32)
         lat = ctypes.c_double(0)
         code = lib().qcbc_server_wait(self.h, rid, _d(logL), _d(rho), _i(st), ctypes.byref(lat), timeout_ms)
         return code, logL, rho, st, lat.value
