@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-steps", type=int, default=200, help="PT-step requests per analysis")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="analysis arrivals per second")
+    ap.add_argument("--serve-bench-seconds", type=float, default=8.0,
+                    help="seconds of the serving stream inside the default run (0: none)")
     ap.add_argument("--draws", type=int, default=100000)
     ap.add_argument("--het-launches", type=int, default=500)
     ap.add_argument("--only-het", action="store_true", help="time the heterodyned likelihood only (for profiling)")
@@ -521,6 +523,16 @@ def main():
         extra["het"] = het_bench(Q, W, an, ctx, inj, D, SN, init, args.het_launches, args.cpu_seconds / 3,
                                  args.no_cpu)
         extra["intrinsic"] = intrinsic_bench(Q, an, ctx, init, D, SN, args.no_cpu)
+        if args.serve_bench_seconds > 0:
+            # the serving form, briefly: Poisson arrivals of analyses with resident PT ensembles
+            from quickcbc_amd import bench_configs as C
+            sv = C.serving(seconds=args.serve_bench_seconds, rate=args.serve_rate, lifetime_steps=args.serve_steps,
+                           log=log)
+            extra["serving_mixed_logl_per_s"] = sv["logl_per_s"]
+            extra["serving_step_ms_p50"] = sv["step_latency_ms_p50"]
+            extra["serving_step_ms_p99"] = sv["step_latency_ms_p99"]
+            extra["serving"] = sv
+            extra["serving_mixed_over_plain"] = sv["logl_per_s"] / extra["plain_fp64_logl_per_s"]
         log(json.dumps(extra))
 
     cpu = None

@@ -11,10 +11,11 @@ returns a dict of measurements.  bench.py calls them with --config.
   precision(...)  mixed precision against FP64 at Tobs = 16 s, 3 detectors:
                   |dlogL| over many draws and over injections at network SNR
                   10, 25 and 80, and the throughput ratio.
-  serving(...)    Poisson arrivals of analyses (sizes as in uneven) issuing
-                  PT-step requests of 512-4096 points, admitted against a
-                  device-memory budget, continuously batched; p50/p99 request
-                  latency and aggregate logL evaluations per second.
+  serving(...)    Poisson arrivals of analyses (sizes as in uneven), each
+                  admitted against a device-memory budget with a resident PT
+                  ensemble of 512-4096 points and issuing one-step requests,
+                  continuously batched; p50/p99 step latency and aggregate
+                  logL evaluations per second.
 """
 import threading
 import time
@@ -149,9 +150,10 @@ def precision(n_draws=100000, log=print):
 
 
 def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precision=Q.MIXED, seed=7, log=print):
-    """Poisson arrivals of analyses; each issues lifetime_steps PT-step
-    requests of 512-4096 points and leaves.  Returns latency percentiles and
-    the aggregate evaluation rate."""
+    """Poisson arrivals of analyses; each is admitted with a resident PT
+    ensemble of 512-4096 points (8 temperatures), issues lifetime_steps
+    one-step requests against it and leaves.  Returns step-latency
+    percentiles and the aggregate evaluation rate."""
     specs = uneven_specs(seed=seed, n=16)
     pool = []
     for i, (an, inj, _) in enumerate(specs):
@@ -161,32 +163,23 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
     srv = Q.Server(int(budget_gb * 1e9), precision)
     rng = np.random.default_rng(seed)
     lat, lock = [], threading.Lock()
-    counters = {"admitted": 0, "refused": 0, "requests": 0, "points": 0, "errors": 0}
+    counters = {"admitted": 0, "refused": 0, "requests": 0, "points": 0, "bin_points": 0, "errors": 0}
     stop = threading.Event()
     threads = []
 
-    def client(aid, an, inj, cseed):
-        r = np.random.default_rng(cseed)
-        B = int(r.choice([512, 1024, 2048, 4096]))
-        x = W.ensemble_around(inj, B, seed=cseed)
+    def client(aid, B, nbd):
         for _ in range(lifetime_steps):
             if stop.is_set():
                 break
-            y = x + r.standard_normal(x.shape) * SIGMA * 0.3
-            y[:, 2:4] = np.clip(y[:, 2:4], -0.95, 0.95)
-            y[:, 8] = np.clip(y[:, 8], -0.999, 0.999)
-            y[:, 10] = np.clip(y[:, 10], -0.999, 0.999)
-            y[:, 0] = np.minimum(y[:, 0], y[:, 1] + np.log(0.25 ** 0.6) - 1e-9)
-            t = srv.submit(aid, y)
-            code, L, R, S, ms = srv.wait(t)
+            code, L, R, S, ms = srv.wait(srv.submit_steps(aid, 1))
             with lock:
                 if code == 0:
                     lat.append(ms)
                     counters["requests"] += 1
                     counters["points"] += B
+                    counters["bin_points"] += B * nbd
                 else:
                     counters["errors"] += 1
-            x = y
         srv.evict(aid)
 
     t_start = time.perf_counter()
@@ -199,12 +192,14 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
             continue
         next_arrival += rng.exponential(1.0 / rate)
         an, inj, D, SN = pool[k % len(pool)]
-        code, aid = srv.admit(an, D, SN)
+        B = int(rng.choice([512, 1024, 2048, 4096]))
+        init = W.ensemble_around(inj, B, seed=1000 + k)
+        code, aid = srv.admit_pt(an, D, SN, 8, B // 8, 1.2, init, SIGMA * 0.3, 1000 + k)
         if code != 0:
             counters["refused"] += 1
         else:
             counters["admitted"] += 1
-            th = threading.Thread(target=client, args=(aid, an, inj, 1000 + k), daemon=True)
+            th = threading.Thread(target=client, args=(aid, B, (an.imax - an.imin) * an.nifo), daemon=True)
             th.start()
             threads.append(th)
         k += 1
@@ -218,9 +213,12 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
     lat = np.asarray(lat)
     return {"seconds": t_end - t_start, "arrival_rate_per_s": rate, "budget_gb": budget_gb,
             "precision": "mixed" if precision == Q.MIXED else "fp64",
-            "latency_ms_p50": float(np.percentile(lat, 50)) if lat.size else None,
-            "latency_ms_p99": float(np.percentile(lat, 99)) if lat.size else None,
-            "logl_per_s": n_done / (t_end - t_start), "batches": st["batches"], **counters}
+            "form": "resident PT ensembles (8 temperatures, 512-4096 points), one-step requests",
+            "step_latency_ms_p50": float(np.percentile(lat, 50)) if lat.size else None,
+            "step_latency_ms_p99": float(np.percentile(lat, 99)) if lat.size else None,
+            "logl_per_s": n_done / (t_end - t_start),
+            "bin_detector_points_per_s": counters["bin_points"] / (t_end - t_start), "batches": st["batches"],
+            **counters}
 
 
 # FP64 operations of one (point, bin) in the fused FP64 kernel, counted from
