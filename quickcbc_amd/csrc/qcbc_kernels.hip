@@ -361,6 +361,28 @@ struct SConst {
 // constants of the point and the bin's f, f^(1/3), f^(-1/6), log f.  The
 // inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
 // they need no other power of Mf.
+// The amplitude constants of region AR in FP32, for the mixed kernel: the
+// inspiral Horner coefficients, the intermediate Newton coefficients, or the
+// merger-ringdown Lorentzian's numerator, exponent rate (in base 2, for
+// v_exp_f32) and width.
+template <int AR>
+struct AmpF {
+  float c[9];
+  template <class K, class V>
+  __device__ __forceinline__ AmpF(const K& s, const V& k) {
+    if (AR == 0) {
+      c[0] = (float)s.ia0; c[1] = (float)k.a1; c[2] = (float)k.a0; c[3] = (float)s.ia1; c[4] = (float)s.ia43;
+      c[5] = (float)s.ia53; c[6] = (float)s.ia2; c[7] = (float)s.ia73; c[8] = (float)s.ia3;
+    } else if (AR == 1) {
+      c[0] = (float)s.ib0; c[1] = (float)s.ib1; c[2] = (float)s.ib2; c[3] = (float)k.a0; c[4] = (float)s.ib4;
+    } else {
+      c[0] = (float)s.mnum;
+      c[1] = (float)(s.mrd_rate * 1.4426950408889634);  // rate / ln 2
+      c[2] = (float)s.mg2;
+    }
+  }
+};
+
 template <int AR, int PR, class K>
 __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double v3f,
                                           double is6f, double logf, double* a, double* r) {
@@ -398,6 +420,75 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     A = (c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2));
   }
   if (AR > 0) *a = (w3 * i_s6) * A;  // Mf^(-7/6)
+  // the projected argument: f tshift_0, exact product with one rounding, plus
+  // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
+  double g;
+  if (PR == 0) {
+    // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
+    // + (qL + q13L v) log f
+    double np = fma(v, -c.ins_q2, -k.p0);
+    np = hstep(np, v, c.ins_q43);
+    np = np * v;  // no v^3 term: q1 Mf is added below
+    np = hstep(np, v, c.ins_q23);
+    np = hstep(np, v, c.ins_q13);  // -(q13 + q23 v + q43 v^3 + q53 v^4 + q2 v^5)
+    double nn = -(w * c.ins_qm53);
+    nn = hstep(nn, w, c.ins_qm1);
+    nn = hstep(nn, w, c.ins_qm23);
+    nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
+    const double lg = fma(v, c.ins_q13L, k.pL) * logf;
+    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, fma(c.ins_q1, c.Msec * f, lg));
+  } else if (PR == 1) {
+    const double w9 = w3 * w3 * w3;  // 1 / Mf^3
+    g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
+  } else {
+    const double sq = Mf * (w * i_s6);  // Mf^(1/2)
+    const double f34 = sqrt(Mf * sq);
+    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
+        c.mrd_q0;
+  }
+  *r = fma(f, c.tshift[0], g);
+}
+
+// amp_turns with the amplitude in FP32 (the phase as there, in FP64): the
+// powers of Mf that the phase needs anyway, rounded to FP32, drive the same
+// Horner and Newton forms in FP32; the merger-ringdown exponential and
+// reciprocal are the hardware's v_exp_f32 and v_rcp_f32.
+template <int AR, int PR, class K>
+__device__ __forceinline__ void amp32_turns(const K& c, const VConst<AR, PR>& k, const AmpF<AR>& cf, double f,
+                                            double v3f, double is6f, double logf, float* a, double* r) {
+  const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
+  const double w = i_s6 * i_s6;            // Mf^(-1/3)
+  const double w2 = w * w;
+  const double w3 = w2 * w;                // 1 / Mf (unused by the inspiral-inspiral pair)
+  double Mf = 0.0, v = 0.0;
+  if (AR > 0 || PR > 0) {
+    Mf = c.Msec * f;
+    vreg(Mf);  // Mf - node stays one subtraction with the node in a scalar register
+  }
+  if (AR == 0 || PR == 0) v = c.v_scale * v3f;  // Mf^(1/3)
+  if (AR == 0) {
+    const float vf = (float)v, wf = (float)w;
+    float np = fmaf(vf, -cf.c[8], -cf.c[2]);
+    np = fmaf(np, vf, -cf.c[7]);
+    np = fmaf(np, vf, -cf.c[6]);
+    np = fmaf(np, vf, -cf.c[5]);
+    np = fmaf(np, vf, -cf.c[4]);
+    np = fmaf(np, vf, -cf.c[3]);
+    *a = (float)i_s6 * fmaf(wf, fmaf((float)w2, cf.c[0], cf.c[1]), -np);
+  } else {
+    float A;
+    if (AR == 1) {
+      const float x1 = (float)(Mf - c.col_f1);
+      float np = fmaf((float)(Mf - c.col_f3), -cf.c[4], -cf.c[3]);
+      np = fmaf(np, (float)(Mf - c.col_f2), -cf.c[2]);
+      np = fmaf(np, x1, -cf.c[1]);
+      A = -fmaf(np, x1, -cf.c[0]);
+    } else {
+      const float df = (float)(Mf - c.fRD);
+      A = (cf.c[0] * __builtin_amdgcn_exp2f(-df * cf.c[1])) * __builtin_amdgcn_rcpf(fmaf(df, df, cf.c[2]));
+    }
+    *a = (float)(w3 * i_s6) * A;  // Mf^(-7/6)
+  }
   // the projected argument: f tshift_0, exact product with one rounding, plus
   // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
   double g;
@@ -625,13 +716,14 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   }
 }
 
-// ---- mixed precision.  FP64: powers, amplitude and the projected argument
-// f (ts - td) + (lambda - Psi)/2 pi in turns (gfx950 issues FP64 FMAs at the
-// FP32 rate, and the phase needs FP64), reduced to a fraction of a turn
-// (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to ~1e-11.
-// FP32: sin/cos of the reduced turn (hardware v_sin/v_cos take turns), the
-// products with the scaled data, and per-lane Kahan-compensated sums; the
-// wave and split combine is FP64.
+// ---- mixed precision.  FP64: the powers of Mf and the projected argument
+// f (ts - td) + (lambda - Psi)/2 pi in turns, reduced to a fraction of a
+// turn (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to
+// ~1e-11.  FP32: the amplitude (amp32_turns: the Horner and Newton forms on
+// the FP64 powers rounded to FP32, v_exp_f32 and v_rcp_f32 in
+// merger-ringdown), sin/cos of the reduced turn (hardware v_sin/v_cos take
+// turns), the products with the scaled data, and per-lane Kahan-compensated
+// sums; the wave and split combine is FP64.
 template <int ND, int TILE>
 struct MixState {
   const Point* c;
@@ -650,15 +742,16 @@ struct MixSeg {
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
       c.template load<ND, false>(*st.c);
+      const AmpF<AR> cf(c, k);
       const int nit = (end - pos + 63) >> 6;
       for (int n = 0; n < nit; n++) {
         if (st.j < end) {
           const double* L = st.lds + (st.j - st.t0);
           const float* M = st.lds32 + (st.j - st.t0);
           const double f = L[0];
-          double ad, r;
-          amp_turns<AR, PR>(c, k, f, L[TILE], L[2 * TILE], L[3 * TILE], &ad, &r);
-          const float a = (float)ad;
+          float a;
+          double r;
+          amp32_turns<AR, PR>(c, k, cf, f, L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
           const float a2 = a * a;
 #pragma unroll
           for (int d = 0; d < ND; d++) {
