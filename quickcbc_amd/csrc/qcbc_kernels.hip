@@ -121,6 +121,42 @@ __device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
   sincos_turns(r, s, c, SinCosK{-0x1.e30071afc3e59p-19, -0x1.a6c9c1be9eb49p-16});
 }
 
+// Twice sin and cos of 2 pi r: sincos_turns' polynomials with every
+// coefficient doubled (exact), which makes each double-angle step one
+// product and one FMA, (2 sin 2x) = (2 sin x)(2 cos x) and (2 cos 2x) =
+// (2 cos x)^2 - 2, in place of two products, an FMA and an add.  The sine
+// chain is bit for bit twice sincos_turns'; the cosine differs from it by an
+// ulp or two.  k2: sincos_consts2().
+__device__ __forceinline__ void sincos2_turns(double r, double* s2, double* c2, const SinCosK& k2) {
+  const double t = r - rint(r);
+  const double z = t * t;
+  double p = fma(z, sreg(0x1.e3f38399551bfp-24), k2.p1);
+  p = fma(p, z, -sreg(-0x1.50782fda12d96p-12));
+  p = fma(p, z, -sreg(0x1.32d2cce2e5b19p-7));
+  p = fma(p, z, -sreg(-0x1.466bc677587f8p-3));
+  p = fma(p, z, -sreg(0x1.4abbce625be41p+0));
+  p = fma(p, z, -sreg(-0x1.921fb54442d18p+1));
+  double q = fma(z, sreg(0x1.f3dbcea61b1a4p-21), k2.q1);
+  q = fma(q, z, -sreg(-0x1.e1f4fb60281f6p-10));
+  q = fma(q, z, -sreg(0x1.55d3c7dbfd139p-5));
+  q = fma(q, z, -sreg(-0x1.03c1f081b0780p-1));
+  q = fma(q, z, -sreg(0x1.3bd3cc9be458bp+1));
+  q = fma(q, z, 2.0);
+  double sn = t * p, cs = q;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const double sx = sn * cs;
+    cs = fma(cs, cs, -2.0);
+    sn = sx;
+  }
+  *s2 = sn;
+  *c2 = cs;
+}
+
+__device__ __forceinline__ SinCosK sincos_consts2() {
+  return SinCosK{vreg_v(-0x1.e30071afc3e59p-18), vreg_v(-0x1.a6c9c1be9eb49p-15)};
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
   float y = x - comp;
   float t = s + y;
@@ -315,12 +351,14 @@ struct SConst {
   double int_q0, int_q1, int_qL, int_qm3;
   double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
   double wc[3], ws[3], dt[3], dl[3];
+  double tsq;  // tshift_0 plus the phase's Mf-linear coefficient times Msec: the argument's f-linear term
   // ND detectors; the Horner rotation (wc, ws) for the FP64 kernel, the
   // detector phase offsets (dt, dl) for the mixed one
   template <int ND, bool ROT = true, class PT>
   __device__ __forceinline__ void load(const PT& c) {
     Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
+    tsq = spin(fma(PR == 0 ? c.ins_q1 : PR == 1 ? c.int_q1 : c.mrd_q1, c.Msec, c.tshift[0]));
     if (AR == 0 || PR == 0) v_scale = spin(c.v_scale);
     is6_scale = spin(c.is6_scale);
     if (AR == 0) {
@@ -420,8 +458,9 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     A = (c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2));
   }
   if (AR > 0) *a = (w3 * i_s6) * A;  // Mf^(-7/6)
-  // the projected argument: f tshift_0, exact product with one rounding, plus
-  // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
+  // the projected argument: f tsq (tshift_0 plus the region's Mf-linear
+  // coefficient times Msec, per point) in the last FMA, plus the rest of the
+  // region's function of Mf
   double g;
   if (PR == 0) {
     // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
@@ -436,17 +475,16 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     nn = hstep(nn, w, c.ins_qm23);
     nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
     const double lg = fma(v, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, fma(c.ins_q1, c.Msec * f, lg));
+    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, lg);  // q1 Mf is in tsq
   } else if (PR == 1) {
     const double w9 = w3 * w3 * w3;  // 1 / Mf^3
-    g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
+    g = fma(c.int_qm3, w9, c.int_qL * logf) + c.int_q0;
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
-    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
-        c.mrd_q0;
+    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, c.mrd_qm1 * w3)) + c.mrd_q0;
   }
-  *r = fma(f, c.tshift[0], g);
+  *r = fma(f, c.tsq, g);
 }
 
 // amp_turns with the amplitude in FP32 (the phase as there, in FP64): the
@@ -489,8 +527,9 @@ __device__ __forceinline__ void amp32_turns(const K& c, const VConst<AR, PR>& k,
     }
     *a = (float)(w3 * i_s6) * A;  // Mf^(-7/6)
   }
-  // the projected argument: f tshift_0, exact product with one rounding, plus
-  // the region's function of Mf (its Mf-linear term from Mf = Msec f too)
+  // the projected argument: f tsq (tshift_0 plus the region's Mf-linear
+  // coefficient times Msec, per point) in the last FMA, plus the rest of the
+  // region's function of Mf
   double g;
   if (PR == 0) {
     // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
@@ -505,17 +544,16 @@ __device__ __forceinline__ void amp32_turns(const K& c, const VConst<AR, PR>& k,
     nn = hstep(nn, w, c.ins_qm23);
     nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
     const double lg = fma(v, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, fma(c.ins_q1, c.Msec * f, lg));
+    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, lg);  // q1 Mf is in tsq
   } else if (PR == 1) {
     const double w9 = w3 * w3 * w3;  // 1 / Mf^3
-    g = fma(c.int_qm3, w9, fma(c.int_qL, logf, c.int_q1 * Mf)) + c.int_q0;
+    g = fma(c.int_qm3, w9, c.int_qL * logf) + c.int_q0;
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
-    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, fma(c.mrd_qm1, w3, c.mrd_q1 * Mf))) +
-        c.mrd_q0;
+    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, c.mrd_qm1 * w3)) + c.mrd_q0;
   }
-  *r = fma(f, c.tshift[0], g);
+  *r = fma(f, c.tsq, g);
 }
 
 // Walks the segments of [pos, hi): SEG<AR, PR>::run(st, pos, end) for each
@@ -577,10 +615,10 @@ struct F64Seg {
       double a, r;
       amp_turns<AR, PR>(c, k, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
       double s0, c0;
-      sincos_turns(r, &s0, &c0, st.sk);
+      sincos2_turns(r, &s0, &c0, st.sk);  // twice sin and cos
       // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the sums carry
-      // a^2 / S and a D e^{-i x_d} / S, and k_finalize applies the per-point
-      // detector scale once.
+      // a^2 / S and (twice) a D e^{-i x_d} / S, and k_finalize applies the
+      // per-point detector scale once.
       const double a2 = a * a, uc = a * c0, us = a * s0;
 #pragma unroll
       for (int d = 0; d < ND; d++) {
@@ -632,7 +670,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   F64State<ND, TILE> st;
   st.c = (CPoint*)(in.pts + uniform(active ? b : 0));
   st.j = ks + lane;
-  st.sk = sincos_consts();
+  st.sk = sincos_consts2();
   st.dh0 = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
@@ -708,7 +746,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
       }
     }
     acc[2 * d] = wave_sum(st.hh[d]);
-    acc[2 * d + 1] = wave_sum(dh);
+    acc[2 * d + 1] = wave_sum(0.5 * dh);  // the bins' sums carry twice the products (exact)
   }
   if (lane == SUM_LANE && b < in.B) {
 #pragma unroll
