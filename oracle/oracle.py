@@ -186,6 +186,19 @@ def waveform_params(params, N, Tobs, fref=100.0):
     return st, amp, ph
 
 
+def collocation(m1, m2, chi1, chi2):
+    """The intermediate amplitude's collocation (IMRPhenomD_internals.c:848-1063) of one system: the
+    closed-form deltas, the elimination's deltas, (f1, f2, f3, v1, v2, v3, d1, d2) and the term-wise
+    derivative of the inspiral ansatz at f1."""
+    L = lib()
+    L.qo_collocation_probe.restype = ctypes.c_int
+    L.qo_collocation_probe.argtypes = [ctypes.c_double] * 4 + [_dp]
+    out = np.zeros(19)
+    st = L.qo_collocation_probe(m1, m2, chi1, chi2, _d(out))
+    return st, {"closed": out[0:5].copy(), "elim": out[5:10].copy(), "f": out[10:13].copy(),
+                "v": out[13:16].copy(), "d": out[16:18].copy(), "d1_termwise": float(out[18])}
+
+
 def probe(which, m1, m2, chi1, chi2, Mf):
     L = lib()
     L.qo_probe.restype = ctypes.c_double

@@ -413,11 +413,12 @@ double qo_amp_int_ansatz(const qo_phenomd *p, double Mf) {
   return p->delta[0] + p->delta[1] * Mf + p->delta[2] * Mf2 + p->delta[3] * Mf3 + p->delta[4] * Mf4;
 }
 
-/* ComputeDeltasFromCollocation, :1005-1063: the quartic through (f1,v1),
- * (f2,v2), (f3,v3) with slopes d1 at f1 and d2 at f3.  Solved in long double
+/* The quartic of ComputeDeltasFromCollocation (:1005-1063) through (f1,v1),
+ * (f2,v2), (f3,v3) with slopes d1 at f1 and d2 at f3, solved in long double
  * by Gaussian elimination with partial pivoting on the variable t = Mf/f3,
- * then rescaled. */
-static void qo_collocation(qo_phenomd *p) {
+ * then rescaled: the pin of the closed forms below (and the oracle's former
+ * route to the deltas). */
+static void qo_collocation_elim(qo_phenomd *p) {
   long double A[5][6];
   long double s = p->f3;
   long double t[3] = {p->f1 / s, p->f2 / s, 1.0L};
@@ -469,6 +470,117 @@ static void qo_collocation(qo_phenomd *p) {
       sp *= s;
     }
   }
+}
+
+/* DAmpInsAnsatz, IMRPhenomD_internals.c:671-702: d/dMf of the inspiral
+ * amplitude ansatz as the reference writes it, from eta, chi1, chi2 and
+ * rho1..3 (the Mathematica form, pow as there); qo_damp_ins_ansatz's
+ * term-wise form pins it. */
+double qo_damp_ins_ansatz_ref(const qo_phenomd *p, double Mf) {
+  const double eta = p->eta, chi1 = p->chi1, chi2 = p->chi2;
+  const double rho1 = p->rho1, rho2 = p->rho2, rho3 = p->rho3;
+  const double chi12 = chi1 * chi1, chi22 = chi2 * chi2;
+  const double eta2 = eta * eta, eta3 = eta * eta2;
+  const double Mf2 = Mf * Mf;
+  const double Pi = QO_PI;
+  const double Pi2 = qo_pi_powers().two;
+  const double Seta = sqrt(1.0 - 4.0 * eta);
+  return ((-969 + 1804 * eta) * pow(Pi, 2.0 / 3.0)) / (1008. * pow(Mf, 1.0 / 3.0)) +
+         ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48. +
+         ((-27312085 - 10287648 * chi22 - 10287648 * chi12 * (1 + Seta) + 10287648 * chi22 * Seta +
+           24 * (-1975055 + 857304 * chi12 - 994896 * chi1 * chi2 + 857304 * chi22) * eta + 35371056 * eta2) *
+          pow(Mf, 1.0 / 3.0) * pow(Pi, 4.0 / 3.0)) /
+             6.096384e6 +
+         (5 * pow(Mf, 2.0 / 3.0) * pow(Pi, 5.0 / 3.0) *
+          (chi2 * (-285197 * (-1 + Seta) + 4 * (-91902 + 1579 * Seta) * eta - 35632 * eta2) +
+           chi1 * (285197 * (1 + Seta) - 4 * (91902 + 1579 * Seta) * eta - 35632 * eta2) +
+           42840 * (-1 + 4 * eta) * Pi)) /
+             96768. -
+         (Mf * Pi2 *
+          (-336 * (-3248849057.0 + 2943675504 * chi12 - 3339284256 * chi1 * chi2 + 2943675504 * chi22) * eta2 -
+           324322727232 * eta3 -
+           7 * (-177520268561 + 107414046432 * chi22 + 107414046432 * chi12 * (1 + Seta) -
+                107414046432 * chi22 * Seta + 11087290368 * (chi1 + chi2 + chi1 * Seta - chi2 * Seta) * Pi) +
+           12 * eta *
+               (-545384828789.0 - 176491177632 * chi1 * chi2 + 202603761360 * chi22 +
+                77616 * chi12 * (2610335 + 995766 * Seta) - 77287373856 * chi22 * Seta +
+                5841690624 * (chi1 + chi2) * Pi + 21384760320 * Pi2))) /
+             3.0042980352e10 +
+         (7.0 / 3.0) * pow(Mf, 4.0 / 3.0) * rho1 + (8.0 / 3.0) * pow(Mf, 5.0 / 3.0) * rho2 + 3 * Mf2 * rho3;
+}
+
+/* ComputeDeltasFromCollocation's closed forms delta0_fun .. delta4_fun
+ * (IMRPhenomD_internals.c:848-999, the solution of arXiv:1508.07253 Eq. 21
+ * under its constraints 22-26, as the reference writes them) from the
+ * collocation values in p and the powers f1^2..f1^5, f2^2..f2^4,
+ * f3^2..f3^5 formed by successive products. */
+static void qo_collocation(qo_phenomd *p) {
+  const double f1 = p->f1, f2 = p->f2, f3 = p->f3;
+  const double v1 = p->v1, v2 = p->v2, v3 = p->v3, d1 = p->d1, d2 = p->d2;
+  const double f12 = f1 * f1, f13 = f1 * f12, f14 = f1 * f13, f15 = f1 * f14;
+  const double f22 = f2 * f2, f23 = f2 * f22, f24 = f2 * f23;
+  const double f32 = f3 * f3, f33 = f3 * f32, f34 = f3 * f33, f35 = f3 * f34;
+  const double a = f1 - f2, b = f1 - f3, c = f3 - f2, cm = -f2 + f3;
+  p->delta[0] =
+      -((d2 * f15 * f22 * f3 - 2 * d2 * f14 * f23 * f3 + d2 * f13 * f24 * f3 - d2 * f15 * f2 * f32 +
+         d2 * f14 * f22 * f32 - d1 * f13 * f23 * f32 + d2 * f13 * f23 * f32 + d1 * f12 * f24 * f32 -
+         d2 * f12 * f24 * f32 + d2 * f14 * f2 * f33 + 2 * d1 * f13 * f22 * f33 - 2 * d2 * f13 * f22 * f33 -
+         d1 * f12 * f23 * f33 + d2 * f12 * f23 * f33 - d1 * f1 * f24 * f33 - d1 * f13 * f2 * f34 -
+         d1 * f12 * f22 * f34 + 2 * d1 * f1 * f23 * f34 + d1 * f12 * f2 * f35 - d1 * f1 * f22 * f35 +
+         4 * f12 * f23 * f32 * v1 - 3 * f1 * f24 * f32 * v1 - 8 * f12 * f22 * f33 * v1 + 4 * f1 * f23 * f33 * v1 +
+         f24 * f33 * v1 + 4 * f12 * f2 * f34 * v1 + f1 * f22 * f34 * v1 - 2 * f23 * f34 * v1 -
+         2 * f1 * f2 * f35 * v1 + f22 * f35 * v1 - f15 * f32 * v2 + 3 * f14 * f33 * v2 - 3 * f13 * f34 * v2 +
+         f12 * f35 * v2 - f15 * f22 * v3 + 2 * f14 * f23 * v3 - f13 * f24 * v3 + 2 * f15 * f2 * f3 * v3 -
+         f14 * f22 * f3 * v3 - 4 * f13 * f23 * f3 * v3 + 3 * f12 * f24 * f3 * v3 - 4 * f14 * f2 * f32 * v3 +
+         8 * f13 * f22 * f32 * v3 - 4 * f12 * f23 * f32 * v3) /
+        ((a * a) * (b * b * b) * (c * c)));
+  p->delta[1] =
+      -((-(d2 * f15 * f22) + 2 * d2 * f14 * f23 - d2 * f13 * f24 - d2 * f14 * f22 * f3 + 2 * d1 * f13 * f23 * f3 +
+         2 * d2 * f13 * f23 * f3 - 2 * d1 * f12 * f24 * f3 - d2 * f12 * f24 * f3 + d2 * f15 * f32 -
+         3 * d1 * f13 * f22 * f32 - d2 * f13 * f22 * f32 + 2 * d1 * f12 * f23 * f32 - 2 * d2 * f12 * f23 * f32 +
+         d1 * f1 * f24 * f32 + 2 * d2 * f1 * f24 * f32 - d2 * f14 * f33 + d1 * f12 * f22 * f33 +
+         3 * d2 * f12 * f22 * f33 - 2 * d1 * f1 * f23 * f33 - 2 * d2 * f1 * f23 * f33 + d1 * f24 * f33 +
+         d1 * f13 * f34 + d1 * f1 * f22 * f34 - 2 * d1 * f23 * f34 - d1 * f12 * f35 + d1 * f22 * f35 -
+         8 * f12 * f23 * f3 * v1 + 6 * f1 * f24 * f3 * v1 + 12 * f12 * f22 * f32 * v1 - 8 * f1 * f23 * f32 * v1 -
+         4 * f12 * f34 * v1 + 2 * f1 * f35 * v1 + 2 * f15 * f3 * v2 - 4 * f14 * f32 * v2 + 4 * f12 * f34 * v2 -
+         2 * f1 * f35 * v2 - 2 * f15 * f3 * v3 + 8 * f12 * f23 * f3 * v3 - 6 * f1 * f24 * f3 * v3 +
+         4 * f14 * f32 * v3 - 12 * f12 * f22 * f32 * v3 + 8 * f1 * f23 * f32 * v3) /
+        ((a * a) * (b * b * b) * (cm * cm)));
+  p->delta[2] =
+      -((d2 * f15 * f2 - d1 * f13 * f23 - 3 * d2 * f13 * f23 + d1 * f12 * f24 + 2 * d2 * f12 * f24 - d2 * f15 * f3 +
+         d2 * f14 * f2 * f3 - d1 * f12 * f23 * f3 + d2 * f12 * f23 * f3 + d1 * f1 * f24 * f3 - d2 * f1 * f24 * f3 -
+         d2 * f14 * f32 + 3 * d1 * f13 * f2 * f32 + d2 * f13 * f2 * f32 - d1 * f1 * f23 * f32 +
+         d2 * f1 * f23 * f32 - 2 * d1 * f24 * f32 - d2 * f24 * f32 - 2 * d1 * f13 * f33 + 2 * d2 * f13 * f33 -
+         d1 * f12 * f2 * f33 - 3 * d2 * f12 * f2 * f33 + 3 * d1 * f23 * f33 + d2 * f23 * f33 + d1 * f12 * f34 -
+         d1 * f1 * f2 * f34 + d1 * f1 * f35 - d1 * f2 * f35 + 4 * f12 * f23 * v1 - 3 * f1 * f24 * v1 +
+         4 * f1 * f23 * f3 * v1 - 3 * f24 * f3 * v1 - 12 * f12 * f2 * f32 * v1 + 4 * f23 * f32 * v1 +
+         8 * f12 * f33 * v1 - f1 * f34 * v1 - f35 * v1 - f15 * v2 - f14 * f3 * v2 + 8 * f13 * f32 * v2 -
+         8 * f12 * f33 * v2 + f1 * f34 * v2 + f35 * v2 + f15 * v3 - 4 * f12 * f23 * v3 + 3 * f1 * f24 * v3 +
+         f14 * f3 * v3 - 4 * f1 * f23 * f3 * v3 + 3 * f24 * f3 * v3 - 8 * f13 * f32 * v3 +
+         12 * f12 * f2 * f32 * v3 - 4 * f23 * f32 * v3) /
+        ((a * a) * (b * b * b) * (cm * cm)));
+  p->delta[3] =
+      -((-2 * d2 * f14 * f2 + d1 * f13 * f22 + 3 * d2 * f13 * f22 - d1 * f1 * f24 - d2 * f1 * f24 +
+         2 * d2 * f14 * f3 - 2 * d1 * f13 * f2 * f3 - 2 * d2 * f13 * f2 * f3 + d1 * f12 * f22 * f3 -
+         d2 * f12 * f22 * f3 + d1 * f24 * f3 + d2 * f24 * f3 + d1 * f13 * f32 - d2 * f13 * f32 -
+         2 * d1 * f12 * f2 * f32 + 2 * d2 * f12 * f2 * f32 + d1 * f1 * f22 * f32 - d2 * f1 * f22 * f32 +
+         d1 * f12 * f33 - d2 * f12 * f33 + 2 * d1 * f1 * f2 * f33 + 2 * d2 * f1 * f2 * f33 - 3 * d1 * f22 * f33 -
+         d2 * f22 * f33 - 2 * d1 * f1 * f34 + 2 * d1 * f2 * f34 - 4 * f12 * f22 * v1 + 2 * f24 * v1 +
+         8 * f12 * f2 * f3 * v1 - 4 * f1 * f22 * f3 * v1 - 4 * f12 * f32 * v1 + 8 * f1 * f2 * f32 * v1 -
+         4 * f22 * f32 * v1 - 4 * f1 * f33 * v1 + 2 * f34 * v1 + 2 * f14 * v2 - 4 * f13 * f3 * v2 +
+         4 * f1 * f33 * v2 - 2 * f34 * v2 - 2 * f14 * v3 + 4 * f12 * f22 * v3 - 2 * f24 * v3 + 4 * f13 * f3 * v3 -
+         8 * f12 * f2 * f3 * v3 + 4 * f1 * f22 * f3 * v3 + 4 * f12 * f32 * v3 - 8 * f1 * f2 * f32 * v3 +
+         4 * f22 * f32 * v3) /
+        ((a * a) * (b * b * b) * (cm * cm)));
+  p->delta[4] =
+      -((d2 * f13 * f2 - d1 * f12 * f22 - 2 * d2 * f12 * f22 + d1 * f1 * f23 + d2 * f1 * f23 - d2 * f13 * f3 +
+         2 * d1 * f12 * f2 * f3 + d2 * f12 * f2 * f3 - d1 * f1 * f22 * f3 + d2 * f1 * f22 * f3 - d1 * f23 * f3 -
+         d2 * f23 * f3 - d1 * f12 * f32 + d2 * f12 * f32 - d1 * f1 * f2 * f32 - 2 * d2 * f1 * f2 * f32 +
+         2 * d1 * f22 * f32 + d2 * f22 * f32 + d1 * f1 * f33 - d1 * f2 * f33 + 3 * f1 * f22 * v1 - 2 * f23 * v1 -
+         6 * f1 * f2 * f3 * v1 + 3 * f22 * f3 * v1 + 3 * f1 * f32 * v1 - f33 * v1 - f13 * v2 + 3 * f12 * f3 * v2 -
+         3 * f1 * f32 * v2 + f33 * v2 + f13 * v3 - 3 * f1 * f22 * v3 + 2 * f23 * v3 - 3 * f12 * f3 * v3 +
+         6 * f1 * f2 * f3 * v3 - 3 * f22 * f3 * v3) /
+        ((a * a) * (b * b * b) * (cm * cm)));
 }
 
 /* ---------------------------------------------------------------- phase */
@@ -608,7 +720,7 @@ int qo_phenomd_setup(qo_phenomd *p, double m1_in, double m2_in, double chi1_in, 
   p->f2 = p->f1 + (p->f3 - p->f1) / 2.0;
   qo_powers(&w, p->f1);
   p->v1 = qo_amp_ins_pw(p, p->f1, &w);
-  p->d1 = qo_damp_ins_ansatz(p, p->f1);
+  p->d1 = qo_damp_ins_ansatz_ref(p, p->f1);
   p->v3 = qo_amp_mrd_ansatz(p, p->f3);
   p->d2 = qo_damp_mrd_ansatz(p, p->f3);
   p->v2 = qo_fit(QCBC_FIT_V2, p->eta, p->chi);
@@ -3575,6 +3687,30 @@ int qo_pdwave_flat(double *wavef, int N, double Tobs, double tcmin, const double
   st = qo_pdwave(wavef, freq, tcmin, params, fref, N);
   free(freq);
   return st;
+}
+
+/* The intermediate amplitude's collocation of one system (masses in solar
+ * masses): out[0..4] the closed-form deltas (the oracle's), out[5..9] the
+ * elimination's, out[10..17] f1 f2 f3 v1 v2 v3 d1 d2, out[18] the
+ * term-wise derivative of the inspiral ansatz at f1.  Returns the set-up
+ * status. */
+int qo_collocation_probe(double m1, double m2, double chi1, double chi2, double *out) {
+  qo_phenomd p, q;
+  int st, k;
+  qo_spline_init();
+  st = qo_phenomd_setup(&p, m1, m2, chi1, chi2, 1.0, 0.0, 100.0);
+  if (st) return st;
+  q = p;
+  qo_collocation_elim(&q);
+  for (k = 0; k < 5; k++) {
+    out[k] = p.delta[k];
+    out[5 + k] = q.delta[k];
+  }
+  out[10] = p.f1; out[11] = p.f2; out[12] = p.f3;
+  out[13] = p.v1; out[14] = p.v2; out[15] = p.v3;
+  out[16] = p.d1; out[17] = p.d2;
+  out[18] = qo_damp_ins_ansatz(&p, p.f1);
+  return 0;
 }
 
 /* ------------------------------------------------ text formats (f4) */

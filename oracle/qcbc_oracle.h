@@ -173,6 +173,7 @@ int qo_makespec(int Nspline, int Nlines, const double *ffit, const double *Sspli
 int qo_amp_phase(double *Af, double *Pf, const double *freq, const double *params, double fref, int N);
 int qo_fisher_full_flat(int nifo, double fref, double fmin, double fmax, const double *SN, const double *params, int d,
                         int N, double Tobs, double *fish);
+int qo_collocation_probe(double m1, double m2, double chi1, double chi2, double *out);
 int qo_read_framed(const char *path, int cap, double *time, double *x);
 int qo_read_spec(const char *path, int n2, double *SN, double *SM);
 

@@ -128,6 +128,30 @@ def test_collocation_conditions(oracle, sysp):
     assert oracle.probe(2, *sysp, f3) == pytest.approx(s["v3"], rel=1e-15)
 
 
+def test_collocation_closed_forms_match_the_elimination(oracle):
+    """The oracle's deltas are the reference's closed forms (delta0_fun ..
+    delta4_fun, IMRPhenomD_internals.c:848-999); the quartic they give agrees
+    with the long-double elimination of the same five conditions across the
+    model's domain, and the reference's written-out DAmpInsAnsatz (:671-702)
+    with the derivative of the ansatz taken term by term."""
+    rng = np.random.default_rng(21)
+    worst_A, worst_d = 0.0, 0.0
+    for _ in range(240):
+        m1 = float(rng.uniform(1.0, 150.0))
+        m2 = float(rng.uniform(1.0, m1))
+        c1, c2 = (float(x) for x in rng.uniform(-0.95, 0.95, 2))
+        st, c = oracle.collocation(m1, m2, c1, c2)
+        assert st == 0
+        f1, f2, f3 = c["f"]
+        Pc, Pe = np.polynomial.Polynomial(c["closed"]), np.polynomial.Polynomial(c["elim"])
+        for f in np.linspace(f1, f3, 9):
+            worst_A = max(worst_A, abs(Pc(f) - Pe(f)) / abs(Pe(f)))
+        worst_d = max(worst_d, abs(c["d"][0] - c["d1_termwise"]) / abs(c["d1_termwise"]))
+    print("closed-form vs eliminated A_int: max %.2e relative; DAmpInsAnsatz: %.2e" % (worst_A, worst_d))
+    assert worst_A <= 1e-11
+    assert worst_d <= 1e-12
+
+
 @pytest.mark.parametrize("sysp", SYSTEMS)
 @pytest.mark.parametrize("Mf", [0.004, 0.011, 0.05, 0.09])
 def test_amplitude_derivatives_by_finite_differences(oracle, sysp, Mf):
@@ -249,6 +273,63 @@ def test_light_travel_delays(oracle):
         assert np.all(np.abs(t) < 6.4e6 / 299792458.0)
         mx = np.maximum(mx, [abs(t[0] - t[1]), abs(t[0] - t[2]), abs(t[1] - t[2])])
     assert mx == pytest.approx([0.0100, 0.0273, 0.0264], rel=0.01)
+
+
+def test_projection_is_the_polarisation_sum(oracle):
+    """fulltemplates' projection (QuickCBC.c:3735-3771) against the sum
+    F+ h+ + Fx hx built here from the bare h22 of the same masses: with
+    x0 = 2 pi f (Tobs - tc - td_d) - Psi and A = h22fac amp / sqrt(Tobs),
+    h+ = (1 + cos^2 iota)/2 A e^{i x0} and hx = -i cos iota A e^{i x0}
+    (circular polarisation), in every detector, above fbegin.  A transposed
+    F+/Fx, a wrong sign of the cross term or of the delay fails it."""
+    an = W.config1()
+    p = W.injection1()
+    st, h = oracle.fulltemplates(an.labels, an.gmst, an.fref, an.N, an.Tobs, p)
+    assert st == 0
+    mc, mt = np.exp(p[0]), np.exp(p[1])
+    eta = (mc / mt) ** (5.0 / 3.0)
+    dm = np.sqrt(1.0 - 4.0 * eta)
+    f = an.freq()
+    st, amp, psi = oracle.h22(f, p[4] / 2.0, an.fref, mt * (1 + dm) / 2, mt * (1 - dm) / 2, p[2], p[3], np.exp(p[6]))
+    assert st == 0
+    td = oracle.times(p[7], p[8], an.gmst)
+    ci = p[10]
+    k = np.arange(1, an.N // 2)
+    k = k[f[k] > oracle.fbegin(p)]
+    A = 0.31539156525252 * amp[k] / np.sqrt(an.Tobs)  # h22fac, ConstCBC.h:12
+    for d, lab in enumerate(an.labels):
+        Fp, Fc = oracle.f_ant(p[9], p[7], p[8], an.gmst, lab)
+        x0 = 2 * np.pi * f[k] * (an.Tobs - p[5] - td[lab]) - psi[k]
+        hp = 0.5 * (1 + ci * ci) * A * np.exp(1j * x0)
+        hx = -1j * ci * A * np.exp(1j * x0)
+        T = h[d, k] + 1j * h[d, an.N - k]
+        assert np.abs(T - (Fp * hp + Fc * hx)).max() <= 1e-9 * np.abs(T).max(), lab
+
+
+def test_templates_peak_at_the_arrival_times(oracle):
+    """In the time domain each detector's template (the inverse transform of
+    its half-complex spectrum) peaks at its arrival time: the peaks of two
+    detectors are td_1 - td_0 apart to a sample, and detector 0's lies within
+    10 ms before tc + td_0 (IMRPhenomD aligns tc with the frequency of peak
+    amplitude, the envelope peaks a few ms earlier).  tc is off the middle of
+    the segment, so a sign error in tc or td moves the peaks."""
+    an = W.config0()
+    p = W.injection0().copy()
+    p[5] = 1.3
+    st, h = oracle.fulltemplates(an.labels, an.gmst, an.fref, an.N, an.Tobs, p)
+    assert st == 0
+    N, dt = an.N, an.Tobs / an.N
+    td = oracle.times(p[7], p[8], an.gmst)
+    peaks = []
+    for d in range(an.nifo):
+        X = np.zeros(N // 2 + 1, complex)
+        k = np.arange(1, N // 2)
+        X[k] = h[d, k] + 1j * h[d, N - k]
+        env = np.hypot(np.fft.irfft(X, N), np.fft.irfft(-1j * X, N))
+        peaks.append(np.argmax(env) * dt)
+    l0, l1 = an.labels[0], an.labels[1]
+    assert abs((peaks[1] - peaks[0]) - (td[l1] - td[l0])) <= 1.5 * dt
+    assert 0.0 <= (p[5] + td[l0]) - peaks[0] <= 0.010, (peaks[0], p[5] + td[l0])
 
 
 def test_fbegin_tracks_newtonian_chirp(oracle):
