@@ -2,8 +2,10 @@
 
 Tolerances (FP64 path): waveform amplitude and phase 1e-12 relative
 (|dA|/A and |dPsi|/max(|Psi|, 2 pi), measured before projection because the
-projected argument reaches ~1e5 rad); logL and rho 1e-8 absolute.  Mixed
-precision: |dlogL| <= 0.01.
+projected argument reaches ~1e5 rad); logL 1e-8 absolute plus 1e-12 of
+|logL| for the fused launch (1e-8 absolute for the plain path), rho 1e-8.
+Config 1 is the PT ensemble's full 2048 points.  Mixed precision:
+|dlogL| <= 0.01.
 """
 import numpy as np
 import pytest
@@ -63,14 +65,18 @@ def test_templates(ctx_cfg, oracle):
 
 def test_loglike_fp64(ctx_cfg):
     which, an, inj, D, SN, c = ctx_cfg
-    n = 1024 if which == 0 else 256
+    n = 1024 if which == 0 else 2048  # config 1: the PT ensemble's full 2048 points
     P = np.vstack([inj, _data.points(which, n - 1, seed=23)])
     L, R, S = c.loglike(P, Q.FP64)
     Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
     assert (S == Sr).all()
     print("fused FP64 |dlogL| max %.3e (|logL| up to %.0f), |drho| max %.3e" %
           (np.abs(L - Lr).max(), np.abs(Lr).max(), np.abs(R - Rr).max()))
-    assert np.abs(L - Lr).max() <= 1e-8, np.abs(L - Lr).max()
+    # 1e-8 absolute, plus 1e-12 of |logL|: two FP64 sums over ~1e5 terms in
+    # different orders differ by ~1e-12 relative, which passes 1e-8 at
+    # |logL| ~ 1e4 (2 of the 2048 config-1 points, at logL = -1.04e4, reach
+    # 1.2e-8 with either fused kernel)
+    assert (np.abs(L - Lr) <= 1e-8 + 1e-12 * np.abs(Lr)).all(), np.abs(L - Lr).max()
     assert np.abs(R - Rr).max() <= 1e-8
 
 
@@ -85,7 +91,7 @@ def test_loglike_plain(ctx_cfg):
 
 def test_loglike_mixed(ctx_cfg):
     which, an, inj, D, SN, c = ctx_cfg
-    n = 1024 if which == 0 else 256
+    n = 1024 if which == 0 else 2048  # config 1: the PT ensemble's full 2048 points
     P = np.vstack([inj, _data.points(which, n - 1, seed=25)])
     L, R, S = c.loglike(P, Q.MIXED)
     Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
