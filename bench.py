@@ -33,7 +33,7 @@ FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
 # over the 2048 points), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (profiles/k_ll_fp64_pt_step_hbm_traffic.md), each scaled by what the counter
 # reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-K_LL_FP64_TRAFFIC_BYTES = 30_670_000
+K_LL_FP64_TRAFFIC_BYTES = 30_660_000
 
 
 def log(*a):
@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="autotune the launch shape first")
     ap.add_argument("--only-main", action="store_true", help="time the FP64 PT steps only (for profiling)")
+    ap.add_argument("--main-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="with --only-main: the precision of the timed PT steps (mixed: to profile that kernel)")
     ap.add_argument("--pt-groups", type=int, default=0, help="PT column groups (0: the library's choice)")
     ap.add_argument("--config", type=int, default=1,
                     help="1: the PT-ensemble headline (default); 2: uneven persistent batch; "
@@ -409,7 +411,8 @@ def main():
 
     # ---- headline: FP64 fused path, PT steps (the library's column groups,
     # overlapping on their own streams), wall clock over the timed steps
-    pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=args.pt_groups)
+    main_prec = Q.MIXED if (args.only_main and args.main_precision == "mixed") else Q.FP64
+    pt = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=main_prec, groups=args.pt_groups)
     warm_clocks(pt, args.warm_seconds)
     pt.run(args.warmup)
     ctx.sync()
@@ -419,11 +422,12 @@ def main():
     T = time.perf_counter() - t0
     value = B * args.steps / T
     ms_per_step = 1000.0 * T / args.steps
-    log("FP64 fused: %.1f logL/s, %.3f ms/step (%d column groups)" % (value, ms_per_step, pt.groups))
+    log("%s fused: %.1f logL/s, %.3f ms/step (%d column groups)" %
+        ("FP64" if main_prec == Q.FP64 else "mixed", value, ms_per_step, pt.groups))
 
     # ---- the likelihood kernel alone: one group, each launch over all 2048
     # points, timed with HIP events on the stream it runs on
-    pt1 = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=Q.FP64, groups=1)
+    pt1 = Q.PT(ctx, n_temps, n_chains, 1.15, init, sigma, seed=7, precision=main_prec, groups=1)
     pt1.run(args.warmup)
     ctx.timing(1)
     ctx.sync()
@@ -431,13 +435,19 @@ def main():
     ctx.sync()
     kms, nl = ctx.timing(0)
     kernel_ms = kms / max(nl, 1)
-    cpb, tile, _ = ctx.get_shape(Q.FP64)
+    cpb, tile, _ = ctx.get_shape(main_prec)
     log("likelihood kernel alone: %.4f ms/launch over %d launches (%d points per block, %d-bin tiles)" %
         (kernel_ms, nl, cpb, tile))
     # algorithmic FP64 work of one launch, on the ensemble as the timed steps left it
     from quickcbc_amd import bench_configs as C
-    bin_points, flops = C.fused_work(ctx, an, pt1.get()[0])
+    bin_points, src_flops = C.fused_work(ctx, an, pt1.get()[0])
     pt1.close()
+    # the path's algorithmic FP64 work per bin-point: the waveform's 60 flops
+    # (Mf 1, powers 7, amplitude 20, inspiral phase 27, combine 5) and per
+    # detector 19 (projection 9, the two inner products 10), times the
+    # bin-points the launch evaluates
+    flops_per_bp = 60 + 19 * an.nifo
+    flops = flops_per_bp * bin_points
     achieved_tf = flops / (kernel_ms * 1e-3) / 1e12
     extra = {}
     if not args.only_main:
@@ -543,6 +553,7 @@ def main():
                          "%.1f s, gcc -O2 -fopenmp, OpenMP over points" % (n, dt)}
         log("cpu baseline:", cpu)
 
+    traffic = K_LL_FP64_TRAFFIC_BYTES if main_prec == Q.FP64 else None
     out = {
         "metric": "logL evals/sec (waveform+logL) at Tobs=16s 3-det; p99 PT-step latency",
         "value": value,
@@ -554,25 +565,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if main_prec == Q.FP64 else "mixed (f64 phase, f32 amplitude and sums)",
         "data": "synthetic: seeded Gaussian noise coloured by the analytic PSD plus an IMRPhenomD injection",
         "config": {"workload": "batched PT ensemble, Tobs=16 s, fs=4096 Hz, 3 det (H/L/V), band 20-2048 Hz "
-                               "(32448 bins), 32 temps x 64 chains = 2048 points/step, FP64 fused launch",
+                               "(32448 bins), 32 temps x 64 chains = 2048 points/step, %s fused launch"
+                               % ("FP64" if main_prec == Q.FP64 else "mixed-precision"),
                    "points_per_step": B, "bins": nb, "detectors": an.nifo},
         "roofline": {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP64_PEAK_TF, "traffic": K_LL_FP64_TRAFFIC_BYTES,
-                     "traffic_source": "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
-                                       "workload, scaled by a 1 GiB copy (profiles/k_ll_fp64_pt_step_hbm_traffic.md);"
-                                       " not measured in this run",
-                     "ceiling": "FP64 vector ALU issue: the kernel has no MFMA; MI355X's FP64 vector peak "
-                                "(78.6 TFLOP/s, an FMA counted as 2)",
-                     "kernel": "k_ll<3,FP64,%d>" % tile, "kernel_ms": kernel_ms, "points_per_block": cpb,
+                     "frac": achieved_tf / FP64_PEAK_TF, "traffic": traffic,
+                     "traffic_source": ("constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
+                                        "workload, scaled by a 1 GiB copy (profiles/k_ll_fp64_pt_step_hbm_traffic.md);"
+                                        " not measured in this run" if traffic else
+                                        "not measured for the mixed kernel (profiles/k_ll_mixed_pt_step_valu.md)"),
+                     "ceiling": ("FP64 vector ALU issue: the kernel has no MFMA; MI355X's FP64 vector peak "
+                                 "(78.6 TFLOP/s, an FMA counted as 2)" if main_prec == Q.FP64 else
+                                 "mixed kernel (about a third of its flops in FP64, the rest FP32): the path's "
+                                 "flops against the FP64 vector peak, a comparison with the FP64 kernel only"),
+                     "kernel": "k_ll<3,%s,%d>" % ("FP64" if main_prec == Q.FP64 else "MIXED", tile),
+                     "kernel_ms": kernel_ms, "points_per_block": cpb,
                      "flops_per_launch": flops, "bin_points_per_launch": bin_points,
-                     "flops_per_bin_point": flops / max(bin_points, 1)},
+                     "flops_per_bin_point": flops_per_bp,
+                     "kernel_source_flops_per_bin_point": src_flops / max(bin_points, 1)},
         "hbm": {"data_stream_bytes_per_launch": plane_bytes,
-                "traffic_bytes_per_launch": K_LL_FP64_TRAFFIC_BYTES,
-                "achieved_gbs": (K_LL_FP64_TRAFFIC_BYTES / (kernel_ms * 1e-3) / 1e9
-                                 if K_LL_FP64_TRAFFIC_BYTES else None),
+                "traffic_bytes_per_launch": traffic,
+                "achieved_gbs": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
                 "peak_gbs": HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }

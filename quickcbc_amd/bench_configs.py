@@ -229,7 +229,8 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
 # (5), Mf^(1/3) (2, inspiral), Mf = Msec f and 1/Mf (1 each, intermediate and
 # merger), and each region's amplitude with its Mf^(-7/6) scale and phase:
 _FUSED_AMP = (17, 13, 9)     # Horner in Mf^(1/3) and w^2; Newton form; exp / Lorentzian
-_FUSED_PHASE = (27, 8, 17)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan
+_FUSED_PHASE = (24, 6, 15)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan (the Mf-linear
+                             # term rides in the f coefficient's FMA)
 
 
 def _fused_waveform_flops(a, p):
@@ -239,8 +240,8 @@ def _fused_waveform_flops(a, p):
 
 FUSED_WAVEFORM_FLOPS = {(a, p): _fused_waveform_flops(a, p) for a in range(3) for p in range(3)}
 # reduction to t in [-1/2, 1/2] (2), t^2 (1), the two degree-6 polynomials
-# (12 + 12), t P (1), two complex squarings (5 each)
-FUSED_SINCOS_FLOPS = 38
+# (12 + 12), t P (1), two double-angle steps of twice sin and cos (3 each)
+FUSED_SINCOS_FLOPS = 34
 
 
 def fused_detector_flops(nd):
