@@ -171,7 +171,7 @@ def _oracle_het(O, an, D, SN, inj, g, MN, MM):
     return oh
 
 
-def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=1000, lat_steps=200):
+def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=1000, lat_steps=200, groups=0):
     """QuickCBC's own PT step on the heterodyned likelihood (update()'s
     Fisher / differential-evolution / jiggle / sky-ring proposals, 0.5 / 0.2 /
     0.1 / 0.2, with the reference's priors, MCMC_all's swap-or-update step,
@@ -183,7 +183,7 @@ def update_bench(Q, W, an, ctx, het, inj, B, no_cpu, D, SN, g, MN, MM, steps=100
     init = W.ensemble_in_prior(inj, n_temps * n_chains, lo, hi, seed=99)
     desc = Q.update_desc(lo, hi, efix_zs=1.0)  # efix of the Fisher jumps, QuickCBC.c:1927
     t0 = time.perf_counter()
-    pt = Q.PTUpdate(ctx, het, n_temps, n_chains, 1.15, init, desc, seed=5)
+    pt = Q.PTUpdate(ctx, het, n_temps, n_chains, 1.15, init, desc, seed=5, groups=groups)
     create_ms = (time.perf_counter() - t0) * 1e3
     pt.run(20)
     t0 = time.perf_counter()
@@ -384,7 +384,7 @@ def main():
         dp = Q.DeviceBuffer.from_array(np.ascontiguousarray(init, np.float64))
         r = {"fisher": fisher_bench(Q, het, init, dp, 100, True, an, D, SN, inj, g, MN, MM),
              "pt_update": update_bench(Q, W, an, ctx, het, inj, B, True, D, SN, g, MN, MM, steps=args.steps,
-                                       lat_steps=20)}
+                                       lat_steps=20, groups=args.pt_groups)}
         print(json.dumps({"config": "QuickCBC PT step, heterodyned likelihood", **r}), flush=True)
         het.close()
         dp.free()

@@ -422,7 +422,8 @@ int qcbc_pt_create(qcbc_ctx *ctx, int n_temps, int n_chains, double ladder, cons
                    const double *sigma, uint64_t seed, int precision, qcbc_pt **out);
 /* As qcbc_pt_create, with the chain columns split into `groups` (1..4, at
  * most n_chains; 0 = 3 when n_chains >= 3 and n_temps*n_chains >= 1536, else
- * 1) groups that step on streams of their own, so that one group's
+ * 1; on the heterodyned likelihood, qcbc_pt_create_het and
+ * qcbc_pt_create_update, 0 = 1) groups that step on streams of their own, so that one group's
  * latency-bound kernels and likelihood tail overlap another group's
  * likelihood kernel.  The chains do not depend on the number of groups. */
 int qcbc_pt_create_groups(qcbc_ctx *ctx, int n_temps, int n_chains, double ladder, const double *init,

@@ -203,7 +203,11 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
   pt->ladder = ladder;
   pt->seed = seed;
   pt->precision = precision;
-  pt->G = groups ? groups : (n_chains >= 3 && pt->P >= 1536 ? 3 : 1);
+  // groups by default: three for the full likelihood (one group's set-up and
+  // post kernels run beside another group's likelihood kernel), one on the
+  // heterodyned likelihood, whose kernels are all short and latency-bound
+  // (more groups only add launches)
+  pt->G = groups ? groups : (!het && n_chains >= 3 && pt->P >= 1536 ? 3 : 1);
   if (ud) {
     pt->upd = true;
     for (int i = 0; i < 11; i++) {

@@ -35,6 +35,7 @@
 
 #include "qcbc_device.h"
 #include "qcbc_update_internal.h"
+#include "qcbc_wrap.h"
 
 namespace qcbc {
 
@@ -364,15 +365,15 @@ __device__ double propose_component(const double* __restrict__ x, const double* 
                : j == 6 ? o.y.p6
                : j == 7 ? o.y.p7 : j == 8 ? o.y.p8 : j == 9 ? o.y.p9 : j == 10 ? o.y.p10 : x[j];
   }
-  // angles back into range (:2336-2342).  Only an angle within +-ANGLE_WRAP_MAX
-  // rad is wrapped, so each loop ends within ~1600 turns; one beyond (or not
-  // finite) stays out of its prior range and the proposal is rejected.
+  // angles back into range (:2336-2342), with the loops' own result (a run of
+  // steps within one binade taken at once, so an angle of 1e4 rad costs ~40
+  // turns instead of 1600).  Only an angle within +-ANGLE_WRAP_MAX rad is
+  // wrapped; one beyond (or not finite) stays out of its prior range and the
+  // proposal is rejected.
   if ((j == 4 || j == 7) && fabs(y) <= ANGLE_WRAP_MAX) {
-    while (y > TPI_U) y -= TPI_U;
-    while (y < 0.0) y += TPI_U;
+    y = wrap_up(wrap_down(y, TPI_U), TPI_U);
   } else if (j == 9 && fabs(y) <= ANGLE_WRAP_MAX) {
-    while (y > PI_U) y -= PI_U;
-    while (y < 0.0) y += PI_U;
+    y = wrap_up(wrap_down(y, PI_U), PI_U);
   }
   return y;
 }
