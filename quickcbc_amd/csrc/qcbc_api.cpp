@@ -17,6 +17,7 @@
 
 #include "../../include/qcbc.h"
 #include "qcbc_api_internal.h"
+#include "qcbc_update_internal.h"
 #include "qcbc_condition_internal.h"
 #include "qcbc_internal.h"
 
@@ -43,7 +44,8 @@ int ensure_device() {
     g_ready_dev[dev] = -1;
     return QCBC_ENODEV;
   }
-  if (launch_qnm_init(nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+  if (launch_qnm_init(nullptr) != hipSuccess || launch_update_qnm_init(nullptr) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
     g_ready_dev[dev] = -1;
     return QCBC_ENODEV;
   }

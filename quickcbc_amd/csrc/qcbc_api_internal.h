@@ -97,7 +97,8 @@ int het_fisher_enqueue(qcbc_het* h, const double* d_params, int B, double* d_ej,
 struct HetGridView {
   int device, M, J, NR, nd;
   AnalysisConst an;
-  const double *freq, *pl, *wf, *amp;
+  const double *freq, *pl, *wf, *amp, *phase;
+  double logLb;
 };
 int het_grid_view(const qcbc_het* h, HetGridView* out);
 

@@ -247,8 +247,48 @@ __device__ inline DD dd_log(double x) {
 __device__ inline double exp_cr(double x) { return dd_exp({x, 0.0}).hi; }
 __device__ inline double pow_cr(double x, double y) { return dd_exp(dd_mul_d(dd_log(x), y)).hi; }
 
-// QNM spline slopes, filled once per device by qcbc_qnm_init_kernel.
-extern __device__ double g_qnm_slope[2][QCBC_QNM_N];
+// Slopes of the natural cubic splines through the QNM tables (row 0 f_ring,
+// row 1 f_damp), one copy per translation unit (device globals do not link
+// across units without relocatable device code): each unit that evaluates the
+// set-up fills its copy once per device with a kernel calling
+// qnm_init_tables.
+[[maybe_unused]] static __device__ double g_qnm_slope[2][QCBC_QNM_N];
+[[maybe_unused]] static __device__ double g_qnm_sweep[2][QCBC_QNM_N];  // modified super-diagonal of the solve
+
+// The tridiagonal system h_i m_{i-1} + 2(h_{i-1}+h_i) m_i + h_{i-1} m_{i+1}
+// = 3(h_i s_{i-1} + h_{i-1} s_i), with natural end rows, by the Thomas
+// algorithm; thread w = 0 builds the f_ring spline, w = 1 the f_damp spline.
+__device__ inline void qnm_init_tables(int w) {
+  if (w > 1) return;
+  const double* x = qcbc_qnm_a;
+  const double* y = w ? qcbc_qnm_fdamp : qcbc_qnm_fring;
+  double* m = g_qnm_slope[w];
+  const int n = QCBC_QNM_N;
+  double* cprime = g_qnm_sweep[w];
+  double dprime;
+  double prev_c = 0.0, prev_d = 0.0;
+  for (int i = 0; i < n; i++) {
+    double a, b, cc, r;
+    if (i == 0) {
+      double h0 = x[1] - x[0];
+      a = 0.0; b = 2.0; cc = 1.0; r = 3.0 * (y[1] - y[0]) / h0;
+    } else if (i == n - 1) {
+      double h = x[n - 1] - x[n - 2];
+      a = 1.0; b = 2.0; cc = 0.0; r = 3.0 * (y[n - 1] - y[n - 2]) / h;
+    } else {
+      double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+      double sl = (y[i] - y[i - 1]) / hl, sr = (y[i + 1] - y[i]) / hr;
+      a = hr; b = 2.0 * (hl + hr); cc = hl; r = 3.0 * (hr * sl + hl * sr);
+    }
+    double den = b - a * prev_c;
+    prev_c = cc / den;
+    dprime = (r - a * prev_d) / den;
+    prev_d = dprime;
+    cprime[i] = prev_c;
+    m[i] = dprime;
+  }
+  for (int i = n - 2; i >= 0; i--) m[i] -= cprime[i] * m[i + 1];
+}
 
 // ------------------------------------------------------------- helpers
 __device__ __forceinline__ double fit11(int w, double eta, double chi) {
@@ -645,6 +685,13 @@ __device__ inline int phenomd_prefix(double m1_SI, double m2_SI, double chi1_in,
   return ST_OK;
 }
 
+// fmaxCalc (IMRPhenomD_internals.c:793-808, set at :1090): the peak of the
+// merger-ringdown amplitude, from the ringdown and damping frequencies and the
+// fits gamma2, gamma3
+__device__ __forceinline__ double fmax_calc(double fRD, double fDM, double g2, double g3) {
+  return (g2 <= 1) ? fabs(fRD + (fDM * (-1 + sqrt(1 - g2 * g2)) * g3) / g2) : fabs(fRD + (fDM * (-1) * g3) / g2);
+}
+
 __device__ inline void phenomd_amp(const PDPre& o, Point& c) {
   const double chi1 = o.chi1, chi2 = o.chi2, eta = o.eta, chi = o.chi, delta_m = o.delta_m;
   const double pi_s6 = PI_S6, pi_two_thirds = PI_TWO_THIRDS;
@@ -656,8 +703,7 @@ __device__ inline void phenomd_amp(const PDPre& o, Point& c) {
   c.fDMg3 = c.fDM * g3;
   c.mrd_num = c.fDMg3 * g1;
   c.mrd_rate = g2 / c.fDMg3;
-  c.fmaxCalc = (g2 <= 1) ? fabs(c.fRD + (c.fDM * (-1 + sqrt(1 - g2 * g2)) * g3) / g2)
-                         : fabs(c.fRD + (c.fDM * (-1) * g3) / g2);
+  c.fmaxCalc = fmax_calc(c.fRD, c.fDM, g2, g3);
   const double rho1 = fit11(QCBC_FIT_RHO1, eta, chi), rho2 = fit11(QCBC_FIT_RHO2, eta, chi),
                rho3 = fit11(QCBC_FIT_RHO3, eta, chi);
   // init_amp_ins_prefactors (:612-665)
@@ -751,15 +797,20 @@ __device__ inline void phenomd_phase(const PDPre& o, Point& c) {
   }
 }
 
-__device__ inline void phenomd_refphase(double phi0, double fref, Point& c) {
-  // time shift and reference phase (IMRPhenomD.c:504-521)
-  c.t0 = dphi_mrd(c, c.fmaxCalc);
+// time shift and reference phase (IMRPhenomD.c:504-521) with fmaxCalc given:
+// reads only the prefix and phase fields of c
+__device__ inline void phenomd_refphase_at(double phi0, double fref, double fmax, Point& c) {
+  c.t0 = dphi_mrd(c, fmax);
   c.MfRef = c.Msec * fref;
   c.phi_precalc = 0.0;
   {
     Pow pr = pow_direct(c.MfRef);
     c.phi_precalc = 2. * phi0 + phase(c, pr);  // phase() subtracts t0*0 + 0 here
   }
+}
+
+__device__ inline void phenomd_refphase(double phi0, double fref, Point& c) {
+  phenomd_refphase_at(phi0, fref, c.fmaxCalc, c);
 }
 
 __device__ inline int phenomd_setup(double m1_SI, double m2_SI, double chi1_in, double chi2_in, double distance,
@@ -889,6 +940,32 @@ __device__ inline int point_setup_model_half(const double* prm, const AnalysisCo
 
 __device__ inline void point_setup_refphase(const double* prm, const AnalysisConst& an, Point& c) {
   phenomd_refphase(0.5 * prm[4], an.fref, c);
+}
+
+// Half 1 of the model (the prefix and the phase coefficients) followed by the
+// time shift and reference phase, with fmaxCalc recomputed from the same fits
+// as half 0 computes it (the same operations on the same values): after half
+// 0, fbegin and the projection the record holds everything the heterodyned
+// likelihood reads except the final detector scales (Ascale amp0 / sqrt(Tobs),
+// which its evaluation applies).
+__device__ inline int point_setup_phase_ref(const double* prm, const AnalysisConst& an, Point& c) {
+  c.status = ST_OK;
+  c.kstart = c.kend = an.imin;
+  const double mc = exp_cr(prm[0]), mt = exp_cr(prm[1]);
+  const double eta_in = pow_cr(mc / mt, 5.0 / 3.0);
+  const double dm = eta_in > 0.25 ? 0.0 : sqrt(1.0 - 4.0 * eta_in);
+  const double m1_SI = mt * (1.0 + dm) / 2.0, m2_SI = mt * (1.0 - dm) / 2.0;
+  PDPre o;
+  const int st = phenomd_prefix(m1_SI, m2_SI, prm[2], prm[3], exp_cr(prm[6]), an.fref, an.fmin0, c, o);
+  if (st != ST_OK) {
+    c.status = st;
+    return st;
+  }
+  phenomd_phase(o, c);
+  const double fmax =
+      fmax_calc(c.fRD, c.fDM, fit11(QCBC_FIT_GAMMA2, o.eta, o.chi), fit11(QCBC_FIT_GAMMA3, o.eta, o.chi));
+  phenomd_refphase_at(0.5 * prm[4], an.fref, fmax, c);
+  return ST_OK;
 }
 
 // The projection constants (QuickCBC.c:3734-3750); Ascale without amp0,

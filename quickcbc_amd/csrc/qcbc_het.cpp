@@ -253,6 +253,8 @@ int qcbc::het_grid_view(const qcbc_het* h, HetGridView* out) {
   out->pl = h->d_pl;
   out->wf = h->d_wf;
   out->amp = h->d_amp;
+  out->phase = h->d_phase;
+  out->logLb = h->logLb;
   return QCBC_OK;
 }
 

@@ -10,9 +10,6 @@
 
 namespace qcbc {
 
-__device__ double g_qnm_slope[2][QCBC_QNM_N];
-__device__ double g_qnm_sweep[2][QCBC_QNM_N];  // modified super-diagonal of the spline solve
-
 // ------------------------------------------------------------ utilities
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -22,42 +19,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Slopes of the natural cubic spline through (a_i, y_i): the tridiagonal
-// system h_i m_{i-1} + 2(h_{i-1}+h_i) m_i + h_{i-1} m_{i+1}
-// = 3(h_i s_{i-1} + h_{i-1} s_i), with natural end rows; Thomas algorithm.
-// Thread 0 builds the f_ring spline, thread 1 the f_damp spline.
-__global__ void k_qnm_init() {
-  int w = threadIdx.x;
-  if (w > 1) return;
-  const double* x = qcbc_qnm_a;
-  const double* y = w ? qcbc_qnm_fdamp : qcbc_qnm_fring;
-  double* m = g_qnm_slope[w];
-  const int n = QCBC_QNM_N;
-  double* cprime = g_qnm_sweep[w];
-  double dprime;
-  double prev_c = 0.0, prev_d = 0.0;
-  for (int i = 0; i < n; i++) {
-    double a, b, cc, r;
-    if (i == 0) {
-      double h0 = x[1] - x[0];
-      a = 0.0; b = 2.0; cc = 1.0; r = 3.0 * (y[1] - y[0]) / h0;
-    } else if (i == n - 1) {
-      double h = x[n - 1] - x[n - 2];
-      a = 1.0; b = 2.0; cc = 0.0; r = 3.0 * (y[n - 1] - y[n - 2]) / h;
-    } else {
-      double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
-      double sl = (y[i] - y[i - 1]) / hl, sr = (y[i + 1] - y[i]) / hr;
-      a = hr; b = 2.0 * (hl + hr); cc = hl; r = 3.0 * (hr * sl + hl * sr);
-    }
-    double den = b - a * prev_c;
-    prev_c = cc / den;
-    dprime = (r - a * prev_d) / den;
-    prev_d = dprime;
-    cprime[i] = prev_c;
-    m[i] = dprime;
-  }
-  for (int i = n - 2; i >= 0; i--) m[i] -= cprime[i] * m[i + 1];
-}
+// this unit's copy of the QNM spline slopes (the set-up kernels below)
+__global__ void k_qnm_init() { qnm_init_tables(threadIdx.x); }
 
 // ------------------------------------------------------------- set-up
 // 64 points per block, four waves: waves 0 and 3 the model (both the masses

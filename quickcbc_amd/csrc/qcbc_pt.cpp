@@ -85,6 +85,24 @@ struct qcbc_pt {
     }                            \
   } while (0)
 
+// n of QuickCBC's steps of group k in one launch of the fused step kernel:
+// per step the set-up and heterodyned logL of the pending proposals y, the
+// acceptance or swap, the history and the next proposals.
+static int group_update_steps(qcbc_pt* pt, int k, int n) {
+  Group& gr = pt->g[k];
+  const size_t o = gr.off;
+  HetGridView hv{};
+  int st = het_grid_view(pt->het, &hv);
+  if (st) return st;
+  const HetArgs h{hv.freq, hv.pl, hv.amp, hv.phase, hv.wf, hv.M, hv.nd, hv.logLb};
+  HIPCHK(launch_pt_steps(pt->d_x + 11 * o, pt->d_L + o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
+                         pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_hcount,
+                         pt->d_y + 11 * o, pt->d_st + o, pt->d_typ + o, pt->d_jac + o, pt->d_tc, gr.d_pts, pt->uc,
+                         hv.an, h, pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, n, gr.d_step, pt->d_ustats,
+                         gr.d_done, gr.s));
+  return QCBC_OK;
+}
+
 // One PT step of group k: the proposals y are already on the device (made by
 // the previous step, or at creation).  Set-up and likelihood partial sums of
 // y, then k_pt_post: logL of y, acceptance, swaps, the next proposals and the
@@ -94,17 +112,7 @@ static int group_step(qcbc_pt* pt, int k) {
   Group& gr = pt->g[k];
   const int prec = pt->precision;
   const size_t o = gr.off;
-  if (pt->upd) {  // heterodyned logL of y, then QuickCBC's step
-    int st = het_enqueue(pt->het, pt->d_y + 11 * o, gr.P, gr.d_pts, pt->d_st + o, pt->d_Ly + o, gr.s);
-    if (st) return st;
-    HIPCHK(launch_pt_update(pt->d_x + 11 * o, pt->d_L + o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
-                            pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_hcount,
-                            pt->d_y + 11 * o, pt->d_Ly + o, pt->d_st + o, pt->d_typ + o, pt->d_jac + o, pt->d_tc,
-                            pt->uc,
-                            pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, gr.d_step, pt->d_ustats, gr.d_done,
-                            gr.s));
-    return QCBC_OK;
-  }
+  if (pt->upd) return group_update_steps(pt, k, 1);  // QuickCBC's step, fused
   if (pt->het) {  // set-up and heterodyned logL of y, then the same post kernel
     int st = het_enqueue(pt->het, pt->d_y + 11 * o, gr.P, gr.d_pts, pt->d_st + o, pt->d_Ly + o, gr.s);
     if (st) return st;
@@ -467,6 +475,10 @@ static int pt_sync(qcbc_pt* pt) {
 // do on the free-running streams; a one-step graph would join every step.
 constexpr int kGraphSteps = 32;
 
+// Steps per launch of QuickCBC's fused step kernel in a run (a launch of
+// 128 steps of a 32 x 64 ensemble takes a few ms)
+constexpr int kUpdateStepsPerLaunch = 128;
+
 // Captures n consecutive steps of the whole ensemble into pt->graph/exec[slot].
 static int pt_capture(qcbc_pt* pt, int slot, int n) {
   qcbc_ctx* c = pt->ctx;
@@ -506,6 +518,19 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
   // no memset nodes in any graph: every fill is a kernel.  Timed steps
   // replay the one-step graph; otherwise the run is kGraphSteps-step graphs
   // and one graph of the remaining steps, captured for that count.
+  // QuickCBC's step runs whole runs of steps per launch of its fused kernel,
+  // each group on its own stream: graphs would capture the same launches
+  if (pt->upd && !pt->step_timing) {
+    for (int done = 0; done < nsteps;) {
+      const int n = nsteps - done < kUpdateStepsPerLaunch ? nsteps - done : kUpdateStepsPerLaunch;
+      for (int k = 0; k < pt->G; k++) {
+        int st = group_update_steps(pt, k, n);
+        if (st) return st;
+      }
+      done += n;
+    }
+    return pt_sync(pt);
+  }
   const bool multi = use_graph && !pt->step_timing;
   const int full = multi ? nsteps / kGraphSteps : 0, rem = multi ? nsteps % kGraphSteps : 0;
   if (use_graph && !multi && !pt->exec[0]) {

@@ -6,7 +6,8 @@
 // Per column and step s, as MCMC_all's iteration: with probability 0.2 (more
 // than one temperature) one swap of a random adjacent pair of temperatures,
 // otherwise update() at every temperature: the proposal y made at the end of
-// step s - 1 (its heterodyned logL already evaluated) is accepted when
+// step s - 1, set up and its heterodyned logL evaluated at the start of step
+// s, is accepted when
 //   H = (logL_y - logL_x)/T + pMcM(y) + 3 lnDL_y - 3 lnDL_x - pMcM(x) > log u
 // and y lies in the prior (bounds, eta <= 1/4, eta >= etamin), with
 // pMcM = 2 ln Mt + ln eta - ln(1 - 4 eta)/2 the Jacobian that makes the prior
@@ -34,6 +35,7 @@
 #include <cstdint>
 
 #include "qcbc_device.h"
+#include "qcbc_het_device.h"
 #include "qcbc_update_internal.h"
 #include "qcbc_wrap.h"
 
@@ -410,7 +412,7 @@ __device__ void propose_column(const double* __restrict__ x, const double* __res
 }
 
 // Proposals of step *step for the group's points (used at creation and after
-// a Fisher refresh; later steps get theirs from k_pt_update).  One block per
+// a Fisher refresh; later steps get theirs from k_pt_steps).  One block per
 // column.
 __global__ __launch_bounds__(512) void k_update_propose(const double* __restrict__ x, const double* __restrict__ ej,
                                                         const double* __restrict__ ev, const double* __restrict__ hist,
@@ -423,103 +425,237 @@ __global__ __launch_bounds__(512) void k_update_propose(const double* __restrict
                  blockDim.x, y, typ, jac);
 }
 
-// One PT step of a group of chain columns after the heterodyned logL of the
-// proposals (Ly, status st_y): block b owns column c0 + b.
-__global__ __launch_bounds__(512) void k_pt_update(
-    double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej, double* __restrict__ ev,
-    double* __restrict__ hist, unsigned long long* __restrict__ hcount, double* __restrict__ y,
-    const double* __restrict__ Ly, const int* __restrict__ st_y, int* __restrict__ typ, double* __restrict__ jac,
-    const double* __restrict__ tc,
-    UpdateConst u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed, unsigned long long* __restrict__ step,
-    unsigned long long* __restrict__ stats, unsigned int* __restrict__ done) {
-  const int b = blockIdx.x, c = c0 + b;
-  const uint64_t P = (uint64_t)n_temps * n_chains, s = *step;
-  const uint64_t cs = s * (uint64_t)n_chains + c;  // the column's counter for this step
-  // stats per column: [0] update steps, [1] swap steps, [2] swaps accepted,
-  // [3 + 2 (typ - 1)] proposals of type typ, [4 + 2 (typ - 1)] accepted
-  unsigned long long* cst = stats + (size_t)c * UPD_STATS;
-  const bool swap_step = n_temps > 1 && uni(seed, SALT_SWAP, cs, 0) < 0.2;
-  if (!swap_step) {
-    for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
-      const int lp = t * ncols + b;
-      const uint64_t gp = (uint64_t)t * n_chains + c;
-      const double* q = y + 11 * lp;
-      double pMcMy;
-      const bool flag = prior_flag(q, u, &pMcMy) || st_y[lp] != 0;
-      const int k = typ[lp];
-      atomicAdd(&cst[3 + 2 * (k - 1)], 1ULL);
-      if (!flag) {
-        const double* xp = x + 11 * lp;
-        const double H =
-            jac[lp] + (Ly[lp] - L[lp]) * tc[3 * t + 1] + pMcMy + 3.0 * q[6] - 3.0 * xp[6] - mass_jacobian(xp);
-        if (H > log(uni(seed, SALT_ACCEPT, s * P + gp, 0))) {
-          for (int i = 0; i < 11; i++) x[11 * lp + i] = q[i];
-          L[lp] = Ly[lp];
-          atomicAdd(&cst[4 + 2 * (k - 1)], 1ULL);
-        }
-      }
-    }
-    __syncthreads();
-    if (s % (uint64_t)u.hist_every == 0) {  // the column's states enter its history
-      const uint64_t m = hcount[c];
-      for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
-        double* h = hist + (((size_t)b * n_temps + t) * u.n_hist + (size_t)(m % (uint64_t)u.n_hist)) * 11;
-        for (int i = 0; i < 11; i++) h[i] = x[11 * (t * ncols + b) + i];
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) hcount[c] = m + 1;
-    }
-    if (threadIdx.x == 0) cst[0] += 1;
-  } else {
-    // one swap of temperatures j, j + 1 (:2038-2051)
-    __shared__ int sw;
-    if (threadIdx.x == 0) {
-      const int j = (int)((double)(n_temps - 1) * uni(seed, SALT_SWAP, cs, 1));
-      const double dL = L[j * ncols + b] - L[(j + 1) * ncols + b];
-      const double beta = exp(dL * tc[3 * (j + 1) + 1] - dL * tc[3 * j + 1]);
-      sw = beta > uni(seed, SALT_SWAP, cs, 2) ? j : -1;
-      cst[1] += 1;
-      if (sw >= 0) cst[2] += 1;
-    }
-    __syncthreads();
-    if (sw >= 0) {
-      const int la = sw * ncols + b, lb = (sw + 1) * ncols + b;
-      for (int i = threadIdx.x; i < 11 + 11 + 121 + 1; i += blockDim.x) {
-        double* pa;
-        double* pb;
-        if (i < 11) {
-          pa = x + 11 * la + i;
-          pb = x + 11 * lb + i;
-        } else if (i < 22) {
-          pa = ej + 11 * la + (i - 11);
-          pb = ej + 11 * lb + (i - 11);
-        } else if (i < 143) {
-          pa = ev + 121 * la + (i - 22);
-          pb = ev + 121 * lb + (i - 22);
-        } else {
-          pa = L + la;
-          pb = L + lb;
-        }
-        const double tmp = *pa;
-        *pa = *pb;
-        *pb = tmp;
-      }
+// update()'s acceptance of one proposal q (logL Lq, set-up status st, type
+// k, sky-ring log density jc) against the state xs (logL *Ls) at
+// temperature index t (:2345-2372), counted in the column's statistics cst:
+// [3 + 2 (k - 1)] proposed, [4 + 2 (k - 1)] accepted
+__device__ __forceinline__ void accept_one(double* __restrict__ xs, double* __restrict__ Ls,
+                                           const double* __restrict__ q, double Lq, int st, int k, double jc,
+                                           const double* __restrict__ tc, int t, const UpdateConst& u, uint64_t seed,
+                                           uint64_t ctr, unsigned long long* __restrict__ cst) {
+  double pMcMy;
+  const bool flag = prior_flag(q, u, &pMcMy) || st != 0;
+  atomicAdd(&cst[3 + 2 * (k - 1)], 1ULL);
+  if (!flag) {
+    const double H = jc + (Lq - *Ls) * tc[3 * t + 1] + pMcMy + 3.0 * q[6] - 3.0 * xs[6] - mass_jacobian(xs);
+    if (H > log(uni(seed, SALT_ACCEPT, ctr, 0))) {
+      for (int i = 0; i < 11; i++) xs[i] = q[i];
+      *Ls = Lq;
+      atomicAdd(&cst[4 + 2 * (k - 1)], 1ULL);
     }
   }
+}
+
+// On update steps with s % hist_every == 0 the column's states enter its
+// history (:2071-2080).  All threads of the block; ends synchronised.
+__device__ void column_history(const double* __restrict__ x, double* __restrict__ hist,
+                               unsigned long long* __restrict__ hcount, const UpdateConst& u, int n_temps, int ncols,
+                               int b, int c, uint64_t s) {
+  if (s % (uint64_t)u.hist_every != 0) return;
+  const uint64_t m = hcount[c];
+  for (int t = threadIdx.x; t < n_temps; t += blockDim.x) {
+    double* h = hist + (((size_t)b * n_temps + t) * u.n_hist + (size_t)(m % (uint64_t)u.n_hist)) * 11;
+    for (int i = 0; i < 11; i++) h[i] = x[11 * (t * ncols + b) + i];
+  }
   __syncthreads();
-  // proposals of step s + 1
-  propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, b, seed, s + 1, threadIdx.x, blockDim.x, y,
-                 typ, jac);
+  if (threadIdx.x == 0) hcount[c] = m + 1;
+}
+
+// One swap of temperatures j, j + 1 of the column (:2038-2051), with the
+// Fisher jumps that belong to the states.  All threads of the block; *sw is
+// block-shared.
+__device__ void column_swap(double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej,
+                            double* __restrict__ ev, const double* __restrict__ tc, int n_temps, int ncols, int b,
+                            uint64_t seed, uint64_t cs, unsigned long long* __restrict__ cst, int* sw) {
+  if (threadIdx.x == 0) {
+    const int j = (int)((double)(n_temps - 1) * uni(seed, SALT_SWAP, cs, 1));
+    const double dL = L[j * ncols + b] - L[(j + 1) * ncols + b];
+    const double beta = exp(dL * tc[3 * (j + 1) + 1] - dL * tc[3 * j + 1]);
+    *sw = beta > uni(seed, SALT_SWAP, cs, 2) ? j : -1;
+    cst[1] += 1;
+    if (*sw >= 0) cst[2] += 1;
+  }
   __syncthreads();
+  const int w = *sw;
+  if (w >= 0) {
+    const int la = w * ncols + b, lb = (w + 1) * ncols + b;
+    for (int i = threadIdx.x; i < 11 + 11 + 121 + 1; i += blockDim.x) {
+      double* pa;
+      double* pb;
+      if (i < 11) {
+        pa = x + 11 * la + i;
+        pb = x + 11 * lb + i;
+      } else if (i < 22) {
+        pa = ej + 11 * la + (i - 11);
+        pb = ej + 11 * lb + (i - 11);
+      } else if (i < 143) {
+        pa = ev + 121 * la + (i - 22);
+        pb = ev + 121 * lb + (i - 22);
+      } else {
+        pa = L + la;
+        pb = L + lb;
+      }
+      const double tmp = *pa;
+      *pa = *pb;
+      *pb = tmp;
+    }
+  }
+}
+
+// The last block of a launch to finish advances the group's step counter.
+__device__ __forceinline__ void finish_launch(unsigned long long* __restrict__ step, unsigned int* __restrict__ done,
+                                              uint64_t next) {
   if (threadIdx.x == 0) {
     __threadfence();
     if (atomicAdd(done, 1u) == gridDim.x - 1) {
-      *step = s + 1;
+      *step = next;
       *done = 0u;
       __threadfence();
     }
   }
 }
+
+// ---------------------------------------------------------- fused steps
+// Set-up of the column's proposals y (storage t ncols + b) for the
+// heterodyned likelihood, the parts of a point spread over waves as k_setup
+// spreads them: role (wave & 3) 0 the amplitude half of the model (and the
+// status), 3 the phase half with the time shift and reference phase, 1
+// fbegin, 2 the projection; temperature lane + 64 (wave >> 2), then every
+// 64 (blockDim / 256)-th after it.  The caller synchronises after it.
+// one part of one point's set-up (kept out of line: each part's registers are
+// its own, not added to the step kernel's)
+__device__ __noinline__ void setup_part(int role, const double* __restrict__ q, const AnalysisConst& an,
+                                        Point& c, int* __restrict__ st) {
+  double prm[11];
+#pragma unroll
+  for (int i = 0; i < 11; i++) prm[i] = q[i];
+  if (role == 0)
+    *st = point_setup_model_half(prm, an, c, 0);
+  else if (role == 3)
+    point_setup_phase_ref(prm, an, c);
+  else if (role == 1)
+    c.fs = fbegin(prm);
+  else
+    point_setup_projection(prm, an, c);
+}
+
+__device__ void column_setup(const double* __restrict__ y, const AnalysisConst& an, Point* __restrict__ pts,
+                             int* __restrict__ st_y, int n_temps, int ncols, int b) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), role = wave & 3;
+  const int per = (int)(blockDim.x >> 8) * 64;
+  for (int t = (wave >> 2) * 64 + (int)(threadIdx.x & 63); t < n_temps; t += per) {
+    const int lp = t * ncols + b;
+    setup_part(role, y + 11 * lp, an, pts[lp], st_y + lp);
+  }
+}
+
+__device__ __forceinline__ double seg_sum(double v) {
+#pragma unroll
+  for (int o = PROP_LANES / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, PROP_LANES);
+  return v;
+}
+
+// log_likelihood_het (QuickCBC.c:3871-3940) of one set-up point by the
+// PROP_LANES lanes of its segment (lane j takes grid samples j, j +
+// PROP_LANES, ...), as k_het_ll evaluates each sample: the h22 amplitude and
+// phase from the sample planes, the taper, each detector's projection against
+// the heterodyne's amplitude and phase, and the folded weights;
+// logL = logLb - sum_d (HR_d + HH_d / 2), 0 for a point whose set-up failed.
+// Every lane returns the sum.
+template <int ND>
+__device__ double het_ll_segment(const Point& c, const AnalysisConst& an, const HetArgs& h, int j) {
+  double hr[ND], hh[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) hr[d] = hh[d] = 0.0;
+  const bool live = c.status == ST_OK;
+  if (live) {
+    const int M = h.M;
+    const double i_Msec = 1.0 / c.Msec;
+    double As[ND];  // the detector scales Ascale amp0 / sqrt(Tobs), as point_setup_finish forms them
+#pragma unroll
+    for (int d = 0; d < ND; d++) As[d] = c.Ascale[d] * c.amp0 / an.sqT;
+    for (int m = j; m < M; m += PROP_LANES) {
+      // the point's constants are loaded again per sample rather than held
+      // across the loop (the segments of a wave hold different points, so
+      // they are vector registers, ~70 doubles)
+      asm volatile("" ::: "memory");
+      const double f = h.freq[m];
+      double a, p;
+      h22_planes(c, i_Msec, f, h.pl, M, m, &a, &p);
+      const double yt = taper(f, c.fs);
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        const double xd = TPI * f * c.tshift[d] + c.lambda[d] - p;
+        const double ad = As[d] * a * yt;
+        double sn, cs;
+        sincos(h.hp[(size_t)d * M + m] - xd, &sn, &cs);
+        const double cc = 2.0 * (h.ha[(size_t)d * M + m] - ad * cs);
+        const double ss = 2.0 * ad * sn;
+        hr[d] += h.wf[(size_t)(1 * ND + d) * M + m] * cc + h.wf[(size_t)(2 * ND + d) * M + m] * ss;
+        hh[d] += h.wf[(size_t)(0 * ND + d) * M + m] * (cc * cc + ss * ss);
+      }
+    }
+  }
+  double Lv = h.logLb;
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    const double r = seg_sum(hr[d]), q = seg_sum(hh[d]);
+    Lv -= (r + 0.5 * q);
+  }
+  return live ? Lv : 0.0;
+}
+
+// nsteps PT steps of a group of chain columns in one launch; block b runs
+// column c0 + b through all of them (the columns are independent chains, so
+// no block waits for another).  Per step: the set-up and heterodyned logL of
+// the pending proposals, the logL evaluated by each temperature's
+// PROP_LANES-lane segment and its acceptance decided there (update steps),
+// or one swap (a swap step skips the set-up and logL: its proposals are
+// discarded unevaluated); then the history and the proposals of the next step.
+template <int ND>
+__global__ __launch_bounds__(512) void k_pt_steps(
+    double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej, double* __restrict__ ev,
+    double* __restrict__ hist, unsigned long long* __restrict__ hcount, double* __restrict__ y,
+    int* __restrict__ st_y, int* __restrict__ typ, double* __restrict__ jac, const double* __restrict__ tc,
+    Point* __restrict__ pts, UpdateConst u, AnalysisConst an, HetArgs h, int n_temps, int n_chains, int c0,
+    int ncols, uint64_t seed, int nsteps, unsigned long long* __restrict__ step,
+    unsigned long long* __restrict__ stats, unsigned int* __restrict__ done) {
+  const int b = blockIdx.x, c = c0 + b;
+  const uint64_t P = (uint64_t)n_temps * n_chains, s0 = *step;
+  unsigned long long* cst = stats + (size_t)c * UPD_STATS;
+  const int j = (int)(threadIdx.x % PROP_LANES), seg = (int)(threadIdx.x / PROP_LANES);
+  const int nseg = (int)(blockDim.x / PROP_LANES);
+  __shared__ int sw;
+  for (int k = 0; k < nsteps; k++) {
+    const uint64_t s = s0 + (uint64_t)k;
+    const uint64_t cs = s * (uint64_t)n_chains + c;
+    const bool swap_step = n_temps > 1 && uni(seed, SALT_SWAP, cs, 0) < 0.2;
+    if (!swap_step) {
+      column_setup(y, an, pts, st_y, n_temps, ncols, b);
+      __syncthreads();
+      for (int t = seg; t < n_temps; t += nseg) {
+        const int lp = t * ncols + b;
+        const double Lq = het_ll_segment<ND>(pts[lp], an, h, j);
+        if (j == 0)
+          accept_one(x + 11 * lp, L + lp, y + 11 * lp, Lq, st_y[lp], typ[lp], jac[lp], tc, t, u, seed,
+                     s * P + (uint64_t)t * n_chains + c, cst);
+      }
+      __syncthreads();
+      column_history(x, hist, hcount, u, n_temps, ncols, b, c, s);
+      if (threadIdx.x == 0) cst[0] += 1;
+    } else {
+      column_swap(x, L, ej, ev, tc, n_temps, ncols, b, seed, cs, cst, &sw);
+    }
+    __syncthreads();
+    propose_column(x, ej, ev, hist, tc, u, n_temps, n_chains, c0, ncols, b, seed, s + 1, threadIdx.x, blockDim.x,
+                   y, typ, jac);
+    __syncthreads();
+  }
+  finish_launch(step, done, s0 + (uint64_t)nsteps);
+}
+
+// this unit's copy of the QNM spline slopes (the fused step kernel's set-up)
+__global__ void k_update_qnm_init() { qnm_init_tables(threadIdx.x); }
 
 // The initial history of each column (MCMC_all, :1946-1954): row i of
 // temperature t holds the initial state of a temperature q drawn uniformly
@@ -562,12 +698,28 @@ hipError_t launch_update_propose(const double* x, const double* ej, const double
   return hipGetLastError();
 }
 
-hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                            double* y, const double* Ly, const int* st_y, int* typ, double* jac, const double* tc,
-                            const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
-                            unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s) {
-  hipLaunchKernelGGL(k_pt_update, dim3(ncols), dim3(update_threads(n_temps)), 0, s, x, L, ej, ev, hist, hcount, y, Ly, st_y, typ, jac, tc,
-                     u, n_temps, n_chains, c0, ncols, seed, step, stats, done);
+hipError_t launch_update_qnm_init(hipStream_t s) {
+  hipLaunchKernelGGL(k_update_qnm_init, dim3(1), dim3(64), 0, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_pt_steps(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
+                           double* y, int* st_y, int* typ, double* jac, const double* tc, Point* pts,
+                           const UpdateConst& u, const AnalysisConst& an, const HetArgs& h, int n_temps,
+                           int n_chains, int c0, int ncols, uint64_t seed, int nsteps, unsigned long long* step,
+                           unsigned long long* stats, unsigned int* done, hipStream_t s) {
+  if (nsteps <= 0 || ncols <= 0) return hipSuccess;
+  // 512 threads whatever n_temps: the set-up's four parts need four waves
+  const dim3 g(ncols), blk(512);
+  if (h.nd == 1)
+    hipLaunchKernelGGL(k_pt_steps<1>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
+                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
+  else if (h.nd == 2)
+    hipLaunchKernelGGL(k_pt_steps<2>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
+                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
+  else
+    hipLaunchKernelGGL(k_pt_steps<3>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
+                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
   return hipGetLastError();
 }
 

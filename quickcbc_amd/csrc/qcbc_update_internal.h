@@ -7,6 +7,8 @@
 
 #include <cstdint>
 
+#include "qcbc_types.h"
+
 namespace qcbc {
 
 // Prior box and proposal mix of one ensemble.
@@ -49,9 +51,26 @@ hipError_t launch_update_propose(const double* x, const double* ej, const double
                                  const double* tc, const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols,
                                  uint64_t seed, const unsigned long long* step, double* y, int* typ, double* jac,
                                  hipStream_t s);
-hipError_t launch_pt_update(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                            double* y, const double* Ly, const int* st_y, int* typ, double* jac, const double* tc,
-                            const UpdateConst& u, int n_temps, int n_chains, int c0, int ncols, uint64_t seed,
-                            unsigned long long* step, unsigned long long* stats, unsigned int* done, hipStream_t s);
+// fills the fused step kernel's copy of the QNM spline slopes (once per device)
+hipError_t launch_update_qnm_init(hipStream_t s);
+
+// The heterodyned likelihood's arrays (qcbc_het.cpp's grid, nd x M rows).
+struct HetArgs {
+  const double *freq, *pl, *ha, *hp, *wf;
+  int M, nd;
+  double logLb;
+};
+
+// nsteps whole PT steps of a group of chain columns in one launch (block b
+// runs column c0 + b through all of them): the set-up and heterodyned logL of
+// the pending proposals y (skipped on a swap step, whose proposals are
+// discarded), acceptance or the swap, the history and the next proposals.
+// pts: a set-up record per point of the group (scratch).
+hipError_t launch_pt_steps(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
+                           double* y, int* st_y, int* typ, double* jac, const double* tc, Point* pts,
+                           const UpdateConst& u, const AnalysisConst& an, const HetArgs& h, int n_temps,
+                           int n_chains, int c0, int ncols, uint64_t seed, int nsteps, unsigned long long* step,
+                           unsigned long long* stats, unsigned int* done, hipStream_t s);
+
 
 }  // namespace qcbc
