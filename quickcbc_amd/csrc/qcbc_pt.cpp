@@ -97,9 +97,8 @@ static int group_update_steps(qcbc_pt* pt, int k, int n) {
   const HetArgs h{hv.freq, hv.pl, hv.amp, hv.phase, hv.wf, hv.M, hv.nd, hv.logLb};
   HIPCHK(launch_pt_steps(pt->d_x + 11 * o, pt->d_L + o, pt->d_ej + 11 * o, pt->d_ev + 121 * o,
                          pt->d_hist + (size_t)gr.c0 * pt->n_temps * pt->uc.n_hist * 11, pt->d_hcount,
-                         pt->d_y + 11 * o, pt->d_st + o, pt->d_typ + o, pt->d_jac + o, pt->d_tc, gr.d_pts, pt->uc,
-                         hv.an, h, pt->n_temps, pt->n_chains, gr.c0, gr.ncols, pt->seed, n, gr.d_step, pt->d_ustats,
-                         gr.d_done, gr.s));
+                         pt->d_y + 11 * o, pt->d_typ + o, pt->d_jac + o, pt->d_tc, pt->uc, hv.an, h, pt->n_temps,
+                         pt->n_chains, gr.c0, gr.ncols, pt->seed, n, gr.d_step, pt->d_ustats, gr.d_done, gr.s));
   return QCBC_OK;
 }
 

@@ -425,27 +425,6 @@ __global__ __launch_bounds__(512) void k_update_propose(const double* __restrict
                  blockDim.x, y, typ, jac);
 }
 
-// update()'s acceptance of one proposal q (logL Lq, set-up status st, type
-// k, sky-ring log density jc) against the state xs (logL *Ls) at
-// temperature index t (:2345-2372), counted in the column's statistics cst:
-// [3 + 2 (k - 1)] proposed, [4 + 2 (k - 1)] accepted
-__device__ __forceinline__ void accept_one(double* __restrict__ xs, double* __restrict__ Ls,
-                                           const double* __restrict__ q, double Lq, int st, int k, double jc,
-                                           const double* __restrict__ tc, int t, const UpdateConst& u, uint64_t seed,
-                                           uint64_t ctr, unsigned long long* __restrict__ cst) {
-  double pMcMy;
-  const bool flag = prior_flag(q, u, &pMcMy) || st != 0;
-  atomicAdd(&cst[3 + 2 * (k - 1)], 1ULL);
-  if (!flag) {
-    const double H = jc + (Lq - *Ls) * tc[3 * t + 1] + pMcMy + 3.0 * q[6] - 3.0 * xs[6] - mass_jacobian(xs);
-    if (H > log(uni(seed, SALT_ACCEPT, ctr, 0))) {
-      for (int i = 0; i < 11; i++) xs[i] = q[i];
-      *Ls = Lq;
-      atomicAdd(&cst[4 + 2 * (k - 1)], 1ULL);
-    }
-  }
-}
-
 // On update steps with s % hist_every == 0 the column's states enter its
 // history (:2071-2080).  All threads of the block; ends synchronised.
 __device__ void column_history(const double* __restrict__ x, double* __restrict__ hist,
@@ -516,36 +495,63 @@ __device__ __forceinline__ void finish_launch(unsigned long long* __restrict__ s
 }
 
 // ---------------------------------------------------------- fused steps
-// Set-up of the column's proposals y (storage t ncols + b) for the
-// heterodyned likelihood, the parts of a point spread over waves as k_setup
-// spreads them: role (wave & 3) 0 the amplitude half of the model (and the
-// status), 3 the phase half with the time shift and reference phase, 1
-// fbegin, 2 the projection; temperature lane + 64 (wave >> 2), then every
-// 64 (blockDim / 256)-th after it.  The caller synchronises after it.
-// one part of one point's set-up (kept out of line: each part's registers are
-// its own, not added to the step kernel's)
-__device__ __noinline__ void setup_part(int role, const double* __restrict__ q, const AnalysisConst& an,
-                                        Point& c, int* __restrict__ st) {
+// temperatures per chunk of the fused step: their set-up records live in LDS
+// (one 16-lane segment of the 512-thread block per temperature)
+constexpr int PT_CHUNK = 32;
+
+// The terms of update()'s acceptance (:2345-2372) that do not need the
+// proposal's logL, for one temperature of the chunk: whether q lies outside
+// the prior, and H's other terms, each formed as H = jc + (Lq - Lx) / T
+// + pMcM(q) + 3 q[6] - 3 x[6] - pMcM(x) forms it; then log u.
+struct AcceptPre {
+  double pMcMy[PT_CHUNK], q6[PT_CHUNK], x6[PT_CHUNK], mjx[PT_CHUNK], logu[PT_CHUNK];
+  int out[PT_CHUNK];
+};
+
+// One part of one point's set-up, written to its record c: role 0 the
+// amplitude half of the model, 3 the phase half with the time shift and
+// reference phase, 1 fbegin, 2 the projection.
+__device__ __forceinline__ void setup_part(int role, const double* __restrict__ q, const AnalysisConst& an,
+                                           Point& c) {
   double prm[11];
 #pragma unroll
   for (int i = 0; i < 11; i++) prm[i] = q[i];
   if (role == 0)
-    *st = point_setup_model_half(prm, an, c, 0);
+    (void)point_setup_model_half(prm, an, c, 0);
   else if (role == 3)
-    point_setup_phase_ref(prm, an, c);
+    (void)point_setup_phase_ref(prm, an, c);
   else if (role == 1)
     c.fs = fbegin(prm);
   else
     point_setup_projection(prm, an, c);
 }
 
-__device__ void column_setup(const double* __restrict__ y, const AnalysisConst& an, Point* __restrict__ pts,
-                             int* __restrict__ st_y, int n_temps, int ncols, int b) {
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), role = wave & 3;
-  const int per = (int)(blockDim.x >> 8) * 64;
-  for (int t = (wave >> 2) * 64 + (int)(threadIdx.x & 63); t < n_temps; t += per) {
-    const int lp = t * ncols + b;
-    setup_part(role, y + 11 * lp, an, pts[lp], st_y + lp);
+// One chunk (temperatures t0 .. t0 + nt - 1) before its logL: waves 0-3 set
+// up the proposals into the LDS records spt, one part per wave as k_setup
+// spreads them, lane l the temperature t0 + l; waves 4-7 meanwhile form the
+// acceptance terms that do not need the logL, one kind per wave.  The caller
+// synchronises after it.
+__device__ void chunk_setup(const double* __restrict__ x, const double* __restrict__ y, const AnalysisConst& an,
+                            const UpdateConst& u, Point* spt, AcceptPre* pre, int t0, int nt, int n_chains, int ncols,
+                            int b, int c, uint64_t seed, uint64_t s, uint64_t P) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+  if (l >= nt) return;
+  const int t = t0 + l, lp = t * ncols + b;
+  const double* q = y + 11 * lp;
+  const double* xs = x + 11 * lp;
+  if (wave < 4) {
+    setup_part(wave, q, an, spt[l]);
+  } else if (wave == 4) {
+    double pMcMy;
+    pre->out[l] = prior_flag(q, u, &pMcMy) ? 1 : 0;
+    pre->pMcMy[l] = pMcMy;
+  } else if (wave == 5) {
+    pre->mjx[l] = mass_jacobian(xs);
+  } else if (wave == 6) {
+    pre->logu[l] = log(uni(seed, SALT_ACCEPT, s * P + (uint64_t)t * n_chains + c, 0));
+  } else {
+    pre->q6[l] = 3.0 * q[6];
+    pre->x6[l] = 3.0 * xs[6];
   }
 }
 
@@ -575,9 +581,9 @@ __device__ double het_ll_segment(const Point& c, const AnalysisConst& an, const 
 #pragma unroll
     for (int d = 0; d < ND; d++) As[d] = c.Ascale[d] * c.amp0 / an.sqT;
     for (int m = j; m < M; m += PROP_LANES) {
-      // the point's constants are loaded again per sample rather than held
-      // across the loop (the segments of a wave hold different points, so
-      // they are vector registers, ~70 doubles)
+      // the point's constants are read again per sample (LDS) rather than
+      // held in registers across the loop: the segments of a wave hold
+      // different points, so they would be vector registers, ~70 doubles
       asm volatile("" ::: "memory");
       const double f = h.freq[m];
       double a, p;
@@ -607,40 +613,54 @@ __device__ double het_ll_segment(const Point& c, const AnalysisConst& an, const 
 
 // nsteps PT steps of a group of chain columns in one launch; block b runs
 // column c0 + b through all of them (the columns are independent chains, so
-// no block waits for another).  Per step: the set-up and heterodyned logL of
-// the pending proposals, the logL evaluated by each temperature's
-// PROP_LANES-lane segment and its acceptance decided there (update steps),
-// or one swap (a swap step skips the set-up and logL: its proposals are
-// discarded unevaluated); then the history and the proposals of the next step.
+// no block waits for another).  An update step takes the temperatures in
+// chunks of PT_CHUNK: the chunk's set-up (and the acceptance terms without
+// the logL), then each temperature's 16-lane segment evaluates its
+// proposal's heterodyned logL and decides its acceptance.  A swap step makes
+// one swap (its proposals are discarded unevaluated).  Then the history and
+// the proposals of the next step.
 template <int ND>
 __global__ __launch_bounds__(512) void k_pt_steps(
     double* __restrict__ x, double* __restrict__ L, double* __restrict__ ej, double* __restrict__ ev,
     double* __restrict__ hist, unsigned long long* __restrict__ hcount, double* __restrict__ y,
-    int* __restrict__ st_y, int* __restrict__ typ, double* __restrict__ jac, const double* __restrict__ tc,
-    Point* __restrict__ pts, UpdateConst u, AnalysisConst an, HetArgs h, int n_temps, int n_chains, int c0,
-    int ncols, uint64_t seed, int nsteps, unsigned long long* __restrict__ step,
-    unsigned long long* __restrict__ stats, unsigned int* __restrict__ done) {
+    int* __restrict__ typ, double* __restrict__ jac, const double* __restrict__ tc, UpdateConst u, AnalysisConst an,
+    HetArgs h, int n_temps, int n_chains, int c0, int ncols, uint64_t seed, int nsteps,
+    unsigned long long* __restrict__ step, unsigned long long* __restrict__ stats, unsigned int* __restrict__ done) {
+  __shared__ Point spt[PT_CHUNK];
+  __shared__ AcceptPre pre;
+  __shared__ int sw;
   const int b = blockIdx.x, c = c0 + b;
   const uint64_t P = (uint64_t)n_temps * n_chains, s0 = *step;
   unsigned long long* cst = stats + (size_t)c * UPD_STATS;
-  const int j = (int)(threadIdx.x % PROP_LANES), seg = (int)(threadIdx.x / PROP_LANES);
-  const int nseg = (int)(blockDim.x / PROP_LANES);
-  __shared__ int sw;
+  const int j = (int)(threadIdx.x % PROP_LANES), seg = (int)(threadIdx.x / PROP_LANES);  // seg < PT_CHUNK
   for (int k = 0; k < nsteps; k++) {
     const uint64_t s = s0 + (uint64_t)k;
     const uint64_t cs = s * (uint64_t)n_chains + c;
     const bool swap_step = n_temps > 1 && uni(seed, SALT_SWAP, cs, 0) < 0.2;
     if (!swap_step) {
-      column_setup(y, an, pts, st_y, n_temps, ncols, b);
-      __syncthreads();
-      for (int t = seg; t < n_temps; t += nseg) {
-        const int lp = t * ncols + b;
-        const double Lq = het_ll_segment<ND>(pts[lp], an, h, j);
-        if (j == 0)
-          accept_one(x + 11 * lp, L + lp, y + 11 * lp, Lq, st_y[lp], typ[lp], jac[lp], tc, t, u, seed,
-                     s * P + (uint64_t)t * n_chains + c, cst);
+      for (int t0 = 0; t0 < n_temps; t0 += PT_CHUNK) {
+        const int nt = n_temps - t0 < PT_CHUNK ? n_temps - t0 : PT_CHUNK;
+        chunk_setup(x, y, an, u, spt, &pre, t0, nt, n_chains, ncols, b, c, seed, s, P);
+        __syncthreads();
+        if (seg < nt) {
+          const int t = t0 + seg, lp = t * ncols + b;
+          const double Lq = het_ll_segment<ND>(spt[seg], an, h, j);
+          if (j == 0) {  // the acceptance, H summed in update()'s order
+            const int kt = typ[lp];
+            atomicAdd(&cst[3 + 2 * (kt - 1)], 1ULL);
+            if (!(pre.out[seg] || spt[seg].status != ST_OK)) {
+              const double H = jac[lp] + (Lq - L[lp]) * tc[3 * t + 1] + pre.pMcMy[seg] + pre.q6[seg] -
+                               pre.x6[seg] - pre.mjx[seg];
+              if (H > pre.logu[seg]) {
+                for (int i = 0; i < 11; i++) x[11 * lp + i] = y[11 * lp + i];
+                L[lp] = Lq;
+                atomicAdd(&cst[4 + 2 * (kt - 1)], 1ULL);
+              }
+            }
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
       column_history(x, hist, hcount, u, n_temps, ncols, b, c, s);
       if (threadIdx.x == 0) cst[0] += 1;
     } else {
@@ -704,22 +724,23 @@ hipError_t launch_update_qnm_init(hipStream_t s) {
 }
 
 hipError_t launch_pt_steps(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                           double* y, int* st_y, int* typ, double* jac, const double* tc, Point* pts,
-                           const UpdateConst& u, const AnalysisConst& an, const HetArgs& h, int n_temps,
-                           int n_chains, int c0, int ncols, uint64_t seed, int nsteps, unsigned long long* step,
-                           unsigned long long* stats, unsigned int* done, hipStream_t s) {
+                           double* y, int* typ, double* jac, const double* tc, const UpdateConst& u,
+                           const AnalysisConst& an, const HetArgs& h, int n_temps, int n_chains, int c0, int ncols,
+                           uint64_t seed, int nsteps, unsigned long long* step, unsigned long long* stats,
+                           unsigned int* done, hipStream_t s) {
   if (nsteps <= 0 || ncols <= 0) return hipSuccess;
-  // 512 threads whatever n_temps: the set-up's four parts need four waves
-  const dim3 g(ncols), blk(512);
+  // 512 threads whatever n_temps: a chunk of PT_CHUNK 16-lane segments, and
+  // the set-up's four parts and the acceptance terms on eight waves
+  const dim3 g(ncols), blk(PT_CHUNK * PROP_LANES);
   if (h.nd == 1)
-    hipLaunchKernelGGL(k_pt_steps<1>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
-                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
+    hipLaunchKernelGGL(k_pt_steps<1>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, typ, jac, tc, u, an, h, n_temps,
+                       n_chains, c0, ncols, seed, nsteps, step, stats, done);
   else if (h.nd == 2)
-    hipLaunchKernelGGL(k_pt_steps<2>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
-                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
+    hipLaunchKernelGGL(k_pt_steps<2>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, typ, jac, tc, u, an, h, n_temps,
+                       n_chains, c0, ncols, seed, nsteps, step, stats, done);
   else
-    hipLaunchKernelGGL(k_pt_steps<3>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, st_y, typ, jac, tc, pts, u, an, h,
-                       n_temps, n_chains, c0, ncols, seed, nsteps, step, stats, done);
+    hipLaunchKernelGGL(k_pt_steps<3>, g, blk, 0, s, x, L, ej, ev, hist, hcount, y, typ, jac, tc, u, an, h, n_temps,
+                       n_chains, c0, ncols, seed, nsteps, step, stats, done);
   return hipGetLastError();
 }
 

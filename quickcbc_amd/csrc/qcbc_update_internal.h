@@ -65,12 +65,11 @@ struct HetArgs {
 // runs column c0 + b through all of them): the set-up and heterodyned logL of
 // the pending proposals y (skipped on a swap step, whose proposals are
 // discarded), acceptance or the swap, the history and the next proposals.
-// pts: a set-up record per point of the group (scratch).
 hipError_t launch_pt_steps(double* x, double* L, double* ej, double* ev, double* hist, unsigned long long* hcount,
-                           double* y, int* st_y, int* typ, double* jac, const double* tc, Point* pts,
-                           const UpdateConst& u, const AnalysisConst& an, const HetArgs& h, int n_temps,
-                           int n_chains, int c0, int ncols, uint64_t seed, int nsteps, unsigned long long* step,
-                           unsigned long long* stats, unsigned int* done, hipStream_t s);
+                           double* y, int* typ, double* jac, const double* tc, const UpdateConst& u,
+                           const AnalysisConst& an, const HetArgs& h, int n_temps, int n_chains, int c0, int ncols,
+                           uint64_t seed, int nsteps, unsigned long long* step, unsigned long long* stats,
+                           unsigned int* done, hipStream_t s);
 
 
 }  // namespace qcbc
