@@ -97,7 +97,10 @@ struct Ext {
 
 // Ring_all (Utilities.c:1445-1594): y's sky location rotated by 2 pi u_rot
 // about the baseline of detectors d1, d2 (the vertex vectors of Times, in s).
-__device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot) {
+// Called by all lanes of a point's segment (lane j): lane 0 takes the cosine
+// and sine of alpha - gmst and lane 1 those of the rotation angle, in one
+// evaluation, and the segment shares them.
+__device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot, int j) {
   double zv[3], uv[3], vv[3], kv[3], kvr[3];
   double zmag = 0.0;
 #pragma unroll
@@ -116,7 +119,11 @@ __device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot) {
 #pragma unroll
   for (int i = 0; i < 3; i++) zv[i] /= zmag;
   const double sind = x.p8, cosd = sqrt(1.0 - sind * sind);
-  const double calpha = cos(x.p7 - gmst), salpha = -sin(x.p7 - gmst);
+  double rot = TPI_U * u_rot;
+  const double arg = j == 1 ? rot : x.p7 - gmst;
+  const double cv = cos(arg), sv = sin(arg);
+  const double calpha = __shfl(cv, 0, 16), salpha = -__shfl(sv, 0, 16);
+  const double cr = __shfl(cv, 1, 16), sr = __shfl(sv, 1, 16);
   kv[0] = cosd * calpha;
   kv[1] = cosd * salpha;
   kv[2] = sind;
@@ -131,8 +138,6 @@ __device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot) {
   vv[0] = (zv[1] * kv[2] - zv[2] * kv[1]) / smu;
   vv[1] = (zv[2] * kv[0] - zv[0] * kv[2]) / smu;
   vv[2] = (zv[0] * kv[1] - zv[1] * kv[0]) / smu;
-  double rot = TPI_U * u_rot;
-  const double cr = cos(rot), sr = sin(rot);
 #pragma unroll
   for (int i = 0; i < 3; i++) kvr[i] = cmu * zv[i] + smu * (cr * uv[i] + sr * vv[i]);
   Ext y = x;
@@ -144,20 +149,26 @@ __device__ Ext ring_all(Ext x, int d1, int d2, double gmst, double u_rot) {
   return y;
 }
 
+// What skymap_all needs besides x's extrinsics: the antenna patterns of
+// detectors ifo1, ifo2 at x's and at y's sky location, detector ifo1's delay
+// at both, and for x the cosine and sine of phi and of 2 psi and exp(ln DL).
+struct SkyGeo {
+  double fxp1, fxc1, fxp2, fxc2, fyp1, fyc1, fyp2, fyc2, dtx, dty;
+  double cphi, sphi, c2p, s2p, eDL;
+};
+
 // skymap_all (QuickCBC.c:7085-7146) with uvwz_all (:7295-7323), uvwz_sol
 // (:7526-7544) and exsolve_all (:7325-7407): the extrinsics at y's sky
-// location that keep the signal in detectors ifo1 and ifo2
-__device__ __noinline__ Ext skymap_all(Ext x, Ext y, double gmst, int ifo1, int ifo2) {
-  double fxp1, fxc1, fxp2, fxc2, fyp1, fyc1, fyp2, fyc2;
-  antenna(0.0, x.p7, x.p8, gmst, ifo1, &fxp1, &fxc1);
-  antenna(0.0, x.p7, x.p8, gmst, ifo2, &fxp2, &fxc2);
-  antenna(0.0, y.p7, y.p8, gmst, ifo1, &fyp1, &fyc1);
-  antenna(0.0, y.p7, y.p8, gmst, ifo2, &fyp2, &fyc2);
-  const double dtx = delay(x.p7, x.p8, gmst, ifo1), dty = delay(y.p7, y.p8, gmst, ifo1);
+// location that keep the signal in detectors ifo1 and ifo2, given the
+// geometry g of x and y
+__device__ Ext skymap_core(Ext x, Ext y, const SkyGeo& g) {
+  const double fxp1 = g.fxp1, fxc1 = g.fxc1, fxp2 = g.fxp2, fxc2 = g.fxc2;
+  const double fyp1 = g.fyp1, fyc1 = g.fyc1, fyp2 = g.fyp2, fyc2 = g.fyc2;
+  const double dtx = g.dtx, dty = g.dty;
   // uvwz_all
   const double ciota = x.p10, Ap = 0.5 * (1.0 + ciota * ciota), Ac = -ciota;
-  const double ecc0 = Ac / Ap, Amp = Ap / exp(x.p6), phi = x.p4;
-  const double cphi = cos(phi), sphi = sin(phi), c2p = cos(2.0 * x.p9), s2p = sin(2.0 * x.p9);
+  const double ecc0 = Ac / Ap, Amp = Ap / g.eDL;
+  const double cphi = g.cphi, sphi = g.sphi, c2p = g.c2p, s2p = g.s2p;
   const double ux = Amp * (cphi * c2p + ecc0 * sphi * s2p);
   const double vx = Amp * (cphi * s2p - ecc0 * sphi * c2p);
   const double wx = Amp * (sphi * c2p - ecc0 * cphi * s2p);
@@ -180,14 +191,23 @@ __device__ __noinline__ Ext skymap_all(Ext x, Ext y, double gmst, int ifo1, int 
       (2.0 * rad);
   const double ecc = flag1 ? (uy * zy - vy * wy) / p : p / (uy * zy - vy * wy);
   const double ci = -(1.0 - sqrt(1.0 - ecc * ecc)) / ecc;
-  if (flag1 && flag2)
-    a = atan2((1.0 + rad) * vy + q * zy, (1.0 + rad) * uy + q * wy);
-  else if (flag1 && !flag2)
-    a = atan2(-((1.0 + rad) * vy + q * zy), -((1.0 + rad) * uy + q * wy));
-  else if (!flag1 && flag3)
-    a = atan2(ecc * ((1.0 + rad) * uy + q * wy), -ecc * ((1.0 + rad) * vy + q * zy));
-  else
-    a = atan2(-ecc * ((1.0 + rad) * uy + q * wy), ecc * ((1.0 + rad) * vy + q * zy));
+  // the four branches' arguments, then one atan2 (lanes of a segment take
+  // different branches)
+  double ya, xa;
+  if (flag1 && flag2) {
+    ya = (1.0 + rad) * vy + q * zy;
+    xa = (1.0 + rad) * uy + q * wy;
+  } else if (flag1 && !flag2) {
+    ya = -((1.0 + rad) * vy + q * zy);
+    xa = -((1.0 + rad) * uy + q * wy);
+  } else if (!flag1 && flag3) {
+    ya = ecc * ((1.0 + rad) * uy + q * wy);
+    xa = -ecc * ((1.0 + rad) * vy + q * zy);
+  } else {
+    ya = -ecc * ((1.0 + rad) * uy + q * wy);
+    xa = ecc * ((1.0 + rad) * vy + q * zy);
+  }
+  a = atan2(ya, xa);
   if (a < 0.0) a += TPI_U;
   const double Ay = flag1 ? sqrt(p) : sqrt(p) / fabs(ecc);
   y.p9 = 0.5 * a;
@@ -271,16 +291,48 @@ __device__ __noinline__ SkyOut sky_proposal(Ext x, double gmst, int n_det, int l
   }
   const int id1 = i1 == 0 ? l0 : (i1 == 1 ? l1 : l2);
   const int id2 = i2 == 0 ? l0 : (i2 == 1 ? l1 : l2);
-  const Ext yr = ring_all(x, id1, id2, gmst, uni(seed, SALT_PROPOSE, ctr, D_SKY_ROT));
+  const Ext yr = ring_all(x, id1, id2, gmst, uni(seed, SALT_PROPOSE, ctr, D_SKY_ROT), j);
   // skymap_all reads only the sky location of its second argument, so the
-  // map of x (lane 0) and the perturbed maps (lanes 1-4) run side by side
+  // map of x (lane 0) and the perturbed maps (lanes 1-4) run side by side.
+  // Their geometry is evaluated once per function across the lanes: lanes
+  // 0-3 the antenna patterns at x's (0, 1) and y's (2, 3) sky location for
+  // detectors id1 (0, 2) and id2 (1, 3), lanes 0 and 2 detector id1's delay
+  // there; lanes 4-7 cos and sin of phi (4), phi + ep (5), 2 psi (6) and
+  // 2 (psi + ep) (7); lanes 8, 9 exp of ln DL and ln DL + ep.
   constexpr double ep = 1.0e-6;
+  SkyGeo g;
+  {
+    const bool ysky = (j & 2) != 0;
+    const double al = ysky ? yr.p7 : x.p7, sd = ysky ? yr.p8 : x.p8;
+    double fp, fc;
+    antenna(0.0, al, sd, gmst, (j & 1) ? id2 : id1, &fp, &fc);
+    const double dl = delay(al, sd, gmst, id1);
+    const double ang = (j == 6 || j == 7) ? 2.0 * (j == 7 ? x.p9 + ep : x.p9) : (j == 5 ? x.p4 + ep : x.p4);
+    const double cv = cos(ang), sv = sin(ang);
+    const double ev = exp(j == 9 ? x.p6 + ep : x.p6);
+    g.fxp1 = __shfl(fp, 0, 16);
+    g.fxc1 = __shfl(fc, 0, 16);
+    g.fxp2 = __shfl(fp, 1, 16);
+    g.fxc2 = __shfl(fc, 1, 16);
+    g.fyp1 = __shfl(fp, 2, 16);
+    g.fyc1 = __shfl(fc, 2, 16);
+    g.fyp2 = __shfl(fp, 3, 16);
+    g.fyc2 = __shfl(fc, 3, 16);
+    g.dtx = __shfl(dl, 0, 16);
+    g.dty = __shfl(dl, 2, 16);
+    const int lph = j == 3 ? 5 : 4, lps = j == 2 ? 7 : 6, lex = j == 1 ? 9 : 8;
+    g.cphi = __shfl(cv, lph, 16);
+    g.sphi = __shfl(sv, lph, 16);
+    g.c2p = __shfl(cv, lps, 16);
+    g.s2p = __shfl(sv, lps, 16);
+    g.eDL = __shfl(ev, lex, 16);
+  }
   Ext xp = x;
   if (j == 1) xp.p6 += ep;
   if (j == 2) xp.p9 += ep;
   if (j == 3) xp.p4 += ep;
   if (j == 4) xp.p10 += ep;
-  const Ext ys = skymap_all(xp, yr, gmst, id1, id2);
+  const Ext ys = skymap_core(xp, yr, g);
   Ext y = yr;  // lane 0's map of x, to every lane
   y.p4 = __shfl(ys.p4, 0, 16);
   y.p5 = __shfl(ys.p5, 0, 16);
