@@ -594,17 +594,83 @@ __device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int
 //   H <- H e^{i phi_d} + y_m,  sum = e^{-i theta_d,last} H,
 // which takes one complex multiply-add per bin and detector and one sincos
 // per lane and detector at the end.
+//
+// Bins and tiles.  The block's waves walk the split in 64-bin chunks on the
+// split's own grid (chunk n holds bins j0 + 64 n + lane), so lane l of a
+// point's wave takes every bin j = j0 + l (mod 64) of the point's range, in
+// order, whatever the tiles or the other points of the block.  The planes
+// come through LDS in tiles of HT = TILE / 2 bins, two buffers: while the
+// waves read one tile, the next one arrives in the other buffer by
+// buffer-to-LDS transfers (no registers, no LDS stores), and one barrier per
+// tile hands it over.  A wave walks its point's segments (runs of bins over
+// which neither the amplitude nor the phase changes region) each in a loop
+// of its own that holds only that region pair's constants, and moves to the
+// next tile from inside that loop, so a segment's constants are set up once
+// per segment rather than once per tile.  Every wave takes every tile
+// hand-over of the block in the same order (waves whose point has no bins
+// left take the remaining hand-overs after their segments), so the barriers
+// match.
 template <int ND, int TILE>
 struct F64State {
+  static constexpr int NP = 4 + 3 * ND;
+  static constexpr int HT = TILE / 2;  // bins per LDS buffer
   CPoint* c;
-  const double* lds;  // the LDS tile
-  int t0;             // its first band bin
-  int j;              // the lane's next band bin
+  double* lds;                // two buffers of NP x HT doubles
+  const double* p64;          // the band planes from bin 0
+  int nbp;                    // plane stride (doubles)
+  int j0;                     // split start: origin of the 64-bin chunks
+  int t0;                     // first band bin of the tile being read
+  int cur;                    // its buffer
+  int bend;                   // the block's last band bin + 1
+  int wave, cpb;
+  unsigned lane16;            // 16 x lane
   double hh[ND];     // sum a^2 / S_d
   double dh0;        // sum a Re(D_0 e^{-i x0}) / S_0
   double hr[ND], hi[ND];  // Horner sums of detectors d > 0
   SinCosK sk;
 };
+
+// Buffer-to-LDS transfer of the tile at band bin t into buffer q: each
+// transfer moves 64 lanes x LB bytes of one plane; the block's waves take
+// the transfers in turn.  The whole byte offset is in the vector offset, so
+// the buffer's range check covers it (bins past the last plane read as 0).
+template <int NP, int HT>
+__device__ __forceinline__ void f64_fetch(double* lds, const double* p64, int nbp, int wave, int cpb, unsigned lane16,
+                                          int t, int q) {
+  constexpr int LB = (HT % 128 == 0) ? 16 : 4;  // bytes per lane
+  constexpr int PER = HT * 8 / (64 * LB);       // transfers per plane
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p64), (short)0, NP * nbp * 8, 0x00020000);
+  for (int k = wave; k < NP * PER; k += cpb) {
+    const int p = k / PER, h = k - p * PER;
+    const int first = p * HT + h * (64 * LB / 8);  // doubles into the buffer
+    auto dst = (__attribute__((address_space(3))) void*)(lds + q * NP * HT + first);
+    const unsigned off = 8u * (unsigned)(p * nbp + t + h * (64 * LB / 8));
+    if constexpr (LB == 16)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst, 16, lane16 + off, 0, 0, 0);
+    else
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst, 4, lane16 / 4 + off, 0, 0, 0);
+  }
+}
+
+template <int ND, int TILE>
+__device__ __forceinline__ void f64_fetch(F64State<ND, TILE>& st, int t, int q) {
+  f64_fetch<F64State<ND, TILE>::NP, F64State<ND, TILE>::HT>(st.lds, st.p64, st.nbp, st.wave, st.cpb, st.lane16, t, q);
+}
+
+// Hand-over to the next tile: this wave's transfers have landed and it is
+// done with the current tile; after the barrier so are every wave's, the
+// next tile is complete and the buffer just read can take the one after.
+template <int ND, int TILE>
+__device__ __forceinline__ void f64_advance(F64State<ND, TILE>& st) {
+  constexpr int HT = F64State<ND, TILE>::HT;
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  st.t0 += HT;
+  st.cur ^= 1;
+  if (st.t0 + HT < st.bend) f64_fetch(st, st.t0 + HT, st.cur ^ 1);
+}
 
 template <int ND, int TILE>
 struct F64Seg {
@@ -612,8 +678,9 @@ struct F64Seg {
   struct S {
     static __device__ __forceinline__ void bin(F64State<ND, TILE>& st, const double* L, const SConst<AR, PR>& c,
                                                const VConst<AR, PR>& k) {
+      constexpr int HT = F64State<ND, TILE>::HT;
       double a, r;
-      amp_turns<AR, PR>(c, k, L[0], L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
+      amp_turns<AR, PR>(c, k, L[0], L[HT], L[2 * HT], L[3 * HT], &a, &r);
       double s0, c0;
       sincos2_turns(r, &s0, &c0, st.sk);  // twice sin and cos
       // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the sums carry
@@ -622,8 +689,8 @@ struct F64Seg {
       const double a2 = a * a, uc = a * c0, us = a * s0;
 #pragma unroll
       for (int d = 0; d < ND; d++) {
-        const double Dre = L[(4 + 3 * d) * TILE], Dim = L[(5 + 3 * d) * TILE];
-        st.hh[d] = fma(a2, L[(6 + 3 * d) * TILE], st.hh[d]);
+        const double Dre = L[(4 + 3 * d) * HT], Dim = L[(5 + 3 * d) * HT];
+        st.hh[d] = fma(a2, L[(6 + 3 * d) * HT], st.hh[d]);
         if (d == 0) {
           st.dh0 = fma(Dre, uc, fma(Dim, us, st.dh0));
         } else {
@@ -635,30 +702,27 @@ struct F64Seg {
         }
       }
     }
-    // The lane walks its bins by their byte offset in the tile (one add and
-    // one compare per bin); its band index advances by the bins it took.
+    // The segment's bins [pos, end), chunk by chunk; lanes outside it idle.
     static __device__ __forceinline__ void run(F64State<ND, TILE>& st, int pos, int end) {
+      constexpr int NP = F64State<ND, TILE>::NP, HT = F64State<ND, TILE>::HT;
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
       c.template load<ND>(*st.c);
-      const int nit = (end - pos + 63) >> 6;
-      const int j0 = st.j;
-      unsigned o = 8u * (unsigned)(j0 - st.t0);
-      const unsigned oend = 8u * (unsigned)(end - st.t0);
-      for (int n = 0; n < nit; n++) {
-        if (o < oend) {
-          bin(st, reinterpret_cast<const double*>(reinterpret_cast<const char*>(st.lds) + o), c, k);
-          o += 8u * 64u;
-        }
+      const int lane = threadIdx.x & 63;
+      const unsigned span = (unsigned)(end - pos);
+      const int ce = uniform(st.j0 + 64 * ((end - 1 - st.j0) >> 6));
+      const double* Ll = st.lds + lane;  // the lane's bin of a chunk at the start of buffer 0
+      for (int jb = uniform(st.j0 + 64 * ((pos - st.j0) >> 6)); jb <= ce; jb += 64) {
+        while (jb >= st.t0 + HT) f64_advance(st);
+        if ((unsigned)(jb + lane - pos) < span) bin(st, Ll + uniform(st.cur * NP * HT + (jb - st.t0)), c, k);
       }
-      if (j0 < end) st.j = j0 + 64 * ((end - j0 + 63) >> 6);
     }
   };
 };
 
 template <int ND, int TILE>
 __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds, int* s_lohi) {
-  constexpr int NP = 4 + 3 * ND;
+  constexpr int NP = 4 + 3 * ND, HT = TILE / 2;
   const int cpb = blockDim.x >> 6;
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -666,71 +730,37 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   const int j0 = split * in.split_len, j1 = min(in.nb, j0 + in.split_len);
   const bool active = b < in.B && in.pts[b].status == ST_OK;
   int ks, ke, blo, bhi;
-  block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
+  block_range<HT>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
   F64State<ND, TILE> st;
   st.c = (CPoint*)(in.pts + uniform(active ? b : 0));
-  st.j = ks + lane;
+  st.lds = lds;
+  st.p64 = in.p64;
+  st.nbp = in.nbp;
+  st.j0 = j0;
+  st.t0 = blo;
+  st.cur = 0;
+  st.bend = bhi;
+  st.wave = wave;
+  st.cpb = uniform(cpb);
+  st.lane16 = 16u * (unsigned)lane;
   st.sk = sincos_consts2();
   st.dh0 = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
-  int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
-  if (active) {
+  if (blo < bhi) {
+    // first tile, then the second on its way while the waves read the first
+    f64_fetch(st, blo, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (blo + HT < bhi) f64_fetch(st, blo + HT, 1);
+  }
+  if (active && ks < ke) {
     CPoint& c = *st.c;
-    a1 = c.seg_a1 - in.imin;
-    a2 = c.seg_a2 - in.imin;
-    p1 = c.seg_p1 - in.imin;
-    p2 = c.seg_p2 - in.imin;
+    run_segments<F64Seg<ND, TILE>::template S>(st, ks, ke, c.seg_a1 - in.imin, c.seg_a2 - in.imin,
+                                               c.seg_p1 - in.imin, c.seg_p2 - in.imin);
   }
-  if (blockDim.x >= TILE) {
-    // One bin per thread: a thread loads its bin of the next tile as soon as
-    // its wave is done with this one, so the loads are in flight while the
-    // wave waits at the barrier for the others.  Loads and stores of a tile
-    // sit in one iteration, so their registers are never live across the
-    // bin loops.
-    auto stage = [&](int t0) {
-      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
-      double pre[NP];
-      const bool mine = (int)threadIdx.x < min(TILE, bhi - t0);
-      if (mine) {
-#pragma unroll
-        for (int p = 0; p < NP; p++) pre[p] = buffer_load_f64(rs, 8u * threadIdx.x, p * in.nbp * 8);
-      }
-      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's reads of the tile before are done
-      __builtin_amdgcn_s_barrier();                        // and every wave's
-      if (mine) {
-#pragma unroll
-        for (int p = 0; p < NP; p++) lds[p * TILE + threadIdx.x] = pre[p];
-      }
-      __syncthreads();
-    };
-    if (blo < bhi) stage(blo);
-    for (int t0 = blo; t0 < bhi; t0 += TILE) {
-      if (active) {
-        st.lds = lds;
-        st.t0 = t0;
-        pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + TILE), a1, a2, p1, p2);
-      }
-      if (t0 + TILE < bhi) stage(t0 + TILE);
-    }
-  } else {
-    for (int t0 = blo; t0 < bhi; t0 += TILE) {
-      const int tn = min(TILE, bhi - t0);
-      __syncthreads();
-      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, NP);
-      for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-        const unsigned off = 8u * i;
-#pragma unroll
-        for (int p = 0; p < NP; p++) lds[p * TILE + i] = buffer_load_f64(rs, off, p * in.nbp * 8);
-      }
-      __syncthreads();
-      if (active) {
-        st.lds = lds;
-        st.t0 = t0;
-        pos = run_segments<F64Seg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
-      }
-    }
-  }
+  while (st.t0 + HT < bhi) f64_advance(st);  // the hand-overs after this wave's last bin
   double acc[2 * ND];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
@@ -738,10 +768,12 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
     if (d > 0) {
       // back from the lane's last bin: e^{-i theta_d,last} H (zero if the lane took no bin)
       dh = 0.0;
-      if (active && st.j - 64 >= ks) {
+      int jl = j0 + 64 * ((ke - 1 - j0) >> 6) + lane;
+      if (jl >= ke) jl -= 64;
+      if (active && jl >= ks) {
         CPoint& c = *st.c;
         double s, co;
-        sincos_turns(fma(in.p64[st.j - 64], c.dt[d], c.dl[d]), &s, &co);
+        sincos_turns(fma(in.p64[jl], c.dt[d], c.dl[d]), &s, &co);
         dh = fma(st.hr[d], co, st.hi[d] * s);
       }
     }
