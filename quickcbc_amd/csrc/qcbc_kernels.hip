@@ -46,6 +46,67 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 constexpr int SUM_LANE = 63;
 
+// Lane exchanges of a double across the halves (permlane32) or across
+// neighbouring rows of 16 lanes (permlane16) of two registers, gfx950's
+// v_permlane*_swap: after swap_halves, x holds the lower halves of x and y
+// and y their upper halves, so x + y has x's half-folded sums in lanes 0-31
+// and y's in lanes 32-63; after swap_rows, x holds rows 0 and 2 of x and y
+// and y their rows 1 and 3, so x + y holds x's row-pair sums in rows 0 and 2
+// and y's in rows 1 and 3.
+__device__ __forceinline__ void swap_halves(double& x, double& y) {
+  const unsigned long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ux, (unsigned)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32), false, false);
+  x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void swap_rows(double& x, double& y) {
+  const unsigned long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ux, (unsigned)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32), false, false);
+  x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+// Sum over each row of 16 lanes, in the row's last lane.
+__device__ __forceinline__ double row_sum(double v) {
+  v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  return v;
+}
+
+// The wave totals of the 2 nd per-lane values v[0..2nd) written to out[0..2nd)
+// (when write): the values are folded pairwise, halves then rows, so that
+// one row sum finishes four totals at once.  Value q's total lands in the
+// last lane of row rho(q): for nd >= 2, v[0..3] in rows 0, 2, 1, 3; the last
+// pair (v[4], v[5] at nd = 3, v[0], v[1] at nd = 1) in rows 0 and 2.  Every
+// lane must be active.
+template <int ND>
+__device__ __forceinline__ void wave_sums_store(double (&v)[2 * ND], double* out, bool write) {
+  const int lane = threadIdx.x & 63, row = lane >> 4;
+  double t = 0.0, u = 0.0;
+  if (ND >= 2) {
+    double a0 = v[0], a1 = v[1], b0 = v[2], b1 = v[3];
+    swap_halves(a0, a1);
+    swap_halves(b0, b1);
+    double a = a0 + a1, b = b0 + b1;  // (v0 | v1), (v2 | v3) by halves
+    swap_rows(a, b);
+    t = row_sum(a + b);               // rows: v0, v2, v1, v3
+  }
+  if (ND != 2) {
+    double c0 = v[2 * ND - 2], c1 = v[2 * ND - 1];
+    swap_halves(c0, c1);
+    double c = c0 + c1, cc = c;
+    swap_rows(c, cc);
+    u = row_sum(c + cc);              // rows: last pair's first, same, second, same
+  }
+  if (write && (lane & 15) == 15) {
+    if (ND >= 2) out[(row & 1) * 2 + (row >> 1)] = t;
+    if (ND != 2 && (row & 1) == 0) out[2 * ND - 2 + (row >> 1)] = u;
+  }
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Hides a constant from the optimiser in a scalar register, so that an FMA
@@ -80,6 +141,7 @@ __device__ __forceinline__ void fma_acc_z(double& acc, double x, double y) {  //
 // of copying one of them every bin.
 struct SinCosK {
   double p1, q1;
+  double p2, p3, q2, q3, q4;  // sincos2_turns only: its next coefficients, taken from vector registers too
 };
 
 __device__ __forceinline__ double vreg_v(double v) {
@@ -131,15 +193,15 @@ __device__ __forceinline__ void sincos2_turns(double r, double* s2, double* c2, 
   const double t = r - rint(r);
   const double z = t * t;
   double p = fma(z, sreg(0x1.e3f38399551bfp-24), k2.p1);
-  p = fma(p, z, -sreg(-0x1.50782fda12d96p-12));
-  p = fma(p, z, -sreg(0x1.32d2cce2e5b19p-7));
+  p = fma(p, z, k2.p2);
+  p = fma(p, z, k2.p3);
   p = fma(p, z, -sreg(-0x1.466bc677587f8p-3));
   p = fma(p, z, -sreg(0x1.4abbce625be41p+0));
   p = fma(p, z, -sreg(-0x1.921fb54442d18p+1));
   double q = fma(z, sreg(0x1.f3dbcea61b1a4p-21), k2.q1);
-  q = fma(q, z, -sreg(-0x1.e1f4fb60281f6p-10));
-  q = fma(q, z, -sreg(0x1.55d3c7dbfd139p-5));
-  q = fma(q, z, -sreg(-0x1.03c1f081b0780p-1));
+  q = fma(q, z, k2.q2);
+  q = fma(q, z, k2.q3);
+  q = fma(q, z, k2.q4);
   q = fma(q, z, -sreg(0x1.3bd3cc9be458bp+1));
   q = fma(q, z, 2.0);
   double sn = t * p, cs = q;
@@ -154,7 +216,10 @@ __device__ __forceinline__ void sincos2_turns(double r, double* s2, double* c2, 
 }
 
 __device__ __forceinline__ SinCosK sincos_consts2() {
-  return SinCosK{vreg_v(-0x1.e30071afc3e59p-18), vreg_v(-0x1.a6c9c1be9eb49p-15)};
+  return SinCosK{vreg_v(-0x1.e30071afc3e59p-18), vreg_v(-0x1.a6c9c1be9eb49p-15),
+                 vreg_v(0x1.50782fda12d96p-12),   vreg_v(-0x1.32d2cce2e5b19p-7),
+                 vreg_v(0x1.e1f4fb60281f6p-10),   vreg_v(-0x1.55d3c7dbfd139p-5),
+                 vreg_v(0x1.03c1f081b0780p-1)};
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
@@ -777,13 +842,10 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
         dh = fma(st.hr[d], co, st.hi[d] * s);
       }
     }
-    acc[2 * d] = wave_sum(st.hh[d]);
-    acc[2 * d + 1] = wave_sum(0.5 * dh);  // the bins' sums carry twice the products (exact)
+    acc[2 * d] = st.hh[d];
+    acc[2 * d + 1] = 0.5 * dh;  // the bins' sums carry twice the products (exact)
   }
-  if (lane == SUM_LANE && b < in.B) {
-#pragma unroll
-    for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
-  }
+  wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }
 
 // ---- mixed precision.  FP64: the powers of Mf and the projected argument
