@@ -225,16 +225,16 @@ def serving(seconds=60.0, rate=20.0, budget_gb=200.0, lifetime_steps=20, precisi
 # its source (qcbc_kernels.hip: amp_turns, sincos_turns, F64Seg; an FMA
 # counts 2, exp, atan, sqrt and a division 1 each).  The waveform part by
 # (amplitude region, phase region), 0 inspiral, 1 intermediate, 2
-# merger-ringdown: Mf^(-1/6), Mf^(-1/3), Mf^(-2/3) and the f tshift_0 term
-# (5), Mf^(1/3) (2, inspiral), Mf = Msec f and 1/Mf (1 each, intermediate and
-# merger), and each region's amplitude with its Mf^(-7/6) scale and phase:
-_FUSED_AMP = (17, 13, 9)     # Horner in Mf^(1/3) and w^2; Newton form; exp / Lorentzian
-_FUSED_PHASE = (24, 6, 15)   # Horner in Mf^(+-1/3) + log terms; + log, 1/Mf^3; + sqrt, atan (the Mf-linear
+# merger-ringdown: f^(-1/3), f^(-2/3) and the f tshift_0 term (4), f^(-1)
+# (1, intermediate), Mf = Msec f, Mf^(-1/6), Mf^(-1/3) and 1/Mf (5, merger),
+# and each region's amplitude with its f^(-7/6) or Mf^(-7/6) scale and phase:
+_FUSED_AMP = (17, 13, 9)     # Horner in f^(1/3) and f^(-2/3); Newton form in f; exp / Lorentzian
+_FUSED_PHASE = (24, 6, 15)   # Horner in f^(+-1/3) + log terms; + log, f^(-3); + sqrt, atan (the Mf-linear
                              # term rides in the f coefficient's FMA)
 
 
 def _fused_waveform_flops(a, p):
-    return (5 + (2 if a == 0 or p == 0 else 0) + (2 if a or p else 0)
+    return (4 + (1 if a == 1 or p == 1 else 0) + (5 if a == 2 or p == 2 else 0)
             + _FUSED_AMP[a] + _FUSED_PHASE[p])
 
 

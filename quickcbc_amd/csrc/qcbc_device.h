@@ -849,25 +849,32 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   const double K = -INV_TPI;
   const double G = (LOG_PI + c.logMsec) / 3.0;  // log v = G + log(f) / 3
   const double base = (c.lambda[0] + c.phi_precalc - c.t0 * c.MfRef) * INV_TPI;
+  // Powers of Mf in bin units: Mf^(1/3) = s u and Mf^(-1/3) = sg^2 E with
+  // u = f^(1/3), E = f^(-1/3) = e^2, e = f^(-1/6) (the band planes), s =
+  // Msec^(1/3), sg = Msec^(-1/6); each coefficient below carries its power
+  // of s or sg, so the kernels evaluate the polynomials in u and E directly.
+  const double sg = c.is6_scale, sg2 = sg * sg, sg3 = sg2 * sg, sg4 = sg2 * sg2, sg6 = sg3 * sg3;
+  const double sg7 = sg6 * sg, sg10 = sg6 * sg4, sg18 = sg6 * sg6 * sg6;
+  const double s1 = c.v_scale, s2 = s1 * s1, s4 = s2 * s2, s5 = s4 * s1, s6 = s4 * s2;
   // inspiral (PhiInsAnsatzInt)
   c.ins_q0 = base - (c.pi_init + c.pi_L * G) * INV_TPI;
   c.ins_q1 = (c.t0 - c.pi_1 * c.inv_eta) * INV_TPI;
-  c.ins_q13 = K * (c.pi_13 + c.pi_13L * G);
-  c.ins_q13L = K * c.pi_13L / 3.0;
+  c.ins_q13 = K * (c.pi_13 + c.pi_13L * G) * s1;
+  c.ins_q13L = K * c.pi_13L / 3.0 * s1;
   c.ins_qL = K * c.pi_L / 3.0;
-  c.ins_q23 = K * c.pi_23;
-  c.ins_q43 = K * c.inv_eta * c.pi_43;
-  c.ins_q53 = K * c.inv_eta * c.pi_53;
-  c.ins_q2 = K * c.inv_eta * c.pi_2;
-  c.ins_qm13 = K * c.pi_m13;
-  c.ins_qm23 = K * c.pi_m23;
-  c.ins_qm1 = K * c.pi_m1;
-  c.ins_qm53 = K * c.pi_m53;
+  c.ins_q23 = K * c.pi_23 * s2;
+  c.ins_q43 = K * c.inv_eta * c.pi_43 * s4;
+  c.ins_q53 = K * c.inv_eta * c.pi_53 * s5;
+  c.ins_q2 = K * c.inv_eta * c.pi_2 * s6;
+  c.ins_qm13 = K * c.pi_m13 * sg2;
+  c.ins_qm23 = K * c.pi_m23 * sg4;
+  c.ins_qm1 = K * c.pi_m1 * sg6;
+  c.ins_qm53 = K * c.pi_m53 * sg10;
   // intermediate: inv_eta (beta1 Mf - beta3 / (3 Mf^3) + beta2 log Mf) + C1Int + C2Int Mf
   c.int_qL = K * c.inv_eta * c.beta2;
   c.int_q0 = base - c.C1Int * INV_TPI + c.int_qL * c.logMsec;
   c.int_q1 = (c.t0 - c.C2Int - c.inv_eta * c.beta1) * INV_TPI;
-  c.int_qm3 = c.inv_eta * c.beta3 * INV_TPI / 3.0;
+  c.int_qm3 = c.inv_eta * c.beta3 * INV_TPI / 3.0 * sg18;
   // merger-ringdown: inv_eta (-alpha2 / Mf + 4/3 alpha3 Mf^(3/4) + alpha1 Mf
   //                  + alpha4 atan((Mf - a5fRD) / fDM)) + C1MRD + C2MRD Mf
   c.mrd_q0 = base - c.C1MRD * INV_TPI;
@@ -877,21 +884,29 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   c.mrd_qat = K * c.inv_eta * c.alpha4;
   c.mrd_a5 = c.a5fRD;
   c.mrd_ifDM = 1.0 / c.fDM;
-  // amplitude, amp_pre folded in
-  c.ia0 = c.amp_pre;
-  c.ia23 = c.amp_pre * c.ai_23;
-  c.ia1 = c.amp_pre * c.ai_1;
-  c.ia43 = c.amp_pre * c.ai_43;
-  c.ia53 = c.amp_pre * c.ai_53;
-  c.ia2 = c.amp_pre * c.ai_2;
-  c.ia73 = c.amp_pre * c.ai_73;
-  c.ia83 = c.amp_pre * c.ai_83;
-  c.ia3 = c.amp_pre * c.ai_3;
-  c.ib0 = c.amp_pre * c.col_c0;
-  c.ib1 = c.amp_pre * c.col_c1;
-  c.ib2 = c.amp_pre * c.col_c2;
-  c.ib3 = c.amp_pre * c.col_c3;
-  c.ib4 = c.amp_pre * c.col_c4;
+  // amplitude, amp_pre and the bin-unit powers folded in: inspiral
+  // Mf^(-7/6) amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3) = e [E (ia0 E^2 +
+  // ia23) + ia1 + ia43 u + ia53 u^2 + ia2 u^3 + ia73 u^4 + ia83 u^5 + ia3 u^6];
+  // intermediate Mf^(-7/6) amp_pre N(Mf) = e E^3 N'(f) with the Newton form
+  // N' on the nodes col_f* in Hz
+  c.ia0 = c.amp_pre * sg7;
+  c.ia23 = c.amp_pre * c.ai_23 * sg3;
+  c.ia1 = c.amp_pre * c.ai_1 * sg;
+  c.ia43 = c.amp_pre * c.ai_43 * sg * s1;
+  c.ia53 = c.amp_pre * c.ai_53 * sg * s2;
+  c.ia2 = c.amp_pre * c.ai_2 * sg * (s2 * s1);
+  c.ia73 = c.amp_pre * c.ai_73 * sg * s4;
+  c.ia83 = c.amp_pre * c.ai_83 * sg * s5;
+  c.ia3 = c.amp_pre * c.ai_3 * sg * s6;
+  const double M = c.Msec;
+  c.ib0 = c.amp_pre * c.col_c0 * sg7;
+  c.ib1 = c.amp_pre * c.col_c1 * sg7 * M;
+  c.ib2 = c.amp_pre * c.col_c2 * sg7 * (M * M);
+  c.ib3 = c.amp_pre * c.col_c3 * sg7 * (M * M * M);
+  c.ib4 = c.amp_pre * c.col_c4 * sg7 * ((M * M) * (M * M));
+  c.nf1 = c.col_f1 / M;
+  c.nf2 = c.col_f2 / M;
+  c.nf3 = c.col_f3 / M;
   c.mnum = c.amp_pre * c.mrd_num;
   c.mg2 = c.fDMg3 * c.fDMg3;
   for (int d = 0; d < 3; d++) {

@@ -408,9 +408,9 @@ __device__ __forceinline__ double spin(double v) {
 // Field names follow Point, so amp_turns reads either.
 template <int AR, int PR>
 struct SConst {
-  double Msec, v_scale, is6_scale, tshift[1];
+  double Msec, is6_scale, tshift[1];
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia3;
-  double col_f1, col_f2, col_f3, ib0, ib1, ib2, ib4;
+  double nf1, nf2, nf3, ib0, ib1, ib2, ib4;
   double fRD, mrd_rate, mnum, mg2;
   double ins_q0, ins_q1, ins_q13, ins_q13L, ins_q23, ins_q43, ins_q2, ins_qm13, ins_qm23, ins_qm1, ins_qm53;
   double int_q0, int_q1, int_qL, int_qm3;
@@ -421,16 +421,15 @@ struct SConst {
   // detector phase offsets (dt, dl) for the mixed one
   template <int ND, bool ROT = true, class PT>
   __device__ __forceinline__ void load(const PT& c) {
-    Msec = spin(c.Msec);
+    if (AR == 2 || PR == 2) Msec = spin(c.Msec);
     tshift[0] = spin(c.tshift[0]);
     tsq = spin(fma(PR == 0 ? c.ins_q1 : PR == 1 ? c.int_q1 : c.mrd_q1, c.Msec, c.tshift[0]));
-    if (AR == 0 || PR == 0) v_scale = spin(c.v_scale);
-    is6_scale = spin(c.is6_scale);
+    if (AR == 2 || PR == 2) is6_scale = spin(c.is6_scale);
     if (AR == 0) {
       ia0 = spin(c.ia0); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
       ia53 = spin(c.ia53); ia2 = spin(c.ia2); ia73 = spin(c.ia73); ia3 = spin(c.ia3);
     } else if (AR == 1) {
-      col_f1 = spin(c.col_f1); col_f2 = spin(c.col_f2); col_f3 = spin(c.col_f3);
+      nf1 = spin(c.nf1); nf2 = spin(c.nf2); nf3 = spin(c.nf3);
       ib0 = spin(c.ib0); ib1 = spin(c.ib1); ib2 = spin(c.ib2); ib4 = spin(c.ib4);
     } else {
       fRD = spin(c.fRD); mrd_rate = spin(c.mrd_rate); mnum = spin(c.mnum); mg2 = spin(c.mg2);
@@ -458,12 +457,14 @@ struct SConst {
   }
 };
 
-// Amplitude without the detector scale, and the projected detector-0
-// argument in turns, of one bin in amplitude region AR and phase region PR
-// (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
-// constants of the point and the bin's f, f^(1/3), f^(-1/6), log f.  The
-// inspiral pieces are polynomials in v = Mf^(1/3) and w = 1/v (Horner), so
-// they need no other power of Mf.
+// amp_turns (below): amplitude without the detector scale, and the projected
+// detector-0 argument in turns, of one bin in amplitude region AR and phase
+// region PR (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
+// constants of the point and the bin's f, u = f^(1/3), e = f^(-1/6), log f.
+// The inspiral and intermediate pieces are polynomials in u and E = e^2 =
+// f^(-1/3) (Horner) and a Newton form in f, with the point's powers of
+// Msec folded into their coefficients, so they need no power of Mf; the
+// merger-ringdown pieces work in Mf = Msec f.
 // The amplitude constants of region AR in FP32, for the mixed kernel: the
 // inspiral Horner coefficients, the intermediate Newton coefficients, or the
 // merger-ringdown Lorentzian's numerator, exponent rate (in base 2, for
@@ -487,63 +488,63 @@ struct AmpF {
 };
 
 template <int AR, int PR, class K>
-__device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double v3f,
-                                          double is6f, double logf, double* a, double* r) {
-  const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
-  const double w = i_s6 * i_s6;            // Mf^(-1/3)
-  const double w2 = w * w;
-  const double w3 = w2 * w;                // 1 / Mf (unused by the inspiral-inspiral pair)
-  double Mf = 0.0, v = 0.0;
-  if (AR > 0 || PR > 0) {
+__device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double u,
+                                          double e, double logf, double* a, double* r) {
+  // powers of the bin's f: u = f^(1/3), e = f^(-1/6) from the planes, E = f^(-1/3)
+  const double E = e * e;
+  const double E2 = E * E;
+  double E3 = 0.0;
+  if (AR == 1 || PR == 1) E3 = E2 * E;
+  // the merger-ringdown pieces work in Mf and its powers
+  double Mf = 0.0, i_s6 = 0.0, w = 0.0, w3 = 0.0;
+  if (AR == 2 || PR == 2) {
     Mf = c.Msec * f;
     vreg(Mf);  // Mf - node stays one subtraction with the node in a scalar register
+    i_s6 = c.is6_scale * e;  // Mf^(-1/6)
+    w = i_s6 * i_s6;         // Mf^(-1/3)
+    w3 = (w * w) * w;        // 1 / Mf
   }
-  if (AR == 0 || PR == 0) v = c.v_scale * v3f;  // Mf^(1/3)
-  double A;
   if (AR == 0) {
-    // Mf^(-7/6) amp_pre (1 + ai_23 v^2 + ai_1 v^3 + ... + ai_3 v^9) with
-    // Mf^(-7/6) = Mf^(-1/6) w^3: Mf^(-1/6) [w (ia0 w^2 + ia23) + ia1 + ia43 v
-    // + ... + ia3 v^6], the v part by Horner on the negated partial sums
-    double np = fma(v, -c.ia3, -k.a0);
-    np = hstep(np, v, c.ia73);
-    np = hstep(np, v, c.ia2);
-    np = hstep(np, v, c.ia53);
-    np = hstep(np, v, c.ia43);
-    np = hstep(np, v, c.ia1);
-    *a = i_s6 * fma(w, fma(w2, c.ia0, k.a1), -np);
+    // e [E (ia0 E^2 + ia23) + ia1 + ia43 u + ... + ia3 u^6], the u part by
+    // Horner on the negated partial sums
+    double np = fma(u, -c.ia3, -k.a0);
+    np = hstep(np, u, c.ia73);
+    np = hstep(np, u, c.ia2);
+    np = hstep(np, u, c.ia53);
+    np = hstep(np, u, c.ia43);
+    np = hstep(np, u, c.ia1);
+    *a = e * fma(E, fma(E2, c.ia0, k.a1), -np);
   } else if (AR == 1) {
-    // Newton form on the nodes (f1, f1, f2, f3), Horner on negated partials
-    const double x1 = Mf - c.col_f1;
-    double np = fma(Mf - c.col_f3, -c.ib4, -k.a0);
-    np = hstep(np, Mf - c.col_f2, c.ib2);
+    // e E^3 times the Newton form in f on the nodes (f1, f1, f2, f3), Horner on negated partials
+    const double x1 = f - c.nf1;
+    double np = fma(f - c.nf3, -c.ib4, -k.a0);
+    np = hstep(np, f - c.nf2, c.ib2);
     np = hstep(np, x1, c.ib1);
-    A = -hstep(np, x1, c.ib0);
+    *a = (e * E3) * -hstep(np, x1, c.ib0);
   } else {
     const double df = Mf - c.fRD;
-    A = (c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2));
+    *a = (w3 * i_s6) * ((c.mnum * exp_mrd(-df * c.mrd_rate)) * rcp_pos(fma(df, df, c.mg2)));  // Mf^(-7/6)
   }
-  if (AR > 0) *a = (w3 * i_s6) * A;  // Mf^(-7/6)
   // the projected argument: f tsq (tshift_0 plus the region's Mf-linear
   // coefficient times Msec, per point) in the last FMA, plus the rest of the
-  // region's function of Mf
+  // region's function of f
   double g;
   if (PR == 0) {
-    // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
-    // + (qL + q13L v) log f
-    double np = fma(v, -c.ins_q2, -k.p0);
-    np = hstep(np, v, c.ins_q43);
-    np = np * v;  // no v^3 term: q1 Mf is added below
-    np = hstep(np, v, c.ins_q23);
-    np = hstep(np, v, c.ins_q13);  // -(q13 + q23 v + q43 v^3 + q53 v^4 + q2 v^5)
-    double nn = -(w * c.ins_qm53);
-    nn = hstep(nn, w, c.ins_qm1);
-    nn = hstep(nn, w, c.ins_qm23);
-    nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
-    const double lg = fma(v, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, lg);  // q1 Mf is in tsq
+    // q0 + sum_k q_k u^k (k = 1, 2, 4, 5, 6) + sum_k q_-k E^k (k = 1, 2, 3, 5)
+    // + (qL + q13L u) log f
+    double np = fma(u, -c.ins_q2, -k.p0);
+    np = hstep(np, u, c.ins_q43);
+    np = np * u;  // no u^3 term: q1 Mf is added below
+    np = hstep(np, u, c.ins_q23);
+    np = hstep(np, u, c.ins_q13);  // -(q13 + q23 u + q43 u^3 + q53 u^4 + q2 u^5)
+    double nn = -(E * c.ins_qm53);
+    nn = hstep(nn, E, c.ins_qm1);
+    nn = hstep(nn, E, c.ins_qm23);
+    nn = hstep(nn, E, c.ins_qm13);  // -(qm13 + qm23 E + qm1 E^2 + qm53 E^4)
+    const double lg = fma(u, c.ins_q13L, k.pL) * logf;
+    g = -hstep(nn, E, c.ins_q0) + fma(-np, u, lg);  // q1 Mf is in tsq
   } else if (PR == 1) {
-    const double w9 = w3 * w3 * w3;  // 1 / Mf^3
-    g = fma(c.int_qm3, w9, c.int_qL * logf) + c.int_q0;
+    g = fma(c.int_qm3, (E3 * E3) * E3, c.int_qL * logf) + c.int_q0;  // qm3 E^9 = qm3' / Mf^3
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
@@ -558,61 +559,55 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
 // reciprocal are the hardware's v_exp_f32 and v_rcp_f32.
 template <int AR, int PR, class K>
 __device__ __forceinline__ void amp32_turns(const K& c, const VConst<AR, PR>& k, const AmpF<AR>& cf, double f,
-                                            double v3f, double is6f, double logf, float* a, double* r) {
-  const double i_s6 = c.is6_scale * is6f;  // Mf^(-1/6)
-  const double w = i_s6 * i_s6;            // Mf^(-1/3)
-  const double w2 = w * w;
-  const double w3 = w2 * w;                // 1 / Mf (unused by the inspiral-inspiral pair)
-  double Mf = 0.0, v = 0.0;
-  if (AR > 0 || PR > 0) {
+                                            double u, double e, double logf, float* a, double* r) {
+  const double E = e * e;  // f^(-1/3)
+  const double E2 = E * E;
+  double E3 = 0.0;
+  if (AR == 1 || PR == 1) E3 = E2 * E;
+  double Mf = 0.0, i_s6 = 0.0, w = 0.0, w3 = 0.0;
+  if (AR == 2 || PR == 2) {
     Mf = c.Msec * f;
-    vreg(Mf);  // Mf - node stays one subtraction with the node in a scalar register
+    vreg(Mf);
+    i_s6 = c.is6_scale * e;  // Mf^(-1/6)
+    w = i_s6 * i_s6;         // Mf^(-1/3)
+    w3 = (w * w) * w;        // 1 / Mf
   }
-  if (AR == 0 || PR == 0) v = c.v_scale * v3f;  // Mf^(1/3)
   if (AR == 0) {
-    const float vf = (float)v, wf = (float)w;
-    float np = fmaf(vf, -cf.c[8], -cf.c[2]);
-    np = fmaf(np, vf, -cf.c[7]);
-    np = fmaf(np, vf, -cf.c[6]);
-    np = fmaf(np, vf, -cf.c[5]);
-    np = fmaf(np, vf, -cf.c[4]);
-    np = fmaf(np, vf, -cf.c[3]);
-    *a = (float)i_s6 * fmaf(wf, fmaf((float)w2, cf.c[0], cf.c[1]), -np);
+    const float uf = (float)u, Ef = (float)E;
+    float np = fmaf(uf, -cf.c[8], -cf.c[2]);
+    np = fmaf(np, uf, -cf.c[7]);
+    np = fmaf(np, uf, -cf.c[6]);
+    np = fmaf(np, uf, -cf.c[5]);
+    np = fmaf(np, uf, -cf.c[4]);
+    np = fmaf(np, uf, -cf.c[3]);
+    *a = (float)e * fmaf(Ef, fmaf((float)E2, cf.c[0], cf.c[1]), -np);
+  } else if (AR == 1) {
+    const float x1 = (float)(f - c.nf1);
+    float np = fmaf((float)(f - c.nf3), -cf.c[4], -cf.c[3]);
+    np = fmaf(np, (float)(f - c.nf2), -cf.c[2]);
+    np = fmaf(np, x1, -cf.c[1]);
+    *a = (float)(e * E3) * -fmaf(np, x1, -cf.c[0]);
   } else {
-    float A;
-    if (AR == 1) {
-      const float x1 = (float)(Mf - c.col_f1);
-      float np = fmaf((float)(Mf - c.col_f3), -cf.c[4], -cf.c[3]);
-      np = fmaf(np, (float)(Mf - c.col_f2), -cf.c[2]);
-      np = fmaf(np, x1, -cf.c[1]);
-      A = -fmaf(np, x1, -cf.c[0]);
-    } else {
-      const float df = (float)(Mf - c.fRD);
-      A = (cf.c[0] * __builtin_amdgcn_exp2f(-df * cf.c[1])) * __builtin_amdgcn_rcpf(fmaf(df, df, cf.c[2]));
-    }
+    const float df = (float)(Mf - c.fRD);
+    const float A = (cf.c[0] * __builtin_amdgcn_exp2f(-df * cf.c[1])) * __builtin_amdgcn_rcpf(fmaf(df, df, cf.c[2]));
     *a = (float)(w3 * i_s6) * A;  // Mf^(-7/6)
   }
-  // the projected argument: f tsq (tshift_0 plus the region's Mf-linear
-  // coefficient times Msec, per point) in the last FMA, plus the rest of the
-  // region's function of Mf
+  // the projected argument, in FP64 as amp_turns forms it
   double g;
   if (PR == 0) {
-    // q0 + q1 Mf + sum_k q_k v^k (k = 1, 2, 4, 5, 6) + sum_k q_-k w^k (k = 1, 2, 3, 5)
-    // + (qL + q13L v) log f
-    double np = fma(v, -c.ins_q2, -k.p0);
-    np = hstep(np, v, c.ins_q43);
-    np = np * v;  // no v^3 term: q1 Mf is added below
-    np = hstep(np, v, c.ins_q23);
-    np = hstep(np, v, c.ins_q13);  // -(q13 + q23 v + q43 v^3 + q53 v^4 + q2 v^5)
-    double nn = -(w * c.ins_qm53);
-    nn = hstep(nn, w, c.ins_qm1);
-    nn = hstep(nn, w, c.ins_qm23);
-    nn = hstep(nn, w, c.ins_qm13);  // -(qm13 + qm23 w + qm1 w^2 + qm53 w^4)
-    const double lg = fma(v, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, w, c.ins_q0) + fma(-np, v, lg);  // q1 Mf is in tsq
+    double np = fma(u, -c.ins_q2, -k.p0);
+    np = hstep(np, u, c.ins_q43);
+    np = np * u;  // no u^3 term: q1 Mf is added below
+    np = hstep(np, u, c.ins_q23);
+    np = hstep(np, u, c.ins_q13);
+    double nn = -(E * c.ins_qm53);
+    nn = hstep(nn, E, c.ins_qm1);
+    nn = hstep(nn, E, c.ins_qm23);
+    nn = hstep(nn, E, c.ins_qm13);
+    const double lg = fma(u, c.ins_q13L, k.pL) * logf;
+    g = -hstep(nn, E, c.ins_q0) + fma(-np, u, lg);  // q1 Mf is in tsq
   } else if (PR == 1) {
-    const double w9 = w3 * w3 * w3;  // 1 / Mf^3
-    g = fma(c.int_qm3, w9, c.int_qL * logf) + c.int_q0;
+    g = fma(c.int_qm3, (E3 * E3) * E3, c.int_qL * logf) + c.int_q0;
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);

@@ -31,23 +31,26 @@ struct alignas(16) Point {
   double tshift[3];
   double wc[3], ws[3];
   // Projected detector-0 argument in turns, f tshift_0 + (lambda_0 - Psi)/2 pi,
-  // is f tshift_0 plus a function of Mf per phase region:
-  //   inspiral:      q0 + q1 Mf + Mf^(1/3) (q13 + q13L log f) + qL log f + q23 Mf^(2/3)
-  //                  + q43 Mf^(4/3) + q53 Mf^(5/3) + q2 Mf^2 + qm13 Mf^(-1/3)
-  //                  + qm23 Mf^(-2/3) + qm1 / Mf + qm53 Mf^(-5/3)
-  //   intermediate:  q0 + q1 Mf + qL log f + qm3 / Mf^3
+  // is f tshift_0 plus a function of f per phase region, in the bin's
+  // u = f^(1/3) and E = f^(-1/3) (Mf^(1/3) = Msec^(1/3) u, Mf^(-1/3) =
+  // Msec^(-1/3) E: the coefficients carry those powers):
+  //   inspiral:      q0 + q1 Mf + u (q13 + q13L log f) + qL log f + q23 u^2
+  //                  + q43 u^4 + q53 u^5 + q2 u^6 + qm13 E + qm23 E^2 + qm1 E^3 + qm53 E^5
+  //   intermediate:  q0 + q1 Mf + qL log f + qm3 E^9
   //   merger:        q0 + q1 Mf + qm1 / Mf + q34 Mf^(3/4) + qat atan((Mf - a5) ifDM)
   double ins_q0, ins_q1, ins_q13, ins_q13L, ins_qL, ins_q23, ins_q43, ins_q53, ins_q2;
   double ins_qm13, ins_qm23, ins_qm1, ins_qm53;
-  // amplitude without Ascale: Mf^(-7/6) times
-  //   inspiral:      amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3)   (coefficients ia*)
-  //   intermediate:  amp_pre x the Newton form (coefficients ib*, nodes col_f*)
-  //   merger:        mnum exp(-(Mf - fRD) mrd_rate) / ((Mf - fRD)^2 + fDMg3^2)
+  // amplitude without Ascale, with e = f^(-1/6):
+  //   inspiral:      e [E (ia0 E^2 + ia23) + ia1 + ia43 u + ia53 u^2 + ia2 u^3 + ia73 u^4 + ia83 u^5 + ia3 u^6]
+  //                  (Mf^(-7/6) amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3))
+  //   intermediate:  e E^3 x the Newton form in f (coefficients ib*, nodes nf*)
+  //   merger:        Mf^(-7/6) mnum exp(-(Mf - fRD) mrd_rate) / ((Mf - fRD)^2 + fDMg3^2)
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia83, ia3;
   double int_q0, int_q1, int_qL, int_qm3;
-  // intermediate amplitude in Newton form on nodes (f1, f1, f2, f3)
+  // intermediate amplitude in Newton form on nodes (f1, f1, f2, f3) in Mf
   double col_f1, col_f2, col_f3;
   double ib0, ib1, ib2, ib3, ib4;
+  double nf1, nf2, nf3;  // the nodes in Hz (col_f / Msec)
   double fRD, mrd_rate, mnum, mg2;  // mrd_rate = g2/(fDM g3)
   double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
   // detector d relative to detector 0: f dt_d + dl_d turns
@@ -70,7 +73,6 @@ struct alignas(16) Point {
   // projection per detector slot
   double Fs[3], lambda[3], Ascale[3];
   double eta, finspin;
-  double pad;  // sizeof a multiple of the 16-byte alignment without hidden padding
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are

@@ -58,7 +58,10 @@ def test_ragged_batches(ctx_cfg, B):
     Lr, Rr, Sr = _data.oracle_loglike(an, D, SN, P)
     L, R, S = c.loglike(P, Q.FP64)
     assert (S == Sr).all()
-    assert np.abs(L - Lr).max() <= 1e-8, np.abs(L - Lr).max()
+    # the fused launch's bound, as in test_loglike_fp64: 1e-8 plus 1e-12 of
+    # |logL| (FP64 sums of ~1e5 terms in different orders; the 65-point batch
+    # holds a point at logL ~ -6.8e3 that differs by ~1e-8)
+    assert (np.abs(L - Lr) <= 1e-8 + 1e-12 * np.abs(Lr)).all(), np.abs(L - Lr).max()
     assert np.abs(R - Rr).max() <= 1e-8
     Lm, _, Sm = c.loglike(P, Q.MIXED)
     assert (Sm == Sr).all()
