@@ -388,7 +388,8 @@ int qcbc::effective_splits(const qcbc_ctx* c, int prec, int B) {
   return std::max(s, 1);
 }
 
-// Kernel view of a context for B points: split length a multiple of the tile.
+// Kernel view of a context for B points: split length a multiple of 64 bins
+// (the kernels' chunk), at least one tile.
 LLInst qcbc::make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Point* pts, double* partial) {
   LLInst in{};
   in.p64 = c->d_p64;
@@ -402,7 +403,7 @@ LLInst qcbc::make_inst(const qcbc_ctx* c, int tile, int B, int splits, const Poi
   in.nd = c->desc.n_det;
   in.B = B;
   int len = (c->nb + splits - 1) / std::max(splits, 1);
-  len = ((len + tile - 1) / tile) * tile;
+  len = ((len + 63) / 64) * 64;
   in.split_len = std::max(len, tile);
   in.splits = (c->nb + in.split_len - 1) / in.split_len;
   if (in.splits < 1) in.splits = 1;

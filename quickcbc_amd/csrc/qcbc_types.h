@@ -76,8 +76,9 @@ struct alignas(16) Point {
 };
 
 // One analysis as the fused likelihood kernels see it.  Bin splits are
-// split_len band bins long (a multiple of the LDS tile) so that tile and
-// split boundaries depend only on the analysis, never on the batch.
+// split_len band bins long (a multiple of 64; tiles start at each split's
+// first bin) so that tile and split boundaries depend only on the analysis,
+// never on the batch.
 struct LLInst {
   const double* p64;
   const float* p32;
