@@ -33,7 +33,7 @@ FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
 # over the 2048 points), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (profiles/k_ll_fp64_pt_step_hbm_traffic.md), each scaled by what the counter
 # reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-K_LL_FP64_TRAFFIC_BYTES = 30_660_000
+K_LL_FP64_TRAFFIC_BYTES = 30_700_000
 
 
 def log(*a):
@@ -434,13 +434,19 @@ def main():
     pt1.run(args.kernel_steps)
     ctx.sync()
     kms, nl = ctx.timing(0)
-    kernel_ms = kms / max(nl, 1)
+    kernel_ms_in_step = kms / max(nl, 1)  # each launch inside the PT steps between its own two events
+    # the same kernel on the ensemble as the steps left it, launched back to
+    # back between two events on its stream: the kernel's own duration
+    x_end = np.ascontiguousarray(pt1.get()[0], np.float64)
+    dx = Q.DeviceBuffer.from_array(x_end)
+    kernel_ms = ctx.time_kernel(dx.ptr, B, main_prec, args.kernel_steps)
+    dx.free()
     cpb, tile, _ = ctx.get_shape(main_prec)
-    log("likelihood kernel alone: %.4f ms/launch over %d launches (%d points per block, %d-bin tiles)" %
-        (kernel_ms, nl, cpb, tile))
+    log("likelihood kernel alone: %.4f ms/launch over %d back-to-back launches, %.4f ms inside the PT steps "
+        "(%d points per block, %d-bin tiles)" % (kernel_ms, args.kernel_steps, kernel_ms_in_step, cpb, tile))
     # algorithmic FP64 work of one launch, on the ensemble as the timed steps left it
     from quickcbc_amd import bench_configs as C
-    bin_points, src_flops = C.fused_work(ctx, an, pt1.get()[0])
+    bin_points, src_flops = C.fused_work(ctx, an, x_end)
     pt1.close()
     # the path's algorithmic FP64 work per bin-point: the waveform's 60 flops
     # (Mf 1, powers 7, amplitude 20, inspiral phase 27, combine 5) and per
@@ -500,8 +506,11 @@ def main():
         ctx.sync()
         pm1.run(args.kernel_steps)
         mms, mnl = ctx.timing(0)
+        dxm = Q.DeviceBuffer.from_array(np.ascontiguousarray(pm1.get()[0], np.float64))
+        extra["mixed_kernel_ms"] = ctx.time_kernel(dxm.ptr, B, Q.MIXED, args.kernel_steps)
+        extra["mixed_kernel_ms_in_pt_step"] = mms / max(mnl, 1)
+        dxm.free()
         pm1.close()
-        extra["mixed_kernel_ms"] = mms / max(mnl, 1)
         ptm.step_timing(True)
         ptm.run(args.latency_steps)
         lat = ptm.latencies()
@@ -582,7 +591,9 @@ def main():
                                  "mixed kernel (about a third of its flops in FP64, the rest FP32): the path's "
                                  "flops against the FP64 vector peak, a comparison with the FP64 kernel only"),
                      "kernel": "k_ll<3,%s,%d>" % ("FP64" if main_prec == Q.FP64 else "MIXED", tile),
-                     "kernel_ms": kernel_ms, "points_per_block": cpb,
+                     "kernel_ms": kernel_ms, "kernel_ms_source": "HIP events around %d back-to-back launches on "
+                     "the kernel's stream, over the ensemble the timed PT steps left" % args.kernel_steps,
+                     "kernel_ms_in_pt_step": kernel_ms_in_step, "points_per_block": cpb,
                      "flops_per_launch": flops, "bin_points_per_launch": bin_points,
                      "flops_per_bin_point": flops_per_bp,
                      "kernel_source_flops_per_bin_point": src_flops / max(bin_points, 1)},

@@ -354,6 +354,12 @@ int qcbc_ctx_get_shape(qcbc_ctx *ctx, int precision, int *chains_per_block, int 
                        int *splits);
 /* Time candidate shapes on synthetic points and keep the fastest. */
 int qcbc_autotune(qcbc_ctx *ctx, int precision, int B);
+/* The fused likelihood kernel alone, for measurement (no counterpart in the
+ * reference): set-up of B points (device memory) once, then reps launches
+ * back to back on the context's stream between two HIP events;
+ * *ms_per_launch = their average duration, launch gaps included. */
+int qcbc_time_loglike_kernel(qcbc_ctx *ctx, const double *d_params, int B, int precision, int reps,
+                             double *ms_per_launch);
 /* Cumulative device time (ms) and launch count of the fused likelihood
  * kernel, measured with HIP events on the context's stream. */
 int qcbc_ctx_timing(qcbc_ctx *ctx, int enable, double *kernel_ms, long long *launches);

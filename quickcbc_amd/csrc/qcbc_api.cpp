@@ -468,6 +468,42 @@ int qcbc_loglike_device(qcbc_ctx* c, const double* d_params, int B, int prec, do
   return loglike_core(c, d_params, B, prec, d_logL, d_rho, d_status);
 }
 
+// The fused likelihood kernel alone: set-up of B points (device memory) once,
+// then reps launches back to back on the context's stream between two HIP
+// events; their average duration, launch gaps included.
+int qcbc_time_loglike_kernel(qcbc_ctx* c, const double* d_params, int B, int prec, int reps, double* ms_per_launch) {
+  if (!c || B < 1 || !d_params || reps < 1 || !ms_per_launch) return QCBC_EINVAL;
+  if (prec != QCBC_FP64 && prec != QCBC_MIXED) return QCBC_EINVAL;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  if (!c->bound) return QCBC_ENOTBOUND;
+  int st = ctx_reserve(c, B);
+  if (st) return st;
+  HIPCHK(launch_setup(d_params, B, c->an, c->d_pts, c->d_status, c->stream));
+  const LLInst in = make_inst(c, c->tile[prec], B, effective_splits(c, prec, B), c->d_pts, c->d_partial);
+  HIPCHK(launch_ll(prec, in, c->cpb[prec], c->tile[prec], c->stream));  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return QCBC_EHIP;
+  }
+  hipError_t e = hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_ll(prec, in, c->cpb[prec], c->tile[prec], c->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) {
+    qcbc::t_last_hip = e;
+    return QCBC_EHIP;
+  }
+  *ms_per_launch = ms / reps;
+  return QCBC_OK;
+}
+
 int qcbc_loglike(qcbc_ctx* c, const double* params, int B, int prec, double* logL, double* rho, int* status) {
   if (!c || B < 0 || (B > 0 && (!params || !logL))) return QCBC_EINVAL;
   if (B == 0) return QCBC_OK;

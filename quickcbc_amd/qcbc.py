@@ -529,6 +529,17 @@ class Context(_Handle):
         _check(lib().qcbc_ctx_timing(self.h, enable, ctypes.byref(ms), ctypes.byref(n)), "qcbc_ctx_timing")
         return ms.value, n.value
 
+    def time_kernel(self, d_params, B, precision, reps):
+        """Average duration (ms) of the fused likelihood kernel over reps
+        back-to-back launches on B points (device memory)."""
+        L = lib()
+        L.qcbc_time_loglike_kernel.argtypes = [_vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double)]
+        ms = ctypes.c_double(0)
+        _check(L.qcbc_time_loglike_kernel(self.h, d_params, B, precision, reps, ctypes.byref(ms)),
+               "qcbc_time_loglike_kernel")
+        return ms.value
+
     def sync(self):
         _check(lib().qcbc_ctx_sync(self.h), "qcbc_ctx_sync")
 
