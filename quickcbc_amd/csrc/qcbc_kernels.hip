@@ -268,29 +268,6 @@ __device__ __forceinline__ void block_range(const LLInst& in, int b, bool active
   *hi = s_lohi[1];
 }
 
-// The band planes from bin t0 on as a raw buffer (np planes of nbp doubles):
-// a tile's loads then take the plane offset in a scalar register and the
-// lane's byte offset in a vector register, with no per-load address
-// arithmetic, and a read past the planes returns 0 instead of faulting.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const LLInst& in, int t0, int np) {
-  const int bytes = (np * in.nbp - t0) * 8;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(in.p64 + t0), (short)0, bytes, 0x00020000);
-}
-
-__device__ __forceinline__ double buffer_load_f64(__amdgpu_buffer_rsrc_t rs, unsigned voff, int soff) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
-}
-
-// The mixed kernel's FP32 planes (np planes of nbp floats) from bin t0 on.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc32(const LLInst& in, int t0, int np) {
-  const int bytes = (np * in.nbp - t0) * 4;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in.p32 + t0), (short)0, bytes, 0x00020000);
-}
-
-__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t rs, unsigned voff, int soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
-}
-
 // Makes v opaque to the optimiser in place (empty asm): later subtractions of
 // scalar constants from it stay single subtractions instead of FMAs that need
 // the constant copied into a vector register.
@@ -851,45 +828,99 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
 // merger-ringdown), sin/cos of the reduced turn (hardware v_sin/v_cos take
 // turns), the products with the scaled data, and per-lane Kahan-compensated
 // sums; the wave and split combine is FP64.
+//
+// The block walks the split as the FP64 block does (lanes on the split's
+// 64-bin grid, segment loops outermost, two LDS buffers of HT = TILE / 2 bins
+// filled by buffer-to-LDS transfers, one barrier per tile).  A buffer holds
+// the four FP64 frequency planes, then the 3 nd FP32 planes.
 template <int ND, int TILE>
 struct MixState {
+  static constexpr int HT = TILE / 2;
+  static constexpr int BYTES64 = 4 * HT * 8;             // the FP64 planes of a buffer
+  static constexpr int BUFB = BYTES64 + 3 * ND * HT * 4;  // bytes per buffer
   const Point* c;
-  const double* lds;  // the LDS tiles and the first band bin
-  const float* lds32;
-  int t0;
-  int j;
+  char* lds;  // two buffers
+  const double* p64;
+  const float* p32;
+  int nbp, j0, t0, cur, bend, wave, cpb;
+  unsigned lane16;
   float sum[2 * ND], comp[2 * ND];
 };
+
+// Buffer-to-LDS transfer of np planes of HT elements of E bytes (plane
+// stride nbp elements, from element t on) into LDS at dst: units of 64 lanes
+// x LB bytes, taken by the block's waves in turn; the whole byte offset is
+// in the vector offset, so reads past the planes return 0.
+template <int E, int HT>
+__device__ __forceinline__ void lds_fetch_planes(char* dst, const void* src, int np, int nbp, int t, int wave, int cpb,
+                                                 unsigned lane16) {
+  constexpr int LB = (HT * E) % 1024 == 0 ? 16 : 4;
+  constexpr int PER = HT * E / (64 * LB);
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(src), (short)0, np * nbp * E, 0x00020000);
+  for (int k = wave; k < np * PER; k += cpb) {
+    const int p = k / PER, h = k - p * PER;
+    auto d = (__attribute__((address_space(3))) void*)(dst + p * HT * E + h * 64 * LB);
+    const unsigned off = (unsigned)E * (unsigned)(p * nbp + t) + (unsigned)(h * 64 * LB);
+    if constexpr (LB == 16)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 16, lane16 + off, 0, 0, 0);
+    else
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, lane16 / 4 + off, 0, 0, 0);
+  }
+}
+
+template <int ND, int TILE>
+__device__ __forceinline__ void mix_fetch(MixState<ND, TILE>& st, int t, int q) {
+  constexpr int HT = MixState<ND, TILE>::HT;
+  char* buf = st.lds + q * MixState<ND, TILE>::BUFB;
+  lds_fetch_planes<8, HT>(buf, st.p64, 4, st.nbp, t, st.wave, st.cpb, st.lane16);
+  lds_fetch_planes<4, HT>(buf + MixState<ND, TILE>::BYTES64, st.p32, 3 * ND, st.nbp, t, st.wave, st.cpb, st.lane16);
+}
+
+template <int ND, int TILE>
+__device__ __forceinline__ void mix_advance(MixState<ND, TILE>& st) {
+  constexpr int HT = MixState<ND, TILE>::HT;
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  st.t0 += HT;
+  st.cur ^= 1;
+  if (st.t0 + HT < st.bend) mix_fetch(st, st.t0 + HT, st.cur ^ 1);
+}
 
 template <int ND, int TILE>
 struct MixSeg {
   template <int AR, int PR>
   struct S {
     static __device__ __forceinline__ void run(MixState<ND, TILE>& st, int pos, int end) {
+      constexpr int HT = MixState<ND, TILE>::HT;
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
       c.template load<ND, false>(*st.c);
       const AmpF<AR> cf(c, k);
-      const int nit = (end - pos + 63) >> 6;
-      for (int n = 0; n < nit; n++) {
-        if (st.j < end) {
-          const double* L = st.lds + (st.j - st.t0);
-          const float* M = st.lds32 + (st.j - st.t0);
+      const int lane = threadIdx.x & 63;
+      const unsigned span = (unsigned)(end - pos);
+      const int ce = uniform(st.j0 + 64 * ((end - 1 - st.j0) >> 6));
+      for (int jb = uniform(st.j0 + 64 * ((pos - st.j0) >> 6)); jb <= ce; jb += 64) {
+        while (jb >= st.t0 + HT) mix_advance(st);
+        if ((unsigned)(jb + lane - pos) < span) {
+          const char* buf = st.lds + uniform(st.cur * MixState<ND, TILE>::BUFB);
+          const double* L = reinterpret_cast<const double*>(buf) + (jb - st.t0) + lane;
+          const float* M = reinterpret_cast<const float*>(buf + MixState<ND, TILE>::BYTES64) + (jb - st.t0) + lane;
           const double f = L[0];
           float a;
           double r;
-          amp32_turns<AR, PR>(c, k, cf, f, L[TILE], L[2 * TILE], L[3 * TILE], &a, &r);
+          amp32_turns<AR, PR>(c, k, cf, f, L[HT], L[2 * HT], L[3 * HT], &a, &r);
           const float a2 = a * a;
 #pragma unroll
           for (int d = 0; d < ND; d++) {
             const double q = (d == 0) ? r : r + fma(f, c.dt[d], c.dl[d]);
             const float u = (float)__builtin_amdgcn_fract(q);
             const float sn = __builtin_amdgcn_sinf(u), cs = __builtin_amdgcn_cosf(u);
-            kahan_add(st.sum[2 * d], st.comp[2 * d], a2 * M[(3 * d + 2) * TILE]);
+            kahan_add(st.sum[2 * d], st.comp[2 * d], a2 * M[(3 * d + 2) * HT]);
             kahan_add(st.sum[2 * d + 1], st.comp[2 * d + 1],
-                      a * fmaf(M[(3 * d) * TILE], cs, M[(3 * d + 1) * TILE] * sn));
+                      a * fmaf(M[(3 * d) * HT], cs, M[(3 * d + 1) * HT] * sn));
           }
-          st.j += 64;
         }
       }
     }
@@ -898,7 +929,7 @@ struct MixSeg {
 
 template <int ND, int TILE>
 __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* lds, int* s_lohi) {
-  float* lds32 = reinterpret_cast<float*>(lds + 4 * TILE);
+  constexpr int HT = MixState<ND, TILE>::HT;
   const int cpb = blockDim.x >> 6;
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -906,46 +937,39 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   const int j0 = split * in.split_len, j1 = min(in.nb, j0 + in.split_len);
   const bool active = b < in.B && in.pts[b].status == ST_OK;
   int ks, ke, blo, bhi;
-  block_range<TILE>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
+  block_range<HT>(in, b, active, j0, j1, s_lohi, &ks, &ke, &blo, &bhi);
   MixState<ND, TILE> st;
   st.c = in.pts + uniform(active ? b : 0);
-  st.j = ks + lane;
+  st.lds = reinterpret_cast<char*>(lds);
+  st.p64 = in.p64;
+  st.p32 = in.p32;
+  st.nbp = in.nbp;
+  st.j0 = j0;
+  st.t0 = blo;
+  st.cur = 0;
+  st.bend = bhi;
+  st.wave = wave;
+  st.cpb = uniform(cpb);
+  st.lane16 = 16u * (unsigned)lane;
 #pragma unroll
   for (int q = 0; q < 2 * ND; q++) st.sum[q] = st.comp[q] = 0.0f;
-  int a1 = 0, a2 = 0, p1 = 0, p2 = 0, pos = ks;
-  if (active) {
+  if (blo < bhi) {
+    mix_fetch(st, blo, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (blo + HT < bhi) mix_fetch(st, blo + HT, 1);
+  }
+  if (active && ks < ke) {
     const Point& c = *st.c;
-    a1 = c.seg_a1 - in.imin;
-    a2 = c.seg_a2 - in.imin;
-    p1 = c.seg_p1 - in.imin;
-    p2 = c.seg_p2 - in.imin;
+    run_segments<MixSeg<ND, TILE>::template S>(st, ks, ke, c.seg_a1 - in.imin, c.seg_a2 - in.imin,
+                                               c.seg_p1 - in.imin, c.seg_p2 - in.imin);
   }
-  for (int t0 = blo; t0 < bhi; t0 += TILE) {
-    const int tn = min(TILE, bhi - t0);
-    __syncthreads();
-    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(in, t0, 4);
-    const __amdgpu_buffer_rsrc_t rs32 = plane_rsrc32(in, t0, 3 * ND);
-    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-#pragma unroll
-      for (int p = 0; p < 4; p++) lds[p * TILE + i] = buffer_load_f64(rs, 8u * i, p * in.nbp * 8);
-#pragma unroll
-      for (int p = 0; p < 3 * ND; p++) lds32[p * TILE + i] = buffer_load_f32(rs32, 4u * i, p * in.nbp * 4);
-    }
-    __syncthreads();
-    if (active) {
-      st.lds = lds;
-      st.lds32 = lds32;
-      st.t0 = t0;
-      pos = run_segments<MixSeg<ND, TILE>::template S>(st, pos, min(ke, t0 + tn), a1, a2, p1, p2);
-    }
-  }
+  while (st.t0 + HT < bhi) mix_advance(st);  // the hand-overs after this wave's last bin
   double acc[2 * ND];
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = wave_sum((double)st.sum[q] - (double)st.comp[q]);
-  if (lane == SUM_LANE && b < in.B) {
-#pragma unroll
-    for (int q = 0; q < 2 * ND; q++) in.partial[((size_t)b * in.splits + split) * (2 * ND) + q] = acc[q];
-  }
+  for (int q = 0; q < 2 * ND; q++) acc[q] = (double)st.sum[q] - (double)st.comp[q];
+  wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }
 
 template <int PREC, int TILE>
