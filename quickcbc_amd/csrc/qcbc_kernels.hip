@@ -222,12 +222,6 @@ __device__ __forceinline__ SinCosK sincos_consts2() {
                  vreg_v(0x1.03c1f081b0780p-1)};
 }
 
-__device__ __forceinline__ void kahan_add(float& s, float& comp, float x) {
-  float y = x - comp;
-  float t = s + y;
-  comp = (t - s) - y;
-  s = t;
-}
 
 // ------------------------------------------------- fused likelihood blocks
 // A block works on one group of cpb points of one analysis (one wave per
@@ -820,19 +814,16 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }
 
-// ---- mixed precision.  FP64: the powers of Mf and the projected argument
-// f (ts - td) + (lambda - Psi)/2 pi in turns, reduced to a fraction of a
-// turn (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to
-// ~1e-11.  FP32: the amplitude (amp32_turns: the Horner and Newton forms on
-// the FP64 powers rounded to FP32, v_exp_f32 and v_rcp_f32 in
-// merger-ringdown), sin/cos of the reduced turn (hardware v_sin/v_cos take
-// turns), the products with the scaled data, and per-lane Kahan-compensated
-// sums; the wave and split combine is FP64.
-//
-// The block walks the split as the FP64 block does (lanes on the split's
-// 64-bin grid, segment loops outermost, two LDS buffers of HT = TILE / 2 bins
-// filled by buffer-to-LDS transfers, one barrier per tile).  A buffer holds
-// the four FP64 frequency planes, then the 3 nd FP32 planes.
+// ---- mixed precision.  FP64: the powers of f, the projected detector-0
+// argument f (ts - td) + (lambda - Psi)/2 pi in turns reduced to a fraction
+// of a turn (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to
+// ~1e-11, the detectors' Horner sums (their phases theta_d, linear in f,
+// enter exactly, as in the FP64 kernel) and the per-lane sums.  FP32: the
+// amplitude (amp32_turns: the Horner and Newton forms on the FP64 powers
+// rounded to FP32, v_exp_f32 and v_rcp_f32 in merger-ringdown), sin/cos of
+// the reduced detector-0 turn (hardware v_sin/v_cos take turns; one pair per
+// bin, shared by every detector), and the products with the scaled data
+// that feed the FP64 sums.
 template <int ND, int TILE>
 struct MixState {
   static constexpr int HT = TILE / 2;
@@ -844,7 +835,9 @@ struct MixState {
   const float* p32;
   int nbp, j0, t0, cur, bend, wave, cpb;
   unsigned lane16;
-  float sum[2 * ND], comp[2 * ND];
+  double hh[ND];          // sum a^2 / (S w_ref)
+  double dh0;             // sum a Re(D_0 e^{-i x0}) / (S sqrt(w_ref))
+  double hr[ND], hi[ND];  // Horner sums of detectors d > 0
 };
 
 // Buffer-to-LDS transfer of np planes of HT elements of E bytes (plane
@@ -896,7 +889,7 @@ struct MixSeg {
       constexpr int HT = MixState<ND, TILE>::HT;
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
-      c.template load<ND, false>(*st.c);
+      c.template load<ND>(*st.c);
       const AmpF<AR> cf(c, k);
       const int lane = threadIdx.x & 63;
       const unsigned span = (unsigned)(end - pos);
@@ -911,15 +904,21 @@ struct MixSeg {
           float a;
           double r;
           amp32_turns<AR, PR>(c, k, cf, f, L[HT], L[2 * HT], L[3 * HT], &a, &r);
-          const float a2 = a * a;
+          const float t = (float)__builtin_amdgcn_fract(r);  // the turn, reduced in FP64
+          const float uc = a * __builtin_amdgcn_cosf(t), us = a * __builtin_amdgcn_sinf(t), a2 = a * a;
 #pragma unroll
           for (int d = 0; d < ND; d++) {
-            const double q = (d == 0) ? r : r + fma(f, c.dt[d], c.dl[d]);
-            const float u = (float)__builtin_amdgcn_fract(q);
-            const float sn = __builtin_amdgcn_sinf(u), cs = __builtin_amdgcn_cosf(u);
-            kahan_add(st.sum[2 * d], st.comp[2 * d], a2 * M[(3 * d + 2) * HT]);
-            kahan_add(st.sum[2 * d + 1], st.comp[2 * d + 1],
-                      a * fmaf(M[(3 * d) * HT], cs, M[(3 * d + 1) * HT] * sn));
+            const float Dre = M[(3 * d) * HT], Dim = M[(3 * d + 1) * HT];
+            st.hh[d] += (double)(a2 * M[(3 * d + 2) * HT]);
+            if (d == 0) {
+              st.dh0 += (double)fmaf(Dre, uc, Dim * us);
+            } else {
+              const double yr = (double)fmaf(Dre, uc, Dim * us), yi = (double)fmaf(Dim, uc, -(Dre * us));
+              const double tr = fma(-st.hi[d], c.ws[d], yr);
+              fma_acc_x(st.hi[d], c.wc[d], yi);       // hi wc + yi
+              fma_acc_z(st.hi[d], st.hr[d], c.ws[d]);  // + hr ws
+              fma_acc_x(st.hr[d], c.wc[d], tr);       // hr wc - hi ws + yr
+            }
           }
         }
       }
@@ -951,8 +950,9 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   st.wave = wave;
   st.cpb = uniform(cpb);
   st.lane16 = 16u * (unsigned)lane;
+  st.dh0 = 0.0;
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) st.sum[q] = st.comp[q] = 0.0f;
+  for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
   if (blo < bhi) {
     mix_fetch(st, blo, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -968,7 +968,23 @@ __device__ void ll_mixed_block(const LLInst& in, int group, int split, double* l
   while (st.t0 + HT < bhi) mix_advance(st);  // the hand-overs after this wave's last bin
   double acc[2 * ND];
 #pragma unroll
-  for (int q = 0; q < 2 * ND; q++) acc[q] = (double)st.sum[q] - (double)st.comp[q];
+  for (int d = 0; d < ND; d++) {
+    double dh = st.dh0;
+    if (d > 0) {
+      // back from the lane's last bin: e^{-i theta_d,last} H (zero if the lane took no bin)
+      dh = 0.0;
+      int jl = j0 + 64 * ((ke - 1 - j0) >> 6) + lane;
+      if (jl >= ke) jl -= 64;
+      if (active && jl >= ks) {
+        const Point& c = *st.c;
+        double sn, co;
+        sincos_turns(fma(in.p64[jl], c.dt[d], c.dl[d]), &sn, &co);
+        dh = fma(st.hr[d], co, st.hi[d] * sn);
+      }
+    }
+    acc[2 * d] = st.hh[d];
+    acc[2 * d + 1] = dh;
+  }
   wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }
 
