@@ -51,13 +51,13 @@ struct qcbc_pt {
   int* d_st = nullptr;
   unsigned long long *d_acc = nullptr, *d_swaps = nullptr;
   std::vector<int> perm;  // storage index -> point number t * n_chains + c
-  hipEvent_t ev_fork = nullptr, ev_join[QCBC_PT_MAX_GROUPS] = {};
-  // captured steps: [0] one step, [1] kGraphSteps steps, [2] the last
-  // run's remainder of rem_steps (< kGraphSteps) steps (in [1] and [2] the
-  // column groups run through the steps without meeting, as the
-  // free-running streams do)
-  hipGraph_t graph[3] = {};
-  hipGraphExec_t exec[3] = {};
+  hipEvent_t ev_fork = nullptr;  // a run's groups start after the work queued on the context stream
+  // captured steps, one graph per column group, replayed on the group's own
+  // stream (so the groups overlap as the free-running streams do): [0] one
+  // step, [1] kGraphSteps steps, [2] the last run's remainder of rem_steps
+  // (< kGraphSteps) steps
+  hipGraph_t graph[3][QCBC_PT_MAX_GROUPS] = {};
+  hipGraphExec_t exec[3][QCBC_PT_MAX_GROUPS] = {};
   int rem_steps = 0;
   bool step_timing = false;
   std::vector<double> lat_ms;
@@ -151,12 +151,11 @@ static void pt_free(qcbc_pt* pt) {
       if (b) (void)hipFree(b);
   }
   if (pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
-  for (hipEvent_t e : pt->ev_join)
-    if (e) (void)hipEventDestroy(e);
-  for (int i = 0; i < 3; i++) {
-    if (pt->exec[i]) (void)hipGraphExecDestroy(pt->exec[i]);
-    if (pt->graph[i]) (void)hipGraphDestroy(pt->graph[i]);
-  }
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < QCBC_PT_MAX_GROUPS; k++) {
+      if (pt->exec[i][k]) (void)hipGraphExecDestroy(pt->exec[i][k]);
+      if (pt->graph[i][k]) (void)hipGraphDestroy(pt->graph[i][k]);
+    }
   void* bufs[] = {pt->d_x,     pt->d_L,  pt->d_y,  pt->d_Ly,   pt->d_rho,    pt->d_sigma,  pt->d_tc,
                   pt->d_st,    pt->d_acc, pt->d_swaps, pt->d_ej, pt->d_ev, pt->d_hist, pt->d_hcount,
                   pt->d_ustats, pt->d_typ, pt->d_jac};
@@ -272,8 +271,7 @@ static int pt_create(qcbc_ctx* ctx, qcbc_het* het, int n_temps, int n_chains, do
     ok = hipMalloc(&gr.d_pts, sizeof(Point) * gr.P) == hipSuccess &&
          hipMalloc(&gr.d_partial, sizeof(double) * 6 * (size_t)QCBC_MAX_SPLITS * gr.P) == hipSuccess &&
          hipMalloc(&gr.d_step, sizeof(unsigned long long)) == hipSuccess &&
-         hipMalloc(&gr.d_done, sizeof(unsigned int)) == hipSuccess &&
-         hipEventCreateWithFlags(&pt->ev_join[k], hipEventDisableTiming) == hipSuccess;
+         hipMalloc(&gr.d_done, sizeof(unsigned int)) == hipSuccess;
     if (ok && k == 0) {
       gr.s = ctx->stream;
     } else if (ok) {
@@ -468,42 +466,52 @@ static int pt_sync(qcbc_pt* pt) {
   return QCBC_OK;
 }
 
-// Steps per launch of the multi-step graph.  Within one graph each column
-// group's steps form a chain of its own and the groups join only at the end,
-// so one group's set-up and post kernels overlap another's likelihood as they
-// do on the free-running streams; a one-step graph would join every step.
+// Steps per launch of the multi-step graphs.
 constexpr int kGraphSteps = 32;
 
 // Steps per launch of QuickCBC's fused step kernel in a run (a launch of
 // 128 steps of a 32 x 64 ensemble takes a few ms)
 constexpr int kUpdateStepsPerLaunch = 128;
 
-// Captures n consecutive steps of the whole ensemble into pt->graph/exec[slot].
+// Captures n consecutive steps of each column group into
+// pt->graph/exec[slot][k], on the group's own stream.
 static int pt_capture(qcbc_pt* pt, int slot, int n) {
   qcbc_ctx* c = pt->ctx;
-  hipStream_t s = c->stream;
   bool timing = c->timing;
-  c->timing = false;  // event records stay out of the graph
-  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  // the other groups' streams fork from the context stream and join it again
+  c->timing = false;  // event records stay out of the graphs
   int st = QCBC_OK;
-  if (pt->G > 1 && hipEventRecord(pt->ev_fork, s) != hipSuccess) st = QCBC_EHIP;
-  for (int k = 1; k < pt->G && !st; k++)
-    if (hipStreamWaitEvent(pt->g[k].s, pt->ev_fork, 0) != hipSuccess) st = QCBC_EHIP;
-  for (int i = 0; i < n && !st; i++) st = pt_enqueue_step(pt);
-  for (int k = 1; k < pt->G && !st; k++)
-    if (hipEventRecord(pt->ev_join[k], pt->g[k].s) != hipSuccess || hipStreamWaitEvent(s, pt->ev_join[k], 0) != hipSuccess)
+  for (int k = 0; k < pt->G && !st; k++) {
+    hipStream_t s = pt->g[k].s;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
       st = QCBC_EHIP;
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(s, &g);
-  c->timing = timing;
-  if (st) {
-    if (g) hipGraphDestroy(g);
-    return st;
+      break;
+    }
+    for (int i = 0; i < n && !st; i++) st = group_step(pt, k);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (!st && e != hipSuccess) st = QCBC_EHIP;
+    if (st) {
+      if (g) hipGraphDestroy(g);
+      break;
+    }
+    pt->graph[slot][k] = g;
+    if (hipGraphInstantiate(&pt->exec[slot][k], g, nullptr, nullptr, 0) != hipSuccess) st = QCBC_EHIP;
   }
-  HIPCHK(e);
-  pt->graph[slot] = g;
-  HIPCHK(hipGraphInstantiate(&pt->exec[slot], g, nullptr, nullptr, 0));
+  c->timing = timing;
+  return st;
+}
+
+static void pt_drop_graphs(qcbc_pt* pt, int slot) {
+  for (int k = 0; k < QCBC_PT_MAX_GROUPS; k++) {
+    if (pt->exec[slot][k]) (void)hipGraphExecDestroy(pt->exec[slot][k]);
+    if (pt->graph[slot][k]) (void)hipGraphDestroy(pt->graph[slot][k]);
+    pt->exec[slot][k] = nullptr;
+    pt->graph[slot][k] = nullptr;
+  }
+}
+
+static int pt_launch_graphs(qcbc_pt* pt, int slot) {
+  for (int k = 0; k < pt->G; k++) HIPCHK(hipGraphLaunch(pt->exec[slot][k], pt->g[k].s));
   return QCBC_OK;
 }
 
@@ -532,33 +540,51 @@ int qcbc_pt_run(qcbc_pt* pt, int nsteps, int use_graph) {
   }
   const bool multi = use_graph && !pt->step_timing;
   const int full = multi ? nsteps / kGraphSteps : 0, rem = multi ? nsteps % kGraphSteps : 0;
-  if (use_graph && !multi && !pt->exec[0]) {
+  if (use_graph && !multi && !pt->exec[0][0]) {
     int st = pt_capture(pt, 0, 1);
-    if (st) return st;
+    if (st) {
+      pt_drop_graphs(pt, 0);
+      return st;
+    }
   }
-  if (full > 0 && !pt->exec[1]) {
+  if (full > 0 && !pt->exec[1][0]) {
     int st = pt_capture(pt, 1, kGraphSteps);
-    if (st) return st;
+    if (st) {
+      pt_drop_graphs(pt, 1);
+      return st;
+    }
   }
-  if (rem > 0 && (!pt->exec[2] || pt->rem_steps != rem)) {
-    if (pt->exec[2]) (void)hipGraphExecDestroy(pt->exec[2]);
-    if (pt->graph[2]) (void)hipGraphDestroy(pt->graph[2]);
-    pt->exec[2] = nullptr;
-    pt->graph[2] = nullptr;
+  if (rem > 0 && (!pt->exec[2][0] || pt->rem_steps != rem)) {
+    pt_drop_graphs(pt, 2);
     pt->rem_steps = 0;
     int st = pt_capture(pt, 2, rem);
-    if (st) return st;
+    if (st) {
+      pt_drop_graphs(pt, 2);
+      return st;
+    }
     pt->rem_steps = rem;
   }
+  // the groups' streams start after the work queued on the context stream
+  if (pt->G > 1) {
+    HIPCHK(hipEventRecord(pt->ev_fork, s));
+    for (int k = 1; k < pt->G; k++) HIPCHK(hipStreamWaitEvent(pt->g[k].s, pt->ev_fork, 0));
+  }
   if (multi) {
-    for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(pt->exec[1], s));
-    if (rem > 0) HIPCHK(hipGraphLaunch(pt->exec[2], s));
+    for (int i = 0; i < full; i++) {
+      int st = pt_launch_graphs(pt, 1);
+      if (st) return st;
+    }
+    if (rem > 0) {
+      int st = pt_launch_graphs(pt, 2);
+      if (st) return st;
+    }
     return pt_sync(pt);
   }
   for (int i = 0; i < nsteps; i++) {
     auto t0 = std::chrono::steady_clock::now();
     if (use_graph) {
-      HIPCHK(hipGraphLaunch(pt->exec[0], s));
+      int st = pt_launch_graphs(pt, 0);
+      if (st) return st;
     } else {
       int st = pt_enqueue_step(pt);
       if (st) return st;
