@@ -436,28 +436,6 @@ struct SConst {
 // f^(-1/3) (Horner) and a Newton form in f, with the point's powers of
 // Msec folded into their coefficients, so they need no power of Mf; the
 // merger-ringdown pieces work in Mf = Msec f.
-// The amplitude constants of region AR in FP32, for the mixed kernel: the
-// inspiral Horner coefficients, the intermediate Newton coefficients, or the
-// merger-ringdown Lorentzian's numerator, exponent rate (in base 2, for
-// v_exp_f32) and width.
-template <int AR>
-struct AmpF {
-  float c[9];
-  template <class K, class V>
-  __device__ __forceinline__ AmpF(const K& s, const V& k) {
-    if (AR == 0) {
-      c[0] = (float)s.ia0; c[1] = (float)k.a1; c[2] = (float)k.a0; c[3] = (float)s.ia1; c[4] = (float)s.ia43;
-      c[5] = (float)s.ia53; c[6] = (float)s.ia2; c[7] = (float)s.ia73; c[8] = (float)s.ia3;
-    } else if (AR == 1) {
-      c[0] = (float)s.ib0; c[1] = (float)s.ib1; c[2] = (float)s.ib2; c[3] = (float)k.a0; c[4] = (float)s.ib4;
-    } else {
-      c[0] = (float)s.mnum;
-      c[1] = (float)(s.mrd_rate * 1.4426950408889634);  // rate / ln 2
-      c[2] = (float)s.mg2;
-    }
-  }
-};
-
 template <int AR, int PR, class K>
 __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, double f, double u,
                                           double e, double logf, double* a, double* r) {
@@ -516,69 +494,6 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     g = -hstep(nn, E, c.ins_q0) + fma(-np, u, lg);  // q1 Mf is in tsq
   } else if (PR == 1) {
     g = fma(c.int_qm3, (E3 * E3) * E3, c.int_qL * logf) + c.int_q0;  // qm3 E^9 = qm3' / Mf^3
-  } else {
-    const double sq = Mf * (w * i_s6);  // Mf^(1/2)
-    const double f34 = sqrt(Mf * sq);
-    g = fma(c.mrd_qat, atan_mrd((Mf - c.mrd_a5) * c.mrd_ifDM), fma(c.mrd_q34, f34, c.mrd_qm1 * w3)) + c.mrd_q0;
-  }
-  *r = fma(f, c.tsq, g);
-}
-
-// amp_turns with the amplitude in FP32 (the phase as there, in FP64): the
-// powers of Mf that the phase needs anyway, rounded to FP32, drive the same
-// Horner and Newton forms in FP32; the merger-ringdown exponential and
-// reciprocal are the hardware's v_exp_f32 and v_rcp_f32.
-template <int AR, int PR, class K>
-__device__ __forceinline__ void amp32_turns(const K& c, const VConst<AR, PR>& k, const AmpF<AR>& cf, double f,
-                                            double u, double e, double logf, float* a, double* r) {
-  const double E = e * e;  // f^(-1/3)
-  const double E2 = E * E;
-  double E3 = 0.0;
-  if (AR == 1 || PR == 1) E3 = E2 * E;
-  double Mf = 0.0, i_s6 = 0.0, w = 0.0, w3 = 0.0;
-  if (AR == 2 || PR == 2) {
-    Mf = c.Msec * f;
-    vreg(Mf);
-    i_s6 = c.is6_scale * e;  // Mf^(-1/6)
-    w = i_s6 * i_s6;         // Mf^(-1/3)
-    w3 = (w * w) * w;        // 1 / Mf
-  }
-  if (AR == 0) {
-    const float uf = (float)u, Ef = (float)E;
-    float np = fmaf(uf, -cf.c[8], -cf.c[2]);
-    np = fmaf(np, uf, -cf.c[7]);
-    np = fmaf(np, uf, -cf.c[6]);
-    np = fmaf(np, uf, -cf.c[5]);
-    np = fmaf(np, uf, -cf.c[4]);
-    np = fmaf(np, uf, -cf.c[3]);
-    *a = (float)e * fmaf(Ef, fmaf((float)E2, cf.c[0], cf.c[1]), -np);
-  } else if (AR == 1) {
-    const float x1 = (float)(f - c.nf1);
-    float np = fmaf((float)(f - c.nf3), -cf.c[4], -cf.c[3]);
-    np = fmaf(np, (float)(f - c.nf2), -cf.c[2]);
-    np = fmaf(np, x1, -cf.c[1]);
-    *a = (float)(e * E3) * -fmaf(np, x1, -cf.c[0]);
-  } else {
-    const float df = (float)(Mf - c.fRD);
-    const float A = (cf.c[0] * __builtin_amdgcn_exp2f(-df * cf.c[1])) * __builtin_amdgcn_rcpf(fmaf(df, df, cf.c[2]));
-    *a = (float)(w3 * i_s6) * A;  // Mf^(-7/6)
-  }
-  // the projected argument, in FP64 as amp_turns forms it
-  double g;
-  if (PR == 0) {
-    double np = fma(u, -c.ins_q2, -k.p0);
-    np = hstep(np, u, c.ins_q43);
-    np = np * u;  // no u^3 term: q1 Mf is added below
-    np = hstep(np, u, c.ins_q23);
-    np = hstep(np, u, c.ins_q13);
-    double nn = -(E * c.ins_qm53);
-    nn = hstep(nn, E, c.ins_qm1);
-    nn = hstep(nn, E, c.ins_qm23);
-    nn = hstep(nn, E, c.ins_qm13);
-    const double lg = fma(u, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, E, c.ins_q0) + fma(-np, u, lg);  // q1 Mf is in tsq
-  } else if (PR == 1) {
-    g = fma(c.int_qm3, (E3 * E3) * E3, c.int_qL * logf) + c.int_q0;
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);
@@ -814,16 +729,22 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }
 
-// ---- mixed precision.  FP64: the powers of f, the projected detector-0
-// argument f (ts - td) + (lambda - Psi)/2 pi in turns reduced to a fraction
-// of a turn (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad exact to
-// ~1e-11, the detectors' Horner sums (their phases theta_d, linear in f,
-// enter exactly, as in the FP64 kernel) and the per-lane sums.  FP32: the
-// amplitude (amp32_turns: the Horner and Newton forms on the FP64 powers
-// rounded to FP32, v_exp_f32 and v_rcp_f32 in merger-ringdown), sin/cos of
-// the reduced detector-0 turn (hardware v_sin/v_cos take turns; one pair per
-// bin, shared by every detector), and the products with the scaled data
-// that feed the FP64 sums.
+// ---- mixed precision.  FP64: the waveform as amp_turns evaluates it (the
+// powers of f, the amplitude and the projected detector-0 argument
+// f (ts - td) + (lambda - Psi)/2 pi in turns), the argument's reduction to a
+// fraction of a turn (v_fract_f64), which keeps 2 pi f t_c + Psi ~ 1e5 rad
+// exact to ~1e-11, the detectors' Horner sums (their phases theta_d, linear
+// in f, enter exactly, as in the FP64 kernel) and the per-lane sums.  FP32:
+// sin/cos of the reduced detector-0 turn (hardware v_sin/v_cos take turns;
+// one pair per bin, shared by every detector), the amplitude rounded once,
+// and its products with the scaled FP32 data that feed the FP64 sums.  (An
+// FP32 amplitude, its coefficients rounded to FP32, biases (h|h) by ~6e-8 of
+// itself: 0.019 in logL at the loudest prior draws.)
+//
+// The block walks the split as the FP64 block does (lanes on the split's
+// 64-bin grid, segment loops outermost, two LDS buffers of HT = TILE / 2 bins
+// filled by buffer-to-LDS transfers, one barrier per tile).  A buffer holds
+// the four FP64 frequency planes, then the 3 nd FP32 planes.
 template <int ND, int TILE>
 struct MixState {
   static constexpr int HT = TILE / 2;
@@ -890,7 +811,6 @@ struct MixSeg {
       const VConst<AR, PR> k(*st.c);
       SConst<AR, PR> c;
       c.template load<ND>(*st.c);
-      const AmpF<AR> cf(c, k);
       const int lane = threadIdx.x & 63;
       const unsigned span = (unsigned)(end - pos);
       const int ce = uniform(st.j0 + 64 * ((end - 1 - st.j0) >> 6));
@@ -901,9 +821,9 @@ struct MixSeg {
           const double* L = reinterpret_cast<const double*>(buf) + (jb - st.t0) + lane;
           const float* M = reinterpret_cast<const float*>(buf + MixState<ND, TILE>::BYTES64) + (jb - st.t0) + lane;
           const double f = L[0];
-          float a;
-          double r;
-          amp32_turns<AR, PR>(c, k, cf, f, L[HT], L[2 * HT], L[3 * HT], &a, &r);
+          double a64, r;
+          amp_turns<AR, PR>(c, k, f, L[HT], L[2 * HT], L[3 * HT], &a64, &r);
+          const float a = (float)a64;
           const float t = (float)__builtin_amdgcn_fract(r);  // the turn, reduced in FP64
           const float uc = a * __builtin_amdgcn_cosf(t), us = a * __builtin_amdgcn_sinf(t), a2 = a * a;
 #pragma unroll
