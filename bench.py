@@ -34,6 +34,8 @@ FP32_VALU_PEAK_TF = 157.3  # MI355X FP32 vector (packed issue)
 # (profiles/k_ll_fp64_pt_step_hbm_traffic.md), each scaled by what the counter
 # reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 K_LL_FP64_TRAFFIC_BYTES = 30_700_000
+# The same for the mixed-precision kernel (profiles/k_ll_mixed_pt_step_valu.md).
+K_LL_MIXED_TRAFFIC_BYTES = 20_670_000
 
 
 def log(*a):
@@ -562,7 +564,7 @@ def main():
                          "%.1f s, gcc -O2 -fopenmp, OpenMP over points" % (n, dt)}
         log("cpu baseline:", cpu)
 
-    traffic = K_LL_FP64_TRAFFIC_BYTES if main_prec == Q.FP64 else None
+    traffic = K_LL_FP64_TRAFFIC_BYTES if main_prec == Q.FP64 else K_LL_MIXED_TRAFFIC_BYTES
     out = {
         "metric": "logL evals/sec (waveform+logL) at Tobs=16s 3-det; p99 PT-step latency",
         "value": value,
@@ -574,7 +576,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_prec == Q.FP64 else "mixed (f64 phase, f32 amplitude and sums)",
+        "dtype": "f64" if main_prec == Q.FP64 else "mixed (f64 waveform and sums, f32 sin/cos and data products)",
         "data": "synthetic: seeded Gaussian noise coloured by the analytic PSD plus an IMRPhenomD injection",
         "config": {"workload": "batched PT ensemble, Tobs=16 s, fs=4096 Hz, 3 det (H/L/V), band 20-2048 Hz "
                                "(32448 bins), 32 temps x 64 chains = 2048 points/step, %s fused launch"
@@ -582,13 +584,13 @@ def main():
                    "points_per_step": B, "bins": nb, "detectors": an.nifo},
         "roofline": {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TF, "unit": "TFLOP/s",
                      "frac": achieved_tf / FP64_PEAK_TF, "traffic": traffic,
-                     "traffic_source": ("constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
-                                        "workload, scaled by a 1 GiB copy (profiles/k_ll_fp64_pt_step_hbm_traffic.md);"
-                                        " not measured in this run" if traffic else
-                                        "not measured for the mixed kernel (profiles/k_ll_mixed_pt_step_valu.md)"),
+                     "traffic_source": "constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
+                                       "workload, scaled by a 1 GiB copy (profiles/%s); not measured in this run"
+                                       % ("k_ll_fp64_pt_step_hbm_traffic.md" if main_prec == Q.FP64
+                                          else "k_ll_mixed_pt_step_valu.md"),
                      "ceiling": ("FP64 vector ALU issue: the kernel has no MFMA; MI355X's FP64 vector peak "
                                  "(78.6 TFLOP/s, an FMA counted as 2)" if main_prec == Q.FP64 else
-                                 "mixed kernel (about a third of its flops in FP64, the rest FP32): the path's "
+                                 "mixed kernel (about 70% of its executed flops in FP64, the rest FP32): the path's "
                                  "flops against the FP64 vector peak, a comparison with the FP64 kernel only"),
                      "kernel": "k_ll<3,%s,%d>" % ("FP64" if main_prec == Q.FP64 else "MIXED", tile),
                      "kernel_ms": kernel_ms, "kernel_ms_source": "HIP events around %d back-to-back launches on "
