@@ -556,6 +556,37 @@ __device__ __forceinline__ int run_segments(ST& st, int pos, int hi, int a1, int
 // hand-over of the block in the same order (waves whose point has no bins
 // left take the remaining hand-overs after their segments), so the barriers
 // match.
+// Buffer-to-LDS transfer of np planes of HT elements of E bytes (plane
+// stride nbp elements, from element t on) into LDS at dst, plane after plane.
+// Plane p goes to wave (p + rot) mod cpb as PER transfers of 64 lanes x LB
+// bytes; the k-th carries the instruction offset k x 64 x LB, which the
+// hardware adds to the global and the LDS address alike.  The whole byte
+// offset but that is in the vector offset (the range check counts the scalar
+// offset too), so elements past the last plane read as 0.  0 <= rot < cpb.
+template <int E, int HT>
+__device__ __forceinline__ void lds_fetch_planes(char* dst, const void* src, int np, int nbp, int t, int wave, int cpb,
+                                                 int rot, unsigned lane16) {
+  constexpr int LB = (HT * E) % 1024 == 0 ? 16 : 4;  // bytes per lane
+  constexpr int PER = HT * E / (64 * LB);            // transfers per plane
+  static_assert(PER == 1 || PER == 2, "one or two transfers a plane");
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(src), (short)0, np * nbp * E, 0x00020000);
+  int p = wave - rot;
+  if (p < 0) p += cpb;
+  unsigned voff = (LB == 16 ? lane16 : lane16 / 4) + (unsigned)E * (unsigned)(p * nbp + t);
+  const unsigned step = (unsigned)E * (unsigned)(cpb * nbp);
+  for (; p < np; p += cpb, voff += step) {
+    auto d = (__attribute__((address_space(3))) void*)(dst + p * HT * E);
+    if constexpr (LB == 16) {
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 16, voff, 0, 0, 0);
+      if constexpr (PER == 2) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 16, voff, 0, 64 * 16, 0);
+    } else {
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, voff, 0, 0, 0);
+      if constexpr (PER == 2) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, voff, 0, 64 * 4, 0);
+    }
+  }
+}
+
 template <int ND, int TILE>
 struct F64State {
   static constexpr int NP = 4 + 3 * ND;
@@ -576,27 +607,12 @@ struct F64State {
   SinCosK sk;
 };
 
-// Buffer-to-LDS transfer of the tile at band bin t into buffer q: each
-// transfer moves 64 lanes x LB bytes of one plane; the block's waves take
-// the transfers in turn.  The whole byte offset is in the vector offset, so
-// the buffer's range check covers it (bins past the last plane read as 0).
+// Buffer-to-LDS transfer of the tile at band bin t into buffer q (the
+// planes' layout in LDS: buffer, plane, bin).
 template <int NP, int HT>
 __device__ __forceinline__ void f64_fetch(double* lds, const double* p64, int nbp, int wave, int cpb, unsigned lane16,
                                           int t, int q) {
-  constexpr int LB = (HT % 128 == 0) ? 16 : 4;  // bytes per lane
-  constexpr int PER = HT * 8 / (64 * LB);       // transfers per plane
-  const __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p64), (short)0, NP * nbp * 8, 0x00020000);
-  for (int k = wave; k < NP * PER; k += cpb) {
-    const int p = k / PER, h = k - p * PER;
-    const int first = p * HT + h * (64 * LB / 8);  // doubles into the buffer
-    auto dst = (__attribute__((address_space(3))) void*)(lds + q * NP * HT + first);
-    const unsigned off = 8u * (unsigned)(p * nbp + t + h * (64 * LB / 8));
-    if constexpr (LB == 16)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst, 16, lane16 + off, 0, 0, 0);
-    else
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst, 4, lane16 / 4 + off, 0, 0, 0);
-  }
+  lds_fetch_planes<8, HT>((char*)(lds + q * NP * HT), p64, NP, nbp, t, wave, cpb, 0, lane16);
 }
 
 template <int ND, int TILE>
@@ -761,34 +777,14 @@ struct MixState {
   double hr[ND], hi[ND];  // Horner sums of detectors d > 0
 };
 
-// Buffer-to-LDS transfer of np planes of HT elements of E bytes (plane
-// stride nbp elements, from element t on) into LDS at dst: units of 64 lanes
-// x LB bytes, taken by the block's waves in turn; the whole byte offset is
-// in the vector offset, so reads past the planes return 0.
-template <int E, int HT>
-__device__ __forceinline__ void lds_fetch_planes(char* dst, const void* src, int np, int nbp, int t, int wave, int cpb,
-                                                 unsigned lane16) {
-  constexpr int LB = (HT * E) % 1024 == 0 ? 16 : 4;
-  constexpr int PER = HT * E / (64 * LB);
-  const __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(src), (short)0, np * nbp * E, 0x00020000);
-  for (int k = wave; k < np * PER; k += cpb) {
-    const int p = k / PER, h = k - p * PER;
-    auto d = (__attribute__((address_space(3))) void*)(dst + p * HT * E + h * 64 * LB);
-    const unsigned off = (unsigned)E * (unsigned)(p * nbp + t) + (unsigned)(h * 64 * LB);
-    if constexpr (LB == 16)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 16, lane16 + off, 0, 0, 0);
-    else
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, lane16 / 4 + off, 0, 0, 0);
-  }
-}
-
 template <int ND, int TILE>
 __device__ __forceinline__ void mix_fetch(MixState<ND, TILE>& st, int t, int q) {
   constexpr int HT = MixState<ND, TILE>::HT;
   char* buf = st.lds + q * MixState<ND, TILE>::BUFB;
-  lds_fetch_planes<8, HT>(buf, st.p64, 4, st.nbp, t, st.wave, st.cpb, st.lane16);
-  lds_fetch_planes<4, HT>(buf + MixState<ND, TILE>::BYTES64, st.p32, 3 * ND, st.nbp, t, st.wave, st.cpb, st.lane16);
+  // the four FP64 planes to waves 0-3, the FP32 planes from wave 4 on
+  lds_fetch_planes<8, HT>(buf, st.p64, 4, st.nbp, t, st.wave, st.cpb, 0, st.lane16);
+  lds_fetch_planes<4, HT>(buf + MixState<ND, TILE>::BYTES64, st.p32, 3 * ND, st.nbp, t, st.wave, st.cpb, 4 % st.cpb,
+                          st.lane16);
 }
 
 template <int ND, int TILE>
