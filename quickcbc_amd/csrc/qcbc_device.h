@@ -888,7 +888,10 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   // Mf^(-7/6) amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3) = e [E (ia0 E^2 +
   // ia23) + ia1 + ia43 u + ia53 u^2 + ia2 u^3 + ia73 u^4 + ia83 u^5 + ia3 u^6];
   // intermediate Mf^(-7/6) amp_pre N(Mf) = e E^3 N'(f) with the Newton form
-  // N' on the nodes col_f* in Hz
+  // N' on the nodes col_f* in Hz, rewritten as a quartic in x1 = f - nf1:
+  //   N' = ib0 + x1 [ib1 + x1 [ib2 + (x1 + d2) [ib3 + ib4 (x1 + d3)]]],
+  //   d2 = f1 - f2, d3 = f1 - f3 (nodes in Hz), so the x1^2 and x1^3 coefficients are
+  //   ib2 + d2 (ib3 + ib4 d3) and ib3 + ib4 (d2 + d3)
   c.ia0 = c.amp_pre * sg7;
   c.ia23 = c.amp_pre * c.ai_23 * sg3;
   c.ia1 = c.amp_pre * c.ai_1 * sg;
@@ -905,8 +908,11 @@ __device__ inline void fused_setup(const AnalysisConst& an, Point& c) {
   c.ib3 = c.amp_pre * c.col_c3 * sg7 * (M * M * M);
   c.ib4 = c.amp_pre * c.col_c4 * sg7 * ((M * M) * (M * M));
   c.nf1 = c.col_f1 / M;
-  c.nf2 = c.col_f2 / M;
-  c.nf3 = c.col_f3 / M;
+  {
+    const double d2 = c.nf1 - c.col_f2 / M, d3 = c.nf1 - c.col_f3 / M;
+    c.im2 = c.ib2 + d2 * (c.ib3 + c.ib4 * d3);
+    c.im3 = c.ib3 + c.ib4 * (d2 + d3);
+  }
   c.mnum = c.amp_pre * c.mrd_num;
   c.mg2 = c.fDMg3 * c.fDMg3;
   for (int d = 0; d < 3; d++) {

@@ -356,7 +356,7 @@ struct VConst {
       a0 = vreg_v(c.ia83);
       a1 = vreg_v(c.ia23);
     }
-    if (AR == 1) a0 = vreg_v(c.ib3);
+    if (AR == 1) a0 = vreg_v(c.im3);
     if (PR == 0) {
       p0 = vreg_v(c.ins_q53);
       pL = vreg_v(c.ins_qL);
@@ -381,7 +381,7 @@ template <int AR, int PR>
 struct SConst {
   double Msec, is6_scale, tshift[1];
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia3;
-  double nf1, nf2, nf3, ib0, ib1, ib2, ib4;
+  double nf1, ib0, ib1, im2, ib4;
   double fRD, mrd_rate, mnum, mg2;
   double ins_q0, ins_q1, ins_q13, ins_q13L, ins_q23, ins_q43, ins_q2, ins_qm13, ins_qm23, ins_qm1, ins_qm53;
   double int_q0, int_q1, int_qL, int_qm3;
@@ -400,8 +400,8 @@ struct SConst {
       ia0 = spin(c.ia0); ia1 = spin(c.ia1); ia43 = spin(c.ia43);
       ia53 = spin(c.ia53); ia2 = spin(c.ia2); ia73 = spin(c.ia73); ia3 = spin(c.ia3);
     } else if (AR == 1) {
-      nf1 = spin(c.nf1); nf2 = spin(c.nf2); nf3 = spin(c.nf3);
-      ib0 = spin(c.ib0); ib1 = spin(c.ib1); ib2 = spin(c.ib2); ib4 = spin(c.ib4);
+      nf1 = spin(c.nf1);
+      ib0 = spin(c.ib0); ib1 = spin(c.ib1); im2 = spin(c.im2); ib4 = spin(c.ib4);
     } else {
       fRD = spin(c.fRD); mrd_rate = spin(c.mrd_rate); mnum = spin(c.mnum); mg2 = spin(c.mg2);
     }
@@ -433,7 +433,7 @@ struct SConst {
 // region PR (0 inspiral, 1 intermediate, 2 merger-ringdown), from the folded
 // constants of the point and the bin's f, u = f^(1/3), e = f^(-1/6), log f.
 // The inspiral and intermediate pieces are polynomials in u and E = e^2 =
-// f^(-1/3) (Horner) and a Newton form in f, with the point's powers of
+// f^(-1/3) (Horner) and a quartic in f - f1 (Hz), with the point's powers of
 // Msec folded into their coefficients, so they need no power of Mf; the
 // merger-ringdown pieces work in Mf = Msec f.
 template <int AR, int PR, class K>
@@ -464,10 +464,11 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     np = hstep(np, u, c.ia1);
     *a = e * fma(E, fma(E2, c.ia0, k.a1), -np);
   } else if (AR == 1) {
-    // e E^3 times the Newton form in f on the nodes (f1, f1, f2, f3), Horner on negated partials
+    // e E^3 times the quartic in x1 = f - f1 (the Newton form on the nodes
+    // (f1, f1, f2, f3) expanded about f1), Horner on negated partials
     const double x1 = f - c.nf1;
-    double np = fma(f - c.nf3, -c.ib4, -k.a0);
-    np = hstep(np, f - c.nf2, c.ib2);
+    double np = fma(x1, -c.ib4, -k.a0);
+    np = hstep(np, x1, c.im2);
     np = hstep(np, x1, c.ib1);
     *a = (e * E3) * -hstep(np, x1, c.ib0);
   } else {
@@ -490,10 +491,11 @@ __device__ __forceinline__ void amp_turns(const K& c, const VConst<AR, PR>& k, d
     nn = hstep(nn, E, c.ins_qm1);
     nn = hstep(nn, E, c.ins_qm23);
     nn = hstep(nn, E, c.ins_qm13);  // -(qm13 + qm23 E + qm1 E^2 + qm53 E^4)
-    const double lg = fma(u, c.ins_q13L, k.pL) * logf;
-    g = -hstep(nn, E, c.ins_q0) + fma(-np, u, lg);  // q1 Mf is in tsq
+    // q0 rides in the log term's FMA, and the E chain's last step takes the rest as its addend
+    const double lg = fma(fma(u, c.ins_q13L, k.pL), logf, c.ins_q0);
+    g = fma(-nn, E, fma(-np, u, lg));  // q1 Mf is in tsq
   } else if (PR == 1) {
-    g = fma(c.int_qm3, (E3 * E3) * E3, c.int_qL * logf) + c.int_q0;  // qm3 E^9 = qm3' / Mf^3
+    g = fma(c.int_qm3, (E3 * E3) * E3, fma(c.int_qL, logf, c.int_q0));  // qm3 E^9 = qm3' / Mf^3
   } else {
     const double sq = Mf * (w * i_s6);  // Mf^(1/2)
     const double f34 = sqrt(Mf * sq);

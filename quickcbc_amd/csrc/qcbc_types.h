@@ -43,14 +43,15 @@ struct alignas(16) Point {
   // amplitude without Ascale, with e = f^(-1/6):
   //   inspiral:      e [E (ia0 E^2 + ia23) + ia1 + ia43 u + ia53 u^2 + ia2 u^3 + ia73 u^4 + ia83 u^5 + ia3 u^6]
   //                  (Mf^(-7/6) amp_pre (1 + ai_23 Mf^(2/3) + ... + ai_3 Mf^3))
-  //   intermediate:  e E^3 x the Newton form in f (coefficients ib*, nodes nf*)
+  //   intermediate:  e E^3 x the quartic in x1 = f - nf1 (Hz): ib0 + ib1 x1 + im2 x1^2 + im3 x1^3 + ib4 x1^4
   //   merger:        Mf^(-7/6) mnum exp(-(Mf - fRD) mrd_rate) / ((Mf - fRD)^2 + fDMg3^2)
   double ia0, ia23, ia1, ia43, ia53, ia2, ia73, ia83, ia3;
   double int_q0, int_q1, int_qL, int_qm3;
   // intermediate amplitude in Newton form on nodes (f1, f1, f2, f3) in Mf
   double col_f1, col_f2, col_f3;
   double ib0, ib1, ib2, ib3, ib4;
-  double nf1, nf2, nf3;  // the nodes in Hz (col_f / Msec)
+  double nf1;       // the first node in Hz (col_f1 / Msec)
+  double im2, im3;  // the Newton form's x1^2 and x1^3 coefficients as a polynomial in x1 = f - nf1
   double fRD, mrd_rate, mnum, mg2;  // mrd_rate = g2/(fDM g3)
   double mrd_q0, mrd_q1, mrd_qm1, mrd_q34, mrd_qat, mrd_a5, mrd_ifDM;
   // detector d relative to detector 0: f dt_d + dl_d turns

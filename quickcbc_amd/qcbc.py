@@ -1020,7 +1020,7 @@ class PointRecord(ctypes.Structure):
                 [(n, ctypes.c_double) for n in (
                     "ins_q0 ins_q1 ins_q13 ins_q13L ins_qL ins_q23 ins_q43 ins_q53 ins_q2 ins_qm13 ins_qm23 "
                     "ins_qm1 ins_qm53 ia0 ia23 ia1 ia43 ia53 ia2 ia73 ia83 ia3 int_q0 int_q1 int_qL int_qm3 "
-                    "col_f1 col_f2 col_f3 ib0 ib1 ib2 ib3 ib4 nf1 nf2 nf3 fRD mrd_rate mnum mg2 "
+                    "col_f1 col_f2 col_f3 ib0 ib1 ib2 ib3 ib4 nf1 im2 im3 fRD mrd_rate mnum mg2 "
                     "mrd_q0 mrd_q1 mrd_qm1 mrd_q34 mrd_qat mrd_a5 mrd_ifDM").split()] +
                 [(n, ctypes.c_double * 3) for n in ("dt", "dl")] +
                 [(n, ctypes.c_double) for n in (
