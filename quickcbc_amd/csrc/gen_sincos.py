@@ -23,13 +23,30 @@ T = mp.mpf(1) / 2
 DEG = 6
 
 
-def fit(fun):
+def fit(fun, scale=1):
     c = mp.chebyfit(fun, [0, T * T], DEG + 1)
-    return [float(c[DEG - i]) for i in range(DEG + 1)]  # ascending powers of z
+    return [float(scale * c[DEG - i]) for i in range(DEG + 1)]  # ascending powers of z
 
 
-P = fit(lambda z: mp.sin(mp.pi / 2 * mp.sqrt(z)) / mp.sqrt(z) if z != 0 else mp.pi / 2)
-Q = fit(lambda z: mp.cos(mp.pi / 2 * mp.sqrt(z)))
+def _sin_q(z):
+    return mp.sin(mp.pi / 2 * mp.sqrt(z)) / mp.sqrt(z) if z != 0 else mp.pi / 2
+
+
+def _cos_q(z):
+    return mp.cos(mp.pi / 2 * mp.sqrt(z))
+
+
+P = fit(_sin_q)
+Q = fit(_cos_q)
+
+# sincos_amp_turns: the same fits times 2^(3/4), each rounded once, so that one
+# double-angle step leaves sin and cos of the half angle times sqrt(2), whose
+# last double-angle step folds with the amplitude:
+#   a sin 2y = P (sqrt2 sin y), a cos 2y = P (sqrt2 cos y) - a, P = a (sqrt2 cos y).
+SCALE34 = mp.mpf(2) ** (mp.mpf(3) / 4)
+PS = fit(_sin_q, SCALE34)
+QS = fit(_cos_q, SCALE34)
+SQRT2 = float(mp.sqrt(2))
 
 
 def power_basis(c, z):
@@ -65,3 +82,7 @@ if __name__ == "__main__":
     print("{" + fmt(Q) + "}")
     es, ec = ulp_error()
     print("// worst error on |t| <= 1/2: sin %.2f ulp, cos %.2f ulp" % (es, ec))
+    print("// times 2^(3/4), for sincos_amp_turns: P, Q ascending, then sqrt(2)")
+    print("{" + fmt(PS) + "}")
+    print("{" + fmt(QS) + "}")
+    print(float.hex(SQRT2))

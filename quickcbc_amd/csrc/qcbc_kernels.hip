@@ -141,7 +141,7 @@ __device__ __forceinline__ void fma_acc_z(double& acc, double x, double y) {  //
 // of copying one of them every bin.
 struct SinCosK {
   double p1, q1;
-  double p2, p3, q2, q3, q4;  // sincos2_turns only: its next coefficients, taken from vector registers too
+  double p2, p3, q2, q3, q4;  // sincos_amp_turns only: its next coefficients, taken from vector registers too
 };
 
 __device__ __forceinline__ double vreg_v(double v) {
@@ -183,43 +183,40 @@ __device__ __forceinline__ void sincos_turns(double r, double* s, double* c) {
   sincos_turns(r, s, c, SinCosK{-0x1.e30071afc3e59p-19, -0x1.a6c9c1be9eb49p-16});
 }
 
-// Twice sin and cos of 2 pi r: sincos_turns' polynomials with every
-// coefficient doubled (exact), which makes each double-angle step one
-// product and one FMA, (2 sin 2x) = (2 sin x)(2 cos x) and (2 cos 2x) =
-// (2 cos x)^2 - 2, in place of two products, an FMA and an add.  The sine
-// chain is bit for bit twice sincos_turns'; the cosine differs from it by an
-// ulp or two.  k2: sincos_consts2().
-__device__ __forceinline__ void sincos2_turns(double r, double* s2, double* c2, const SinCosK& k2) {
+// a sin and a cos of 2 pi r.  sincos_turns' quarter-angle polynomials with
+// every coefficient times 2^(3/4) (gen_sincos.py: PS, QS, each rounded once),
+// one double-angle step at that scale, (2^(3/4) s)(2^(3/4) c) = sqrt2 sin 2x
+// and (2^(3/4) c)^2 - sqrt2 = sqrt2 cos 2x, and the last step folded with the
+// amplitude: with P = a sqrt2 cos y, a sin 2y = P (sqrt2 sin y) and
+// a cos 2y = P (sqrt2 cos y) - a, three operations for the last step and the
+// amplitude's two products.  k: sincos_consts_amp().
+__device__ __forceinline__ void sincos_amp_turns(double r, double a, double* as, double* ac, const SinCosK& k) {
   const double t = r - rint(r);
   const double z = t * t;
-  double p = fma(z, sreg(0x1.e3f38399551bfp-24), k2.p1);
-  p = fma(p, z, k2.p2);
-  p = fma(p, z, k2.p3);
-  p = fma(p, z, -sreg(-0x1.466bc677587f8p-3));
-  p = fma(p, z, -sreg(0x1.4abbce625be41p+0));
-  p = fma(p, z, -sreg(-0x1.921fb54442d18p+1));
-  double q = fma(z, sreg(0x1.f3dbcea61b1a4p-21), k2.q1);
-  q = fma(q, z, k2.q2);
-  q = fma(q, z, k2.q3);
-  q = fma(q, z, k2.q4);
-  q = fma(q, z, -sreg(0x1.3bd3cc9be458bp+1));
-  q = fma(q, z, 2.0);
-  double sn = t * p, cs = q;
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const double sx = sn * cs;
-    cs = fma(cs, cs, -2.0);
-    sn = sx;
-  }
-  *s2 = sn;
-  *c2 = cs;
+  double p = fma(z, sreg(0x1.96f3ee17e49d0p-24), k.p1);
+  p = fma(p, z, k.p2);
+  p = fma(p, z, k.p3);
+  p = fma(p, z, -sreg(-0x1.127c7aa3ad157p-3));
+  p = fma(p, z, -sreg(0x1.161cdace59eb0p+0));
+  p = fma(p, z, -sreg(-0x1.5224febe4ed1bp+1));
+  double q = fma(z, sreg(0x1.a4544e88d2cbfp-21), k.q1);
+  q = fma(q, z, k.q2);
+  q = fma(q, z, k.q3);
+  q = fma(q, z, k.q4);
+  q = fma(q, z, -sreg(0x1.0993fcf8017d5p+1));
+  q = fma(q, z, -sreg(-0x1.ae89f995ad3adp+0));
+  const double sn = t * p;  // 2^(3/4) sin and cos of the quarter angle: sn, q
+  const double s1 = sn * q, c1 = fma(q, q, -sreg(0x1.6a09e667f3bcdp+0));  // sqrt2 sin, cos of the half angle
+  const double pa = a * c1;
+  *as = pa * s1;
+  *ac = fma(pa, c1, -a);
 }
 
-__device__ __forceinline__ SinCosK sincos_consts2() {
-  return SinCosK{vreg_v(-0x1.e30071afc3e59p-18), vreg_v(-0x1.a6c9c1be9eb49p-15),
-                 vreg_v(0x1.50782fda12d96p-12),   vreg_v(-0x1.32d2cce2e5b19p-7),
-                 vreg_v(0x1.e1f4fb60281f6p-10),   vreg_v(-0x1.55d3c7dbfd139p-5),
-                 vreg_v(0x1.03c1f081b0780p-1)};
+__device__ __forceinline__ SinCosK sincos_consts_amp() {
+  return SinCosK{vreg_v(-0x1.9627888bf363fp-18), vreg_v(-0x1.638560cc2646fp-15),
+                 vreg_v(0x1.1aef9c5dc3c56p-12),  vreg_v(-0x1.0201b90b986b1p-7),
+                 vreg_v(0x1.9546a01b8e092p-10),  vreg_v(-0x1.1f70faa628642p-5),
+                 vreg_v(0x1.b4dbc2030e65dp-2)};
 }
 
 
@@ -645,12 +642,12 @@ struct F64Seg {
       constexpr int HT = F64State<ND, TILE>::HT;
       double a, r;
       amp_turns<AR, PR>(c, k, L[0], L[HT], L[2 * HT], L[3 * HT], &a, &r);
-      double s0, c0;
-      sincos2_turns(r, &s0, &c0, st.sk);  // twice sin and cos
+      double us, uc;
+      sincos_amp_turns(r, a, &us, &uc, st.sk);  // a sin, a cos
       // h_d = Ascale_d a e^{i x_d}: |h_d|^2 = Ascale_d^2 a^2, so the sums carry
-      // a^2 / S and (twice) a D e^{-i x_d} / S, and k_finalize applies the
-      // per-point detector scale once.
-      const double a2 = a * a, uc = a * c0, us = a * s0;
+      // a^2 / S and a D e^{-i x_d} / S, and k_finalize applies the per-point
+      // detector scale once.
+      const double a2 = a * a;
 #pragma unroll
       for (int d = 0; d < ND; d++) {
         const double Dre = L[(4 + 3 * d) * HT], Dim = L[(5 + 3 * d) * HT];
@@ -707,7 +704,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
   st.wave = wave;
   st.cpb = uniform(cpb);
   st.lane16 = 16u * (unsigned)lane;
-  st.sk = sincos_consts2();
+  st.sk = sincos_consts_amp();
   st.dh0 = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; d++) { st.hh[d] = 0.0; st.hr[d] = 0.0; st.hi[d] = 0.0; }
@@ -742,7 +739,7 @@ __device__ void ll_f64_block(const LLInst& in, int group, int split, double* lds
       }
     }
     acc[2 * d] = st.hh[d];
-    acc[2 * d + 1] = 0.5 * dh;  // the bins' sums carry twice the products (exact)
+    acc[2 * d + 1] = dh;
   }
   wave_sums_store<ND>(acc, in.partial + ((size_t)b * in.splits + split) * (2 * ND), b < in.B);
 }

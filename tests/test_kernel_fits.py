@@ -6,6 +6,7 @@ merger-ringdown exp and atan)."""
 import importlib.util
 import os
 import re
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -80,6 +81,63 @@ def test_sincos_turns_accuracy():
         rs, rc = mp.sin(2 * mp.pi * t), mp.cos(2 * mp.pi * t)
         worst = max(worst, float(abs(s - rs)), float(abs(c - rc)))
     assert worst < 8 * 2.0 ** -53, worst  # a few ulp of 1 (the results are in [-1, 1])
+
+
+def test_sincos_amp_literals_are_the_fit():
+    g = _load("gen_sincos")
+    k = _kernel_consts("void sincos_amp_turns(double r, double a, double* as, double* ac, const SinCosK& k)")
+    body = SRC[SRC.index("__device__ __forceinline__ SinCosK sincos_consts_amp()"):]
+    p1, q1, p2, p3, q2, q3, q4 = _hex_literals(body[:body.index("}")])[:7]
+    # the literals in source order: P's z^6, z^2, z^1, z^0, Q's z^6, z^1, z^0,
+    # then sqrt(2) (subtracted); z^5..z^3 (z^5..z^2 for Q) come from SinCosK
+    assert [k[0], p1, p2, p3, k[1], k[2], k[3]] == list(g.PS)[::-1]
+    assert [k[4], q1, q2, q3, q4, k[5], k[6]] == list(g.QS)[::-1]
+    assert k[7] == -g.SQRT2
+    # the scaled fits are the fits times 2^(3/4), to a rounding each
+    for a, b in zip(g.PS + g.QS, g.P + g.Q):
+        assert abs(a - b * 2 ** 0.75) <= 2 * np.spacing(abs(a))
+
+
+def _fma(x, y, z):
+    """x y + z rounded once, as the GPU's FMA (Fraction to float rounds correctly)."""
+    return float(Fraction(x) * Fraction(y) + Fraction(z))
+
+
+def _sincos_amp_kernel(r, a, PS, QS, sqrt2):
+    """sincos_amp_turns' operations in its order, each FMA rounded once."""
+    t = r - float(np.rint(r))
+    z = t * t
+    p = _fma(z, PS[6], PS[5])
+    for c in PS[4::-1]:
+        p = _fma(p, z, c)
+    q = _fma(z, QS[6], QS[5])
+    for c in QS[4::-1]:
+        q = _fma(q, z, c)
+    sn = t * p
+    s1, c1 = sn * q, _fma(q, q, -sqrt2)
+    pa = a * c1
+    return pa * s1, _fma(pa, c1, -a)
+
+
+def test_sincos_amp_turns_accuracy():
+    """a sin and a cos of 2 pi r within 24 ulp of a (the results are in [-a, a]).
+
+    Each double-angle step doubles the error it is handed (the cosine's most:
+    d(2 c^2 - 1) = 4 c dc), so the full angle carries ~10 ulp (sine) and ~20
+    ulp (cosine) of the quarter angle's 1-2 ulp; the form with doubled
+    coefficients and two plain double-angle steps measures 10.3 and 21.5 ulp
+    under the same emulation, this one 9.9 and 19.9.  (h|h) does not see it
+    (a^2 is formed apart); (d|h) sees it as a relative error of ~2e-15 per bin."""
+    g = _load("gen_sincos")
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for r, a in zip(np.concatenate([rng.uniform(-0.5, 0.5, 1500), rng.uniform(-3e4, 3e4, 1500)]),
+                    10.0 ** rng.uniform(-25, 5, 3000)):
+        s, c = _sincos_amp_kernel(float(r), float(a), g.PS, g.QS, g.SQRT2)
+        t = mp.mpf(float(r)) - mp.nint(mp.mpf(float(r)))
+        rs, rc = a * mp.sin(2 * mp.pi * t), a * mp.cos(2 * mp.pi * t)
+        worst = max(worst, float(abs(s - rs)) / a, float(abs(c - rc)) / a)
+    assert worst < 24 * 2.0 ** -53, worst
 
 
 def test_atan_literals_are_the_fit():
