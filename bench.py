@@ -107,8 +107,8 @@ def het_bench(Q, W, an, ctx, inj, D, SN, P, launches, cpu_seconds, no_cpu):
     Lfd, _, _ = ctx.loglike(sp["dparams"], Q.FP64)
     near = float(np.abs(Lhd - Lfd).max())
     out = {"logl_per_s": B * launches / T, "launch_ms": T / launches * 1e3, "kernel_ms": kernel_ms,
-           "kernel_ms_source": "HIP events around %d back-to-back launches: at this size the launch interval, "
-                               "longer than the kernel's own duration (rocprof, "
+           "kernel_ms_source": "HIP events around %d back-to-back launches: the kernel's duration plus the "
+                               "launch gaps, which show at this size (rocprof's duration: "
                                "profiles/k_het_ll_fp64_2048_points.md)" % launches,
            "kernel": "k_het_ll<%d>" % an.nifo, "points": B, "grid_samples": het.M, "regions": het.NR,
            "grid": "het_space (limits, Fisher_All, FisherEvec, efix to 30, tolerance 0.003)",
