@@ -8,8 +8,10 @@ jumps): proposal types, accepted moves, swaps and history fills must agree
 exactly, states to rounding (the GPU's log, cos and pow are within a few ulps
 of glibc's, so a proposal differs from the oracle's by ~1e-15 relative; a
 sky-ring move's finite-difference density amplifies that to ~1e-9 of its
-log; logL by <= 1e-8 as the heterodyned-likelihood tests).  Column groups and
-graph replay give bit-identical chains.
+log on the small ensemble, ~1e-7 on the PT ensemble's full 32 x 64 shape and
+~1e-5 where the move maps near-degenerately; logL by <= 1e-8 as the
+heterodyned-likelihood tests).  Column groups and graph replay give
+bit-identical chains.
 """
 import numpy as np
 import pytest
@@ -103,6 +105,53 @@ def test_chains_match_the_oracle_replay(setup, c):
     assert (np.abs(y1 - S["y"]) <= 1e-12 * (1.0 + np.abs(y1))).all()
     assert (np.abs(jac1 - S["jac"]) <= 1e-6 * (1.0 + np.abs(jac1))).all()
     pt.close()
+
+
+def test_chains_match_the_oracle_replay_full_shape():
+    """The PT ensemble's full shape (32 temperatures x 64 chains, Tobs = 16 s, three
+    detectors, the default three column groups), a shorter replay."""
+    an, inj, D, SN = _data.analysis(1)
+    h = _data.het(1)
+    ctx = Q.Context(an, D, SN)
+    g = Q.Het(ctx, arrays=h)
+    lo, hi = W.prior_box(an)
+    nt, nc, nsteps = 32, 64, 12
+    init = W.ensemble_in_prior(inj, nt * nc, lo, hi, seed=9, scale=0.3)
+    desc = Q.update_desc(lo, hi, c=0.2, n_hist=N_HIST, hist_every=EVERY)
+    pt = Q.PTUpdate(ctx, g, nt, nc, LADDER, init, desc, seed=SEED, groups=3)
+    x, L, _, _ = pt.get()
+    ej, ev, y, typ, jac, stats = pt.get_update()
+    hist, cnt = pt.get_history()
+    Lo, _ = O.loglike_het(an.labels, an.gmst, an.fref, an.Tobs, h, x)
+    S = dict(x=x.copy(), L=Lo, ej=ej.copy(), ev=ev.copy(), hist=hist.copy(), hcount=cnt.copy(), y=y.copy(),
+             typ=typ.copy(), jac=jac.copy(), stats=stats.copy(), step=0)
+    print("update replay: %d det, %d x %d points, %d steps" % (an.nifo, nt, nc, nsteps), flush=True)
+    pt.run(nsteps)
+    O.pt_update_run(an.labels, an.gmst, an.fref, an.Tobs, h, _uconst(lo, hi, an, 0.2), N_HIST, EVERY, nt, nc,
+                    LADDER, SEED, nsteps, S)
+    x1, L1, _, _ = pt.get()
+    ej1, ev1, y1, typ1, jac1, stats1 = pt.get_update()
+    hist1, cnt1 = pt.get_history()
+    assert np.array_equal(stats1, S["stats"])
+    assert np.array_equal(cnt1, S["hcount"]) and np.array_equal(typ1, S["typ"])
+    assert stats1[:, 4::2].sum() > 0 and stats1[:, 2].sum() > 0
+    assert (np.abs(x1 - S["x"]) <= 1e-12 * (1.0 + np.abs(x1))).all()
+    assert np.abs(L1 - S["L"]).max() <= 1e-8
+    assert np.array_equal(ej1, S["ej"]) and np.array_equal(ev1, S["ev"])
+    assert (np.abs(hist1 - S["hist"]) <= 1e-12 * (1.0 + np.abs(hist1))).all()
+    assert (np.abs(y1 - S["y"]) <= 1e-12 * (1.0 + np.abs(y1))).all()
+    # the proposals' log densities: sky-ring moves carry the finite differences of the sky
+    # map, ~1e-7 of their log at this size; where a move maps near-degenerately (log density
+    # ~ -15) that grows to ~1e-5, and what enters the acceptance, the density e^jac (2e-7
+    # there), still agrees to 3e-11.  Each proposal passes on its log (1e-6 relative) or on
+    # its density (1e-9 absolute plus relative).
+    d1, d0 = np.exp(jac1), np.exp(S["jac"])
+    ok_log = np.abs(jac1 - S["jac"]) <= 1e-6 * (1.0 + np.abs(jac1))
+    ok_density = np.abs(d1 - d0) <= 1e-9 * (1.0 + np.maximum(d1, d0))
+    assert (ok_log | ok_density).all(), np.flatnonzero(~(ok_log | ok_density))
+    pt.close()
+    g.close()
+    ctx.close()
 
 
 def test_groups_and_graph_replay_give_the_same_chains(setup):
