@@ -318,8 +318,9 @@ def main():
                     help="time QuickCBC's own PT step on the heterodyned likelihood only (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how the "
-              "likelihood would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (the points of a PT step "
+              "would shard over GPUs by column groups, each GPU holding the band planes); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     from quickcbc_amd import qcbc as Q
